@@ -1,0 +1,226 @@
+// Internal declarations of the pwgpu library (not part of the C ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/pwgpu.h"
+
+// ---------------------------------------------------------------------------
+// error plumbing
+// ---------------------------------------------------------------------------
+void pw_set_error(const std::string &msg);
+int pw_fail(int code, const std::string &msg);
+
+#define PW_HIP(call)                                                         \
+  do {                                                                       \
+    hipError_t e_ = (call);                                                  \
+    if (e_ != hipSuccess)                                                    \
+      return pw_fail(PW_ERR_HIP, std::string(#call) + ": " +                 \
+                                     hipGetErrorString(e_));                 \
+  } while (0)
+
+#define PW_TRY(call)                                                         \
+  do {                                                                       \
+    int rc_ = (call);                                                        \
+    if (rc_ != PW_OK) return rc_;                                            \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// complex helpers (interleaved re,im as double2 / float2)
+// ---------------------------------------------------------------------------
+__host__ __device__ inline double2 cmake(double r, double i) { return make_double2(r, i); }
+__host__ __device__ inline float2 cmake(float r, float i) { return make_float2(r, i); }
+template <typename T2> struct Real;
+template <> struct Real<double2> { typedef double type; };
+template <> struct Real<float2> { typedef float type; };
+
+template <typename T2>
+__host__ __device__ inline T2 cadd(T2 a, T2 b) { T2 r; r.x = a.x + b.x; r.y = a.y + b.y; return r; }
+template <typename T2>
+__host__ __device__ inline T2 csub(T2 a, T2 b) { T2 r; r.x = a.x - b.x; r.y = a.y - b.y; return r; }
+template <typename T2>
+__host__ __device__ inline T2 cmul(T2 a, T2 b) {
+  T2 r; r.x = a.x * b.x - a.y * b.y; r.y = a.x * b.y + a.y * b.x; return r;
+}
+template <typename T2>
+__host__ __device__ inline T2 cconj(T2 a) { a.y = -a.y; return a; }
+template <typename T2, typename S>
+__host__ __device__ inline T2 cscale(T2 a, S s) { a.x *= s; a.y *= s; return a; }
+
+// ---------------------------------------------------------------------------
+// 1-D FFT factorisation
+// ---------------------------------------------------------------------------
+struct Fft1D {
+  int n = 0;
+  int nstage = 0;
+  int radix[12] = {0};
+};
+bool fft_factor(int n, Fft1D &f);
+
+// ---------------------------------------------------------------------------
+// PLANEWAVE instance
+// ---------------------------------------------------------------------------
+struct pw_plan {
+  double rbas[9];
+  double gbas[9];     // GBAS(3,3) column-major: gbas[i + 3*j] = GBAS(i+1,j+1)
+  double vol = 0.0;
+  int nr1 = 0, nr2 = 0, nr3 = 0, nrl = 0;
+  int ngl = 0;
+  int ngamma = -1;    // 0-based
+  std::vector<int> igvec;   // 3*ngl
+  std::vector<int> minusg;  // 0-based
+  std::vector<double> gvec; // 3*ngl
+  std::vector<double> g2;   // ngl
+  // stripes: (y,z) lines along x that hold sphere vectors
+  int nstripe = 0;
+  std::vector<int> stripe_off; // nstripe+1, CSR into stripe_g / stripe_x
+  std::vector<int> stripe_g;   // ngl sphere indices grouped by stripe
+  std::vector<int> stripe_x;   // ngl x-index of each entry
+  std::vector<int> stripe_s;   // ngl stripe of each entry
+  std::vector<int> yz2stripe;  // nr2*nr3, -1 if empty
+  int nzp = 0;                 // z planes holding stripes
+  std::vector<int> z2iz;       // nr3, -1 if empty
+  std::vector<int> zlist;      // nzp: z of each used plane
+  int nmax[3] = {0, 0, 0};     // max |IGVEC(k)|
+  Fft1D f[3];
+  // device copies
+  int *d_minusg = nullptr, *d_igvec = nullptr;
+  int *d_stripe_off = nullptr, *d_stripe_g = nullptr, *d_stripe_x = nullptr;
+  int *d_stripe_s = nullptr;
+  int *d_yz2stripe = nullptr, *d_z2iz = nullptr, *d_zlist = nullptr;
+  double *d_g2 = nullptr, *d_gvec = nullptr;
+  double2 *d_tw[3] = {nullptr, nullptr, nullptr};
+  float2 *d_twf[3] = {nullptr, nullptr, nullptr};
+  hipStream_t stream = 0;
+  bool uploaded = false;
+  // growable work buffers
+  void *d_stripe_buf = nullptr; size_t stripe_bytes = 0;
+  void *d_plane_buf = nullptr; size_t plane_bytes = 0;
+};
+
+int plan_work(pw_plan *p, int nbatch, int prec);
+int plan_device(pw_plan *p);   // uploads maps and twiddles on first use
+
+// FFT pipelines (pw_fft.hip).  T selects the arithmetic (FP64 / FP32).
+// psi/fofg are complex128 sphere arrays (NGL per band).
+struct FftCtx;
+int fft_gtor_full(pw_plan *p, int nfft, const double2 *fofg, void *fofr, int prec, hipStream_t s);
+int fft_rtog_full(pw_plan *p, int nfft, double2 *fofg, const void *fofr, int prec, hipStream_t s);
+// hpsi = FFT^-1[v FFT psi] + (add_kin ? G^2/2 psi : 0)
+int fft_vpsi(pw_plan *p, int nb, const double2 *psi, const double *v, double2 *hpsi,
+             int add_kin, int prec, hipStream_t s);
+// rho(r) = sum_b s1[b] |psi_b(r)|^2 + s2[b] Re psi_b(r)^2; occ2 = (s1,s2) per band
+int fft_density(pw_plan *p, int nb, const double2 *psi, const double *occ2_dev,
+                double *rho, int prec, hipStream_t s);
+
+// ---------------------------------------------------------------------------
+// WAVES state
+// ---------------------------------------------------------------------------
+struct KernelTimer;
+
+struct pw_waveset {
+  pw_plan *plan = nullptr;
+  int prec = PW_PREC_FP64;
+  int nb = 0, nbh = 0, nat = 0, nsp = 0, lnxx = 0, lmnxx = 0, lmx = 0;
+  int npro = 0, nbarepro = 0;
+  std::vector<int> isp;        // 0-based
+  std::vector<int> lnx, lmnx;  // per species
+  std::vector<int> lox;        // lnxx*nsp
+  std::vector<int> ipro0;      // per atom first projector
+  std::vector<int> ibpro;      // per species first bare projector
+  // per species lmn -> (ln, lm, l), lmnxx slots each
+  std::vector<int> lmn_ln, lmn_lm, lmn_l;
+  // parameters
+  double delt = 5.0, emass = 1000.0, emasscg2 = 0.0, annee = 0.0;
+  bool have_mass = false;
+  std::vector<double> occ;
+  std::vector<double> do_ln;   // lnxx*lnxx*nsp
+  std::vector<double> do_lmn;  // per species lmnxx*lmnxx (row-major [i*lmnxx+j])
+  std::vector<double> dh;      // per atom lmnxx*lmnxx complex interleaved
+  std::vector<double> r0, rp, rm;
+  // lambda (real, Gamma point)
+  std::vector<double> rlam0, rlamm;
+  bool have_rlamm = false;
+  bool first = true;
+  // results
+  double ekin = 0, epot = 0, e1c = 0, etot = 0, wavekin1 = 0, wavekin2 = 0;
+  int ortho_iter = 0;
+  int eig_refresh = 0;
+  std::vector<double> force;   // 3*nat
+  std::vector<double> eig;     // nb
+  std::vector<double> mat, omat, oomat;
+  std::vector<double> denmat;  // lmnxx*lmnxx*nat
+  bool have_hpsi = false;
+  bool have_v = false, have_psi0 = false, have_psim = false, have_dh = false;
+  double efict = 0.0;
+  // device arrays
+  double2 *d_psi0 = nullptr, *d_psim = nullptr, *d_hpsi = nullptr, *d_opsi = nullptr;
+  double2 *d_tmp = nullptr;    // NBH*NGL scratch (Gram-Schmidt copy)
+  double2 *d_proj = nullptr, *d_oproj = nullptr, *d_xproj = nullptr;
+  double *d_pro = nullptr, *d_ylm = nullptr, *d_mpsi = nullptr;
+  double *d_v = nullptr, *d_rho = nullptr;
+  double *d_occ2 = nullptr;    // (s1,s2) per super band
+  double *d_fpm = nullptr;     // (FP,FM) of WAVES_EKIN per super band
+  double *d_fab = nullptr;     // ((f1+f2)/2,(f1-f2)/2) of WAVES_WAVEKINETIC
+  double2 *d_domat = nullptr;  // per atom DO (LMNXX x LMNXX) as complex
+  double2 *d_dhmat = nullptr;  // per atom DH (LMNXX x LMNXX)
+  int *d_atom_isp = nullptr, *d_ipro0 = nullptr, *d_ibpro = nullptr, *d_lmnx = nullptr;
+  int *d_lmn_ln = nullptr, *d_lmn_lm = nullptr, *d_lmn_l = nullptr;
+  double2 *d_phase = nullptr;  // nat*3*(2*nmaxx+1)
+  int nphase = 0;              // 2*nmaxx+1
+  double *d_part = nullptr; size_t part_bytes = 0;   // split-K partials
+  double *d_small = nullptr;   // small matrices / scalars scratch
+  double *d_small_mats = nullptr;  // MAT, OMAT, OOMAT, LAMBDA, GS work (NB*NB each)
+  size_t small_bytes = 0;
+  // orthogonalisation state (device, FP64)
+  double *d_ux = nullptr;      // NB*NB rotation basis U~
+  double *d_ud = nullptr;      // NB eigenvalues of the basis
+  bool have_basis = false;
+  hipStream_t stream = 0;
+  bool own_stream = false;
+  // timing
+  bool profile = false;
+  std::map<std::string, std::pair<double, int>> ktime;
+  std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
+};
+
+// kernel launch bracket for optional per-kernel timing
+struct ScopeTimer {
+  pw_waveset *ws; const char *name; hipEvent_t a = nullptr, b = nullptr;
+  ScopeTimer(pw_waveset *w, const char *n);
+  ~ScopeTimer();
+};
+void ws_flush_timers(pw_waveset *ws);
+
+// kernels in pw_kernels.hip
+int k_phase_tables(pw_waveset *ws, const double *r_dev_host, double2 *phase);
+int k_projections(pw_waveset *ws, const double2 *psi, const double2 *phase, double2 *proj);
+int k_addpro(pw_waveset *ws, double2 *psi, const double2 *phase, const double2 *proj);
+int k_force(pw_waveset *ws, const double2 *psi, const double2 *phase, const double2 *dedproj, double *force_host);
+int k_overlap(pw_waveset *ws, const double2 *psi1, const double2 *psi2, double *mat_host_or_dev, bool to_host, bool tid);
+int k_overlap_1c(pw_waveset *ws, const double2 *proj1, const double2 *proj2, double *mat_dev);
+int k_addopsi(pw_waveset *ws, double2 *psibar, const double2 *chi, const double *lam_dev);
+int k_addoproj(pw_waveset *ws, double2 *proj, const double2 *oproj, const double *lam_dev);
+int k_atom_matrix(pw_waveset *ws, const double2 *proj, double2 *out, const double2 *mats);
+int k_fold_occ(pw_waveset *ws, double2 *x);
+int k_ekin(pw_waveset *ws, const double2 *psi, const double *fpm_dev, double *out_host);
+int k_wavekinetic(pw_waveset *ws, const double2 *psi0, const double2 *psim, const double *fab_dev,
+                  double *out_host);
+int k_expect(pw_waveset *ws, const double2 *psi, const double2 *hpsi, double *eig_host);
+int k_propagate(pw_waveset *ws);
+int k_opsi_scale(pw_waveset *ws, double2 *opsi);
+int k_copy(pw_waveset *ws, double2 *dst, const double2 *src, size_t n);
+int k_denmat(pw_waveset *ws, const double2 *proj, double *denmat_host);
+int k_dot_vrho(pw_waveset *ws, double *out_host);
+int ws_part(pw_waveset *ws, size_t bytes);
+int ws_small(pw_waveset *ws, size_t bytes);
+
+// orthogonalisation (pw_ortho.hip)
+int ortho_x_device(pw_waveset *ws, double *d_lam, const double *d_chichi,
+                   const double *d_psipsi, const double *d_chipsi, int *iter);
+// host symmetric eigen-solver (cyclic Jacobi), a: n*n row-major symmetric
+void host_syev(int n, std::vector<double> &a, std::vector<double> &e, std::vector<double> &u);

@@ -1,0 +1,583 @@
+// Batched 3D plane-wave FFTs for gfx950.
+//
+// Semantics follow PLANEWAVE$FFT (src/paw_fft.f90:1948-2341) with the
+// conventions of LIB$FFTC8 (src/paw_library.f90:2396-2426):
+//   GTOR: f(r) = sum_G f(G) exp(+iGr)          (unnormalised)
+//   RTOG: f(G) = 1/N sum_r f(r) exp(-iGr)      (on the G sphere)
+// The reference's pruning (x only on stripes, y only on z planes holding
+// stripes) is kept, so the result equals the full 3D DFT of the scattered
+// sphere.  Unlike the reference, which runs one band and one FFTW line at a
+// time, each pass here is one batched kernel over all bands:
+//   x pass : sphere -> stripe rows (scatter fused), radix-R Stockham in LDS
+//   y pass : stripe rows -> z planes that hold stripes (zero lines implicit)
+//   z pass : full z pencils; fused with V(r) (VPSI), with the density sum, or
+//            with the full-grid read/write of the PLANEWAVE$FFT interface
+// Work buffers: stripes [band][stripe][x], planes [band][iz][y][x].
+#include "pw_common.h"
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kTX = 16;       // x columns per y/z tile
+constexpr int kPX = 16;       // stripe rows per x tile
+
+template <typename T2> struct Prec;
+template <> struct Prec<double2> {
+  typedef double T;
+  static __device__ inline double2 from(double2 v) { return v; }
+};
+template <> struct Prec<float2> {
+  typedef float T;
+  static __device__ inline float2 from(double2 v) { return make_float2((float)v.x, (float)v.y); }
+};
+
+template <typename T2>
+__device__ inline T2 czero() { T2 r; r.x = 0; r.y = 0; return r; }
+
+// a * (sign * i)
+template <typename T2>
+__device__ inline T2 mul_si(T2 a, int sign) {
+  T2 r;
+  r.x = sign > 0 ? -a.y : a.y;
+  r.y = sign > 0 ? a.x : -a.x;
+  return r;
+}
+
+// cos/sin(2 pi m / R)
+template <int R> struct Roots;
+template <> struct Roots<3> {
+  static constexpr double c[3] = {1.0, -0.5, -0.5};
+  static constexpr double s[3] = {0.0, 0.86602540378443864676, -0.86602540378443864676};
+};
+template <> struct Roots<5> {
+  static constexpr double c[5] = {1.0, 0.30901699437494742410, -0.80901699437494742410,
+                                  -0.80901699437494742410, 0.30901699437494742410};
+  static constexpr double s[5] = {0.0, 0.95105651629515357212, 0.58778525229247312917,
+                                  -0.58778525229247312917, -0.95105651629515357212};
+};
+template <> struct Roots<7> {
+  static constexpr double c[7] = {1.0, 0.62348980185873353053, -0.22252093395631440429,
+                                  -0.90096886790241912624, -0.90096886790241912624,
+                                  -0.22252093395631440429, 0.62348980185873353053};
+  static constexpr double s[7] = {0.0, 0.78183148246802980871, 0.97492791218182360702,
+                                  0.43388373911755812048, -0.43388373911755812048,
+                                  -0.97492791218182360702, -0.78183148246802980871};
+};
+template <> struct Roots<11> {
+  static constexpr double c[11] = {1.0, 0.84125353283118116886, 0.41541501300188642553,
+                                   -0.14231483827328514044, -0.65486073394528506406,
+                                   -0.95949297361449738989, -0.95949297361449738989,
+                                   -0.65486073394528506406, -0.14231483827328514044,
+                                   0.41541501300188642553, 0.84125353283118116886};
+  static constexpr double s[11] = {0.0, 0.54064081745559758211, 0.90963199535451837141,
+                                   0.98982144188093273238, 0.75574957435425828377,
+                                   0.28173255684142969771, -0.28173255684142969771,
+                                   -0.75574957435425828377, -0.98982144188093273238,
+                                   -0.90963199535451837141, -0.54064081745559758211};
+};
+
+// in-register DFT of size R: v[q] <- sum_r v[r] exp(sign 2 pi i r q / R)
+template <typename T2, int R>
+__device__ inline void dft(T2 *v, int sign) {
+  typedef typename Prec<T2>::T T;
+  if constexpr (R == 2) {
+    T2 a = v[0], b = v[1];
+    v[0] = cadd(a, b); v[1] = csub(a, b);
+  } else if constexpr (R == 4) {
+    T2 t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+    T2 t2 = cadd(v[1], v[3]), t3 = mul_si(csub(v[1], v[3]), sign);
+    v[0] = cadd(t0, t2); v[2] = csub(t0, t2);
+    v[1] = cadd(t1, t3); v[3] = csub(t1, t3);
+  } else if constexpr (R == 8) {
+    T2 e[4] = {v[0], v[2], v[4], v[6]};
+    T2 o[4] = {v[1], v[3], v[5], v[7]};
+    dft<T2, 4>(e, sign);
+    dft<T2, 4>(o, sign);
+    const T r2 = (T)0.70710678118654752440;
+    // w^q, w = exp(sign 2 pi i/8)
+    T2 w1; w1.x = r2; w1.y = sign > 0 ? r2 : -r2;
+    T2 w3; w3.x = -r2; w3.y = sign > 0 ? r2 : -r2;
+    T2 o1 = cmul(o[1], w1), o2 = mul_si(o[2], sign), o3 = cmul(o[3], w3);
+    v[0] = cadd(e[0], o[0]); v[4] = csub(e[0], o[0]);
+    v[1] = cadd(e[1], o1);   v[5] = csub(e[1], o1);
+    v[2] = cadd(e[2], o2);   v[6] = csub(e[2], o2);
+    v[3] = cadd(e[3], o3);   v[7] = csub(e[3], o3);
+  } else {
+    constexpr int H = (R - 1) / 2;
+    T2 sp[H], sm[H];
+#pragma unroll
+    for (int r = 1; r <= H; ++r) {
+      sp[r - 1] = cadd(v[r], v[R - r]);
+      sm[r - 1] = mul_si(csub(v[r], v[R - r]), sign);
+    }
+    T2 out[R];
+    out[0] = v[0];
+#pragma unroll
+    for (int r = 0; r < H; ++r) out[0] = cadd(out[0], sp[r]);
+#pragma unroll
+    for (int q = 1; q < R; ++q) {
+      T2 acc = v[0];
+#pragma unroll
+      for (int r = 1; r <= H; ++r) {
+        const int m = (r * q) % R;
+        const T c = (T)Roots<R>::c[m], s = (T)Roots<R>::s[m];
+        acc.x += sp[r - 1].x * c + sm[r - 1].x * s;
+        acc.y += sp[r - 1].y * c + sm[r - 1].y * s;
+      }
+      out[q] = acc;
+    }
+#pragma unroll
+    for (int q = 0; q < R; ++q) v[q] = out[q];
+  }
+}
+
+// One Stockham stage over npen pencils of length n in LDS.
+// Element e of pencil p lives at base[p*ps + e*es].
+template <typename T2, int R>
+__device__ inline void stage(const T2 *__restrict__ in, T2 *__restrict__ out, int n, int ns,
+                             int npen, int es, int ps, const T2 *__restrict__ tw, int sign,
+                             bool pen_fast) {
+  const int nbf = n / R;
+  const int total = nbf * npen;
+  const int twm = n / (ns * R);
+  for (int w = threadIdx.x; w < total; w += blockDim.x) {
+    int pen, j;
+    if (pen_fast) { pen = w % npen; j = w / npen; }
+    else { j = w % nbf; pen = w / nbf; }
+    const int k = j % ns;
+    const T2 *src = in + pen * ps;
+    T2 v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = src[(j + r * nbf) * es];
+    if (ns > 1) {
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        T2 t = tw[twm * k * r];
+        if (sign < 0) t.y = -t.y;
+        v[r] = cmul(v[r], t);
+      }
+    }
+    dft<T2, R>(v, sign);
+    const int idxd = (j / ns) * ns * R + k;
+    T2 *dst = out + pen * ps;
+#pragma unroll
+    for (int r = 0; r < R; ++r) dst[(idxd + r * ns) * es] = v[r];
+  }
+}
+
+// Full 1D FFT of npen pencils; returns the buffer holding the result.
+template <typename T2>
+__device__ T2 *lds_fft(T2 *b0, T2 *b1, const Fft1D &f, int npen, int es, int ps,
+                       const T2 *tw, int sign, bool pen_fast) {
+  int ns = 1;
+  T2 *in = b0, *out = b1;
+  for (int st = 0; st < f.nstage; ++st) {
+    const int r = f.radix[st];
+    switch (r) {
+      case 2: stage<T2, 2>(in, out, f.n, ns, npen, es, ps, tw, sign, pen_fast); break;
+      case 3: stage<T2, 3>(in, out, f.n, ns, npen, es, ps, tw, sign, pen_fast); break;
+      case 4: stage<T2, 4>(in, out, f.n, ns, npen, es, ps, tw, sign, pen_fast); break;
+      case 5: stage<T2, 5>(in, out, f.n, ns, npen, es, ps, tw, sign, pen_fast); break;
+      case 7: stage<T2, 7>(in, out, f.n, ns, npen, es, ps, tw, sign, pen_fast); break;
+      case 8: stage<T2, 8>(in, out, f.n, ns, npen, es, ps, tw, sign, pen_fast); break;
+      case 11: stage<T2, 11>(in, out, f.n, ns, npen, es, ps, tw, sign, pen_fast); break;
+      default: break;
+    }
+    ns *= r;
+    __syncthreads();
+    T2 *t = in; in = out; out = t;
+  }
+  return in;
+}
+
+struct Geo {
+  int nr1, nr2, nr3, ngl, nstripe, nzp;
+  const int *stripe_off, *stripe_g, *stripe_x, *stripe_s, *yz2stripe, *z2iz, *zlist;
+  const double *g2;
+};
+
+// ---- x pass, forward: sphere -> stripes --------------------------------------
+template <typename T2>
+__global__ void __launch_bounds__(kBlock) k_xfwd(Geo g, Fft1D f, const T2 *tw, const double2 *psi,
+                                                 T2 *stripes) {
+  extern __shared__ unsigned char smem_raw[];
+  T2 *b0 = reinterpret_cast<T2 *>(smem_raw);
+  T2 *b1 = b0 + kPX * g.nr1;
+  const int s0 = blockIdx.x * kPX;
+  const int band = blockIdx.y;
+  const int ns = min(kPX, g.nstripe - s0);
+  for (int i = threadIdx.x; i < kPX * g.nr1; i += blockDim.x) b0[i] = czero<T2>();
+  __syncthreads();
+  const double2 *src = psi + (size_t)band * g.ngl;
+  const int e0 = g.stripe_off[s0], e1 = g.stripe_off[s0 + ns];
+  for (int e = e0 + threadIdx.x; e < e1; e += blockDim.x)
+    b0[(g.stripe_s[e] - s0) * g.nr1 + g.stripe_x[e]] = Prec<T2>::from(src[g.stripe_g[e]]);
+  __syncthreads();
+  T2 *r = lds_fft<T2>(b0, b1, f, kPX, 1, g.nr1, tw, +1, false);
+  T2 *dst = stripes + ((size_t)band * g.nstripe + s0) * g.nr1;
+  for (int i = threadIdx.x; i < ns * g.nr1; i += blockDim.x) dst[i] = r[i];
+}
+
+// ---- x pass, inverse: stripes -> sphere (gather, scale, optional G^2/2 psi) ---
+template <typename T2>
+__global__ void __launch_bounds__(kBlock) k_xinv(Geo g, Fft1D f, const T2 *tw, const T2 *stripes,
+                                                 double2 *out, const double2 *psi, double scale,
+                                                 int add_kin) {
+  extern __shared__ unsigned char smem_raw[];
+  T2 *b0 = reinterpret_cast<T2 *>(smem_raw);
+  T2 *b1 = b0 + kPX * g.nr1;
+  const int s0 = blockIdx.x * kPX;
+  const int band = blockIdx.y;
+  const int ns = min(kPX, g.nstripe - s0);
+  const T2 *src = stripes + ((size_t)band * g.nstripe + s0) * g.nr1;
+  for (int i = threadIdx.x; i < kPX * g.nr1; i += blockDim.x)
+    b0[i] = i < ns * g.nr1 ? src[i] : czero<T2>();
+  __syncthreads();
+  T2 *r = lds_fft<T2>(b0, b1, f, kPX, 1, g.nr1, tw, -1, false);
+  const int e0 = g.stripe_off[s0], e1 = g.stripe_off[s0 + ns];
+  double2 *dst = out + (size_t)band * g.ngl;
+  const double2 *ps = psi + (size_t)band * g.ngl;
+  for (int e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
+    const int ig = g.stripe_g[e];
+    T2 v = r[(g.stripe_s[e] - s0) * g.nr1 + g.stripe_x[e]];
+    double2 o = make_double2((double)v.x * scale, (double)v.y * scale);
+    if (add_kin) {
+      const double h = 0.5 * g.g2[ig];
+      const double2 p = ps[ig];
+      o.x += h * p.x;
+      o.y += h * p.y;
+    }
+    dst[ig] = o;
+  }
+}
+
+// ---- y pass forward: stripes -> planes -----------------------------------------
+template <typename T2>
+__global__ void __launch_bounds__(kBlock) k_yfwd(Geo g, Fft1D f, const T2 *tw, const T2 *stripes,
+                                                 T2 *planes) {
+  extern __shared__ unsigned char smem_raw[];
+  T2 *b0 = reinterpret_cast<T2 *>(smem_raw);
+  T2 *b1 = b0 + kTX * g.nr2;
+  const int ntx = (g.nr1 + kTX - 1) / kTX;
+  const int iz = blockIdx.x / ntx;
+  const int x0 = (blockIdx.x % ntx) * kTX;
+  const int band = blockIdx.y;
+  const int z = g.zlist[iz];
+  const T2 *src = stripes + (size_t)band * g.nstripe * g.nr1;
+  for (int w = threadIdx.x; w < g.nr2 * kTX; w += blockDim.x) {
+    const int y = w / kTX, x = w % kTX, gx = x0 + x;
+    const int s = g.yz2stripe[y + g.nr2 * z];
+    b0[w] = (s >= 0 && gx < g.nr1) ? src[(size_t)s * g.nr1 + gx] : czero<T2>();
+  }
+  __syncthreads();
+  T2 *r = lds_fft<T2>(b0, b1, f, kTX, kTX, 1, tw, +1, true);
+  T2 *dst = planes + (((size_t)band * g.nzp + iz) * g.nr2) * g.nr1;
+  for (int w = threadIdx.x; w < g.nr2 * kTX; w += blockDim.x) {
+    const int y = w / kTX, x = w % kTX, gx = x0 + x;
+    if (gx < g.nr1) dst[(size_t)y * g.nr1 + gx] = r[w];
+  }
+}
+
+// ---- y pass inverse: planes -> stripes ------------------------------------------
+template <typename T2>
+__global__ void __launch_bounds__(kBlock) k_yinv(Geo g, Fft1D f, const T2 *tw, const T2 *planes,
+                                                 T2 *stripes) {
+  extern __shared__ unsigned char smem_raw[];
+  T2 *b0 = reinterpret_cast<T2 *>(smem_raw);
+  T2 *b1 = b0 + kTX * g.nr2;
+  const int ntx = (g.nr1 + kTX - 1) / kTX;
+  const int iz = blockIdx.x / ntx;
+  const int x0 = (blockIdx.x % ntx) * kTX;
+  const int band = blockIdx.y;
+  const int z = g.zlist[iz];
+  const T2 *src = planes + (((size_t)band * g.nzp + iz) * g.nr2) * g.nr1;
+  for (int w = threadIdx.x; w < g.nr2 * kTX; w += blockDim.x) {
+    const int y = w / kTX, x = w % kTX, gx = x0 + x;
+    b0[w] = gx < g.nr1 ? src[(size_t)y * g.nr1 + gx] : czero<T2>();
+  }
+  __syncthreads();
+  T2 *r = lds_fft<T2>(b0, b1, f, kTX, kTX, 1, tw, -1, true);
+  T2 *dst = stripes + (size_t)band * g.nstripe * g.nr1;
+  for (int w = threadIdx.x; w < g.nr2 * kTX; w += blockDim.x) {
+    const int y = w / kTX, x = w % kTX, gx = x0 + x;
+    const int s = g.yz2stripe[y + g.nr2 * z];
+    if (s >= 0 && gx < g.nr1) dst[(size_t)s * g.nr1 + gx] = r[w];
+  }
+}
+
+// z pencils of one (y, x tile) from the plane buffer (zero on empty planes)
+template <typename T2>
+__device__ inline void load_zpencils(const Geo &g, const T2 *planes_band, int y, int x0, T2 *b0) {
+  for (int w = threadIdx.x; w < g.nr3 * kTX; w += blockDim.x) {
+    const int z = w / kTX, x = w % kTX, gx = x0 + x;
+    const int iz = g.z2iz[z];
+    b0[w] = (iz >= 0 && gx < g.nr1) ? planes_band[((size_t)iz * g.nr2 + y) * g.nr1 + gx]
+                                    : czero<T2>();
+  }
+}
+
+template <typename T2>
+__device__ inline void store_zpencils(const Geo &g, T2 *planes_band, int y, int x0, const T2 *r) {
+  for (int w = threadIdx.x; w < g.nr3 * kTX; w += blockDim.x) {
+    const int z = w / kTX, x = w % kTX, gx = x0 + x;
+    const int iz = g.z2iz[z];
+    if (iz >= 0 && gx < g.nr1) planes_band[((size_t)iz * g.nr2 + y) * g.nr1 + gx] = r[w];
+  }
+}
+
+// ---- z pass: forward z, times V(r), inverse z (WAVES_VPSI core) -------------------
+template <typename T2>
+__global__ void __launch_bounds__(kBlock) k_zvz(Geo g, Fft1D f, const T2 *tw, T2 *planes,
+                                                const double *v) {
+  typedef typename Prec<T2>::T T;
+  extern __shared__ unsigned char smem_raw[];
+  T2 *b0 = reinterpret_cast<T2 *>(smem_raw);
+  T2 *b1 = b0 + kTX * g.nr3;
+  const int ntx = (g.nr1 + kTX - 1) / kTX;
+  const int y = blockIdx.x / ntx;
+  const int x0 = (blockIdx.x % ntx) * kTX;
+  const int band = blockIdx.y;
+  T2 *pb = planes + (size_t)band * g.nzp * g.nr2 * g.nr1;
+  load_zpencils<T2>(g, pb, y, x0, b0);
+  __syncthreads();
+  T2 *r = lds_fft<T2>(b0, b1, f, kTX, kTX, 1, tw, +1, true);
+  for (int w = threadIdx.x; w < g.nr3 * kTX; w += blockDim.x) {
+    const int z = w / kTX, x = w % kTX, gx = x0 + x;
+    const T vv = gx < g.nr1 ? (T)v[((size_t)z * g.nr2 + y) * g.nr1 + gx] : (T)0;
+    r[w].x *= vv;
+    r[w].y *= vv;
+  }
+  __syncthreads();
+  T2 *other = (r == b0) ? b1 : b0;
+  T2 *r2 = lds_fft<T2>(r, other, f, kTX, kTX, 1, tw, -1, true);
+  store_zpencils<T2>(g, pb, y, x0, r2);
+}
+
+// ---- z pass: density sum over all bands --------------------------------------------
+template <typename T2>
+__global__ void __launch_bounds__(kBlock) k_zdens(Geo g, Fft1D f, const T2 *tw, const T2 *planes,
+                                                  int nb, const double *occ2, double *rho) {
+  extern __shared__ unsigned char smem_raw[];
+  T2 *b0 = reinterpret_cast<T2 *>(smem_raw);
+  T2 *b1 = b0 + kTX * g.nr3;
+  double *acc = reinterpret_cast<double *>(b1 + kTX * g.nr3);
+  const int ntx = (g.nr1 + kTX - 1) / kTX;
+  const int y = blockIdx.x / ntx;
+  const int x0 = (blockIdx.x % ntx) * kTX;
+  for (int w = threadIdx.x; w < g.nr3 * kTX; w += blockDim.x) acc[w] = 0.0;
+  for (int band = 0; band < nb; ++band) {
+    const double s1 = occ2[2 * band], s2 = occ2[2 * band + 1];
+    if (s1 == 0.0 && s2 == 0.0) continue;   // uniform across the block
+    const T2 *pb = planes + (size_t)band * g.nzp * g.nr2 * g.nr1;
+    __syncthreads();
+    load_zpencils<T2>(g, pb, y, x0, b0);
+    __syncthreads();
+    T2 *r = lds_fft<T2>(b0, b1, f, kTX, kTX, 1, tw, +1, true);
+    for (int w = threadIdx.x; w < g.nr3 * kTX; w += blockDim.x) {
+      const double re = r[w].x, im = r[w].y;
+      // Re(psi (s1 conj(psi) + s2 psi)) = s1 |psi|^2 + s2 (re^2 - im^2)
+      acc[w] += s1 * (re * re + im * im) + s2 * (re * re - im * im);
+    }
+  }
+  __syncthreads();
+  for (int w = threadIdx.x; w < g.nr3 * kTX; w += blockDim.x) {
+    const int z = w / kTX, x = w % kTX, gx = x0 + x;
+    if (gx < g.nr1) rho[((size_t)z * g.nr2 + y) * g.nr1 + gx] = acc[w];
+  }
+}
+
+// ---- z pass with full-grid output (PLANEWAVE$FFT 'GTOR') ----------------------------
+template <typename T2>
+__global__ void __launch_bounds__(kBlock) k_zout(Geo g, Fft1D f, const T2 *tw, const T2 *planes,
+                                                 T2 *fofr) {
+  extern __shared__ unsigned char smem_raw[];
+  T2 *b0 = reinterpret_cast<T2 *>(smem_raw);
+  T2 *b1 = b0 + kTX * g.nr3;
+  const int ntx = (g.nr1 + kTX - 1) / kTX;
+  const int y = blockIdx.x / ntx;
+  const int x0 = (blockIdx.x % ntx) * kTX;
+  const int band = blockIdx.y;
+  load_zpencils<T2>(g, planes + (size_t)band * g.nzp * g.nr2 * g.nr1, y, x0, b0);
+  __syncthreads();
+  T2 *r = lds_fft<T2>(b0, b1, f, kTX, kTX, 1, tw, +1, true);
+  T2 *dst = fofr + (size_t)band * g.nr1 * g.nr2 * g.nr3;
+  for (int w = threadIdx.x; w < g.nr3 * kTX; w += blockDim.x) {
+    const int z = w / kTX, x = w % kTX, gx = x0 + x;
+    if (gx < g.nr1) dst[((size_t)z * g.nr2 + y) * g.nr1 + gx] = r[w];
+  }
+}
+
+// ---- z pass with full-grid input (PLANEWAVE$FFT 'RTOG') -----------------------------
+template <typename T2>
+__global__ void __launch_bounds__(kBlock) k_zin(Geo g, Fft1D f, const T2 *tw, const T2 *fofr,
+                                                T2 *planes) {
+  extern __shared__ unsigned char smem_raw[];
+  T2 *b0 = reinterpret_cast<T2 *>(smem_raw);
+  T2 *b1 = b0 + kTX * g.nr3;
+  const int ntx = (g.nr1 + kTX - 1) / kTX;
+  const int y = blockIdx.x / ntx;
+  const int x0 = (blockIdx.x % ntx) * kTX;
+  const int band = blockIdx.y;
+  const T2 *src = fofr + (size_t)band * g.nr1 * g.nr2 * g.nr3;
+  for (int w = threadIdx.x; w < g.nr3 * kTX; w += blockDim.x) {
+    const int z = w / kTX, x = w % kTX, gx = x0 + x;
+    b0[w] = gx < g.nr1 ? src[((size_t)z * g.nr2 + y) * g.nr1 + gx] : czero<T2>();
+  }
+  __syncthreads();
+  T2 *r = lds_fft<T2>(b0, b1, f, kTX, kTX, 1, tw, -1, true);
+  store_zpencils<T2>(g, planes + (size_t)band * g.nzp * g.nr2 * g.nr1, y, x0, r);
+}
+
+Geo make_geo(const pw_plan *p) {
+  Geo g;
+  g.nr1 = p->nr1; g.nr2 = p->nr2; g.nr3 = p->nr3; g.ngl = p->ngl;
+  g.nstripe = p->nstripe; g.nzp = p->nzp;
+  g.stripe_off = p->d_stripe_off; g.stripe_g = p->d_stripe_g; g.stripe_x = p->d_stripe_x;
+  g.stripe_s = p->d_stripe_s; g.yz2stripe = p->d_yz2stripe; g.z2iz = p->d_z2iz;
+  g.zlist = p->d_zlist; g.g2 = p->d_g2;
+  return g;
+}
+
+template <typename T2> const T2 *twid(const pw_plan *p, int k);
+template <> const double2 *twid<double2>(const pw_plan *p, int k) { return p->d_tw[k]; }
+template <> const float2 *twid<float2>(const pw_plan *p, int k) { return p->d_twf[k]; }
+
+template <typename T2>
+int run_forward_xy(pw_plan *p, const Geo &g, int nb, const double2 *psi, hipStream_t s) {
+  T2 *stripes = (T2 *)p->d_stripe_buf;
+  T2 *planes = (T2 *)p->d_plane_buf;
+  const int ntx = (p->nr1 + kTX - 1) / kTX;
+  dim3 gx((p->nstripe + kPX - 1) / kPX, nb);
+  k_xfwd<T2><<<gx, kBlock, 2 * kPX * p->nr1 * sizeof(T2), s>>>(g, p->f[0], twid<T2>(p, 0), psi,
+                                                               stripes);
+  dim3 gy(p->nzp * ntx, nb);
+  k_yfwd<T2><<<gy, kBlock, 2 * kTX * p->nr2 * sizeof(T2), s>>>(g, p->f[1], twid<T2>(p, 1),
+                                                               stripes, planes);
+  PW_HIP(hipGetLastError());
+  return PW_OK;
+}
+
+template <typename T2>
+int run_inverse_yx(pw_plan *p, const Geo &g, int nb, double2 *out, const double2 *psi,
+                   double scale, int add_kin, hipStream_t s) {
+  T2 *stripes = (T2 *)p->d_stripe_buf;
+  T2 *planes = (T2 *)p->d_plane_buf;
+  const int ntx = (p->nr1 + kTX - 1) / kTX;
+  dim3 gy(p->nzp * ntx, nb);
+  k_yinv<T2><<<gy, kBlock, 2 * kTX * p->nr2 * sizeof(T2), s>>>(g, p->f[1], twid<T2>(p, 1),
+                                                               planes, stripes);
+  dim3 gx((p->nstripe + kPX - 1) / kPX, nb);
+  k_xinv<T2><<<gx, kBlock, 2 * kPX * p->nr1 * sizeof(T2), s>>>(g, p->f[0], twid<T2>(p, 0),
+                                                               stripes, out, psi, scale, add_kin);
+  PW_HIP(hipGetLastError());
+  return PW_OK;
+}
+
+template <typename T2>
+int vpsi_t(pw_plan *p, int nb, const double2 *psi, const double *v, double2 *hpsi, int add_kin,
+           hipStream_t s) {
+  Geo g = make_geo(p);
+  PW_TRY(run_forward_xy<T2>(p, g, nb, psi, s));
+  const int ntx = (p->nr1 + kTX - 1) / kTX;
+  dim3 gz(p->nr2 * ntx, nb);
+  k_zvz<T2><<<gz, kBlock, 2 * kTX * p->nr3 * sizeof(T2), s>>>(g, p->f[2], twid<T2>(p, 2),
+                                                              (T2 *)p->d_plane_buf, v);
+  PW_HIP(hipGetLastError());
+  return run_inverse_yx<T2>(p, g, nb, hpsi, psi, 1.0 / (double)p->nrl, add_kin, s);
+}
+
+template <typename T2>
+int density_t(pw_plan *p, int nb, const double2 *psi, const double *occ2, double *rho,
+              hipStream_t s) {
+  Geo g = make_geo(p);
+  PW_TRY(run_forward_xy<T2>(p, g, nb, psi, s));
+  const int ntx = (p->nr1 + kTX - 1) / kTX;
+  size_t lds = 2 * kTX * p->nr3 * sizeof(T2) + kTX * p->nr3 * sizeof(double);
+  k_zdens<T2><<<p->nr2 * ntx, kBlock, lds, s>>>(g, p->f[2], twid<T2>(p, 2),
+                                               (const T2 *)p->d_plane_buf, nb, occ2, rho);
+  PW_HIP(hipGetLastError());
+  return PW_OK;
+}
+
+template <typename T2>
+int gtor_t(pw_plan *p, int nb, const double2 *fofg, T2 *fofr, hipStream_t s) {
+  Geo g = make_geo(p);
+  PW_TRY(run_forward_xy<T2>(p, g, nb, fofg, s));
+  const int ntx = (p->nr1 + kTX - 1) / kTX;
+  dim3 gz(p->nr2 * ntx, nb);
+  k_zout<T2><<<gz, kBlock, 2 * kTX * p->nr3 * sizeof(T2), s>>>(g, p->f[2], twid<T2>(p, 2),
+                                                               (const T2 *)p->d_plane_buf, fofr);
+  PW_HIP(hipGetLastError());
+  return PW_OK;
+}
+
+template <typename T2>
+int rtog_t(pw_plan *p, int nb, double2 *fofg, const T2 *fofr, hipStream_t s) {
+  Geo g = make_geo(p);
+  const int ntx = (p->nr1 + kTX - 1) / kTX;
+  dim3 gz(p->nr2 * ntx, nb);
+  k_zin<T2><<<gz, kBlock, 2 * kTX * p->nr3 * sizeof(T2), s>>>(g, p->f[2], twid<T2>(p, 2), fofr,
+                                                              (T2 *)p->d_plane_buf);
+  PW_HIP(hipGetLastError());
+  return run_inverse_yx<T2>(p, g, nb, fofg, nullptr, 1.0 / (double)p->nrl, 0, s);
+}
+
+}  // namespace
+
+// Largest batch per launch sequence; keeps work buffers bounded.
+static int chunk_of(const pw_plan *p, int prec) {
+  size_t es = prec == PW_PREC_FP64 ? sizeof(double2) : sizeof(float2);
+  size_t per = es * ((size_t)p->nstripe * p->nr1 + (size_t)p->nzp * p->nr2 * p->nr1);
+  size_t budget = (size_t)1 << 30;   // 1 GiB of work buffers
+  size_t c = budget / std::max<size_t>(per, 1);
+  return (int)std::max<size_t>(1, std::min<size_t>(c, 4096));
+}
+
+int fft_vpsi(pw_plan *p, int nb, const double2 *psi, const double *v, double2 *hpsi, int add_kin,
+             int prec, hipStream_t s) {
+  const int c = chunk_of(p, prec);
+  PW_TRY(plan_work(p, std::min(nb, c), prec));
+  for (int b0 = 0; b0 < nb; b0 += c) {
+    const int n = std::min(c, nb - b0);
+    const double2 *ps = psi + (size_t)b0 * p->ngl;
+    double2 *hs = hpsi + (size_t)b0 * p->ngl;
+    if (prec == PW_PREC_FP64) PW_TRY(vpsi_t<double2>(p, n, ps, v, hs, add_kin, s));
+    else PW_TRY(vpsi_t<float2>(p, n, ps, v, hs, add_kin, s));
+  }
+  return PW_OK;
+}
+
+int fft_density(pw_plan *p, int nb, const double2 *psi, const double *occ2, double *rho, int prec,
+                hipStream_t s) {
+  if (nb > chunk_of(p, prec))
+    return pw_fail(PW_ERR_UNSUPPORTED, "WAVES_DENSITY: band set exceeds the work-buffer budget");
+  PW_TRY(plan_work(p, nb, prec));
+  if (prec == PW_PREC_FP64) return density_t<double2>(p, nb, psi, occ2, rho, s);
+  return density_t<float2>(p, nb, psi, occ2, rho, s);
+}
+
+int fft_gtor_full(pw_plan *p, int nfft, const double2 *fofg, void *fofr, int prec, hipStream_t s) {
+  const int c = chunk_of(p, prec);
+  PW_TRY(plan_work(p, std::min(nfft, c), prec));
+  for (int b0 = 0; b0 < nfft; b0 += c) {
+    const int n = std::min(c, nfft - b0);
+    if (prec == PW_PREC_FP64)
+      PW_TRY(gtor_t<double2>(p, n, fofg + (size_t)b0 * p->ngl, (double2 *)fofr + (size_t)b0 * p->nrl, s));
+    else
+      PW_TRY(gtor_t<float2>(p, n, fofg + (size_t)b0 * p->ngl, (float2 *)fofr + (size_t)b0 * p->nrl, s));
+  }
+  return PW_OK;
+}
+
+int fft_rtog_full(pw_plan *p, int nfft, double2 *fofg, const void *fofr, int prec, hipStream_t s) {
+  const int c = chunk_of(p, prec);
+  PW_TRY(plan_work(p, std::min(nfft, c), prec));
+  for (int b0 = 0; b0 < nfft; b0 += c) {
+    const int n = std::min(c, nfft - b0);
+    if (prec == PW_PREC_FP64)
+      PW_TRY(rtog_t<double2>(p, n, fofg + (size_t)b0 * p->ngl, (const double2 *)fofr + (size_t)b0 * p->nrl, s));
+    else
+      PW_TRY(rtog_t<float2>(p, n, fofg + (size_t)b0 * p->ngl, (const float2 *)fofr + (size_t)b0 * p->nrl, s));
+  }
+  return PW_OK;
+}

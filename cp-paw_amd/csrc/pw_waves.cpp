@@ -1,0 +1,740 @@
+// WAVES object on the device: set-up, the CP step and the C ABI.
+//
+// Step order follows WAVES$ETOT (src/paw_waves1.f90:1610-2233),
+// WAVES$PROPAGATE (:6479), WAVES$ORTHOGONALIZE (src/paw_waves2.f90:11) and
+// WAVES$SWITCH (src/paw_waves2.f90:2180) for one k-point, one spin, NDIM=1
+// and super wave functions.  The density->potential step and AUGMENTATION are
+// outside this library: the caller provides V(r) ("VOFR") and the one-centre
+// matrices DH and DO; the energies reported are the ones the wave-function
+// step computes from them (EKIN, int V rho, sum DH D).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "pw_common.h"
+
+int k_fold_occ(pw_waveset *ws, double2 *x);
+int k_expect(pw_waveset *ws, const double2 *psi, const double2 *hpsi, double *eig_host);
+
+// ---------------------------------------------------------------------------
+// timers
+// ---------------------------------------------------------------------------
+ScopeTimer::ScopeTimer(pw_waveset *w, const char *n) : ws(w), name(n) {
+  if (!ws || !ws->profile) return;
+  if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) { a = b = nullptr; return; }
+  (void)hipEventRecord(a, ws->stream);
+}
+ScopeTimer::~ScopeTimer() {
+  if (!a) return;
+  (void)hipEventRecord(b, ws->stream);
+  ws->pending.push_back({name, {a, b}});
+}
+void ws_flush_timers(pw_waveset *ws) {
+  if (ws->pending.empty()) return;
+  (void)hipStreamSynchronize(ws->stream);
+  for (auto &p : ws->pending) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, p.second.first, p.second.second) == hipSuccess) {
+      auto &t = ws->ktime[p.first];
+      t.first += ms;
+      t.second += 1;
+    }
+    (void)hipEventDestroy(p.second.first);
+    (void)hipEventDestroy(p.second.second);
+  }
+  ws->pending.clear();
+}
+
+// ---------------------------------------------------------------------------
+// helpers
+// ---------------------------------------------------------------------------
+static int dalloc(void **p, size_t bytes) {
+  PW_HIP(hipMalloc(p, std::max<size_t>(bytes, 8)));
+  PW_HIP(hipMemset(*p, 0, std::max<size_t>(bytes, 8)));
+  return PW_OK;
+}
+
+static int need(bool ok, const char *what) {
+  if (!ok) return pw_fail(PW_ERR_STATE, std::string("WAVES: ") + what + " has not been set");
+  return PW_OK;
+}
+
+static double *d_mat(pw_waveset *ws, int k) {   // 0 MAT, 1 OMAT, 2 OOMAT, 3 LAMBDA, 4 GS
+  return ws->d_small_mats + (size_t)k * ws->nb * ws->nb;
+}
+
+static int upload_occ(pw_waveset *ws) {
+  const int nbh = ws->nbh;
+  std::vector<double> occ2(2 * nbh), fpm(2 * nbh), fab(2 * nbh);
+  for (int b = 0; b < nbh; ++b) {
+    const double f1 = ws->occ[2 * b];
+    const double f2 = (2 * b + 1 < ws->nb) ? ws->occ[2 * b + 1] : 0.0;
+    occ2[2 * b] = 0.5 * (f1 + f2);
+    occ2[2 * b + 1] = 0.5 * (f1 - f2);
+    // WAVES_EKIN (src/paw_waves1.f90:2848-2884)
+    double fp = 0.5 * (f1 + f2), fm = 0.5 * (f1 - f2);
+    if (std::fabs(fm) < 1.0e-12) fm = 0.0;
+    fpm[2 * b] = fp;
+    fpm[2 * b + 1] = fm;
+    // WAVES_WAVEKINETIC (src/paw_waves1.f90:6778-6822)
+    if (f1 == 0.0 && f2 == 0.0) { fab[2 * b] = 0.0; fab[2 * b + 1] = 0.0; }
+    else { fab[2 * b] = 0.5 * (f1 + f2); fab[2 * b + 1] = (f1 - f2 == 0.0) ? 0.0 : 0.5 * (f1 - f2); }
+  }
+  PW_HIP(hipMemcpy(ws->d_occ2, occ2.data(), occ2.size() * sizeof(double), hipMemcpyHostToDevice));
+  PW_HIP(hipMemcpy(ws->d_fpm, fpm.data(), fpm.size() * sizeof(double), hipMemcpyHostToDevice));
+  PW_HIP(hipMemcpy(ws->d_fab, fab.data(), fab.size() * sizeof(double), hipMemcpyHostToDevice));
+  return PW_OK;
+}
+
+static int upload_do(pw_waveset *ws) {
+  // DO(LN,LN) -> per atom LMN x LMN (WAVES_OPROJ, src/paw_waves2.f90:497-542)
+  const int L = ws->lmnxx;
+  std::vector<double2> m((size_t)ws->nat * L * L, make_double2(0.0, 0.0));
+  for (int a = 0; a < ws->nat; ++a) {
+    const int isp = ws->isp[a];
+    int o1 = 0;
+    for (int ln1 = 0; ln1 < ws->lnx[isp]; ++ln1) {
+      const int l1 = ws->lox[ln1 + ws->lnxx * isp];
+      int o2 = 0;
+      for (int ln2 = 0; ln2 < ws->lnx[isp]; ++ln2) {
+        const int l2 = ws->lox[ln2 + ws->lnxx * isp];
+        if (l1 == l2) {
+          const double v = ws->do_ln[ln1 + ws->lnxx * (ln2 + ws->lnxx * isp)];
+          for (int mm = 0; mm < 2 * l1 + 1; ++mm)
+            m[((size_t)a * L + o1 + mm) * L + o2 + mm] = make_double2(v, 0.0);
+        }
+        o2 += 2 * l2 + 1;
+      }
+      o1 += 2 * l1 + 1;
+    }
+  }
+  PW_HIP(hipMemcpy(ws->d_domat, m.data(), m.size() * sizeof(double2), hipMemcpyHostToDevice));
+  return PW_OK;
+}
+
+static int ylm_table(pw_waveset *ws) {
+  // real spherical harmonics, order 1,x,z,y for l<=1 (src/paw_spherical.f90:66-77)
+  pw_plan *p = ws->plan;
+  if (ws->lmx > 4) return pw_fail(PW_ERR_UNSUPPORTED, "WAVES: projectors with l>1 are not supported");
+  std::vector<double> y((size_t)ws->lmx * p->ngl, 0.0);
+  const double c0 = 1.0 / std::sqrt(4.0 * M_PI), c1 = std::sqrt(3.0 / (4.0 * M_PI));
+  for (int g = 0; g < p->ngl; ++g) {
+    const double gx = p->gvec[3 * g], gy = p->gvec[3 * g + 1], gz = p->gvec[3 * g + 2];
+    const double r = std::sqrt(gx * gx + gy * gy + gz * gz);
+    y[g] = c0;
+    if (ws->lmx > 1 && std::fabs(r) >= 1.0e-12) {
+      y[(size_t)1 * p->ngl + g] = c1 * gx / r;
+      y[(size_t)2 * p->ngl + g] = c1 * gz / r;
+      y[(size_t)3 * p->ngl + g] = c1 * gy / r;
+    }
+  }
+  PW_HIP(hipMemcpy(ws->d_ylm, y.data(), y.size() * sizeof(double), hipMemcpyHostToDevice));
+  return PW_OK;
+}
+
+static int upload_mass(pw_waveset *ws) {
+  // WAVES_PSIMASS (src/paw_waves1.f90:6641-6666), no bucket
+  pw_plan *p = ws->plan;
+  std::vector<double> m(p->ngl);
+  for (int g = 0; g < p->ngl; ++g) m[g] = ws->emass * (1.0 + ws->emasscg2 * p->g2[g]);
+  PW_HIP(hipMemcpy(ws->d_mpsi, m.data(), m.size() * sizeof(double), hipMemcpyHostToDevice));
+  ws->have_mass = true;
+  return PW_OK;
+}
+
+static int phase_at(pw_waveset *ws, const std::vector<double> &r) {
+  return k_phase_tables(ws, r.data(), ws->d_phase);
+}
+
+// DEDPROJ (src/paw_waves1.f90:5592-5691) into d_xproj
+static int dedproj(pw_waveset *ws) {
+  const int nb = ws->nb;
+  PW_TRY(k_atom_matrix(ws, ws->d_proj, ws->d_xproj, ws->d_dhmat));
+  PW_TRY(k_fold_occ(ws, ws->d_xproj));
+  // LAMBDA = -RLAM0 (f_i+f_j)/2
+  std::vector<double> lam((size_t)nb * nb);
+  for (int i = 0; i < nb; ++i)
+    for (int j = 0; j < nb; ++j)
+      lam[(size_t)i * nb + j] = -ws->rlam0[(size_t)i * nb + j] * 0.5 * (ws->occ[i] + ws->occ[j]);
+  PW_HIP(hipMemcpyAsync(d_mat(ws, 3), lam.data(), lam.size() * sizeof(double), hipMemcpyHostToDevice,
+                        ws->stream));
+  PW_TRY(k_atom_matrix(ws, ws->d_proj, ws->d_oproj, ws->d_domat));
+  PW_TRY(k_addoproj(ws, ws->d_xproj, ws->d_oproj, d_mat(ws, 3)));
+  PW_HIP(hipStreamSynchronize(ws->stream));
+  return PW_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Gram-Schmidt (WAVES_GRAMSCHMIDT, src/paw_waves2.f90:2025-2177)
+// ---------------------------------------------------------------------------
+static int gramschmidt_one(pw_waveset *ws, double2 *psi, const std::vector<double> &r) {
+  const int nb = ws->nb;
+  pw_plan *p = ws->plan;
+  PW_TRY(phase_at(ws, r));
+  PW_TRY(k_projections(ws, psi, ws->d_phase, ws->d_oproj));
+  PW_TRY(k_overlap(ws, psi, psi, d_mat(ws, 4), false, true));
+  PW_TRY(k_overlap_1c(ws, ws->d_oproj, ws->d_oproj, d_mat(ws, 4)));
+  std::vector<double> s((size_t)nb * nb);
+  PW_HIP(hipMemcpyAsync(s.data(), d_mat(ws, 4), s.size() * sizeof(double), hipMemcpyDeviceToHost, ws->stream));
+  PW_HIP(hipStreamSynchronize(ws->stream));
+  // Cholesky S = R^T R (R upper, positive diagonal); T = R^-1; X = T - 1.
+  // WAVES_ORTHO_Y_C with PHI=CHI builds this unique triangular transform.
+  std::vector<double> R((size_t)nb * nb, 0.0);
+  for (int j = 0; j < nb; ++j) {
+    for (int i = 0; i <= j; ++i) {
+      double v = s[(size_t)i * nb + j];
+      for (int k = 0; k < i; ++k) v -= R[(size_t)k * nb + i] * R[(size_t)k * nb + j];
+      if (i == j) {
+        if (!(v > 0.0)) return pw_fail(PW_ERR_NOCONV, "WAVES_GRAMSCHMIDT: overlap not positive definite");
+        R[(size_t)i * nb + j] = std::sqrt(v);
+      } else {
+        R[(size_t)i * nb + j] = v / R[(size_t)i * nb + i];
+      }
+    }
+  }
+  std::vector<double> T((size_t)nb * nb, 0.0);
+  for (int j = 0; j < nb; ++j) {          // column j of R^-1 (upper triangular)
+    T[(size_t)j * nb + j] = 1.0 / R[(size_t)j * nb + j];
+    for (int i = j - 1; i >= 0; --i) {
+      double v = 0.0;
+      for (int k = i + 1; k <= j; ++k) v += R[(size_t)i * nb + k] * T[(size_t)k * nb + j];
+      T[(size_t)i * nb + j] = -v / R[(size_t)i * nb + i];
+    }
+  }
+  for (int i = 0; i < nb; ++i) T[(size_t)i * nb + i] -= 1.0;
+  PW_HIP(hipMemcpyAsync(d_mat(ws, 3), T.data(), T.size() * sizeof(double), hipMemcpyHostToDevice, ws->stream));
+  PW_TRY(k_copy(ws, ws->d_tmp, psi, (size_t)ws->nbh * p->ngl));
+  PW_TRY(k_addopsi(ws, psi, ws->d_tmp, d_mat(ws, 3)));
+  PW_HIP(hipStreamSynchronize(ws->stream));
+  return PW_OK;
+}
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" int pw_waves_create(pw_plan *plan, int nb, int nat, int nsp, const int *isp,
+                               const int *lnx, int lnxx, const int *lox, int prec, pw_waveset **out) {
+  if (!plan || !isp || !lnx || !lox || !out) return pw_fail(PW_ERR_ARG, "pw_waves_create: null argument");
+  if (nb <= 0 || nb % 2 != 0)
+    return pw_fail(PW_ERR_UNSUPPORTED, "pw_waves_create: NB must be positive and even for super wave functions");
+  if (nat <= 0 || nsp <= 0 || lnxx <= 0) return pw_fail(PW_ERR_ARG, "pw_waves_create: bad sizes");
+  if (prec != PW_PREC_FP64 && prec != PW_PREC_MIXED) return pw_fail(PW_ERR_ARG, "pw_waves_create: bad precision");
+  PW_TRY(plan_device(plan));
+  pw_waveset *ws = new pw_waveset();
+  ws->plan = plan;
+  ws->prec = prec;
+  ws->nb = nb; ws->nbh = (nb + 1) / 2; ws->nat = nat; ws->nsp = nsp; ws->lnxx = lnxx;
+  ws->isp.resize(nat);
+  for (int a = 0; a < nat; ++a) {
+    if (isp[a] < 1 || isp[a] > nsp) { delete ws; return pw_fail(PW_ERR_ARG, "pw_waves_create: ISP out of range"); }
+    ws->isp[a] = isp[a] - 1;
+  }
+  ws->lnx.assign(lnx, lnx + nsp);
+  ws->lox.assign(lox, lox + (size_t)lnxx * nsp);
+  ws->lmnx.assign(nsp, 0);
+  ws->ibpro.assign(nsp, 0);
+  int lmax = 0;
+  for (int s = 0; s < nsp; ++s) {
+    if (lnx[s] < 0 || lnx[s] > lnxx) { delete ws; return pw_fail(PW_ERR_ARG, "pw_waves_create: LNX > LNXX"); }
+    ws->ibpro[s] = ws->nbarepro;
+    ws->nbarepro += lnx[s];
+    for (int ln = 0; ln < lnx[s]; ++ln) {
+      const int l = lox[ln + lnxx * s];
+      ws->lmnx[s] += 2 * l + 1;
+      lmax = std::max(lmax, l);
+    }
+    ws->lmnxx = std::max(ws->lmnxx, ws->lmnx[s]);
+  }
+  ws->lmx = (lmax + 1) * (lmax + 1);
+  ws->lmn_ln.assign((size_t)nsp * ws->lmnxx, 0);
+  ws->lmn_lm.assign((size_t)nsp * ws->lmnxx, 0);
+  ws->lmn_l.assign((size_t)nsp * ws->lmnxx, 0);
+  for (int s = 0; s < nsp; ++s) {
+    int lmn = 0;
+    for (int ln = 0; ln < lnx[s]; ++ln) {
+      const int l = lox[ln + lnxx * s];
+      for (int m = 0; m < 2 * l + 1; ++m, ++lmn) {
+        ws->lmn_ln[s * ws->lmnxx + lmn] = ln;
+        ws->lmn_lm[s * ws->lmnxx + lmn] = l * l + m;
+        ws->lmn_l[s * ws->lmnxx + lmn] = l;
+      }
+    }
+  }
+  ws->ipro0.assign(nat, 0);
+  for (int a = 0; a < nat; ++a) {
+    ws->ipro0[a] = ws->npro;
+    ws->npro += ws->lmnx[ws->isp[a]];
+  }
+  ws->emasscg2 = 0.0;
+  ws->rlam0.assign((size_t)nb * nb, 0.0);
+  ws->force.assign(3 * nat, 0.0);
+  ws->eig.assign(nb, 0.0);
+  ws->denmat.assign((size_t)ws->lmnxx * ws->lmnxx * nat, 0.0);
+  const int nmaxx = std::max(plan->nmax[0], std::max(plan->nmax[1], plan->nmax[2]));
+  ws->nphase = 2 * nmaxx + 1;
+  auto fail = [&](int rc) { pw_waves_destroy(ws); return rc; };
+  const size_t psz = (size_t)ws->nbh * plan->ngl * sizeof(double2);
+  const size_t jsz = (size_t)ws->npro * ws->nbh * sizeof(double2);
+  int rc = PW_OK;
+  double2 **cbufs[] = {&ws->d_psi0, &ws->d_psim, &ws->d_hpsi, &ws->d_opsi, &ws->d_tmp};
+  for (auto b : cbufs) if (rc == PW_OK) rc = dalloc((void **)b, psz);
+  double2 **pbufs[] = {&ws->d_proj, &ws->d_oproj, &ws->d_xproj};
+  for (auto b : pbufs) if (rc == PW_OK) rc = dalloc((void **)b, jsz);
+  if (rc == PW_OK) rc = dalloc((void **)&ws->d_pro, (size_t)ws->nbarepro * plan->ngl * sizeof(double));
+  if (rc == PW_OK) rc = dalloc((void **)&ws->d_ylm, (size_t)ws->lmx * plan->ngl * sizeof(double));
+  if (rc == PW_OK) rc = dalloc((void **)&ws->d_mpsi, plan->ngl * sizeof(double));
+  if (rc == PW_OK) rc = dalloc((void **)&ws->d_v, plan->nrl * sizeof(double));
+  if (rc == PW_OK) rc = dalloc((void **)&ws->d_rho, plan->nrl * sizeof(double));
+  if (rc == PW_OK) rc = dalloc((void **)&ws->d_occ2, 2 * ws->nbh * sizeof(double));
+  if (rc == PW_OK) rc = dalloc((void **)&ws->d_fpm, 2 * ws->nbh * sizeof(double));
+  if (rc == PW_OK) rc = dalloc((void **)&ws->d_fab, 2 * ws->nbh * sizeof(double));
+  if (rc == PW_OK) rc = dalloc((void **)&ws->d_domat, (size_t)nat * ws->lmnxx * ws->lmnxx * sizeof(double2));
+  if (rc == PW_OK) rc = dalloc((void **)&ws->d_dhmat, (size_t)nat * ws->lmnxx * ws->lmnxx * sizeof(double2));
+  if (rc == PW_OK) rc = dalloc((void **)&ws->d_phase, (size_t)nat * 3 * ws->nphase * sizeof(double2));
+  if (rc == PW_OK) rc = dalloc((void **)&ws->d_small_mats, (size_t)6 * nb * nb * sizeof(double));
+  std::vector<int> iv;
+  auto up = [&](int **dst, const std::vector<int> &src) {
+    if (rc != PW_OK) return;
+    rc = dalloc((void **)dst, src.size() * sizeof(int));
+    if (rc == PW_OK && !src.empty() &&
+        hipMemcpy(*dst, src.data(), src.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)
+      rc = pw_fail(PW_ERR_HIP, "pw_waves_create: upload failed");
+  };
+  up(&ws->d_atom_isp, ws->isp);
+  up(&ws->d_ipro0, ws->ipro0);
+  up(&ws->d_ibpro, ws->ibpro);
+  up(&ws->d_lmnx, ws->lmnx);
+  up(&ws->d_lmn_ln, ws->lmn_ln);
+  up(&ws->d_lmn_lm, ws->lmn_lm);
+  up(&ws->d_lmn_l, ws->lmn_l);
+  if (rc == PW_OK) rc = ylm_table(ws);
+  if (rc != PW_OK) return fail(rc);
+  if (hipStreamCreate(&ws->stream) != hipSuccess) return fail(pw_fail(PW_ERR_HIP, "hipStreamCreate failed"));
+  ws->own_stream = true;
+  *out = ws;
+  return PW_OK;
+}
+
+extern "C" int pw_waves_destroy(pw_waveset *ws) {
+  if (!ws) return PW_OK;
+  ws_flush_timers(ws);
+  void *ptrs[] = {ws->d_psi0, ws->d_psim, ws->d_hpsi, ws->d_opsi, ws->d_tmp, ws->d_proj, ws->d_oproj,
+                  ws->d_xproj, ws->d_pro, ws->d_ylm, ws->d_mpsi, ws->d_v, ws->d_rho, ws->d_occ2,
+                  ws->d_fpm, ws->d_fab, ws->d_domat, ws->d_dhmat, ws->d_atom_isp, ws->d_ipro0,
+                  ws->d_ibpro, ws->d_lmnx, ws->d_lmn_ln, ws->d_lmn_lm, ws->d_lmn_l, ws->d_phase,
+                  ws->d_part, ws->d_small, ws->d_ux, ws->d_ud, ws->d_small_mats};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  if (ws->own_stream && ws->stream) (void)hipStreamDestroy(ws->stream);
+  delete ws;
+  return PW_OK;
+}
+
+extern "C" int pw_waves_set_stream(pw_waveset *ws, void *stream) {
+  if (!ws) return pw_fail(PW_ERR_ARG, "pw_waves_set_stream: null waveset");
+  if (ws->own_stream && ws->stream) (void)hipStreamDestroy(ws->stream);
+  ws->stream = (hipStream_t)stream;
+  ws->own_stream = false;
+  return PW_OK;
+}
+
+extern "C" int pw_waves_setr8(pw_waveset *ws, const char *id, double val) {
+  if (!ws || !id) return pw_fail(PW_ERR_ARG, "pw_waves_setr8: null argument");
+  std::string s(id);
+  if (s == "TIMESTEP") ws->delt = val;
+  else if (s == "EMASS") ws->emass = val;
+  else if (s == "EMASSCG2") ws->emasscg2 = val;
+  else if (s == "FRICTION") ws->annee = val;
+  else return pw_fail(PW_ERR_ARG, "WAVES$SETR8: ID not recognised: " + s);
+  return upload_mass(ws);
+}
+
+extern "C" int pw_waves_setr8a(pw_waveset *ws, const char *id, int len, const double *val) {
+  if (!ws || !id || !val) return pw_fail(PW_ERR_ARG, "pw_waves_setr8a: null argument");
+  std::string s(id);
+  pw_plan *p = ws->plan;
+  auto chk = [&](size_t n) {
+    return (size_t)len == n ? PW_OK : pw_fail(PW_ERR_ARG, "WAVES$SETR8A: size inconsistent for " + s);
+  };
+  if (s == "PRO") {
+    PW_TRY(chk((size_t)p->ngl * ws->nbarepro));
+    // caller layout PRO(NGL,NBAREPRO) column-major == [ibare][ngl]
+    PW_HIP(hipMemcpy(ws->d_pro, val, (size_t)len * sizeof(double), hipMemcpyHostToDevice));
+  } else if (s == "OCC") {
+    PW_TRY(chk(ws->nb));
+    ws->occ.assign(val, val + ws->nb);
+    PW_TRY(upload_occ(ws));
+  } else if (s == "DO") {
+    PW_TRY(chk((size_t)ws->lnxx * ws->lnxx * ws->nsp));
+    ws->do_ln.assign(val, val + len);
+    PW_TRY(upload_do(ws));
+  } else if (s == "VOFR") {
+    PW_TRY(chk(p->nrl));
+    PW_HIP(hipMemcpy(ws->d_v, val, (size_t)len * sizeof(double), hipMemcpyHostToDevice));
+    ws->have_v = true;
+  } else if (s == "R0" || s == "RP" || s == "RM") {
+    PW_TRY(chk((size_t)3 * ws->nat));
+    std::vector<double> r(val, val + len);
+    if (s == "R0") ws->r0 = r; else if (s == "RP") ws->rp = r; else ws->rm = r;
+  } else {
+    return pw_fail(PW_ERR_ARG, "WAVES$SETR8A: ID not recognised: " + s);
+  }
+  return PW_OK;
+}
+
+extern "C" int pw_waves_setc8a(pw_waveset *ws, const char *id, int len, const double *val) {
+  if (!ws || !id || !val) return pw_fail(PW_ERR_ARG, "pw_waves_setc8a: null argument");
+  std::string s(id);
+  pw_plan *p = ws->plan;
+  const size_t npsi = (size_t)ws->nbh * p->ngl;
+  if (s == "PSI0" || s == "PSIM") {
+    if ((size_t)len != npsi) return pw_fail(PW_ERR_ARG, "WAVES$SETC8A: size inconsistent for " + s);
+    PW_HIP(hipMemcpy(s == "PSI0" ? ws->d_psi0 : ws->d_psim, val, npsi * sizeof(double2), hipMemcpyHostToDevice));
+    if (s == "PSI0") ws->have_psi0 = true; else ws->have_psim = true;
+  } else if (s == "DH") {
+    const size_t n = (size_t)ws->lmnxx * ws->lmnxx * ws->nat;
+    if ((size_t)len != n) return pw_fail(PW_ERR_ARG, "WAVES$SETC8A: size inconsistent for DH");
+    // DH(LMNXX,LMNXX,NAT) column-major -> per atom row-major [i][j]
+    std::vector<double2> m(n);
+    const int L = ws->lmnxx;
+    for (int a = 0; a < ws->nat; ++a)
+      for (int i = 0; i < L; ++i)
+        for (int j = 0; j < L; ++j) {
+          const size_t src = ((size_t)a * L + j) * L + i;   // DH(i,j,a)
+          m[((size_t)a * L + i) * L + j] = make_double2(val[2 * src], val[2 * src + 1]);
+        }
+    PW_HIP(hipMemcpy(ws->d_dhmat, m.data(), n * sizeof(double2), hipMemcpyHostToDevice));
+    ws->dh.assign(val, val + 2 * n);
+    ws->have_dh = true;
+  } else if (s == "RLAM0" || s == "RLAMM") {
+    const size_t n = (size_t)ws->nb * ws->nb;
+    if ((size_t)len != n) return pw_fail(PW_ERR_ARG, "WAVES$SETC8A: size inconsistent for " + s);
+    std::vector<double> &dst = s == "RLAM0" ? ws->rlam0 : ws->rlamm;
+    dst.assign(n, 0.0);
+    for (int i = 0; i < ws->nb; ++i)
+      for (int j = 0; j < ws->nb; ++j) dst[(size_t)i * ws->nb + j] = val[2 * ((size_t)j * ws->nb + i)];
+    if (s == "RLAMM") ws->have_rlamm = true;
+  } else {
+    return pw_fail(PW_ERR_ARG, "WAVES$SETC8A: ID not recognised: " + s);
+  }
+  return PW_OK;
+}
+
+static int get_dev_c(pw_waveset *ws, const double2 *src, size_t n, int len, double *val) {
+  if ((size_t)len != n) return pw_fail(PW_ERR_ARG, "WAVES$GETC8A: size inconsistent");
+  PW_HIP(hipStreamSynchronize(ws->stream));
+  PW_HIP(hipMemcpy(val, src, n * sizeof(double2), hipMemcpyDeviceToHost));
+  return PW_OK;
+}
+
+extern "C" int pw_waves_getc8a(pw_waveset *ws, const char *id, int len, double *val) {
+  if (!ws || !id || !val) return pw_fail(PW_ERR_ARG, "pw_waves_getc8a: null argument");
+  std::string s(id);
+  const size_t npsi = (size_t)ws->nbh * ws->plan->ngl;
+  const size_t nproj = (size_t)ws->nbh * ws->npro;
+  if (s == "PSI0") return get_dev_c(ws, ws->d_psi0, npsi, len, val);
+  if (s == "PSIM") return get_dev_c(ws, ws->d_psim, npsi, len, val);
+  if (s == "HPSI") return get_dev_c(ws, ws->d_hpsi, npsi, len, val);
+  if (s == "OPSI") return get_dev_c(ws, ws->d_opsi, npsi, len, val);
+  if (s == "PROJ") return get_dev_c(ws, ws->d_proj, nproj, len, val);
+  if (s == "OPROJ") return get_dev_c(ws, ws->d_oproj, nproj, len, val);
+  if (s == "RLAM0") {
+    const size_t n = (size_t)ws->nb * ws->nb;
+    if ((size_t)len != n) return pw_fail(PW_ERR_ARG, "WAVES$GETC8A: size inconsistent");
+    for (int i = 0; i < ws->nb; ++i)
+      for (int j = 0; j < ws->nb; ++j) {
+        val[2 * ((size_t)j * ws->nb + i)] = ws->rlam0[(size_t)i * ws->nb + j];
+        val[2 * ((size_t)j * ws->nb + i) + 1] = 0.0;
+      }
+    return PW_OK;
+  }
+  return pw_fail(PW_ERR_ARG, "WAVES$GETC8A: ID not recognised: " + s);
+}
+
+static void to_colmajor(const std::vector<double> &m, int n, double *out) {
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) out[(size_t)j * n + i] = m[(size_t)i * n + j];
+}
+
+extern "C" int pw_waves_getr8a(pw_waveset *ws, const char *id, int len, double *val) {
+  if (!ws || !id || !val) return pw_fail(PW_ERR_ARG, "pw_waves_getr8a: null argument");
+  std::string s(id);
+  auto chk = [&](size_t n) {
+    return (size_t)len == n ? PW_OK : pw_fail(PW_ERR_ARG, "WAVES$GETR8A: size inconsistent for " + s);
+  };
+  if (s == "RHO") {
+    PW_TRY(chk(ws->plan->nrl));
+    PW_HIP(hipStreamSynchronize(ws->stream));
+    PW_HIP(hipMemcpy(val, ws->d_rho, (size_t)len * sizeof(double), hipMemcpyDeviceToHost));
+  } else if (s == "FORCE") {
+    PW_TRY(chk((size_t)3 * ws->nat));
+    std::memcpy(val, ws->force.data(), (size_t)len * sizeof(double));
+  } else if (s == "EIG") {
+    PW_TRY(chk(ws->nb));
+    std::memcpy(val, ws->eig.data(), (size_t)len * sizeof(double));
+  } else if (s == "MAT" || s == "OMAT" || s == "OOMAT") {
+    PW_TRY(chk((size_t)ws->nb * ws->nb));
+    const std::vector<double> &m = s == "MAT" ? ws->mat : (s == "OMAT" ? ws->omat : ws->oomat);
+    if (m.size() != (size_t)len) return pw_fail(PW_ERR_STATE, "WAVES: no orthogonalisation yet");
+    to_colmajor(m, ws->nb, val);
+  } else if (s == "DENMAT") {
+    PW_TRY(chk(ws->denmat.size()));
+    // row-major per atom -> DENMAT(LMNXX,LMNXX,NAT) column-major
+    const int L = ws->lmnxx;
+    for (int a = 0; a < ws->nat; ++a)
+      for (int i = 0; i < L; ++i)
+        for (int j = 0; j < L; ++j) val[((size_t)a * L + j) * L + i] = ws->denmat[((size_t)a * L + i) * L + j];
+  } else {
+    return pw_fail(PW_ERR_ARG, "WAVES$GETR8A: ID not recognised: " + s);
+  }
+  return PW_OK;
+}
+
+extern "C" int pw_waves_getr8(pw_waveset *ws, const char *id, double *val) {
+  if (!ws || !id || !val) return pw_fail(PW_ERR_ARG, "pw_waves_getr8: null argument");
+  std::string s(id);
+  if (s == "EKIN") *val = ws->ekin;
+  else if (s == "EPOT") *val = ws->epot;
+  else if (s == "E1C") *val = ws->e1c;
+  else if (s == "ETOT") *val = ws->etot;
+  else if (s == "WAVEEKIN1") *val = ws->wavekin1;
+  else if (s == "WAVEEKIN2") *val = ws->wavekin2;
+  else if (s == "EFICT") *val = ws->efict;
+  else return pw_fail(PW_ERR_ARG, "WAVES$GETR8: ID not recognised: " + s);
+  return PW_OK;
+}
+
+extern "C" int pw_waves_geti4(pw_waveset *ws, const char *id, int *val) {
+  if (!ws || !id || !val) return pw_fail(PW_ERR_ARG, "pw_waves_geti4: null argument");
+  std::string s(id);
+  if (s == "ORTHOITER") *val = ws->ortho_iter;
+  else if (s == "NPRO") *val = ws->npro;
+  else if (s == "NBH") *val = ws->nbh;
+  else if (s == "LMNXX") *val = ws->lmnxx;
+  else if (s == "EIGREFRESH") *val = ws->eig_refresh;
+  else return pw_fail(PW_ERR_ARG, "WAVES$GETI4: ID not recognised: " + s);
+  return PW_OK;
+}
+
+extern "C" int pw_waves_devptr(pw_waveset *ws, const char *id, void **ptr) {
+  if (!ws || !id || !ptr) return pw_fail(PW_ERR_ARG, "pw_waves_devptr: null argument");
+  std::string s(id);
+  if (s == "PSI0") *ptr = ws->d_psi0;
+  else if (s == "PSIM") *ptr = ws->d_psim;
+  else if (s == "HPSI") *ptr = ws->d_hpsi;
+  else if (s == "PROJ") *ptr = ws->d_proj;
+  else if (s == "VOFR") *ptr = ws->d_v;
+  else if (s == "RHO") *ptr = ws->d_rho;
+  else return pw_fail(PW_ERR_ARG, "pw_waves_devptr: ID not recognised: " + s);
+  return PW_OK;
+}
+
+extern "C" int pw_waves_profile(pw_waveset *ws, int enable) {
+  if (!ws) return pw_fail(PW_ERR_ARG, "pw_waves_profile: null waveset");
+  ws_flush_timers(ws);
+  ws->profile = enable != 0;
+  ws->ktime.clear();
+  return PW_OK;
+}
+
+extern "C" int pw_waves_kernel_time(pw_waveset *ws, const char *name, double *ms, int *count) {
+  if (!ws || !name || !ms || !count) return pw_fail(PW_ERR_ARG, "pw_waves_kernel_time: null argument");
+  ws_flush_timers(ws);
+  auto it = ws->ktime.find(name);
+  *ms = it == ws->ktime.end() ? 0.0 : it->second.first;
+  *count = it == ws->ktime.end() ? 0 : it->second.second;
+  return PW_OK;
+}
+
+static int ready(pw_waveset *ws) {
+  PW_TRY(need(ws->have_psi0, "PSI0"));
+  PW_TRY(need(ws->have_psim, "PSIM"));
+  PW_TRY(need(!ws->occ.empty(), "OCC"));
+  PW_TRY(need(!ws->do_ln.empty(), "DO"));
+  PW_TRY(need(ws->have_dh, "DH"));
+  PW_TRY(need(ws->have_v, "VOFR"));
+  PW_TRY(need(!ws->r0.empty(), "R0"));
+  PW_TRY(need(ws->have_mass, "TIMESTEP/EMASS"));
+  return PW_OK;
+}
+
+extern "C" int pw_waves_gramschmidt(pw_waveset *ws) {
+  if (!ws) return pw_fail(PW_ERR_ARG, "pw_waves_gramschmidt: null waveset");
+  PW_TRY(ready(ws));
+  PW_TRY(gramschmidt_one(ws, ws->d_psi0, ws->r0));
+  PW_TRY(gramschmidt_one(ws, ws->d_psim, ws->rm.empty() ? ws->r0 : ws->rm));
+  return PW_OK;
+}
+
+extern "C" int pw_waves_projections(pw_waveset *ws) {
+  if (!ws) return pw_fail(PW_ERR_ARG, "pw_waves_projections: null waveset");
+  PW_TRY(need(ws->have_psi0, "PSI0"));
+  PW_TRY(need(!ws->r0.empty(), "R0"));
+  PW_TRY(phase_at(ws, ws->r0));
+  PW_TRY(k_projections(ws, ws->d_psi0, ws->d_phase, ws->d_proj));
+  PW_HIP(hipStreamSynchronize(ws->stream));
+  return PW_OK;
+}
+
+extern "C" int pw_waves_etot(pw_waveset *ws) {
+  if (!ws) return pw_fail(PW_ERR_ARG, "pw_waves_etot: null waveset");
+  PW_TRY(ready(ws));
+  pw_plan *p = ws->plan;
+  if (ws->first) {
+    PW_TRY(pw_waves_gramschmidt(ws));
+    PW_TRY(pw_waves_projections(ws));
+    ws->first = false;
+  }
+  // WAVES_EKIN
+  PW_TRY(k_ekin(ws, ws->d_psi0, ws->d_fpm, &ws->ekin));
+  // WAVES$DENMAT (hermitised, real)
+  std::vector<double> dm(ws->denmat.size());
+  PW_TRY(k_denmat(ws, ws->d_proj, dm.data()));
+  const int L = ws->lmnxx;
+  for (int a = 0; a < ws->nat; ++a)
+    for (int i = 0; i < L; ++i)
+      for (int j = 0; j < L; ++j) {
+        const size_t ij = ((size_t)a * L + i) * L + j, ji = ((size_t)a * L + j) * L + i;
+        ws->denmat[ij] = 0.5 * (dm[ij] + dm[ji]);
+      }
+  // WAVES_DENSITY
+  {
+    ScopeTimer tm(ws, "density");
+    PW_TRY(fft_density(p, ws->nbh, ws->d_psi0, ws->d_occ2, ws->d_rho, ws->prec, ws->stream));
+  }
+  // energy of the fixed local potential: RWEIGHT sum_r V rho
+  PW_TRY(k_dot_vrho(ws, &ws->epot));
+  // one-centre energy sum_ij DH_ij D_ji
+  double e1 = 0.0;
+  for (int a = 0; a < ws->nat; ++a)
+    for (int i = 0; i < L; ++i)
+      for (int j = 0; j < L; ++j) {
+        const size_t src = ((size_t)a * L + j) * L + i;   // DH(i,j,a) column-major
+        e1 += ws->dh[2 * src] * ws->denmat[((size_t)a * L + j) * L + i];
+      }
+  ws->e1c = e1;
+  ws->etot = ws->ekin + ws->epot + ws->e1c;
+  // WAVES$FORCE
+  PW_TRY(phase_at(ws, ws->r0));
+  PW_TRY(dedproj(ws));
+  PW_TRY(k_force(ws, ws->d_psi0, ws->d_phase, ws->d_xproj, ws->force.data()));
+  // WAVES$HPSI: VPSI + HPROJ + ADDPRO
+  {
+    ScopeTimer tm(ws, "vpsi");
+    PW_TRY(fft_vpsi(p, ws->nbh, ws->d_psi0, ws->d_v, ws->d_hpsi, 1, ws->prec, ws->stream));
+  }
+  PW_TRY(k_atom_matrix(ws, ws->d_proj, ws->d_xproj, ws->d_dhmat));
+  PW_TRY(k_addpro(ws, ws->d_hpsi, ws->d_phase, ws->d_xproj));
+  // <psi|H|psi>
+  PW_TRY(k_expect(ws, ws->d_psi0, ws->d_hpsi, ws->eig.data()));
+  // WAVES_WAVEKINETIC (first half)
+  PW_TRY(k_wavekinetic(ws, ws->d_psi0, ws->d_psim, ws->d_fab, &ws->wavekin1));
+  ws->have_hpsi = true;
+  return PW_OK;
+}
+
+extern "C" int pw_waves_propagate(pw_waveset *ws) {
+  if (!ws) return pw_fail(PW_ERR_ARG, "pw_waves_propagate: null waveset");
+  if (!ws->have_hpsi)
+    return pw_fail(PW_ERR_STATE, "WAVES$PROPAGATE: HPSI not available (call WAVES$ETOT first)");
+  PW_TRY(k_propagate(ws));
+  ws->have_hpsi = false;
+  return PW_OK;
+}
+
+extern "C" int pw_waves_orthogonalize(pw_waveset *ws) {
+  if (!ws) return pw_fail(PW_ERR_ARG, "pw_waves_orthogonalize: null waveset");
+  PW_TRY(ready(ws));
+  pw_plan *p = ws->plan;
+  const int nb = ws->nb;
+  const std::vector<double> &rp = ws->rp.empty() ? ws->r0 : ws->rp;
+  // OPSI = dt^2/((1+a) m(G)) (psi0 + |p>DO<p|psi0>) at R(0)
+  PW_TRY(k_copy(ws, ws->d_opsi, ws->d_psi0, (size_t)ws->nbh * p->ngl));
+  PW_TRY(phase_at(ws, ws->r0));
+  PW_TRY(k_atom_matrix(ws, ws->d_proj, ws->d_xproj, ws->d_domat));
+  PW_TRY(k_addpro(ws, ws->d_opsi, ws->d_phase, ws->d_xproj));
+  PW_TRY(k_opsi_scale(ws, ws->d_opsi));
+  // projections at R(+)
+  PW_TRY(phase_at(ws, rp));
+  PW_TRY(k_projections(ws, ws->d_psim, ws->d_phase, ws->d_proj));
+  PW_TRY(k_projections(ws, ws->d_opsi, ws->d_phase, ws->d_oproj));
+  // MAT, OMAT, OOMAT
+  PW_TRY(k_overlap(ws, ws->d_psim, ws->d_psim, d_mat(ws, 0), false, true));
+  PW_TRY(k_overlap_1c(ws, ws->d_proj, ws->d_proj, d_mat(ws, 0)));
+  PW_TRY(k_overlap(ws, ws->d_opsi, ws->d_psim, d_mat(ws, 1), false, false));
+  PW_TRY(k_overlap_1c(ws, ws->d_oproj, ws->d_proj, d_mat(ws, 1)));
+  PW_TRY(k_overlap(ws, ws->d_opsi, ws->d_opsi, d_mat(ws, 2), false, true));
+  PW_TRY(k_overlap_1c(ws, ws->d_oproj, ws->d_oproj, d_mat(ws, 2)));
+  // initial LAMBDA: hermitised RLAM0 times 0.5 f_i/(f_i+f_j) (src/paw_waves2.f90:295-359)
+  std::vector<double> lam((size_t)nb * nb);
+  for (int i = 0; i < nb; ++i)
+    for (int j = 0; j < nb; ++j) {
+      double v = ws->rlam0[(size_t)i * nb + j];
+      if (i != j) v = 0.5 * (ws->rlam0[(size_t)i * nb + j] + ws->rlam0[(size_t)j * nb + i]);
+      const double oi = ws->occ[i];
+      double oj = ws->occ[j];
+      if (oi + oj < 1.0e-12) oj = 1.0e-12;
+      lam[(size_t)i * nb + j] = v * 0.5 * oi / (oi + oj);
+    }
+  PW_HIP(hipMemcpyAsync(d_mat(ws, 3), lam.data(), lam.size() * sizeof(double), hipMemcpyHostToDevice,
+                        ws->stream));
+  PW_TRY(ortho_x_device(ws, d_mat(ws, 3), d_mat(ws, 2), d_mat(ws, 0), d_mat(ws, 1), &ws->ortho_iter));
+  // psi(+) = psibar + chi Lambda ; proj(+) = proj + oproj Lambda
+  PW_TRY(k_addopsi(ws, ws->d_psim, ws->d_opsi, d_mat(ws, 3)));
+  PW_TRY(k_addoproj(ws, ws->d_proj, ws->d_oproj, d_mat(ws, 3)));
+  PW_HIP(hipMemcpyAsync(ws->rlam0.data(), d_mat(ws, 3), (size_t)nb * nb * sizeof(double),
+                        hipMemcpyDeviceToHost, ws->stream));
+  ws->mat.resize((size_t)nb * nb);
+  ws->omat.resize((size_t)nb * nb);
+  ws->oomat.resize((size_t)nb * nb);
+  PW_HIP(hipMemcpyAsync(ws->mat.data(), d_mat(ws, 0), ws->mat.size() * sizeof(double), hipMemcpyDeviceToHost,
+                        ws->stream));
+  PW_HIP(hipMemcpyAsync(ws->omat.data(), d_mat(ws, 1), ws->omat.size() * sizeof(double),
+                        hipMemcpyDeviceToHost, ws->stream));
+  PW_HIP(hipMemcpyAsync(ws->oomat.data(), d_mat(ws, 2), ws->oomat.size() * sizeof(double),
+                        hipMemcpyDeviceToHost, ws->stream));
+  PW_HIP(hipStreamSynchronize(ws->stream));
+  PW_TRY(k_wavekinetic(ws, ws->d_psi0, ws->d_psim, ws->d_fab, &ws->wavekin2));
+  ws->efict = 0.5 * (ws->wavekin1 + ws->wavekin2);
+  return PW_OK;
+}
+
+extern "C" int pw_waves_switch(pw_waveset *ws) {
+  if (!ws) return pw_fail(PW_ERR_ARG, "pw_waves_switch: null waveset");
+  std::swap(ws->d_psi0, ws->d_psim);
+  // NLAMBDAX=2 extrapolation (src/paw_waves2.f90:2236-2297)
+  if (!ws->have_rlamm) {
+    ws->rlamm = ws->rlam0;
+    ws->have_rlamm = true;
+  } else {
+    std::vector<double> rlamp(ws->rlam0.size());
+    for (size_t k = 0; k < rlamp.size(); ++k) rlamp[k] = 2.0 * ws->rlam0[k] - ws->rlamm[k];
+    ws->rlamm = ws->rlam0;
+    ws->rlam0 = rlamp;
+  }
+  ws->wavekin1 = 0.0;
+  ws->wavekin2 = 0.0;
+  return PW_OK;
+}
+
+extern "C" int pw_waves_step(pw_waveset *ws) {
+  PW_TRY(pw_waves_etot(ws));
+  PW_TRY(pw_waves_propagate(ws));
+  PW_TRY(pw_waves_orthogonalize(ws));
+  PW_TRY(pw_waves_switch(ws));
+  return PW_OK;
+}
+
+extern "C" int pw_waves_vpsi(pw_waveset *ws, const void *psi, void *hpsi) {
+  if (!ws || !psi || !hpsi) return pw_fail(PW_ERR_ARG, "pw_waves_vpsi: null argument");
+  PW_TRY(need(ws->have_v, "VOFR"));
+  ScopeTimer tm(ws, "vpsi");
+  return fft_vpsi(ws->plan, ws->nbh, (const double2 *)psi, ws->d_v, (double2 *)hpsi, 1, ws->prec, ws->stream);
+}
+
+extern "C" int pw_waves_density(pw_waveset *ws, const void *psi, double *rho) {
+  if (!ws || !psi || !rho) return pw_fail(PW_ERR_ARG, "pw_waves_density: null argument");
+  PW_TRY(need(!ws->occ.empty(), "OCC"));
+  ScopeTimer tm(ws, "density");
+  return fft_density(ws->plan, ws->nbh, (const double2 *)psi, ws->d_occ2, rho, ws->prec, ws->stream);
+}

@@ -1,0 +1,269 @@
+"""Seeded model systems for the CP wavefunction step.
+
+The reference obtains the local potential V(r) from POTENTIAL$VOFRHO
+(src/paw_potential.f90:324) and the one-centre matrices DH/DO from
+AUGMENTATION$SPHERE (src/paw_augmentation.f90:223) and SETUP (src/paw_setups.f90).
+Those producers sit outside the wavefunction step.  The systems below feed the
+step with a fixed local potential (a periodic sum of Gaussian wells), analytic
+Gaussian projector functions (2 s + 2 p per site, as the built-in Si setup's
+NPRO 2 2 in tests/fulltests/si2) and fixed DO/DH matrices.  With these inputs the
+total energy is the linear functional E = sum_n f_n <psi_n|H|psi_n>, which is
+what the step's ETOT/FORCE/HPSI compute.
+
+All arrays follow the reference's column-major layouts expressed as numpy C
+order with reversed dimensions (PSI(NGL,NBH) -> psi[nbh, ngl]).
+"""
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+PI = math.pi
+RY = 0.5           # Hartree per Rydberg, src/paw_constants.f90:238
+
+
+@dataclass
+class Species:
+    lox: list                    # angular momentum of each radial projector
+    sigma: list                  # Gaussian width of each radial projector (bohr)
+    do_ln: np.ndarray            # (LNX,LNX) one-centre overlap DO
+    dh_ln: np.ndarray            # (LNX,LNX) one-centre Hamiltonian (spherical part)
+
+    @property
+    def lnx(self):
+        return len(self.lox)
+
+    @property
+    def lmnx(self):
+        return sum(2 * l + 1 for l in self.lox)
+
+
+@dataclass
+class System:
+    name: str
+    rbas: np.ndarray             # (3,3), lattice vectors as columns (RBAS)
+    pos: np.ndarray              # (NAT,3) cartesian positions R(0)
+    isp: np.ndarray              # (NAT,) 0-based species index
+    species: list
+    nb: int
+    occ: np.ndarray              # (NB,)
+    epw_psi: float               # Hartree
+    epw_rho: float               # Hartree
+    wells: list = field(default_factory=list)   # (center(3), depth, width)
+    dt: float = 5.0
+    emass: float = 1000.0
+    emasscg2: float = None
+    annee: float = 0.0
+    seed: int = 0
+    dh_noise: float = 0.02
+
+    def __post_init__(self):
+        if self.emasscg2 is None:
+            # reference default, src/paw_ioroutines.f90:1405-1414
+            self.emasscg2 = 0.5 * (10.0 / (2.0 * PI)) ** 2 * self.dt ** 2 / self.emass
+
+    # -- derived sizes --------------------------------------------------------
+    @property
+    def nat(self):
+        return len(self.pos)
+
+    @property
+    def nbh(self):
+        return (self.nb + 1) // 2
+
+    @property
+    def vol(self):
+        return abs(float(np.linalg.det(self.rbas)))
+
+    @property
+    def lox_list(self):
+        return [s.lox for s in self.species]
+
+    @property
+    def npro(self):
+        return int(sum(self.species[s].lmnx for s in self.isp))
+
+    def do_list(self):
+        return [np.asarray(s.do_ln, dtype=np.float64) for s in self.species]
+
+    # -- tables the wave-function step consumes ------------------------------
+    def pro_table(self, g2):
+        """Bare projector tables PRO(G,LN)/Omega as (NBAREPRO, NG).
+
+        Radial projector p(r)=N r^l exp(-r^2/(2 s^2)), normalised to
+        int p^2 r^2 dr = 1; its Bessel transform gives
+        4 pi N sqrt(pi/2) s^(2l+3) |G|^l exp(-G^2 s^2/2) (table / Omega, as
+        SETUP$GETFOFG, src/paw_setups.f90:1080).
+        """
+        g = np.sqrt(np.asarray(g2, dtype=np.float64))
+        rows = []
+        for sp in self.species:
+            for l, s in zip(sp.lox, sp.sigma):
+                norm = 1.0 / math.sqrt(math.gamma(l + 1.5) * s ** (2 * l + 3) / 2.0)
+                rows.append(4.0 * PI * norm * math.sqrt(PI / 2.0) * s ** (2 * l + 3)
+                            * g ** l * np.exp(-0.5 * g * g * s * s) / self.vol)
+        return np.array(rows)
+
+    def dh_matrices(self):
+        """Per-atom DH (LMNX,LMNX) complex: spherical part + seeded symmetric noise."""
+        rng = np.random.default_rng(self.seed + 7)
+        out = []
+        for iat in range(self.nat):
+            sp = self.species[self.isp[iat]]
+            n = sp.lmnx
+            d = np.zeros((n, n))
+            o1 = 0
+            for ln1, l1 in enumerate(sp.lox):
+                o2 = 0
+                for ln2, l2 in enumerate(sp.lox):
+                    if l1 == l2:
+                        for m in range(2 * l1 + 1):
+                            d[o1 + m, o2 + m] = sp.dh_ln[ln1, ln2]
+                    o2 += 2 * l2 + 1
+                o1 += 2 * l1 + 1
+            noise = rng.normal(0.0, self.dh_noise, (n, n))
+            d = d + 0.5 * (noise + noise.T)
+            out.append(d.astype(np.complex128))
+        return out
+
+    def potential(self, nr):
+        """V(r) on the (NR1,NR2,NR3) grid, returned as (NR3,NR2,NR1) float64.
+
+        Periodic sum of wells depth*exp(-|r-c|^2/w^2), summed in reciprocal
+        space: V(G) = sum_c depth (pi w^2)^1.5/Omega exp(-G^2 w^2/4) exp(-iGc).
+        """
+        nr1, nr2, nr3 = nr
+        gb = 2.0 * PI * np.linalg.inv(self.rbas).T      # columns = b_k
+        f1 = np.fft.fftfreq(nr1, 1.0 / nr1)
+        f2 = np.fft.fftfreq(nr2, 1.0 / nr2)
+        f3 = np.fft.fftfreq(nr3, 1.0 / nr3)
+        g3, g2_, g1 = np.meshgrid(f3, f2, f1, indexing="ij")
+        gx = g1 * gb[0, 0] + g2_ * gb[0, 1] + g3 * gb[0, 2]
+        gy = g1 * gb[1, 0] + g2_ * gb[1, 1] + g3 * gb[1, 2]
+        gz = g1 * gb[2, 0] + g2_ * gb[2, 1] + g3 * gb[2, 2]
+        gsq = gx * gx + gy * gy + gz * gz
+        vg = np.zeros((nr3, nr2, nr1), dtype=np.complex128)
+        for c, depth, w in self.wells:
+            c = np.asarray(c, dtype=np.float64)
+            ph1 = np.exp(-1j * f1 * (gb[:, 0] @ c))
+            ph2 = np.exp(-1j * f2 * (gb[:, 1] @ c))
+            ph3 = np.exp(-1j * f3 * (gb[:, 2] @ c))
+            amp = depth * (PI * w * w) ** 1.5 / self.vol * np.exp(-gsq * w * w / 4.0)
+            vg += amp * ph3[:, None, None] * ph2[None, :, None] * ph1[None, None, :]
+        v = np.fft.ifftn(vg) * float(nr1 * nr2 * nr3)
+        return np.ascontiguousarray(v.real)
+
+    def initial_psi(self, g2):
+        """c(G) = (N(0,1)+iN(0,1))/(1+|G|^2)^2, seeded; (NBH, NG) complex128."""
+        rng = np.random.default_rng(self.seed + 11)
+        ng = len(g2)
+        c = rng.normal(size=(self.nbh, ng)) + 1j * rng.normal(size=(self.nbh, ng))
+        return (c / (1.0 + np.asarray(g2)[None, :]) ** 2).astype(np.complex128)
+
+
+# ----------------------------------------------------------------------------
+# concrete systems
+# ----------------------------------------------------------------------------
+def si_species():
+    """Model Si site: 2 s + 2 p Gaussian projectors (LNX=4, LMNX=8)."""
+    do = np.array([[0.35, 0.08, 0, 0], [0.08, 0.20, 0, 0],
+                   [0, 0, 0.30, 0.05], [0, 0, 0.05, 0.15]], dtype=np.float64)
+    dh = np.array([[-0.45, 0.10, 0, 0], [0.10, 0.25, 0, 0],
+                   [0, 0, -0.25, 0.06], [0, 0, 0.06, 0.20]], dtype=np.float64)
+    return Species(lox=[0, 0, 1, 1], sigma=[0.9, 1.3, 1.0, 1.4], do_ln=do, dh_ln=dh)
+
+
+def diamond_si(ncell=(1, 1, 1), a=10.26, nempty_per_cell=None, seed=1,
+               disp_sigma=0.05, epw_psi_ry=30.0, epw_rho_ry=None, cdual=4.0,
+               dt=5.0, emass=1000.0, nempty=None):
+    """Diamond Si supercell of the 8-atom simple-cubic cell, Gamma point.
+
+    16 doubly occupied bands per 8 atoms; ``nempty`` empty bands.
+    Atoms displaced by seeded N(0, disp_sigma^2).
+    """
+    base = np.array([[0, 0, 0], [0, .5, .5], [.5, 0, .5], [.5, .5, 0]])
+    frac = np.concatenate([base, base + 0.25])
+    n1, n2, n3 = ncell
+    rbas = np.diag([n1 * a, n2 * a, n3 * a]).astype(np.float64)
+    pos = []
+    for i in range(n1):
+        for j in range(n2):
+            for k in range(n3):
+                for f in frac:
+                    pos.append((np.array([i, j, k]) + f) * a)
+    pos = np.array(pos, dtype=np.float64)
+    rng = np.random.default_rng(seed)
+    pos = pos + rng.normal(0.0, disp_sigma, pos.shape)
+    nat = len(pos)
+    nocc = 2 * nat              # 4 valence electrons per Si, doubly occupied
+    if nempty is None:
+        nempty = 4 if nat == 8 else nat // 4
+    nb = nocc + nempty
+    occ = np.array([2.0] * nocc + [0.0] * nempty)
+    epw_psi = epw_psi_ry * RY
+    epw_rho = (epw_rho_ry * RY) if epw_rho_ry is not None else cdual * epw_psi
+    wells = [(p, -1.6, 1.5) for p in pos]
+    return System(name=f"Si{nat}", rbas=rbas, pos=pos, isp=np.zeros(nat, dtype=np.int64),
+                  species=[si_species()], nb=nb, occ=occ, epw_psi=epw_psi,
+                  epw_rho=epw_rho, wells=wells, dt=dt, emass=emass, seed=seed)
+
+
+def si8(**kw):
+    """Si8 simple-cubic cell, a=10.26, 16 occupied + 4 empty, 30/60 Ry."""
+    kw.setdefault("epw_rho_ry", 60.0)
+    return diamond_si((1, 1, 1), nempty=4, **kw)
+
+
+def si64(**kw):
+    """Si64 = 2x2x2 supercell a=20.52, 128 occupied + 16 empty, 30 Ry, CDUAL=4."""
+    return diamond_si((2, 2, 2), nempty=16, **kw)
+
+
+def si_supercell(nat, **kw):
+    """Si8/Si16/Si32/Si64 for the serving mix."""
+    cells = {8: (1, 1, 1), 16: (1, 1, 2), 32: (1, 2, 2), 64: (2, 2, 2)}
+    return diamond_si(cells[nat], nempty=max(4, nat // 4), **kw)
+
+
+def synthetic(seed=3, nb=256, ncentre=64, nwell=32, a=20.0,
+              epw_psi=9.45, epw_rho=45.0, dt=5.0, emass=1000.0):
+    """Kernel-level synthetic step: ~11k plane waves on a 64^3 grid,
+    256 bands, 512 projectors (64 centres x 2 s + 2 p), 32 Gaussian wells.
+    All bands singly weighted f=2 for the lower half, 0 above."""
+    rng = np.random.default_rng(seed)
+    rbas = np.diag([a, a, a]).astype(np.float64)
+    pos = rng.uniform(0.0, a, (ncentre, 3))
+    sig = rng.uniform(0.8, 1.5, (ncentre, 4))
+    # one species per centre would make NSP large; use 4 width classes
+    species = []
+    for k in range(4):
+        sp = si_species()
+        sp.sigma = [0.8 + 0.15 * k, 1.2 + 0.1 * k, 0.9 + 0.15 * k, 1.3 + 0.1 * k]
+        species.append(sp)
+    isp = np.asarray(np.floor(sig[:, 0] * 0 + rng.integers(0, 4, ncentre)), dtype=np.int64)
+    wells = [(rng.uniform(0.0, a, 3), -rng.uniform(1.0, 3.0), rng.uniform(1.0, 2.0))
+             for _ in range(nwell)]
+    occ = np.array([2.0] * (nb // 2) + [0.0] * (nb - nb // 2))
+    return System(name="synthetic", rbas=rbas, pos=pos, isp=isp, species=species,
+                  nb=nb, occ=occ, epw_psi=epw_psi, epw_rho=epw_rho, wells=wells,
+                  dt=dt, emass=emass, seed=seed)
+
+
+def tiny(seed=5, nb=8, a=7.0, epw_psi=3.0, epw_rho=12.0, nat=2):
+    """Tiny cell for fast CPU parity tests (two sites, small grid)."""
+    rng = np.random.default_rng(seed)
+    rbas = np.array([[a, 0.3, 0.0], [0.0, a * 0.95, 0.2], [0.1, 0.0, a * 1.05]]).T
+    pos = rng.uniform(0.0, a, (nat, 3))
+    wells = [(p, -1.2, 1.2) for p in pos]
+    occ = np.array([2.0] * (nb - 2) + [1.0, 0.0])
+    return System(name="tiny", rbas=rbas, pos=pos, isp=np.zeros(nat, dtype=np.int64),
+                  species=[si_species()], nb=nb, occ=occ, epw_psi=epw_psi,
+                  epw_rho=epw_rho, wells=wells, seed=seed)
+
+
+SYSTEMS = {
+    "si8": si8,
+    "si64": si64,
+    "synthetic": synthetic,
+    "tiny": tiny,
+}

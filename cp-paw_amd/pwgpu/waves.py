@@ -1,0 +1,195 @@
+"""Host mirror of the WAVES object (src/paw_waves1.f90, src/paw_waves2.f90).
+
+Method names follow the reference entry points: etot (WAVES$ETOT), propagate
+(WAVES$PROPAGATE), orthogonalize (WAVES$ORTHOGONALIZE), switch (WAVES$SWITCH),
+gramschmidt (WAVES$GRAMMSCHMIDT), projections (WAVES$PROJECTIONS('PSI0')) and
+the keyword setters/getters WAVES$SETR8/GETR8 etc.  All state lives on the
+device inside the library; the arrays below are host copies in the
+reference's layouts (numpy C order with reversed Fortran dimensions).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, dptr, iptr
+
+FP64 = _lib.PW_PREC_FP64
+MIXED = _lib.PW_PREC_MIXED
+
+
+class Waves:
+    def __init__(self, plane, nb, isp, lox_list, prec=FP64):
+        self.lib = _lib.load()
+        self.plane = plane
+        self.nb = int(nb)
+        self.nbh = (self.nb + 1) // 2
+        self.isp = np.asarray(isp, dtype=np.int64)
+        self.nat = len(self.isp)
+        self.lox_list = [list(map(int, l)) for l in lox_list]
+        self.nsp = len(self.lox_list)
+        self.lnx = np.array([len(l) for l in self.lox_list], dtype=np.int32)
+        self.lnxx = int(self.lnx.max())
+        lox = np.zeros((self.nsp, self.lnxx), dtype=np.int32)      # LOX(LNXX,NSP)
+        for s, l in enumerate(self.lox_list):
+            lox[s, :len(l)] = l
+        self.lmnx = [sum(2 * l + 1 for l in lo) for lo in self.lox_list]
+        self.lmnxx = max(self.lmnx)
+        isp1 = (self.isp + 1).astype(np.int32)
+        h = ctypes.c_void_p()
+        check(self.lib.pw_waves_create(plane.handle, self.nb, self.nat, self.nsp, iptr(isp1),
+                                       iptr(self.lnx), self.lnxx, iptr(np.ascontiguousarray(lox)),
+                                       int(prec), ctypes.byref(h)))
+        self.handle = h
+        self.prec = prec
+        self.npro = self.geti4("NPRO")
+
+    def close(self):
+        if self.handle:
+            self.lib.pw_waves_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- keyword interface ----------------------------------------------------
+    def setr8(self, key, val):
+        check(self.lib.pw_waves_setr8(self.handle, key.encode(), float(val)))
+
+    def setr8a(self, key, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64).ravel()
+        check(self.lib.pw_waves_setr8a(self.handle, key.encode(), a.size, dptr(a)))
+
+    def setc8a(self, key, arr):
+        a = np.ascontiguousarray(arr, dtype=np.complex128).ravel()
+        check(self.lib.pw_waves_setc8a(self.handle, key.encode(), a.size, dptr(a)))
+
+    def getc8a(self, key):
+        shapes = {"PSI0": (self.nbh, self.plane.ngl), "PSIM": (self.nbh, self.plane.ngl),
+                  "HPSI": (self.nbh, self.plane.ngl), "OPSI": (self.nbh, self.plane.ngl),
+                  "PROJ": (self.npro, self.nbh), "OPROJ": (self.npro, self.nbh),
+                  "RLAM0": (self.nb, self.nb)}
+        out = np.empty(shapes[key], dtype=np.complex128)
+        check(self.lib.pw_waves_getc8a(self.handle, key.encode(), out.size, dptr(out)))
+        return out.T.copy() if key == "RLAM0" else out
+
+    def getr8a(self, key):
+        L = self.lmnxx
+        shapes = {"RHO": (self.plane.nr[2], self.plane.nr[1], self.plane.nr[0]),
+                  "FORCE": (self.nat, 3), "EIG": (self.nb,), "MAT": (self.nb, self.nb),
+                  "OMAT": (self.nb, self.nb), "OOMAT": (self.nb, self.nb),
+                  "DENMAT": (self.nat, L, L)}
+        out = np.empty(shapes[key], dtype=np.float64)
+        check(self.lib.pw_waves_getr8a(self.handle, key.encode(), out.size, dptr(out)))
+        if key in ("MAT", "OMAT", "OOMAT"):
+            return out.T.copy()
+        if key == "DENMAT":
+            return np.transpose(out, (0, 2, 1)).copy()
+        return out
+
+    def getr8(self, key):
+        v = ctypes.c_double()
+        check(self.lib.pw_waves_getr8(self.handle, key.encode(), ctypes.byref(v)))
+        return v.value
+
+    def geti4(self, key):
+        v = ctypes.c_int()
+        check(self.lib.pw_waves_geti4(self.handle, key.encode(), ctypes.byref(v)))
+        return v.value
+
+    def devptr(self, key):
+        p = ctypes.c_void_p()
+        check(self.lib.pw_waves_devptr(self.handle, key.encode(), ctypes.byref(p)))
+        return p
+
+    # -- inputs in reference layouts ------------------------------------------
+    def set_dh(self, dh_list):
+        """DH(LMNXX,LMNXX,NAT) from a list of per-atom (LMNX,LMNX) matrices."""
+        L = self.lmnxx
+        a = np.zeros((self.nat, L, L), dtype=np.complex128)
+        for i, d in enumerate(dh_list):
+            n = d.shape[0]
+            a[i, :n, :n] = np.asarray(d).T          # [atom, j, i] = DH(i, j, atom)
+        self.setc8a("DH", a)
+
+    def set_do(self, do_list):
+        """DO(LNXX,LNXX,NSP) from a list of per-species (LNX,LNX) matrices."""
+        a = np.zeros((self.nsp, self.lnxx, self.lnxx))
+        for s, d in enumerate(do_list):
+            n = d.shape[0]
+            a[s, :n, :n] = np.asarray(d).T
+        self.setr8a("DO", a)
+
+    def set_matrix_c(self, key, m):
+        self.setc8a(key, np.asarray(m, dtype=np.complex128).T)
+
+    # -- the step ----------------------------------------------------------------
+    def gramschmidt(self):
+        check(self.lib.pw_waves_gramschmidt(self.handle))
+
+    def projections(self):
+        check(self.lib.pw_waves_projections(self.handle))
+
+    def etot(self):
+        check(self.lib.pw_waves_etot(self.handle))
+        return {k.lower(): self.getr8(k) for k in ("EKIN", "EPOT", "E1C", "ETOT")}
+
+    def propagate(self):
+        check(self.lib.pw_waves_propagate(self.handle))
+
+    def orthogonalize(self):
+        check(self.lib.pw_waves_orthogonalize(self.handle))
+
+    def switch(self):
+        check(self.lib.pw_waves_switch(self.handle))
+
+    def step(self):
+        check(self.lib.pw_waves_step(self.handle))
+
+    def profile(self, enable=True):
+        check(self.lib.pw_waves_profile(self.handle, int(bool(enable))))
+
+    def kernel_time(self, name):
+        ms = ctypes.c_double()
+        n = ctypes.c_int()
+        check(self.lib.pw_waves_kernel_time(self.handle, name.encode(), ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
+
+    # -- set-up from a model system ----------------------------------------------
+    @classmethod
+    def from_system(cls, plane, system, prec=FP64, psi=None):
+        """Waves for a pwgpu.model.System: all inputs of the step set, PSIM=PSI0."""
+        w = cls(plane, system.nb, system.isp, system.lox_list, prec=prec)
+        g2 = plane.getr8a("G2")
+        w.setr8a("PRO", system.pro_table(g2))
+        w.set_do(system.do_list())
+        w.set_dh(system.dh_matrices())
+        w.setr8a("OCC", system.occ)
+        w.setr8a("VOFR", system.potential(plane.nr))
+        w.setr8("TIMESTEP", system.dt)
+        w.setr8("EMASS", system.emass)
+        w.setr8("EMASSCG2", system.emasscg2)
+        w.setr8("FRICTION", system.annee)
+        w.setr8a("R0", system.pos)
+        w.setr8a("RM", system.pos)
+        w.setr8a("RP", system.pos)
+        c = system.initial_psi(g2) if psi is None else psi
+        w.setc8a("PSI0", c)
+        w.setc8a("PSIM", c)
+        return w
+
+
+def ortho_x(occ, chichi, psipsi, chipsi, lam):
+    """WAVES_ORTHO_X on the GPU (FP64); numpy (NB,NB) matrices, returns (lambda, iter)."""
+    lib = _lib.load()
+    nb = len(occ)
+    f = lambda m: np.ascontiguousarray(np.asarray(m, dtype=np.float64).T)
+    occ = np.ascontiguousarray(occ, dtype=np.float64)
+    a, b, c = f(chichi), f(psipsi), f(chipsi)
+    l = f(lam)
+    it = ctypes.c_int()
+    check(lib.pw_ortho_x(nb, dptr(occ), dptr(a), dptr(b), dptr(c), dptr(l), ctypes.byref(it)))
+    return l.T.copy(), it.value

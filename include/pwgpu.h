@@ -1,0 +1,170 @@
+/*
+ * pwgpu.h - C ABI of the MI355X-native Car-Parrinello wavefunction step.
+ *
+ * The entry points replace, one for one, the routines of CP-PAW's PLANEWAVE
+ * and WAVES objects that form the CP wavefunction step.  Each declaration
+ * names the reference routine (file:line, relative to the cp-paw checkout) it
+ * stands in for.  A Fortran host binds them through ISO_C_BINDING (see
+ * cp-paw_amd/fortran/pwgpu_mod.f90); a Python host binds them through ctypes
+ * (cp-paw_amd/pwgpu/_lib.py).
+ *
+ * Conventions
+ *  - Every function returns 0 on success and a non-zero PW_ERR_* code on
+ *    failure; pw_last_error() returns the message of the last failure of the
+ *    calling thread.  The Fortran shim turns a non-zero status into
+ *    ERROR$MSG + ERROR$STOP, the reference's error behaviour
+ *    (src/paw_error.f90:221-281).
+ *  - Arrays are Fortran column-major with the reference's dimensions:
+ *    PSI(NGL,NBH) complex, PROJ(NBH,NPRO) complex (NDIM=1 dropped),
+ *    FOFR(NR1*NR2*NR3) with ir1 fastest, square matrices (NB,NB).
+ *    Complex values are interleaved (re, im) pairs of double (complex128)
+ *    or float (complex64).
+ *  - Pointers named *_dev are device pointers (from pw_malloc or any HIP
+ *    allocation on the current device); all other array pointers are host
+ *    pointers that the library copies from/to.
+ *  - A plan or waveset is not thread safe; callers serialise access.  Work is
+ *    queued on the object's stream; functions that return host data
+ *    synchronise that stream.
+ *  - Only the Gamma point with super wave functions (TINV=.TRUE.), NDIM=1 and
+ *    NSPIN=1 are supported, the case of the CP step configurations; other
+ *    requests fail with PW_ERR_UNSUPPORTED.
+ */
+#ifndef PWGPU_H
+#define PWGPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PW_OK 0
+#define PW_ERR_ARG 1          /* inconsistent size or bad argument          */
+#define PW_ERR_UNSUPPORTED 2  /* case outside the supported set             */
+#define PW_ERR_HIP 3          /* HIP runtime error                          */
+#define PW_ERR_STATE 4        /* call order violated (e.g. no HPSI yet)     */
+#define PW_ERR_NOCONV 5       /* orthogonalisation loop did not converge    */
+
+/* Precision forms of a waveset. */
+#define PW_PREC_FP64 0        /* every kernel in FP64                       */
+#define PW_PREC_MIXED 1       /* FP32 FFT/V.psi, split-BF16x3 MFMA products,
+                                 FP64 constraint solve and reductions       */
+
+typedef struct pw_plan pw_plan;       /* PLANEWAVE instance (PWPARALLEL_TYPE) */
+typedef struct pw_waveset pw_waveset; /* WAVES state (MAP/GSET/WVSET_TYPE)    */
+
+/* ---- library / runtime ------------------------------------------------ */
+int pw_version(void);
+/* Copies the message of the last failure into buf (NUL terminated). */
+int pw_last_error(char *buf, int len);
+int pw_malloc(void **ptr_dev, long long nbytes);
+int pw_free(void *ptr_dev);
+int pw_memcpy_htod(void *dst_dev, const void *src, long long nbytes);
+int pw_memcpy_dtoh(void *dst, const void *src_dev, long long nbytes);
+int pw_device_synchronize(void);
+
+/* ---- PLANEWAVE object ----------------------------------------------------
+ * pw_plan_create replaces PLANEWAVE$DIVIDERGRIDONTASKS (src/paw_fft.f90:823)
+ * followed by PLANEWAVE$INITIALIZE (src/paw_fft.f90:872) for one task:
+ * the real-space grid follows BOXSPH+LIB$FFTADJUSTGRD for epw_rho, the
+ * G sphere 0.5|G|^2<=epw_psi follows PLANEWAVE_GETIG's order and MINUSG.
+ * rbas: RBAS(3,3) column-major (lattice vectors as columns), Hartree units. */
+int pw_plan_create(const double rbas[9], const double kvec[3], int tinv,
+                   double epw_psi, double epw_rho, pw_plan **plan);
+int pw_plan_destroy(pw_plan *plan);
+/* PLANEWAVE$GETI4 (src/paw_fft.f90:286): "NGL","NRL","NR1","NR2","NR3",
+ * "NGAMMA" (1-based), "NSTRIPE", "NZPLANE". */
+int pw_plan_geti4(const pw_plan *plan, const char *id, int *val);
+/* PLANEWAVE$GETR8 (src/paw_fft.f90:334): "GWEIGHT","RWEIGHT". */
+int pw_plan_getr8(const pw_plan *plan, const char *id, double *val);
+/* PLANEWAVE$GETR8A (src/paw_fft.f90:377): "GVEC" (3*NGL), "G2" (NGL). */
+int pw_plan_getr8a(const pw_plan *plan, const char *id, int len, double *val);
+/* Integer maps: "IGVEC" (3*NGL), "MINUSG" (NGL, 1-based as the reference). */
+int pw_plan_geti4a(const pw_plan *plan, const char *id, int len, int *val);
+
+/* PLANEWAVE$FFT(ID,NFFT,NGL,FOFG,NRL,FOFR) (src/paw_fft.f90:1948).
+ * id "GTOR": FOFR(r)=sum_G FOFG(G) exp(+iGr); "RTOG": FOFG(G)=1/N sum_r
+ * FOFR(r) exp(-iGr).  fofg_dev: NGL x NFFT complex128 (FP64 storage of the
+ * coefficients in both precisions); fofr_dev: NRL x NFFT complex128 for
+ * prec=PW_PREC_FP64, complex64 for prec=PW_PREC_MIXED.  Asynchronous on the
+ * plan's stream. */
+int pw_fft(pw_plan *plan, const char *id, int nfft, int ngl, void *fofg_dev,
+           int nrl, void *fofr_dev, int prec);
+/* Stream used by the plan (hipStream_t), or 0 for the default stream. */
+int pw_plan_set_stream(pw_plan *plan, void *stream);
+
+/* ---- WAVES object --------------------------------------------------------
+ * pw_waves_create replaces the MAP/GSET/WVSET parts of WAVES$INITIALIZE
+ * (src/paw_waves1.f90:1187-1420) for one k-point and spin.
+ * isp(NAT): 1-based species of each atom; lnx(NSP); lox(LNXX,NSP).       */
+int pw_waves_create(pw_plan *plan, int nb, int nat, int nsp, const int *isp,
+                    const int *lnx, int lnxx, const int *lox, int prec,
+                    pw_waveset **ws);
+int pw_waves_destroy(pw_waveset *ws);
+/* WAVES$SETR8 (src/paw_waves1.f90:322): "TIMESTEP" (DELT), "EMASS",
+ * "EMASSCG2", "FRICTION" (ANNEE). */
+int pw_waves_setr8(pw_waveset *ws, const char *id, double val);
+/* Real arrays: "PRO" (NGL*NBAREPRO, GSET%PRO, already / cell volume),
+ * "OCC" (NB), "DO" (LNXX*LNXX*NSP), "VOFR" (NRL, local potential),
+ * "R0" / "RP" / "RM" (3*NAT atomic positions R(0), R(+), R(-)). */
+int pw_waves_setr8a(pw_waveset *ws, const char *id, int len, const double *val);
+/* Complex arrays (interleaved re,im): "DH" (LMNXX*LMNXX*NAT),
+ * "PSI0" / "PSIM" (NGL*NBH), "RLAM0" / "RLAMM" (NB*NB). */
+int pw_waves_setc8a(pw_waveset *ws, const char *id, int len, const double *val);
+/* Complex results: "PSI0","PSIM","HPSI","OPSI","PROJ","OPROJ" (NBH*NPRO),
+ * "RLAM0" (NB*NB). */
+int pw_waves_getc8a(pw_waveset *ws, const char *id, int len, double *val);
+/* Real results: "RHO" (NRL), "FORCE" (3*NAT), "EIG" (NB, <psi|H|psi>),
+ * "MAT","OMAT","OOMAT" (NB*NB, last orthogonalisation), "DENMAT"
+ * (LMNXX*LMNXX*NAT). */
+int pw_waves_getr8a(pw_waveset *ws, const char *id, int len, double *val);
+/* Scalars: "EKIN","EPOT","E1C","ETOT","WAVEEKIN1","WAVEEKIN2","EFICT". */
+int pw_waves_getr8(pw_waveset *ws, const char *id, double *val);
+/* "ORTHOITER" (iterations of the last WAVES_ORTHO_X), "NPRO", "NBH",
+ * "LMNXX", "EIGREFRESH" (host eigen-decompositions so far). */
+int pw_waves_geti4(pw_waveset *ws, const char *id, int *val);
+/* Device pointer of a resident array ("PSI0","PSIM","HPSI","PROJ","VOFR",
+ * "RHO") for zero-copy access by the caller. */
+int pw_waves_devptr(pw_waveset *ws, const char *id, void **ptr_dev);
+
+/* WAVES$GRAMMSCHMIDT (src/paw_waves1.f90:2638): orthonormalise PSI0 at R(0)
+ * and PSIM at R(-) (WAVES_GRAMSCHMIDT, src/paw_waves2.f90:2025). */
+int pw_waves_gramschmidt(pw_waveset *ws);
+/* WAVES$PROJECTIONS('PSI0') (src/paw_waves1.f90:2681): PROJ=<p(R0)|PSI0>. */
+int pw_waves_projections(pw_waveset *ws);
+/* WAVES$ETOT hot path (src/paw_waves1.f90:1610-2233): WAVES_EKIN, WAVES$DENMAT,
+ * WAVES_DENSITY, the energy of the fixed potential and one-centre terms,
+ * WAVES$FORCE, WAVES$HPSI (VPSI+HPROJ+ADDPRO), <psi|H|psi>, WAVEKINETIC. */
+int pw_waves_etot(pw_waveset *ws);
+/* WAVES$PROPAGATE (src/paw_waves1.f90:6479). */
+int pw_waves_propagate(pw_waveset *ws);
+/* WAVES$ORTHOGONALIZE (src/paw_waves2.f90:11), TSAFEORTHO=.TRUE. */
+int pw_waves_orthogonalize(pw_waveset *ws);
+/* WAVES$SWITCH (src/paw_waves2.f90:2180), NLAMBDAX=2. */
+int pw_waves_switch(pw_waveset *ws);
+/* One full CP step: etot, propagate, orthogonalize, switch (TIMESTEP order,
+ * src/paw_driver.f90:333-561 restricted to the wave functions). */
+int pw_waves_step(pw_waveset *ws);
+/* Set the stream (hipStream_t) all waveset work is queued on. */
+int pw_waves_set_stream(pw_waveset *ws, void *stream);
+/* Per-kernel GPU timing: enable=1 records HIP events around each launch on
+ * the waveset's stream; pw_waves_kernel_time returns the summed duration (ms)
+ * and launch count of the named kernel since the last reset. */
+int pw_waves_profile(pw_waveset *ws, int enable);
+int pw_waves_kernel_time(pw_waveset *ws, const char *name, double *ms, int *count);
+
+/* WAVES_VPSI (src/paw_waves1.f90:5099) on caller arrays:
+ * hpsi = FFT^-1[V FFT psi] + G^2/2 psi for NBH super bands (device). */
+int pw_waves_vpsi(pw_waveset *ws, const void *psi_dev, void *hpsi_dev);
+/* WAVES_DENSITY (src/paw_waves1.f90:5377) of PSI0 into rho_dev (NRL). */
+int pw_waves_density(pw_waveset *ws, const void *psi_dev, double *rho_dev);
+
+/* WAVES_ORTHO_X (src/paw_waves2.f90:1844) on host matrices (NB,NB):
+ * lambda in/out; returns the iteration count in *iter.  Runs on the GPU in
+ * FP64. */
+int pw_ortho_x(int nb, const double *occ, const double *chichi,
+               const double *psipsi, const double *chipsi, double *lambda,
+               int *iter);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PWGPU_H */

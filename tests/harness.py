@@ -1,0 +1,37 @@
+"""Shared set-up for parity tests: the same seeded system for the oracle and
+the HIP library."""
+import numpy as np
+
+from oracle import planewave as opw
+from oracle import waves as owv
+from pwgpu import model
+
+
+def relerr(a, b):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    den = np.max(np.abs(b))
+    return float(np.max(np.abs(a - b)) / (den if den > 0 else 1.0))
+
+
+def oracle_waveset(system):
+    pl = opw.Plane(system.rbas, system.epw_psi, system.epw_rho)
+    st = owv.Setup(system.isp, system.lox_list, system.do_list())
+    pro = system.pro_table(pl.g2)
+    v = system.potential((pl.nr1, pl.nr2, pl.nr3))
+    ws = owv.WaveSet(pl, st, pro, system.nb, system.occ, v, system.dh_matrices(),
+                     system.dt, system.emass, system.emasscg2, system.annee)
+    c = system.initial_psi(pl.g2)
+    ws.psi0 = c.copy()
+    ws.psim = c.copy()
+    return ws
+
+
+def gpu_waveset(system, prec=0):
+    import pwgpu
+    plane = pwgpu.PlaneWave(system.rbas, system.epw_psi, system.epw_rho)
+    w = pwgpu.Waves.from_system(plane, system, prec=prec)
+    return plane, w
+
+
+SYSTEMS = {"tiny": model.tiny, "si8": model.si8}
