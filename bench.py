@@ -1,0 +1,208 @@
+"""Benchmark of the CP wavefunction step on one MI355X.
+
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--system si64]
+                [--prec fp64|mixed|both] [--no-cpu] [--profile]
+
+A step is one Car-Parrinello time step of the wave functions (WAVES$ETOT,
+WAVES$PROPAGATE, WAVES$ORTHOGONALIZE, WAVES$SWITCH) of the Si64 diamond
+supercell (a=20.52 bohr, Gamma point, 128 occupied + 16 empty bands, 30 Ry
+wave cutoff, 72^3 grid, 512 projectors) with all state resident in HBM.
+Inputs are synthetic (seeded displacements, seeded initial coefficients,
+model potential and one-centre matrices).  Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in (ROOT, os.path.join(ROOT, "cp-paw_amd")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+FP32_PEAK_TFLOPS = 157.3
+BF16_PEAK_TFLOPS = 2500.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def kernel_counts(system, plane, nb):
+    """Algorithmic work per launch of the step's kernels."""
+    nbh = (nb + 1) // 2
+    npro = system.npro
+    ngl = plane.ngl
+    nr = plane.nrl
+    return {
+        "projections": ("flop", 8.0 * npro * nbh * ngl),
+        "addpro": ("flop", 8.0 * npro * nbh * ngl),
+        "force": ("flop", 8.0 * npro * nbh * ngl),
+        "overlap": ("flop", 2 * 8.0 * nbh * nbh * ngl),
+        "addopsi": ("flop", 2 * 8.0 * nbh * nbh * ngl),
+        # V.psi for all super bands: N (2 sizeof(c) + sizeof(r)) per band
+        "vpsi": ("byte", nbh * nr * (2 * 16.0 + 8.0)),
+        # density: N (sizeof(c) + 2 sizeof(r)) per band
+        "density": ("byte", nbh * nr * (16.0 + 2 * 8.0)),
+        "propagate": ("byte", 4.0 * nbh * ngl * 16.0),
+    }
+
+
+def run_gpu(system_name, prec, steps, warmup, profile):
+    import pwgpu
+    from pwgpu import model
+    s = model.SYSTEMS[system_name]()
+    plane = pwgpu.PlaneWave(s.rbas, s.epw_psi, s.epw_rho)
+    w = pwgpu.Waves.from_system(plane, s, prec=prec)
+    lib = w.lib
+    t0 = time.time()
+    for i in range(warmup):
+        w.step()
+    lib.pw_device_synchronize()
+    log(f"[{system_name} prec={prec}] warmup {warmup} steps {time.time() - t0:.2f}s")
+    if profile:
+        w.profile(True)
+    lib.pw_device_synchronize()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        w.step()
+    lib.pw_device_synchronize()
+    dt = time.perf_counter() - t0
+    e = {k: w.getr8(k) for k in ("ETOT", "EKIN", "EFICT")}
+    res = {"steps_per_s": steps / dt, "ms_per_step": 1e3 * dt / steps, "energies": e,
+           "ortho_iter": w.geti4("ORTHOITER"), "ortho_levels": w.geti4("ORTHOLEVELS"),
+           "ortho_ns": w.geti4("ORTHONS")}
+    if profile:
+        names = ["projections", "addpro", "force", "overlap", "overlap1c", "addopsi", "addoproj",
+                 "vpsi", "density", "propagate", "opsi_scale", "ortho_x", "ekin", "wavekinetic",
+                 "expect", "denmat", "atom_mat", "phase", "reduce", "vrho"]
+        kt = {}
+        for n in names:
+            ms, cnt = w.kernel_time(n)
+            if cnt:
+                kt[n] = {"ms_total": ms, "count": cnt, "ms_avg": ms / cnt}
+        res["kernels"] = kt
+        res["counts"] = kernel_counts(s, plane, s.nb)
+    w.close()
+    plane.close()
+    return res
+
+
+def cpu_baseline(system_name, nsteps=2):
+    """The FP64 oracle (numpy) on the host: one untimed step, then nsteps timed."""
+    import numpy as np
+    from pwgpu import model
+    from oracle import planewave as opw
+    from oracle import waves as owv
+    try:
+        from threadpoolctl import threadpool_info
+        threads = max([i.get("num_threads", 1) for i in threadpool_info()] + [1])
+    except Exception:
+        threads = os.cpu_count() or 1
+    s = model.SYSTEMS[system_name]()
+    pl = opw.Plane(s.rbas, s.epw_psi, s.epw_rho)
+    st = owv.Setup(s.isp, s.lox_list, s.do_list())
+    ws = owv.WaveSet(pl, st, s.pro_table(pl.g2), s.nb, s.occ, s.potential((pl.nr1, pl.nr2, pl.nr3)),
+                     s.dh_matrices(), s.dt, s.emass, s.emasscg2, s.annee)
+    ws.psi0 = s.initial_psi(pl.g2)
+    ws.psim = ws.psi0.copy()
+
+    def step():
+        ws.etot(s.pos)
+        ws.propagate()
+        ws.orthogonalize(s.pos, s.pos)
+        ws.switch()
+
+    step()
+    t0 = time.perf_counter()
+    for _ in range(nsteps):
+        step()
+    dt = time.perf_counter() - t0
+    del np
+    return {"value": nsteps / dt, "unit": "CP steps/s", "cores": int(threads), "kind": "port",
+            "sample": f"{nsteps} CP steps of {s.name} (FP64 numpy oracle, one band per FFT as the "
+                      f"reference) after one untimed step"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--system", default="si64")
+    ap.add_argument("--prec", default="fp64", choices=["fp64", "mixed", "both"])
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--profile", action="store_true", help="print per-kernel times to stderr")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+                          "(the multi-GPU decomposition is a design in DESIGN.md only)"}))
+        sys.exit(2)
+
+    import pwgpu
+    pwgpu.load()
+    precs = {"fp64": [0], "mixed": [1], "both": [0, 1]}[args.prec]
+    results = {}
+    for pr in precs:
+        results[pr] = run_gpu(args.system, pr, args.steps, args.warmup, False)
+        log(f"prec={pr}: {results[pr]['steps_per_s']:.2f} steps/s "
+            f"({results[pr]['ms_per_step']:.3f} ms/step) E={results[pr]['energies']} "
+            f"ortho iter/levels/ns={results[pr]['ortho_iter']}/{results[pr]['ortho_levels']}/{results[pr]['ortho_ns']}")
+    head = precs[-1]
+    # kernel timing in a separate short run (events add host overhead)
+    prof = run_gpu(args.system, head, max(5, min(20, args.steps)), 2, True)
+    kt = prof["kernels"]
+    counts = prof["counts"]
+    for n, v in sorted(kt.items(), key=lambda kv: -kv[1]["ms_total"]):
+        log(f"  {n:12s} {v['ms_avg']*1e3:9.1f} us x{v['count']:4d}  total {v['ms_total']:.2f} ms")
+    # dominant kernel among those with an algorithmic count
+    cand = [(v["ms_total"], n) for n, v in kt.items() if n in counts]
+    dom = max(cand)[1]
+    kind, work = counts[dom]
+    avg_s = kt[dom]["ms_avg"] * 1e-3
+    if kind == "flop":
+        peak = FP64_PEAK_TFLOPS if head == 0 else BF16_PEAK_TFLOPS
+        achieved = work / avg_s / 1e12
+        roof = {"kernel": dom, "bound": "mfma", "achieved": achieved, "peak": peak,
+                "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None}
+    else:
+        achieved = work / avg_s / 1e9
+        roof = {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None}
+    cpu = None
+    if not args.no_cpu:
+        log("cpu baseline (oracle) ...")
+        cpu = cpu_baseline(args.system, args.cpu_steps)
+        log(f"cpu baseline: {cpu['value']:.4f} steps/s on {cpu['cores']} threads")
+    r = results[head]
+    out = {
+        "metric": "CP steps/s Si64 mixed vs FP64 @|dE|<1e-6 Ha/atom; serving req/s + p50/p99 lat",
+        "value": r["steps_per_s"],
+        "unit": "CP steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if head == 0 else "mixed(f32 fft, f64 constraint)",
+        "data": "synthetic (seeded Si64 displacements and coefficients, model potential)",
+        "config": {"workload": f"{args.system} CP wavefunction step (Gamma, 144 bands, 72^3, 512 projectors)",
+                   "form": "fp64" if head == 0 else "mixed"},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    if 0 in results and 1 in results:
+        out["fp64_steps_per_s"] = results[0]["steps_per_s"]
+        out["mixed_over_fp64"] = results[1]["steps_per_s"] / results[0]["steps_per_s"]
+    out["ortho_iter_last"] = r["ortho_iter"]
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

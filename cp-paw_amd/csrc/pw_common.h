@@ -149,7 +149,7 @@ struct pw_waveset {
   // results
   double ekin = 0, epot = 0, e1c = 0, etot = 0, wavekin1 = 0, wavekin2 = 0;
   int ortho_iter = 0;
-  int eig_refresh = 0;
+  int ortho_levels = 0, ortho_ns = 0;
   std::vector<double> force;   // 3*nat
   std::vector<double> eig;     // nb
   std::vector<double> mat, omat, oomat;
@@ -176,10 +176,9 @@ struct pw_waveset {
   double *d_small = nullptr;   // small matrices / scalars scratch
   double *d_small_mats = nullptr;  // MAT, OMAT, OOMAT, LAMBDA, GS work (NB*NB each)
   size_t small_bytes = 0;
-  // orthogonalisation state (device, FP64)
-  double *d_ux = nullptr;      // NB*NB rotation basis U~
-  double *d_ud = nullptr;      // NB eigenvalues of the basis
-  bool have_basis = false;
+  // orthogonalisation work (device, FP64)
+  double *d_ortho = nullptr;
+  size_t ortho_cap = 0;
   hipStream_t stream = 0;
   bool own_stream = false;
   // timing

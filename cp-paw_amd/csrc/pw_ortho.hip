@@ -3,16 +3,17 @@
 // The reference iterates
 //   GAMN   = sym[LAMBDA^T (CHICHI LAMBDA + 2 CHIPSI)] + PSIPSI - 1
 //   LAMBDA -= 2 f_i/(f_i+f_j) * S^-1(GAMN),   S(X) = B X + X B,  B = sym(CHIPSI)
-// until max|GAMN| < 1e-8, with S^-1 applied through the eigenvectors of B
-// (DSYEV).  The iterates depend on S^-1 only, not on the eigen-basis, so this
+// until max|GAMN| < 1e-8, applying S^-1 through the eigenvectors of B (DSYEV).
+// The iterates depend on S^-1 only, not on an eigen-basis, so this
 // implementation keeps the reference's outer iteration and stopping rule and
-// applies S^-1 exactly (to rounding) in a slowly varying basis U:
-//   U^T B U = D + E (E off-diagonal, small once U is an eigen-basis of a
-//   nearby B), Y solves (D+E) Y + Y (D+E) = U^T GAMN U by the contraction
-//   Y <- (Gt - E Y - (E Y)^T) / (d_i + d_j), and S^-1(GAMN) = U Y U^T.
-// U is refreshed with a host Jacobi eigen-decomposition of B when the
-// contraction bound ||E||_F / min(d) exceeds 0.05 (first call, or when B has
-// drifted).  Every matrix product is an NB^3 GEMM on the device.
+// applies S^-1 exactly (to rounding) with GEMMs only, by the Cayley transform
+// and the squared Smith iteration:
+//   M = (B + qI)^-1 (Newton-Schulz), A = I - 2qM = (B - qI)M,
+//   S^-1(G) = sum_k A^k (2q M G M) A^k, summed by X <- X + A_j X A_j with
+//   A_j = A^(2^j); the error after J levels is rho^(2^J), rho =
+//   (sqrt(k)-1)/(sqrt(k)+1) for the spectral bounds of B (host Lanczos on
+//   sym(B), widened).  B is positive definite in the CP step (B ~ dt^2/m).
+// Every product is an NB^3 GEMM on the device; nothing is diagonalised.
 #include <algorithm>
 #include <cmath>
 
@@ -84,39 +85,6 @@ __global__ void k_gamma(int n, const double *T, const double *A, double *G, doub
   if (threadIdx.x == 0) bmax[blockIdx.x] = red[0];
 }
 
-// d = diag(Bt), E = Bt - diag(d); per-block sum of E^2 and min d
-__global__ void k_split(int n, const double *Bt, double *d, double *E, double *bsum, double *bmin) {
-  __shared__ double rs[256], rm[256];
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  double s = 0.0, mn = 1.0e300;
-  if (idx < n * n) {
-    const int i = idx / n, j = idx % n;
-    const double v = Bt[idx];
-    if (i == j) { d[i] = v; E[idx] = 0.0; mn = v; }
-    else { E[idx] = v; s = v * v; }
-  }
-  rs[threadIdx.x] = s;
-  rm[threadIdx.x] = mn;
-  __syncthreads();
-  for (int k = 128; k > 0; k >>= 1) {
-    if ((int)threadIdx.x < k) {
-      rs[threadIdx.x] += rs[threadIdx.x + k];
-      rm[threadIdx.x] = fmin(rm[threadIdx.x], rm[threadIdx.x + k]);
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) { bsum[blockIdx.x] = rs[0]; bmin[blockIdx.x] = rm[0]; }
-}
-
-__global__ void k_sylv(int n, const double *Gt, const double *EY, const double *d, double *Y) {
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= n * n) return;
-  const int i = idx / n, j = idx % n;
-  double r = Gt[idx];
-  if (EY) r -= EY[idx] + EY[(size_t)j * n + i];
-  Y[idx] = r / (d[i] + d[j]);
-}
-
 // LAMBDA(i,j) -= 2 (f_i+ds)/(f_i+f_j+2ds) X(i,j)   (src/paw_waves2.f90:1979-1987)
 __global__ void k_lam_update(int n, const double *occ, const double *X, double *lam) {
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
@@ -180,26 +148,41 @@ void host_syev(int n, std::vector<double> &a, std::vector<double> &e, std::vecto
 
 namespace {
 
+__global__ void k_scaled_identity(int n, double a, double *X) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n * n) return;
+  X[idx] = (idx / n == idx % n) ? a : 0.0;
+}
+
+// A = I - 2 q M
+__global__ void k_cayley(int n, double q, const double *M, double *A) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n * n) return;
+  A[idx] = ((idx / n == idx % n) ? 1.0 : 0.0) - 2.0 * q * M[idx];
+}
+
+constexpr int kMaxLevels = 12;
+
 struct OrthoWork {
-  double *bs, *bt, *E, *d, *gam, *gt, *y, *ey, *x, *t1, *t2, *lam2, *occ, *red;
+  double *bs, *m0, *m1, *gam, *x0, *x1, *t1, *t2, *lam2, *occ, *red;
+  double *apow[kMaxLevels];
 };
 
 int ortho_work(pw_waveset *ws, OrthoWork &w) {
   const size_t n2 = (size_t)ws->nb * ws->nb;
   const size_t nblk = (n2 + 255) / 256;
-  const size_t need = 12 * n2 + 2 * ws->nb + 2 * nblk + 64;
-  static thread_local double *buf = nullptr;
-  static thread_local size_t cap = 0;
-  if (need > cap) {
-    if (buf) PW_HIP(hipFree(buf));
-    PW_HIP(hipMalloc(&buf, need * sizeof(double)));
-    cap = need;
+  const size_t need = (10 + kMaxLevels) * n2 + ws->nb + 2 * nblk + 64;
+  if (need > ws->ortho_cap) {
+    if (ws->d_ortho) PW_HIP(hipFree(ws->d_ortho));
+    PW_HIP(hipMalloc(&ws->d_ortho, need * sizeof(double)));
+    ws->ortho_cap = need;
   }
-  double *p = buf;
-  w.bs = p; p += n2; w.bt = p; p += n2; w.E = p; p += n2; w.gam = p; p += n2;
-  w.gt = p; p += n2; w.y = p; p += n2; w.ey = p; p += n2; w.x = p; p += n2;
-  w.t1 = p; p += n2; w.t2 = p; p += n2; w.lam2 = p; p += n2; p += n2;
-  w.d = p; p += ws->nb; w.occ = p; p += ws->nb; w.red = p;
+  double *p = ws->d_ortho;
+  w.bs = p; p += n2; w.m0 = p; p += n2; w.m1 = p; p += n2; w.gam = p; p += n2;
+  w.x0 = p; p += n2; w.x1 = p; p += n2; w.t1 = p; p += n2; w.t2 = p; p += n2;
+  w.lam2 = p; p += n2; p += n2;
+  for (int j = 0; j < kMaxLevels; ++j) { w.apow[j] = p; p += n2; }
+  w.occ = p; p += ws->nb; w.red = p;
   return PW_OK;
 }
 
@@ -212,35 +195,51 @@ int gemm(pw_waveset *ws, const double *A, int ta, const double *B, int tb, doubl
   return PW_OK;
 }
 
-int refresh_basis(pw_waveset *ws, const double *d_bs) {
-  const int n = ws->nb;
-  std::vector<double> a((size_t)n * n), e, u;
-  PW_HIP(hipMemcpyAsync(a.data(), d_bs, a.size() * sizeof(double), hipMemcpyDeviceToHost, ws->stream));
-  PW_HIP(hipStreamSynchronize(ws->stream));
-  host_syev(n, a, e, u);
-  if (!ws->d_ux) PW_HIP(hipMalloc(&ws->d_ux, (size_t)n * n * sizeof(double)));
-  PW_HIP(hipMemcpyAsync(ws->d_ux, u.data(), u.size() * sizeof(double), hipMemcpyHostToDevice, ws->stream));
-  PW_HIP(hipStreamSynchronize(ws->stream));
-  ws->have_basis = true;
-  ws->eig_refresh++;
-  return PW_OK;
-}
-
-// rotated split of Bs; returns the contraction bound
-int split_basis(pw_waveset *ws, OrthoWork &w, double *rho) {
-  const int n = ws->nb;
-  const int nblk = (n * n + 255) / 256;
-  PW_TRY(gemm(ws, ws->d_ux, 1, w.bs, 0, w.t1));
-  PW_TRY(gemm(ws, w.t1, 0, ws->d_ux, 0, w.bt));
-  k_split<<<nblk, 256, 0, ws->stream>>>(n, w.bt, w.d, w.E, w.red, w.red + nblk);
-  PW_HIP(hipGetLastError());
-  std::vector<double> h(2 * nblk);
-  PW_HIP(hipMemcpyAsync(h.data(), w.red, h.size() * sizeof(double), hipMemcpyDeviceToHost, ws->stream));
-  PW_HIP(hipStreamSynchronize(ws->stream));
-  double s = 0.0, mn = 1.0e300;
-  for (int k = 0; k < nblk; ++k) { s += h[k]; mn = std::min(mn, h[nblk + k]); }
-  *rho = mn > 0.0 ? std::sqrt(s) / mn : 1.0e300;
-  return PW_OK;
+// Extreme eigenvalues of a symmetric matrix (row-major) by Lanczos with full
+// re-orthogonalisation; returns inner (Ritz) bounds.
+void host_lanczos_bounds(int n, const std::vector<double> &a, double *lo, double *hi) {
+  const int m = std::min(n, 40);
+  std::vector<double> V((size_t)(m + 1) * n, 0.0), alpha(m, 0.0), beta(m + 1, 0.0), w(n);
+  // start vector with all components (deterministic)
+  double nrm = 0.0;
+  for (int i = 0; i < n; ++i) { V[i] = 1.0 + 0.37 * std::sin(1.7 * i + 0.3); nrm += V[i] * V[i]; }
+  nrm = std::sqrt(nrm);
+  for (int i = 0; i < n; ++i) V[i] /= nrm;
+  int k = 0;
+  for (; k < m; ++k) {
+    const double *v = &V[(size_t)k * n];
+    for (int i = 0; i < n; ++i) {
+      double s = 0.0;
+      const double *ar = &a[(size_t)i * n];
+      for (int j = 0; j < n; ++j) s += ar[j] * v[j];
+      w[i] = s;
+    }
+    double al = 0.0;
+    for (int i = 0; i < n; ++i) al += w[i] * v[i];
+    alpha[k] = al;
+    for (int rep = 0; rep < 2; ++rep)
+      for (int j = 0; j <= k; ++j) {
+        const double *vj = &V[(size_t)j * n];
+        double c = 0.0;
+        for (int i = 0; i < n; ++i) c += w[i] * vj[i];
+        for (int i = 0; i < n; ++i) w[i] -= c * vj[i];
+      }
+    double b = 0.0;
+    for (int i = 0; i < n; ++i) b += w[i] * w[i];
+    b = std::sqrt(b);
+    beta[k + 1] = b;
+    if (b < 1.0e-14 * std::fabs(al) || k + 1 == m) { ++k; break; }
+    double *vn = &V[(size_t)(k + 1) * n];
+    for (int i = 0; i < n; ++i) vn[i] = w[i] / b;
+  }
+  std::vector<double> t((size_t)k * k, 0.0), e, u;
+  for (int i = 0; i < k; ++i) {
+    t[(size_t)i * k + i] = alpha[i];
+    if (i + 1 < k) t[(size_t)i * k + i + 1] = t[(size_t)(i + 1) * k + i] = beta[i + 1];
+  }
+  host_syev(k, t, e, u);
+  *lo = *std::min_element(e.begin(), e.end());
+  *hi = *std::max_element(e.begin(), e.end());
 }
 
 }  // namespace
@@ -249,23 +248,50 @@ int ortho_x_device(pw_waveset *ws, double *d_lam, const double *d_chichi, const 
                    const double *d_chipsi, int *iter) {
   ScopeTimer tm(ws, "ortho_x");
   const int n = ws->nb;
-  const int nblk = (n * n + 255) / 256;
+  const size_t n2 = (size_t)n * n;
+  const int nblk = (int)((n2 + 255) / 256);
   OrthoWork w;
   PW_TRY(ortho_work(ws, w));
   PW_HIP(hipMemcpyAsync(w.occ, ws->occ.data(), n * sizeof(double), hipMemcpyHostToDevice, ws->stream));
   k_symm<<<nblk, 256, 0, ws->stream>>>(n, d_chipsi, w.bs);
   PW_HIP(hipGetLastError());
-  double rho = 1.0e300;
-  if (ws->have_basis) PW_TRY(split_basis(ws, w, &rho));
-  if (!(rho <= 0.05)) {
-    PW_TRY(refresh_basis(ws, w.bs));
-    PW_TRY(split_basis(ws, w, &rho));
-    if (!(rho < 0.5))
-      return pw_fail(PW_ERR_NOCONV, "WAVES_ORTHO_X: CHIPSI is not positive definite");
+  // spectral bounds of B on the host (inner Ritz bounds, widened by 3%)
+  std::vector<double> hb(n2);
+  PW_HIP(hipMemcpyAsync(hb.data(), w.bs, n2 * sizeof(double), hipMemcpyDeviceToHost, ws->stream));
+  PW_HIP(hipStreamSynchronize(ws->stream));
+  double lo, hi;
+  host_lanczos_bounds(n, hb, &lo, &hi);
+  if (!(lo > 0.0) || !(hi >= lo))
+    return pw_fail(PW_ERR_NOCONV, "WAVES_ORTHO_X: sym(CHIPSI) is not positive definite");
+  lo *= 0.97;
+  hi *= 1.03;
+  const double q = std::sqrt(lo * hi);
+  const double kap = hi / lo;
+  const double rho = (std::sqrt(kap) - 1.0) / (std::sqrt(kap) + 1.0);
+  int nlev = 1;
+  if (rho > 1.0e-300) {
+    const double need = std::log(1.0e-17) / std::log(rho);
+    nlev = std::max(1, (int)std::ceil(std::log2(std::max(need, 1.0)))) + 1;
   }
-  // inner iterations needed for a 1e-17 relative fixed point
-  int ninner = 1;
-  if (rho > 1.0e-17) ninner = std::min(80, (int)std::ceil(std::log(1.0e-17) / std::log(rho)) + 1);
+  if (nlev > kMaxLevels) return pw_fail(PW_ERR_NOCONV, "WAVES_ORTHO_X: CHIPSI too ill-conditioned");
+  // M = (B + qI)^-1 by Newton-Schulz, X0 = 2/(lo+hi+2q) I
+  const double kp = (hi + q) / (lo + q);
+  const double e0 = (kp - 1.0) / (kp + 1.0);
+  int nns = 1;
+  if (e0 > 1.0e-300) nns = (int)std::ceil(std::log2(std::max(1.0, std::log(1.0e-18) / std::log(e0)))) + 1;
+  ws->ortho_levels = nlev;
+  ws->ortho_ns = nns;
+  k_scaled_identity<<<nblk, 256, 0, ws->stream>>>(n, 2.0 / (lo + hi + 2.0 * q), w.m0);
+  double *mc = w.m0, *mn = w.m1;
+  for (int k = 0; k < nns; ++k) {
+    PW_TRY(gemm(ws, w.bs, 0, mc, 0, w.t1, 1.0, mc, q));       // T = (B+qI) X
+    PW_TRY(gemm(ws, mc, 0, w.t1, 0, mn, -1.0, mc, 2.0));      // X' = 2X - X T
+    std::swap(mc, mn);
+  }
+  // A^(2^j), j = 0..nlev-1
+  k_cayley<<<nblk, 256, 0, ws->stream>>>(n, q, mc, w.apow[0]);
+  for (int j = 1; j < nlev; ++j) PW_TRY(gemm(ws, w.apow[j - 1], 0, w.apow[j - 1], 0, w.apow[j]));
+  PW_HIP(hipGetLastError());
   const double eps = 1.0e-8;
   std::vector<double> hm(nblk);
   for (int it = 1; it <= 200; ++it) {
@@ -285,20 +311,18 @@ int ortho_x_device(pw_waveset *ws, double *d_lam, const double *d_chichi, const 
     }
     if (digam > 1.0e200)
       return pw_fail(PW_ERR_NOCONV, "WAVES_ORTHO_X: iteration loop for orthogonalization is about to diverge");
-    // Gt = U^T GAMN U ; Y = S^-1 in the rotated basis ; X = U Y U^T
-    PW_TRY(gemm(ws, ws->d_ux, 1, w.gam, 0, w.t1));
-    PW_TRY(gemm(ws, w.t1, 0, ws->d_ux, 0, w.gt));
-    k_sylv<<<nblk, 256, 0, ws->stream>>>(n, w.gt, nullptr, w.d, w.y);
-    for (int k = 0; k < ninner; ++k) {
-      PW_TRY(gemm(ws, w.E, 0, w.y, 0, w.ey));
-      k_sylv<<<nblk, 256, 0, ws->stream>>>(n, w.gt, w.ey, w.d, w.y);
+    // X = S^-1(GAMN): X0 = 2q M G M ; X <- X + A_j X A_j
+    PW_TRY(gemm(ws, w.gam, 0, mc, 0, w.t1));
+    PW_TRY(gemm(ws, mc, 0, w.t1, 0, w.x0, 2.0 * q));
+    double *xc = w.x0, *xn = w.x1;
+    for (int j = 0; j < nlev; ++j) {
+      PW_TRY(gemm(ws, xc, 0, w.apow[j], 0, w.t1));
+      PW_TRY(gemm(ws, w.apow[j], 0, w.t1, 0, xn, 1.0, xc, 1.0));
+      std::swap(xc, xn);
     }
-    PW_TRY(gemm(ws, ws->d_ux, 0, w.y, 0, w.t1));
-    PW_TRY(gemm(ws, w.t1, 0, ws->d_ux, 1, w.x));
-    k_lam_update<<<nblk, 256, 0, ws->stream>>>(n, w.occ, w.x, d_lam);
+    k_lam_update<<<nblk, 256, 0, ws->stream>>>(n, w.occ, xc, d_lam);
     k_lam_sym<<<nblk, 256, 0, ws->stream>>>(n, w.occ, d_lam, w.lam2);
-    PW_HIP(hipMemcpyAsync(d_lam, w.lam2, (size_t)n * n * sizeof(double), hipMemcpyDeviceToDevice,
-                          ws->stream));
+    PW_HIP(hipMemcpyAsync(d_lam, w.lam2, n2 * sizeof(double), hipMemcpyDeviceToDevice, ws->stream));
     PW_HIP(hipGetLastError());
   }
   return pw_fail(PW_ERR_NOCONV,
@@ -334,6 +358,6 @@ extern "C" int pw_ortho_x(int nb, const double *occ, const double *chichi, const
     for (int i = 0; i < nb; ++i)
       for (int j = 0; j < nb; ++j) lambda[(size_t)j * nb + i] = h[3 * n2 + (size_t)i * nb + j];
   (void)hipFree(d);
-  if (ws.d_ux) (void)hipFree(ws.d_ux);
+  if (ws.d_ortho) (void)hipFree(ws.d_ortho);
   return rc;
 }

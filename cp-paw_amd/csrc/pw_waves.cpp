@@ -322,7 +322,7 @@ extern "C" int pw_waves_destroy(pw_waveset *ws) {
                   ws->d_xproj, ws->d_pro, ws->d_ylm, ws->d_mpsi, ws->d_v, ws->d_rho, ws->d_occ2,
                   ws->d_fpm, ws->d_fab, ws->d_domat, ws->d_dhmat, ws->d_atom_isp, ws->d_ipro0,
                   ws->d_ibpro, ws->d_lmnx, ws->d_lmn_ln, ws->d_lmn_lm, ws->d_lmn_l, ws->d_phase,
-                  ws->d_part, ws->d_small, ws->d_ux, ws->d_ud, ws->d_small_mats};
+                  ws->d_part, ws->d_small, ws->d_ortho, ws->d_small_mats};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (ws->own_stream && ws->stream) (void)hipStreamDestroy(ws->stream);
@@ -511,7 +511,8 @@ extern "C" int pw_waves_geti4(pw_waveset *ws, const char *id, int *val) {
   else if (s == "NPRO") *val = ws->npro;
   else if (s == "NBH") *val = ws->nbh;
   else if (s == "LMNXX") *val = ws->lmnxx;
-  else if (s == "EIGREFRESH") *val = ws->eig_refresh;
+  else if (s == "ORTHOLEVELS") *val = ws->ortho_levels;
+  else if (s == "ORTHONS") *val = ws->ortho_ns;
   else return pw_fail(PW_ERR_ARG, "WAVES$GETI4: ID not recognised: " + s);
   return PW_OK;
 }

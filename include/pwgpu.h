@@ -119,7 +119,8 @@ int pw_waves_getr8a(pw_waveset *ws, const char *id, int len, double *val);
 /* Scalars: "EKIN","EPOT","E1C","ETOT","WAVEEKIN1","WAVEEKIN2","EFICT". */
 int pw_waves_getr8(pw_waveset *ws, const char *id, double *val);
 /* "ORTHOITER" (iterations of the last WAVES_ORTHO_X), "NPRO", "NBH",
- * "LMNXX", "EIGREFRESH" (host eigen-decompositions so far). */
+ * "LMNXX", "ORTHOLEVELS" (squared-Smith levels of the last solve), "ORTHONS"
+ * (Newton-Schulz steps of the last solve). */
 int pw_waves_geti4(pw_waveset *ws, const char *id, int *val);
 /* Device pointer of a resident array ("PSI0","PSIM","HPSI","PROJ","VOFR",
  * "RHO") for zero-copy access by the caller. */
