@@ -429,9 +429,9 @@ extern "C" int pw_fft(pw_plan *p, const char *id, int nfft, int ngl, void *fofg,
   if (nfft <= 0) return pw_fail(PW_ERR_ARG, "PLANEWAVE$FFT: NFFT must be positive");
   if (prec != PW_PREC_FP64 && prec != PW_PREC_MIXED)
     return pw_fail(PW_ERR_ARG, "pw_fft: unknown precision");
-  PW_TRY(upload_plan(p));
   std::string s(id);
+  if (s != "GTOR" && s != "RTOG") return pw_fail(PW_ERR_ARG, "PLANEWAVE$FFT: ID must be either GTOR or RTOG");
+  PW_TRY(upload_plan(p));
   if (s == "GTOR") return fft_gtor_full(p, nfft, (const double2 *)fofg, fofr, prec, p->stream);
-  if (s == "RTOG") return fft_rtog_full(p, nfft, (double2 *)fofg, fofr, prec, p->stream);
-  return pw_fail(PW_ERR_ARG, "PLANEWAVE$FFT: ID must be either GTOR or RTOG");
+  return fft_rtog_full(p, nfft, (double2 *)fofg, fofr, prec, p->stream);
 }

@@ -285,7 +285,7 @@ def dedpro(plane, psi, dedproj_atom):
     """|dE/dp> = psi conj(dedproj), symmetrised with INVERTG.
 
     src/paw_waves1.f90:5694-5743.  dedproj_atom: (LMNX, NB) -> (LMNX, NGL)."""
-    d = np.conj(dedproj_atom).T.T @ psi if False else np.conj(dedproj_atom) @ psi
+    d = np.conj(dedproj_atom) @ psi
     return 0.5 * (d + invertg(plane.minusg, d))
 
 
