@@ -100,6 +100,7 @@ struct pw_plan {
   // growable work buffers
   void *d_stripe_buf = nullptr; size_t stripe_bytes = 0;
   void *d_plane_buf = nullptr; size_t plane_bytes = 0;
+  void *d_dens_part = nullptr; size_t dens_bytes = 0;
 };
 
 int plan_work(pw_plan *p, int nbatch, int prec);

@@ -353,10 +353,10 @@ __global__ void __launch_bounds__(kBlock) k_zvz(Geo g, Fft1D f, const T2 *tw, T2
   store_zpencils<T2>(g, pb, y, x0, r2);
 }
 
-// ---- z pass: density sum over all bands --------------------------------------------
+// ---- z pass: density sum over a group of bands -> partial rho[group] ---------
 template <typename T2>
 __global__ void __launch_bounds__(kBlock) k_zdens(Geo g, Fft1D f, const T2 *tw, const T2 *planes,
-                                                  int nb, const double *occ2, double *rho) {
+                                                  int nb, int bpg, const double *occ2, double *part) {
   extern __shared__ unsigned char smem_raw[];
   T2 *b0 = reinterpret_cast<T2 *>(smem_raw);
   T2 *b1 = b0 + kTX * g.nr3;
@@ -364,8 +364,10 @@ __global__ void __launch_bounds__(kBlock) k_zdens(Geo g, Fft1D f, const T2 *tw, 
   const int ntx = (g.nr1 + kTX - 1) / kTX;
   const int y = blockIdx.x / ntx;
   const int x0 = (blockIdx.x % ntx) * kTX;
+  const int grp = blockIdx.y;
+  const int bstart = grp * bpg, bend = min(nb, bstart + bpg);
   for (int w = threadIdx.x; w < g.nr3 * kTX; w += blockDim.x) acc[w] = 0.0;
-  for (int band = 0; band < nb; ++band) {
+  for (int band = bstart; band < bend; ++band) {
     const double s1 = occ2[2 * band], s2 = occ2[2 * band + 1];
     if (s1 == 0.0 && s2 == 0.0) continue;   // uniform across the block
     const T2 *pb = planes + (size_t)band * g.nzp * g.nr2 * g.nr1;
@@ -380,10 +382,20 @@ __global__ void __launch_bounds__(kBlock) k_zdens(Geo g, Fft1D f, const T2 *tw, 
     }
   }
   __syncthreads();
+  double *dst = part + (size_t)grp * g.nr1 * g.nr2 * g.nr3;
   for (int w = threadIdx.x; w < g.nr3 * kTX; w += blockDim.x) {
     const int z = w / kTX, x = w % kTX, gx = x0 + x;
-    if (gx < g.nr1) rho[((size_t)z * g.nr2 + y) * g.nr1 + gx] = acc[w];
+    if (gx < g.nr1) dst[((size_t)z * g.nr2 + y) * g.nr1 + gx] = acc[w];
   }
+}
+
+// rho = sum over band groups of the partial densities (fixed order)
+__global__ void k_sum_groups(size_t n, int ngrp, const double *part, double *rho) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s = 0.0;
+  for (int k = 0; k < ngrp; ++k) s += part[(size_t)k * n + i];
+  rho[i] = s;
 }
 
 // ---- z pass with full-grid output (PLANEWAVE$FFT 'GTOR') ----------------------------
@@ -492,9 +504,22 @@ int density_t(pw_plan *p, int nb, const double2 *psi, const double *occ2, double
   Geo g = make_geo(p);
   PW_TRY(run_forward_xy<T2>(p, g, nb, psi, s));
   const int ntx = (p->nr1 + kTX - 1) / kTX;
+  // band groups so that the grid holds ~4 workgroups per CU
+  int ngrp = std::max(1, std::min(nb, (1024 + p->nr2 * ntx - 1) / (p->nr2 * ntx)));
+  const int bpg = (nb + ngrp - 1) / ngrp;
+  ngrp = (nb + bpg - 1) / bpg;
+  const size_t need = (size_t)ngrp * p->nrl * sizeof(double);
+  if (need > p->dens_bytes) {
+    if (p->d_dens_part) PW_HIP(hipFree(p->d_dens_part));
+    PW_HIP(hipMalloc(&p->d_dens_part, need));
+    p->dens_bytes = need;
+  }
   size_t lds = 2 * kTX * p->nr3 * sizeof(T2) + kTX * p->nr3 * sizeof(double);
-  k_zdens<T2><<<p->nr2 * ntx, kBlock, lds, s>>>(g, p->f[2], twid<T2>(p, 2),
-                                               (const T2 *)p->d_plane_buf, nb, occ2, rho);
+  k_zdens<T2><<<dim3(p->nr2 * ntx, ngrp), kBlock, lds, s>>>(g, p->f[2], twid<T2>(p, 2),
+                                                           (const T2 *)p->d_plane_buf, nb, bpg, occ2,
+                                                           (double *)p->d_dens_part);
+  k_sum_groups<<<(p->nrl + 255) / 256, 256, 0, s>>>((size_t)p->nrl, ngrp, (const double *)p->d_dens_part,
+                                                    rho);
   PW_HIP(hipGetLastError());
   return PW_OK;
 }

@@ -380,19 +380,33 @@ __global__ void k_unravel_pw(int nbh, int nb, const double2 *t, const double2 *t
 // one-centre overlap (WAVES_1COVERLAP, src/paw_waves2.f90:856-1022), added to mat
 // c1 = sum_k conj(p1_ki) p2a_kj, c2 = sum_k p1_ki p2a_kj
 // ---------------------------------------------------------------------------
-__global__ void k_overlap1c(int npro, int nbh, int nb, const double2 *p1, const double2 *p2a,
-                            double *mat) {
-  const int i = blockIdx.x * 16 + threadIdx.x / 16;
-  const int j = blockIdx.y * 16 + threadIdx.x % 16;
-  if (i >= nbh || j >= nbh) return;
+__global__ void __launch_bounds__(256) k_overlap1c(int npro, int nbh, int nb, const double2 *p1,
+                                                   const double2 *p2a, double *mat) {
+  __shared__ double2 sa[64][17];
+  __shared__ double2 sb[64][17];
+  const int ti = threadIdx.x / 16, tj = threadIdx.x % 16;
+  const int i0 = blockIdx.x * 16, j0 = blockIdx.y * 16;
+  const int i = i0 + ti, j = j0 + tj;
   double2 c1 = czero(), c2 = czero();
-  for (int k = 0; k < npro; ++k) {
-    const double2 a = p1[(size_t)k * nbh + i], b = p2a[(size_t)k * nbh + j];
-    c1.x += a.x * b.x + a.y * b.y;
-    c1.y += a.x * b.y - a.y * b.x;
-    c2.x += a.x * b.x - a.y * b.y;
-    c2.y += a.x * b.y + a.y * b.x;
+  for (int k0 = 0; k0 < npro; k0 += 64) {
+    for (int w = threadIdx.x; w < 64 * 16; w += 256) {
+      const int k = w / 16, c = w % 16;
+      const bool ok = k0 + k < npro;
+      sa[k][c] = (ok && i0 + c < nbh) ? p1[(size_t)(k0 + k) * nbh + i0 + c] : czero();
+      sb[k][c] = (ok && j0 + c < nbh) ? p2a[(size_t)(k0 + k) * nbh + j0 + c] : czero();
+    }
+    __syncthreads();
+#pragma unroll 8
+    for (int k = 0; k < 64; ++k) {
+      const double2 a = sa[k][ti], b = sb[k][tj];
+      c1.x += a.x * b.x + a.y * b.y;
+      c1.y += a.x * b.y - a.y * b.x;
+      c2.x += a.x * b.x - a.y * b.y;
+      c2.y += a.x * b.y + a.y * b.x;
+    }
+    __syncthreads();
   }
+  if (i >= nbh || j >= nbh) return;
   const double2 s1 = make_double2(0.5 * (c1.x + c2.x), 0.5 * (c1.y + c2.y));
   const double2 s2 = make_double2(0.5 * (c1.x - c2.x), 0.5 * (c1.y - c2.y));
   const int ia = 2 * i, ja = 2 * j;

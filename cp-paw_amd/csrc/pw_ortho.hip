@@ -23,34 +23,44 @@ namespace {
 
 constexpr int T16 = 16;
 
-// C = alpha op(A) op(B) + beta D   (row-major n x n; D may alias nothing)
+// C = alpha op(A) op(B) + beta D   (row-major n x n).  One 16x16 tile of C
+// per workgroup; the A row panel and B column panel (16 x n each) are loaded
+// into LDS in one pass, so the kernel pays one memory latency, not n/16.
 __global__ void __launch_bounds__(256) k_gemm(int n, const double *A, int ta, const double *B, int tb,
                                               double *C, double alpha, const double *D, double beta) {
-  __shared__ double sa[T16][T16 + 1];
-  __shared__ double sb[T16][T16 + 1];
+  extern __shared__ double smem[];
+  double *sa = smem;                 // [16][n+1]
+  double *sb = smem + T16 * (n + 1); // [n][17]
   const int ty = threadIdx.x / T16, tx = threadIdx.x % T16;
-  const int row = blockIdx.y * T16 + ty, col = blockIdx.x * T16 + tx;
-  double acc = 0.0;
-  for (int k0 = 0; k0 < n; k0 += T16) {
-    {
-      const int r = blockIdx.y * T16 + ty, k = k0 + tx;   // A(r,k)
-      double v = 0.0;
-      if (r < n && k < n) v = ta ? A[(size_t)k * n + r] : A[(size_t)r * n + k];
-      sa[ty][tx] = v;
-    }
-    {
-      const int k = k0 + ty, c = blockIdx.x * T16 + tx;   // B(k,c)
-      double v = 0.0;
-      if (k < n && c < n) v = tb ? B[(size_t)c * n + k] : B[(size_t)k * n + c];
-      sb[ty][tx] = v;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < T16; ++k) acc += sa[ty][k] * sb[k][tx];
-    __syncthreads();
+  const int r0 = blockIdx.y * T16, c0 = blockIdx.x * T16;
+  for (int w = threadIdx.x; w < T16 * n; w += 256) {
+    int r, k;
+    if (ta) { r = w % T16; k = w / T16; } else { k = w % n; r = w / n; }
+    double v = 0.0;
+    if (r0 + r < n) v = ta ? A[(size_t)k * n + r0 + r] : A[(size_t)(r0 + r) * n + k];
+    sa[r * (n + 1) + k] = v;
   }
+  for (int w = threadIdx.x; w < T16 * n; w += 256) {
+    int k, c;
+    if (tb) { k = w % n; c = w / n; } else { c = w % T16; k = w / T16; }
+    double v = 0.0;
+    if (c0 + c < n) v = tb ? B[(size_t)(c0 + c) * n + k] : B[(size_t)k * n + c0 + c];
+    sb[k * (T16 + 1) + c] = v;
+  }
+  __syncthreads();
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  const double *ra = sa + ty * (n + 1);
+  int k = 0;
+  for (; k + 3 < n; k += 4) {
+    a0 += ra[k] * sb[k * (T16 + 1) + tx];
+    a1 += ra[k + 1] * sb[(k + 1) * (T16 + 1) + tx];
+    a2 += ra[k + 2] * sb[(k + 2) * (T16 + 1) + tx];
+    a3 += ra[k + 3] * sb[(k + 3) * (T16 + 1) + tx];
+  }
+  for (; k < n; ++k) a0 += ra[k] * sb[k * (T16 + 1) + tx];
+  const int row = r0 + ty, col = c0 + tx;
   if (row < n && col < n) {
-    double v = alpha * acc;
+    double v = alpha * ((a0 + a1) + (a2 + a3));
     if (D) v += beta * D[(size_t)row * n + col];
     C[(size_t)row * n + col] = v;
   }
@@ -190,7 +200,15 @@ int gemm(pw_waveset *ws, const double *A, int ta, const double *B, int tb, doubl
          double alpha = 1.0, const double *D = nullptr, double beta = 0.0) {
   const int n = ws->nb;
   dim3 grid((n + T16 - 1) / T16, (n + T16 - 1) / T16);
-  k_gemm<<<grid, 256, 0, ws->stream>>>(n, A, ta, B, tb, C, alpha, D, beta);
+  const size_t lds = (size_t)(T16 * (n + 1) + n * (T16 + 1)) * sizeof(double);
+  if (lds > 65536) {
+    static bool raised = false;
+    if (!raised) {
+      PW_HIP(hipFuncSetAttribute((const void *)k_gemm, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+      raised = true;
+    }
+  }
+  k_gemm<<<grid, 256, lds, ws->stream>>>(n, A, ta, B, tb, C, alpha, D, beta);
   PW_HIP(hipGetLastError());
   return PW_OK;
 }
