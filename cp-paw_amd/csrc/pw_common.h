@@ -222,5 +222,3 @@ int ws_small(pw_waveset *ws, size_t bytes);
 // orthogonalisation (pw_ortho.hip)
 int ortho_x_device(pw_waveset *ws, double *d_lam, const double *d_chichi,
                    const double *d_psipsi, const double *d_chipsi, int *iter);
-// host symmetric eigen-solver (cyclic Jacobi), a: n*n row-major symmetric
-void host_syev(int n, std::vector<double> &a, std::vector<double> &e, std::vector<double> &u);

@@ -11,9 +11,12 @@
 //   M = (B + qI)^-1 (Newton-Schulz), A = I - 2qM = (B - qI)M,
 //   S^-1(G) = sum_k A^k (2q M G M) A^k, summed by X <- X + A_j X A_j with
 //   A_j = A^(2^j); the error after J levels is rho^(2^J), rho =
-//   (sqrt(k)-1)/(sqrt(k)+1) for the spectral bounds of B (host Lanczos on
-//   sym(B), widened).  B is positive definite in the CP step (B ~ dt^2/m).
-// Every product is an NB^3 GEMM on the device; nothing is diagonalised.
+//   (sqrt(k)-1)/(sqrt(k)+1) for the spectral bounds of B (Lanczos on the
+//   device, widened).  B is positive definite in the CP step (B ~ dt^2/m).
+// Every product is an NB^3 FP64-MFMA GEMM; nothing is diagonalised.  The
+// outer loop runs on the device: the convergence test sets a flag that makes
+// every later kernel of the batch return at once, so the host waits only
+// once per batch of iterations (batch = the previous call's count).
 #include <algorithm>
 #include <cmath>
 
@@ -21,48 +24,74 @@
 
 namespace {
 
-constexpr int T16 = 16;
+typedef double dbl4 __attribute__((ext_vector_type(4)));
 
-// C = alpha op(A) op(B) + beta D   (row-major n x n).  One 16x16 tile of C
-// per workgroup; the A row panel and B column panel (16 x n each) are loaded
-// into LDS in one pass, so the kernel pays one memory latency, not n/16.
-__global__ void __launch_bounds__(256) k_gemm(int n, const double *A, int ta, const double *B, int tb,
-                                              double *C, double alpha, const double *D, double beta) {
-  extern __shared__ double smem[];
-  double *sa = smem;                 // [16][n+1]
-  double *sb = smem + T16 * (n + 1); // [n][17]
-  const int ty = threadIdx.x / T16, tx = threadIdx.x % T16;
-  const int r0 = blockIdx.y * T16, c0 = blockIdx.x * T16;
-  for (int w = threadIdx.x; w < T16 * n; w += 256) {
-    int r, k;
-    if (ta) { r = w % T16; k = w / T16; } else { k = w % n; r = w / n; }
-    double v = 0.0;
-    if (r0 + r < n) v = ta ? A[(size_t)k * n + r0 + r] : A[(size_t)(r0 + r) * n + k];
-    sa[r * (n + 1) + k] = v;
+constexpr int T16 = 16;
+constexpr int kMaxLevels = 12;
+constexpr int kMaxIter = 200;      // src/paw_waves2.f90:1870 (NITERX)
+constexpr int kLzThreads = 1024;
+constexpr int kLzMaxSteps = 40;
+
+// Loop state of one ORTHO_X call, written only by k_check.
+struct OrthoState {
+  int done;   // 1 once converged or failed
+  int iter;   // iteration at which max|GAMN| < tolerance
+  int code;   // 0 ok, 1 NaN, 2 divergence
+  int pad;
+};
+
+__device__ inline bool stopped(const OrthoState *st) { return st != nullptr && st->done != 0; }
+
+// C = alpha op(A) op(B) + beta D   (row-major n x n) on FP64 MFMA.
+// One 16x16 tile of C per workgroup; its four waves take interleaved K slices
+// of four and are summed through LDS in a fixed order.  Lane layout of
+// v_mfma_f64_16x16x4f64: A(i,k) in lane i+16k, B(k,j) in lane j+16k, C(i,j)
+// in lane j+16(i%4), element i/4.
+__global__ void __launch_bounds__(256) k_gemm(int n, const double *__restrict__ A, int ta,
+                                              const double *__restrict__ B, int tb, double *C,
+                                              double alpha, const double *D, double beta,
+                                              const OrthoState *st) {
+  if (stopped(st)) return;
+  __shared__ dbl4 part[3][64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int i0 = blockIdx.y * T16, j0 = blockIdx.x * T16;
+  const int li = lane & 15, lk = lane >> 4;
+  const int ia = i0 + li, jb = j0 + li;
+  const bool aok = ia < n, bok = jb < n;
+  dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+  const int nk = (n + 3) / 4;
+  constexpr int U = 6;
+  for (int ks0 = wv; ks0 < nk; ks0 += 4 * U) {
+    double a[U], b[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int k = 4 * (ks0 + 4 * u) + lk;
+      a[u] = 0.0;
+      b[u] = 0.0;
+      if (k < n) {
+        if (aok) a[u] = ta ? A[(size_t)k * n + ia] : A[(size_t)ia * n + k];
+        if (bok) b[u] = tb ? B[(size_t)jb * n + k] : B[(size_t)k * n + jb];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b[u], acc, 0, 0, 0);
   }
-  for (int w = threadIdx.x; w < T16 * n; w += 256) {
-    int k, c;
-    if (tb) { k = w % n; c = w / n; } else { c = w % T16; k = w / T16; }
-    double v = 0.0;
-    if (c0 + c < n) v = tb ? B[(size_t)(c0 + c) * n + k] : B[(size_t)k * n + c0 + c];
-    sb[k * (T16 + 1) + c] = v;
-  }
+  if (wv > 0) part[wv - 1][lane] = acc;
   __syncthreads();
-  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-  const double *ra = sa + ty * (n + 1);
-  int k = 0;
-  for (; k + 3 < n; k += 4) {
-    a0 += ra[k] * sb[k * (T16 + 1) + tx];
-    a1 += ra[k + 1] * sb[(k + 1) * (T16 + 1) + tx];
-    a2 += ra[k + 2] * sb[(k + 2) * (T16 + 1) + tx];
-    a3 += ra[k + 3] * sb[(k + 3) * (T16 + 1) + tx];
-  }
-  for (; k < n; ++k) a0 += ra[k] * sb[k * (T16 + 1) + tx];
-  const int row = r0 + ty, col = c0 + tx;
-  if (row < n && col < n) {
-    double v = alpha * ((a0 + a1) + (a2 + a3));
-    if (D) v += beta * D[(size_t)row * n + col];
-    C[(size_t)row * n + col] = v;
+  if (wv != 0) return;
+  acc += part[0][lane];
+  acc += part[1][lane];
+  acc += part[2][lane];
+  const int j = j0 + li;
+  if (j >= n) return;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int i = i0 + lk + 4 * r;
+    if (i < n) {
+      double v = alpha * acc[r];
+      if (D) v += beta * D[(size_t)i * n + j];
+      C[(size_t)i * n + j] = v;
+    }
   }
 }
 
@@ -73,8 +102,152 @@ __global__ void k_symm(int n, const double *B, double *Bs) {
   Bs[idx] = 0.5 * (B[(size_t)i * n + j] + B[(size_t)j * n + i]);
 }
 
+__device__ inline double wave_sum(double s) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+  return s;
+}
+
+// number of eigenvalues below x of the symmetric tridiagonal (al, be[1..k-1])
+__device__ int sturm_count(int k, const double *al, const double *be, double x) {
+  int cnt = 0;
+  double d = 1.0;
+  for (int i = 0; i < k; ++i) {
+    d = (al[i] - x) - (i > 0 ? be[i] * be[i] / d : 0.0);
+    if (fabs(d) < 1.0e-300) d = -1.0e-300;
+    if (d < 0.0) ++cnt;
+  }
+  return cnt;
+}
+
+// Extreme eigenvalues of the symmetric n x n matrix B (row-major) by m-step
+// Lanczos with two passes of classical Gram-Schmidt against all previous
+// vectors, then multisection on the Sturm sequence of the tridiagonal matrix.
+// out = {smallest, largest} Ritz value (inner bounds of the spectrum).
+// One workgroup of kLzThreads; the Krylov basis lives in LDS.
+__global__ void __launch_bounds__(kLzThreads) k_lanczos(int n, int m, const double *__restrict__ B,
+                                                        double *out) {
+  extern __shared__ double sm[];
+  double *V = sm;                          // [m+1][n]
+  double *w = V + (size_t)(m + 1) * n;     // [n]
+  double *wp = w + n;                      // [G][n] partial matvec, G*n <= kLzThreads
+  double *c = wp + kLzThreads;             // [m+1]
+  double *al = c + (m + 1);                // [m]
+  double *be = al + m;                     // [m+1]
+  __shared__ double red[kLzThreads / 64];
+  __shared__ int kend_s;
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  constexpr int NW = kLzThreads / 64;
+  const int G = max(1, kLzThreads / n);
+  auto block_sum = [&](double s) -> double {
+    s = wave_sum(s);
+    if (lane == 0) red[wv] = s;
+    __syncthreads();
+    double r = 0.0;
+    for (int q = 0; q < NW; ++q) r += red[q];
+    __syncthreads();
+    return r;
+  };
+  double s = 0.0;
+  for (int i = t; i < n; i += kLzThreads) {
+    const double v = 1.0 + 0.37 * sin(1.7 * i + 0.3);
+    V[i] = v;
+    s += v * v;
+  }
+  const double nrm = sqrt(block_sum(s));
+  for (int i = t; i < n; i += kLzThreads) V[i] /= nrm;
+  if (t == 0) kend_s = m;
+  __syncthreads();
+  for (int k = 0; k < m; ++k) {
+    const double *v = V + (size_t)k * n;
+    // w = B v (B symmetric: column reads are coalesced); G row groups per column
+    for (int e = t; e < G * n; e += kLzThreads) {
+      const int i = e % n, g = e / n;
+      const int jlo = (int)((long)n * g / G), jhi = (int)((long)n * (g + 1) / G);
+      double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+      int j = jlo;
+      for (; j + 3 < jhi; j += 4) {
+        a0 += B[(size_t)j * n + i] * v[j];
+        a1 += B[(size_t)(j + 1) * n + i] * v[j + 1];
+        a2 += B[(size_t)(j + 2) * n + i] * v[j + 2];
+        a3 += B[(size_t)(j + 3) * n + i] * v[j + 3];
+      }
+      for (; j < jhi; ++j) a0 += B[(size_t)j * n + i] * v[j];
+      wp[(size_t)g * n + i] = (a0 + a1) + (a2 + a3);
+    }
+    __syncthreads();
+    for (int i = t; i < n; i += kLzThreads) {
+      double r = 0.0;
+      for (int g = 0; g < G; ++g) r += wp[(size_t)g * n + i];
+      w[i] = r;
+    }
+    __syncthreads();
+    for (int pass = 0; pass < 2; ++pass) {
+      for (int j = wv; j <= k; j += NW) {
+        double d = 0.0;
+        for (int i = lane; i < n; i += 64) d += w[i] * V[(size_t)j * n + i];
+        d = wave_sum(d);
+        if (lane == 0) c[j] = d;
+      }
+      __syncthreads();
+      if (pass == 0 && t == 0) al[k] = c[k];
+      for (int i = t; i < n; i += kLzThreads) {
+        double r = w[i];
+        for (int j = 0; j <= k; ++j) r -= c[j] * V[(size_t)j * n + i];
+        w[i] = r;
+      }
+      __syncthreads();
+    }
+    s = 0.0;
+    for (int i = t; i < n; i += kLzThreads) s += w[i] * w[i];
+    const double b = sqrt(block_sum(s));
+    if (t == 0) be[k + 1] = b;
+    if (b < 1.0e-14 * fabs(al[k]) || k + 1 == m) {
+      if (t == 0) kend_s = k + 1;
+      break;
+    }
+    for (int i = t; i < n; i += kLzThreads) V[(size_t)(k + 1) * n + i] = w[i] / b;
+    __syncthreads();
+  }
+  __syncthreads();
+  const int kend = kend_s;
+  if (wv != 0) return;
+  // Gershgorin interval of the tridiagonal matrix (every lane of wave 0)
+  double glo = 1.0e300, ghi = -1.0e300;
+  for (int i = 0; i < kend; ++i) {
+    const double r = (i > 0 ? fabs(be[i]) : 0.0) + (i + 1 < kend ? fabs(be[i + 1]) : 0.0);
+    glo = fmin(glo, al[i] - r);
+    ghi = fmax(ghi, al[i] + r);
+  }
+  // multisection: each round splits the bracket into 65 parts, one lane per point
+  double res[2];
+  for (int side = 0; side < 2; ++side) {
+    double lo = glo, hi = ghi;
+    for (int round = 0; round < 9; ++round) {
+      const double x = lo + (hi - lo) * (double)(lane + 1) / 65.0;
+      const int cnt = sturm_count(kend, al, be, x);
+      // side 0: smallest eigenvalue = first point with count >= 1
+      // side 1: largest eigenvalue = first point with count >= kend
+      const bool above = side == 0 ? cnt >= 1 : cnt >= kend;
+      const unsigned long long mask = __ballot(above);
+      const int first = mask ? __ffsll((long long)mask) - 1 : 64;
+      const double nlo = lo + (hi - lo) * (double)first / 65.0;
+      const double nhi = lo + (hi - lo) * (double)(first + 1) / 65.0;
+      lo = nlo;
+      hi = nhi;
+    }
+    res[side] = 0.5 * (lo + hi);
+  }
+  if (lane == 0) {
+    out[0] = res[0];
+    out[1] = res[1];
+  }
+}
+
 // GAMN(i,j) = 0.5(T(i,j)+T(j,i)) + A(min,max) - delta; block maxima of |GAMN|
-__global__ void k_gamma(int n, const double *T, const double *A, double *G, double *bmax) {
+__global__ void k_gamma(int n, const double *T, const double *A, double *G, double *bmax,
+                        const OrthoState *st) {
+  if (stopped(st)) return;
   __shared__ double red[256];
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   double m = 0.0;
@@ -85,6 +258,32 @@ __global__ void k_gamma(int n, const double *T, const double *A, double *G, doub
     if (i == j) v -= 1.0;
     G[idx] = v;
     m = fabs(v);
+    if (v != v) m = v;
+  }
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      const double a = red[threadIdx.x], b = red[threadIdx.x + s];
+      red[threadIdx.x] = (a != a || b != b) ? (a != a ? a : b) : fmax(a, b);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) bmax[blockIdx.x] = red[0];
+}
+
+// convergence test of iteration it (src/paw_waves2.f90:1960-1975)
+__global__ void k_check(int nblk, const double *bmax, double tol, int it, OrthoState *st) {
+  if (st->done) return;
+  __shared__ double red[256];
+  __shared__ int nan_s;
+  if (threadIdx.x == 0) nan_s = 0;
+  __syncthreads();
+  double m = 0.0;
+  for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
+    const double v = bmax[b];
+    if (v != v) nan_s = 1;
+    else m = fmax(m, v);
   }
   red[threadIdx.x] = m;
   __syncthreads();
@@ -92,71 +291,52 @@ __global__ void k_gamma(int n, const double *T, const double *A, double *G, doub
     if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
     __syncthreads();
   }
-  if (threadIdx.x == 0) bmax[blockIdx.x] = red[0];
+  if (threadIdx.x != 0) return;
+  const double digam = red[0];
+  if (nan_s) {
+    st->code = 1;
+    st->done = 1;
+  } else if (digam < tol) {
+    st->iter = it;
+    st->done = 1;
+  } else if (digam > 1.0e200) {
+    st->code = 2;
+    st->done = 1;
+  }
 }
 
-// LAMBDA(i,j) -= 2 (f_i+ds)/(f_i+f_j+2ds) X(i,j)   (src/paw_waves2.f90:1979-1987)
-__global__ void k_lam_update(int n, const double *occ, const double *X, double *lam) {
+__global__ void k_state_init(OrthoState *st) {
+  if (threadIdx.x == 0) {
+    st->done = 0;
+    st->iter = 0;
+    st->code = 0;
+    st->pad = 0;
+  }
+}
+
+// One LAMBDA update (src/paw_waves2.f90:1979-2005):
+//   L'(i,j) = L(i,j) - 2 (f_i+ds)/(f_i+f_j+2ds) X(i,j)
+//   Lnew(i,j) = L'(j,i) (f_i+ds)/(f_j+ds) if f_i < f_j, else L'(i,j)
+__global__ void k_lam_step(int n, const double *occ, const double *X, const double *lam,
+                           double *lnew, const OrthoState *st) {
+  if (stopped(st)) return;
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= n * n) return;
   const int i = idx / n, j = idx % n;
   const double oi = occ[i] + 1.0e-12, oj = occ[j] + 1.0e-12;
-  lam[idx] -= 2.0 * oi / (oi + oj) * X[idx];
-}
-
-// LAMBDA(i,j) = LAMBDA(j,i) (f_i+ds)/(f_j+ds) where f_i < f_j (src/paw_waves2.f90:1999-2005)
-__global__ void k_lam_sym(int n, const double *occ, const double *lin, double *lout) {
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= n * n) return;
-  const int i = idx / n, j = idx % n;
-  if (occ[i] < occ[j])
-    lout[idx] = lin[(size_t)j * n + i] * (occ[i] + 1.0e-12) / (occ[j] + 1.0e-12);
-  else
-    lout[idx] = lin[idx];
-}
-
-}  // namespace
-
-// cyclic Jacobi: a (n x n row-major symmetric) -> eigenvalues e, eigenvectors
-// as columns of u (row-major u[i*n+k] = component i of vector k)
-void host_syev(int n, std::vector<double> &a, std::vector<double> &e, std::vector<double> &u) {
-  u.assign((size_t)n * n, 0.0);
-  for (int i = 0; i < n; ++i) u[(size_t)i * n + i] = 1.0;
-  for (int sweep = 0; sweep < 100; ++sweep) {
-    bool rotated = false;
-    for (int p = 0; p < n - 1; ++p) {
-      for (int q = p + 1; q < n; ++q) {
-        const double apq = a[(size_t)p * n + q];
-        const double app = a[(size_t)p * n + p], aqq = a[(size_t)q * n + q];
-        if (std::fabs(apq) <= 1.0e-18 * std::sqrt(std::fabs(app * aqq)) || apq == 0.0) continue;
-        rotated = true;
-        const double theta = (aqq - app) / (2.0 * apq);
-        const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
-        const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
-        for (int k = 0; k < n; ++k) {
-          const double akp = a[(size_t)k * n + p], akq = a[(size_t)k * n + q];
-          a[(size_t)k * n + p] = c * akp - s * akq;
-          a[(size_t)k * n + q] = s * akp + c * akq;
-        }
-        for (int k = 0; k < n; ++k) {
-          const double apk = a[(size_t)p * n + k], aqk = a[(size_t)q * n + k];
-          a[(size_t)p * n + k] = c * apk - s * aqk;
-          a[(size_t)q * n + k] = s * apk + c * aqk;
-        }
-        for (int k = 0; k < n; ++k) {
-          const double ukp = u[(size_t)k * n + p], ukq = u[(size_t)k * n + q];
-          u[(size_t)k * n + p] = c * ukp - s * ukq;
-          u[(size_t)k * n + q] = s * ukp + c * ukq;
-        }
-      }
-    }
-    if (!rotated) break;
+  if (occ[i] < occ[j]) {
+    const size_t t = (size_t)j * n + i;
+    const double lt = lam[t] - 2.0 * oj / (oj + oi) * X[t];
+    lnew[idx] = lt * oi / oj;
+  } else {
+    lnew[idx] = lam[idx] - 2.0 * oi / (oi + oj) * X[idx];
   }
-  e.resize(n);
-  for (int i = 0; i < n; ++i) e[i] = a[(size_t)i * n + i];
 }
 
-namespace {
+__global__ void k_copy_d(size_t n, const double *src, double *dst) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
 
 __global__ void k_scaled_identity(int n, double a, double *X) {
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
@@ -171,17 +351,16 @@ __global__ void k_cayley(int n, double q, const double *M, double *A) {
   A[idx] = ((idx / n == idx % n) ? 1.0 : 0.0) - 2.0 * q * M[idx];
 }
 
-constexpr int kMaxLevels = 12;
-
 struct OrthoWork {
-  double *bs, *m0, *m1, *gam, *x0, *x1, *t1, *t2, *lam2, *occ, *red;
+  double *bs, *m0, *m1, *gam, *x0, *x1, *t1, *t2, *lam2, *occ, *red, *bounds;
   double *apow[kMaxLevels];
+  OrthoState *st;
 };
 
 int ortho_work(pw_waveset *ws, OrthoWork &w) {
   const size_t n2 = (size_t)ws->nb * ws->nb;
   const size_t nblk = (n2 + 255) / 256;
-  const size_t need = (10 + kMaxLevels) * n2 + ws->nb + 2 * nblk + 64;
+  const size_t need = (9 + kMaxLevels) * n2 + ws->nb + nblk + 64;
   if (need > ws->ortho_cap) {
     if (ws->d_ortho) PW_HIP(hipFree(ws->d_ortho));
     PW_HIP(hipMalloc(&ws->d_ortho, need * sizeof(double)));
@@ -190,74 +369,43 @@ int ortho_work(pw_waveset *ws, OrthoWork &w) {
   double *p = ws->d_ortho;
   w.bs = p; p += n2; w.m0 = p; p += n2; w.m1 = p; p += n2; w.gam = p; p += n2;
   w.x0 = p; p += n2; w.x1 = p; p += n2; w.t1 = p; p += n2; w.t2 = p; p += n2;
-  w.lam2 = p; p += n2; p += n2;
+  w.lam2 = p; p += n2;
   for (int j = 0; j < kMaxLevels; ++j) { w.apow[j] = p; p += n2; }
-  w.occ = p; p += ws->nb; w.red = p;
+  w.occ = p; p += ws->nb;
+  w.bounds = p; p += 8;
+  w.st = reinterpret_cast<OrthoState *>(p); p += 8;
+  w.red = p;
   return PW_OK;
 }
 
 int gemm(pw_waveset *ws, const double *A, int ta, const double *B, int tb, double *C,
-         double alpha = 1.0, const double *D = nullptr, double beta = 0.0) {
+         double alpha = 1.0, const double *D = nullptr, double beta = 0.0,
+         const OrthoState *st = nullptr) {
   const int n = ws->nb;
   dim3 grid((n + T16 - 1) / T16, (n + T16 - 1) / T16);
-  const size_t lds = (size_t)(T16 * (n + 1) + n * (T16 + 1)) * sizeof(double);
-  if (lds > 65536) {
-    static bool raised = false;
-    if (!raised) {
-      PW_HIP(hipFuncSetAttribute((const void *)k_gemm, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-      raised = true;
-    }
-  }
-  k_gemm<<<grid, 256, lds, ws->stream>>>(n, A, ta, B, tb, C, alpha, D, beta);
-  PW_HIP(hipGetLastError());
+  k_gemm<<<grid, 256, 0, ws->stream>>>(n, A, ta, B, tb, C, alpha, D, beta, st);
   return PW_OK;
 }
 
-// Extreme eigenvalues of a symmetric matrix (row-major) by Lanczos with full
-// re-orthogonalisation; returns inner (Ritz) bounds.
-void host_lanczos_bounds(int n, const std::vector<double> &a, double *lo, double *hi) {
-  const int m = std::min(n, 40);
-  std::vector<double> V((size_t)(m + 1) * n, 0.0), alpha(m, 0.0), beta(m + 1, 0.0), w(n);
-  // start vector with all components (deterministic)
-  double nrm = 0.0;
-  for (int i = 0; i < n; ++i) { V[i] = 1.0 + 0.37 * std::sin(1.7 * i + 0.3); nrm += V[i] * V[i]; }
-  nrm = std::sqrt(nrm);
-  for (int i = 0; i < n; ++i) V[i] /= nrm;
-  int k = 0;
-  for (; k < m; ++k) {
-    const double *v = &V[(size_t)k * n];
-    for (int i = 0; i < n; ++i) {
-      double s = 0.0;
-      const double *ar = &a[(size_t)i * n];
-      for (int j = 0; j < n; ++j) s += ar[j] * v[j];
-      w[i] = s;
-    }
-    double al = 0.0;
-    for (int i = 0; i < n; ++i) al += w[i] * v[i];
-    alpha[k] = al;
-    for (int rep = 0; rep < 2; ++rep)
-      for (int j = 0; j <= k; ++j) {
-        const double *vj = &V[(size_t)j * n];
-        double c = 0.0;
-        for (int i = 0; i < n; ++i) c += w[i] * vj[i];
-        for (int i = 0; i < n; ++i) w[i] -= c * vj[i];
-      }
-    double b = 0.0;
-    for (int i = 0; i < n; ++i) b += w[i] * w[i];
-    b = std::sqrt(b);
-    beta[k + 1] = b;
-    if (b < 1.0e-14 * std::fabs(al) || k + 1 == m) { ++k; break; }
-    double *vn = &V[(size_t)(k + 1) * n];
-    for (int i = 0; i < n; ++i) vn[i] = w[i] / b;
-  }
-  std::vector<double> t((size_t)k * k, 0.0), e, u;
-  for (int i = 0; i < k; ++i) {
-    t[(size_t)i * k + i] = alpha[i];
-    if (i + 1 < k) t[(size_t)i * k + i + 1] = t[(size_t)(i + 1) * k + i] = beta[i + 1];
-  }
-  host_syev(k, t, e, u);
-  *lo = *std::min_element(e.begin(), e.end());
-  *hi = *std::max_element(e.begin(), e.end());
+int lanczos_bounds(pw_waveset *ws, const double *d_b, double *d_out, double *lo, double *hi) {
+  const int n = ws->nb;
+  if (n > kLzThreads) return pw_fail(PW_ERR_UNSUPPORTED, "WAVES_ORTHO_X: NB too large");
+  // LDS: (m+1) n + n + kLzThreads + 3m + 2 doubles, at most 144 KiB
+  const long budget = 144L * 1024 / 8 - n - kLzThreads - 3 * kLzMaxSteps - 2;
+  int m = std::min({n, kLzMaxSteps, (int)(budget / n) - 1});
+  if (m < 2) return pw_fail(PW_ERR_UNSUPPORTED, "WAVES_ORTHO_X: NB too large for the Lanczos bounds");
+  const size_t lds = ((size_t)(m + 1) * n + n + kLzThreads + 3 * (size_t)m + 2) * sizeof(double);
+  if (lds > 65536)
+    PW_HIP(hipFuncSetAttribute((const void *)k_lanczos, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)lds));
+  k_lanczos<<<1, kLzThreads, lds, ws->stream>>>(n, m, d_b, d_out);
+  PW_HIP(hipGetLastError());
+  double h[2];
+  PW_HIP(hipMemcpyAsync(h, d_out, 2 * sizeof(double), hipMemcpyDeviceToHost, ws->stream));
+  PW_HIP(hipStreamSynchronize(ws->stream));
+  *lo = h[0];
+  *hi = h[1];
+  return PW_OK;
 }
 
 }  // namespace
@@ -272,13 +420,11 @@ int ortho_x_device(pw_waveset *ws, double *d_lam, const double *d_chichi, const 
   PW_TRY(ortho_work(ws, w));
   PW_HIP(hipMemcpyAsync(w.occ, ws->occ.data(), n * sizeof(double), hipMemcpyHostToDevice, ws->stream));
   k_symm<<<nblk, 256, 0, ws->stream>>>(n, d_chipsi, w.bs);
+  k_state_init<<<1, 64, 0, ws->stream>>>(w.st);
   PW_HIP(hipGetLastError());
-  // spectral bounds of B on the host (inner Ritz bounds, widened by 3%)
-  std::vector<double> hb(n2);
-  PW_HIP(hipMemcpyAsync(hb.data(), w.bs, n2 * sizeof(double), hipMemcpyDeviceToHost, ws->stream));
-  PW_HIP(hipStreamSynchronize(ws->stream));
+  // spectral bounds of B (inner Ritz bounds, widened by 3%)
   double lo, hi;
-  host_lanczos_bounds(n, hb, &lo, &hi);
+  PW_TRY(lanczos_bounds(ws, w.bs, w.bounds, &lo, &hi));
   if (!(lo > 0.0) || !(hi >= lo))
     return pw_fail(PW_ERR_NOCONV, "WAVES_ORTHO_X: sym(CHIPSI) is not positive definite");
   lo *= 0.97;
@@ -311,41 +457,53 @@ int ortho_x_device(pw_waveset *ws, double *d_lam, const double *d_chichi, const 
   for (int j = 1; j < nlev; ++j) PW_TRY(gemm(ws, w.apow[j - 1], 0, w.apow[j - 1], 0, w.apow[j]));
   PW_HIP(hipGetLastError());
   const double eps = 1.0e-8;
-  std::vector<double> hm(nblk);
-  for (int it = 1; it <= 200; ++it) {
-    // HAUX = CHICHI LAMBDA + 2 CHIPSI ; T = LAMBDA^T HAUX
-    PW_TRY(gemm(ws, d_chichi, 0, d_lam, 0, w.t1, 1.0, d_chipsi, 2.0));
-    PW_TRY(gemm(ws, d_lam, 1, w.t1, 0, w.t2));
-    k_gamma<<<nblk, 256, 0, ws->stream>>>(n, w.t2, d_psipsi, w.gam, w.red);
+  // LAMBDA ping-pongs between the caller's buffer (even number of updates) and lam2
+  double *lbuf[2] = {d_lam, w.lam2};
+  int launched = 0;
+  int batch = std::max(2, std::min(ws->ortho_iter > 0 ? ws->ortho_iter : 8, kMaxIter));
+  OrthoState hs{};
+  while (launched < kMaxIter) {
+    const int nit = std::min(batch, kMaxIter - launched);
+    for (int it = launched + 1; it <= launched + nit; ++it) {
+      const double *lam = lbuf[(it - 1) & 1];
+      double *lnew = lbuf[it & 1];
+      // HAUX = CHICHI LAMBDA + 2 CHIPSI ; T = LAMBDA^T HAUX
+      PW_TRY(gemm(ws, d_chichi, 0, lam, 0, w.t1, 1.0, d_chipsi, 2.0, w.st));
+      PW_TRY(gemm(ws, lam, 1, w.t1, 0, w.t2, 1.0, nullptr, 0.0, w.st));
+      k_gamma<<<nblk, 256, 0, ws->stream>>>(n, w.t2, d_psipsi, w.gam, w.red, w.st);
+      k_check<<<1, 256, 0, ws->stream>>>(nblk, w.red, eps, it, w.st);
+      // X = S^-1(GAMN): X0 = 2q M G M ; X <- X + A_j X A_j
+      PW_TRY(gemm(ws, w.gam, 0, mc, 0, w.t1, 1.0, nullptr, 0.0, w.st));
+      PW_TRY(gemm(ws, mc, 0, w.t1, 0, w.x0, 2.0 * q, nullptr, 0.0, w.st));
+      double *xc = w.x0, *xn = w.x1;
+      for (int j = 0; j < nlev; ++j) {
+        PW_TRY(gemm(ws, xc, 0, w.apow[j], 0, w.t1, 1.0, nullptr, 0.0, w.st));
+        PW_TRY(gemm(ws, w.apow[j], 0, w.t1, 0, xn, 1.0, xc, 1.0, w.st));
+        std::swap(xc, xn);
+      }
+      k_lam_step<<<nblk, 256, 0, ws->stream>>>(n, w.occ, xc, lam, lnew, w.st);
+    }
     PW_HIP(hipGetLastError());
-    PW_HIP(hipMemcpyAsync(hm.data(), w.red, nblk * sizeof(double), hipMemcpyDeviceToHost, ws->stream));
+    launched += nit;
+    PW_HIP(hipMemcpyAsync(&hs, w.st, sizeof(OrthoState), hipMemcpyDeviceToHost, ws->stream));
     PW_HIP(hipStreamSynchronize(ws->stream));
-    double digam = 0.0;
-    for (double v : hm) digam = std::max(digam, v);
-    if (!(digam == digam)) return pw_fail(PW_ERR_NOCONV, "WAVES_ORTHO_X: NaN in overlap");
-    if (digam < eps) {
-      *iter = it;
-      return PW_OK;
-    }
-    if (digam > 1.0e200)
-      return pw_fail(PW_ERR_NOCONV, "WAVES_ORTHO_X: iteration loop for orthogonalization is about to diverge");
-    // X = S^-1(GAMN): X0 = 2q M G M ; X <- X + A_j X A_j
-    PW_TRY(gemm(ws, w.gam, 0, mc, 0, w.t1));
-    PW_TRY(gemm(ws, mc, 0, w.t1, 0, w.x0, 2.0 * q));
-    double *xc = w.x0, *xn = w.x1;
-    for (int j = 0; j < nlev; ++j) {
-      PW_TRY(gemm(ws, xc, 0, w.apow[j], 0, w.t1));
-      PW_TRY(gemm(ws, w.apow[j], 0, w.t1, 0, xn, 1.0, xc, 1.0));
-      std::swap(xc, xn);
-    }
-    k_lam_update<<<nblk, 256, 0, ws->stream>>>(n, w.occ, xc, d_lam);
-    k_lam_sym<<<nblk, 256, 0, ws->stream>>>(n, w.occ, d_lam, w.lam2);
-    PW_HIP(hipMemcpyAsync(d_lam, w.lam2, n2 * sizeof(double), hipMemcpyDeviceToDevice, ws->stream));
+    if (hs.done) break;
+    batch = 4;
+  }
+  if (hs.code == 1) return pw_fail(PW_ERR_NOCONV, "WAVES_ORTHO_X: NaN in overlap");
+  if (hs.code == 2)
+    return pw_fail(PW_ERR_NOCONV, "WAVES_ORTHO_X: iteration loop for orthogonalization is about to diverge");
+  if (!hs.done)
+    return pw_fail(PW_ERR_NOCONV,
+                   "WAVES_ORTHO_X: loop for orthogonalization is not converged (specify SAFEORTHO=F, "
+                   "reduce the cutoff or the time step)");
+  // iteration hs.iter converged after hs.iter-1 updates
+  if ((hs.iter - 1) & 1) {
+    k_copy_d<<<nblk, 256, 0, ws->stream>>>(n2, w.lam2, d_lam);
     PW_HIP(hipGetLastError());
   }
-  return pw_fail(PW_ERR_NOCONV,
-                 "WAVES_ORTHO_X: loop for orthogonalization is not converged (specify SAFEORTHO=F, "
-                 "reduce the cutoff or the time step)");
+  *iter = hs.iter;
+  return PW_OK;
 }
 
 // ---------------------------------------------------------------------------

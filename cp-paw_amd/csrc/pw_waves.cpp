@@ -218,7 +218,7 @@ extern "C" int pw_waves_create(pw_plan *plan, int nb, int nat, int nsp, const in
   if (nb <= 0 || nb % 2 != 0)
     return pw_fail(PW_ERR_UNSUPPORTED, "pw_waves_create: NB must be positive and even for super wave functions");
   if (nat <= 0 || nsp <= 0 || lnxx <= 0) return pw_fail(PW_ERR_ARG, "pw_waves_create: bad sizes");
-  if (nb > 480) return pw_fail(PW_ERR_UNSUPPORTED, "pw_waves_create: NB > 480 is not supported");
+  if (nb > 1024) return pw_fail(PW_ERR_UNSUPPORTED, "pw_waves_create: NB > 1024 is not supported");
   if (prec != PW_PREC_FP64 && prec != PW_PREC_MIXED) return pw_fail(PW_ERR_ARG, "pw_waves_create: bad precision");
   PW_TRY(plan_device(plan));
   pw_waveset *ws = new pw_waveset();
