@@ -151,6 +151,11 @@ struct pw_waveset {
   double ekin = 0, epot = 0, e1c = 0, etot = 0, wavekin1 = 0, wavekin2 = 0;
   int ortho_iter = 0;
   int ortho_levels = 0, ortho_ns = 0;
+  // spectral bounds of sym(CHIPSI) from the last Lanczos run, reused while
+  // the device checks of ORTHO_X accept them
+  double lz_lo = 0.0, lz_hi = 0.0;
+  bool lz_valid = false;
+  int ortho_lanczos = 0;       // Lanczos runs in the last ORTHO_X call
   std::vector<double> force;   // 3*nat
   std::vector<double> eig;     // nb
   std::vector<double> mat, omat, oomat;
@@ -171,6 +176,7 @@ struct pw_waveset {
   double2 *d_dhmat = nullptr;  // per atom DH (LMNXX x LMNXX)
   int *d_atom_isp = nullptr, *d_ipro0 = nullptr, *d_ibpro = nullptr, *d_lmnx = nullptr;
   int *d_lmn_ln = nullptr, *d_lmn_lm = nullptr, *d_lmn_l = nullptr;
+  int2 *d_ipro_tab = nullptr;  // (atom, lmn) of each projector
   double2 *d_phase = nullptr;  // nat*3*(2*nmaxx+1)
   int nphase = 0;              // 2*nmaxx+1
   double *d_part = nullptr; size_t part_bytes = 0;   // split-K partials

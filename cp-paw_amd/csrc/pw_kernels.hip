@@ -65,74 +65,104 @@ __global__ void k_phase(int nat, int nphase, int nmaxx, const double *gr, double
 }
 
 // ---------------------------------------------------------------------------
-// projections: part[split][ipro][b] = sum_G conj(P) psi over the split's G range
-// tile: 64 projector rows x 32 bands, K chunk 32
+// FP64 MFMA building blocks.  v_mfma_f64_16x16x4f64 lane layout: A(i,k) in
+// lane i+16k, B(k,j) in lane j+16k, C(i,j) in lane j+16(i%4), element i/4.
+// A complex product is four real MFMAs on split re/im operands.  LDS rows are
+// padded to 16 (mod 32) doubles so that the 16 x 4 operand reads of a wave
+// are free of bank conflicts.
 // ---------------------------------------------------------------------------
-constexpr int PR = 64, PBT = 32, PKC = 32;
+typedef double dbl4 __attribute__((ext_vector_type(4)));
 
-__global__ void __launch_bounds__(kB) k_proj(ProjGeo q, const double2 *psi, int gsplit, int npro,
-                                             double2 *part) {
-  __shared__ double2 sP[PKC][PR];
-  __shared__ double2 sS[PKC][PBT];
-  __shared__ double2 sE[8][PKC];
-  const int apt = min(8, PR / q.lmnxx);             // atoms per tile
-  const int atom0 = blockIdx.x * apt;
-  const int b0 = blockIdx.y * PBT;
-  const int split = blockIdx.z;
-  const int g0 = split * gsplit, g1 = min(q.ngl, g0 + gsplit);
-  const int t = threadIdx.x;
-  const int tr = t / 16, tb = t % 16;
-  double2 acc[4][2];
+constexpr int KC = 16;   // K depth staged in LDS per step (four MFMA K slices)
+
+constexpr int padw(int w) { return w + ((16 - w % 32) + 32) % 32; }
+
+__device__ inline dbl4 mfma4(double a, double b, dbl4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// (cr + i ci) += (ar + i ai)(br + i bi)
+__device__ inline void cmfma(dbl4 &cr, dbl4 &ci, double ar, double ai, double br, double bi) {
+  cr = mfma4(ar, br, cr);
+  cr = mfma4(-ai, bi, cr);
+  ci = mfma4(ar, bi, ci);
+  ci = mfma4(ai, br, ci);
+}
+
+// conj(P(ipro, G)) for the projections
+struct GenProj {
+  ProjGeo q;
+  const int2 *iptab;
+  __device__ double2 operator()(int r, int ig) const {
+    const int2 al = iptab[r];
+    return cconj(pro_val(q, al.x, al.y, ig, eigr_of(q, al.x, ig)));
+  }
+};
+
+// stacked rows [conj(psi1_i(G)) ; psi1_i(-G)] for the band overlaps; rows
+// [0,nbhp) give T, rows [nbhp, 2 nbhp) give TM (nbhp = nbh rounded up to 16)
+struct GenOvl {
+  const double2 *psi1;
+  const int *minusg;
+  int ngl, nbh, nbhp;
+  __device__ double2 operator()(int r, int ig) const {
+    if (r < nbhp) return r < nbh ? cconj(psi1[(size_t)r * ngl + ig]) : czero();
+    r -= nbhp;
+    return r < nbh ? psi1[(size_t)r * ngl + minusg[ig]] : czero();
+  }
+};
+
+// part[split][r][b] = sum_{G in split} A(r,G) psi[b][G], A from a generator.
+// Workgroup: NW waves, one 16-row block each, all NCB 16-band blocks; the
+// psi chunk is shared through LDS.
+template <class Gen, int NCB, int NW>
+__global__ void __launch_bounds__(64 * NW) k_rowgemm(Gen gen, int nrow, int ngl, int nbh,
+                                                     const double2 *__restrict__ psi, int gsplit,
+                                                     double2 *__restrict__ part) {
+  constexpr int WR = 16 * NW, SA = padw(WR), NB16 = 16 * NCB, SB = padw(NB16);
+  __shared__ double sAr[KC][SA], sAi[KC][SA], sBr[KC][SB], sBi[KC][SB];
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6, li = lane & 15, lk = lane >> 4;
+  const int r0 = blockIdx.x * WR, b0 = blockIdx.y * NB16;
+  const int g0 = blockIdx.z * gsplit, g1 = min(ngl, g0 + gsplit);
+  dbl4 accr[NCB], acci[NCB];
 #pragma unroll
-  for (int i = 0; i < 4; ++i) { acc[i][0] = czero(); acc[i][1] = czero(); }
-  for (int gc = g0; gc < g1; gc += PKC) {
-    // structure factors of the tile's atoms
-    for (int w = t; w < apt * PKC; w += kB) {
-      const int a = w / PKC, k = w % PKC, ig = gc + k, atom = atom0 + a;
-      sE[a][k] = (atom < q.nat && ig < g1) ? eigr_of(q, atom, ig) : czero();
-    }
-    for (int w = t; w < PKC * PBT; w += kB) {
-      const int b = w / PKC, k = w % PKC, ig = gc + k;
-      sS[k][b] = (b0 + b < q.nbh && ig < g1) ? psi[(size_t)(b0 + b) * q.ngl + ig] : czero();
-    }
-    __syncthreads();
-    for (int w = t; w < PKC * PR; w += kB) {
-      const int k = w % PKC, r = w / PKC, ig = gc + k;
-      const int a = r / q.lmnxx, lmn = r % q.lmnxx, atom = atom0 + a;
+  for (int c = 0; c < NCB; ++c) {
+    accr[c] = dbl4{0.0, 0.0, 0.0, 0.0};
+    acci[c] = dbl4{0.0, 0.0, 0.0, 0.0};
+  }
+  for (int gc = g0; gc < g1; gc += KC) {
+    for (int e = t; e < WR * KC; e += 64 * NW) {
+      const int k = e % KC, r = e / KC, ig = gc + k;
       double2 v = czero();
-      if (a < apt && atom < q.nat && ig < g1) v = cconj(pro_val(q, atom, lmn, ig, sE[a][k]));
-      sP[k][r] = v;
+      if (r0 + r < nrow && ig < g1) v = gen(r0 + r, ig);
+      sAr[k][r] = v.x;
+      sAi[k][r] = v.y;
+    }
+    for (int e = t; e < NB16 * KC; e += 64 * NW) {
+      const int k = e % KC, b = e / KC, ig = gc + k;
+      double2 v = czero();
+      if (b0 + b < nbh && ig < g1) v = psi[(size_t)(b0 + b) * ngl + ig];
+      sBr[k][b] = v.x;
+      sBi[k][b] = v.y;
     }
     __syncthreads();
-#pragma unroll 4
-    for (int k = 0; k < PKC; ++k) {
-      double2 a[4], b[2];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) a[i] = sP[k][tr * 4 + i];
-      b[0] = sS[k][tb * 2];
-      b[1] = sS[k][tb * 2 + 1];
+    for (int ks = 0; ks < KC / 4; ++ks) {
+      const int k = 4 * ks + lk;
+      const double ar = sAr[k][16 * wv + li], ai = sAi[k][16 * wv + li];
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-          acc[i][j].x += a[i].x * b[j].x - a[i].y * b[j].y;
-          acc[i][j].y += a[i].x * b[j].y + a[i].y * b[j].x;
-        }
+      for (int c = 0; c < NCB; ++c) cmfma(accr[c], acci[c], ar, ai, sBr[k][16 * c + li], sBi[k][16 * c + li]);
     }
     __syncthreads();
   }
-  double2 *dst = part + (size_t)split * npro * q.nbh;
+  double2 *dst = part + (size_t)blockIdx.z * nrow * nbh;
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const int r = tr * 4 + i;
-    const int a = r / q.lmnxx, lmn = r % q.lmnxx, atom = atom0 + a;
-    if (a >= apt || atom >= q.nat) continue;
-    if (lmn >= q.lmnx[q.atom_isp[atom]]) continue;
-    const int ipro = q.ipro0[atom] + lmn;
+  for (int c = 0; c < NCB; ++c) {
+    const int j = b0 + 16 * c + li;
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      const int b = b0 + tb * 2 + j;
-      if (b < q.nbh) dst[(size_t)ipro * q.nbh + b] = acc[i][j];
+    for (int r = 0; r < 4; ++r) {
+      const int i = r0 + 16 * wv + lk + 4 * r;
+      if (i < nrow && j < nbh) dst[(size_t)i * nbh + j] = make_double2(accr[c][r], acci[c][r]);
     }
   }
 }
@@ -147,67 +177,74 @@ __global__ void k_reduce_c(const double2 *part, int nsplit, int n, double scale,
 }
 
 // ---------------------------------------------------------------------------
-// ADDPRO: psi[b][G] += sum_ipro P[ipro][G] X[ipro][b]; tile 64 G x 32 bands
+// psi[b][G] += sum_k A(b,k) B(k,G): the band-major output is C^T, so a lane's
+// MFMA results are 16 consecutive G of one band (coalesced read-modify-write).
+// A^T(k,b) and B(k,G) come from generators; NW waves of 16 G each, NRB
+// 16-band blocks per wave.
 // ---------------------------------------------------------------------------
-constexpr int AG = 64, ABT = 32, AAT = 4;
+// ADDPRO (src/paw_waves1.f90:5930-5980 WAVES_ADDPRO): A^T(ipro,b) = X[ipro][b], B = P
+struct AddproA {
+  const double2 *x;
+  int nbh;
+  __device__ double2 operator()(int k, int b) const { return x[(size_t)k * nbh + b]; }
+};
+struct AddproB {
+  ProjGeo q;
+  const int2 *iptab;
+  __device__ double2 operator()(int k, int ig) const {
+    const int2 al = iptab[k];
+    return pro_val(q, al.x, al.y, ig, eigr_of(q, al.x, ig));
+  }
+};
 
-__global__ void __launch_bounds__(kB) k_addpro(ProjGeo q, double2 *psi, const double2 *x) {
-  __shared__ double2 sP[AAT * 16][AG];
-  __shared__ double2 sX[AAT * 16][ABT];
-  __shared__ double2 sE[AAT][AG];
-  const int g0 = blockIdx.x * AG;
-  const int b0 = blockIdx.y * ABT;
-  const int t = threadIdx.x;
-  const int tb = t / 16, tg = t % 16;
-  const int rows = AAT * q.lmnxx;
-  double2 acc[2][4];
+template <class GA, class GB, int NRB, int NW>
+__global__ void __launch_bounds__(64 * NW) k_colgemm(GA ga, GB gb, int nk, int ngl, int nbh,
+                                                     double2 *__restrict__ psi) {
+  constexpr int WG = 16 * NW, SP = padw(WG), NB16 = 16 * NRB, SX = padw(NB16);
+  __shared__ double sBr[KC][SP], sBi[KC][SP], sAr[KC][SX], sAi[KC][SX];
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6, li = lane & 15, lk = lane >> 4;
+  const int g0 = blockIdx.x * WG, b0 = blockIdx.y * NB16;
+  dbl4 accr[NRB], acci[NRB];
 #pragma unroll
-  for (int j = 0; j < 2; ++j)
-#pragma unroll
-    for (int i = 0; i < 4; ++i) acc[j][i] = czero();
-  for (int atom0 = 0; atom0 < q.nat; atom0 += AAT) {
-    for (int w = t; w < AAT * AG; w += kB) {
-      const int a = w / AG, k = w % AG, ig = g0 + k, atom = atom0 + a;
-      sE[a][k] = (atom < q.nat && ig < q.ngl) ? eigr_of(q, atom, ig) : czero();
-    }
-    for (int w = t; w < rows * ABT; w += kB) {
-      const int r = w / ABT, b = w % ABT;
-      const int a = r / q.lmnxx, lmn = r % q.lmnxx, atom = atom0 + a;
+  for (int c = 0; c < NRB; ++c) {
+    accr[c] = dbl4{0.0, 0.0, 0.0, 0.0};
+    acci[c] = dbl4{0.0, 0.0, 0.0, 0.0};
+  }
+  for (int k0 = 0; k0 < nk; k0 += KC) {
+    for (int e = t; e < WG * KC; e += 64 * NW) {
+      const int g = e % WG, k = e / WG, ig = g0 + g;
       double2 v = czero();
-      if (atom < q.nat && b0 + b < q.nbh && lmn < q.lmnx[q.atom_isp[atom]])
-        v = x[(size_t)(q.ipro0[atom] + lmn) * q.nbh + b0 + b];
-      sX[r][b] = v;
+      if (k0 + k < nk && ig < ngl) v = gb(k0 + k, ig);
+      sBr[k][g] = v.x;
+      sBi[k][g] = v.y;
+    }
+    for (int e = t; e < NB16 * KC; e += 64 * NW) {
+      const int b = e % NB16, k = e / NB16;
+      double2 v = czero();
+      if (k0 + k < nk && b0 + b < nbh) v = ga(k0 + k, b0 + b);
+      sAr[k][b] = v.x;
+      sAi[k][b] = v.y;
     }
     __syncthreads();
-    for (int w = t; w < rows * AG; w += kB) {
-      const int r = w / AG, k = w % AG, ig = g0 + k;
-      const int a = r / q.lmnxx, lmn = r % q.lmnxx, atom = atom0 + a;
-      sP[r][k] = (atom < q.nat && ig < q.ngl) ? pro_val(q, atom, lmn, ig, sE[a][k]) : czero();
-    }
-    __syncthreads();
-    for (int r = 0; r < rows; ++r) {
-      const double2 x0 = sX[r][tb * 2], x1 = sX[r][tb * 2 + 1];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const double2 p = sP[r][tg + 16 * i];
-        acc[0][i].x += p.x * x0.x - p.y * x0.y;
-        acc[0][i].y += p.x * x0.y + p.y * x0.x;
-        acc[1][i].x += p.x * x1.x - p.y * x1.y;
-        acc[1][i].y += p.x * x1.y + p.y * x1.x;
-      }
+    for (int ks = 0; ks < KC / 4; ++ks) {
+      const int k = 4 * ks + lk;
+      const double br = sBr[k][16 * wv + li], bi = sBi[k][16 * wv + li];
+#pragma unroll
+      for (int c = 0; c < NRB; ++c) cmfma(accr[c], acci[c], sAr[k][16 * c + li], sAi[k][16 * c + li], br, bi);
     }
     __syncthreads();
   }
+  const int ig = g0 + 16 * wv + li;
+  if (ig >= ngl) return;
 #pragma unroll
-  for (int j = 0; j < 2; ++j) {
-    const int b = b0 + tb * 2 + j;
-    if (b >= q.nbh) continue;
+  for (int c = 0; c < NRB; ++c) {
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int ig = g0 + tg + 16 * i;
-      if (ig < q.ngl) {
-        double2 &o = psi[(size_t)b * q.ngl + ig];
-        o = cadd(o, acc[j][i]);
+    for (int r = 0; r < 4; ++r) {
+      const int b = b0 + 16 * c + lk + 4 * r;
+      if (b < nbh) {
+        double2 &o = psi[(size_t)b * ngl + ig];
+        o = make_double2(o.x + accr[c][r], o.y + acci[c][r]);
       }
     }
   }
@@ -216,79 +253,90 @@ __global__ void __launch_bounds__(kB) k_addpro(ProjGeo q, double2 *psi, const do
 // ---------------------------------------------------------------------------
 // forces: F(atom) = 2 Omega sum_G G Re[i P conj(D)], D = psi conj(dE/dproj)
 // (WAVES_DEDPRO + WAVES_PROFORCE fused; the INVERTG symmetrisation of DEDPRO
-//  leaves this sum unchanged because P(-G) = conj(P(G)))
-// one wavefront per atom, one lane per G; part[gtile][atom][3]
+//  leaves this sum unchanged because P(-G) = conj(P(G))).
+// D(ipro,G) = sum_b conj(ded[ipro][b]) psi[b][G] on MFMA (NW waves x 16 ipro
+// rows, NGB 16-G blocks); the epilogue contracts D with P and G and sums the
+// workgroup's G range: part[gtile][ipro][3].
 // ---------------------------------------------------------------------------
-constexpr int FG = 64, FAT = 4, FBK = 32, FMAXL = 16;
-
-__global__ void __launch_bounds__(kB) k_force(ProjGeo q, const double2 *psi, const double2 *ded,
-                                              const double *gvec, double *part) {
-  __shared__ double2 sS[FBK][FG];
-  __shared__ double2 sD[FAT][FMAXL][FBK];
-  const int g0 = blockIdx.x * FG;
-  const int atom0 = blockIdx.y * FAT;
-  const int t = threadIdx.x;
-  const int a = t / 64, lane = t % 64;
-  const int atom = atom0 + a;
-  const int ig = g0 + lane;
-  const bool act = atom < q.nat;
-  const int lmnx = act ? q.lmnx[q.atom_isp[atom]] : 0;
-  double2 acc[FMAXL];
+template <int NGB, int NW>
+__global__ void __launch_bounds__(64 * NW) k_force_m(ProjGeo q, const int2 *iptab, int npro,
+                                                     const double2 *__restrict__ psi,
+                                                     const double2 *__restrict__ ded,
+                                                     const double *__restrict__ gvec, double *part) {
+  constexpr int WR = 16 * NW, SA = padw(WR), NG16 = 16 * NGB, SB = padw(NG16);
+  __shared__ double sAr[KC][SA], sAi[KC][SA], sBr[KC][SB], sBi[KC][SB];
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6, li = lane & 15, lk = lane >> 4;
+  const int g0 = blockIdx.x * NG16, r0 = blockIdx.y * WR;
+  const int nbh = q.nbh, ngl = q.ngl;
+  dbl4 accr[NGB], acci[NGB];
 #pragma unroll
-  for (int i = 0; i < FMAXL; ++i) acc[i] = czero();
-  for (int bc = 0; bc < q.nbh; bc += FBK) {
-    for (int w = t; w < FBK * FG; w += kB) {
-      const int b = w / FG, k = w % FG, gg = g0 + k;
-      sS[b][k] = (bc + b < q.nbh && gg < q.ngl) ? psi[(size_t)(bc + b) * q.ngl + gg] : czero();
-    }
-    for (int w = t; w < FAT * FMAXL * FBK; w += kB) {
-      const int aa = w / (FMAXL * FBK), rem = w % (FMAXL * FBK);
-      const int lmn = rem / FBK, b = rem % FBK, at = atom0 + aa;
+  for (int c = 0; c < NGB; ++c) {
+    accr[c] = dbl4{0.0, 0.0, 0.0, 0.0};
+    acci[c] = dbl4{0.0, 0.0, 0.0, 0.0};
+  }
+  for (int k0 = 0; k0 < nbh; k0 += KC) {
+    for (int e = t; e < WR * KC; e += 64 * NW) {
+      const int k = e % KC, r = e / KC, b = k0 + k, ip = r0 + r;
       double2 v = czero();
-      if (at < q.nat && lmn < q.lmnx[q.atom_isp[at]] && bc + b < q.nbh)
-        v = cconj(ded[(size_t)(q.ipro0[at] + lmn) * q.nbh + bc + b]);
-      sD[aa][lmn][b] = v;
+      if (ip < npro && b < nbh) v = ded[(size_t)ip * nbh + b];
+      sAr[k][r] = v.x;
+      sAi[k][r] = -v.y;
+    }
+    for (int e = t; e < NG16 * KC; e += 64 * NW) {
+      const int g = e % NG16, k = e / NG16, b = k0 + k, ig = g0 + g;
+      double2 v = czero();
+      if (b < nbh && ig < ngl) v = psi[(size_t)b * ngl + ig];
+      sBr[k][g] = v.x;
+      sBi[k][g] = v.y;
     }
     __syncthreads();
-    const int nbk = min(FBK, q.nbh - bc);
-    for (int b = 0; b < nbk; ++b) {
-      const double2 s = sS[b][lane];
 #pragma unroll
-      for (int i = 0; i < FMAXL; ++i) {
-        if (i < lmnx) {
-          const double2 d = sD[a][i][b];
-          acc[i].x += d.x * s.x - d.y * s.y;
-          acc[i].y += d.x * s.y + d.y * s.x;
-        }
-      }
+    for (int ks = 0; ks < KC / 4; ++ks) {
+      const int k = 4 * ks + lk;
+      const double ar = sAr[k][16 * wv + li], ai = sAi[k][16 * wv + li];
+#pragma unroll
+      for (int c = 0; c < NGB; ++c) cmfma(accr[c], acci[c], ar, ai, sBr[k][16 * c + li], sBi[k][16 * c + li]);
     }
     __syncthreads();
   }
-  double f0 = 0.0, f1 = 0.0, f2 = 0.0;
-  if (act && ig < q.ngl) {
-    const double2 e = eigr_of(q, atom, ig);
-    double c = 0.0;
+  double f[4][3];
 #pragma unroll
-    for (int i = 0; i < FMAXL; ++i) {
-      if (i < lmnx) {
-        const double2 p = pro_val(q, atom, i, ig, e);
-        // Re(i p conj(d)) = -Im(p conj(d)) = -(p.y d.x - p.x d.y)
-        c += p.x * acc[i].y - p.y * acc[i].x;
-      }
+  for (int r = 0; r < 4; ++r) f[r][0] = f[r][1] = f[r][2] = 0.0;
+#pragma unroll
+  for (int c = 0; c < NGB; ++c) {
+    const int ig = g0 + 16 * c + li;
+    if (ig >= ngl) continue;
+    const double gx = gvec[3 * ig], gy = gvec[3 * ig + 1], gz = gvec[3 * ig + 2];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int ip = r0 + 16 * wv + lk + 4 * r;
+      if (ip >= npro) continue;
+      const int2 al = iptab[ip];
+      const double2 pv = pro_val(q, al.x, al.y, ig, eigr_of(q, al.x, ig));
+      // Re(i p conj(d)) = p.x d.y - p.y d.x
+      const double cc = pv.x * acci[c][r] - pv.y * accr[c][r];
+      f[r][0] += cc * gx;
+      f[r][1] += cc * gy;
+      f[r][2] += cc * gz;
     }
-    f0 = c * gvec[3 * ig];
-    f1 = c * gvec[3 * ig + 1];
-    f2 = c * gvec[3 * ig + 2];
   }
-  // wavefront reduction (fixed tree)
-  for (int off = 32; off > 0; off >>= 1) {
-    f0 += __shfl_down(f0, off, 64);
-    f1 += __shfl_down(f1, off, 64);
-    f2 += __shfl_down(f2, off, 64);
-  }
-  if (lane == 0 && act) {
-    double *dst = part + ((size_t)blockIdx.x * q.nat + atom) * 3;
-    dst[0] = f0; dst[1] = f1; dst[2] = f2;
+  // sum over the 16 lanes of a row (fixed butterfly)
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+      for (int off = 8; off > 0; off >>= 1) f[r][d] += __shfl_xor(f[r][d], off);
+  if (li != 0) return;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int ip = r0 + 16 * wv + lk + 4 * r;
+    if (ip < npro) {
+      double *dst = part + ((size_t)blockIdx.x * npro + ip) * 3;
+      dst[0] = f[r][0];
+      dst[1] = f[r][1];
+      dst[2] = f[r][2];
+    }
   }
 }
 
@@ -298,67 +346,6 @@ __global__ void k_reduce_r(const double *part, int nsplit, int n, double scale, 
   double s = 0.0;
   for (int k = 0; k < nsplit; ++k) s += part[(size_t)k * n + i];
   out[i] = s * scale;
-}
-
-// ---------------------------------------------------------------------------
-// overlaps: T = sum_G conj(psi1_i) psi2_j, TM = sum_G psi1_i psi2_j(-G)
-// tile 32 x 32 bands, split over G; part[split][2][nbh][nbh]
-// ---------------------------------------------------------------------------
-constexpr int OT = 32, OKC = 32;
-
-__global__ void __launch_bounds__(kB) k_overlap(int ngl, int nbh, const int *minusg,
-                                                const double2 *psi1, const double2 *psi2,
-                                                int gsplit, double2 *part) {
-  __shared__ double2 s1[OKC][OT];
-  __shared__ double2 s2[OKC][OT];
-  __shared__ double2 s2m[OKC][OT];
-  const int i0 = blockIdx.x * OT, j0 = blockIdx.y * OT, split = blockIdx.z;
-  const int g0 = split * gsplit, g1 = min(ngl, g0 + gsplit);
-  const int t = threadIdx.x, ti = t / 16, tj = t % 16;
-  double2 at[2][2], am[2][2];
-#pragma unroll
-  for (int a = 0; a < 2; ++a)
-#pragma unroll
-    for (int b = 0; b < 2; ++b) { at[a][b] = czero(); am[a][b] = czero(); }
-  for (int gc = g0; gc < g1; gc += OKC) {
-    for (int w = t; w < OKC * OT; w += kB) {
-      const int b = w / OKC, k = w % OKC, ig = gc + k;
-      const bool ok = ig < g1;
-      s1[k][b] = (ok && i0 + b < nbh) ? psi1[(size_t)(i0 + b) * ngl + ig] : czero();
-      s2[k][b] = (ok && j0 + b < nbh) ? psi2[(size_t)(j0 + b) * ngl + ig] : czero();
-      s2m[k][b] = (ok && j0 + b < nbh) ? psi2[(size_t)(j0 + b) * ngl + minusg[ig]] : czero();
-    }
-    __syncthreads();
-#pragma unroll 4
-    for (int k = 0; k < OKC; ++k) {
-      double2 a[2] = {s1[k][ti * 2], s1[k][ti * 2 + 1]};
-      double2 b[2] = {s2[k][tj * 2], s2[k][tj * 2 + 1]};
-      double2 bm[2] = {s2m[k][tj * 2], s2m[k][tj * 2 + 1]};
-#pragma unroll
-      for (int x = 0; x < 2; ++x)
-#pragma unroll
-        for (int y = 0; y < 2; ++y) {
-          // conj(a) b
-          at[x][y].x += a[x].x * b[y].x + a[x].y * b[y].y;
-          at[x][y].y += a[x].x * b[y].y - a[x].y * b[y].x;
-          // a bm
-          am[x][y].x += a[x].x * bm[y].x - a[x].y * bm[y].y;
-          am[x][y].y += a[x].x * bm[y].y + a[x].y * bm[y].x;
-        }
-    }
-    __syncthreads();
-  }
-  double2 *dst = part + (size_t)split * 2 * nbh * nbh;
-#pragma unroll
-  for (int x = 0; x < 2; ++x)
-#pragma unroll
-    for (int y = 0; y < 2; ++y) {
-      const int i = i0 + ti * 2 + x, j = j0 + tj * 2 + y;
-      if (i < nbh && j < nbh) {
-        dst[(size_t)i * nbh + j] = at[x][y];
-        dst[(size_t)nbh * nbh + (size_t)i * nbh + j] = am[x][y];
-      }
-    }
 }
 
 // unravel super-wave products into the NB x NB real matrix (row-major),
@@ -442,63 +429,24 @@ __device__ inline void fold_lambda(const double *lam, int nb, int i, int j, doub
 }
 
 // ADDOPSI: psibar[j][G] += sum_i L1[i][j] chi[i][G] + L2[i][j] conj(chi[i][-G])
-constexpr int XG = 64, XJ = 32, XK = 16;
-
-__global__ void __launch_bounds__(kB) k_addopsi(int ngl, int nbh, int nb, const int *minusg,
-                                                double2 *psibar, const double2 *chi,
-                                                const double *lam) {
-  __shared__ double2 sC[XK][XG];
-  __shared__ double2 sM[XK][XG];
-  __shared__ double2 sL1[XK][XJ];
-  __shared__ double2 sL2[XK][XJ];
-  const int g0 = blockIdx.x * XG, j0 = blockIdx.y * XJ;
-  const int t = threadIdx.x, tj = t / 16, tg = t % 16;
-  double2 acc[2][4];
-#pragma unroll
-  for (int j = 0; j < 2; ++j)
-#pragma unroll
-    for (int i = 0; i < 4; ++i) acc[j][i] = czero();
-  for (int i0 = 0; i0 < nbh; i0 += XK) {
-    for (int w = t; w < XK * XG; w += kB) {
-      const int i = w / XG, k = w % XG, ig = g0 + k;
-      const bool ok = i0 + i < nbh && ig < ngl;
-      sC[i][k] = ok ? chi[(size_t)(i0 + i) * ngl + ig] : czero();
-      sM[i][k] = ok ? cconj(chi[(size_t)(i0 + i) * ngl + minusg[ig]]) : czero();
-    }
-    for (int w = t; w < XK * XJ; w += kB) {
-      const int i = w / XJ, j = w % XJ;
-      double2 l1 = czero(), l2 = czero();
-      if (i0 + i < nbh && j0 + j < nbh) fold_lambda(lam, nb, i0 + i, j0 + j, l1, l2);
-      sL1[i][j] = l1;
-      sL2[i][j] = l2;
-    }
-    __syncthreads();
-    for (int i = 0; i < XK; ++i) {
-      const double2 a0 = sL1[i][tj * 2], a1 = sL1[i][tj * 2 + 1];
-      const double2 c0 = sL2[i][tj * 2], c1 = sL2[i][tj * 2 + 1];
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const double2 x = sC[i][tg + 16 * q], xm = sM[i][tg + 16 * q];
-        acc[0][q] = cadd(acc[0][q], cadd(cmul(a0, x), cmul(c0, xm)));
-        acc[1][q] = cadd(acc[1][q], cadd(cmul(a1, x), cmul(c1, xm)));
-      }
-    }
-    __syncthreads();
+// as one column GEMM over the stacked K = 2 NBH
+struct AddopsiA {
+  const double *lam;
+  int nb, nbh;
+  __device__ double2 operator()(int k, int j) const {
+    double2 l1, l2;
+    fold_lambda(lam, nb, k < nbh ? k : k - nbh, j, l1, l2);
+    return k < nbh ? l1 : l2;
   }
-#pragma unroll
-  for (int j = 0; j < 2; ++j) {
-    const int jj = j0 + tj * 2 + j;
-    if (jj >= nbh) continue;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int ig = g0 + tg + 16 * q;
-      if (ig < ngl) {
-        double2 &o = psibar[(size_t)jj * ngl + ig];
-        o = cadd(o, acc[j][q]);
-      }
-    }
+};
+struct AddopsiB {
+  const double2 *chi;
+  const int *minusg;
+  int ngl, nbh;
+  __device__ double2 operator()(int k, int ig) const {
+    return k < nbh ? chi[(size_t)k * ngl + ig] : cconj(chi[(size_t)(k - nbh) * ngl + minusg[ig]]);
   }
-}
+};
 
 // ADDOPROJ: proj[k][j] += sum_i oproj[k][i] L1[i][j] + conj(oproj[k][i]) L2[i][j]
 __global__ void k_addoproj(int npro, int nbh, int nb, double2 *proj, const double2 *oproj,
@@ -669,10 +617,79 @@ ProjGeo make_pgeo(pw_waveset *ws) {
   return q;
 }
 
-int nsplit_for(int ngl, int tiles, int chunk) {
-  int want = std::max(1, 1024 / std::max(tiles, 1));
-  int maxs = std::max(1, (ngl + chunk - 1) / chunk);
-  return std::min(want, maxs);
+// blocks of 16 bands per workgroup (at most 8) and the number of band tiles
+void band_tiles(int nbh, int *nblk, int *ntiles) {
+  const int total = (nbh + 15) / 16;
+  *ntiles = (total + 7) / 8;
+  *nblk = (total + *ntiles - 1) / *ntiles;
+}
+
+#define PW_NB_SWITCH(nblk, LAUNCH)                                      \
+  switch (nblk) {                                                       \
+    case 1: LAUNCH(1); break;                                           \
+    case 2: LAUNCH(2); break;                                           \
+    case 3: LAUNCH(3); break;                                           \
+    case 4: LAUNCH(4); break;                                           \
+    case 5: LAUNCH(5); break;                                           \
+    case 6: LAUNCH(6); break;                                           \
+    case 7: LAUNCH(7); break;                                           \
+    default: LAUNCH(8); break;                                          \
+  }
+
+}  // namespace
+
+int ws_part(pw_waveset *ws, size_t bytes);
+
+namespace {
+
+// out[r][b] = scale sum_G A(r,G) psi[b][G]: split over G (about two waves per
+// SIMD), deterministic reduction of the splits.  out == nullptr puts the
+// result behind the partials and returns it in *res.
+template <int NW, class Gen>
+int run_rowgemm(pw_waveset *ws, const Gen &gen, int nrow, const double2 *psi, double scale,
+                double2 *out, double2 **res, const char *name) {
+  const int ngl = ws->plan->ngl, nbh = ws->nbh;
+  int nblk, tb;
+  band_tiles(nbh, &nblk, &tb);
+  const int tr = (nrow + 16 * NW - 1) / (16 * NW);
+  const int want = std::max(1, 2048 / NW / std::max(1, tr * tb));
+  int ns = std::min(want, (ngl + KC - 1) / KC);
+  const int gsplit = ((ngl + ns - 1) / ns + KC - 1) / KC * KC;
+  ns = (ngl + gsplit - 1) / gsplit;
+  const size_t n = (size_t)nrow * nbh;
+  PW_TRY(ws_part(ws, (size_t)(ns + (out ? 0 : 1)) * n * sizeof(double2)));
+  double2 *part = (double2 *)ws->d_part;
+  if (!out) out = part + (size_t)ns * n;
+  {
+    ScopeTimer tm(ws, name);
+    dim3 grid(tr, tb, ns);
+#define PW_LAUNCH(NB) k_rowgemm<Gen, NB, NW><<<grid, 64 * NW, 0, ws->stream>>>(gen, nrow, ngl, nbh, psi, gsplit, part)
+    PW_NB_SWITCH(nblk, PW_LAUNCH)
+#undef PW_LAUNCH
+    PW_HIP(hipGetLastError());
+  }
+  {
+    ScopeTimer tm(ws, "reduce");
+    k_reduce_c<<<(n + 255) / 256, 256, 0, ws->stream>>>(part, ns, (int)n, scale, out);
+    PW_HIP(hipGetLastError());
+  }
+  if (res) *res = out;
+  return PW_OK;
+}
+
+// psi[b][G] += sum_k A(b,k) B(k,G)
+template <int NW, class GA, class GB>
+int run_colgemm(pw_waveset *ws, const GA &ga, const GB &gb, int nk, double2 *psi, const char *name) {
+  const int ngl = ws->plan->ngl, nbh = ws->nbh;
+  int nblk, tb;
+  band_tiles(nbh, &nblk, &tb);
+  dim3 grid((ngl + 16 * NW - 1) / (16 * NW), tb);
+  ScopeTimer tm(ws, name);
+#define PW_LAUNCH(NB) k_colgemm<GA, GB, NB, NW><<<grid, 64 * NW, 0, ws->stream>>>(ga, gb, nk, ngl, nbh, psi)
+  PW_NB_SWITCH(nblk, PW_LAUNCH)
+#undef PW_LAUNCH
+  PW_HIP(hipGetLastError());
+  return PW_OK;
 }
 
 }  // namespace
@@ -722,82 +739,62 @@ int k_phase_tables(pw_waveset *ws, const double *r, double2 *phase) {
 int k_projections(pw_waveset *ws, const double2 *psi, const double2 *phase, double2 *proj) {
   ProjGeo q = make_pgeo(ws);
   q.phase = phase;
-  const int apt = std::min(8, PR / ws->lmnxx);
-  const int ta = (ws->nat + apt - 1) / apt;
-  const int tb = (ws->nbh + PBT - 1) / PBT;
-  const int ns = nsplit_for(q.ngl, ta * tb, PKC);
-  const int gsplit = ((q.ngl + ns - 1) / ns + PKC - 1) / PKC * PKC;
-  const int nsr = (q.ngl + gsplit - 1) / gsplit;
-  const size_t n = (size_t)ws->npro * ws->nbh;
-  PW_TRY(ws_part(ws, (size_t)nsr * n * sizeof(double2)));
-  {
-    ScopeTimer tm(ws, "projections");
-    k_proj<<<dim3(ta, tb, nsr), kB, 0, ws->stream>>>(q, psi, gsplit, ws->npro, (double2 *)ws->d_part);
-    PW_HIP(hipGetLastError());
-  }
-  {
-    ScopeTimer tm(ws, "reduce");
-    k_reduce_c<<<(n + 255) / 256, 256, 0, ws->stream>>>((const double2 *)ws->d_part, nsr, (int)n,
-                                                        ws->plan->vol, proj);
-    PW_HIP(hipGetLastError());
-  }
-  return PW_OK;
+  GenProj gen{q, ws->d_ipro_tab};
+  return run_rowgemm<4>(ws, gen, ws->npro, psi, ws->plan->vol, proj, nullptr, "projections");
 }
 
 int k_addpro(pw_waveset *ws, double2 *psi, const double2 *phase, const double2 *x) {
-  if (ws->lmnxx > 16) return pw_fail(PW_ERR_UNSUPPORTED, "WAVES_ADDPRO: LMNX > 16");
   ProjGeo q = make_pgeo(ws);
   q.phase = phase;
-  dim3 grid((q.ngl + AG - 1) / AG, (ws->nbh + ABT - 1) / ABT);
-  ScopeTimer tm(ws, "addpro");
-  k_addpro<<<grid, kB, 0, ws->stream>>>(q, psi, x);
-  PW_HIP(hipGetLastError());
-  return PW_OK;
+  AddproA ga{x, ws->nbh};
+  AddproB gb{q, ws->d_ipro_tab};
+  return run_colgemm<2>(ws, ga, gb, ws->npro, psi, "addpro");
 }
 
 int k_force(pw_waveset *ws, const double2 *psi, const double2 *phase, const double2 *ded,
             double *force_host) {
-  if (ws->lmnxx > FMAXL) return pw_fail(PW_ERR_UNSUPPORTED, "WAVES$FORCE: LMNX > 16");
+  constexpr int NW = 4, NGB = 4;
   ProjGeo q = make_pgeo(ws);
   q.phase = phase;
-  const int ngt = (q.ngl + FG - 1) / FG;
-  const size_t n = (size_t)ws->nat * 3;
+  const int ngt = (q.ngl + 16 * NGB - 1) / (16 * NGB);
+  const size_t n = (size_t)ws->npro * 3;
   PW_TRY(ws_part(ws, (size_t)ngt * n * sizeof(double)));
   PW_TRY(ws_small(ws, n * sizeof(double)));
   {
     ScopeTimer tm(ws, "force");
-    k_force<<<dim3(ngt, (ws->nat + FAT - 1) / FAT), kB, 0, ws->stream>>>(q, psi, ded, ws->plan->d_gvec,
-                                                                        ws->d_part);
+    dim3 grid(ngt, (ws->npro + 16 * NW - 1) / (16 * NW));
+    k_force_m<NGB, NW><<<grid, 64 * NW, 0, ws->stream>>>(q, ws->d_ipro_tab, ws->npro, psi, ded,
+                                                         ws->plan->d_gvec, ws->d_part);
     PW_HIP(hipGetLastError());
   }
   k_reduce_r<<<(n + 255) / 256, 256, 0, ws->stream>>>(ws->d_part, ngt, (int)n, 2.0 * ws->plan->vol,
                                                       ws->d_small);
   PW_HIP(hipGetLastError());
-  PW_HIP(hipMemcpyAsync(force_host, ws->d_small, n * sizeof(double), hipMemcpyDeviceToHost, ws->stream));
+  std::vector<double> fp(n);
+  PW_HIP(hipMemcpyAsync(fp.data(), ws->d_small, n * sizeof(double), hipMemcpyDeviceToHost, ws->stream));
   PW_HIP(hipStreamSynchronize(ws->stream));
+  // sum the projector contributions of each atom (fixed order)
+  for (int a = 0; a < ws->nat; ++a) {
+    const int i0 = ws->ipro0[a], nl = ws->lmnx[ws->isp[a]];
+    for (int d = 0; d < 3; ++d) {
+      double f = 0.0;
+      for (int l = 0; l < nl; ++l) f += fp[(size_t)(i0 + l) * 3 + d];
+      force_host[3 * a + d] = f;
+    }
+  }
   return PW_OK;
 }
 
 // mat_dev (row-major NB x NB) = WAVES_OVERLAP(psi1, psi2)
+// T = Omega sum_G conj(psi1_i) psi2_j and TM = Omega sum_G psi1_i psi2_j(-G)
+//   = Omega sum_G psi1_i(-G) psi2_j(G), one GEMM over stacked rows
 int k_overlap(pw_waveset *ws, const double2 *psi1, const double2 *psi2, double *mat, bool, bool) {
   pw_plan *p = ws->plan;
-  const int nbh = ws->nbh;
-  const int tiles = ((nbh + OT - 1) / OT) * ((nbh + OT - 1) / OT);
-  const int ns = nsplit_for(p->ngl, tiles, OKC);
-  const int gsplit = ((p->ngl + ns - 1) / ns + OKC - 1) / OKC * OKC;
-  const int nsr = (p->ngl + gsplit - 1) / gsplit;
-  const size_t n = (size_t)2 * nbh * nbh;
-  PW_TRY(ws_part(ws, (size_t)(nsr + 1) * n * sizeof(double2)));
-  double2 *red = (double2 *)ws->d_part + (size_t)nsr * n;
-  {
-    ScopeTimer tm(ws, "overlap");
-    k_overlap<<<dim3((nbh + OT - 1) / OT, (nbh + OT - 1) / OT, nsr), kB, 0, ws->stream>>>(
-        p->ngl, nbh, p->d_minusg, psi1, psi2, gsplit, (double2 *)ws->d_part);
-    PW_HIP(hipGetLastError());
-  }
-  k_reduce_c<<<(n + 255) / 256, 256, 0, ws->stream>>>((const double2 *)ws->d_part, nsr, (int)n, p->vol,
-                                                      red);
-  k_unravel_pw<<<(nbh * nbh + 255) / 256, 256, 0, ws->stream>>>(nbh, ws->nb, red, red + (size_t)nbh * nbh,
+  const int nbh = ws->nbh, nbhp = (nbh + 15) / 16 * 16;
+  GenOvl gen{psi1, p->d_minusg, p->ngl, nbh, nbhp};
+  double2 *red = nullptr;
+  PW_TRY(run_rowgemm<2>(ws, gen, 2 * nbhp, psi2, p->vol, nullptr, &red, "overlap"));
+  k_unravel_pw<<<(nbh * nbh + 255) / 256, 256, 0, ws->stream>>>(nbh, ws->nb, red, red + (size_t)nbhp * nbh,
                                                                   mat);
   PW_HIP(hipGetLastError());
   return PW_OK;
@@ -824,11 +821,9 @@ int k_atom_matrix(pw_waveset *ws, const double2 *proj, double2 *out, const doubl
 
 int k_addopsi(pw_waveset *ws, double2 *psibar, const double2 *chi, const double *lam) {
   pw_plan *p = ws->plan;
-  dim3 grid((p->ngl + XG - 1) / XG, (ws->nbh + XJ - 1) / XJ);
-  ScopeTimer tm(ws, "addopsi");
-  k_addopsi<<<grid, kB, 0, ws->stream>>>(p->ngl, ws->nbh, ws->nb, p->d_minusg, psibar, chi, lam);
-  PW_HIP(hipGetLastError());
-  return PW_OK;
+  AddopsiA ga{lam, ws->nb, ws->nbh};
+  AddopsiB gb{chi, p->d_minusg, p->ngl, ws->nbh};
+  return run_colgemm<2>(ws, ga, gb, 2 * ws->nbh, psibar, "addopsi");
 }
 
 int k_addoproj(pw_waveset *ws, double2 *proj, const double2 *oproj, const double *lam) {

@@ -30,7 +30,7 @@ constexpr int T16 = 16;
 constexpr int kMaxLevels = 12;
 constexpr int kMaxIter = 200;      // src/paw_waves2.f90:1870 (NITERX)
 constexpr int kLzThreads = 1024;
-constexpr int kLzMaxSteps = 40;
+constexpr int kLzMaxSteps = 30;
 
 // Loop state of one ORTHO_X call, written only by k_check.
 struct OrthoState {
@@ -43,16 +43,18 @@ struct OrthoState {
 __device__ inline bool stopped(const OrthoState *st) { return st != nullptr && st->done != 0; }
 
 // C = alpha op(A) op(B) + beta D   (row-major n x n) on FP64 MFMA.
-// One 16x16 tile of C per workgroup; its four waves take interleaved K slices
-// of four and are summed through LDS in a fixed order.  Lane layout of
+// One 16x16 tile of C per workgroup; its eight waves take interleaved K slices
+// of four (all loads of a wave issued before its MFMAs, one memory round trip
+// for n <= 256) and are summed through LDS in a fixed order.  Lane layout of
 // v_mfma_f64_16x16x4f64: A(i,k) in lane i+16k, B(k,j) in lane j+16k, C(i,j)
 // in lane j+16(i%4), element i/4.
-__global__ void __launch_bounds__(256) k_gemm(int n, const double *__restrict__ A, int ta,
-                                              const double *__restrict__ B, int tb, double *C,
-                                              double alpha, const double *D, double beta,
-                                              const OrthoState *st) {
+constexpr int kGemmWaves = 8;
+__global__ void __launch_bounds__(64 * kGemmWaves) k_gemm(int n, const double *__restrict__ A, int ta,
+                                                          const double *__restrict__ B, int tb, double *C,
+                                                          double alpha, const double *D, double beta,
+                                                          const OrthoState *st) {
   if (stopped(st)) return;
-  __shared__ dbl4 part[3][64];
+  __shared__ dbl4 part[kGemmWaves - 1][64];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int i0 = blockIdx.y * T16, j0 = blockIdx.x * T16;
   const int li = lane & 15, lk = lane >> 4;
@@ -60,12 +62,12 @@ __global__ void __launch_bounds__(256) k_gemm(int n, const double *__restrict__ 
   const bool aok = ia < n, bok = jb < n;
   dbl4 acc = {0.0, 0.0, 0.0, 0.0};
   const int nk = (n + 3) / 4;
-  constexpr int U = 6;
-  for (int ks0 = wv; ks0 < nk; ks0 += 4 * U) {
+  constexpr int U = 8;
+  for (int ks0 = wv; ks0 < nk; ks0 += kGemmWaves * U) {
     double a[U], b[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const int k = 4 * (ks0 + 4 * u) + lk;
+      const int k = 4 * (ks0 + kGemmWaves * u) + lk;
       a[u] = 0.0;
       b[u] = 0.0;
       if (k < n) {
@@ -79,9 +81,8 @@ __global__ void __launch_bounds__(256) k_gemm(int n, const double *__restrict__ 
   if (wv > 0) part[wv - 1][lane] = acc;
   __syncthreads();
   if (wv != 0) return;
-  acc += part[0][lane];
-  acc += part[1][lane];
-  acc += part[2][lane];
+#pragma unroll
+  for (int w = 0; w < kGemmWaves - 1; ++w) acc += part[w][lane];
   const int j = j0 + li;
   if (j >= n) return;
 #pragma unroll
@@ -164,16 +165,14 @@ __global__ void __launch_bounds__(kLzThreads) k_lanczos(int n, int m, const doub
     for (int e = t; e < G * n; e += kLzThreads) {
       const int i = e % n, g = e / n;
       const int jlo = (int)((long)n * g / G), jhi = (int)((long)n * (g + 1) / G);
-      double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+      double a[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
       int j = jlo;
-      for (; j + 3 < jhi; j += 4) {
-        a0 += B[(size_t)j * n + i] * v[j];
-        a1 += B[(size_t)(j + 1) * n + i] * v[j + 1];
-        a2 += B[(size_t)(j + 2) * n + i] * v[j + 2];
-        a3 += B[(size_t)(j + 3) * n + i] * v[j + 3];
+      for (; j + 7 < jhi; j += 8) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) a[u] += B[(size_t)(j + u) * n + i] * v[j + u];
       }
-      for (; j < jhi; ++j) a0 += B[(size_t)j * n + i] * v[j];
-      wp[(size_t)g * n + i] = (a0 + a1) + (a2 + a3);
+      for (; j < jhi; ++j) a[0] += B[(size_t)j * n + i] * v[j];
+      wp[(size_t)g * n + i] = ((a[0] + a[1]) + (a[2] + a[3])) + ((a[4] + a[5]) + (a[6] + a[7]));
     }
     __syncthreads();
     for (int i = t; i < n; i += kLzThreads) {
@@ -192,9 +191,14 @@ __global__ void __launch_bounds__(kLzThreads) k_lanczos(int n, int m, const doub
       __syncthreads();
       if (pass == 0 && t == 0) al[k] = c[k];
       for (int i = t; i < n; i += kLzThreads) {
-        double r = w[i];
-        for (int j = 0; j <= k; ++j) r -= c[j] * V[(size_t)j * n + i];
-        w[i] = r;
+        double r[4] = {w[i], 0.0, 0.0, 0.0};
+        int j = 0;
+        for (; j + 3 <= k; j += 4) {
+#pragma unroll
+          for (int u = 0; u < 4; ++u) r[u] -= c[j + u] * V[(size_t)(j + u) * n + i];
+        }
+        for (; j <= k; ++j) r[0] -= c[j] * V[(size_t)j * n + i];
+        w[i] = (r[0] + r[1]) + (r[2] + r[3]);
       }
       __syncthreads();
     }
@@ -305,6 +309,38 @@ __global__ void k_check(int nblk, const double *bmax, double tol, int it, OrthoS
   }
 }
 
+// Acceptance test of the spectral bounds (before the outer loop):
+// R = (B+qI)M - I must vanish and the last Smith power A^(2^J) must be small
+// enough that the truncated sum is exact to rounding; otherwise code 3 makes
+// the host recompute the bounds.
+__global__ void k_prep_check(int n, const double *T, const double *AJ, double rtol, double atol,
+                             OrthoState *st) {
+  __shared__ double r1[256], r2[256];
+  double m1 = 0.0, m2 = 0.0;
+  for (int idx = threadIdx.x; idx < n * n; idx += blockDim.x) {
+    const double d = T[idx] - ((idx / n == idx % n) ? 1.0 : 0.0);
+    const double a = AJ[idx];
+    m1 = (d != d || m1 != m1) ? NAN : fmax(m1, fabs(d));
+    m2 = (a != a || m2 != m2) ? NAN : fmax(m2, fabs(a));
+  }
+  r1[threadIdx.x] = m1;
+  r2[threadIdx.x] = m2;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      const double a = r1[threadIdx.x], b = r1[threadIdx.x + s];
+      r1[threadIdx.x] = (a != a || b != b) ? NAN : fmax(a, b);
+      const double c = r2[threadIdx.x], e = r2[threadIdx.x + s];
+      r2[threadIdx.x] = (c != c || e != e) ? NAN : fmax(c, e);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && !(r1[0] <= rtol && (double)n * r2[0] <= atol)) {
+    st->code = 3;
+    st->done = 1;
+  }
+}
+
 __global__ void k_state_init(OrthoState *st) {
   if (threadIdx.x == 0) {
     st->done = 0;
@@ -383,7 +419,7 @@ int gemm(pw_waveset *ws, const double *A, int ta, const double *B, int tb, doubl
          const OrthoState *st = nullptr) {
   const int n = ws->nb;
   dim3 grid((n + T16 - 1) / T16, (n + T16 - 1) / T16);
-  k_gemm<<<grid, 256, 0, ws->stream>>>(n, A, ta, B, tb, C, alpha, D, beta, st);
+  k_gemm<<<grid, 64 * kGemmWaves, 0, ws->stream>>>(n, A, ta, B, tb, C, alpha, D, beta, st);
   return PW_OK;
 }
 
@@ -420,90 +456,111 @@ int ortho_x_device(pw_waveset *ws, double *d_lam, const double *d_chichi, const 
   PW_TRY(ortho_work(ws, w));
   PW_HIP(hipMemcpyAsync(w.occ, ws->occ.data(), n * sizeof(double), hipMemcpyHostToDevice, ws->stream));
   k_symm<<<nblk, 256, 0, ws->stream>>>(n, d_chipsi, w.bs);
-  k_state_init<<<1, 64, 0, ws->stream>>>(w.st);
   PW_HIP(hipGetLastError());
-  // spectral bounds of B (inner Ritz bounds, widened by 3%)
-  double lo, hi;
-  PW_TRY(lanczos_bounds(ws, w.bs, w.bounds, &lo, &hi));
-  if (!(lo > 0.0) || !(hi >= lo))
-    return pw_fail(PW_ERR_NOCONV, "WAVES_ORTHO_X: sym(CHIPSI) is not positive definite");
-  lo *= 0.97;
-  hi *= 1.03;
-  const double q = std::sqrt(lo * hi);
-  const double kap = hi / lo;
-  const double rho = (std::sqrt(kap) - 1.0) / (std::sqrt(kap) + 1.0);
-  int nlev = 1;
-  if (rho > 1.0e-300) {
-    const double need = std::log(1.0e-17) / std::log(rho);
-    nlev = std::max(1, (int)std::ceil(std::log2(std::max(need, 1.0)))) + 1;
-  }
-  if (nlev > kMaxLevels) return pw_fail(PW_ERR_NOCONV, "WAVES_ORTHO_X: CHIPSI too ill-conditioned");
-  // M = (B + qI)^-1 by Newton-Schulz, X0 = 2/(lo+hi+2q) I
-  const double kp = (hi + q) / (lo + q);
-  const double e0 = (kp - 1.0) / (kp + 1.0);
-  int nns = 1;
-  if (e0 > 1.0e-300) nns = (int)std::ceil(std::log2(std::max(1.0, std::log(1.0e-18) / std::log(e0)))) + 1;
-  ws->ortho_levels = nlev;
-  ws->ortho_ns = nns;
-  k_scaled_identity<<<nblk, 256, 0, ws->stream>>>(n, 2.0 / (lo + hi + 2.0 * q), w.m0);
-  double *mc = w.m0, *mn = w.m1;
-  for (int k = 0; k < nns; ++k) {
-    PW_TRY(gemm(ws, w.bs, 0, mc, 0, w.t1, 1.0, mc, q));       // T = (B+qI) X
-    PW_TRY(gemm(ws, mc, 0, w.t1, 0, mn, -1.0, mc, 2.0));      // X' = 2X - X T
-    std::swap(mc, mn);
-  }
-  // A^(2^j), j = 0..nlev-1
-  k_cayley<<<nblk, 256, 0, ws->stream>>>(n, q, mc, w.apow[0]);
-  for (int j = 1; j < nlev; ++j) PW_TRY(gemm(ws, w.apow[j - 1], 0, w.apow[j - 1], 0, w.apow[j]));
-  PW_HIP(hipGetLastError());
-  const double eps = 1.0e-8;
-  // LAMBDA ping-pongs between the caller's buffer (even number of updates) and lam2
-  double *lbuf[2] = {d_lam, w.lam2};
-  int launched = 0;
-  int batch = std::max(2, std::min(ws->ortho_iter > 0 ? ws->ortho_iter : 8, kMaxIter));
-  OrthoState hs{};
-  while (launched < kMaxIter) {
-    const int nit = std::min(batch, kMaxIter - launched);
-    for (int it = launched + 1; it <= launched + nit; ++it) {
-      const double *lam = lbuf[(it - 1) & 1];
-      double *lnew = lbuf[it & 1];
-      // HAUX = CHICHI LAMBDA + 2 CHIPSI ; T = LAMBDA^T HAUX
-      PW_TRY(gemm(ws, d_chichi, 0, lam, 0, w.t1, 1.0, d_chipsi, 2.0, w.st));
-      PW_TRY(gemm(ws, lam, 1, w.t1, 0, w.t2, 1.0, nullptr, 0.0, w.st));
-      k_gamma<<<nblk, 256, 0, ws->stream>>>(n, w.t2, d_psipsi, w.gam, w.red, w.st);
-      k_check<<<1, 256, 0, ws->stream>>>(nblk, w.red, eps, it, w.st);
-      // X = S^-1(GAMN): X0 = 2q M G M ; X <- X + A_j X A_j
-      PW_TRY(gemm(ws, w.gam, 0, mc, 0, w.t1, 1.0, nullptr, 0.0, w.st));
-      PW_TRY(gemm(ws, mc, 0, w.t1, 0, w.x0, 2.0 * q, nullptr, 0.0, w.st));
-      double *xc = w.x0, *xn = w.x1;
-      for (int j = 0; j < nlev; ++j) {
-        PW_TRY(gemm(ws, xc, 0, w.apow[j], 0, w.t1, 1.0, nullptr, 0.0, w.st));
-        PW_TRY(gemm(ws, w.apow[j], 0, w.t1, 0, xn, 1.0, xc, 1.0, w.st));
-        std::swap(xc, xn);
-      }
-      k_lam_step<<<nblk, 256, 0, ws->stream>>>(n, w.occ, xc, lam, lnew, w.st);
+  ws->ortho_lanczos = 0;
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    k_state_init<<<1, 64, 0, ws->stream>>>(w.st);
+    // spectral bounds of B: inner Ritz bounds, reused from the previous call
+    // until the acceptance test below rejects them
+    if (!ws->lz_valid || attempt > 0) {
+      double lo0, hi0;
+      PW_TRY(lanczos_bounds(ws, w.bs, w.bounds, &lo0, &hi0));
+      ws->lz_lo = lo0;
+      ws->lz_hi = hi0;
+      ws->lz_valid = true;
+      ws->ortho_lanczos++;
     }
+    double lo = ws->lz_lo, hi = ws->lz_hi;
+    if (!(lo > 0.0) || !(hi >= lo)) {
+      ws->lz_valid = false;
+      return pw_fail(PW_ERR_NOCONV, "WAVES_ORTHO_X: sym(CHIPSI) is not positive definite");
+    }
+    lo *= 0.97;
+    hi *= 1.03;
+    const double q = std::sqrt(lo * hi);
+    const double kap = hi / lo;
+    const double rho = (std::sqrt(kap) - 1.0) / (std::sqrt(kap) + 1.0);
+    int nlev = 1;
+    if (rho > 1.0e-300) {
+      const double need = std::log(1.0e-17) / std::log(rho);
+      nlev = std::max(1, (int)std::ceil(std::log2(std::max(need, 1.0)))) + 1;
+    }
+    if (nlev > kMaxLevels) return pw_fail(PW_ERR_NOCONV, "WAVES_ORTHO_X: CHIPSI too ill-conditioned");
+    // M = (B + qI)^-1 by Newton-Schulz, X0 = 2/(lo+hi+2q) I
+    const double kp = (hi + q) / (lo + q);
+    const double e0 = (kp - 1.0) / (kp + 1.0);
+    int nns = 1;
+    if (e0 > 1.0e-300) nns = (int)std::ceil(std::log2(std::max(1.0, std::log(1.0e-18) / std::log(e0)))) + 1;
+    ws->ortho_levels = nlev;
+    ws->ortho_ns = nns;
+    k_scaled_identity<<<nblk, 256, 0, ws->stream>>>(n, 2.0 / (lo + hi + 2.0 * q), w.m0);
+    double *mc = w.m0, *mn = w.m1;
+    for (int k = 0; k < nns; ++k) {
+      PW_TRY(gemm(ws, w.bs, 0, mc, 0, w.t1, 1.0, mc, q));       // T = (B+qI) X
+      PW_TRY(gemm(ws, mc, 0, w.t1, 0, mn, -1.0, mc, 2.0));      // X' = 2X - X T
+      std::swap(mc, mn);
+    }
+    // A^(2^j), j = 0..nlev-1
+    k_cayley<<<nblk, 256, 0, ws->stream>>>(n, q, mc, w.apow[0]);
+    for (int j = 1; j < nlev; ++j) PW_TRY(gemm(ws, w.apow[j - 1], 0, w.apow[j - 1], 0, w.apow[j]));
+    PW_TRY(gemm(ws, w.bs, 0, mc, 0, w.t1, 1.0, mc, q));         // (B+qI) M
+    k_prep_check<<<1, 256, 0, ws->stream>>>(n, w.t1, w.apow[nlev - 1], 1.0e-11, 1.0e-5, w.st);
     PW_HIP(hipGetLastError());
-    launched += nit;
-    PW_HIP(hipMemcpyAsync(&hs, w.st, sizeof(OrthoState), hipMemcpyDeviceToHost, ws->stream));
-    PW_HIP(hipStreamSynchronize(ws->stream));
-    if (hs.done) break;
-    batch = 4;
+    const double eps = 1.0e-8;
+    // LAMBDA ping-pongs between the caller's buffer (even number of updates) and lam2
+    double *lbuf[2] = {d_lam, w.lam2};
+    int launched = 0;
+    int batch = std::max(2, std::min(ws->ortho_iter > 0 ? ws->ortho_iter : 8, kMaxIter));
+    OrthoState hs{};
+    while (launched < kMaxIter) {
+      const int nit = std::min(batch, kMaxIter - launched);
+      for (int it = launched + 1; it <= launched + nit; ++it) {
+        const double *lam = lbuf[(it - 1) & 1];
+        double *lnew = lbuf[it & 1];
+        // HAUX = CHICHI LAMBDA + 2 CHIPSI ; T = LAMBDA^T HAUX
+        PW_TRY(gemm(ws, d_chichi, 0, lam, 0, w.t1, 1.0, d_chipsi, 2.0, w.st));
+        PW_TRY(gemm(ws, lam, 1, w.t1, 0, w.t2, 1.0, nullptr, 0.0, w.st));
+        k_gamma<<<nblk, 256, 0, ws->stream>>>(n, w.t2, d_psipsi, w.gam, w.red, w.st);
+        k_check<<<1, 256, 0, ws->stream>>>(nblk, w.red, eps, it, w.st);
+        // X = S^-1(GAMN): X0 = 2q M G M ; X <- X + A_j X A_j
+        PW_TRY(gemm(ws, w.gam, 0, mc, 0, w.t1, 1.0, nullptr, 0.0, w.st));
+        PW_TRY(gemm(ws, mc, 0, w.t1, 0, w.x0, 2.0 * q, nullptr, 0.0, w.st));
+        double *xc = w.x0, *xn = w.x1;
+        for (int j = 0; j < nlev; ++j) {
+          PW_TRY(gemm(ws, xc, 0, w.apow[j], 0, w.t1, 1.0, nullptr, 0.0, w.st));
+          PW_TRY(gemm(ws, w.apow[j], 0, w.t1, 0, xn, 1.0, xc, 1.0, w.st));
+          std::swap(xc, xn);
+        }
+        k_lam_step<<<nblk, 256, 0, ws->stream>>>(n, w.occ, xc, lam, lnew, w.st);
+      }
+      PW_HIP(hipGetLastError());
+      launched += nit;
+      PW_HIP(hipMemcpyAsync(&hs, w.st, sizeof(OrthoState), hipMemcpyDeviceToHost, ws->stream));
+      PW_HIP(hipStreamSynchronize(ws->stream));
+      if (hs.done) break;
+      batch = 4;
+    }
+    if (hs.code == 3) {
+      if (attempt == 0) continue;
+      ws->lz_valid = false;
+      return pw_fail(PW_ERR_NOCONV, "WAVES_ORTHO_X: spectral bounds of sym(CHIPSI) not accepted");
+    }
+    if (hs.code == 1) return pw_fail(PW_ERR_NOCONV, "WAVES_ORTHO_X: NaN in overlap");
+    if (hs.code == 2)
+      return pw_fail(PW_ERR_NOCONV, "WAVES_ORTHO_X: iteration loop for orthogonalization is about to diverge");
+    if (!hs.done)
+      return pw_fail(PW_ERR_NOCONV,
+                     "WAVES_ORTHO_X: loop for orthogonalization is not converged (specify SAFEORTHO=F, "
+                     "reduce the cutoff or the time step)");
+    // iteration hs.iter converged after hs.iter-1 updates
+    if ((hs.iter - 1) & 1) {
+      k_copy_d<<<nblk, 256, 0, ws->stream>>>(n2, w.lam2, d_lam);
+      PW_HIP(hipGetLastError());
+    }
+    *iter = hs.iter;
+    return PW_OK;
   }
-  if (hs.code == 1) return pw_fail(PW_ERR_NOCONV, "WAVES_ORTHO_X: NaN in overlap");
-  if (hs.code == 2)
-    return pw_fail(PW_ERR_NOCONV, "WAVES_ORTHO_X: iteration loop for orthogonalization is about to diverge");
-  if (!hs.done)
-    return pw_fail(PW_ERR_NOCONV,
-                   "WAVES_ORTHO_X: loop for orthogonalization is not converged (specify SAFEORTHO=F, "
-                   "reduce the cutoff or the time step)");
-  // iteration hs.iter converged after hs.iter-1 updates
-  if ((hs.iter - 1) & 1) {
-    k_copy_d<<<nblk, 256, 0, ws->stream>>>(n2, w.lam2, d_lam);
-    PW_HIP(hipGetLastError());
-  }
-  *iter = hs.iter;
-  return PW_OK;
+  return pw_fail(PW_ERR_NOCONV, "WAVES_ORTHO_X: spectral bounds of sym(CHIPSI) not accepted");
 }
 
 // ---------------------------------------------------------------------------

@@ -308,6 +308,15 @@ extern "C" int pw_waves_create(pw_plan *plan, int nb, int nat, int nsp, const in
   up(&ws->d_lmn_ln, ws->lmn_ln);
   up(&ws->d_lmn_lm, ws->lmn_lm);
   up(&ws->d_lmn_l, ws->lmn_l);
+  {
+    std::vector<int> tab;
+    for (int a = 0; a < nat; ++a)
+      for (int l = 0; l < ws->lmnx[ws->isp[a]]; ++l) {
+        tab.push_back(a);
+        tab.push_back(l);
+      }
+    up((int **)&ws->d_ipro_tab, tab);
+  }
   if (rc == PW_OK) rc = ylm_table(ws);
   if (rc != PW_OK) return fail(rc);
   if (hipStreamCreate(&ws->stream) != hipSuccess) return fail(pw_fail(PW_ERR_HIP, "hipStreamCreate failed"));
@@ -323,7 +332,7 @@ extern "C" int pw_waves_destroy(pw_waveset *ws) {
                   ws->d_xproj, ws->d_pro, ws->d_ylm, ws->d_mpsi, ws->d_v, ws->d_rho, ws->d_occ2,
                   ws->d_fpm, ws->d_fab, ws->d_domat, ws->d_dhmat, ws->d_atom_isp, ws->d_ipro0,
                   ws->d_ibpro, ws->d_lmnx, ws->d_lmn_ln, ws->d_lmn_lm, ws->d_lmn_l, ws->d_phase,
-                  ws->d_part, ws->d_small, ws->d_ortho, ws->d_small_mats};
+                  ws->d_part, ws->d_small, ws->d_ortho, ws->d_small_mats, ws->d_ipro_tab};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (ws->own_stream && ws->stream) (void)hipStreamDestroy(ws->stream);
@@ -377,6 +386,13 @@ extern "C" int pw_waves_setr8a(pw_waveset *ws, const char *id, int len, const do
     PW_TRY(chk((size_t)3 * ws->nat));
     std::vector<double> r(val, val + len);
     if (s == "R0") ws->r0 = r; else if (s == "RP") ws->rp = r; else ws->rm = r;
+  } else if (s == "ORTHOBOUNDS") {
+    // spectral bounds of sym(CHIPSI) for the next ORTHO_X call; ORTHO_X checks
+    // them on the device and recomputes them when they are not accepted
+    PW_TRY(chk(2));
+    ws->lz_lo = val[0];
+    ws->lz_hi = val[1];
+    ws->lz_valid = true;
   } else {
     return pw_fail(PW_ERR_ARG, "WAVES$SETR8A: ID not recognised: " + s);
   }
@@ -514,6 +530,7 @@ extern "C" int pw_waves_geti4(pw_waveset *ws, const char *id, int *val) {
   else if (s == "LMNXX") *val = ws->lmnxx;
   else if (s == "ORTHOLEVELS") *val = ws->ortho_levels;
   else if (s == "ORTHONS") *val = ws->ortho_ns;
+  else if (s == "ORTHOLANCZOS") *val = ws->ortho_lanczos;
   else return pw_fail(PW_ERR_ARG, "WAVES$GETI4: ID not recognised: " + s);
   return PW_OK;
 }

@@ -104,7 +104,10 @@ int pw_waves_destroy(pw_waveset *ws);
 int pw_waves_setr8(pw_waveset *ws, const char *id, double val);
 /* Real arrays: "PRO" (NGL*NBAREPRO, GSET%PRO, already / cell volume),
  * "OCC" (NB), "DO" (LNXX*LNXX*NSP), "VOFR" (NRL, local potential),
- * "R0" / "RP" / "RM" (3*NAT atomic positions R(0), R(+), R(-)). */
+ * "R0" / "RP" / "RM" (3*NAT atomic positions R(0), R(+), R(-)),
+ * "ORTHOBOUNDS" (2: spectral bounds of sym(CHIPSI) for the next
+ * orthogonalisation; checked on the device and recomputed if rejected --
+ * no counterpart in the reference, whose DSYEV needs none). */
 int pw_waves_setr8a(pw_waveset *ws, const char *id, int len, const double *val);
 /* Complex arrays (interleaved re,im): "DH" (LMNXX*LMNXX*NAT),
  * "PSI0" / "PSIM" (NGL*NBH), "RLAM0" / "RLAMM" (NB*NB). */
@@ -120,7 +123,9 @@ int pw_waves_getr8a(pw_waveset *ws, const char *id, int len, double *val);
 int pw_waves_getr8(pw_waveset *ws, const char *id, double *val);
 /* "ORTHOITER" (iterations of the last WAVES_ORTHO_X), "NPRO", "NBH",
  * "LMNXX", "ORTHOLEVELS" (squared-Smith levels of the last solve), "ORTHONS"
- * (Newton-Schulz steps of the last solve). */
+ * (Newton-Schulz steps of the last solve), "ORTHOLANCZOS" (Lanczos runs for
+ * the spectral bounds in the last solve: 0 when the previous bounds were
+ * accepted). */
 int pw_waves_geti4(pw_waveset *ws, const char *id, int *val);
 /* Device pointer of a resident array ("PSI0","PSIM","HPSI","PROJ","VOFR",
  * "RHO") for zero-copy access by the caller. */

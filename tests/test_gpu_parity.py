@@ -114,6 +114,31 @@ def test_cp_steps(name, nsteps):
     assert relerr(w.getc8a("PSI0"), o.psi0) < 1e-9
 
 
+@pytest.mark.parametrize("bounds", [(10.0, 20.0), (0.01, 0.02)])
+def test_ortho_bounds_rejected(bounds):
+    # spectral bounds far above (Newton-Schulz too slow) or far below (it
+    # diverges) the spectrum of sym(CHIPSI) are rejected by the device checks
+    # and recomputed; the constraint solution stays that of the oracle
+    s = SYSTEMS["si8"]()
+    say(f"case ortho bounds {bounds}")
+    o = oracle_waveset(s)
+    plane, w = gpu_waveset(s)
+    for it in range(3):
+        o.etot(s.pos)
+        o.propagate()
+        o.orthogonalize(s.pos, s.pos)
+        w.etot()
+        w.propagate()
+        if it == 1:
+            w.setr8a("ORTHOBOUNDS", np.array(bounds))
+        w.orthogonalize()
+        assert w.geti4("ORTHOLANCZOS") == (0 if it == 2 else 1), it
+        assert w.geti4("ORTHOITER") == o.ortho_iter, it
+        assert relerr(w.getc8a("RLAM0").real, o.rlam0) < 1e-9, it
+        o.switch()
+        w.switch()
+
+
 def test_ortho_x_standalone():
     import pwgpu
     s = SYSTEMS["si8"]()
