@@ -8,7 +8,9 @@ WAVES$PROPAGATE, WAVES$ORTHOGONALIZE, WAVES$SWITCH) of the Si64 diamond
 supercell (a=20.52 bohr, Gamma point, 128 occupied + 16 empty bands, 30 Ry
 wave cutoff, 72^3 grid, 512 projectors) with all state resident in HBM.
 Inputs are synthetic (seeded displacements, seeded initial coefficients,
-model potential and one-centre matrices).  Prints one JSON line.
+model potential and one-centre matrices).  Prints one JSON line whose value
+is the mixed form's steps/s; the plain FP64 form is timed in the same run and
+the error of the mixed form against it is measured step by step.
 """
 import argparse
 import json
@@ -24,7 +26,6 @@ for p in (ROOT, os.path.join(ROOT, "cp-paw_amd")):
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 FP32_PEAK_TFLOPS = 157.3
-BF16_PEAK_TFLOPS = 2500.0
 
 
 def log(msg):
@@ -49,6 +50,43 @@ def kernel_counts(system, plane, nb):
         "density": ("byte", nbh * nr * (16.0 + 2 * 8.0)),
         "propagate": ("byte", 4.0 * nbh * ngl * 16.0),
     }
+
+
+# peak of each counted kernel's arithmetic in each form: the projector and
+# overlap GEMMs run on FP64 MFMA in both forms; the others are HBM-bound
+GEMM_KERNELS = ("projections", "addpro", "force", "overlap", "addopsi")
+
+
+def kernel_peak(name, prec):
+    if name in GEMM_KERNELS:
+        return FP64_PEAK_TFLOPS, "mfma"
+    return HBM_PEAK_GBS, "hbm"
+
+
+def accuracy(system_name, nsteps):
+    """Per-step error of the mixed form against the FP64 form from the same
+    start: max |dE| per atom (Ha) and max |dF| (Ha/bohr) over nsteps steps."""
+    import numpy as np
+    import pwgpu
+    from pwgpu import model
+    s = model.SYSTEMS[system_name]()
+    plane = pwgpu.PlaneWave(s.rbas, s.epw_psi, s.epw_rho)
+    ws = [pwgpu.Waves.from_system(plane, s, prec=p) for p in (0, 1)]
+    de = df = 0.0
+    for _ in range(nsteps):
+        e = [w.etot()["etot"] for w in ws]
+        f = [w.getr8a("FORCE") for w in ws]
+        de = max(de, abs(e[1] - e[0]) / s.nat)
+        df = max(df, float(np.max(np.abs(f[1] - f[0]))))
+        for w in ws:
+            w.propagate()
+            w.orthogonalize()
+            w.switch()
+    for w in ws:
+        w.close()
+    plane.close()
+    return {"steps": nsteps, "max_dE_per_atom_Ha": de, "max_dF_Ha_per_bohr": df,
+            "gates": {"dE_per_atom": 1e-6, "dF": 1e-4}, "pass": bool(de < 1e-6 and df < 1e-4)}
 
 
 def run_gpu(system_name, prec, steps, warmup, profile):
@@ -92,7 +130,8 @@ def run_gpu(system_name, prec, steps, warmup, profile):
 
 
 def cpu_baseline(system_name, nsteps=2):
-    """The FP64 oracle (numpy) on the host: one untimed step, then nsteps timed."""
+    """The FP64 oracle (numpy) on the host: one untimed step, then nsteps timed;
+    the rate is that of the median step."""
     import numpy as np
     from pwgpu import model
     from oracle import planewave as opw
@@ -117,14 +156,29 @@ def cpu_baseline(system_name, nsteps=2):
         ws.switch()
 
     step()
-    t0 = time.perf_counter()
+    times = []
     for _ in range(nsteps):
+        t0 = time.perf_counter()
         step()
-    dt = time.perf_counter() - t0
-    del np
-    return {"value": nsteps / dt, "unit": "CP steps/s", "cores": int(threads), "kind": "port",
-            "sample": f"{nsteps} CP steps of {s.name} (FP64 numpy oracle, one band per FFT as the "
-                      f"reference) after one untimed step"}
+        times.append(time.perf_counter() - t0)
+    med = float(np.median(times))
+    return {"value": 1.0 / med, "unit": "CP steps/s", "cores": int(threads), "kind": "port",
+            "sample": f"median of {nsteps} CP steps of {s.name} (FP64 numpy oracle, one band per "
+                      f"FFT as the reference) after one untimed step"}
+
+
+def profiled_traffic(kernel, system, prec):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary
+    (profiles/traffic.json: FETCH_SIZE + WRITE_SIZE per dispatch, scaled by the
+    copy-kernel calibration), or None when that kernel/workload was not measured."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            tab = json.load(f)
+    except (OSError, ValueError):
+        return None
+    v = tab.get(f"{system}/{'fp64' if prec == 0 else 'mixed'}/{kernel}")
+    return None if v is None else v.get("bytes_per_launch")
 
 
 def main():
@@ -133,11 +187,26 @@ def main():
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--system", default="si64")
-    ap.add_argument("--prec", default="fp64", choices=["fp64", "mixed", "both"])
+    ap.add_argument("--prec", default="both", choices=["fp64", "mixed", "both"])
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--acc-steps", type=int, default=20,
+                    help="steps of the mixed-vs-FP64 error measurement (0: skip)")
     ap.add_argument("--profile", action="store_true", help="print per-kernel times to stderr")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy of MIB MiB (calibrates the HBM "
+                         "byte counters of a profiler run) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration * (1 << 20)
+        a = torch.ones(n, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": n, "launches": 3}))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
                           "(the multi-GPU decomposition is a design in DESIGN.md only)"}))
@@ -164,15 +233,17 @@ def main():
     dom = max(cand)[1]
     kind, work = counts[dom]
     avg_s = kt[dom]["ms_avg"] * 1e-3
+    peak, bound = kernel_peak(dom, head)
     if kind == "flop":
-        peak = FP64_PEAK_TFLOPS if head == 0 else BF16_PEAK_TFLOPS
         achieved = work / avg_s / 1e12
-        roof = {"kernel": dom, "bound": "mfma", "achieved": achieved, "peak": peak,
-                "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None}
+        unit = "TFLOP/s"
     else:
         achieved = work / avg_s / 1e9
-        roof = {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None}
+        unit = "GB/s"
+    roof = {"kernel": dom, "bound": bound, "achieved": achieved, "peak": peak, "unit": unit,
+            "frac": achieved / peak, "traffic": profiled_traffic(dom, args.system, head),
+            "work_per_launch": work, "avg_us": avg_s * 1e6}
+    acc = accuracy(args.system, args.acc_steps) if (args.acc_steps > 0 and head == 1) else None
     cpu = None
     if not args.no_cpu:
         log("cpu baseline (oracle) ...")
@@ -197,6 +268,8 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
     }
+    if acc is not None:
+        out["mixed_error_vs_fp64"] = acc
     if 0 in results and 1 in results:
         out["fp64_steps_per_s"] = results[0]["steps_per_s"]
         out["mixed_over_fp64"] = results[1]["steps_per_s"] / results[0]["steps_per_s"]

@@ -167,14 +167,42 @@ __global__ void __launch_bounds__(64 * NW) k_rowgemm(Gen gen, int nrow, int ngl,
   }
 }
 
-// sum the split partials: out[i] = scale * sum_s part[s][i]
-__global__ void k_reduce_c(const double2 *part, int nsplit, int n, double scale, double2 *out) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  double2 s = czero();
-  for (int k = 0; k < nsplit; ++k) s = cadd(s, part[(size_t)k * n + i]);
-  out[i] = make_double2(s.x * scale, s.y * scale);
+// sum of split partials: out[i] = scale * sum_s part[s][i] in a fixed order.
+// A block takes 32 outputs; its 8 lanes per output stride over the splits
+// with four accumulators each and are combined through LDS.
+constexpr int kRedC = 32, kRedS = 8;
+
+__device__ inline double rz(double) { return 0.0; }
+__device__ inline double2 rz(double2) { return make_double2(0.0, 0.0); }
+__device__ inline double radd(double a, double b) { return a + b; }
+__device__ inline double2 radd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ inline double rscale(double a, double s) { return a * s; }
+__device__ inline double2 rscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
+
+template <typename T>
+__global__ void __launch_bounds__(kRedC * kRedS) k_reduce_split(const T *__restrict__ part, int ns, int n,
+                                                                double scale, T *__restrict__ out) {
+  __shared__ T red[kRedS][kRedC];
+  const int c = threadIdx.x % kRedC, sl = threadIdx.x / kRedC;
+  const int i = blockIdx.x * kRedC + c;
+  T acc[4] = {rz(T()), rz(T()), rz(T()), rz(T())};
+  if (i < n) {
+    int k = sl;
+    for (; k + 3 * kRedS < ns; k += 4 * kRedS) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) acc[u] = radd(acc[u], part[(size_t)(k + u * kRedS) * n + i]);
+    }
+    for (int u = 0; k < ns; k += kRedS, ++u) acc[u] = radd(acc[u], part[(size_t)k * n + i]);
+  }
+  red[sl][c] = radd(radd(acc[0], acc[1]), radd(acc[2], acc[3]));
+  __syncthreads();
+  if (sl != 0 || i >= n) return;
+  T r = red[0][c];
+#pragma unroll
+  for (int q = 1; q < kRedS; ++q) r = radd(r, red[q][c]);
+  out[i] = rscale(r, scale);
 }
+
 
 // ---------------------------------------------------------------------------
 // psi[b][G] += sum_k A(b,k) B(k,G): the band-major output is C^T, so a lane's
@@ -340,13 +368,6 @@ __global__ void __launch_bounds__(64 * NW) k_force_m(ProjGeo q, const int2 *ipta
   }
 }
 
-__global__ void k_reduce_r(const double *part, int nsplit, int n, double scale, double *out) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  double s = 0.0;
-  for (int k = 0; k < nsplit; ++k) s += part[(size_t)k * n + i];
-  out[i] = s * scale;
-}
 
 // unravel super-wave products into the NB x NB real matrix (row-major),
 // WAVES_OVERLAP (src/paw_waves2.f90:805-848); t, tm already times Omega
@@ -670,7 +691,8 @@ int run_rowgemm(pw_waveset *ws, const Gen &gen, int nrow, const double2 *psi, do
   }
   {
     ScopeTimer tm(ws, "reduce");
-    k_reduce_c<<<(n + 255) / 256, 256, 0, ws->stream>>>(part, ns, (int)n, scale, out);
+    k_reduce_split<double2><<<(n + kRedC - 1) / kRedC, kRedC * kRedS, 0, ws->stream>>>(part, ns, (int)n, scale,
+                                                                                         out);
     PW_HIP(hipGetLastError());
   }
   if (res) *res = out;
@@ -767,7 +789,8 @@ int k_force(pw_waveset *ws, const double2 *psi, const double2 *phase, const doub
                                                          ws->plan->d_gvec, ws->d_part);
     PW_HIP(hipGetLastError());
   }
-  k_reduce_r<<<(n + 255) / 256, 256, 0, ws->stream>>>(ws->d_part, ngt, (int)n, 2.0 * ws->plan->vol,
+  k_reduce_split<double><<<(n + kRedC - 1) / kRedC, kRedC * kRedS, 0, ws->stream>>>(ws->d_part, ngt, (int)n,
+                                                                                      2.0 * ws->plan->vol,
                                                       ws->d_small);
   PW_HIP(hipGetLastError());
   std::vector<double> fp(n);

@@ -312,26 +312,49 @@ __global__ void k_check(int nblk, const double *bmax, double tol, int it, OrthoS
 // Acceptance test of the spectral bounds (before the outer loop):
 // R = (B+qI)M - I must vanish and the last Smith power A^(2^J) must be small
 // enough that the truncated sum is exact to rounding; otherwise code 3 makes
-// the host recompute the bounds.
-__global__ void k_prep_check(int n, const double *T, const double *AJ, double rtol, double atol,
-                             OrthoState *st) {
+// the host recompute the bounds.  Block maxima first (NaN propagates), then
+// one block decides.
+__device__ inline double nanmax(double a, double b) { return (a != a || b != b) ? NAN : fmax(a, b); }
+
+__global__ void k_prep_part(int n, const double *T, const double *AJ, double *bmax) {
   __shared__ double r1[256], r2[256];
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   double m1 = 0.0, m2 = 0.0;
-  for (int idx = threadIdx.x; idx < n * n; idx += blockDim.x) {
-    const double d = T[idx] - ((idx / n == idx % n) ? 1.0 : 0.0);
-    const double a = AJ[idx];
-    m1 = (d != d || m1 != m1) ? NAN : fmax(m1, fabs(d));
-    m2 = (a != a || m2 != m2) ? NAN : fmax(m2, fabs(a));
+  if (idx < n * n) {
+    m1 = fabs(T[idx] - ((idx / n == idx % n) ? 1.0 : 0.0));
+    m2 = fabs(AJ[idx]);
   }
   r1[threadIdx.x] = m1;
   r2[threadIdx.x] = m2;
   __syncthreads();
   for (int s = 128; s > 0; s >>= 1) {
     if ((int)threadIdx.x < s) {
-      const double a = r1[threadIdx.x], b = r1[threadIdx.x + s];
-      r1[threadIdx.x] = (a != a || b != b) ? NAN : fmax(a, b);
-      const double c = r2[threadIdx.x], e = r2[threadIdx.x + s];
-      r2[threadIdx.x] = (c != c || e != e) ? NAN : fmax(c, e);
+      r1[threadIdx.x] = nanmax(r1[threadIdx.x], r1[threadIdx.x + s]);
+      r2[threadIdx.x] = nanmax(r2[threadIdx.x], r2[threadIdx.x + s]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    bmax[2 * blockIdx.x] = r1[0];
+    bmax[2 * blockIdx.x + 1] = r2[0];
+  }
+}
+
+__global__ void k_prep_check(int n, int nblk, const double *bmax, double rtol, double atol,
+                             OrthoState *st) {
+  __shared__ double r1[256], r2[256];
+  double m1 = 0.0, m2 = 0.0;
+  for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
+    m1 = nanmax(m1, bmax[2 * b]);
+    m2 = nanmax(m2, bmax[2 * b + 1]);
+  }
+  r1[threadIdx.x] = m1;
+  r2[threadIdx.x] = m2;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      r1[threadIdx.x] = nanmax(r1[threadIdx.x], r1[threadIdx.x + s]);
+      r2[threadIdx.x] = nanmax(r2[threadIdx.x], r2[threadIdx.x + s]);
     }
     __syncthreads();
   }
@@ -396,7 +419,7 @@ struct OrthoWork {
 int ortho_work(pw_waveset *ws, OrthoWork &w) {
   const size_t n2 = (size_t)ws->nb * ws->nb;
   const size_t nblk = (n2 + 255) / 256;
-  const size_t need = (9 + kMaxLevels) * n2 + ws->nb + nblk + 64;
+  const size_t need = (9 + kMaxLevels) * n2 + ws->nb + 2 * nblk + 64;
   if (need > ws->ortho_cap) {
     if (ws->d_ortho) PW_HIP(hipFree(ws->d_ortho));
     PW_HIP(hipMalloc(&ws->d_ortho, need * sizeof(double)));
@@ -504,7 +527,8 @@ int ortho_x_device(pw_waveset *ws, double *d_lam, const double *d_chichi, const 
     k_cayley<<<nblk, 256, 0, ws->stream>>>(n, q, mc, w.apow[0]);
     for (int j = 1; j < nlev; ++j) PW_TRY(gemm(ws, w.apow[j - 1], 0, w.apow[j - 1], 0, w.apow[j]));
     PW_TRY(gemm(ws, w.bs, 0, mc, 0, w.t1, 1.0, mc, q));         // (B+qI) M
-    k_prep_check<<<1, 256, 0, ws->stream>>>(n, w.t1, w.apow[nlev - 1], 1.0e-11, 1.0e-5, w.st);
+    k_prep_part<<<nblk, 256, 0, ws->stream>>>(n, w.t1, w.apow[nlev - 1], w.red);
+    k_prep_check<<<1, 256, 0, ws->stream>>>(n, nblk, w.red, 1.0e-11, 1.0e-5, w.st);
     PW_HIP(hipGetLastError());
     const double eps = 1.0e-8;
     // LAMBDA ping-pongs between the caller's buffer (even number of updates) and lam2

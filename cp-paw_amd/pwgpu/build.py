@@ -1,4 +1,6 @@
-"""Build the pwgpu HIP library in-tree (libpwgpu.so next to this file).
+"""Build the pwgpu HIP library in-tree (libpwgpu.so next to this file) and the
+Fortran host layer (cp-paw_amd/fortran: ISO_C_BINDING module, reference-style
+wrappers and the pawgpu_cpstep driver) against it.
 
 hipcc cross-compiles for gfx950 without a GPU.  Packed FP32 instructions are
 disabled (-packed-fp32-ops) for every HIP translation unit.
@@ -23,6 +25,46 @@ def hipcc():
         if cand and (os.path.isabs(cand) and os.path.exists(cand) or not os.path.isabs(cand)):
             return cand
     return "hipcc"
+
+
+FDIR = os.path.join(os.path.dirname(HERE), "fortran")
+FSOURCES = ["pwgpu_c.f90", "pawgpu.f90", "pawgpu_cpstep.f90"]
+FDRIVER = os.path.join(FDIR, "pawgpu_cpstep")
+
+
+def flang():
+    for cand in (os.environ.get("FLANG"), "/opt/rocm/lib/llvm/bin/flang", "/opt/rocm/llvm/bin/flang"):
+        if cand and os.path.exists(cand):
+            return cand
+    return None
+
+
+def build_fortran(verbose=True):
+    """Compile the Fortran layer and link the driver against libpwgpu.so
+    (rpath $ORIGIN/../pwgpu).  Returns the driver path, or None when no
+    Fortran compiler is installed."""
+    fc = flang()
+    if fc is None:
+        if verbose:
+            sys.stderr.write("no Fortran compiler found; Fortran host layer not built\n")
+        return None
+    bdir = os.path.join(FDIR, "_build")
+    os.makedirs(bdir, exist_ok=True)
+    objs = []
+    for src in FSOURCES:
+        obj = os.path.join(bdir, os.path.splitext(src)[0] + ".o")
+        cmd = [fc, "-O2", "-c", os.path.join(FDIR, src), "-o", obj, "-module-dir", bdir, "-I", bdir]
+        r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
+        if r.returncode != 0:
+            raise RuntimeError(f"flang failed on {src}:\n" + r.stdout.decode(errors="replace")[-6000:])
+        objs.append(obj)
+    tmp = FDRIVER + ".tmp"
+    cmd = [fc, "-o", tmp] + objs + ["-L", HERE, "-lpwgpu", "-Wl,-rpath,$ORIGIN/../pwgpu"]
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
+    if r.returncode != 0:
+        raise RuntimeError("Fortran link failed:\n" + r.stdout.decode(errors="replace")[-6000:])
+    os.replace(tmp, FDRIVER)
+    return FDRIVER
 
 
 def build(verbose=True, jobs=None):
@@ -54,6 +96,7 @@ def build(verbose=True, jobs=None):
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stdout.decode(errors="replace")[-6000:])
     os.replace(tmp, LIB)
+    build_fortran(verbose)
     return LIB
 
 

@@ -106,22 +106,28 @@ class Waves:
         return p
 
     # -- inputs in reference layouts ------------------------------------------
-    def set_dh(self, dh_list):
+    def dh_array(self, dh_list):
         """DH(LMNXX,LMNXX,NAT) from a list of per-atom (LMNX,LMNX) matrices."""
         L = self.lmnxx
         a = np.zeros((self.nat, L, L), dtype=np.complex128)
         for i, d in enumerate(dh_list):
             n = d.shape[0]
             a[i, :n, :n] = np.asarray(d).T          # [atom, j, i] = DH(i, j, atom)
-        self.setc8a("DH", a)
+        return a
 
-    def set_do(self, do_list):
+    def do_array(self, do_list):
         """DO(LNXX,LNXX,NSP) from a list of per-species (LNX,LNX) matrices."""
         a = np.zeros((self.nsp, self.lnxx, self.lnxx))
         for s, d in enumerate(do_list):
             n = d.shape[0]
             a[s, :n, :n] = np.asarray(d).T
-        self.setr8a("DO", a)
+        return a
+
+    def set_dh(self, dh_list):
+        self.setc8a("DH", self.dh_array(dh_list))
+
+    def set_do(self, do_list):
+        self.setr8a("DO", self.do_array(do_list))
 
     def set_matrix_c(self, key, m):
         self.setc8a(key, np.asarray(m, dtype=np.complex128).T)
@@ -159,26 +165,33 @@ class Waves:
         return ms.value, n.value
 
     # -- set-up from a model system ----------------------------------------------
+    def system_inputs(self, system, psi=None):
+        """The keyword inputs of the step for a pwgpu.model.System, in call order:
+        [(setter, ID, value)] with setter "r8", "r8a" or "c8a" and values in the
+        reference layouts (PSIM = PSI0, R0 = RP = RM)."""
+        g2 = self.plane.getr8a("G2")
+        c = system.initial_psi(g2) if psi is None else psi
+        return [("r8a", "PRO", system.pro_table(g2)),
+                ("r8a", "DO", self.do_array(system.do_list())),
+                ("c8a", "DH", self.dh_array(system.dh_matrices())),
+                ("r8a", "OCC", np.asarray(system.occ, dtype=np.float64)),
+                ("r8a", "VOFR", system.potential(self.plane.nr)),
+                ("r8", "TIMESTEP", system.dt),
+                ("r8", "EMASS", system.emass),
+                ("r8", "EMASSCG2", system.emasscg2),
+                ("r8", "FRICTION", system.annee),
+                ("r8a", "R0", system.pos),
+                ("r8a", "RM", system.pos),
+                ("r8a", "RP", system.pos),
+                ("c8a", "PSI0", c),
+                ("c8a", "PSIM", c)]
+
     @classmethod
     def from_system(cls, plane, system, prec=FP64, psi=None):
         """Waves for a pwgpu.model.System: all inputs of the step set, PSIM=PSI0."""
         w = cls(plane, system.nb, system.isp, system.lox_list, prec=prec)
-        g2 = plane.getr8a("G2")
-        w.setr8a("PRO", system.pro_table(g2))
-        w.set_do(system.do_list())
-        w.set_dh(system.dh_matrices())
-        w.setr8a("OCC", system.occ)
-        w.setr8a("VOFR", system.potential(plane.nr))
-        w.setr8("TIMESTEP", system.dt)
-        w.setr8("EMASS", system.emass)
-        w.setr8("EMASSCG2", system.emasscg2)
-        w.setr8("FRICTION", system.annee)
-        w.setr8a("R0", system.pos)
-        w.setr8a("RM", system.pos)
-        w.setr8a("RP", system.pos)
-        c = system.initial_psi(g2) if psi is None else psi
-        w.setc8a("PSI0", c)
-        w.setc8a("PSIM", c)
+        for kind, key, val in w.system_inputs(system, psi):
+            getattr(w, "set" + kind)(key, val)
         return w
 
 
