@@ -469,6 +469,338 @@ struct AddopsiB {
   }
 };
 
+// ---------------------------------------------------------------------------
+// Split-BF16x3 MFMA building blocks (mixed form).  A value x is carried as
+// three bf16 pieces x = h + m + l (h = bf16(x), m = bf16(x - h),
+// l = bf16(x - h - m): 24 significant bits, the precision of FP32), and a
+// product a*b as the six piece products of order >= 2^-16,
+//   ah bh + ah bm + am bh + ah bl + am bm + al bh,
+// on v_mfma_f32_16x16x32_bf16 with FP32 accumulation.  Lane layout (checked
+// on the device): A(i,k) in lane i + 16(k/8), element k%8; B(k,j) in lane
+// j + 16(k/8), element k%8; C(i,j) in lane j + 16(i/4), element i%4.
+// Operands live in LDS as six planes (re h,m,l, im h,m,l), K contiguous per
+// row/column, rows padded to 80 B so that the 16-byte operand reads of 16
+// lanes cover all 64 banks once.
+// ---------------------------------------------------------------------------
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float flt4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int KB = 32;        // K of one MFMA = K of one staged chunk
+constexpr int KBP = KB + 8;   // LDS row stride in bf16
+constexpr int NPL = 6;        // planes per complex operand
+
+__device__ inline flt4 mfma_bf(const bf16x8 &a, const bf16x8 &b, flt4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+__device__ inline bf16x8 bf_neg(const bf16x8 &a) {
+  return __builtin_bit_cast(bf16x8, __builtin_bit_cast(u32x4, a) ^ 0x80008000u);
+}
+
+// split planes of 8 consecutive complex values: p[0..2] re h,m,l, p[3..5] im h,m,l
+struct Planes8 {
+  bf16x8 p[NPL];
+};
+
+struct Split3 {
+  __bf16 h, m, l;
+};
+
+__device__ inline Split3 split3(double x) {
+  Split3 s;
+  const float f = (float)x;
+  s.h = (__bf16)f;
+  const float r = f - (float)s.h;
+  s.m = (__bf16)r;
+  s.l = (__bf16)(r - (float)s.m);
+  return s;
+}
+
+__device__ inline void put(Planes8 &o, int e, double2 v) {
+  const Split3 a = split3(v.x), b = split3(v.y);
+  o.p[0][e] = a.h;
+  o.p[1][e] = a.m;
+  o.p[2][e] = a.l;
+  o.p[3][e] = b.h;
+  o.p[4][e] = b.m;
+  o.p[5][e] = b.l;
+}
+
+__device__ inline void zero8(Planes8 &o) {
+  const bf16x8 z = __builtin_bit_cast(bf16x8, u32x4{0u, 0u, 0u, 0u});
+#pragma unroll
+  for (int q = 0; q < NPL; ++q) o.p[q] = z;
+}
+
+// real product x*y of split operands (three pieces each) accumulated in c
+__device__ inline flt4 rmfma3(const bf16x8 *x, const bf16x8 *y, flt4 c) {
+  c = mfma_bf(x[2], y[0], c);
+  c = mfma_bf(x[1], y[1], c);
+  c = mfma_bf(x[0], y[2], c);
+  c = mfma_bf(x[1], y[0], c);
+  c = mfma_bf(x[0], y[1], c);
+  c = mfma_bf(x[0], y[0], c);
+  return c;
+}
+
+// (cr + i ci) += a b, a and b split; an = the negated im pieces of a
+__device__ inline void cmfma_bf(flt4 &cr, flt4 &ci, const bf16x8 *a, const bf16x8 *an, const bf16x8 *b) {
+  cr = rmfma3(a, b, cr);
+  cr = rmfma3(an, b + 3, cr);
+  ci = rmfma3(a, b + 3, ci);
+  ci = rmfma3(a + 3, b, ci);
+}
+
+__device__ inline void store_planes(__bf16 (*pl)[KBP], int plane_rows, int row, int k8, const Planes8 &v) {
+#pragma unroll
+  for (int q = 0; q < NPL; ++q) *reinterpret_cast<bf16x8 *>(&pl[q * plane_rows + row][8 * k8]) = v.p[q];
+}
+
+__device__ inline void load_planes(const __bf16 (*pl)[KBP], int plane_rows, int row, int k8, bf16x8 *v) {
+#pragma unroll
+  for (int q = 0; q < NPL; ++q) v[q] = *reinterpret_cast<const bf16x8 *>(&pl[q * plane_rows + row][8 * k8]);
+}
+
+// Projector matrix in split form, PS[ipro][G] = 16 B per element holding the
+// six bf16 pieces (re h,m,l, im h,m,l) and two zero halves; G padded to a
+// multiple of 64 with zeros (WAVES_EXPANDPRO, src/paw_waves1.f90:6070-6109,
+// evaluated in FP64 then split).
+__device__ inline unsigned pack2(__bf16 a, __bf16 b) {
+  return (unsigned)__builtin_bit_cast(unsigned short, a) | ((unsigned)__builtin_bit_cast(unsigned short, b) << 16);
+}
+
+__global__ void k_pgen_bf(ProjGeo q, const int2 *iptab, int npro, int nglp, uint4 *ps) {
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (size_t)npro * nglp) return;
+  const int ip = (int)(idx / nglp), ig = (int)(idx % nglp);
+  double2 v = czero();
+  if (ig < q.ngl) {
+    const int2 al = iptab[ip];
+    v = pro_val(q, al.x, al.y, ig, eigr_of(q, al.x, ig));
+  }
+  const Split3 a = split3(v.x), b = split3(v.y);
+  ps[idx] = make_uint4(pack2(a.h, a.m), pack2(a.l, b.h), pack2(b.m, b.l), 0u);
+}
+
+__device__ inline void put_bits(Planes8 &o, int e, uint4 u) {
+  o.p[0][e] = __builtin_bit_cast(__bf16, (unsigned short)(u.x & 0xffffu));
+  o.p[1][e] = __builtin_bit_cast(__bf16, (unsigned short)(u.x >> 16));
+  o.p[2][e] = __builtin_bit_cast(__bf16, (unsigned short)(u.y & 0xffffu));
+  o.p[3][e] = __builtin_bit_cast(__bf16, (unsigned short)(u.y >> 16));
+  o.p[4][e] = __builtin_bit_cast(__bf16, (unsigned short)(u.z & 0xffffu));
+  o.p[5][e] = __builtin_bit_cast(__bf16, (unsigned short)(u.z >> 16));
+}
+
+// ---- row operands (8 consecutive G of one row) -------------------------------
+// conj(P(row, G)) from the split projector matrix
+struct RowP8 {
+  const uint4 *ps;
+  int nglp;
+  __device__ void operator()(int row, int g0, int, Planes8 &o) const {
+    const uint4 *src = ps + (size_t)row * nglp + g0;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) put_bits(o, e, src[e]);
+#pragma unroll
+    for (int q = 3; q < NPL; ++q) o.p[q] = bf_neg(o.p[q]);
+  }
+};
+
+// stacked [conj(psi1_i(G)); psi1_i(-G)] rows of the band overlaps
+struct RowOvl {
+  const double2 *psi1;
+  const int *minusg;
+  int ngl, nbh, nbhp;
+  __device__ void operator()(int r, int g0, int, Planes8 &o) const {
+    const bool tm = r >= nbhp;
+    const int b = tm ? r - nbhp : r;
+    if (b >= nbh) { zero8(o); return; }
+    const double2 *src = psi1 + (size_t)b * ngl;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int ig = g0 + e;
+      double2 v = czero();
+      if (ig < ngl) v = tm ? src[minusg[ig]] : cconj(src[ig]);
+      put(o, e, v);
+    }
+  }
+};
+
+// psi[b][G] as the second operand (8 consecutive G of band b)
+__device__ inline void psi_run(const double2 *psi, int ngl, int nbh, int b, int g0, Planes8 &o) {
+  if (b >= nbh) { zero8(o); return; }
+  const double2 *src = psi + (size_t)b * ngl;
+#pragma unroll
+  for (int e = 0; e < 8; ++e) put(o, e, g0 + e < ngl ? src[g0 + e] : czero());
+}
+
+// part[split][r][b] = sum_{G in split} A(r,G) psi[b][G] (split-BF16x3)
+template <class Gen, int NCB, int NW>
+__global__ void __launch_bounds__(64 * NW) k_rowgemm_bf(Gen gen, int nrow, int ngl, int nbh,
+                                                        const double2 *__restrict__ psi, int gsplit,
+                                                        double2 *__restrict__ part) {
+  constexpr int WR = 16 * NW, NB16 = 16 * NCB;
+  __shared__ __bf16 sA[NPL * WR][KBP];
+  __shared__ __bf16 sB[NPL * NB16][KBP];
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6, li = lane & 15, lk = lane >> 4;
+  const int r0 = blockIdx.x * WR, b0 = blockIdx.y * NB16;
+  const int g0 = blockIdx.z * gsplit, g1 = min(ngl, g0 + gsplit);
+  flt4 accr[NCB], acci[NCB];
+#pragma unroll
+  for (int c = 0; c < NCB; ++c) {
+    accr[c] = flt4{0.f, 0.f, 0.f, 0.f};
+    acci[c] = flt4{0.f, 0.f, 0.f, 0.f};
+  }
+  for (int gc = g0; gc < g1; gc += KB) {
+    {
+      const int r = t >> 2, k8 = t & 3;      // WR rows x 4 runs = 64 NW runs, one per thread
+      Planes8 v;
+      if (r0 + r < nrow) gen(r0 + r, gc + 8 * k8, g1, v);
+      else zero8(v);
+      store_planes(sA, WR, r, k8, v);
+    }
+    for (int e = t; e < NB16 * 4; e += 64 * NW) {
+      const int b = e >> 2, k8 = e & 3;
+      Planes8 v;
+      psi_run(psi, ngl, nbh, b0 + b, gc + 8 * k8, v);
+      store_planes(sB, NB16, b, k8, v);
+    }
+    __syncthreads();
+    bf16x8 a[NPL], an[3];
+    load_planes(sA, WR, 16 * wv + li, lk, a);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) an[q] = bf_neg(a[3 + q]);
+#pragma unroll
+    for (int c = 0; c < NCB; ++c) {
+      bf16x8 b[NPL];
+      load_planes(sB, NB16, 16 * c + li, lk, b);
+      cmfma_bf(accr[c], acci[c], a, an, b);
+    }
+    __syncthreads();
+  }
+  double2 *dst = part + (size_t)blockIdx.z * nrow * nbh;
+#pragma unroll
+  for (int c = 0; c < NCB; ++c) {
+    const int j = b0 + 16 * c + li;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = r0 + 16 * wv + 4 * lk + r;
+      if (i < nrow && j < nbh) dst[(size_t)i * nbh + j] = make_double2(accr[c][r], acci[c][r]);
+    }
+  }
+}
+
+// ---- column GEMM: psi[b][G] += sum_k A(b,k) B(k,G) (split-BF16x3) -----------
+// A rows: 8 consecutive k of band b
+struct AddproAbf {
+  const double2 *x;
+  int nbh, nk;
+  __device__ void operator()(int b, int k0, Planes8 &o) const {
+#pragma unroll
+    for (int e = 0; e < 8; ++e) put(o, e, (k0 + e < nk && b < nbh) ? x[(size_t)(k0 + e) * nbh + b] : czero());
+  }
+};
+struct AddopsiAbf {
+  const double *lam;
+  int nb, nbh;
+  __device__ void operator()(int b, int k0, Planes8 &o) const {
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int k = k0 + e;
+      double2 v = czero();
+      if (k < 2 * nbh && b < nbh) {
+        double2 l1, l2;
+        fold_lambda(lam, nb, k < nbh ? k : k - nbh, b, l1, l2);
+        v = k < nbh ? l1 : l2;
+      }
+      put(o, e, v);
+    }
+  }
+};
+// B columns: 8 consecutive k at one G
+struct AddproBbf {
+  const uint4 *ps;
+  int nglp, nk;
+  __device__ void operator()(int k0, int ig, Planes8 &o) const {
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      uint4 u = make_uint4(0u, 0u, 0u, 0u);
+      if (k0 + e < nk && ig < nglp) u = ps[(size_t)(k0 + e) * nglp + ig];
+      put_bits(o, e, u);
+    }
+  }
+};
+struct AddopsiBbf {
+  const double2 *chi;
+  const int *minusg;
+  int ngl, nbh;
+  __device__ void operator()(int k0, int ig, Planes8 &o) const {
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int k = k0 + e;
+      double2 v = czero();
+      if (ig < ngl && k < 2 * nbh)
+        v = k < nbh ? chi[(size_t)k * ngl + ig] : cconj(chi[(size_t)(k - nbh) * ngl + minusg[ig]]);
+      put(o, e, v);
+    }
+  }
+};
+
+template <class GA, class GB, int NRB, int NW>
+__global__ void __launch_bounds__(64 * NW) k_colgemm_bf(GA ga, GB gb, int nk, int ngl, int nbh,
+                                                        double2 *__restrict__ psi) {
+  constexpr int WG = 16 * NW, NB16 = 16 * NRB;
+  __shared__ __bf16 sB[NPL * WG][KBP];
+  __shared__ __bf16 sA[NPL * NB16][KBP];
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6, li = lane & 15, lk = lane >> 4;
+  const int g0 = blockIdx.x * WG, b0 = blockIdx.y * NB16;
+  flt4 accr[NRB], acci[NRB];
+#pragma unroll
+  for (int c = 0; c < NRB; ++c) {
+    accr[c] = flt4{0.f, 0.f, 0.f, 0.f};
+    acci[c] = flt4{0.f, 0.f, 0.f, 0.f};
+  }
+  for (int k0 = 0; k0 < nk; k0 += KB) {
+    {
+      const int g = t % WG, k8 = t / WG;      // WG columns x 4 runs, one per thread
+      Planes8 v;
+      gb(k0 + 8 * k8, g0 + g, v);
+      store_planes(sB, WG, g, k8, v);
+    }
+    for (int e = t; e < NB16 * 4; e += 64 * NW) {
+      const int b = e >> 2, k8 = e & 3;
+      Planes8 v;
+      ga(b0 + b, k0 + 8 * k8, v);
+      store_planes(sA, NB16, b, k8, v);
+    }
+    __syncthreads();
+    bf16x8 bb[NPL];
+    load_planes(sB, WG, 16 * wv + li, lk, bb);
+#pragma unroll
+    for (int c = 0; c < NRB; ++c) {
+      bf16x8 a[NPL], an[3];
+      load_planes(sA, NB16, 16 * c + li, lk, a);
+#pragma unroll
+      for (int q = 0; q < 3; ++q) an[q] = bf_neg(a[3 + q]);
+      cmfma_bf(accr[c], acci[c], a, an, bb);
+    }
+    __syncthreads();
+  }
+  const int ig = g0 + 16 * wv + li;
+  if (ig >= ngl) return;
+#pragma unroll
+  for (int c = 0; c < NRB; ++c) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int b = b0 + 16 * c + 4 * lk + r;
+      if (b < nbh) {
+        double2 &o = psi[(size_t)b * ngl + ig];
+        o = make_double2(o.x + (double)accr[c][r], o.y + (double)acci[c][r]);
+      }
+    }
+  }
+}
+
 // ADDOPROJ: proj[k][j] += sum_i oproj[k][i] L1[i][j] + conj(oproj[k][i]) L2[i][j]
 __global__ void k_addoproj(int npro, int nbh, int nb, double2 *proj, const double2 *oproj,
                            const double *lam) {
@@ -714,6 +1046,56 @@ int run_colgemm(pw_waveset *ws, const GA &ga, const GB &gb, int nk, double2 *psi
   return PW_OK;
 }
 
+// split-BF16x3 counterpart of run_rowgemm (G splits are whole 32-chunks)
+template <int NW, class Gen>
+int run_rowgemm_bf(pw_waveset *ws, const Gen &gen, int nrow, const double2 *psi, double scale,
+                   double2 *out, double2 **res, const char *name) {
+  const int ngl = ws->plan->ngl, nbh = ws->nbh;
+  int nblk, tb;
+  band_tiles(nbh, &nblk, &tb);
+  const int tr = (nrow + 16 * NW - 1) / (16 * NW);
+  const int want = std::max(1, 2048 / NW / std::max(1, tr * tb));
+  int ns = std::min(want, (ngl + KB - 1) / KB);
+  const int gsplit = ((ngl + ns - 1) / ns + KB - 1) / KB * KB;
+  ns = (ngl + gsplit - 1) / gsplit;
+  const size_t n = (size_t)nrow * nbh;
+  PW_TRY(ws_part(ws, (size_t)(ns + (out ? 0 : 1)) * n * sizeof(double2)));
+  double2 *part = (double2 *)ws->d_part;
+  if (!out) out = part + (size_t)ns * n;
+  {
+    ScopeTimer tm(ws, name);
+    dim3 grid(tr, tb, ns);
+#define PW_LAUNCH(NB) k_rowgemm_bf<Gen, NB, NW><<<grid, 64 * NW, 0, ws->stream>>>(gen, nrow, ngl, nbh, psi, gsplit, part)
+    PW_NB_SWITCH(nblk, PW_LAUNCH)
+#undef PW_LAUNCH
+    PW_HIP(hipGetLastError());
+  }
+  {
+    ScopeTimer tm(ws, "reduce");
+    k_reduce_split<double2><<<(n + kRedC - 1) / kRedC, kRedC * kRedS, 0, ws->stream>>>(part, ns, (int)n, scale,
+                                                                                         out);
+    PW_HIP(hipGetLastError());
+  }
+  if (res) *res = out;
+  return PW_OK;
+}
+
+template <int NW, class GA, class GB>
+int run_colgemm_bf(pw_waveset *ws, const GA &ga, const GB &gb, int nk, double2 *psi, const char *name) {
+  const int ngl = ws->plan->ngl, nbh = ws->nbh;
+  int nblk, tb;
+  band_tiles(nbh, &nblk, &tb);
+  dim3 grid((ngl + 16 * NW - 1) / (16 * NW), tb);
+  ScopeTimer tm(ws, name);
+#define PW_LAUNCH(NB) k_colgemm_bf<GA, GB, NB, NW><<<grid, 64 * NW, 0, ws->stream>>>(ga, gb, nk, ngl, nbh, psi)
+  PW_NB_SWITCH(nblk, PW_LAUNCH)
+#undef PW_LAUNCH
+  PW_HIP(hipGetLastError());
+  return PW_OK;
+}
+
+int nglp_of(const pw_waveset *ws) { return (ws->plan->ngl + 63) / 64 * 64; }
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -758,7 +1140,27 @@ int k_phase_tables(pw_waveset *ws, const double *r, double2 *phase) {
   return PW_OK;
 }
 
+// P8 (split projector matrix) for the positions of the current phase tables
+int k_pgen(pw_waveset *ws) {
+  const int nglp = nglp_of(ws);
+  const size_t n = (size_t)ws->npro * nglp;
+  if (n * sizeof(uint4) > ws->p8_bytes) {
+    if (ws->d_p8) PW_HIP(hipFree(ws->d_p8));
+    PW_HIP(hipMalloc(&ws->d_p8, n * sizeof(uint4)));
+    ws->p8_bytes = n * sizeof(uint4);
+  }
+  ProjGeo q = make_pgeo(ws);
+  ScopeTimer tm(ws, "pgen");
+  k_pgen_bf<<<(n + 255) / 256, 256, 0, ws->stream>>>(q, ws->d_ipro_tab, ws->npro, nglp, (uint4 *)ws->d_p8);
+  PW_HIP(hipGetLastError());
+  return PW_OK;
+}
+
 int k_projections(pw_waveset *ws, const double2 *psi, const double2 *phase, double2 *proj) {
+  if (ws->prec == PW_PREC_MIXED) {
+    RowP8 gen{(const uint4 *)ws->d_p8, nglp_of(ws)};
+    return run_rowgemm_bf<4>(ws, gen, ws->npro, psi, ws->plan->vol, proj, nullptr, "projections");
+  }
   ProjGeo q = make_pgeo(ws);
   q.phase = phase;
   GenProj gen{q, ws->d_ipro_tab};
@@ -766,6 +1168,11 @@ int k_projections(pw_waveset *ws, const double2 *psi, const double2 *phase, doub
 }
 
 int k_addpro(pw_waveset *ws, double2 *psi, const double2 *phase, const double2 *x) {
+  if (ws->prec == PW_PREC_MIXED) {
+    AddproAbf ga{x, ws->nbh, ws->npro};
+    AddproBbf gb{(const uint4 *)ws->d_p8, nglp_of(ws), ws->npro};
+    return run_colgemm_bf<2>(ws, ga, gb, ws->npro, psi, "addpro");
+  }
   ProjGeo q = make_pgeo(ws);
   q.phase = phase;
   AddproA ga{x, ws->nbh};
@@ -814,9 +1221,14 @@ int k_force(pw_waveset *ws, const double2 *psi, const double2 *phase, const doub
 int k_overlap(pw_waveset *ws, const double2 *psi1, const double2 *psi2, double *mat, bool, bool) {
   pw_plan *p = ws->plan;
   const int nbh = ws->nbh, nbhp = (nbh + 15) / 16 * 16;
-  GenOvl gen{psi1, p->d_minusg, p->ngl, nbh, nbhp};
   double2 *red = nullptr;
-  PW_TRY(run_rowgemm<2>(ws, gen, 2 * nbhp, psi2, p->vol, nullptr, &red, "overlap"));
+  if (ws->prec == PW_PREC_MIXED) {
+    RowOvl gen{psi1, p->d_minusg, p->ngl, nbh, nbhp};
+    PW_TRY(run_rowgemm_bf<2>(ws, gen, 2 * nbhp, psi2, p->vol, nullptr, &red, "overlap"));
+  } else {
+    GenOvl gen{psi1, p->d_minusg, p->ngl, nbh, nbhp};
+    PW_TRY(run_rowgemm<2>(ws, gen, 2 * nbhp, psi2, p->vol, nullptr, &red, "overlap"));
+  }
   k_unravel_pw<<<(nbh * nbh + 255) / 256, 256, 0, ws->stream>>>(nbh, ws->nb, red, red + (size_t)nbhp * nbh,
                                                                   mat);
   PW_HIP(hipGetLastError());
@@ -844,6 +1256,11 @@ int k_atom_matrix(pw_waveset *ws, const double2 *proj, double2 *out, const doubl
 
 int k_addopsi(pw_waveset *ws, double2 *psibar, const double2 *chi, const double *lam) {
   pw_plan *p = ws->plan;
+  if (ws->prec == PW_PREC_MIXED) {
+    AddopsiAbf ga{lam, ws->nb, ws->nbh};
+    AddopsiBbf gb{chi, p->d_minusg, p->ngl, ws->nbh};
+    return run_colgemm_bf<2>(ws, ga, gb, 2 * ws->nbh, psibar, "addopsi");
+  }
   AddopsiA ga{lam, ws->nb, ws->nbh};
   AddopsiB gb{chi, p->d_minusg, p->ngl, ws->nbh};
   return run_colgemm<2>(ws, ga, gb, 2 * ws->nbh, psibar, "addopsi");

@@ -143,7 +143,14 @@ static int upload_mass(pw_waveset *ws) {
 }
 
 static int phase_at(pw_waveset *ws, const std::vector<double> &r) {
-  return k_phase_tables(ws, r.data(), ws->d_phase);
+  PW_TRY(k_phase_tables(ws, r.data(), ws->d_phase));
+  // the mixed form's products read the split projector matrix of these positions
+  if (ws->prec == PW_PREC_MIXED && (!ws->p8_valid || ws->p8_pos != r)) {
+    PW_TRY(k_pgen(ws));
+    ws->p8_pos = r;
+    ws->p8_valid = true;
+  }
+  return PW_OK;
 }
 
 // DEDPROJ (src/paw_waves1.f90:5592-5691) into d_xproj
@@ -332,7 +339,7 @@ extern "C" int pw_waves_destroy(pw_waveset *ws) {
                   ws->d_xproj, ws->d_pro, ws->d_ylm, ws->d_mpsi, ws->d_v, ws->d_rho, ws->d_occ2,
                   ws->d_fpm, ws->d_fab, ws->d_domat, ws->d_dhmat, ws->d_atom_isp, ws->d_ipro0,
                   ws->d_ibpro, ws->d_lmnx, ws->d_lmn_ln, ws->d_lmn_lm, ws->d_lmn_l, ws->d_phase,
-                  ws->d_part, ws->d_small, ws->d_ortho, ws->d_small_mats, ws->d_ipro_tab};
+                  ws->d_part, ws->d_small, ws->d_ortho, ws->d_small_mats, ws->d_ipro_tab, ws->d_p8};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (ws->own_stream && ws->stream) (void)hipStreamDestroy(ws->stream);
@@ -599,6 +606,7 @@ extern "C" int pw_waves_etot(pw_waveset *ws) {
   if (!ws) return pw_fail(PW_ERR_ARG, "pw_waves_etot: null waveset");
   PW_TRY(ready(ws));
   pw_plan *p = ws->plan;
+  ws->p8_valid = false;   // the projector expansion is redone at least once per step
   if (ws->first) {
     PW_TRY(pw_waves_gramschmidt(ws));
     PW_TRY(pw_waves_projections(ws));

@@ -1,0 +1,69 @@
+"""The mixed-precision form (FP32 FFT passes, split-BF16x3 MFMA products,
+FP64 constraint solve, energies and forces) against the FP64 oracle: the
+error gates |dE| < 1e-6 Ha/atom and |dF| < 1e-4 Ha/bohr per step, and the
+orthonormality residual ||psi^H S psi - 1|| of the mixed form's wave
+functions measured in the oracle's FP64 metric."""
+import numpy as np
+import pytest
+
+from harness import SYSTEMS, gpu_waveset, oracle_waveset, relerr
+from oracle import waves as owv
+
+pytestmark = pytest.mark.gpu
+
+DE_ATOM = 1.0e-6      # Ha/atom
+DF = 1.0e-4           # Ha/bohr
+ORTHO_RES = 1.0e-5    # max |psi^H S psi - 1| in FP64, mixed-form wave functions
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+def ortho_residual(o, psi):
+    """max |<psi_i|S|psi_j> - delta_ij| with the oracle's FP64 overlaps."""
+    pl, st = o.plane, o.setup
+    proj = owv.projections(pl, st, o.pro, o.ylm, o.r0, psi)
+    s = owv.overlap(pl, o.nb, psi, psi) + owv.overlap_1c(st, o.nb, proj, proj)
+    return float(np.max(np.abs(s - np.eye(o.nb))))
+
+
+@pytest.mark.parametrize("name,nsteps", [("tiny", 4), ("si8", 5)])
+def test_mixed_form_error_gates(name, nsteps):
+    s = SYSTEMS[name]()
+    say(f"case mixed gates {name} x{nsteps}")
+    o = oracle_waveset(s)
+    o.r0 = s.pos
+    plane, w = gpu_waveset(s, prec=1)
+    worst = [0.0, 0.0, 0.0]
+    for it in range(nsteps):
+        eo = o.etot(s.pos)
+        e = w.etot()
+        de = abs(e["etot"] - eo["etot"]) / s.nat
+        df = float(np.max(np.abs(w.getr8a("FORCE") - o.force)))
+        o.propagate()
+        o.orthogonalize(s.pos, s.pos)
+        w.propagate()
+        w.orthogonalize()
+        res = ortho_residual(o, w.getc8a("PSIM"))
+        worst = [max(worst[0], de), max(worst[1], df), max(worst[2], res)]
+        assert de < DE_ATOM, (it, de)
+        assert df < DF, (it, df)
+        assert res < ORTHO_RES, (it, res)
+        o.switch()
+        w.switch()
+    say(f"  worst |dE|/atom {worst[0]:.2e}  |dF| {worst[1]:.2e}  ortho residual {worst[2]:.2e}")
+
+
+def test_mixed_projections_accuracy():
+    s = SYSTEMS["si8"]()
+    say("case mixed projections si8")
+    o = oracle_waveset(s)
+    plane, w = gpu_waveset(s, prec=1)
+    w.gramschmidt()
+    w.projections()
+    psi = w.getc8a("PSI0")
+    want = owv.projections(o.plane, o.setup, o.pro, o.ylm, s.pos, psi)
+    err = relerr(w.getc8a("PROJ"), want)
+    say(f"  projections rel err {err:.2e}")
+    assert err < 1e-5
