@@ -82,6 +82,8 @@ struct pw_plan {
   std::vector<int> stripe_x;   // ngl x-index of each entry
   std::vector<int> stripe_s;   // ngl stripe of each entry
   std::vector<int> yz2stripe;  // nr2*nr3, -1 if empty
+  std::vector<int> stripe_pos; // nstripe: digit-reversed z*nr2 + y of each stripe
+  std::vector<int> zpos;       // nzp: digit-reversed z*nr2 of each stripe-holding z plane
   int nzp = 0;                 // z planes holding stripes
   std::vector<int> z2iz;       // nr3, -1 if empty
   std::vector<int> zlist;      // nzp: z of each used plane
@@ -92,6 +94,7 @@ struct pw_plan {
   int *d_stripe_off = nullptr, *d_stripe_g = nullptr, *d_stripe_x = nullptr;
   int *d_stripe_s = nullptr;
   int *d_yz2stripe = nullptr, *d_z2iz = nullptr, *d_zlist = nullptr;
+  int *d_stripe_pos = nullptr, *d_zpos = nullptr;
   double *d_g2 = nullptr, *d_gvec = nullptr;
   double2 *d_tw[3] = {nullptr, nullptr, nullptr};
   float2 *d_twf[3] = {nullptr, nullptr, nullptr};
@@ -101,6 +104,7 @@ struct pw_plan {
   void *d_stripe_buf = nullptr; size_t stripe_bytes = 0;
   void *d_plane_buf = nullptr; size_t plane_bytes = 0;
   void *d_dens_part = nullptr; size_t dens_bytes = 0;
+  void *d_vt = nullptr; size_t vt_bytes = 0;   // V(r) as [x][z][y] in the work precision
 };
 
 int plan_work(pw_plan *p, int nbatch, int prec);

@@ -9,17 +9,18 @@
 // sphere.  Unlike the reference, which runs one band and one FFTW line at a
 // time, each pass here is one batched kernel over all bands:
 //   x pass : sphere -> stripe rows (scatter fused), radix-R Stockham in LDS
-//   y pass : stripe rows -> z planes that hold stripes (zero lines implicit)
-//   z pass : full z pencils; fused with V(r) (VPSI), with the density sum, or
-//            with the full-grid read/write of the PLANEWAVE$FFT interface
-// Work buffers: stripes [band][stripe][x], planes [band][iz][y][x].
+//   yz pass: one (x, band) plane of NR2 x NR3 resident in LDS: y transforms
+//            on the z planes that hold stripes, full z transforms, then
+//            V(r) and the inverse transforms (VPSI) or the occupation-
+//            weighted |psi|^2 sum (density); no grid round trip through HBM
+//   y / z passes over HBM z planes serve the full-grid PLANEWAVE$FFT calls
+// Work buffers: stripes [band][x][stripe], planes [band][iz][y][x].
 #include "pw_common.h"
 
 namespace {
 
 constexpr int kBlock = 256;
 constexpr int kTX = 16;       // x columns per y/z tile
-constexpr int kPX = 16;       // stripe rows per x tile
 
 template <typename T2> struct Prec;
 template <> struct Prec<double2> {
@@ -193,29 +194,39 @@ __device__ T2 *lds_fft(T2 *b0, T2 *b1, const Fft1D &f, int npen, int es, int ps,
 struct Geo {
   int nr1, nr2, nr3, ngl, nstripe, nzp;
   const int *stripe_off, *stripe_g, *stripe_x, *stripe_s, *yz2stripe, *z2iz, *zlist;
+  const int *stripe_pos, *zpos;   // digit-reversed plane offsets of stripes / stripe rows
   const double *g2;
 };
 
-// ---- x pass, forward: sphere -> stripes --------------------------------------
+// stripes per x-pass workgroup (LDS holds two kXT x NR1 buffers)
+template <typename T2> struct XTile;
+template <> struct XTile<double2> { static constexpr int n = 16; };
+template <> struct XTile<float2> { static constexpr int n = 32; };
+
+// ---- x pass, forward: sphere -> stripes [band][x][stripe] --------------------
 template <typename T2>
 __global__ void __launch_bounds__(kBlock) k_xfwd(Geo g, Fft1D f, const T2 *tw, const double2 *psi,
                                                  T2 *stripes) {
+  constexpr int PX = XTile<T2>::n;
   extern __shared__ unsigned char smem_raw[];
   T2 *b0 = reinterpret_cast<T2 *>(smem_raw);
-  T2 *b1 = b0 + kPX * g.nr1;
-  const int s0 = blockIdx.x * kPX;
+  T2 *b1 = b0 + PX * g.nr1;
+  const int s0 = blockIdx.x * PX;
   const int band = blockIdx.y;
-  const int ns = min(kPX, g.nstripe - s0);
-  for (int i = threadIdx.x; i < kPX * g.nr1; i += blockDim.x) b0[i] = czero<T2>();
+  const int ns = min(PX, g.nstripe - s0);
+  for (int i = threadIdx.x; i < PX * g.nr1; i += blockDim.x) b0[i] = czero<T2>();
   __syncthreads();
   const double2 *src = psi + (size_t)band * g.ngl;
   const int e0 = g.stripe_off[s0], e1 = g.stripe_off[s0 + ns];
   for (int e = e0 + threadIdx.x; e < e1; e += blockDim.x)
     b0[(g.stripe_s[e] - s0) * g.nr1 + g.stripe_x[e]] = Prec<T2>::from(src[g.stripe_g[e]]);
   __syncthreads();
-  T2 *r = lds_fft<T2>(b0, b1, f, kPX, 1, g.nr1, tw, +1, false);
-  T2 *dst = stripes + ((size_t)band * g.nstripe + s0) * g.nr1;
-  for (int i = threadIdx.x; i < ns * g.nr1; i += blockDim.x) dst[i] = r[i];
+  T2 *r = lds_fft<T2>(b0, b1, f, PX, 1, g.nr1, tw, +1, false);
+  T2 *dst = stripes + (size_t)band * g.nr1 * g.nstripe + s0;
+  for (int i = threadIdx.x; i < ns * g.nr1; i += blockDim.x) {
+    const int x = i / ns, sl = i % ns;
+    dst[(size_t)x * g.nstripe + sl] = r[sl * g.nr1 + x];
+  }
 }
 
 // ---- x pass, inverse: stripes -> sphere (gather, scale, optional G^2/2 psi) ---
@@ -223,17 +234,20 @@ template <typename T2>
 __global__ void __launch_bounds__(kBlock) k_xinv(Geo g, Fft1D f, const T2 *tw, const T2 *stripes,
                                                  double2 *out, const double2 *psi, double scale,
                                                  int add_kin) {
+  constexpr int PX = XTile<T2>::n;
   extern __shared__ unsigned char smem_raw[];
   T2 *b0 = reinterpret_cast<T2 *>(smem_raw);
-  T2 *b1 = b0 + kPX * g.nr1;
-  const int s0 = blockIdx.x * kPX;
+  T2 *b1 = b0 + PX * g.nr1;
+  const int s0 = blockIdx.x * PX;
   const int band = blockIdx.y;
-  const int ns = min(kPX, g.nstripe - s0);
-  const T2 *src = stripes + ((size_t)band * g.nstripe + s0) * g.nr1;
-  for (int i = threadIdx.x; i < kPX * g.nr1; i += blockDim.x)
-    b0[i] = i < ns * g.nr1 ? src[i] : czero<T2>();
+  const int ns = min(PX, g.nstripe - s0);
+  const T2 *src = stripes + (size_t)band * g.nr1 * g.nstripe + s0;
+  for (int i = threadIdx.x; i < PX * g.nr1; i += blockDim.x) {
+    const int x = i / PX, sl = i % PX;
+    b0[sl * g.nr1 + x] = sl < ns ? src[(size_t)x * g.nstripe + sl] : czero<T2>();
+  }
   __syncthreads();
-  T2 *r = lds_fft<T2>(b0, b1, f, kPX, 1, g.nr1, tw, -1, false);
+  T2 *r = lds_fft<T2>(b0, b1, f, PX, 1, g.nr1, tw, -1, false);
   const int e0 = g.stripe_off[s0], e1 = g.stripe_off[s0 + ns];
   double2 *dst = out + (size_t)band * g.ngl;
   const double2 *ps = psi + (size_t)band * g.ngl;
@@ -267,7 +281,7 @@ __global__ void __launch_bounds__(kBlock) k_yfwd(Geo g, Fft1D f, const T2 *tw, c
   for (int w = threadIdx.x; w < g.nr2 * kTX; w += blockDim.x) {
     const int y = w / kTX, x = w % kTX, gx = x0 + x;
     const int s = g.yz2stripe[y + g.nr2 * z];
-    b0[w] = (s >= 0 && gx < g.nr1) ? src[(size_t)s * g.nr1 + gx] : czero<T2>();
+    b0[w] = (s >= 0 && gx < g.nr1) ? src[(size_t)gx * g.nstripe + s] : czero<T2>();
   }
   __syncthreads();
   T2 *r = lds_fft<T2>(b0, b1, f, kTX, kTX, 1, tw, +1, true);
@@ -301,7 +315,7 @@ __global__ void __launch_bounds__(kBlock) k_yinv(Geo g, Fft1D f, const T2 *tw, c
   for (int w = threadIdx.x; w < g.nr2 * kTX; w += blockDim.x) {
     const int y = w / kTX, x = w % kTX, gx = x0 + x;
     const int s = g.yz2stripe[y + g.nr2 * z];
-    if (s >= 0 && gx < g.nr1) dst[(size_t)s * g.nr1 + gx] = r[w];
+    if (s >= 0 && gx < g.nr1) dst[(size_t)gx * g.nstripe + s] = r[w];
   }
 }
 
@@ -325,77 +339,199 @@ __device__ inline void store_zpencils(const Geo &g, T2 *planes_band, int y, int 
   }
 }
 
-// ---- z pass: forward z, times V(r), inverse z (WAVES_VPSI core) -------------------
-template <typename T2>
-__global__ void __launch_bounds__(kBlock) k_zvz(Geo g, Fft1D f, const T2 *tw, T2 *planes,
-                                                const double *v) {
-  typedef typename Prec<T2>::T T;
-  extern __shared__ unsigned char smem_raw[];
-  T2 *b0 = reinterpret_cast<T2 *>(smem_raw);
-  T2 *b1 = b0 + kTX * g.nr3;
-  const int ntx = (g.nr1 + kTX - 1) / kTX;
-  const int y = blockIdx.x / ntx;
-  const int x0 = (blockIdx.x % ntx) * kTX;
-  const int band = blockIdx.y;
-  T2 *pb = planes + (size_t)band * g.nzp * g.nr2 * g.nr1;
-  load_zpencils<T2>(g, pb, y, x0, b0);
-  __syncthreads();
-  T2 *r = lds_fft<T2>(b0, b1, f, kTX, kTX, 1, tw, +1, true);
-  for (int w = threadIdx.x; w < g.nr3 * kTX; w += blockDim.x) {
-    const int z = w / kTX, x = w % kTX, gx = x0 + x;
-    const T vv = gx < g.nr1 ? (T)v[((size_t)z * g.nr2 + y) * g.nr1 + gx] : (T)0;
-    r[w].x *= vv;
-    r[w].y *= vv;
+// ---- in-place LDS transforms for the resident yz plane -----------------------
+// Mixed-radix Cooley-Tukey in place: every butterfly reads and writes the
+// same R positions, so a thread handles one butterfly at a time and only the
+// stages are separated by barriers.  DIT takes digit-reversed input and gives
+// natural order; DIF takes natural order and gives digit-reversed output
+// (the reversal of a length follows its radix sequence, Fft1D::radix).
+constexpr int kYZ = 512;
+
+template <typename T2, int R>
+__device__ inline void dit_stage(T2 *buf, int n, int L, int npen, int es, int ps, const int *pmap,
+                                 const T2 *__restrict__ tw, int sign, bool pen_fast) {
+  const int Lp = L / R, nbf = n / R, total = nbf * npen, tws = n / L;
+  for (int w = threadIdx.x; w < total; w += kYZ) {
+    int pen, b;
+    if (pen_fast) { pen = w % npen; b = w / npen; }
+    else { b = w % nbf; pen = w / nbf; }
+    const int g = b / Lp, j = b % Lp;
+    T2 *base = buf + (pmap ? pmap[pen] : pen * ps) + (g * L + j) * es;
+    T2 v[R];
+#pragma unroll
+    for (int q = 0; q < R; ++q) v[q] = base[q * Lp * es];
+    if (Lp > 1) {
+#pragma unroll
+      for (int q = 1; q < R; ++q) {
+        T2 t = tw[j * q * tws];
+        if (sign < 0) t.y = -t.y;
+        v[q] = cmul(v[q], t);
+      }
+    }
+    dft<T2, R>(v, sign);
+#pragma unroll
+    for (int q = 0; q < R; ++q) base[q * Lp * es] = v[q];
   }
   __syncthreads();
-  T2 *other = (r == b0) ? b1 : b0;
-  T2 *r2 = lds_fft<T2>(r, other, f, kTX, kTX, 1, tw, -1, true);
-  store_zpencils<T2>(g, pb, y, x0, r2);
 }
 
-// ---- z pass: density sum over a group of bands -> partial rho[group] ---------
+template <typename T2, int R>
+__device__ inline void dif_stage(T2 *buf, int n, int L, int npen, int es, int ps, const int *pmap,
+                                 const T2 *__restrict__ tw, int sign, bool pen_fast) {
+  const int Lp = L / R, nbf = n / R, total = nbf * npen, tws = n / L;
+  for (int w = threadIdx.x; w < total; w += kYZ) {
+    int pen, b;
+    if (pen_fast) { pen = w % npen; b = w / npen; }
+    else { b = w % nbf; pen = w / nbf; }
+    const int g = b / Lp, j = b % Lp;
+    T2 *base = buf + (pmap ? pmap[pen] : pen * ps) + (g * L + j) * es;
+    T2 v[R];
+#pragma unroll
+    for (int q = 0; q < R; ++q) v[q] = base[q * Lp * es];
+    dft<T2, R>(v, sign);
+    if (Lp > 1) {
+#pragma unroll
+      for (int q = 1; q < R; ++q) {
+        T2 t = tw[j * q * tws];
+        if (sign < 0) t.y = -t.y;
+        v[q] = cmul(v[q], t);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < R; ++q) base[q * Lp * es] = v[q];
+  }
+  __syncthreads();
+}
+
+#define PW_RADIX_SWITCH(r, CALL) \
+  switch (r) {                   \
+    case 2: CALL(2); break;      \
+    case 3: CALL(3); break;      \
+    case 4: CALL(4); break;      \
+    case 5: CALL(5); break;      \
+    case 7: CALL(7); break;      \
+    case 8: CALL(8); break;      \
+    case 11: CALL(11); break;    \
+    default: break;              \
+  }
+
 template <typename T2>
-__global__ void __launch_bounds__(kBlock) k_zdens(Geo g, Fft1D f, const T2 *tw, const T2 *planes,
-                                                  int nb, int bpg, const double *occ2, double *part) {
+__device__ void fft_dit(T2 *buf, const Fft1D &f, int npen, int es, int ps, const int *pmap, const T2 *tw,
+                        int sign, bool pen_fast) {
+  int L = 1;
+  for (int st = 0; st < f.nstage; ++st) {
+    const int r = f.radix[st];
+    L *= r;
+#define PW_CALL(R) dit_stage<T2, R>(buf, f.n, L, npen, es, ps, pmap, tw, sign, pen_fast)
+    PW_RADIX_SWITCH(r, PW_CALL)
+#undef PW_CALL
+  }
+}
+
+template <typename T2>
+__device__ void fft_dif(T2 *buf, const Fft1D &f, int npen, int es, int ps, const int *pmap, const T2 *tw,
+                        int sign, bool pen_fast) {
+  int L = f.n;
+  for (int st = f.nstage - 1; st >= 0; --st) {
+    const int r = f.radix[st];
+#define PW_CALL(R) dif_stage<T2, R>(buf, f.n, L, npen, es, ps, pmap, tw, sign, pen_fast)
+    PW_RADIX_SWITCH(r, PW_CALL)
+#undef PW_CALL
+    L /= r;
+  }
+}
+
+// zero the plane, scatter the x column of one band's stripes into the
+// digit-reversed (z, y) positions, transform y (DIT, on the rows of the z
+// planes that hold stripes) and z (DIT): pl becomes psi(x, y, z) in natural
+// [z][y] order (forward, +i).  zpos: offset of each stripe-holding row.
+template <typename T2>
+__device__ inline void yz_forward(const Geo &g, const Fft1D &fy, const Fft1D &fz, const T2 *twy,
+                                  const T2 *twz, const T2 *col, T2 *pl, const int *zpos) {
+  const int n23 = g.nr2 * g.nr3;
+  for (int i = threadIdx.x; i < n23; i += kYZ) pl[i] = czero<T2>();
+  __syncthreads();
+  for (int s = threadIdx.x; s < g.nstripe; s += kYZ) pl[g.stripe_pos[s]] = col[s];
+  __syncthreads();
+  fft_dit<T2>(pl, fy, g.nzp, 1, 0, zpos, twy, +1, false);
+  fft_dit<T2>(pl, fz, g.nr2, g.nr2, 1, nullptr, twz, +1, true);
+}
+
+// ---- yz pass of WAVES_VPSI: psi(x,y,z) * V(r), back to the stripes ----------------
+// One workgroup per (x, band); vt = V as [x][z][y] in the work precision.
+template <typename T2>
+__global__ void __launch_bounds__(kYZ) k_yz_vpsi(Geo g, Fft1D fy, Fft1D fz, const T2 *twy, const T2 *twz,
+                                                 T2 *stripes, const typename Prec<T2>::T *vt) {
+  typedef typename Prec<T2>::T T;
   extern __shared__ unsigned char smem_raw[];
-  T2 *b0 = reinterpret_cast<T2 *>(smem_raw);
-  T2 *b1 = b0 + kTX * g.nr3;
-  double *acc = reinterpret_cast<double *>(b1 + kTX * g.nr3);
-  const int ntx = (g.nr1 + kTX - 1) / kTX;
-  const int y = blockIdx.x / ntx;
-  const int x0 = (blockIdx.x % ntx) * kTX;
-  const int grp = blockIdx.y;
+  const int n23 = g.nr2 * g.nr3;
+  T2 *pl = reinterpret_cast<T2 *>(smem_raw);
+  int *zpos = reinterpret_cast<int *>(pl + n23);
+  const int x = blockIdx.x, band = blockIdx.y;
+  for (int i = threadIdx.x; i < g.nzp; i += kYZ) zpos[i] = g.zpos[i];
+  T2 *col = stripes + ((size_t)band * g.nr1 + x) * g.nstripe;
+  yz_forward<T2>(g, fy, fz, twy, twz, col, pl, zpos);
+  const T *vx = vt + (size_t)x * n23;
+  for (int i = threadIdx.x; i < n23; i += kYZ) {
+    const T vv = vx[i];
+    pl[i].x *= vv;
+    pl[i].y *= vv;
+  }
+  __syncthreads();
+  fft_dif<T2>(pl, fz, g.nr2, g.nr2, 1, nullptr, twz, -1, true);
+  fft_dif<T2>(pl, fy, g.nzp, 1, 0, zpos, twy, -1, false);
+  for (int s = threadIdx.x; s < g.nstripe; s += kYZ) col[s] = pl[g.stripe_pos[s]];
+}
+
+// ---- yz pass of WAVES_DENSITY: sum over a group of bands ----------------------
+// part[group][x][z][y] = sum_b s1 |psi|^2 + s2 Re(psi^2) over the group's bands
+template <typename T2>
+__global__ void __launch_bounds__(kYZ) k_yz_dens(Geo g, Fft1D fy, Fft1D fz, const T2 *twy, const T2 *twz,
+                                                 const T2 *stripes, int nb, int bpg, const double *occ2,
+                                                 double *part) {
+  extern __shared__ unsigned char smem_raw[];
+  const int n23 = g.nr2 * g.nr3;
+  double *acc = reinterpret_cast<double *>(smem_raw);
+  T2 *pl = reinterpret_cast<T2 *>(acc + n23);
+  int *zpos = reinterpret_cast<int *>(pl + n23);
+  const int x = blockIdx.x, grp = blockIdx.y;
   const int bstart = grp * bpg, bend = min(nb, bstart + bpg);
-  for (int w = threadIdx.x; w < g.nr3 * kTX; w += blockDim.x) acc[w] = 0.0;
+  for (int i = threadIdx.x; i < g.nzp; i += kYZ) zpos[i] = g.zpos[i];
+  for (int i = threadIdx.x; i < n23; i += kYZ) acc[i] = 0.0;
   for (int band = bstart; band < bend; ++band) {
     const double s1 = occ2[2 * band], s2 = occ2[2 * band + 1];
-    if (s1 == 0.0 && s2 == 0.0) continue;   // uniform across the block
-    const T2 *pb = planes + (size_t)band * g.nzp * g.nr2 * g.nr1;
+    if (s1 == 0.0 && s2 == 0.0) continue;   // uniform across the workgroup
     __syncthreads();
-    load_zpencils<T2>(g, pb, y, x0, b0);
-    __syncthreads();
-    T2 *r = lds_fft<T2>(b0, b1, f, kTX, kTX, 1, tw, +1, true);
-    for (int w = threadIdx.x; w < g.nr3 * kTX; w += blockDim.x) {
-      const double re = r[w].x, im = r[w].y;
+    yz_forward<T2>(g, fy, fz, twy, twz, stripes + ((size_t)band * g.nr1 + x) * g.nstripe, pl, zpos);
+    for (int i = threadIdx.x; i < n23; i += kYZ) {
+      const double re = pl[i].x, im = pl[i].y;
       // Re(psi (s1 conj(psi) + s2 psi)) = s1 |psi|^2 + s2 (re^2 - im^2)
-      acc[w] += s1 * (re * re + im * im) + s2 * (re * re - im * im);
+      acc[i] += s1 * (re * re + im * im) + s2 * (re * re - im * im);
     }
   }
   __syncthreads();
-  double *dst = part + (size_t)grp * g.nr1 * g.nr2 * g.nr3;
-  for (int w = threadIdx.x; w < g.nr3 * kTX; w += blockDim.x) {
-    const int z = w / kTX, x = w % kTX, gx = x0 + x;
-    if (gx < g.nr1) dst[((size_t)z * g.nr2 + y) * g.nr1 + gx] = acc[w];
-  }
+  double *dst = part + ((size_t)grp * g.nr1 + x) * n23;
+  for (int i = threadIdx.x; i < n23; i += kYZ) dst[i] = acc[i];
 }
 
-// rho = sum over band groups of the partial densities (fixed order)
-__global__ void k_sum_groups(size_t n, int ngrp, const double *part, double *rho) {
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  double s = 0.0;
-  for (int k = 0; k < ngrp; ++k) s += part[(size_t)k * n + i];
-  rho[i] = s;
+// out[b][a] = (sum_{k<nsum} in[k][a][b]) for in as nsum x [A][B]; 32x32 LDS tiles
+template <typename TI, typename TO>
+__global__ void k_transpose_sum(int A, int B, int nsum, const TI *in, TO *out) {
+  __shared__ double tile[32][33];
+  const int a0 = blockIdx.y * 32, b0 = blockIdx.x * 32;
+  const int tx = threadIdx.x % 32, ty = threadIdx.x / 32;   // 32 x 8 threads
+  for (int r = ty; r < 32; r += 8) {
+    const int a = a0 + r, b = b0 + tx;
+    double v = 0.0;
+    if (a < A && b < B)
+      for (int k = 0; k < nsum; ++k) v += (double)in[((size_t)k * A + a) * B + b];
+    tile[r][tx] = v;
+  }
+  __syncthreads();
+  for (int r = ty; r < 32; r += 8) {
+    const int b = b0 + r, a = a0 + tx;
+    if (a < A && b < B) out[(size_t)b * A + a] = (TO)tile[tx][r];
+  }
 }
 
 // ---- z pass with full-grid output (PLANEWAVE$FFT 'GTOR') ----------------------------
@@ -446,7 +582,7 @@ Geo make_geo(const pw_plan *p) {
   g.nstripe = p->nstripe; g.nzp = p->nzp;
   g.stripe_off = p->d_stripe_off; g.stripe_g = p->d_stripe_g; g.stripe_x = p->d_stripe_x;
   g.stripe_s = p->d_stripe_s; g.yz2stripe = p->d_yz2stripe; g.z2iz = p->d_z2iz;
-  g.zlist = p->d_zlist; g.g2 = p->d_g2;
+  g.zlist = p->d_zlist; g.stripe_pos = p->d_stripe_pos; g.zpos = p->d_zpos; g.g2 = p->d_g2;
   return g;
 }
 
@@ -455,13 +591,37 @@ template <> const double2 *twid<double2>(const pw_plan *p, int k) { return p->d_
 template <> const float2 *twid<float2>(const pw_plan *p, int k) { return p->d_twf[k]; }
 
 template <typename T2>
+int run_forward_x(pw_plan *p, const Geo &g, int nb, const double2 *psi, hipStream_t s) {
+  T2 *stripes = (T2 *)p->d_stripe_buf;
+  constexpr int PX = XTile<T2>::n;
+  dim3 gx((p->nstripe + PX - 1) / PX, nb);
+  k_xfwd<T2><<<gx, kBlock, 2 * PX * p->nr1 * sizeof(T2), s>>>(g, p->f[0], twid<T2>(p, 0), psi,
+                                                              stripes);
+  PW_HIP(hipGetLastError());
+  return PW_OK;
+}
+
+template <typename T2>
+int run_inverse_x(pw_plan *p, const Geo &g, int nb, double2 *out, const double2 *psi, double scale,
+                  int add_kin, hipStream_t s) {
+  T2 *stripes = (T2 *)p->d_stripe_buf;
+  constexpr int PX = XTile<T2>::n;
+  dim3 gx((p->nstripe + PX - 1) / PX, nb);
+  k_xinv<T2><<<gx, kBlock, 2 * PX * p->nr1 * sizeof(T2), s>>>(g, p->f[0], twid<T2>(p, 0),
+                                                              stripes, out, psi, scale, add_kin);
+  PW_HIP(hipGetLastError());
+  return PW_OK;
+}
+
+template <typename T2>
 int run_forward_xy(pw_plan *p, const Geo &g, int nb, const double2 *psi, hipStream_t s) {
   T2 *stripes = (T2 *)p->d_stripe_buf;
   T2 *planes = (T2 *)p->d_plane_buf;
   const int ntx = (p->nr1 + kTX - 1) / kTX;
-  dim3 gx((p->nstripe + kPX - 1) / kPX, nb);
-  k_xfwd<T2><<<gx, kBlock, 2 * kPX * p->nr1 * sizeof(T2), s>>>(g, p->f[0], twid<T2>(p, 0), psi,
-                                                               stripes);
+  constexpr int PX = XTile<T2>::n;
+  dim3 gx((p->nstripe + PX - 1) / PX, nb);
+  k_xfwd<T2><<<gx, kBlock, 2 * PX * p->nr1 * sizeof(T2), s>>>(g, p->f[0], twid<T2>(p, 0), psi,
+                                                              stripes);
   dim3 gy(p->nzp * ntx, nb);
   k_yfwd<T2><<<gy, kBlock, 2 * kTX * p->nr2 * sizeof(T2), s>>>(g, p->f[1], twid<T2>(p, 1),
                                                                stripes, planes);
@@ -478,9 +638,41 @@ int run_inverse_yx(pw_plan *p, const Geo &g, int nb, double2 *out, const double2
   dim3 gy(p->nzp * ntx, nb);
   k_yinv<T2><<<gy, kBlock, 2 * kTX * p->nr2 * sizeof(T2), s>>>(g, p->f[1], twid<T2>(p, 1),
                                                                planes, stripes);
-  dim3 gx((p->nstripe + kPX - 1) / kPX, nb);
-  k_xinv<T2><<<gx, kBlock, 2 * kPX * p->nr1 * sizeof(T2), s>>>(g, p->f[0], twid<T2>(p, 0),
-                                                               stripes, out, psi, scale, add_kin);
+  constexpr int PX = XTile<T2>::n;
+  dim3 gx((p->nstripe + PX - 1) / PX, nb);
+  k_xinv<T2><<<gx, kBlock, 2 * PX * p->nr1 * sizeof(T2), s>>>(g, p->f[0], twid<T2>(p, 0),
+                                                              stripes, out, psi, scale, add_kin);
+  PW_HIP(hipGetLastError());
+  return PW_OK;
+}
+
+// dynamic LDS of a yz workgroup; beyond 64 KiB the kernel's limit is raised
+template <typename K>
+int yz_lds(K kernel, size_t lds) {
+  if (lds > 65536)
+    PW_HIP(hipFuncSetAttribute((const void *)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  return PW_OK;
+}
+
+int yz_fits(const pw_plan *p, size_t lds) {
+  // one (x, band) plane must stay resident in LDS
+  if (lds > 160 * 1024 - 1024)
+    return pw_fail(PW_ERR_UNSUPPORTED, "WAVES_VPSI/DENSITY: the NR2 x NR3 plane does not fit in LDS");
+  return PW_OK;
+}
+
+template <typename T2>
+int vt_of(pw_plan *p, const double *v, hipStream_t s) {
+  typedef typename Prec<T2>::T T;
+  const size_t need = (size_t)p->nrl * sizeof(T);
+  if (need > p->vt_bytes) {
+    if (p->d_vt) PW_HIP(hipFree(p->d_vt));
+    PW_HIP(hipMalloc(&p->d_vt, need));
+    p->vt_bytes = need;
+  }
+  const int A = p->nr2 * p->nr3, B = p->nr1;   // v is [zy][x]; vt is [x][zy]
+  dim3 grid((B + 31) / 32, (A + 31) / 32);
+  k_transpose_sum<double, T><<<grid, 256, 0, s>>>(A, B, 1, v, (T *)p->d_vt);
   PW_HIP(hipGetLastError());
   return PW_OK;
 }
@@ -488,24 +680,29 @@ int run_inverse_yx(pw_plan *p, const Geo &g, int nb, double2 *out, const double2
 template <typename T2>
 int vpsi_t(pw_plan *p, int nb, const double2 *psi, const double *v, double2 *hpsi, int add_kin,
            hipStream_t s) {
+  typedef typename Prec<T2>::T T;
+  const size_t lds = (size_t)p->nr2 * p->nr3 * sizeof(T2) + (size_t)p->nzp * sizeof(int);
+  PW_TRY(yz_fits(p, lds));
   Geo g = make_geo(p);
-  PW_TRY(run_forward_xy<T2>(p, g, nb, psi, s));
-  const int ntx = (p->nr1 + kTX - 1) / kTX;
-  dim3 gz(p->nr2 * ntx, nb);
-  k_zvz<T2><<<gz, kBlock, 2 * kTX * p->nr3 * sizeof(T2), s>>>(g, p->f[2], twid<T2>(p, 2),
-                                                              (T2 *)p->d_plane_buf, v);
+  PW_TRY(vt_of<T2>(p, v, s));
+  PW_TRY(run_forward_x<T2>(p, g, nb, psi, s));
+  PW_TRY(yz_lds(k_yz_vpsi<T2>, lds));
+  k_yz_vpsi<T2><<<dim3(p->nr1, nb), kYZ, lds, s>>>(g, p->f[1], p->f[2], twid<T2>(p, 1), twid<T2>(p, 2),
+                                                   (T2 *)p->d_stripe_buf, (const T *)p->d_vt);
   PW_HIP(hipGetLastError());
-  return run_inverse_yx<T2>(p, g, nb, hpsi, psi, 1.0 / (double)p->nrl, add_kin, s);
+  return run_inverse_x<T2>(p, g, nb, hpsi, psi, 1.0 / (double)p->nrl, add_kin, s);
 }
 
 template <typename T2>
 int density_t(pw_plan *p, int nb, const double2 *psi, const double *occ2, double *rho,
               hipStream_t s) {
+  const size_t n23 = (size_t)p->nr2 * p->nr3;
+  const size_t lds = n23 * (sizeof(double) + sizeof(T2)) + (size_t)p->nzp * sizeof(int);
+  PW_TRY(yz_fits(p, lds));
   Geo g = make_geo(p);
-  PW_TRY(run_forward_xy<T2>(p, g, nb, psi, s));
-  const int ntx = (p->nr1 + kTX - 1) / kTX;
-  // band groups so that the grid holds ~4 workgroups per CU
-  int ngrp = std::max(1, std::min(nb, (1024 + p->nr2 * ntx - 1) / (p->nr2 * ntx)));
+  PW_TRY(run_forward_x<T2>(p, g, nb, psi, s));
+  // band groups so that the grid holds about two workgroups per CU
+  int ngrp = std::max(1, std::min(nb, (512 + p->nr1 - 1) / p->nr1));
   const int bpg = (nb + ngrp - 1) / ngrp;
   ngrp = (nb + bpg - 1) / bpg;
   const size_t need = (size_t)ngrp * p->nrl * sizeof(double);
@@ -514,12 +711,14 @@ int density_t(pw_plan *p, int nb, const double2 *psi, const double *occ2, double
     PW_HIP(hipMalloc(&p->d_dens_part, need));
     p->dens_bytes = need;
   }
-  size_t lds = 2 * kTX * p->nr3 * sizeof(T2) + kTX * p->nr3 * sizeof(double);
-  k_zdens<T2><<<dim3(p->nr2 * ntx, ngrp), kBlock, lds, s>>>(g, p->f[2], twid<T2>(p, 2),
-                                                           (const T2 *)p->d_plane_buf, nb, bpg, occ2,
-                                                           (double *)p->d_dens_part);
-  k_sum_groups<<<(p->nrl + 255) / 256, 256, 0, s>>>((size_t)p->nrl, ngrp, (const double *)p->d_dens_part,
-                                                    rho);
+  PW_TRY(yz_lds(k_yz_dens<T2>, lds));
+  k_yz_dens<T2><<<dim3(p->nr1, ngrp), kYZ, lds, s>>>(g, p->f[1], p->f[2], twid<T2>(p, 1), twid<T2>(p, 2),
+                                                     (const T2 *)p->d_stripe_buf, nb, bpg, occ2,
+                                                     (double *)p->d_dens_part);
+  // rho[zy][x] = sum_group part[group][x][zy]
+  const int A = p->nr1, B = (int)n23;
+  dim3 grid((B + 31) / 32, (A + 31) / 32);
+  k_transpose_sum<double, double><<<grid, 256, 0, s>>>(A, B, ngrp, (const double *)p->d_dens_part, rho);
   PW_HIP(hipGetLastError());
   return PW_OK;
 }

@@ -123,6 +123,26 @@ static int fftadjustgrd(int nr) {
   return -1;
 }
 
+// Position of element e in the digit-reversed order of a mixed-radix
+// decimation-in-time transform with the stage radices f.radix[0..nstage):
+// the element index of position p reads p's digits (most significant digit
+// for the last stage) from the least significant end.
+std::vector<int> digit_reversal_positions(const Fft1D &f) {
+  std::vector<int> pos(f.n);
+  for (int p = 0; p < f.n; ++p) {
+    int rem = p, size = f.n, idx = 0, mult = 1;
+    for (int st = f.nstage - 1; st >= 0; --st) {
+      size /= f.radix[st];
+      const int d = rem / size;
+      rem %= size;
+      idx += d * mult;
+      mult *= f.radix[st];
+    }
+    pos[idx] = p;
+  }
+  return pos;
+}
+
 bool fft_factor(int n, Fft1D &f) {
   f = Fft1D();
   f.n = n;
@@ -268,6 +288,17 @@ static int build_plan(pw_plan *p, const double *rbas, double epw_psi, double epw
       p->zlist.push_back(z);
     }
   }
+  // positions of the (y, z) plane in digit-reversed order, the input order
+  // of the in-place decimation-in-time transforms of the yz pass
+  const std::vector<int> py = digit_reversal_positions(p->f[1]), pz = digit_reversal_positions(p->f[2]);
+  p->stripe_pos.assign(p->nstripe, 0);
+  for (int z = 0; z < p->nr3; ++z)
+    for (int y = 0; y < p->nr2; ++y) {
+      const int s = p->yz2stripe[y + p->nr2 * z];
+      if (s >= 0) p->stripe_pos[s] = pz[z] * p->nr2 + py[y];
+    }
+  p->zpos.assign(p->nzp, 0);
+  for (int i = 0; i < p->nzp; ++i) p->zpos[i] = pz[p->zlist[i]] * p->nr2;
   return PW_OK;
 }
 
@@ -290,6 +321,8 @@ static int upload_plan(pw_plan *p) {
   PW_TRY(upload(&p->d_stripe_s, p->stripe_s));
   PW_TRY(upload(&p->d_zlist, p->zlist));
   PW_TRY(upload(&p->d_yz2stripe, p->yz2stripe));
+  PW_TRY(upload(&p->d_stripe_pos, p->stripe_pos));
+  PW_TRY(upload(&p->d_zpos, p->zpos));
   PW_TRY(upload(&p->d_z2iz, p->z2iz));
   PW_TRY(upload(&p->d_g2, p->g2));
   PW_TRY(upload(&p->d_gvec, p->gvec));
@@ -356,7 +389,7 @@ extern "C" int pw_plan_destroy(pw_plan *p) {
   if (!p) return PW_OK;
   void *ptrs[] = {p->d_minusg, p->d_igvec, p->d_stripe_off, p->d_stripe_g, p->d_stripe_x,
                   p->d_stripe_s, p->d_zlist, p->d_yz2stripe, p->d_z2iz, p->d_g2, p->d_gvec, p->d_stripe_buf,
-                  p->d_plane_buf, p->d_dens_part, p->d_tw[0], p->d_tw[1], p->d_tw[2], p->d_twf[0],
+                  p->d_plane_buf, p->d_dens_part, p->d_vt, p->d_stripe_pos, p->d_zpos, p->d_tw[0], p->d_tw[1], p->d_tw[2], p->d_twf[0],
                   p->d_twf[1], p->d_twf[2]};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
