@@ -107,7 +107,9 @@ struct pw_plan {
   void *d_vt = nullptr; size_t vt_bytes = 0;   // V(r) as [x][z][y] in the work precision
 };
 
-int plan_work(pw_plan *p, int nbatch, int prec);
+// stripe work buffer for nbatch bands (and the z-plane buffer of the full-grid
+// transforms when planes is set)
+int plan_work(pw_plan *p, int nbatch, int prec, bool planes);
 int plan_device(pw_plan *p);   // uploads maps and twiddles on first use
 
 // FFT pipelines (pw_fft.hip).  T selects the arithmetic (FP64 / FP32).
@@ -120,12 +122,23 @@ int fft_vpsi(pw_plan *p, int nb, const double2 *psi, const double *v, double2 *h
              int add_kin, int prec, hipStream_t s);
 // rho(r) = sum_b s1[b] |psi_b(r)|^2 + s2[b] Re psi_b(r)^2; occ2 = (s1,s2) per band
 int fft_density(pw_plan *p, int nb, const double2 *psi, const double *occ2_dev,
-                double *rho, int prec, hipStream_t s);
+                double *rho, int prec, hipStream_t s,
+                int target_blocks = 512);
 
 // ---------------------------------------------------------------------------
 // WAVES state
 // ---------------------------------------------------------------------------
 struct KernelTimer;
+
+struct OrthoGraphKey {
+  double lo = 0.0, hi = 0.0;
+  int nlev = 0, nns = 0, batch = 0;
+  const void *lam = nullptr, *chichi = nullptr, *psipsi = nullptr, *chipsi = nullptr, *work = nullptr;
+  bool operator==(const OrthoGraphKey &o) const {
+    return lo == o.lo && hi == o.hi && nlev == o.nlev && nns == o.nns && batch == o.batch && lam == o.lam &&
+           chichi == o.chichi && psipsi == o.psipsi && chipsi == o.chipsi && work == o.work;
+  }
+};
 
 struct pw_waveset {
   pw_plan *plan = nullptr;
@@ -160,6 +173,9 @@ struct pw_waveset {
   double lz_lo = 0.0, lz_hi = 0.0;
   bool lz_valid = false;
   int ortho_lanczos = 0;       // Lanczos runs in the last ORTHO_X call
+  // launch shapes chosen by pw_waves_autotune
+  int tune_split_waves = 2048;  // waves of a G-split GEMM launch
+  int tune_dens_blocks = 512;   // workgroups of the density yz pass
   std::vector<double> force;   // 3*nat
   std::vector<double> eig;     // nb
   std::vector<double> mat, omat, oomat;
@@ -197,6 +213,9 @@ struct pw_waveset {
   // orthogonalisation work (device, FP64)
   double *d_ortho = nullptr;
   size_t ortho_cap = 0;
+  // captured graph of the ORTHO_X preparation and first iteration batch
+  void *ortho_exec = nullptr;
+  OrthoGraphKey ortho_key;
   hipStream_t stream = 0;
   bool own_stream = false;
   // timing
@@ -238,5 +257,6 @@ int ws_part(pw_waveset *ws, size_t bytes);
 int ws_small(pw_waveset *ws, size_t bytes);
 
 // orthogonalisation (pw_ortho.hip)
+void ortho_graph_release(pw_waveset *ws);
 int ortho_x_device(pw_waveset *ws, double *d_lam, const double *d_chichi,
                    const double *d_psipsi, const double *d_chipsi, int *iter);

@@ -347,48 +347,32 @@ __device__ inline void store_zpencils(const Geo &g, T2 *planes_band, int y, int 
 // (the reversal of a length follows its radix sequence, Fft1D::radix).
 constexpr int kYZ = 512;
 
-template <typename T2, int R>
-__device__ inline void dit_stage(T2 *buf, int n, int L, int npen, int es, int ps, const int *pmap,
-                                 const T2 *__restrict__ tw, int sign, bool pen_fast) {
-  const int Lp = L / R, nbf = n / R, total = nbf * npen, tws = n / L;
-  for (int w = threadIdx.x; w < total; w += kYZ) {
-    int pen, b;
-    if (pen_fast) { pen = w % npen; b = w / npen; }
-    else { b = w % nbf; pen = w / nbf; }
-    const int g = b / Lp, j = b % Lp;
-    T2 *base = buf + (pmap ? pmap[pen] : pen * ps) + (g * L + j) * es;
-    T2 v[R];
-#pragma unroll
-    for (int q = 0; q < R; ++q) v[q] = base[q * Lp * es];
-    if (Lp > 1) {
-#pragma unroll
-      for (int q = 1; q < R; ++q) {
-        T2 t = tw[j * q * tws];
-        if (sign < 0) t.y = -t.y;
-        v[q] = cmul(v[q], t);
-      }
-    }
-    dft<T2, R>(v, sign);
-#pragma unroll
-    for (int q = 0; q < R; ++q) base[q * Lp * es] = v[q];
-  }
-  __syncthreads();
-}
+// quotient of w < 2^16 by 0 < d < 2^16 from m = ceil(2^32/d) (exact in that range, d = 1 included)
+__device__ inline unsigned fdiv(unsigned w, unsigned long long m) { return (unsigned)(((unsigned long long)w * m) >> 32); }
+__device__ inline unsigned long long fmagic(unsigned d) { return (0x100000000ull + d - 1) / d; }
 
-template <typename T2, int R>
-__device__ inline void dif_stage(T2 *buf, int n, int L, int npen, int es, int ps, const int *pmap,
-                                 const T2 *__restrict__ tw, int sign, bool pen_fast) {
+// one in-place butterfly stage; dif=false: twiddle then DFT (DIT), dif=true:
+// DFT then twiddle (DIF).  tw: LDS copy of exp(+2 pi i m/n), m < n.
+template <typename T2, int R, bool DIF>
+__device__ inline void ip_stage(T2 *buf, int n, int L, int npen, int es, int ps, const int *pmap,
+                                const T2 *tw, int sign, bool pen_fast) {
   const int Lp = L / R, nbf = n / R, total = nbf * npen, tws = n / L;
+  const unsigned long long mpen = fmagic(npen), mnbf = fmagic(nbf), mlp = fmagic(Lp);
   for (int w = threadIdx.x; w < total; w += kYZ) {
     int pen, b;
-    if (pen_fast) { pen = w % npen; b = w / npen; }
-    else { b = w % nbf; pen = w / nbf; }
-    const int g = b / Lp, j = b % Lp;
+    if (pen_fast) {
+      b = (int)fdiv(w, mpen);
+      pen = w - b * npen;
+    } else {
+      pen = (int)fdiv(w, mnbf);
+      b = w - pen * nbf;
+    }
+    const int g = (int)fdiv(b, mlp), j = b - g * Lp;
     T2 *base = buf + (pmap ? pmap[pen] : pen * ps) + (g * L + j) * es;
     T2 v[R];
 #pragma unroll
     for (int q = 0; q < R; ++q) v[q] = base[q * Lp * es];
-    dft<T2, R>(v, sign);
+    if (DIF) dft<T2, R>(v, sign);
     if (Lp > 1) {
 #pragma unroll
       for (int q = 1; q < R; ++q) {
@@ -397,6 +381,7 @@ __device__ inline void dif_stage(T2 *buf, int n, int L, int npen, int es, int ps
         v[q] = cmul(v[q], t);
       }
     }
+    if (!DIF) dft<T2, R>(v, sign);
 #pragma unroll
     for (int q = 0; q < R; ++q) base[q * Lp * es] = v[q];
   }
@@ -422,7 +407,7 @@ __device__ void fft_dit(T2 *buf, const Fft1D &f, int npen, int es, int ps, const
   for (int st = 0; st < f.nstage; ++st) {
     const int r = f.radix[st];
     L *= r;
-#define PW_CALL(R) dit_stage<T2, R>(buf, f.n, L, npen, es, ps, pmap, tw, sign, pen_fast)
+#define PW_CALL(R) ip_stage<T2, R, false>(buf, f.n, L, npen, es, ps, pmap, tw, sign, pen_fast)
     PW_RADIX_SWITCH(r, PW_CALL)
 #undef PW_CALL
   }
@@ -434,7 +419,7 @@ __device__ void fft_dif(T2 *buf, const Fft1D &f, int npen, int es, int ps, const
   int L = f.n;
   for (int st = f.nstage - 1; st >= 0; --st) {
     const int r = f.radix[st];
-#define PW_CALL(R) dif_stage<T2, R>(buf, f.n, L, npen, es, ps, pmap, tw, sign, pen_fast)
+#define PW_CALL(R) ip_stage<T2, R, true>(buf, f.n, L, npen, es, ps, pmap, tw, sign, pen_fast)
     PW_RADIX_SWITCH(r, PW_CALL)
 #undef PW_CALL
     L /= r;
@@ -466,11 +451,14 @@ __global__ void __launch_bounds__(kYZ) k_yz_vpsi(Geo g, Fft1D fy, Fft1D fz, cons
   extern __shared__ unsigned char smem_raw[];
   const int n23 = g.nr2 * g.nr3;
   T2 *pl = reinterpret_cast<T2 *>(smem_raw);
-  int *zpos = reinterpret_cast<int *>(pl + n23);
+  T2 *ty = pl + n23, *tz = ty + g.nr2;
+  int *zpos = reinterpret_cast<int *>(tz + g.nr3);
   const int x = blockIdx.x, band = blockIdx.y;
   for (int i = threadIdx.x; i < g.nzp; i += kYZ) zpos[i] = g.zpos[i];
+  for (int i = threadIdx.x; i < g.nr2; i += kYZ) ty[i] = twy[i];
+  for (int i = threadIdx.x; i < g.nr3; i += kYZ) tz[i] = twz[i];
   T2 *col = stripes + ((size_t)band * g.nr1 + x) * g.nstripe;
-  yz_forward<T2>(g, fy, fz, twy, twz, col, pl, zpos);
+  yz_forward<T2>(g, fy, fz, ty, tz, col, pl, zpos);
   const T *vx = vt + (size_t)x * n23;
   for (int i = threadIdx.x; i < n23; i += kYZ) {
     const T vv = vx[i];
@@ -478,8 +466,8 @@ __global__ void __launch_bounds__(kYZ) k_yz_vpsi(Geo g, Fft1D fy, Fft1D fz, cons
     pl[i].y *= vv;
   }
   __syncthreads();
-  fft_dif<T2>(pl, fz, g.nr2, g.nr2, 1, nullptr, twz, -1, true);
-  fft_dif<T2>(pl, fy, g.nzp, 1, 0, zpos, twy, -1, false);
+  fft_dif<T2>(pl, fz, g.nr2, g.nr2, 1, nullptr, tz, -1, true);
+  fft_dif<T2>(pl, fy, g.nzp, 1, 0, zpos, ty, -1, false);
   for (int s = threadIdx.x; s < g.nstripe; s += kYZ) col[s] = pl[g.stripe_pos[s]];
 }
 
@@ -493,16 +481,19 @@ __global__ void __launch_bounds__(kYZ) k_yz_dens(Geo g, Fft1D fy, Fft1D fz, cons
   const int n23 = g.nr2 * g.nr3;
   double *acc = reinterpret_cast<double *>(smem_raw);
   T2 *pl = reinterpret_cast<T2 *>(acc + n23);
-  int *zpos = reinterpret_cast<int *>(pl + n23);
+  T2 *ty = pl + n23, *tz = ty + g.nr2;
+  int *zpos = reinterpret_cast<int *>(tz + g.nr3);
   const int x = blockIdx.x, grp = blockIdx.y;
   const int bstart = grp * bpg, bend = min(nb, bstart + bpg);
   for (int i = threadIdx.x; i < g.nzp; i += kYZ) zpos[i] = g.zpos[i];
+  for (int i = threadIdx.x; i < g.nr2; i += kYZ) ty[i] = twy[i];
+  for (int i = threadIdx.x; i < g.nr3; i += kYZ) tz[i] = twz[i];
   for (int i = threadIdx.x; i < n23; i += kYZ) acc[i] = 0.0;
   for (int band = bstart; band < bend; ++band) {
     const double s1 = occ2[2 * band], s2 = occ2[2 * band + 1];
     if (s1 == 0.0 && s2 == 0.0) continue;   // uniform across the workgroup
     __syncthreads();
-    yz_forward<T2>(g, fy, fz, twy, twz, stripes + ((size_t)band * g.nr1 + x) * g.nstripe, pl, zpos);
+    yz_forward<T2>(g, fy, fz, ty, tz, stripes + ((size_t)band * g.nr1 + x) * g.nstripe, pl, zpos);
     for (int i = threadIdx.x; i < n23; i += kYZ) {
       const double re = pl[i].x, im = pl[i].y;
       // Re(psi (s1 conj(psi) + s2 psi)) = s1 |psi|^2 + s2 (re^2 - im^2)
@@ -681,7 +672,7 @@ template <typename T2>
 int vpsi_t(pw_plan *p, int nb, const double2 *psi, const double *v, double2 *hpsi, int add_kin,
            hipStream_t s) {
   typedef typename Prec<T2>::T T;
-  const size_t lds = (size_t)p->nr2 * p->nr3 * sizeof(T2) + (size_t)p->nzp * sizeof(int);
+  const size_t lds = ((size_t)p->nr2 * p->nr3 + p->nr2 + p->nr3) * sizeof(T2) + (size_t)p->nzp * sizeof(int);
   PW_TRY(yz_fits(p, lds));
   Geo g = make_geo(p);
   PW_TRY(vt_of<T2>(p, v, s));
@@ -694,15 +685,16 @@ int vpsi_t(pw_plan *p, int nb, const double2 *psi, const double *v, double2 *hps
 }
 
 template <typename T2>
-int density_t(pw_plan *p, int nb, const double2 *psi, const double *occ2, double *rho,
+int density_t(pw_plan *p, int nb, const double2 *psi, const double *occ2, double *rho, int target_blocks,
               hipStream_t s) {
   const size_t n23 = (size_t)p->nr2 * p->nr3;
-  const size_t lds = n23 * (sizeof(double) + sizeof(T2)) + (size_t)p->nzp * sizeof(int);
+  const size_t lds = n23 * (sizeof(double) + sizeof(T2)) + (size_t)(p->nr2 + p->nr3) * sizeof(T2) +
+                     (size_t)p->nzp * sizeof(int);
   PW_TRY(yz_fits(p, lds));
   Geo g = make_geo(p);
   PW_TRY(run_forward_x<T2>(p, g, nb, psi, s));
-  // band groups so that the grid holds about two workgroups per CU
-  int ngrp = std::max(1, std::min(nb, (512 + p->nr1 - 1) / p->nr1));
+  // band groups so that the grid holds about target_blocks workgroups
+  int ngrp = std::max(1, std::min(nb, (target_blocks + p->nr1 - 1) / p->nr1));
   const int bpg = (nb + ngrp - 1) / ngrp;
   ngrp = (nb + bpg - 1) / bpg;
   const size_t need = (size_t)ngrp * p->nrl * sizeof(double);
@@ -760,7 +752,7 @@ static int chunk_of(const pw_plan *p, int prec) {
 int fft_vpsi(pw_plan *p, int nb, const double2 *psi, const double *v, double2 *hpsi, int add_kin,
              int prec, hipStream_t s) {
   const int c = chunk_of(p, prec);
-  PW_TRY(plan_work(p, std::min(nb, c), prec));
+  PW_TRY(plan_work(p, std::min(nb, c), prec, false));
   for (int b0 = 0; b0 < nb; b0 += c) {
     const int n = std::min(c, nb - b0);
     const double2 *ps = psi + (size_t)b0 * p->ngl;
@@ -772,17 +764,17 @@ int fft_vpsi(pw_plan *p, int nb, const double2 *psi, const double *v, double2 *h
 }
 
 int fft_density(pw_plan *p, int nb, const double2 *psi, const double *occ2, double *rho, int prec,
-                hipStream_t s) {
+                hipStream_t s, int target_blocks) {
   if (nb > chunk_of(p, prec))
     return pw_fail(PW_ERR_UNSUPPORTED, "WAVES_DENSITY: band set exceeds the work-buffer budget");
-  PW_TRY(plan_work(p, nb, prec));
-  if (prec == PW_PREC_FP64) return density_t<double2>(p, nb, psi, occ2, rho, s);
-  return density_t<float2>(p, nb, psi, occ2, rho, s);
+  PW_TRY(plan_work(p, nb, prec, false));
+  if (prec == PW_PREC_FP64) return density_t<double2>(p, nb, psi, occ2, rho, target_blocks, s);
+  return density_t<float2>(p, nb, psi, occ2, rho, target_blocks, s);
 }
 
 int fft_gtor_full(pw_plan *p, int nfft, const double2 *fofg, void *fofr, int prec, hipStream_t s) {
   const int c = chunk_of(p, prec);
-  PW_TRY(plan_work(p, std::min(nfft, c), prec));
+  PW_TRY(plan_work(p, std::min(nfft, c), prec, true));
   for (int b0 = 0; b0 < nfft; b0 += c) {
     const int n = std::min(c, nfft - b0);
     if (prec == PW_PREC_FP64)
@@ -795,7 +787,7 @@ int fft_gtor_full(pw_plan *p, int nfft, const double2 *fofg, void *fofr, int pre
 
 int fft_rtog_full(pw_plan *p, int nfft, double2 *fofg, const void *fofr, int prec, hipStream_t s) {
   const int c = chunk_of(p, prec);
-  PW_TRY(plan_work(p, std::min(nfft, c), prec));
+  PW_TRY(plan_work(p, std::min(nfft, c), prec, true));
   for (int b0 = 0; b0 < nfft; b0 += c) {
     const int n = std::min(c, nfft - b0);
     if (prec == PW_PREC_FP64)

@@ -1005,7 +1005,7 @@ int run_rowgemm(pw_waveset *ws, const Gen &gen, int nrow, const double2 *psi, do
   int nblk, tb;
   band_tiles(nbh, &nblk, &tb);
   const int tr = (nrow + 16 * NW - 1) / (16 * NW);
-  const int want = std::max(1, 2048 / NW / std::max(1, tr * tb));
+  const int want = std::max(1, ws->tune_split_waves / NW / std::max(1, tr * tb));
   int ns = std::min(want, (ngl + KC - 1) / KC);
   const int gsplit = ((ngl + ns - 1) / ns + KC - 1) / KC * KC;
   ns = (ngl + gsplit - 1) / gsplit;
@@ -1054,7 +1054,7 @@ int run_rowgemm_bf(pw_waveset *ws, const Gen &gen, int nrow, const double2 *psi,
   int nblk, tb;
   band_tiles(nbh, &nblk, &tb);
   const int tr = (nrow + 16 * NW - 1) / (16 * NW);
-  const int want = std::max(1, 2048 / NW / std::max(1, tr * tb));
+  const int want = std::max(1, ws->tune_split_waves / NW / std::max(1, tr * tb));
   int ns = std::min(want, (ngl + KB - 1) / KB);
   const int gsplit = ((ngl + ns - 1) / ns + KB - 1) / KB * KB;
   ns = (ngl + gsplit - 1) / gsplit;

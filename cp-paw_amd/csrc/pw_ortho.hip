@@ -469,6 +469,70 @@ int lanczos_bounds(pw_waveset *ws, const double *d_b, double *d_out, double *lo,
 
 }  // namespace
 
+namespace {
+
+struct OrthoParams {
+  double lo, hi, q;
+  int nlev, nns;
+};
+
+// M, A^(2^j) and the acceptance test (everything before the outer loop)
+void enqueue_prep(pw_waveset *ws, const OrthoWork &w, const OrthoParams &P, double **mout) {
+  const int n = ws->nb;
+  const int nblk = (int)(((size_t)n * n + 255) / 256);
+  k_state_init<<<1, 64, 0, ws->stream>>>(w.st);
+  // M = (B + qI)^-1 by Newton-Schulz, X0 = 2/(lo+hi+2q) I
+  k_scaled_identity<<<nblk, 256, 0, ws->stream>>>(n, 2.0 / (P.lo + P.hi + 2.0 * P.q), w.m0);
+  double *mc = w.m0, *mn = w.m1;
+  for (int k = 0; k < P.nns; ++k) {
+    gemm(ws, w.bs, 0, mc, 0, w.t1, 1.0, mc, P.q);       // T = (B+qI) X
+    gemm(ws, mc, 0, w.t1, 0, mn, -1.0, mc, 2.0);        // X' = 2X - X T
+    std::swap(mc, mn);
+  }
+  // A^(2^j), j = 0..nlev-1
+  k_cayley<<<nblk, 256, 0, ws->stream>>>(n, P.q, mc, w.apow[0]);
+  for (int j = 1; j < P.nlev; ++j) gemm(ws, w.apow[j - 1], 0, w.apow[j - 1], 0, w.apow[j]);
+  gemm(ws, w.bs, 0, mc, 0, w.t1, 1.0, mc, P.q);         // (B+qI) M
+  k_prep_part<<<nblk, 256, 0, ws->stream>>>(n, w.t1, w.apow[P.nlev - 1], w.red);
+  k_prep_check<<<1, 256, 0, ws->stream>>>(n, nblk, w.red, 1.0e-11, 1.0e-5, w.st);
+  *mout = mc;
+}
+
+// outer iterations it0..it1 (1-based, src/paw_waves2.f90:1905-2010)
+void enqueue_iters(pw_waveset *ws, const OrthoWork &w, const OrthoParams &P, const double *mc,
+                   double *const lbuf[2], const double *d_chichi, const double *d_psipsi,
+                   const double *d_chipsi, int it0, int it1) {
+  const int n = ws->nb;
+  const int nblk = (int)(((size_t)n * n + 255) / 256);
+  const double eps = 1.0e-8;
+  for (int it = it0; it <= it1; ++it) {
+    const double *lam = lbuf[(it - 1) & 1];
+    double *lnew = lbuf[it & 1];
+    // HAUX = CHICHI LAMBDA + 2 CHIPSI ; T = LAMBDA^T HAUX
+    gemm(ws, d_chichi, 0, lam, 0, w.t1, 1.0, d_chipsi, 2.0, w.st);
+    gemm(ws, lam, 1, w.t1, 0, w.t2, 1.0, nullptr, 0.0, w.st);
+    k_gamma<<<nblk, 256, 0, ws->stream>>>(n, w.t2, d_psipsi, w.gam, w.red, w.st);
+    k_check<<<1, 256, 0, ws->stream>>>(nblk, w.red, eps, it, w.st);
+    // X = S^-1(GAMN): X0 = 2q M G M ; X <- X + A_j X A_j
+    gemm(ws, w.gam, 0, mc, 0, w.t1, 1.0, nullptr, 0.0, w.st);
+    gemm(ws, mc, 0, w.t1, 0, w.x0, 2.0 * P.q, nullptr, 0.0, w.st);
+    double *xc = w.x0, *xn = w.x1;
+    for (int j = 0; j < P.nlev; ++j) {
+      gemm(ws, xc, 0, w.apow[j], 0, w.t1, 1.0, nullptr, 0.0, w.st);
+      gemm(ws, w.apow[j], 0, w.t1, 0, xn, 1.0, xc, 1.0, w.st);
+      std::swap(xc, xn);
+    }
+    k_lam_step<<<nblk, 256, 0, ws->stream>>>(n, w.occ, xc, lam, lnew, w.st);
+  }
+}
+
+}  // namespace
+
+void ortho_graph_release(pw_waveset *ws) {
+  if (ws->ortho_exec) (void)hipGraphExecDestroy((hipGraphExec_t)ws->ortho_exec);
+  ws->ortho_exec = nullptr;
+}
+
 int ortho_x_device(pw_waveset *ws, double *d_lam, const double *d_chichi, const double *d_psipsi,
                    const double *d_chipsi, int *iter) {
   ScopeTimer tm(ws, "ortho_x");
@@ -476,15 +540,17 @@ int ortho_x_device(pw_waveset *ws, double *d_lam, const double *d_chichi, const 
   const size_t n2 = (size_t)n * n;
   const int nblk = (int)((n2 + 255) / 256);
   OrthoWork w;
+  const double *old_base = ws->d_ortho;
   PW_TRY(ortho_work(ws, w));
+  if (ws->d_ortho != old_base) ortho_graph_release(ws);
   PW_HIP(hipMemcpyAsync(w.occ, ws->occ.data(), n * sizeof(double), hipMemcpyHostToDevice, ws->stream));
   k_symm<<<nblk, 256, 0, ws->stream>>>(n, d_chipsi, w.bs);
   PW_HIP(hipGetLastError());
   ws->ortho_lanczos = 0;
+  double *lbuf[2] = {d_lam, w.lam2};
   for (int attempt = 0; attempt < 2; ++attempt) {
-    k_state_init<<<1, 64, 0, ws->stream>>>(w.st);
     // spectral bounds of B: inner Ritz bounds, reused from the previous call
-    // until the acceptance test below rejects them
+    // until the acceptance test rejects them
     if (!ws->lz_valid || attempt > 0) {
       double lo0, hi0;
       PW_TRY(lanczos_bounds(ws, w.bs, w.bounds, &lo0, &hi0));
@@ -493,76 +559,67 @@ int ortho_x_device(pw_waveset *ws, double *d_lam, const double *d_chichi, const 
       ws->lz_valid = true;
       ws->ortho_lanczos++;
     }
-    double lo = ws->lz_lo, hi = ws->lz_hi;
-    if (!(lo > 0.0) || !(hi >= lo)) {
+    OrthoParams P;
+    P.lo = ws->lz_lo;
+    P.hi = ws->lz_hi;
+    if (!(P.lo > 0.0) || !(P.hi >= P.lo)) {
       ws->lz_valid = false;
       return pw_fail(PW_ERR_NOCONV, "WAVES_ORTHO_X: sym(CHIPSI) is not positive definite");
     }
-    lo *= 0.97;
-    hi *= 1.03;
-    const double q = std::sqrt(lo * hi);
-    const double kap = hi / lo;
+    P.lo *= 0.97;
+    P.hi *= 1.03;
+    P.q = std::sqrt(P.lo * P.hi);
+    const double kap = P.hi / P.lo;
     const double rho = (std::sqrt(kap) - 1.0) / (std::sqrt(kap) + 1.0);
-    int nlev = 1;
+    P.nlev = 1;
     if (rho > 1.0e-300) {
       const double need = std::log(1.0e-17) / std::log(rho);
-      nlev = std::max(1, (int)std::ceil(std::log2(std::max(need, 1.0)))) + 1;
+      P.nlev = std::max(1, (int)std::ceil(std::log2(std::max(need, 1.0)))) + 1;
     }
-    if (nlev > kMaxLevels) return pw_fail(PW_ERR_NOCONV, "WAVES_ORTHO_X: CHIPSI too ill-conditioned");
-    // M = (B + qI)^-1 by Newton-Schulz, X0 = 2/(lo+hi+2q) I
-    const double kp = (hi + q) / (lo + q);
+    if (P.nlev > kMaxLevels) return pw_fail(PW_ERR_NOCONV, "WAVES_ORTHO_X: CHIPSI too ill-conditioned");
+    const double kp = (P.hi + P.q) / (P.lo + P.q);
     const double e0 = (kp - 1.0) / (kp + 1.0);
-    int nns = 1;
-    if (e0 > 1.0e-300) nns = (int)std::ceil(std::log2(std::max(1.0, std::log(1.0e-18) / std::log(e0)))) + 1;
-    ws->ortho_levels = nlev;
-    ws->ortho_ns = nns;
-    k_scaled_identity<<<nblk, 256, 0, ws->stream>>>(n, 2.0 / (lo + hi + 2.0 * q), w.m0);
-    double *mc = w.m0, *mn = w.m1;
-    for (int k = 0; k < nns; ++k) {
-      PW_TRY(gemm(ws, w.bs, 0, mc, 0, w.t1, 1.0, mc, q));       // T = (B+qI) X
-      PW_TRY(gemm(ws, mc, 0, w.t1, 0, mn, -1.0, mc, 2.0));      // X' = 2X - X T
-      std::swap(mc, mn);
-    }
-    // A^(2^j), j = 0..nlev-1
-    k_cayley<<<nblk, 256, 0, ws->stream>>>(n, q, mc, w.apow[0]);
-    for (int j = 1; j < nlev; ++j) PW_TRY(gemm(ws, w.apow[j - 1], 0, w.apow[j - 1], 0, w.apow[j]));
-    PW_TRY(gemm(ws, w.bs, 0, mc, 0, w.t1, 1.0, mc, q));         // (B+qI) M
-    k_prep_part<<<nblk, 256, 0, ws->stream>>>(n, w.t1, w.apow[nlev - 1], w.red);
-    k_prep_check<<<1, 256, 0, ws->stream>>>(n, nblk, w.red, 1.0e-11, 1.0e-5, w.st);
-    PW_HIP(hipGetLastError());
-    const double eps = 1.0e-8;
-    // LAMBDA ping-pongs between the caller's buffer (even number of updates) and lam2
-    double *lbuf[2] = {d_lam, w.lam2};
-    int launched = 0;
-    int batch = std::max(2, std::min(ws->ortho_iter > 0 ? ws->ortho_iter : 8, kMaxIter));
-    OrthoState hs{};
-    while (launched < kMaxIter) {
-      const int nit = std::min(batch, kMaxIter - launched);
-      for (int it = launched + 1; it <= launched + nit; ++it) {
-        const double *lam = lbuf[(it - 1) & 1];
-        double *lnew = lbuf[it & 1];
-        // HAUX = CHICHI LAMBDA + 2 CHIPSI ; T = LAMBDA^T HAUX
-        PW_TRY(gemm(ws, d_chichi, 0, lam, 0, w.t1, 1.0, d_chipsi, 2.0, w.st));
-        PW_TRY(gemm(ws, lam, 1, w.t1, 0, w.t2, 1.0, nullptr, 0.0, w.st));
-        k_gamma<<<nblk, 256, 0, ws->stream>>>(n, w.t2, d_psipsi, w.gam, w.red, w.st);
-        k_check<<<1, 256, 0, ws->stream>>>(nblk, w.red, eps, it, w.st);
-        // X = S^-1(GAMN): X0 = 2q M G M ; X <- X + A_j X A_j
-        PW_TRY(gemm(ws, w.gam, 0, mc, 0, w.t1, 1.0, nullptr, 0.0, w.st));
-        PW_TRY(gemm(ws, mc, 0, w.t1, 0, w.x0, 2.0 * q, nullptr, 0.0, w.st));
-        double *xc = w.x0, *xn = w.x1;
-        for (int j = 0; j < nlev; ++j) {
-          PW_TRY(gemm(ws, xc, 0, w.apow[j], 0, w.t1, 1.0, nullptr, 0.0, w.st));
-          PW_TRY(gemm(ws, w.apow[j], 0, w.t1, 0, xn, 1.0, xc, 1.0, w.st));
-          std::swap(xc, xn);
-        }
-        k_lam_step<<<nblk, 256, 0, ws->stream>>>(n, w.occ, xc, lam, lnew, w.st);
+    P.nns = 1;
+    if (e0 > 1.0e-300) P.nns = (int)std::ceil(std::log2(std::max(1.0, std::log(1.0e-18) / std::log(e0)))) + 1;
+    ws->ortho_levels = P.nlev;
+    ws->ortho_ns = P.nns;
+    const int batch = std::max(2, std::min(ws->ortho_iter > 0 ? ws->ortho_iter : 8, kMaxIter));
+    // the preparation and the first batch of iterations are one captured graph,
+    // replayed while the bounds, the batch and the buffers stay the same
+    const OrthoGraphKey key{P.lo, P.hi, P.nlev, P.nns, batch, d_lam, d_chichi, d_psipsi, d_chipsi, ws->d_ortho};
+    double *mc = (P.nns & 1) ? w.m1 : w.m0;
+    if (!ws->ortho_exec || !(ws->ortho_key == key)) {
+      ortho_graph_release(ws);
+      PW_HIP(hipStreamBeginCapture(ws->stream, hipStreamCaptureModeThreadLocal));
+      enqueue_prep(ws, w, P, &mc);
+      enqueue_iters(ws, w, P, mc, lbuf, d_chichi, d_psipsi, d_chipsi, 1, batch);
+      hipGraph_t graph = nullptr;
+      const hipError_t ec = hipStreamEndCapture(ws->stream, &graph);
+      const hipError_t el = hipGetLastError();
+      if (ec != hipSuccess || el != hipSuccess || !graph) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return pw_fail(PW_ERR_HIP, std::string("WAVES_ORTHO_X: graph capture failed: ") +
+                                       hipGetErrorString(ec != hipSuccess ? ec : el));
       }
-      PW_HIP(hipGetLastError());
-      launched += nit;
+      hipGraphExec_t exec = nullptr;
+      const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (ei != hipSuccess) return pw_fail(PW_ERR_HIP, std::string("WAVES_ORTHO_X: graph instantiation failed: ") +
+                                                           hipGetErrorString(ei));
+      ws->ortho_exec = exec;
+      ws->ortho_key = key;
+    }
+    PW_HIP(hipGraphLaunch((hipGraphExec_t)ws->ortho_exec, ws->stream));
+    int launched = batch;
+    OrthoState hs{};
+    for (;;) {
       PW_HIP(hipMemcpyAsync(&hs, w.st, sizeof(OrthoState), hipMemcpyDeviceToHost, ws->stream));
       PW_HIP(hipStreamSynchronize(ws->stream));
-      if (hs.done) break;
-      batch = 4;
+      if (hs.done || launched >= kMaxIter) break;
+      const int nit = std::min(4, kMaxIter - launched);
+      enqueue_iters(ws, w, P, mc, lbuf, d_chichi, d_psipsi, d_chipsi, launched + 1, launched + nit);
+      PW_HIP(hipGetLastError());
+      launched += nit;
     }
     if (hs.code == 3) {
       if (attempt == 0) continue;
@@ -596,6 +653,7 @@ extern "C" int pw_ortho_x(int nb, const double *occ, const double *chichi, const
     return pw_fail(PW_ERR_ARG, "pw_ortho_x: bad argument");
   pw_waveset ws;
   ws.nb = nb;
+  PW_HIP(hipStreamCreate(&ws.stream));
   ws.occ.assign(occ, occ + nb);
   const size_t n2 = (size_t)nb * nb;
   std::vector<double> h(4 * n2);
@@ -609,12 +667,16 @@ extern "C" int pw_ortho_x(int nb, const double *occ, const double *chichi, const
   if (hipMemcpy(d, h.data(), 4 * n2 * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
     rc = pw_fail(PW_ERR_HIP, "pw_ortho_x: upload failed");
   if (rc == PW_OK) rc = ortho_x_device(&ws, d + 3 * n2, d, d + n2, d + 2 * n2, iter);
+  if (rc == PW_OK && hipStreamSynchronize(ws.stream) != hipSuccess)
+    rc = pw_fail(PW_ERR_HIP, "pw_ortho_x: stream synchronisation failed");
   if (rc == PW_OK && hipMemcpy(h.data() + 3 * n2, d + 3 * n2, n2 * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
     rc = pw_fail(PW_ERR_HIP, "pw_ortho_x: download failed");
   if (rc == PW_OK)
     for (int i = 0; i < nb; ++i)
       for (int j = 0; j < nb; ++j) lambda[(size_t)j * nb + i] = h[3 * n2 + (size_t)i * nb + j];
+  ortho_graph_release(&ws);
   (void)hipFree(d);
   if (ws.d_ortho) (void)hipFree(ws.d_ortho);
+  (void)hipStreamDestroy(ws.stream);
   return rc;
 }

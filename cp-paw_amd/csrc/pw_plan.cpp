@@ -345,11 +345,11 @@ static int upload_plan(pw_plan *p) {
 
 int plan_device(pw_plan *p) { return upload_plan(p); }
 
-int plan_work(pw_plan *p, int nbatch, int prec) {
+int plan_work(pw_plan *p, int nbatch, int prec, bool planes) {
   PW_TRY(upload_plan(p));
   size_t es = prec == PW_PREC_FP64 ? sizeof(double2) : sizeof(float2);
   size_t sb = (size_t)nbatch * p->nstripe * p->nr1 * es;
-  size_t pb = (size_t)nbatch * p->nzp * p->nr2 * p->nr1 * es;
+  size_t pb = planes ? (size_t)nbatch * p->nzp * p->nr2 * p->nr1 * es : 0;
   if (sb > p->stripe_bytes) {
     if (p->d_stripe_buf) PW_HIP(hipFree(p->d_stripe_buf));
     PW_HIP(hipMalloc(&p->d_stripe_buf, sb));

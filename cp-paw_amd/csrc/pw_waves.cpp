@@ -335,6 +335,7 @@ extern "C" int pw_waves_create(pw_plan *plan, int nb, int nat, int nsp, const in
 extern "C" int pw_waves_destroy(pw_waveset *ws) {
   if (!ws) return PW_OK;
   ws_flush_timers(ws);
+  ortho_graph_release(ws);
   void *ptrs[] = {ws->d_psi0, ws->d_psim, ws->d_hpsi, ws->d_opsi, ws->d_tmp, ws->d_proj, ws->d_oproj,
                   ws->d_xproj, ws->d_pro, ws->d_ylm, ws->d_mpsi, ws->d_v, ws->d_rho, ws->d_occ2,
                   ws->d_fpm, ws->d_fab, ws->d_domat, ws->d_dhmat, ws->d_atom_isp, ws->d_ipro0,
@@ -627,7 +628,7 @@ extern "C" int pw_waves_etot(pw_waveset *ws) {
   // WAVES_DENSITY
   {
     ScopeTimer tm(ws, "density");
-    PW_TRY(fft_density(p, ws->nbh, ws->d_psi0, ws->d_occ2, ws->d_rho, ws->prec, ws->stream));
+    PW_TRY(fft_density(p, ws->nbh, ws->d_psi0, ws->d_occ2, ws->d_rho, ws->prec, ws->stream, ws->tune_dens_blocks));
   }
   // energy of the fixed local potential: RWEIGHT sum_r V rho
   PW_TRY(k_dot_vrho(ws, &ws->epot));

@@ -18,6 +18,27 @@ _c_dbl_p = ctypes.POINTER(ctypes.c_double)
 _vp = ctypes.c_void_p
 _cp = ctypes.c_char_p
 
+
+class Request(ctypes.Structure):
+    """struct pw_request of include/pwgpu.h."""
+    _fields_ = [("rbas", ctypes.c_double * 9), ("epw_psi", ctypes.c_double), ("epw_rho", ctypes.c_double),
+                ("nb", ctypes.c_int), ("nat", ctypes.c_int), ("nsp", ctypes.c_int), ("lnxx", ctypes.c_int),
+                ("nbarepro", ctypes.c_int), ("prec", ctypes.c_int),
+                ("isp", _c_int_p), ("lnx", _c_int_p), ("lox", _c_int_p),
+                ("pro", _c_dbl_p), ("occ", _c_dbl_p), ("dov", _c_dbl_p), ("dh", _c_dbl_p),
+                ("vofr", _c_dbl_p), ("r", _c_dbl_p), ("psi", _c_dbl_p),
+                ("dt", ctypes.c_double), ("emass", ctypes.c_double), ("emasscg2", ctypes.c_double),
+                ("friction", ctypes.c_double), ("etol", ctypes.c_double),
+                ("nconv", ctypes.c_int), ("max_steps", ctypes.c_int), ("autotune", ctypes.c_int)]
+
+
+class Result(ctypes.Structure):
+    """struct pw_result of include/pwgpu.h."""
+    _fields_ = [("status", ctypes.c_int), ("steps", ctypes.c_int), ("converged", ctypes.c_int),
+                ("etot", ctypes.c_double), ("ekin", ctypes.c_double), ("epot", ctypes.c_double),
+                ("e1c", ctypes.c_double), ("t_submit", ctypes.c_double), ("t_start", ctypes.c_double),
+                ("t_end", ctypes.c_double), ("footprint", ctypes.c_longlong)]
+
 # name -> argument types (all return int status)
 SIGNATURES = {
     "pw_version": [],
@@ -60,6 +81,14 @@ SIGNATURES = {
     "pw_waves_vpsi": [_vp, _vp, _vp],
     "pw_waves_density": [_vp, _vp, _vp],
     "pw_ortho_x": [ctypes.c_int, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_int_p],
+    "pw_waves_autotune": [_vp],
+    "pw_waves_footprint": [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _c_int_p, _c_int_p, ctypes.c_int,
+                           _c_int_p, ctypes.c_int, ctypes.POINTER(ctypes.c_longlong)],
+    "pw_server_create": [ctypes.c_longlong, ctypes.c_int, ctypes.POINTER(_vp)],
+    "pw_server_destroy": [_vp],
+    "pw_server_submit": [_vp, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_longlong)],
+    "pw_server_wait": [_vp, ctypes.c_longlong, ctypes.POINTER(Result)],
+    "pw_server_stats": [_vp, ctypes.POINTER(ctypes.c_longlong), _c_int_p, _c_int_p],
 }
 
 _lib = None

@@ -18,6 +18,24 @@ FP64 = _lib.PW_PREC_FP64
 MIXED = _lib.PW_PREC_MIXED
 
 
+def dh_array(nat, lmnxx, dh_list):
+    """DH(LMNXX,LMNXX,NAT) (numpy [atom][j][i]) from per-atom (LMNX,LMNX) matrices."""
+    a = np.zeros((nat, lmnxx, lmnxx), dtype=np.complex128)
+    for i, d in enumerate(dh_list):
+        n = d.shape[0]
+        a[i, :n, :n] = np.asarray(d).T
+    return a
+
+
+def do_array(nsp, lnxx, do_list):
+    """DO(LNXX,LNXX,NSP) (numpy [species][j][i]) from per-species (LNX,LNX) matrices."""
+    a = np.zeros((nsp, lnxx, lnxx))
+    for s, d in enumerate(do_list):
+        n = d.shape[0]
+        a[s, :n, :n] = np.asarray(d).T
+    return a
+
+
 class Waves:
     def __init__(self, plane, nb, isp, lox_list, prec=FP64):
         self.lib = _lib.load()
@@ -108,20 +126,11 @@ class Waves:
     # -- inputs in reference layouts ------------------------------------------
     def dh_array(self, dh_list):
         """DH(LMNXX,LMNXX,NAT) from a list of per-atom (LMNX,LMNX) matrices."""
-        L = self.lmnxx
-        a = np.zeros((self.nat, L, L), dtype=np.complex128)
-        for i, d in enumerate(dh_list):
-            n = d.shape[0]
-            a[i, :n, :n] = np.asarray(d).T          # [atom, j, i] = DH(i, j, atom)
-        return a
+        return dh_array(self.nat, self.lmnxx, dh_list)
 
     def do_array(self, do_list):
         """DO(LNXX,LNXX,NSP) from a list of per-species (LNX,LNX) matrices."""
-        a = np.zeros((self.nsp, self.lnxx, self.lnxx))
-        for s, d in enumerate(do_list):
-            n = d.shape[0]
-            a[s, :n, :n] = np.asarray(d).T
-        return a
+        return do_array(self.nsp, self.lnxx, do_list)
 
     def set_dh(self, dh_list):
         self.setc8a("DH", self.dh_array(dh_list))

@@ -170,6 +170,65 @@ int pw_ortho_x(int nb, const double *occ, const double *chichi,
                const double *psipsi, const double *chipsi, double *lambda,
                int *iter);
 
+/* ---- launch-shape autotuner and device footprint ---------------------------
+ * No counterpart in the reference (FFTW plans with FFTW_MEASURE,
+ * src/paw_library.f90:2688-2707, are its nearest analogue).
+ * pw_waves_autotune times the G-split width of the projection GEMMs and the
+ * band grouping of the density pass on the waveset's stream (PSI0 and R0
+ * must be set) and keeps the fastest; results are cached per system shape
+ * for the process.  pw_waves_footprint estimates the device bytes a waveset
+ * of this shape holds during a CP step (host-only, no GPU use). */
+int pw_waves_autotune(pw_waveset *ws);
+int pw_waves_footprint(pw_plan *plan, int nb, int nat, int nsp, const int *isp,
+                       const int *lnx, int lnxx, const int *lox, int prec,
+                       long long *bytes);
+
+/* ---- request loop (serving) --------------------------------------------------
+ * No counterpart in the reference, which relaxes one system per run with
+ * damped CP dynamics and the autopilot's energy window
+ * (src/paw_assist.f90:297-400).  The server takes independent relaxation
+ * requests as they arrive, admits them first-fit while their device
+ * footprint fits mem_budget and one of max_active slots is free, and runs
+ * each admitted request's CP loop on its own stream, so requests overlap on
+ * the device.  A request converges when |E(step) - E(step-1)| < etol on
+ * nconv consecutive steps (at most max_steps steps).  All arrays are copied
+ * at submit; layouts as for pw_waves_setr8a/setc8a on the plane-wave set of
+ * the cell (rbas, epw_psi, epw_rho). */
+typedef struct pw_server pw_server;
+typedef struct pw_request {
+  double rbas[9];          /* RBAS(3,3), lattice vectors as columns */
+  double epw_psi, epw_rho;
+  int nb, nat, nsp, lnxx, nbarepro, prec;
+  const int *isp;          /* NAT, 1-based species */
+  const int *lnx;          /* NSP */
+  const int *lox;          /* LNXX*NSP */
+  const double *pro;       /* NGL*NBAREPRO */
+  const double *occ;       /* NB */
+  const double *dov;       /* DO(LNXX,LNXX,NSP) */
+  const double *dh;        /* DH(LMNXX,LMNXX,NAT), complex interleaved */
+  const double *vofr;      /* NRL */
+  const double *r;         /* R(3,NAT) */
+  const double *psi;       /* PSI(NGL,NBH), complex interleaved; PSI0 = PSIM */
+  double dt, emass, emasscg2, friction;
+  double etol;
+  int nconv, max_steps;
+  int autotune;            /* 1: run pw_waves_autotune on admission */
+} pw_request;
+typedef struct pw_result {
+  int status, steps, converged;
+  double etot, ekin, epot, e1c;
+  double t_submit, t_start, t_end;   /* seconds since pw_server_create */
+  long long footprint;               /* bytes reserved while running */
+} pw_result;
+int pw_server_create(long long mem_budget, int max_active, pw_server **srv);
+/* Waits for every submitted request, then stops the workers. */
+int pw_server_destroy(pw_server *srv);
+int pw_server_submit(pw_server *srv, const pw_request *req, long long *id);
+/* Blocks until request id has finished; its record is then released. */
+int pw_server_wait(pw_server *srv, long long id, pw_result *res);
+int pw_server_stats(pw_server *srv, long long *peak_bytes, int *peak_active,
+                    int *pending);
+
 #ifdef __cplusplus
 }
 #endif
