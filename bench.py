@@ -112,6 +112,47 @@ def accuracy(system_name, nsteps):
             "gates": {"dE_per_atom": 1e-6, "dF": 1e-4}, "pass": bool(de < 1e-6 and df < 1e-4)}
 
 
+def serving(nreq, rate, slots, seed=7):
+    """Poisson stream of independent relaxations (Si8/16/32/64, seeded
+    displacements) through the request loop in the mixed form: req/s,
+    p50/p99 latency (submit to result) and, per request, |dE| per atom
+    against the FP64 form run to the same step."""
+    import numpy as np
+    from pwgpu import model, serve
+    rng = np.random.default_rng(seed)
+    sizes = rng.choice([8, 16, 32, 64], size=nreq)
+    gaps = rng.exponential(1.0 / rate, size=nreq)
+    cells = [model.si_supercell(int(n), seed=100 + i, dt=8.0, emass=300.0,
+                                disp_sigma=float(rng.uniform(0.05, 0.1))) for i, n in enumerate(sizes)]
+    opts = dict(friction=0.05, etol=1.0e-5, nconv=3, max_steps=1500)
+    arrays = [serve.request_arrays(c) for c in cells]          # host set-up outside the clock
+    srv = serve.Server(200 << 30, slots)
+    t0 = time.perf_counter()
+    ids = []
+    for c, a, g in zip(cells, arrays, gaps):
+        time.sleep(g)
+        ids.append(srv.submit(c, prec=1, arrays=a, **opts))
+    res = [srv.wait(i) for i in ids]
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    # FP64 form of every request, to the step the mixed form stopped at
+    ref = [srv.submit(c, prec=0, arrays=a, friction=opts["friction"], etol=0.0, nconv=1, max_steps=r["steps"])
+           for c, a, r in zip(cells, arrays, res)]
+    ref = [srv.wait(i) for i in ref]
+    srv.close()
+    lat = np.array([r["t_end"] - r["t_submit"] for r in res])
+    de = [abs(r["etot"] - f["etot"]) / c.nat for r, f, c in zip(res, ref, cells)]
+    return {"requests": nreq, "arrival_rate_per_s": rate, "slots": slots, "form": "mixed",
+            "req_per_s": nreq / wall, "p50_latency_s": float(np.percentile(lat, 50)),
+            "p99_latency_s": float(np.percentile(lat, 99)),
+            "mean_steps": float(np.mean([r["steps"] for r in res])),
+            "converged": int(sum(r["converged"] for r in res)),
+            "peak_active": st["peak_active"], "peak_device_GiB": st["peak_bytes"] / 2**30,
+            "max_dE_per_atom_vs_fp64": float(max(de)),
+            "mix": {f"Si{n}": int(sum(sizes == n)) for n in (8, 16, 32, 64)},
+            "relaxation": "damped CP, emass 300, dt 8, friction 0.05, |dE|<1e-5 Ha on 3 steps"}
+
+
 def run_gpu(system_name, prec, steps, warmup, profile):
     import pwgpu
     from pwgpu import model
@@ -215,6 +256,10 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=8)
     ap.add_argument("--acc-steps", type=int, default=20,
                     help="steps of the mixed-vs-FP64 error measurement (0: skip)")
+    ap.add_argument("--serve-requests", type=int, default=12,
+                    help="requests of the serving measurement (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=4.0, help="Poisson arrival rate, requests/s")
+    ap.add_argument("--serve-slots", type=int, default=4, help="concurrent requests on the GPU")
     ap.add_argument("--profile", action="store_true", help="print per-kernel times to stderr")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (calibrates the HBM "
@@ -269,6 +314,11 @@ def main():
     if peak == BF16_PEAK_TFLOPS:
         roof["split_products"] = SPLIT_PRODUCTS
     acc = accuracy(args.system, args.acc_steps) if (args.acc_steps > 0 and head == 1) else None
+    srv = None
+    if args.serve_requests > 0:
+        log(f"serving: {args.serve_requests} requests at {args.serve_rate}/s, {args.serve_slots} slots ...")
+        srv = serving(args.serve_requests, args.serve_rate, args.serve_slots)
+        log(f"serving: {srv['req_per_s']:.2f} req/s p50 {srv['p50_latency_s']:.2f}s p99 {srv['p99_latency_s']:.2f}s")
     cpu = None
     if not args.no_cpu:
         log("cpu baseline (oracle) ...")
@@ -295,6 +345,8 @@ def main():
     }
     if acc is not None:
         out["mixed_error_vs_fp64"] = acc
+    if srv is not None:
+        out["serving"] = srv
     if 0 in results and 1 in results:
         out["fp64_steps_per_s"] = results[0]["steps_per_s"]
         out["mixed_over_fp64"] = results[1]["steps_per_s"] / results[0]["steps_per_s"]

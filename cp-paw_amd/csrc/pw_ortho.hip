@@ -590,7 +590,7 @@ int ortho_x_device(pw_waveset *ws, double *d_lam, const double *d_chichi, const 
     double *mc = (P.nns & 1) ? w.m1 : w.m0;
     if (!ws->ortho_exec || !(ws->ortho_key == key)) {
       ortho_graph_release(ws);
-      PW_HIP(hipStreamBeginCapture(ws->stream, hipStreamCaptureModeThreadLocal));
+      PW_HIP(hipStreamBeginCapture(ws->stream, hipStreamCaptureModeRelaxed));
       enqueue_prep(ws, w, P, &mc);
       enqueue_iters(ws, w, P, mc, lbuf, d_chichi, d_psipsi, d_chipsi, 1, batch);
       hipGraph_t graph = nullptr;
@@ -653,7 +653,7 @@ extern "C" int pw_ortho_x(int nb, const double *occ, const double *chichi, const
     return pw_fail(PW_ERR_ARG, "pw_ortho_x: bad argument");
   pw_waveset ws;
   ws.nb = nb;
-  PW_HIP(hipStreamCreate(&ws.stream));
+  PW_HIP(hipStreamCreateWithFlags(&ws.stream, hipStreamNonBlocking));
   ws.occ.assign(occ, occ + nb);
   const size_t n2 = (size_t)nb * nb;
   std::vector<double> h(4 * n2);

@@ -326,7 +326,7 @@ extern "C" int pw_waves_create(pw_plan *plan, int nb, int nat, int nsp, const in
   }
   if (rc == PW_OK) rc = ylm_table(ws);
   if (rc != PW_OK) return fail(rc);
-  if (hipStreamCreate(&ws->stream) != hipSuccess) return fail(pw_fail(PW_ERR_HIP, "hipStreamCreate failed"));
+  if (hipStreamCreateWithFlags(&ws->stream, hipStreamNonBlocking) != hipSuccess) return fail(pw_fail(PW_ERR_HIP, "hipStreamCreate failed"));
   ws->own_stream = true;
   *out = ws;
   return PW_OK;
@@ -358,6 +358,9 @@ extern "C" int pw_waves_set_stream(pw_waveset *ws, void *stream) {
 
 extern "C" int pw_waves_setr8(pw_waveset *ws, const char *id, double val) {
   if (!ws || !id) return pw_fail(PW_ERR_ARG, "pw_waves_setr8: null argument");
+  // inputs are copied synchronously; work still queued on the waveset's
+  // (non-blocking) stream must not see them change
+  PW_HIP(hipStreamSynchronize(ws->stream));
   std::string s(id);
   if (s == "TIMESTEP") ws->delt = val;
   else if (s == "EMASS") ws->emass = val;
@@ -369,6 +372,9 @@ extern "C" int pw_waves_setr8(pw_waveset *ws, const char *id, double val) {
 
 extern "C" int pw_waves_setr8a(pw_waveset *ws, const char *id, int len, const double *val) {
   if (!ws || !id || !val) return pw_fail(PW_ERR_ARG, "pw_waves_setr8a: null argument");
+  // inputs are copied synchronously; work still queued on the waveset's
+  // (non-blocking) stream must not see them change
+  PW_HIP(hipStreamSynchronize(ws->stream));
   std::string s(id);
   pw_plan *p = ws->plan;
   auto chk = [&](size_t n) {
@@ -409,6 +415,9 @@ extern "C" int pw_waves_setr8a(pw_waveset *ws, const char *id, int len, const do
 
 extern "C" int pw_waves_setc8a(pw_waveset *ws, const char *id, int len, const double *val) {
   if (!ws || !id || !val) return pw_fail(PW_ERR_ARG, "pw_waves_setc8a: null argument");
+  // inputs are copied synchronously; work still queued on the waveset's
+  // (non-blocking) stream must not see them change
+  PW_HIP(hipStreamSynchronize(ws->stream));
   std::string s(id);
   pw_plan *p = ws->plan;
   const size_t npsi = (size_t)ws->nbh * p->ngl;

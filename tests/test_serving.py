@@ -48,13 +48,15 @@ def test_request_larger_than_budget_is_rejected():
 def test_alone_equals_in_mix():
     from pwgpu import serve
     print("case serving alone vs mix", flush=True)
-    target = model.si_supercell(8, seed=11)
+    # damped CP relaxation: fictitious mass 300, time step 8, friction 0.05
+    cell = lambda nat, seed: model.si_supercell(nat, seed=seed, dt=8.0, emass=300.0)
+    opts = dict(friction=0.05, etol=1.0e-5, nconv=3, max_steps=800)
+    target = cell(8, 11)
     srv = serve.Server(64 * GIB, 4)
     try:
-        alone = srv.wait(srv.submit(target, max_steps=300))
-        mix = [model.si_supercell(16, seed=3), target, model.si_supercell(32, seed=5),
-               model.si_supercell(8, seed=12)]
-        ids = [srv.submit(s, max_steps=300) for s in mix]
+        alone = srv.wait(srv.submit(target, **opts))
+        mix = [cell(16, 3), target, cell(32, 5), cell(8, 12)]
+        ids = [srv.submit(s, **opts) for s in mix]
         res = [srv.wait(i) for i in ids]
         st = srv.stats()
     finally:
