@@ -175,6 +175,11 @@ struct pw_waveset {
   int ortho_lanczos = 0;       // Lanczos runs in the last ORTHO_X call
   // launch shapes chosen by pw_waves_autotune
   int tune_split_waves = 2048;  // waves of a G-split GEMM launch
+  // device results of a step section and their pinned host copy; the host
+  // block has NB*NB more doubles that stage the DEDPROJ constraint matrix
+  double *d_res = nullptr, *h_res = nullptr;
+  size_t res_n = 0;
+  size_t r_ekin = 0, r_vrho = 0, r_wk = 0, r_expect = 0, r_denmat = 0, r_force = 0;
   int tune_dens_blocks = 512;   // workgroups of the density yz pass
   std::vector<double> force;   // 3*nat
   std::vector<double> eig;     // nb
@@ -237,22 +242,24 @@ int k_phase_tables(pw_waveset *ws, const double *r_dev_host, double2 *phase);
 int k_pgen(pw_waveset *ws);
 int k_projections(pw_waveset *ws, const double2 *psi, const double2 *phase, double2 *proj);
 int k_addpro(pw_waveset *ws, double2 *psi, const double2 *phase, const double2 *proj);
-int k_force(pw_waveset *ws, const double2 *psi, const double2 *phase, const double2 *dedproj, double *force_host);
+// reductions: partial sums into device memory dst (read back once per section)
+int k_force(pw_waveset *ws, const double2 *psi, const double2 *phase, const double2 *dedproj, double *dst);
 int k_overlap(pw_waveset *ws, const double2 *psi1, const double2 *psi2, double *mat_host_or_dev, bool to_host, bool tid);
 int k_overlap_1c(pw_waveset *ws, const double2 *proj1, const double2 *proj2, double *mat_dev);
 int k_addopsi(pw_waveset *ws, double2 *psibar, const double2 *chi, const double *lam_dev);
 int k_addoproj(pw_waveset *ws, double2 *proj, const double2 *oproj, const double *lam_dev);
 int k_atom_matrix(pw_waveset *ws, const double2 *proj, double2 *out, const double2 *mats);
 int k_fold_occ(pw_waveset *ws, double2 *x);
-int k_ekin(pw_waveset *ws, const double2 *psi, const double *fpm_dev, double *out_host);
-int k_wavekinetic(pw_waveset *ws, const double2 *psi0, const double2 *psim, const double *fab_dev,
-                  double *out_host);
-int k_expect(pw_waveset *ws, const double2 *psi, const double2 *hpsi, double *eig_host);
+int red_blocks(const pw_waveset *ws);
+int vrho_blocks(const pw_waveset *ws);
+int k_ekin(pw_waveset *ws, const double2 *psi, const double *fpm_dev, double *dst);
+int k_wavekinetic(pw_waveset *ws, const double2 *psi0, const double2 *psim, const double *fab_dev, double *dst);
+int k_expect(pw_waveset *ws, const double2 *psi, const double2 *hpsi, double *dst);
 int k_propagate(pw_waveset *ws);
 int k_opsi_scale(pw_waveset *ws, double2 *opsi);
 int k_copy(pw_waveset *ws, double2 *dst, const double2 *src, size_t n);
-int k_denmat(pw_waveset *ws, const double2 *proj, double *denmat_host);
-int k_dot_vrho(pw_waveset *ws, double *out_host);
+int k_denmat(pw_waveset *ws, const double2 *proj, double *dst);
+int k_dot_vrho(pw_waveset *ws, double *dst);
 int ws_part(pw_waveset *ws, size_t bytes);
 int ws_small(pw_waveset *ws, size_t bytes);
 

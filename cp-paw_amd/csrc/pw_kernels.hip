@@ -53,14 +53,20 @@ __device__ inline double2 pro_val(const ProjGeo &q, int atom, int lmn, int ig, d
 // ---------------------------------------------------------------------------
 // phase tables exp(-i n b_k.R), n in [-nmax, nmax]
 // ---------------------------------------------------------------------------
-__global__ void k_phase(int nat, int nphase, int nmaxx, const double *gr, double2 *phase) {
+// GBAS(:,k).R of up to kPhaseAtoms atoms, passed by value with the launch
+constexpr int kPhaseAtoms = 128;
+struct PhaseArgs {
+  double gr[3 * kPhaseAtoms];
+};
+
+__global__ void k_phase(int nat, int nphase, int nmaxx, PhaseArgs a, double2 *phase) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int total = nat * 3 * nphase;
   if (i >= total) return;
   const int n = i % nphase - nmaxx;
   const int ak = i / nphase;                 // atom*3 + axis
   double s, c;
-  sincos((double)n * gr[ak], &s, &c);
+  sincos((double)n * a.gr[ak], &s, &c);
   phase[i] = make_double2(c, -s);
 }
 
@@ -919,10 +925,10 @@ __global__ void k_denmat(ProjGeo q, const double *occ2, const double2 *proj, dou
   const int atom = blockIdx.x;
   const int lmnx = q.lmnx[q.atom_isp[atom]];
   const int base = q.ipro0[atom];
-  for (int w = threadIdx.x; w < lmnx * lmnx; w += blockDim.x) {
-    const int i = w / lmnx, j = w % lmnx;
+  for (int w = threadIdx.x; w < q.lmnxx * q.lmnxx; w += blockDim.x) {
+    const int i = w / q.lmnxx, j = w % q.lmnxx;
     double s = 0.0;
-    for (int b = 0; b < q.nbh; ++b) {
+    for (int b = 0; b < q.nbh && i < lmnx && j < lmnx; ++b) {
       const double s1 = occ2[2 * b], s2 = occ2[2 * b + 1];
       const double2 pi = proj[(size_t)(base + i) * q.nbh + b];
       const double2 pj = proj[(size_t)(base + j) * q.nbh + b];
@@ -1120,23 +1126,23 @@ int ws_small(pw_waveset *ws, size_t bytes) {
 }
 
 int k_phase_tables(pw_waveset *ws, const double *r, double2 *phase) {
-  // r: host positions (3*nat); gr[atom*3+k] = GBAS(:,k).R
+  // r: host positions (3*nat); gr[atom*3+k] = GBAS(:,k).R, handed to the
+  // kernel by value in chunks of kPhaseAtoms atoms (no staging buffer)
   pw_plan *p = ws->plan;
-  std::vector<double> gr(3 * ws->nat);
-  for (int a = 0; a < ws->nat; ++a)
-    for (int k = 0; k < 3; ++k)
-      gr[3 * a + k] = p->gbas[0 + 3 * k] * r[3 * a] + p->gbas[1 + 3 * k] * r[3 * a + 1] +
-                      p->gbas[2 + 3 * k] * r[3 * a + 2];
-  PW_TRY(ws_small(ws, 3 * ws->nat * sizeof(double)));
-  PW_HIP(hipMemcpyAsync(ws->d_small, gr.data(), gr.size() * sizeof(double), hipMemcpyHostToDevice,
-                        ws->stream));
-  const int total = ws->nat * 3 * ws->nphase;
   ScopeTimer tm(ws, "phase");
-  k_phase<<<(total + 255) / 256, 256, 0, ws->stream>>>(ws->nat, ws->nphase, (ws->nphase - 1) / 2,
-                                                        ws->d_small, phase);
-  PW_HIP(hipGetLastError());
-  // the scratch is reused by the next call; keep the copy ordered before it
-  PW_HIP(hipStreamSynchronize(ws->stream));
+  for (int a0 = 0; a0 < ws->nat; a0 += kPhaseAtoms) {
+    const int na = std::min(kPhaseAtoms, ws->nat - a0);
+    PhaseArgs args;
+    for (int a = 0; a < na; ++a)
+      for (int k = 0; k < 3; ++k) {
+        const double *ra = r + 3 * (a0 + a);
+        args.gr[3 * a + k] = p->gbas[0 + 3 * k] * ra[0] + p->gbas[1 + 3 * k] * ra[1] + p->gbas[2 + 3 * k] * ra[2];
+      }
+    const int total = na * 3 * ws->nphase;
+    k_phase<<<(total + 255) / 256, 256, 0, ws->stream>>>(na, ws->nphase, (ws->nphase - 1) / 2, args,
+                                                          phase + (size_t)a0 * 3 * ws->nphase);
+    PW_HIP(hipGetLastError());
+  }
   return PW_OK;
 }
 
@@ -1180,15 +1186,13 @@ int k_addpro(pw_waveset *ws, double2 *psi, const double2 *phase, const double2 *
   return run_colgemm<2>(ws, ga, gb, ws->npro, psi, "addpro");
 }
 
-int k_force(pw_waveset *ws, const double2 *psi, const double2 *phase, const double2 *ded,
-            double *force_host) {
+int k_force(pw_waveset *ws, const double2 *psi, const double2 *phase, const double2 *ded, double *dst) {
   constexpr int NW = 4, NGB = 4;
   ProjGeo q = make_pgeo(ws);
   q.phase = phase;
   const int ngt = (q.ngl + 16 * NGB - 1) / (16 * NGB);
   const size_t n = (size_t)ws->npro * 3;
   PW_TRY(ws_part(ws, (size_t)ngt * n * sizeof(double)));
-  PW_TRY(ws_small(ws, n * sizeof(double)));
   {
     ScopeTimer tm(ws, "force");
     dim3 grid(ngt, (ws->npro + 16 * NW - 1) / (16 * NW));
@@ -1197,21 +1201,8 @@ int k_force(pw_waveset *ws, const double2 *psi, const double2 *phase, const doub
     PW_HIP(hipGetLastError());
   }
   k_reduce_split<double><<<(n + kRedC - 1) / kRedC, kRedC * kRedS, 0, ws->stream>>>(ws->d_part, ngt, (int)n,
-                                                                                      2.0 * ws->plan->vol,
-                                                      ws->d_small);
+                                                                                      2.0 * ws->plan->vol, dst);
   PW_HIP(hipGetLastError());
-  std::vector<double> fp(n);
-  PW_HIP(hipMemcpyAsync(fp.data(), ws->d_small, n * sizeof(double), hipMemcpyDeviceToHost, ws->stream));
-  PW_HIP(hipStreamSynchronize(ws->stream));
-  // sum the projector contributions of each atom (fixed order)
-  for (int a = 0; a < ws->nat; ++a) {
-    const int i0 = ws->ipro0[a], nl = ws->lmnx[ws->isp[a]];
-    for (int d = 0; d < 3; ++d) {
-      double f = 0.0;
-      for (int l = 0; l < nl; ++l) f += fp[(size_t)(i0 + l) * 3 + d];
-      force_host[3 * a + d] = f;
-    }
-  }
   return PW_OK;
 }
 
@@ -1283,93 +1274,50 @@ int k_fold_occ(pw_waveset *ws, double2 *x) {
 
 static int reduce_blocks(pw_plan *p) { return std::min(1024, (p->ngl + kB - 1) / kB); }
 
-int k_ekin(pw_waveset *ws, const double2 *psi, const double *fpm_dev, double *out_host) {
+// Reductions of the step leave their partial sums on the device (dst); the
+// host reads them back with one copy per step section (see pw_waves.cpp).
+int red_blocks(const pw_waveset *ws) { return reduce_blocks(ws->plan); }
+int vrho_blocks(const pw_waveset *ws) { return std::min(1024, (ws->plan->nrl + kB - 1) / kB); }
+
+int k_ekin(pw_waveset *ws, const double2 *psi, const double *fpm_dev, double *dst) {
   pw_plan *p = ws->plan;
-  const int nbk = reduce_blocks(p);
-  PW_TRY(ws_part(ws, nbk * sizeof(double)));
-  {
-    ScopeTimer tm(ws, "ekin");
-    k_ekin<<<nbk, kB, 0, ws->stream>>>(p->ngl, ws->nbh, p->d_g2, p->d_minusg, fpm_dev, psi, ws->d_part);
-    PW_HIP(hipGetLastError());
-  }
-  std::vector<double> h(nbk);
-  PW_HIP(hipMemcpyAsync(h.data(), ws->d_part, nbk * sizeof(double), hipMemcpyDeviceToHost, ws->stream));
-  PW_HIP(hipStreamSynchronize(ws->stream));
-  double s = 0.0;
-  for (double v : h) s += v;
-  *out_host = 0.5 * s * p->vol;
+  ScopeTimer tm(ws, "ekin");
+  k_ekin<<<red_blocks(ws), kB, 0, ws->stream>>>(p->ngl, ws->nbh, p->d_g2, p->d_minusg, fpm_dev, psi, dst);
+  PW_HIP(hipGetLastError());
   return PW_OK;
 }
 
 int k_wavekinetic(pw_waveset *ws, const double2 *psi0, const double2 *psim, const double *fab_dev,
-                  double *out_host) {
+                  double *dst) {
   pw_plan *p = ws->plan;
-  const int nbk = reduce_blocks(p);
-  PW_TRY(ws_part(ws, nbk * sizeof(double)));
-  {
-    ScopeTimer tm(ws, "wavekinetic");
-    k_wavekin<<<nbk, kB, 0, ws->stream>>>(p->ngl, ws->nbh, ws->d_mpsi, p->d_minusg, fab_dev, psi0, psim,
-                                          ws->d_part);
-    PW_HIP(hipGetLastError());
-  }
-  std::vector<double> h(nbk);
-  PW_HIP(hipMemcpyAsync(h.data(), ws->d_part, nbk * sizeof(double), hipMemcpyDeviceToHost, ws->stream));
-  PW_HIP(hipStreamSynchronize(ws->stream));
-  double s = 0.0;
-  for (double v : h) s += v;
-  *out_host = s / (ws->delt * ws->delt) * p->vol;
+  ScopeTimer tm(ws, "wavekinetic");
+  k_wavekin<<<red_blocks(ws), kB, 0, ws->stream>>>(p->ngl, ws->nbh, ws->d_mpsi, p->d_minusg, fab_dev, psi0,
+                                                   psim, dst);
+  PW_HIP(hipGetLastError());
   return PW_OK;
 }
 
-int k_expect(pw_waveset *ws, const double2 *psi, const double2 *hpsi, double *eig_host) {
+int k_expect(pw_waveset *ws, const double2 *psi, const double2 *hpsi, double *dst) {
   pw_plan *p = ws->plan;
-  PW_TRY(ws_part(ws, 2 * ws->nbh * sizeof(double)));
-  {
-    ScopeTimer tm(ws, "expect");
-    k_expect<<<ws->nbh, kB, 0, ws->stream>>>(p->ngl, p->d_minusg, psi, hpsi, ws->d_part);
-    PW_HIP(hipGetLastError());
-  }
-  std::vector<double> h(2 * ws->nbh);
-  PW_HIP(hipMemcpyAsync(h.data(), ws->d_part, h.size() * sizeof(double), hipMemcpyDeviceToHost, ws->stream));
-  PW_HIP(hipStreamSynchronize(ws->stream));
-  for (int b = 0; b < ws->nbh; ++b) {
-    const double re = 0.5 * h[2 * b] * p->vol, rem = 0.5 * h[2 * b + 1] * p->vol;
-    eig_host[2 * b] = re + rem;
-    if (2 * b + 1 < ws->nb) eig_host[2 * b + 1] = re - rem;
-  }
+  ScopeTimer tm(ws, "expect");
+  k_expect<<<ws->nbh, kB, 0, ws->stream>>>(p->ngl, p->d_minusg, psi, hpsi, dst);
+  PW_HIP(hipGetLastError());
   return PW_OK;
 }
 
-int k_denmat(pw_waveset *ws, const double2 *proj, double *dm_host) {
+int k_denmat(pw_waveset *ws, const double2 *proj, double *dst) {
   ProjGeo q = make_pgeo(ws);
-  const size_t n = (size_t)ws->nat * ws->lmnxx * ws->lmnxx;
-  PW_TRY(ws_part(ws, n * sizeof(double)));
-  PW_HIP(hipMemsetAsync(ws->d_part, 0, n * sizeof(double), ws->stream));
-  {
-    ScopeTimer tm(ws, "denmat");
-    k_denmat<<<ws->nat, 256, 0, ws->stream>>>(q, ws->d_occ2, proj, ws->d_part);
-    PW_HIP(hipGetLastError());
-  }
-  PW_HIP(hipMemcpyAsync(dm_host, ws->d_part, n * sizeof(double), hipMemcpyDeviceToHost, ws->stream));
-  PW_HIP(hipStreamSynchronize(ws->stream));
+  ScopeTimer tm(ws, "denmat");
+  k_denmat<<<ws->nat, 256, 0, ws->stream>>>(q, ws->d_occ2, proj, dst);
+  PW_HIP(hipGetLastError());
   return PW_OK;
 }
 
-int k_dot_vrho(pw_waveset *ws, double *out_host) {
+int k_dot_vrho(pw_waveset *ws, double *dst) {
   pw_plan *p = ws->plan;
-  const int nbk = std::min(1024, (p->nrl + kB - 1) / kB);
-  PW_TRY(ws_part(ws, nbk * sizeof(double)));
-  {
-    ScopeTimer tm(ws, "vrho");
-    k_dot<<<nbk, kB, 0, ws->stream>>>((size_t)p->nrl, ws->d_v, ws->d_rho, ws->d_part);
-    PW_HIP(hipGetLastError());
-  }
-  std::vector<double> h(nbk);
-  PW_HIP(hipMemcpyAsync(h.data(), ws->d_part, nbk * sizeof(double), hipMemcpyDeviceToHost, ws->stream));
-  PW_HIP(hipStreamSynchronize(ws->stream));
-  double s = 0.0;
-  for (double v : h) s += v;
-  *out_host = s * p->vol / (double)p->nrl;
+  ScopeTimer tm(ws, "vrho");
+  k_dot<<<vrho_blocks(ws), kB, 0, ws->stream>>>((size_t)p->nrl, ws->d_v, ws->d_rho, dst);
+  PW_HIP(hipGetLastError());
   return PW_OK;
 }
 

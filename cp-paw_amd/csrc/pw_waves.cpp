@@ -14,7 +14,6 @@
 #include "pw_common.h"
 
 int k_fold_occ(pw_waveset *ws, double2 *x);
-int k_expect(pw_waveset *ws, const double2 *psi, const double2 *hpsi, double *eig_host);
 
 // ---------------------------------------------------------------------------
 // timers
@@ -158,17 +157,29 @@ static int dedproj(pw_waveset *ws) {
   const int nb = ws->nb;
   PW_TRY(k_atom_matrix(ws, ws->d_proj, ws->d_xproj, ws->d_dhmat));
   PW_TRY(k_fold_occ(ws, ws->d_xproj));
-  // LAMBDA = -RLAM0 (f_i+f_j)/2
-  std::vector<double> lam((size_t)nb * nb);
+  // LAMBDA = -RLAM0 (f_i+f_j)/2, staged in pinned memory that is rewritten
+  // only after the step's closing synchronisation
+  double *lam = ws->h_res + ws->res_n;
   for (int i = 0; i < nb; ++i)
     for (int j = 0; j < nb; ++j)
       lam[(size_t)i * nb + j] = -ws->rlam0[(size_t)i * nb + j] * 0.5 * (ws->occ[i] + ws->occ[j]);
-  PW_HIP(hipMemcpyAsync(d_mat(ws, 3), lam.data(), lam.size() * sizeof(double), hipMemcpyHostToDevice,
-                        ws->stream));
+  PW_HIP(hipMemcpyAsync(d_mat(ws, 3), lam, (size_t)nb * nb * sizeof(double), hipMemcpyHostToDevice, ws->stream));
   PW_TRY(k_atom_matrix(ws, ws->d_proj, ws->d_oproj, ws->d_domat));
   PW_TRY(k_addoproj(ws, ws->d_xproj, ws->d_oproj, d_mat(ws, 3)));
+  return PW_OK;
+}
+
+// read the device results of a step section back (one copy, one wait)
+static int fetch_results(pw_waveset *ws) {
+  PW_HIP(hipMemcpyAsync(ws->h_res, ws->d_res, ws->res_n * sizeof(double), hipMemcpyDeviceToHost, ws->stream));
   PW_HIP(hipStreamSynchronize(ws->stream));
   return PW_OK;
+}
+
+static double sum_of(const double *v, int n) {
+  double s = 0.0;
+  for (int i = 0; i < n; ++i) s += v[i];
+  return s;
 }
 
 // ---------------------------------------------------------------------------
@@ -300,6 +311,20 @@ extern "C" int pw_waves_create(pw_plan *plan, int nb, int nat, int nsp, const in
   if (rc == PW_OK) rc = dalloc((void **)&ws->d_dhmat, (size_t)nat * ws->lmnxx * ws->lmnxx * sizeof(double2));
   if (rc == PW_OK) rc = dalloc((void **)&ws->d_phase, (size_t)nat * 3 * ws->nphase * sizeof(double2));
   if (rc == PW_OK) rc = dalloc((void **)&ws->d_small_mats, (size_t)6 * nb * nb * sizeof(double));
+  if (rc == PW_OK) {
+    // result layout of a step section (partial sums, see fetch_results)
+    const size_t rb = red_blocks(ws), vb = vrho_blocks(ws);
+    ws->r_ekin = 0;
+    ws->r_vrho = ws->r_ekin + rb;
+    ws->r_wk = ws->r_vrho + vb;
+    ws->r_expect = ws->r_wk + rb;
+    ws->r_denmat = ws->r_expect + 2 * (size_t)ws->nbh;
+    ws->r_force = ws->r_denmat + (size_t)nat * ws->lmnxx * ws->lmnxx;
+    ws->res_n = ws->r_force + 3 * (size_t)ws->npro;
+    rc = dalloc((void **)&ws->d_res, ws->res_n * sizeof(double));
+    if (rc == PW_OK && hipHostMalloc((void **)&ws->h_res, (ws->res_n + (size_t)nb * nb) * sizeof(double)) != hipSuccess)
+      rc = pw_fail(PW_ERR_HIP, "pw_waves_create: pinned result buffer");
+  }
   std::vector<int> iv;
   auto up = [&](int **dst, const std::vector<int> &src) {
     if (rc != PW_OK) return;
@@ -343,6 +368,8 @@ extern "C" int pw_waves_destroy(pw_waveset *ws) {
                   ws->d_part, ws->d_small, ws->d_ortho, ws->d_small_mats, ws->d_ipro_tab, ws->d_p8};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
+  if (ws->d_res) (void)hipFree(ws->d_res);
+  if (ws->h_res) (void)hipHostFree(ws->h_res);
   if (ws->own_stream && ws->stream) (void)hipStreamDestroy(ws->stream);
   delete ws;
   return PW_OK;
@@ -622,26 +649,44 @@ extern "C" int pw_waves_etot(pw_waveset *ws) {
     PW_TRY(pw_waves_projections(ws));
     ws->first = false;
   }
-  // WAVES_EKIN
-  PW_TRY(k_ekin(ws, ws->d_psi0, ws->d_fpm, &ws->ekin));
-  // WAVES$DENMAT (hermitised, real)
-  std::vector<double> dm(ws->denmat.size());
-  PW_TRY(k_denmat(ws, ws->d_proj, dm.data()));
-  const int L = ws->lmnxx;
-  for (int a = 0; a < ws->nat; ++a)
-    for (int i = 0; i < L; ++i)
-      for (int j = 0; j < L; ++j) {
-        const size_t ij = ((size_t)a * L + i) * L + j, ji = ((size_t)a * L + j) * L + i;
-        ws->denmat[ij] = 0.5 * (dm[ij] + dm[ji]);
-      }
+  // WAVES_EKIN, WAVES$DENMAT
+  PW_TRY(k_ekin(ws, ws->d_psi0, ws->d_fpm, ws->d_res + ws->r_ekin));
+  PW_TRY(k_denmat(ws, ws->d_proj, ws->d_res + ws->r_denmat));
   // WAVES_DENSITY
   {
     ScopeTimer tm(ws, "density");
     PW_TRY(fft_density(p, ws->nbh, ws->d_psi0, ws->d_occ2, ws->d_rho, ws->prec, ws->stream, ws->tune_dens_blocks));
   }
   // energy of the fixed local potential: RWEIGHT sum_r V rho
-  PW_TRY(k_dot_vrho(ws, &ws->epot));
-  // one-centre energy sum_ij DH_ij D_ji
+  PW_TRY(k_dot_vrho(ws, ws->d_res + ws->r_vrho));
+  // WAVES$FORCE
+  PW_TRY(phase_at(ws, ws->r0));
+  PW_TRY(dedproj(ws));
+  PW_TRY(k_force(ws, ws->d_psi0, ws->d_phase, ws->d_xproj, ws->d_res + ws->r_force));
+  // WAVES$HPSI: VPSI + HPROJ + ADDPRO
+  {
+    ScopeTimer tm(ws, "vpsi");
+    PW_TRY(fft_vpsi(p, ws->nbh, ws->d_psi0, ws->d_v, ws->d_hpsi, 1, ws->prec, ws->stream));
+  }
+  PW_TRY(k_atom_matrix(ws, ws->d_proj, ws->d_xproj, ws->d_dhmat));
+  PW_TRY(k_addpro(ws, ws->d_hpsi, ws->d_phase, ws->d_xproj));
+  // <psi|H|psi>, WAVES_WAVEKINETIC (first half)
+  PW_TRY(k_expect(ws, ws->d_psi0, ws->d_hpsi, ws->d_res + ws->r_expect));
+  PW_TRY(k_wavekinetic(ws, ws->d_psi0, ws->d_psim, ws->d_fab, ws->d_res + ws->r_wk));
+  PW_TRY(fetch_results(ws));
+  const double *hr = ws->h_res;
+  ws->ekin = 0.5 * sum_of(hr + ws->r_ekin, red_blocks(ws)) * p->vol;
+  ws->epot = sum_of(hr + ws->r_vrho, vrho_blocks(ws)) * p->vol / (double)p->nrl;
+  ws->wavekin1 = sum_of(hr + ws->r_wk, red_blocks(ws)) / (ws->delt * ws->delt) * p->vol;
+  // density matrix (hermitised, real) and the one-centre energy sum_ij DH_ij D_ji
+  const int L = ws->lmnxx;
+  const double *dm = hr + ws->r_denmat;
+  for (int a = 0; a < ws->nat; ++a)
+    for (int i = 0; i < L; ++i)
+      for (int j = 0; j < L; ++j) {
+        const size_t ij = ((size_t)a * L + i) * L + j, ji = ((size_t)a * L + j) * L + i;
+        ws->denmat[ij] = 0.5 * (dm[ij] + dm[ji]);
+      }
   double e1 = 0.0;
   for (int a = 0; a < ws->nat; ++a)
     for (int i = 0; i < L; ++i)
@@ -651,21 +696,23 @@ extern "C" int pw_waves_etot(pw_waveset *ws) {
       }
   ws->e1c = e1;
   ws->etot = ws->ekin + ws->epot + ws->e1c;
-  // WAVES$FORCE
-  PW_TRY(phase_at(ws, ws->r0));
-  PW_TRY(dedproj(ws));
-  PW_TRY(k_force(ws, ws->d_psi0, ws->d_phase, ws->d_xproj, ws->force.data()));
-  // WAVES$HPSI: VPSI + HPROJ + ADDPRO
-  {
-    ScopeTimer tm(ws, "vpsi");
-    PW_TRY(fft_vpsi(p, ws->nbh, ws->d_psi0, ws->d_v, ws->d_hpsi, 1, ws->prec, ws->stream));
+  // forces: sum the projector contributions of each atom (fixed order)
+  const double *fp = hr + ws->r_force;
+  for (int a = 0; a < ws->nat; ++a) {
+    const int i0 = ws->ipro0[a], nl = ws->lmnx[ws->isp[a]];
+    for (int d = 0; d < 3; ++d) {
+      double f = 0.0;
+      for (int l = 0; l < nl; ++l) f += fp[(size_t)(i0 + l) * 3 + d];
+      ws->force[3 * a + d] = f;
+    }
   }
-  PW_TRY(k_atom_matrix(ws, ws->d_proj, ws->d_xproj, ws->d_dhmat));
-  PW_TRY(k_addpro(ws, ws->d_hpsi, ws->d_phase, ws->d_xproj));
-  // <psi|H|psi>
-  PW_TRY(k_expect(ws, ws->d_psi0, ws->d_hpsi, ws->eig.data()));
-  // WAVES_WAVEKINETIC (first half)
-  PW_TRY(k_wavekinetic(ws, ws->d_psi0, ws->d_psim, ws->d_fab, &ws->wavekin1));
+  // band energies from the super-band expectation values
+  const double *ex = hr + ws->r_expect;
+  for (int b = 0; b < ws->nbh; ++b) {
+    const double re = 0.5 * ex[2 * b] * p->vol, rem = 0.5 * ex[2 * b + 1] * p->vol;
+    ws->eig[2 * b] = re + rem;
+    if (2 * b + 1 < ws->nb) ws->eig[2 * b + 1] = re - rem;
+  }
   ws->have_hpsi = true;
   return PW_OK;
 }
@@ -730,8 +777,11 @@ extern "C" int pw_waves_orthogonalize(pw_waveset *ws) {
                         hipMemcpyDeviceToHost, ws->stream));
   PW_HIP(hipMemcpyAsync(ws->oomat.data(), d_mat(ws, 2), ws->oomat.size() * sizeof(double),
                         hipMemcpyDeviceToHost, ws->stream));
+  PW_TRY(k_wavekinetic(ws, ws->d_psi0, ws->d_psim, ws->d_fab, ws->d_res + ws->r_wk));
+  PW_HIP(hipMemcpyAsync(ws->h_res + ws->r_wk, ws->d_res + ws->r_wk, red_blocks(ws) * sizeof(double),
+                        hipMemcpyDeviceToHost, ws->stream));
   PW_HIP(hipStreamSynchronize(ws->stream));
-  PW_TRY(k_wavekinetic(ws, ws->d_psi0, ws->d_psim, ws->d_fab, &ws->wavekin2));
+  ws->wavekin2 = sum_of(ws->h_res + ws->r_wk, red_blocks(ws)) / (ws->delt * ws->delt) * p->vol;
   ws->efict = 0.5 * (ws->wavekin1 + ws->wavekin2);
   return PW_OK;
 }
