@@ -176,7 +176,9 @@ def run_gpu(system_name, prec, steps, warmup, profile):
     e = {k: w.getr8(k) for k in ("ETOT", "EKIN", "EFICT")}
     res = {"steps_per_s": steps / dt, "ms_per_step": 1e3 * dt / steps, "energies": e,
            "ortho_iter": w.geti4("ORTHOITER"), "ortho_levels": w.geti4("ORTHOLEVELS"),
-           "ortho_ns": w.geti4("ORTHONS")}
+           "ortho_ns": w.geti4("ORTHONS"),
+           "desc": f"Gamma, {s.nat} atoms, {s.nb} bands, {plane.ngl} G, {plane.nr[0]}x{plane.nr[1]}x{plane.nr[2]}, "
+                   f"{s.npro} projectors"}
     if profile:
         names = ["projections", "addpro", "force", "overlap", "overlap1c", "addopsi", "addoproj",
                  "vpsi", "density", "propagate", "opsi_scale", "ortho_x", "ekin", "wavekinetic",
@@ -337,8 +339,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64" if head == 0 else "mixed(f32 fft+V.psi, split-bf16x3 mfma gemms, f64 constraint/energy/force)",
-        "data": "synthetic (seeded Si64 displacements and coefficients, model potential)",
-        "config": {"workload": f"{args.system} CP wavefunction step (Gamma, 144 bands, 72^3, 512 projectors)",
+        "data": f"synthetic (seeded {args.system} positions and coefficients, model potential and projectors)",
+        "config": {"workload": f"{args.system} CP wavefunction step ({r['desc']})",
                    "form": "fp64" if head == 0 else "mixed"},
         "roofline": roof,
         "cpu_baseline": cpu,

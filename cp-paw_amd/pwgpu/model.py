@@ -249,6 +249,67 @@ def synthetic(seed=3, nb=256, ncentre=64, nwell=32, a=20.0,
                   dt=dt, emass=emass, seed=seed)
 
 
+def o_species():
+    """Model O site: 2 s + 2 p Gaussian projectors (LNX=4, LMNX=8), more compact than Si."""
+    do = np.array([[0.40, 0.10, 0, 0], [0.10, 0.22, 0, 0],
+                   [0, 0, 0.34, 0.06], [0, 0, 0.06, 0.18]], dtype=np.float64)
+    dh = np.array([[-0.80, 0.12, 0, 0], [0.12, 0.30, 0, 0],
+                   [0, 0, -0.45, 0.08], [0, 0, 0.08, 0.25]], dtype=np.float64)
+    return Species(lox=[0, 0, 1, 1], sigma=[0.55, 0.85, 0.6, 0.9], do_ln=do, dh_ln=dh)
+
+
+def h_species():
+    """Model H site: 2 s Gaussian projectors (LNX=2, LMNX=2)."""
+    do = np.array([[0.25, 0.05], [0.05, 0.12]], dtype=np.float64)
+    dh = np.array([[-0.35, 0.06], [0.06, 0.18]], dtype=np.float64)
+    return Species(lox=[0, 0], sigma=[0.45, 0.75], do_ln=do, dh_ln=dh)
+
+
+def water(nmol=32, a=19.7, seed=9, epw_psi_ry=40.0, cdual=4.0, dt=4.0, emass=1000.0,
+          doo=4.5, doh=1.81, angle=104.5):
+    """32 H2O in a cubic box (1 g/cm^3 at a=19.7 bohr): seeded random O positions
+    with O-O >= doo (minimum image), O-H = doh, H-O-H = angle, random
+    orientations; 4 doubly occupied bands per molecule, no empty bands."""
+    rng = np.random.default_rng(seed)
+    opos = []
+    tries = 0
+    while len(opos) < nmol:
+        tries += 1
+        if tries > 200000:
+            raise RuntimeError("water: could not place the molecules")
+        c = rng.uniform(0.0, a, 3)
+        ok = True
+        for o in opos:
+            d = c - o
+            d -= a * np.round(d / a)
+            if d @ d < doo * doo:
+                ok = False
+                break
+        if ok:
+            opos.append(c)
+    half = math.radians(angle) / 2.0
+    pos, isp = [], []
+    for o in opos:
+        u = rng.normal(size=3)
+        u /= np.linalg.norm(u)
+        v = rng.normal(size=3)
+        v -= (v @ u) * u
+        v /= np.linalg.norm(v)
+        pos.append(o)
+        isp.append(0)
+        for sgn in (1.0, -1.0):
+            pos.append(o + doh * (math.cos(half) * u + sgn * math.sin(half) * v))
+            isp.append(1)
+    pos = np.array(pos) % a
+    nb = 4 * nmol
+    wells = [(p, -3.0 if k == 0 else -1.0, 1.0 if k == 0 else 0.8) for p, k in zip(pos, isp)]
+    epw_psi = epw_psi_ry * RY
+    return System(name=f"H2O{nmol}", rbas=np.diag([a, a, a]).astype(np.float64), pos=pos,
+                  isp=np.array(isp, dtype=np.int64), species=[o_species(), h_species()], nb=nb,
+                  occ=np.full(nb, 2.0), epw_psi=epw_psi, epw_rho=cdual * epw_psi, wells=wells, dt=dt,
+                  emass=emass, seed=seed)
+
+
 def tiny(seed=5, nb=8, a=7.0, epw_psi=3.0, epw_rho=12.0, nat=2):
     """Tiny cell for fast CPU parity tests (two sites, small grid)."""
     rng = np.random.default_rng(seed)
@@ -265,5 +326,6 @@ SYSTEMS = {
     "si8": si8,
     "si64": si64,
     "synthetic": synthetic,
+    "h2o32": water,
     "tiny": tiny,
 }

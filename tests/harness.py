@@ -34,4 +34,13 @@ def gpu_waveset(system, prec=0):
     return plane, w
 
 
-SYSTEMS = {"tiny": model.tiny, "si8": model.si8}
+SYSTEMS = {
+    "tiny": model.tiny,
+    "si8": model.si8,
+    # two species with different projector sets (O: 2s2p, H: 2s), 12 atoms
+    "h2o4": lambda: model.water(nmol=4, a=9.85, seed=3),
+    # 256 bands, 64^3 grid, 512 projectors (the oracle takes ~11 s per etot)
+    "synthetic": model.synthetic,
+    # 32 H2O, 40 Ry, 80^3 grid, 128 bands, 384 projectors (~13 s per etot)
+    "h2o32": model.water,
+}
