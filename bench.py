@@ -64,7 +64,7 @@ SPLIT_PRODUCTS = 6
 
 def kernel_peak(name, prec):
     if name in GEMM_KERNELS:
-        if prec == 1 and name != "force":
+        if prec == 1:
             return BF16_PEAK_TFLOPS, "mfma"
         return FP64_PEAK_TFLOPS, "mfma"
     return HBM_PEAK_GBS, "hbm"

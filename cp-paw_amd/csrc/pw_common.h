@@ -202,13 +202,14 @@ struct pw_waveset {
   int *d_atom_isp = nullptr, *d_ipro0 = nullptr, *d_ibpro = nullptr, *d_lmnx = nullptr;
   int *d_lmn_ln = nullptr, *d_lmn_lm = nullptr, *d_lmn_l = nullptr;
   int2 *d_ipro_tab = nullptr;  // (atom, lmn) of each projector
-  // mixed form: projector matrix split into three bf16 pieces per real and
-  // imaginary part (16 B per element), [NPRO][NGL padded to 64], for the
-  // positions p8_pos
-  void *d_p8 = nullptr;
-  size_t p8_bytes = 0;
-  std::vector<double> p8_pos;
-  bool p8_valid = false;
+  // mixed form: factors of the projector functions for the positions
+  // pfac_pos, P(atom,lmn,G) = (-i)^l F(isp,lmn,G) E(atom,G):
+  // F = YLM*PRO [NSP*LMNXX][NGLP] doubles, then E = exp(-iG.R) [NAT][NGLP]
+  // double2, NGLP = NGL padded to 64 with zeros
+  void *d_pfac = nullptr;
+  size_t pfac_bytes = 0;
+  std::vector<double> pfac_pos;
+  bool pfac_valid = false;
   double2 *d_phase = nullptr;  // nat*3*(2*nmaxx+1)
   int nphase = 0;              // 2*nmaxx+1
   double *d_part = nullptr; size_t part_bytes = 0;   // split-K partials
@@ -239,7 +240,7 @@ void ws_flush_timers(pw_waveset *ws);
 
 // kernels in pw_kernels.hip
 int k_phase_tables(pw_waveset *ws, const double *r_dev_host, double2 *phase);
-int k_pgen(pw_waveset *ws);
+int k_pfac(pw_waveset *ws);
 int k_projections(pw_waveset *ws, const double2 *psi, const double2 *phase, double2 *proj);
 int k_addpro(pw_waveset *ws, double2 *psi, const double2 *phase, const double2 *proj);
 // reductions: partial sums into device memory dst (read back once per section)

@@ -392,37 +392,52 @@ __global__ void k_unravel_pw(int nbh, int nb, const double2 *t, const double2 *t
 
 // ---------------------------------------------------------------------------
 // one-centre overlap (WAVES_1COVERLAP, src/paw_waves2.f90:856-1022), added to mat
-// c1 = sum_k conj(p1_ki) p2a_kj, c2 = sum_k p1_ki p2a_kj
+// c1 = sum_k conj(p1_ki) p2a_kj, c2 = sum_k p1_ki p2a_kj.  The projector index
+// k is split over blockIdx.z in runs of k1c rows; part[z][i][j] = (c1, c2)
+// of the run, summed in a fixed order by k_overlap1c_sum.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_overlap1c(int npro, int nbh, int nb, const double2 *p1,
-                                                   const double2 *p2a, double *mat) {
-  __shared__ double2 sa[64][17];
-  __shared__ double2 sb[64][17];
+constexpr int k1c = 64;
+__global__ void __launch_bounds__(256) k_overlap1c(int npro, int nbh, const double2 *p1, const double2 *p2a,
+                                                   double4 *part) {
+  __shared__ double2 sa[k1c][17];
+  __shared__ double2 sb[k1c][17];
   const int ti = threadIdx.x / 16, tj = threadIdx.x % 16;
-  const int i0 = blockIdx.x * 16, j0 = blockIdx.y * 16;
+  const int i0 = blockIdx.x * 16, j0 = blockIdx.y * 16, k0 = blockIdx.z * k1c;
   const int i = i0 + ti, j = j0 + tj;
+  for (int w = threadIdx.x; w < k1c * 16; w += 256) {
+    const int k = w / 16, c = w % 16;
+    const bool ok = k0 + k < npro;
+    sa[k][c] = (ok && i0 + c < nbh) ? p1[(size_t)(k0 + k) * nbh + i0 + c] : czero();
+    sb[k][c] = (ok && j0 + c < nbh) ? p2a[(size_t)(k0 + k) * nbh + j0 + c] : czero();
+  }
+  __syncthreads();
   double2 c1 = czero(), c2 = czero();
-  for (int k0 = 0; k0 < npro; k0 += 64) {
-    for (int w = threadIdx.x; w < 64 * 16; w += 256) {
-      const int k = w / 16, c = w % 16;
-      const bool ok = k0 + k < npro;
-      sa[k][c] = (ok && i0 + c < nbh) ? p1[(size_t)(k0 + k) * nbh + i0 + c] : czero();
-      sb[k][c] = (ok && j0 + c < nbh) ? p2a[(size_t)(k0 + k) * nbh + j0 + c] : czero();
-    }
-    __syncthreads();
 #pragma unroll 8
-    for (int k = 0; k < 64; ++k) {
-      const double2 a = sa[k][ti], b = sb[k][tj];
-      c1.x += a.x * b.x + a.y * b.y;
-      c1.y += a.x * b.y - a.y * b.x;
-      c2.x += a.x * b.x - a.y * b.y;
-      c2.y += a.x * b.y + a.y * b.x;
-    }
-    __syncthreads();
+  for (int k = 0; k < k1c; ++k) {
+    const double2 a = sa[k][ti], b = sb[k][tj];
+    c1.x += a.x * b.x + a.y * b.y;
+    c1.y += a.x * b.y - a.y * b.x;
+    c2.x += a.x * b.x - a.y * b.y;
+    c2.y += a.x * b.y + a.y * b.x;
   }
   if (i >= nbh || j >= nbh) return;
-  const double2 s1 = make_double2(0.5 * (c1.x + c2.x), 0.5 * (c1.y + c2.y));
-  const double2 s2 = make_double2(0.5 * (c1.x - c2.x), 0.5 * (c1.y - c2.y));
+  part[((size_t)blockIdx.z * nbh + i) * nbh + j] = make_double4(c1.x, c1.y, c2.x, c2.y);
+}
+
+__global__ void k_overlap1c_sum(int nbh, int nb, int nz, const double4 *part, double *mat) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= nbh * nbh) return;
+  const int i = idx / nbh, j = idx % nbh;
+  double4 c = make_double4(0.0, 0.0, 0.0, 0.0);
+  for (int z = 0; z < nz; ++z) {
+    const double4 v = part[(size_t)z * nbh * nbh + idx];
+    c.x += v.x;
+    c.y += v.y;
+    c.z += v.z;
+    c.w += v.w;
+  }
+  const double2 s1 = make_double2(0.5 * (c.x + c.z), 0.5 * (c.y + c.w));
+  const double2 s2 = make_double2(0.5 * (c.x - c.z), 0.5 * (c.y - c.w));
   const int ia = 2 * i, ja = 2 * j;
   mat[(size_t)ia * nb + ja] += s1.x;
   mat[(size_t)ia * nb + ja + 1] += s1.y;
@@ -568,47 +583,72 @@ __device__ inline void load_planes(const __bf16 (*pl)[KBP], int plane_rows, int 
   for (int q = 0; q < NPL; ++q) v[q] = *reinterpret_cast<const bf16x8 *>(&pl[q * plane_rows + row][8 * k8]);
 }
 
-// Projector matrix in split form, PS[ipro][G] = 16 B per element holding the
-// six bf16 pieces (re h,m,l, im h,m,l) and two zero halves; G padded to a
-// multiple of 64 with zeros (WAVES_EXPANDPRO, src/paw_waves1.f90:6070-6109,
-// evaluated in FP64 then split).
-__device__ inline unsigned pack2(__bf16 a, __bf16 b) {
-  return (unsigned)__builtin_bit_cast(unsigned short, a) | ((unsigned)__builtin_bit_cast(unsigned short, b) << 16);
-}
-
-__global__ void k_pgen_bf(ProjGeo q, const int2 *iptab, int npro, int nglp, uint4 *ps) {
+// Projector functions of the mixed form from two factor tables,
+//   P(atom,lmn,G) = (-i)^l F(isp,lmn,G) E(atom,G),
+//   F = YLM(G,lm) PRO(G,ln) [NSP*LMNXX][NGLP],  E = exp(-iG.R_atom) [NAT][NGLP]
+// (WAVES_EXPANDPRO, src/paw_waves1.f90:6070-6109).  Both tables are FP64 and
+// padded with zeros to NGLP (a multiple of 64) G; they hold (NSP*LMNXX*8 +
+// NAT*16) bytes per G instead of the NPRO*16 of a stored projector matrix, so
+// the products read them from cache, and each product forms P in FP64 while
+// it stages its operands.
+__global__ void k_pfac_f(ProjGeo q, int nsp, int nglp, double *f) {
   const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= (size_t)npro * nglp) return;
-  const int ip = (int)(idx / nglp), ig = (int)(idx % nglp);
-  double2 v = czero();
-  if (ig < q.ngl) {
-    const int2 al = iptab[ip];
-    v = pro_val(q, al.x, al.y, ig, eigr_of(q, al.x, ig));
-  }
-  const Split3 a = split3(v.x), b = split3(v.y);
-  ps[idx] = make_uint4(pack2(a.h, a.m), pack2(a.l, b.h), pack2(b.m, b.l), 0u);
+  if (idx >= (size_t)nsp * q.lmnxx * nglp) return;
+  const int k = (int)(idx / nglp), ig = (int)(idx % nglp);
+  const int isp = k / q.lmnxx, lmn = k % q.lmnxx;
+  double v = 0.0;
+  if (ig < q.ngl && lmn < q.lmnx[isp])
+    v = q.ylm[(size_t)q.lmn_lm[k] * q.ngl + ig] * q.pro[(size_t)(q.ibpro[isp] + q.lmn_ln[k]) * q.ngl + ig];
+  f[idx] = v;
 }
 
-__device__ inline void put_bits(Planes8 &o, int e, uint4 u) {
-  o.p[0][e] = __builtin_bit_cast(__bf16, (unsigned short)(u.x & 0xffffu));
-  o.p[1][e] = __builtin_bit_cast(__bf16, (unsigned short)(u.x >> 16));
-  o.p[2][e] = __builtin_bit_cast(__bf16, (unsigned short)(u.y & 0xffffu));
-  o.p[3][e] = __builtin_bit_cast(__bf16, (unsigned short)(u.y >> 16));
-  o.p[4][e] = __builtin_bit_cast(__bf16, (unsigned short)(u.z & 0xffffu));
-  o.p[5][e] = __builtin_bit_cast(__bf16, (unsigned short)(u.z >> 16));
+__global__ void k_pfac_e(ProjGeo q, int nglp, double2 *e) {
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (size_t)q.nat * nglp) return;
+  const int atom = (int)(idx / nglp), ig = (int)(idx % nglp);
+  e[idx] = ig < q.ngl ? eigr_of(q, atom, ig) : czero();
 }
+
+struct PFac {
+  const double *f;
+  const double2 *e;
+  const int2 *iptab;
+  const int *atom_isp, *lmn_l;
+  int lmnxx, nglp;
+  // row of projector ip: F row, E row and l
+  __device__ void row(int ip, const double *&fr, const double2 *&er, int &l) const {
+    const int2 al = iptab[ip];
+    const int k = atom_isp[al.x] * lmnxx + al.y;
+    fr = f + (size_t)k * nglp;
+    er = e + (size_t)al.x * nglp;
+    l = lmn_l[k];
+  }
+  __device__ double2 at(int ip, int ig) const {
+    const double *fr;
+    const double2 *er;
+    int l;
+    row(ip, fr, er, l);
+    const double s = fr[ig];
+    const double2 ee = er[ig];
+    return rot_mil(make_double2(s * ee.x, s * ee.y), l);
+  }
+};
 
 // ---- row operands (8 consecutive G of one row) -------------------------------
-// conj(P(row, G)) from the split projector matrix
-struct RowP8 {
-  const uint4 *ps;
-  int nglp;
+// conj(P(row, G)) from the projector factors
+struct RowPf {
+  PFac pf;
   __device__ void operator()(int row, int g0, int, Planes8 &o) const {
-    const uint4 *src = ps + (size_t)row * nglp + g0;
+    const double *fr;
+    const double2 *er;
+    int l;
+    pf.row(row, fr, er, l);
 #pragma unroll
-    for (int e = 0; e < 8; ++e) put_bits(o, e, src[e]);
-#pragma unroll
-    for (int q = 3; q < NPL; ++q) o.p[q] = bf_neg(o.p[q]);
+    for (int e = 0; e < 8; ++e) {
+      const double s = fr[g0 + e];
+      const double2 ee = er[g0 + e];
+      put(o, e, cconj(rot_mil(make_double2(s * ee.x, s * ee.y), l)));
+    }
   }
 };
 
@@ -640,7 +680,11 @@ __device__ inline void psi_run(const double2 *psi, int ngl, int nbh, int b, int 
   for (int e = 0; e < 8; ++e) put(o, e, g0 + e < ngl ? src[g0 + e] : czero());
 }
 
-// part[split][r][b] = sum_{G in split} A(r,G) psi[b][G] (split-BF16x3)
+// part[split][r][b] = sum_{G in split} A(r,G) psi[b][G] (split-BF16x3).
+// Each 32-G chunk is summed by the MFMAs from zero in FP32 and then added to
+// FP64 accumulators: the FP32 roundings are those of one chunk's sum, never of
+// the running total (the products of localised projectors and extended waves
+// cancel strongly over G, and an FP32 running total loses ~1e-6 there).
 template <class Gen, int NCB, int NW>
 __global__ void __launch_bounds__(64 * NW) k_rowgemm_bf(Gen gen, int nrow, int ngl, int nbh,
                                                         const double2 *__restrict__ psi, int gsplit,
@@ -651,11 +695,11 @@ __global__ void __launch_bounds__(64 * NW) k_rowgemm_bf(Gen gen, int nrow, int n
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6, li = lane & 15, lk = lane >> 4;
   const int r0 = blockIdx.x * WR, b0 = blockIdx.y * NB16;
   const int g0 = blockIdx.z * gsplit, g1 = min(ngl, g0 + gsplit);
-  flt4 accr[NCB], acci[NCB];
+  dbl4 accr[NCB], acci[NCB];
 #pragma unroll
   for (int c = 0; c < NCB; ++c) {
-    accr[c] = flt4{0.f, 0.f, 0.f, 0.f};
-    acci[c] = flt4{0.f, 0.f, 0.f, 0.f};
+    accr[c] = dbl4{0.0, 0.0, 0.0, 0.0};
+    acci[c] = dbl4{0.0, 0.0, 0.0, 0.0};
   }
   for (int gc = g0; gc < g1; gc += KB) {
     {
@@ -680,7 +724,13 @@ __global__ void __launch_bounds__(64 * NW) k_rowgemm_bf(Gen gen, int nrow, int n
     for (int c = 0; c < NCB; ++c) {
       bf16x8 b[NPL];
       load_planes(sB, NB16, 16 * c + li, lk, b);
-      cmfma_bf(accr[c], acci[c], a, an, b);
+      flt4 cr = flt4{0.f, 0.f, 0.f, 0.f}, ci = flt4{0.f, 0.f, 0.f, 0.f};
+      cmfma_bf(cr, ci, a, an, b);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        accr[c][r] += (double)cr[r];
+        acci[c][r] += (double)ci[r];
+      }
     }
     __syncthreads();
   }
@@ -724,16 +774,12 @@ struct AddopsiAbf {
   }
 };
 // B columns: 8 consecutive k at one G
-struct AddproBbf {
-  const uint4 *ps;
-  int nglp, nk;
+struct AddproBf {
+  PFac pf;
+  int nk;
   __device__ void operator()(int k0, int ig, Planes8 &o) const {
 #pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      uint4 u = make_uint4(0u, 0u, 0u, 0u);
-      if (k0 + e < nk && ig < nglp) u = ps[(size_t)(k0 + e) * nglp + ig];
-      put_bits(o, e, u);
-    }
+    for (int e = 0; e < 8; ++e) put(o, e, (k0 + e < nk && ig < pf.nglp) ? pf.at(k0 + e, ig) : czero());
   }
 };
 struct AddopsiBbf {
@@ -760,11 +806,11 @@ __global__ void __launch_bounds__(64 * NW) k_colgemm_bf(GA ga, GB gb, int nk, in
   __shared__ __bf16 sA[NPL * NB16][KBP];
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6, li = lane & 15, lk = lane >> 4;
   const int g0 = blockIdx.x * WG, b0 = blockIdx.y * NB16;
-  flt4 accr[NRB], acci[NRB];
+  dbl4 accr[NRB], acci[NRB];   // FP64 sums of per-chunk FP32 MFMA sums
 #pragma unroll
   for (int c = 0; c < NRB; ++c) {
-    accr[c] = flt4{0.f, 0.f, 0.f, 0.f};
-    acci[c] = flt4{0.f, 0.f, 0.f, 0.f};
+    accr[c] = dbl4{0.0, 0.0, 0.0, 0.0};
+    acci[c] = dbl4{0.0, 0.0, 0.0, 0.0};
   }
   for (int k0 = 0; k0 < nk; k0 += KB) {
     {
@@ -788,7 +834,13 @@ __global__ void __launch_bounds__(64 * NW) k_colgemm_bf(GA ga, GB gb, int nk, in
       load_planes(sA, NB16, 16 * c + li, lk, a);
 #pragma unroll
       for (int q = 0; q < 3; ++q) an[q] = bf_neg(a[3 + q]);
-      cmfma_bf(accr[c], acci[c], a, an, bb);
+      flt4 cr = flt4{0.f, 0.f, 0.f, 0.f}, ci = flt4{0.f, 0.f, 0.f, 0.f};
+      cmfma_bf(cr, ci, a, an, bb);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        accr[c][r] += (double)cr[r];
+        acci[c][r] += (double)ci[r];
+      }
     }
     __syncthreads();
   }
@@ -801,9 +853,131 @@ __global__ void __launch_bounds__(64 * NW) k_colgemm_bf(GA ga, GB gb, int nk, in
       const int b = b0 + 16 * c + 4 * lk + r;
       if (b < nbh) {
         double2 &o = psi[(size_t)b * ngl + ig];
-        o = make_double2(o.x + (double)accr[c][r], o.y + (double)acci[c][r]);
+        o = make_double2(o.x + accr[c][r], o.y + acci[c][r]);
       }
     }
+  }
+}
+
+// ---- forces in the mixed form ------------------------------------------------
+// D(ipro,G) = sum_b conj(ded[ipro][b]) psi[b][G] as a split-BF16x3 column
+// product (rows ipro, columns G, FP32 accumulation).  The epilogue forms
+// P(ipro,G) in FP64 from the projector factors and sums Re(i P conj(D)) G over the block's G range in FP64:
+// part[gtile][ipro][3] (WAVES_DEDPRO + WAVES_PROFORCE, src/paw_waves1.f90:
+// 5694-5745, 6112-6234).
+struct ForceAbf {
+  const double2 *ded;
+  int npro, nbh;
+  __device__ void operator()(int ip, int k0, Planes8 &o) const {
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int b = k0 + e;
+      put(o, e, (ip < npro && b < nbh) ? cconj(ded[(size_t)ip * nbh + b]) : czero());
+    }
+  }
+};
+struct ForceBbf {
+  const double2 *psi;
+  int ngl, nbh;
+  __device__ void operator()(int k0, int ig, Planes8 &o) const {
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int b = k0 + e;
+      put(o, e, (ig < ngl && b < nbh) ? psi[(size_t)b * ngl + ig] : czero());
+    }
+  }
+};
+
+template <int NRB, int NW>
+__global__ void __launch_bounds__(64 * NW) k_force_bf(ForceAbf ga, ForceBbf gb, PFac pf,
+                                                      const double *__restrict__ gvec,
+                                                      double *__restrict__ part) {
+  constexpr int WG = 16 * NW, NB16 = 16 * NRB;
+  __shared__ __bf16 sB[NPL * WG][KBP];
+  __shared__ __align__(16) __bf16 sA[NPL * NB16][KBP];
+  // per-wave row sums of the epilogue reuse the A staging area (free after the K loop)
+  static_assert(sizeof(double) * NW * NB16 * 3 <= sizeof(__bf16) * NPL * NB16 * KBP, "sF fits in sA");
+  double (*sF)[NB16][3] = reinterpret_cast<double (*)[NB16][3]>(&sA[0][0]);
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6, li = lane & 15, lk = lane >> 4;
+  const int g0 = blockIdx.x * WG, p0 = blockIdx.y * NB16;
+  const int npro = ga.npro, nk = ga.nbh, ngl = gb.ngl;
+  flt4 accr[NRB], acci[NRB];
+#pragma unroll
+  for (int c = 0; c < NRB; ++c) {
+    accr[c] = flt4{0.f, 0.f, 0.f, 0.f};
+    acci[c] = flt4{0.f, 0.f, 0.f, 0.f};
+  }
+  for (int k0 = 0; k0 < nk; k0 += KB) {
+    {
+      const int g = t % WG, k8 = t / WG;
+      Planes8 v;
+      gb(k0 + 8 * k8, g0 + g, v);
+      store_planes(sB, WG, g, k8, v);
+    }
+    for (int e = t; e < NB16 * 4; e += 64 * NW) {
+      const int r = e >> 2, k8 = e & 3;
+      Planes8 v;
+      ga(p0 + r, k0 + 8 * k8, v);
+      store_planes(sA, NB16, r, k8, v);
+    }
+    __syncthreads();
+    bf16x8 bb[NPL];
+    load_planes(sB, WG, 16 * wv + li, lk, bb);
+#pragma unroll
+    for (int c = 0; c < NRB; ++c) {
+      bf16x8 a[NPL], an[3];
+      load_planes(sA, NB16, 16 * c + li, lk, a);
+#pragma unroll
+      for (int q = 0; q < 3; ++q) an[q] = bf_neg(a[3 + q]);
+      cmfma_bf(accr[c], acci[c], a, an, bb);
+    }
+    __syncthreads();
+  }
+  // C(i = ipro row, j = G column) sits in lane j + 16 (i/4), element i%4
+  const int ig = g0 + 16 * wv + li;
+  double gx = 0.0, gy = 0.0, gz = 0.0;
+  if (ig < ngl) {
+    gx = gvec[3 * ig];
+    gy = gvec[3 * ig + 1];
+    gz = gvec[3 * ig + 2];
+  }
+#pragma unroll
+  for (int c = 0; c < NRB; ++c) {
+    double f[4][3];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int ip = p0 + 16 * c + 4 * lk + r;
+      double cc = 0.0;
+      if (ig < ngl && ip < npro) {
+        const double2 pv = pf.at(ip, ig);
+        // Re(i p conj(d)) = p.x d.y - p.y d.x
+        cc = pv.x * (double)acci[c][r] - pv.y * (double)accr[c][r];
+      }
+      f[r][0] = cc * gx;
+      f[r][1] = cc * gy;
+      f[r][2] = cc * gz;
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+#pragma unroll
+        for (int off = 8; off > 0; off >>= 1) f[r][d] += __shfl_xor(f[r][d], off);
+    if (li == 0) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) sF[wv][16 * c + 4 * lk + r][d] = f[r][d];
+    }
+  }
+  __syncthreads();
+  for (int e = t; e < NB16 * 3; e += 64 * NW) {
+    const int r = e / 3, d = e % 3, ip = p0 + r;
+    if (ip >= npro) continue;
+    double s = 0.0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) s += sF[w][r][d];
+    part[((size_t)blockIdx.x * npro + ip) * 3 + d] = s;
   }
 }
 
@@ -1146,25 +1320,36 @@ int k_phase_tables(pw_waveset *ws, const double *r, double2 *phase) {
   return PW_OK;
 }
 
-// P8 (split projector matrix) for the positions of the current phase tables
-int k_pgen(pw_waveset *ws) {
+// projector factors F and E for the positions of the current phase tables
+int k_pfac(pw_waveset *ws) {
   const int nglp = nglp_of(ws);
-  const size_t n = (size_t)ws->npro * nglp;
-  if (n * sizeof(uint4) > ws->p8_bytes) {
-    if (ws->d_p8) PW_HIP(hipFree(ws->d_p8));
-    PW_HIP(hipMalloc(&ws->d_p8, n * sizeof(uint4)));
-    ws->p8_bytes = n * sizeof(uint4);
+  const size_t nf = (size_t)ws->nsp * ws->lmnxx * nglp, ne = (size_t)ws->nat * nglp;
+  const size_t bytes = nf * sizeof(double) + ne * sizeof(double2);
+  if (bytes > ws->pfac_bytes) {
+    if (ws->d_pfac) PW_HIP(hipFree(ws->d_pfac));
+    PW_HIP(hipMalloc(&ws->d_pfac, bytes));
+    ws->pfac_bytes = bytes;
   }
   ProjGeo q = make_pgeo(ws);
+  q.phase = ws->d_phase;
+  double *f = (double *)ws->d_pfac;
   ScopeTimer tm(ws, "pgen");
-  k_pgen_bf<<<(n + 255) / 256, 256, 0, ws->stream>>>(q, ws->d_ipro_tab, ws->npro, nglp, (uint4 *)ws->d_p8);
+  k_pfac_f<<<(nf + 255) / 256, 256, 0, ws->stream>>>(q, ws->nsp, nglp, f);
+  k_pfac_e<<<(ne + 255) / 256, 256, 0, ws->stream>>>(q, nglp, (double2 *)(f + nf));
   PW_HIP(hipGetLastError());
   return PW_OK;
 }
 
+PFac pfac_of(const pw_waveset *ws) {
+  const int nglp = nglp_of(ws);
+  const double *f = (const double *)ws->d_pfac;
+  const size_t nf = (size_t)ws->nsp * ws->lmnxx * nglp;
+  return PFac{f, (const double2 *)(f + nf), ws->d_ipro_tab, ws->d_atom_isp, ws->d_lmn_l, ws->lmnxx, nglp};
+}
+
 int k_projections(pw_waveset *ws, const double2 *psi, const double2 *phase, double2 *proj) {
   if (ws->prec == PW_PREC_MIXED) {
-    RowP8 gen{(const uint4 *)ws->d_p8, nglp_of(ws)};
+    RowPf gen{pfac_of(ws)};
     return run_rowgemm_bf<4>(ws, gen, ws->npro, psi, ws->plan->vol, proj, nullptr, "projections");
   }
   ProjGeo q = make_pgeo(ws);
@@ -1176,7 +1361,7 @@ int k_projections(pw_waveset *ws, const double2 *psi, const double2 *phase, doub
 int k_addpro(pw_waveset *ws, double2 *psi, const double2 *phase, const double2 *x) {
   if (ws->prec == PW_PREC_MIXED) {
     AddproAbf ga{x, ws->nbh, ws->npro};
-    AddproBbf gb{(const uint4 *)ws->d_p8, nglp_of(ws), ws->npro};
+    AddproBf gb{pfac_of(ws), ws->npro};
     return run_colgemm_bf<2>(ws, ga, gb, ws->npro, psi, "addpro");
   }
   ProjGeo q = make_pgeo(ws);
@@ -1190,8 +1375,27 @@ int k_force(pw_waveset *ws, const double2 *psi, const double2 *phase, const doub
   constexpr int NW = 4, NGB = 4;
   ProjGeo q = make_pgeo(ws);
   q.phase = phase;
-  const int ngt = (q.ngl + 16 * NGB - 1) / (16 * NGB);
   const size_t n = (size_t)ws->npro * 3;
+  if (ws->prec == PW_PREC_MIXED) {
+    // the projector factors hold P at the positions of the last phase_at
+    constexpr int NRB = 4;
+    const int ngt = (q.ngl + 16 * NW - 1) / (16 * NW);
+    PW_TRY(ws_part(ws, (size_t)ngt * n * sizeof(double)));
+    {
+      ScopeTimer tm(ws, "force");
+      dim3 grid(ngt, (ws->npro + 16 * NRB - 1) / (16 * NRB));
+      ForceAbf ga{ded, ws->npro, ws->nbh};
+      ForceBbf gb{psi, q.ngl, ws->nbh};
+      k_force_bf<NRB, NW><<<grid, 64 * NW, 0, ws->stream>>>(ga, gb, pfac_of(ws), ws->plan->d_gvec,
+                                                            ws->d_part);
+      PW_HIP(hipGetLastError());
+    }
+    k_reduce_split<double><<<(n + kRedC - 1) / kRedC, kRedC * kRedS, 0, ws->stream>>>(ws->d_part, ngt, (int)n,
+                                                                                        2.0 * ws->plan->vol, dst);
+    PW_HIP(hipGetLastError());
+    return PW_OK;
+  }
+  const int ngt = (q.ngl + 16 * NGB - 1) / (16 * NGB);
   PW_TRY(ws_part(ws, (size_t)ngt * n * sizeof(double)));
   {
     ScopeTimer tm(ws, "force");
@@ -1229,10 +1433,15 @@ int k_overlap(pw_waveset *ws, const double2 *psi1, const double2 *psi2, double *
 // mat_dev += WAVES_1COVERLAP(proj1, proj2); uses d_xproj as DO*proj2 scratch
 int k_overlap_1c(pw_waveset *ws, const double2 *proj1, const double2 *proj2, double *mat) {
   ProjGeo q = make_pgeo(ws);
+  const int nbh = ws->nbh, nz = (ws->npro + k1c - 1) / k1c;
+  // the overlap matrix read from d_part (k_unravel_pw) is ahead on the stream
+  PW_TRY(ws_part(ws, (size_t)nz * nbh * nbh * sizeof(double4)));
   ScopeTimer tm(ws, "overlap1c");
   k_atom_mat<<<ws->nat, 256, 0, ws->stream>>>(q, proj2, ws->d_domat, ws->d_xproj);
-  dim3 grid((ws->nbh + 15) / 16, (ws->nbh + 15) / 16);
-  k_overlap1c<<<grid, 256, 0, ws->stream>>>(ws->npro, ws->nbh, ws->nb, proj1, ws->d_xproj, mat);
+  dim3 grid((nbh + 15) / 16, (nbh + 15) / 16, nz);
+  k_overlap1c<<<grid, 256, 0, ws->stream>>>(ws->npro, nbh, proj1, ws->d_xproj, (double4 *)ws->d_part);
+  k_overlap1c_sum<<<(nbh * nbh + 255) / 256, 256, 0, ws->stream>>>(nbh, ws->nb, nz, (const double4 *)ws->d_part,
+                                                                    mat);
   PW_HIP(hipGetLastError());
   return PW_OK;
 }

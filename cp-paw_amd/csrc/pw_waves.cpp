@@ -143,11 +143,11 @@ static int upload_mass(pw_waveset *ws) {
 
 static int phase_at(pw_waveset *ws, const std::vector<double> &r) {
   PW_TRY(k_phase_tables(ws, r.data(), ws->d_phase));
-  // the mixed form's products read the split projector matrix of these positions
-  if (ws->prec == PW_PREC_MIXED && (!ws->p8_valid || ws->p8_pos != r)) {
-    PW_TRY(k_pgen(ws));
-    ws->p8_pos = r;
-    ws->p8_valid = true;
+  // the mixed form's products read the projector factors of these positions
+  if (ws->prec == PW_PREC_MIXED && (!ws->pfac_valid || ws->pfac_pos != r)) {
+    PW_TRY(k_pfac(ws));
+    ws->pfac_pos = r;
+    ws->pfac_valid = true;
   }
   return PW_OK;
 }
@@ -365,7 +365,7 @@ extern "C" int pw_waves_destroy(pw_waveset *ws) {
                   ws->d_xproj, ws->d_pro, ws->d_ylm, ws->d_mpsi, ws->d_v, ws->d_rho, ws->d_occ2,
                   ws->d_fpm, ws->d_fab, ws->d_domat, ws->d_dhmat, ws->d_atom_isp, ws->d_ipro0,
                   ws->d_ibpro, ws->d_lmnx, ws->d_lmn_ln, ws->d_lmn_lm, ws->d_lmn_l, ws->d_phase,
-                  ws->d_part, ws->d_small, ws->d_ortho, ws->d_small_mats, ws->d_ipro_tab, ws->d_p8};
+                  ws->d_part, ws->d_small, ws->d_ortho, ws->d_small_mats, ws->d_ipro_tab, ws->d_pfac};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (ws->d_res) (void)hipFree(ws->d_res);
@@ -411,6 +411,7 @@ extern "C" int pw_waves_setr8a(pw_waveset *ws, const char *id, int len, const do
     PW_TRY(chk((size_t)p->ngl * ws->nbarepro));
     // caller layout PRO(NGL,NBAREPRO) column-major == [ibare][ngl]
     PW_HIP(hipMemcpy(ws->d_pro, val, (size_t)len * sizeof(double), hipMemcpyHostToDevice));
+    ws->pfac_valid = false;
   } else if (s == "OCC") {
     PW_TRY(chk(ws->nb));
     ws->occ.assign(val, val + ws->nb);
@@ -643,7 +644,7 @@ extern "C" int pw_waves_etot(pw_waveset *ws) {
   if (!ws) return pw_fail(PW_ERR_ARG, "pw_waves_etot: null waveset");
   PW_TRY(ready(ws));
   pw_plan *p = ws->plan;
-  ws->p8_valid = false;   // the projector expansion is redone at least once per step
+  ws->pfac_valid = false;   // the projector factors are redone at least once per step
   if (ws->first) {
     PW_TRY(pw_waves_gramschmidt(ws));
     PW_TRY(pw_waves_projections(ws));

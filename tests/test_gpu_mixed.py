@@ -1,8 +1,16 @@
 """The mixed-precision form (FP32 FFT passes, split-BF16x3 MFMA products,
-FP64 constraint solve, energies and forces) against the FP64 oracle: the
-error gates |dE| < 1e-6 Ha/atom and |dF| < 1e-4 Ha/bohr per step, and the
-orthonormality residual ||psi^H S psi - 1|| of the mixed form's wave
-functions measured in the oracle's FP64 metric."""
+FP64 constraint solve, energies and forces) against the FP64 oracle.
+
+Gates, per step of the oracle's FP64 trajectory, with the mixed form handed
+the oracle's state (PSI0, PSIM, RLAM0): |dE| < 1e-6 Ha/atom and
+|dF| < 1e-4 Ha/bohr of WAVES$ETOT, and the orthonormality residual
+||psi^H S psi - 1|| of the wave functions that one mixed PROPAGATE +
+ORTHOGONALIZE makes from that state, measured in the oracle's FP64 metric.
+
+Free-running trajectories of the two forms separate over the steps: the
+mixed form keeps the constraint to FP32-level accuracy (~1e-7), and the
+CP dynamics carries such differences forward.  That drift is measured and
+bounded loosely (it catches a broken step, not rounding)."""
 import numpy as np
 import pytest
 
@@ -14,6 +22,8 @@ pytestmark = pytest.mark.gpu
 DE_ATOM = 1.0e-6      # Ha/atom
 DF = 1.0e-4           # Ha/bohr
 ORTHO_RES = 1.0e-5    # max |psi^H S psi - 1| in FP64, mixed-form wave functions
+DRIFT_DE_ATOM = 1.0e-4
+DRIFT_DF = 1.0e-2
 
 
 def say(msg):
@@ -36,9 +46,14 @@ def test_mixed_form_error_gates(name, nsteps):
     o = oracle_waveset(s)
     o.r0 = s.pos
     plane, w = gpu_waveset(s, prec=1)
+    w.etot()                          # first call: Gram-Schmidt of its own start
     worst = [0.0, 0.0, 0.0]
     for it in range(nsteps):
         eo = o.etot(s.pos)
+        w.setc8a("PSI0", o.psi0)
+        w.setc8a("PSIM", o.psim)
+        w.set_matrix_c("RLAM0", np.asarray(o.rlam0, dtype=np.complex128))
+        w.projections()
         e = w.etot()
         de = abs(e["etot"] - eo["etot"]) / s.nat
         df = float(np.max(np.abs(w.getr8a("FORCE") - o.force)))
@@ -52,8 +67,29 @@ def test_mixed_form_error_gates(name, nsteps):
         assert df < DF, (it, df)
         assert res < ORTHO_RES, (it, res)
         o.switch()
-        w.switch()
     say(f"  worst |dE|/atom {worst[0]:.2e}  |dF| {worst[1]:.2e}  ortho residual {worst[2]:.2e}")
+
+
+@pytest.mark.parametrize("name,nsteps", [("si8", 5), ("h2o4", 4)])
+def test_mixed_trajectory_drift(name, nsteps):
+    s = SYSTEMS[name]()
+    say(f"case mixed drift {name} x{nsteps}")
+    o = oracle_waveset(s)
+    plane, w = gpu_waveset(s, prec=1)
+    for it in range(nsteps):
+        eo = o.etot(s.pos)
+        e = w.etot()
+        de = abs(e["etot"] - eo["etot"]) / s.nat
+        df = float(np.max(np.abs(w.getr8a("FORCE") - o.force)))
+        say(f"  step {it}: |dE|/atom {de:.2e}  |dF| {df:.2e}")
+        assert de < DRIFT_DE_ATOM, (it, de)
+        assert df < DRIFT_DF, (it, df)
+        o.propagate()
+        o.orthogonalize(s.pos, s.pos)
+        w.propagate()
+        w.orthogonalize()
+        o.switch()
+        w.switch()
 
 
 def test_mixed_projections_accuracy():
