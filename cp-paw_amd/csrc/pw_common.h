@@ -4,6 +4,7 @@
 
 #include <cstdint>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -14,6 +15,17 @@
 // ---------------------------------------------------------------------------
 void pw_set_error(const std::string &msg);
 int pw_fail(int code, const std::string &msg);
+
+// Stream capture (the hipGraph of ORTHO_X) runs only while a single plan and a
+// single waveset exist in the process: a device-synchronising call made by
+// another thread during a capture invalidates it and leaves the stream
+// unusable.  Plans and wavesets are counted under the same lock that a capture
+// holds, before their first allocation and after their last free.
+struct CaptureGate {
+  std::mutex mu;
+  int plans = 0, wavesets = 0;
+};
+CaptureGate &capture_gate();
 
 #define PW_HIP(call)                                                         \
   do {                                                                       \
@@ -173,6 +185,7 @@ struct pw_waveset {
   double lz_lo = 0.0, lz_hi = 0.0;
   bool lz_valid = false;
   int ortho_lanczos = 0;       // Lanczos runs in the last ORTHO_X call
+  int ortho_graph_misses = 0;  // ORTHO_X solves issued without a graph (other objects alive)
   // launch shapes chosen by pw_waves_autotune
   int tune_split_waves = 2048;  // waves of a G-split GEMM launch
   // device results of a step section and their pinned host copy; the host

@@ -77,6 +77,71 @@ template <> struct Roots<11> {
                                    -0.90963199535451837141, -0.54064081745559758211};
 };
 
+template <> struct Roots<9> {
+  static constexpr double c[9] = {1.0, 0.766044443118978035202, 0.173648177666930348852, -0.5,
+                                  -0.939692620785908384054, -0.939692620785908384054, -0.5,
+                                  0.173648177666930348852, 0.766044443118978035202};
+  static constexpr double s[9] = {0.0, 0.642787609686539326323, 0.984807753012208059367,
+                                  0.866025403784438646764, 0.342020143325668733044,
+                                  -0.342020143325668733044, -0.866025403784438646764,
+                                  -0.984807753012208059367, -0.642787609686539326323};
+};
+template <> struct Roots<16> {
+  static constexpr double c[16] = {1.0, 0.923879532511286756128, 0.707106781186547524401,
+                                   0.382683432365089771728, 0.0, -0.382683432365089771728,
+                                   -0.707106781186547524401, -0.923879532511286756128, -1.0,
+                                   -0.923879532511286756128, -0.707106781186547524401,
+                                   -0.382683432365089771728, 0.0, 0.382683432365089771728,
+                                   0.707106781186547524401, 0.923879532511286756128};
+  static constexpr double s[16] = {0.0, 0.382683432365089771728, 0.707106781186547524401,
+                                   0.923879532511286756128, 1.0, 0.923879532511286756128,
+                                   0.707106781186547524401, 0.382683432365089771728, 0.0,
+                                   -0.382683432365089771728, -0.707106781186547524401,
+                                   -0.923879532511286756128, -1.0, -0.923879532511286756128,
+                                   -0.707106781186547524401, -0.382683432365089771728};
+};
+
+template <typename T2, int R>
+__device__ inline void dft(T2 *v, int sign);
+
+// DFT of size R = P Q from P-point and Q-point DFTs: input r = Q a + b,
+// output q = q1 + P q2; Y_b = DFT_P over a, times exp(sign 2 pi i b q1 / R),
+// then DFT_Q over b.
+template <typename T2, int P, int Q>
+__device__ inline void dft_pq(T2 *v, int sign) {
+  typedef typename Prec<T2>::T T;
+  constexpr int R = P * Q;
+  T2 y[Q][P];
+#pragma unroll
+  for (int b = 0; b < Q; ++b) {
+    T2 t[P];
+#pragma unroll
+    for (int a = 0; a < P; ++a) t[a] = v[Q * a + b];
+    dft<T2, P>(t, sign);
+#pragma unroll
+    for (int q1 = 0; q1 < P; ++q1) y[b][q1] = t[q1];
+  }
+#pragma unroll
+  for (int b = 1; b < Q; ++b)
+#pragma unroll
+    for (int q1 = 1; q1 < P; ++q1) {
+      const int m = (b * q1) % R;
+      T2 w;
+      w.x = (T)Roots<R>::c[m];
+      w.y = sign > 0 ? (T)Roots<R>::s[m] : -(T)Roots<R>::s[m];
+      y[b][q1] = cmul(y[b][q1], w);
+    }
+#pragma unroll
+  for (int q1 = 0; q1 < P; ++q1) {
+    T2 t[Q];
+#pragma unroll
+    for (int b = 0; b < Q; ++b) t[b] = y[b][q1];
+    dft<T2, Q>(t, sign);
+#pragma unroll
+    for (int q2 = 0; q2 < Q; ++q2) v[q1 + P * q2] = t[q2];
+  }
+}
+
 // in-register DFT of size R: v[q] <- sum_r v[r] exp(sign 2 pi i r q / R)
 template <typename T2, int R>
 __device__ inline void dft(T2 *v, int sign) {
@@ -89,6 +154,10 @@ __device__ inline void dft(T2 *v, int sign) {
     T2 t2 = cadd(v[1], v[3]), t3 = mul_si(csub(v[1], v[3]), sign);
     v[0] = cadd(t0, t2); v[2] = csub(t0, t2);
     v[1] = cadd(t1, t3); v[3] = csub(t1, t3);
+  } else if constexpr (R == 9) {
+    dft_pq<T2, 3, 3>(v, sign);
+  } else if constexpr (R == 16) {
+    dft_pq<T2, 4, 4>(v, sign);
   } else if constexpr (R == 8) {
     T2 e[4] = {v[0], v[2], v[4], v[6]};
     T2 o[4] = {v[1], v[3], v[5], v[7]};
@@ -182,6 +251,8 @@ __device__ T2 *lds_fft(T2 *b0, T2 *b1, const Fft1D &f, int npen, int es, int ps,
       case 7: stage<T2, 7>(in, out, f.n, ns, npen, es, ps, tw, sign, pen_fast); break;
       case 8: stage<T2, 8>(in, out, f.n, ns, npen, es, ps, tw, sign, pen_fast); break;
       case 11: stage<T2, 11>(in, out, f.n, ns, npen, es, ps, tw, sign, pen_fast); break;
+      case 9: stage<T2, 9>(in, out, f.n, ns, npen, es, ps, tw, sign, pen_fast); break;
+      case 16: stage<T2, 16>(in, out, f.n, ns, npen, es, ps, tw, sign, pen_fast); break;
       default: break;
     }
     ns *= r;
@@ -397,6 +468,8 @@ __device__ inline void ip_stage(T2 *buf, int n, int L, int npen, int es, int ps,
     case 7: CALL(7); break;      \
     case 8: CALL(8); break;      \
     case 11: CALL(11); break;    \
+    case 9: CALL(9); break;      \
+    case 16: CALL(16); break;    \
     default: break;              \
   }
 

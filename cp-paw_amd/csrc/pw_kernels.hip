@@ -778,8 +778,24 @@ struct AddproBf {
   PFac pf;
   int nk;
   __device__ void operator()(int k0, int ig, Planes8 &o) const {
+    if (ig >= pf.nglp) {
+      zero8(o);
+      return;
+    }
+    if (k0 + 7 < nk && pf.iptab[k0].x == pf.iptab[k0 + 7].x) {
+      // eight projectors of one atom: one structure factor
+      const int2 al = pf.iptab[k0];
+      const double2 ee = pf.e[(size_t)al.x * pf.nglp + ig];
+      const int k = pf.atom_isp[al.x] * pf.lmnxx + al.y;
 #pragma unroll
-    for (int e = 0; e < 8; ++e) put(o, e, (k0 + e < nk && ig < pf.nglp) ? pf.at(k0 + e, ig) : czero());
+      for (int e = 0; e < 8; ++e) {
+        const double sv = pf.f[(size_t)(k + e) * pf.nglp + ig];
+        put(o, e, rot_mil(make_double2(sv * ee.x, sv * ee.y), pf.lmn_l[k + e]));
+      }
+      return;
+    }
+#pragma unroll
+    for (int e = 0; e < 8; ++e) put(o, e, k0 + e < nk ? pf.at(k0 + e, ig) : czero());
   }
 };
 struct AddopsiBbf {
@@ -898,9 +914,23 @@ __global__ void __launch_bounds__(64 * NW) k_force_bf(ForceAbf ga, ForceBbf gb, 
   // per-wave row sums of the epilogue reuse the A staging area (free after the K loop)
   static_assert(sizeof(double) * NW * NB16 * 3 <= sizeof(__bf16) * NPL * NB16 * KBP, "sF fits in sA");
   double (*sF)[NB16][3] = reinterpret_cast<double (*)[NB16][3]>(&sA[0][0]);
+  // F row, atom and l of the block's projector rows
+  __shared__ int sRow[NB16], sAtom[NB16], sL[NB16];
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6, li = lane & 15, lk = lane >> 4;
   const int g0 = blockIdx.x * WG, p0 = blockIdx.y * NB16;
   const int npro = ga.npro, nk = ga.nbh, ngl = gb.ngl;
+  if (t < NB16) {
+    int row = 0, atom = 0, l = 0;
+    if (p0 + t < npro) {
+      const int2 al = pf.iptab[p0 + t];
+      row = pf.atom_isp[al.x] * pf.lmnxx + al.y;
+      atom = al.x;
+      l = pf.lmn_l[row];
+    }
+    sRow[t] = row;
+    sAtom[t] = atom;
+    sL[t] = l;
+  }
   flt4 accr[NRB], acci[NRB];
 #pragma unroll
   for (int c = 0; c < NRB; ++c) {
@@ -949,7 +979,10 @@ __global__ void __launch_bounds__(64 * NW) k_force_bf(ForceAbf ga, ForceBbf gb, 
       const int ip = p0 + 16 * c + 4 * lk + r;
       double cc = 0.0;
       if (ig < ngl && ip < npro) {
-        const double2 pv = pf.at(ip, ig);
+        const int lr = 16 * c + 4 * lk + r;
+        const double sv = pf.f[(size_t)sRow[lr] * pf.nglp + ig];
+        const double2 ee = pf.e[(size_t)sAtom[lr] * pf.nglp + ig];
+        const double2 pv = rot_mil(make_double2(sv * ee.x, sv * ee.y), sL[lr]);
         // Re(i p conj(d)) = p.x d.y - p.y d.x
         cc = pv.x * (double)acci[c][r] - pv.y * (double)accr[c][r];
       }

@@ -571,16 +571,23 @@ int ortho_x_device(pw_waveset *ws, double *d_lam, const double *d_chichi, const 
     P.q = std::sqrt(P.lo * P.hi);
     const double kap = P.hi / P.lo;
     const double rho = (std::sqrt(kap) - 1.0) / (std::sqrt(kap) + 1.0);
+    // J Smith levels sum A^k X0 A^k for k < 2^J; the tail is bounded by
+    // rho^(2^(J+1)).  J is the least level count whose last power
+    // A^(2^(J-1)) (<= rho^(2^(J-1))) passes the acceptance test with a
+    // factor 10 to spare, n max|A^(2^(J-1))| <= 1e-6; the tail is then below
+    // (1e-6/n)^4, far under FP64 rounding.
     P.nlev = 1;
     if (rho > 1.0e-300) {
-      const double need = std::log(1.0e-17) / std::log(rho);
-      P.nlev = std::max(1, (int)std::ceil(std::log2(std::max(need, 1.0)))) + 1;
+      const double need = std::log(1.0e-6 / n) / std::log(rho);
+      P.nlev = std::max(1, (int)std::ceil(std::log2(std::max(need, 1.0))) + 1);
     }
     if (P.nlev > kMaxLevels) return pw_fail(PW_ERR_NOCONV, "WAVES_ORTHO_X: CHIPSI too ill-conditioned");
+    // Newton-Schulz for (B+qI)^-1: |I - (B+qI)X_k| = e0^(2^k); stop at 1e-17
+    // (the acceptance test asks for 1e-11)
     const double kp = (P.hi + P.q) / (P.lo + P.q);
     const double e0 = (kp - 1.0) / (kp + 1.0);
     P.nns = 1;
-    if (e0 > 1.0e-300) P.nns = (int)std::ceil(std::log2(std::max(1.0, std::log(1.0e-18) / std::log(e0)))) + 1;
+    if (e0 > 1.0e-300) P.nns = std::max(1, (int)std::ceil(std::log2(std::max(1.0, std::log(1.0e-17) / std::log(e0)))));
     ws->ortho_levels = P.nlev;
     ws->ortho_ns = P.nns;
     const int batch = std::max(2, std::min(ws->ortho_iter > 0 ? ws->ortho_iter : 8, kMaxIter));
@@ -590,26 +597,40 @@ int ortho_x_device(pw_waveset *ws, double *d_lam, const double *d_chichi, const 
     double *mc = (P.nns & 1) ? w.m1 : w.m0;
     if (!ws->ortho_exec || !(ws->ortho_key == key)) {
       ortho_graph_release(ws);
-      PW_HIP(hipStreamBeginCapture(ws->stream, hipStreamCaptureModeRelaxed));
+      // with other plans or wavesets alive (a request loop) the work is issued
+      // directly below; an existing graph is still replayed
+      CaptureGate &gate = capture_gate();
+      std::unique_lock<std::mutex> lk(gate.mu);
+      if (gate.plans <= 1 && gate.wavesets <= 1) {
+        PW_HIP(hipStreamBeginCapture(ws->stream, hipStreamCaptureModeRelaxed));
+        enqueue_prep(ws, w, P, &mc);
+        enqueue_iters(ws, w, P, mc, lbuf, d_chichi, d_psipsi, d_chipsi, 1, batch);
+        hipGraph_t graph = nullptr;
+        const hipError_t ec = hipStreamEndCapture(ws->stream, &graph);
+        const hipError_t el = hipGetLastError();
+        if (ec != hipSuccess || el != hipSuccess || !graph) {
+          if (graph) (void)hipGraphDestroy(graph);
+          return pw_fail(PW_ERR_HIP, std::string("WAVES_ORTHO_X: graph capture failed: ") +
+                                         hipGetErrorString(ec != hipSuccess ? ec : el));
+        }
+        hipGraphExec_t exec = nullptr;
+        const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ei != hipSuccess)
+          return pw_fail(PW_ERR_HIP, std::string("WAVES_ORTHO_X: graph instantiation failed: ") +
+                                         hipGetErrorString(ei));
+        ws->ortho_exec = exec;
+        ws->ortho_key = key;
+      }
+    }
+    if (ws->ortho_exec) {
+      PW_HIP(hipGraphLaunch((hipGraphExec_t)ws->ortho_exec, ws->stream));
+    } else {
       enqueue_prep(ws, w, P, &mc);
       enqueue_iters(ws, w, P, mc, lbuf, d_chichi, d_psipsi, d_chipsi, 1, batch);
-      hipGraph_t graph = nullptr;
-      const hipError_t ec = hipStreamEndCapture(ws->stream, &graph);
-      const hipError_t el = hipGetLastError();
-      if (ec != hipSuccess || el != hipSuccess || !graph) {
-        if (graph) (void)hipGraphDestroy(graph);
-        return pw_fail(PW_ERR_HIP, std::string("WAVES_ORTHO_X: graph capture failed: ") +
-                                       hipGetErrorString(ec != hipSuccess ? ec : el));
-      }
-      hipGraphExec_t exec = nullptr;
-      const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-      (void)hipGraphDestroy(graph);
-      if (ei != hipSuccess) return pw_fail(PW_ERR_HIP, std::string("WAVES_ORTHO_X: graph instantiation failed: ") +
-                                                           hipGetErrorString(ei));
-      ws->ortho_exec = exec;
-      ws->ortho_key = key;
+      PW_HIP(hipGetLastError());
+      ws->ortho_graph_misses++;
     }
-    PW_HIP(hipGraphLaunch((hipGraphExec_t)ws->ortho_exec, ws->stream));
     int launched = batch;
     OrthoState hs{};
     for (;;) {

@@ -147,7 +147,7 @@ bool fft_factor(int n, Fft1D &f) {
   f = Fft1D();
   f.n = n;
   int m = n;
-  const int order[] = {8, 4, 2, 3, 5, 7, 11};
+  const int order[] = {16, 8, 9, 4, 2, 3, 5, 7, 11};
   for (int r : order) {
     while (m % r == 0) {
       f.radix[f.nstage++] = r;
@@ -375,6 +375,10 @@ extern "C" int pw_plan_create(const double rbas[9], const double kvec[3], int ti
     return pw_fail(PW_ERR_UNSUPPORTED, "pw_plan_create: only super wave functions (TINV) are supported");
   if (!(epw_psi > 0.0) || !(epw_rho >= epw_psi))
     return pw_fail(PW_ERR_ARG, "pw_plan_create: need 0 < epw_psi <= epw_rho");
+  {
+    std::lock_guard<std::mutex> lk(capture_gate().mu);
+    capture_gate().plans++;
+  }
   pw_plan *p = new pw_plan();
   int rc = build_plan(p, rbas, epw_psi, epw_rho);
   if (rc != PW_OK) {
@@ -383,6 +387,11 @@ extern "C" int pw_plan_create(const double rbas[9], const double kvec[3], int ti
   }
   *out = p;
   return PW_OK;
+}
+
+CaptureGate &capture_gate() {
+  static CaptureGate g;
+  return g;
 }
 
 extern "C" int pw_plan_destroy(pw_plan *p) {
@@ -394,6 +403,8 @@ extern "C" int pw_plan_destroy(pw_plan *p) {
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   delete p;
+  std::lock_guard<std::mutex> lk(capture_gate().mu);
+  capture_gate().plans--;
   return PW_OK;
 }
 

@@ -239,13 +239,20 @@ extern "C" int pw_waves_create(pw_plan *plan, int nb, int nat, int nsp, const in
   if (nb > 1024) return pw_fail(PW_ERR_UNSUPPORTED, "pw_waves_create: NB > 1024 is not supported");
   if (prec != PW_PREC_FP64 && prec != PW_PREC_MIXED) return pw_fail(PW_ERR_ARG, "pw_waves_create: bad precision");
   PW_TRY(plan_device(plan));
+  {
+    std::lock_guard<std::mutex> lk(capture_gate().mu);
+    capture_gate().wavesets++;
+  }
   pw_waveset *ws = new pw_waveset();
   ws->plan = plan;
   ws->prec = prec;
   ws->nb = nb; ws->nbh = (nb + 1) / 2; ws->nat = nat; ws->nsp = nsp; ws->lnxx = lnxx;
   ws->isp.resize(nat);
   for (int a = 0; a < nat; ++a) {
-    if (isp[a] < 1 || isp[a] > nsp) { delete ws; return pw_fail(PW_ERR_ARG, "pw_waves_create: ISP out of range"); }
+    if (isp[a] < 1 || isp[a] > nsp) {
+      pw_waves_destroy(ws);
+      return pw_fail(PW_ERR_ARG, "pw_waves_create: ISP out of range");
+    }
     ws->isp[a] = isp[a] - 1;
   }
   ws->lnx.assign(lnx, lnx + nsp);
@@ -254,7 +261,10 @@ extern "C" int pw_waves_create(pw_plan *plan, int nb, int nat, int nsp, const in
   ws->ibpro.assign(nsp, 0);
   int lmax = 0;
   for (int s = 0; s < nsp; ++s) {
-    if (lnx[s] < 0 || lnx[s] > lnxx) { delete ws; return pw_fail(PW_ERR_ARG, "pw_waves_create: LNX > LNXX"); }
+    if (lnx[s] < 0 || lnx[s] > lnxx) {
+      pw_waves_destroy(ws);
+      return pw_fail(PW_ERR_ARG, "pw_waves_create: LNX > LNXX");
+    }
     ws->ibpro[s] = ws->nbarepro;
     ws->nbarepro += lnx[s];
     for (int ln = 0; ln < lnx[s]; ++ln) {
@@ -372,6 +382,8 @@ extern "C" int pw_waves_destroy(pw_waveset *ws) {
   if (ws->h_res) (void)hipHostFree(ws->h_res);
   if (ws->own_stream && ws->stream) (void)hipStreamDestroy(ws->stream);
   delete ws;
+  std::lock_guard<std::mutex> lk(capture_gate().mu);
+  capture_gate().wavesets--;
   return PW_OK;
 }
 
@@ -576,6 +588,7 @@ extern "C" int pw_waves_geti4(pw_waveset *ws, const char *id, int *val) {
   else if (s == "ORTHOLEVELS") *val = ws->ortho_levels;
   else if (s == "ORTHONS") *val = ws->ortho_ns;
   else if (s == "ORTHOLANCZOS") *val = ws->ortho_lanczos;
+  else if (s == "ORTHOGRAPHMISSES") *val = ws->ortho_graph_misses;
   else return pw_fail(PW_ERR_ARG, "WAVES$GETI4: ID not recognised: " + s);
   return PW_OK;
 }

@@ -125,7 +125,9 @@ int pw_waves_getr8(pw_waveset *ws, const char *id, double *val);
  * "LMNXX", "ORTHOLEVELS" (squared-Smith levels of the last solve), "ORTHONS"
  * (Newton-Schulz steps of the last solve), "ORTHOLANCZOS" (Lanczos runs for
  * the spectral bounds in the last solve: 0 when the previous bounds were
- * accepted). */
+ * accepted), "ORTHOGRAPHMISSES" (solves issued without a captured hipGraph:
+ * captures are made only while one plan and one waveset exist in the
+ * process). */
 int pw_waves_geti4(pw_waveset *ws, const char *id, int *val);
 /* Device pointer of a resident array ("PSI0","PSIM","HPSI","PROJ","VOFR",
  * "RHO") for zero-copy access by the caller. */
