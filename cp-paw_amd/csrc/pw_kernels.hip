@@ -23,6 +23,28 @@ struct ProjGeo {
 
 __device__ inline double2 czero() { return make_double2(0.0, 0.0); }
 
+// XCD-aware tile order.  Workgroups are dealt to the eight XCDs in turn
+// (linear id mod 8), and each XCD has its own L2.  The tiles that read the
+// same operand block (one G range) get linear ids of one residue mod 8 that
+// follow each other in that XCD's order, so the block is fetched from HBM once
+// instead of once per tile.  Grid: xcd_blocks(ninner, nshared) in x.
+constexpr int kXcd = 8;
+struct XcdTile {
+  int inner, shared;
+  bool valid;
+};
+__device__ inline XcdTile xcd_tile(int ninner, int nshared) {
+  const int L = blockIdx.x, x = L % kXcd, slot = L / kXcd;
+  XcdTile t;
+  t.inner = slot % ninner;
+  t.shared = (slot / ninner) * kXcd + x;
+  t.valid = t.shared < nshared;
+  return t;
+}
+inline unsigned xcd_blocks(int ninner, int nshared) {
+  return (unsigned)((nshared + kXcd - 1) / kXcd) * kXcd * ninner;
+}
+
 __device__ inline double2 eigr_of(const ProjGeo &q, int atom, int ig) {
   const int *iv = q.igvec + 3 * ig;
   const double2 *ph = q.phase + (size_t)atom * 3 * q.nphase;
@@ -123,13 +145,15 @@ struct GenOvl {
 // psi chunk is shared through LDS.
 template <class Gen, int NCB, int NW>
 __global__ void __launch_bounds__(64 * NW) k_rowgemm(Gen gen, int nrow, int ngl, int nbh,
-                                                     const double2 *__restrict__ psi, int gsplit,
-                                                     double2 *__restrict__ part) {
+                                                     const double2 *__restrict__ psi, int gsplit, int tr,
+                                                     int tb, int ns, double2 *__restrict__ part) {
   constexpr int WR = 16 * NW, SA = padw(WR), NB16 = 16 * NCB, SB = padw(NB16);
   __shared__ double sAr[KC][SA], sAi[KC][SA], sBr[KC][SB], sBi[KC][SB];
+  const XcdTile xt = xcd_tile(tr * tb, ns);
+  if (!xt.valid) return;
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6, li = lane & 15, lk = lane >> 4;
-  const int r0 = blockIdx.x * WR, b0 = blockIdx.y * NB16;
-  const int g0 = blockIdx.z * gsplit, g1 = min(ngl, g0 + gsplit);
+  const int r0 = (xt.inner % tr) * WR, b0 = (xt.inner / tr) * NB16;
+  const int g0 = xt.shared * gsplit, g1 = min(ngl, g0 + gsplit);
   dbl4 accr[NCB], acci[NCB];
 #pragma unroll
   for (int c = 0; c < NCB; ++c) {
@@ -161,7 +185,7 @@ __global__ void __launch_bounds__(64 * NW) k_rowgemm(Gen gen, int nrow, int ngl,
     }
     __syncthreads();
   }
-  double2 *dst = part + (size_t)blockIdx.z * nrow * nbh;
+  double2 *dst = part + (size_t)xt.shared * nrow * nbh;
 #pragma unroll
   for (int c = 0; c < NCB; ++c) {
     const int j = b0 + 16 * c + li;
@@ -296,11 +320,14 @@ template <int NGB, int NW>
 __global__ void __launch_bounds__(64 * NW) k_force_m(ProjGeo q, const int2 *iptab, int npro,
                                                      const double2 *__restrict__ psi,
                                                      const double2 *__restrict__ ded,
-                                                     const double *__restrict__ gvec, double *part) {
+                                                     const double *__restrict__ gvec, int nrt, int ngt,
+                                                     double *part) {
   constexpr int WR = 16 * NW, SA = padw(WR), NG16 = 16 * NGB, SB = padw(NG16);
   __shared__ double sAr[KC][SA], sAi[KC][SA], sBr[KC][SB], sBi[KC][SB];
+  const XcdTile xt = xcd_tile(nrt, ngt);
+  if (!xt.valid) return;
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6, li = lane & 15, lk = lane >> 4;
-  const int g0 = blockIdx.x * NG16, r0 = blockIdx.y * WR;
+  const int g0 = xt.shared * NG16, r0 = xt.inner * WR;
   const int nbh = q.nbh, ngl = q.ngl;
   dbl4 accr[NGB], acci[NGB];
 #pragma unroll
@@ -366,7 +393,7 @@ __global__ void __launch_bounds__(64 * NW) k_force_m(ProjGeo q, const int2 *ipta
   for (int r = 0; r < 4; ++r) {
     const int ip = r0 + 16 * wv + lk + 4 * r;
     if (ip < npro) {
-      double *dst = part + ((size_t)blockIdx.x * npro + ip) * 3;
+      double *dst = part + ((size_t)xt.shared * npro + ip) * 3;
       dst[0] = f[r][0];
       dst[1] = f[r][1];
       dst[2] = f[r][2];
@@ -687,14 +714,16 @@ __device__ inline void psi_run(const double2 *psi, int ngl, int nbh, int b, int 
 // cancel strongly over G, and an FP32 running total loses ~1e-6 there).
 template <class Gen, int NCB, int NW>
 __global__ void __launch_bounds__(64 * NW) k_rowgemm_bf(Gen gen, int nrow, int ngl, int nbh,
-                                                        const double2 *__restrict__ psi, int gsplit,
-                                                        double2 *__restrict__ part) {
+                                                        const double2 *__restrict__ psi, int gsplit, int tr,
+                                                        int tb, int ns, double2 *__restrict__ part) {
   constexpr int WR = 16 * NW, NB16 = 16 * NCB;
   __shared__ __bf16 sA[NPL * WR][KBP];
   __shared__ __bf16 sB[NPL * NB16][KBP];
+  const XcdTile xt = xcd_tile(tr * tb, ns);
+  if (!xt.valid) return;
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6, li = lane & 15, lk = lane >> 4;
-  const int r0 = blockIdx.x * WR, b0 = blockIdx.y * NB16;
-  const int g0 = blockIdx.z * gsplit, g1 = min(ngl, g0 + gsplit);
+  const int r0 = (xt.inner % tr) * WR, b0 = (xt.inner / tr) * NB16;
+  const int g0 = xt.shared * gsplit, g1 = min(ngl, g0 + gsplit);
   dbl4 accr[NCB], acci[NCB];
 #pragma unroll
   for (int c = 0; c < NCB; ++c) {
@@ -734,7 +763,7 @@ __global__ void __launch_bounds__(64 * NW) k_rowgemm_bf(Gen gen, int nrow, int n
     }
     __syncthreads();
   }
-  double2 *dst = part + (size_t)blockIdx.z * nrow * nbh;
+  double2 *dst = part + (size_t)xt.shared * nrow * nbh;
 #pragma unroll
   for (int c = 0; c < NCB; ++c) {
     const int j = b0 + 16 * c + li;
@@ -906,7 +935,7 @@ struct ForceBbf {
 
 template <int NRB, int NW>
 __global__ void __launch_bounds__(64 * NW) k_force_bf(ForceAbf ga, ForceBbf gb, PFac pf,
-                                                      const double *__restrict__ gvec,
+                                                      const double *__restrict__ gvec, int nrt, int ngt,
                                                       double *__restrict__ part) {
   constexpr int WG = 16 * NW, NB16 = 16 * NRB;
   __shared__ __bf16 sB[NPL * WG][KBP];
@@ -917,7 +946,9 @@ __global__ void __launch_bounds__(64 * NW) k_force_bf(ForceAbf ga, ForceBbf gb, 
   // F row, atom and l of the block's projector rows
   __shared__ int sRow[NB16], sAtom[NB16], sL[NB16];
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6, li = lane & 15, lk = lane >> 4;
-  const int g0 = blockIdx.x * WG, p0 = blockIdx.y * NB16;
+  const XcdTile xt = xcd_tile(nrt, ngt);
+  if (!xt.valid) return;
+  const int g0 = xt.shared * WG, p0 = xt.inner * NB16;
   const int npro = ga.npro, nk = ga.nbh, ngl = gb.ngl;
   if (t < NB16) {
     int row = 0, atom = 0, l = 0;
@@ -1010,7 +1041,7 @@ __global__ void __launch_bounds__(64 * NW) k_force_bf(ForceAbf ga, ForceBbf gb, 
     double s = 0.0;
 #pragma unroll
     for (int w = 0; w < NW; ++w) s += sF[w][r][d];
-    part[((size_t)blockIdx.x * npro + ip) * 3 + d] = s;
+    part[((size_t)xt.shared * npro + ip) * 3 + d] = s;
   }
 }
 
@@ -1228,8 +1259,9 @@ int run_rowgemm(pw_waveset *ws, const Gen &gen, int nrow, const double2 *psi, do
   if (!out) out = part + (size_t)ns * n;
   {
     ScopeTimer tm(ws, name);
-    dim3 grid(tr, tb, ns);
-#define PW_LAUNCH(NB) k_rowgemm<Gen, NB, NW><<<grid, 64 * NW, 0, ws->stream>>>(gen, nrow, ngl, nbh, psi, gsplit, part)
+    const unsigned grid = xcd_blocks(tr * tb, ns);
+#define PW_LAUNCH(NB) \
+  k_rowgemm<Gen, NB, NW><<<grid, 64 * NW, 0, ws->stream>>>(gen, nrow, ngl, nbh, psi, gsplit, tr, tb, ns, part)
     PW_NB_SWITCH(nblk, PW_LAUNCH)
 #undef PW_LAUNCH
     PW_HIP(hipGetLastError());
@@ -1277,8 +1309,9 @@ int run_rowgemm_bf(pw_waveset *ws, const Gen &gen, int nrow, const double2 *psi,
   if (!out) out = part + (size_t)ns * n;
   {
     ScopeTimer tm(ws, name);
-    dim3 grid(tr, tb, ns);
-#define PW_LAUNCH(NB) k_rowgemm_bf<Gen, NB, NW><<<grid, 64 * NW, 0, ws->stream>>>(gen, nrow, ngl, nbh, psi, gsplit, part)
+    const unsigned grid = xcd_blocks(tr * tb, ns);
+#define PW_LAUNCH(NB) \
+  k_rowgemm_bf<Gen, NB, NW><<<grid, 64 * NW, 0, ws->stream>>>(gen, nrow, ngl, nbh, psi, gsplit, tr, tb, ns, part)
     PW_NB_SWITCH(nblk, PW_LAUNCH)
 #undef PW_LAUNCH
     PW_HIP(hipGetLastError());
@@ -1416,11 +1449,11 @@ int k_force(pw_waveset *ws, const double2 *psi, const double2 *phase, const doub
     PW_TRY(ws_part(ws, (size_t)ngt * n * sizeof(double)));
     {
       ScopeTimer tm(ws, "force");
-      dim3 grid(ngt, (ws->npro + 16 * NRB - 1) / (16 * NRB));
+      const int nrt = (ws->npro + 16 * NRB - 1) / (16 * NRB);
       ForceAbf ga{ded, ws->npro, ws->nbh};
       ForceBbf gb{psi, q.ngl, ws->nbh};
-      k_force_bf<NRB, NW><<<grid, 64 * NW, 0, ws->stream>>>(ga, gb, pfac_of(ws), ws->plan->d_gvec,
-                                                            ws->d_part);
+      k_force_bf<NRB, NW><<<xcd_blocks(nrt, ngt), 64 * NW, 0, ws->stream>>>(ga, gb, pfac_of(ws), ws->plan->d_gvec,
+                                                                            nrt, ngt, ws->d_part);
       PW_HIP(hipGetLastError());
     }
     k_reduce_split<double><<<(n + kRedC - 1) / kRedC, kRedC * kRedS, 0, ws->stream>>>(ws->d_part, ngt, (int)n,
@@ -1432,9 +1465,9 @@ int k_force(pw_waveset *ws, const double2 *psi, const double2 *phase, const doub
   PW_TRY(ws_part(ws, (size_t)ngt * n * sizeof(double)));
   {
     ScopeTimer tm(ws, "force");
-    dim3 grid(ngt, (ws->npro + 16 * NW - 1) / (16 * NW));
-    k_force_m<NGB, NW><<<grid, 64 * NW, 0, ws->stream>>>(q, ws->d_ipro_tab, ws->npro, psi, ded,
-                                                         ws->plan->d_gvec, ws->d_part);
+    const int nrt = (ws->npro + 16 * NW - 1) / (16 * NW);
+    k_force_m<NGB, NW><<<xcd_blocks(nrt, ngt), 64 * NW, 0, ws->stream>>>(q, ws->d_ipro_tab, ws->npro, psi, ded,
+                                                                         ws->plan->d_gvec, nrt, ngt, ws->d_part);
     PW_HIP(hipGetLastError());
   }
   k_reduce_split<double><<<(n + kRedC - 1) / kRedC, kRedC * kRedS, 0, ws->stream>>>(ws->d_part, ngt, (int)n,
