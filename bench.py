@@ -182,7 +182,7 @@ def run_gpu(system_name, prec, steps, warmup, profile):
     if profile:
         names = ["projections", "addpro", "force", "overlap", "overlap1c", "addopsi", "addoproj",
                  "vpsi", "density", "propagate", "opsi_scale", "ortho_x", "ekin", "wavekinetic",
-                 "expect", "denmat", "atom_mat", "phase", "reduce", "vrho"]
+                 "expect", "denmat", "atom_mat", "phase", "reduce", "vrho", "split", "pgen"]
         kt = {}
         for n in names:
             ms, cnt = w.kernel_time(n)

@@ -226,6 +226,17 @@ struct pw_waveset {
   double2 *d_phase = nullptr;  // nat*3*(2*nmaxx+1)
   int nphase = 0;              // 2*nmaxx+1
   double *d_part = nullptr; size_t part_bytes = 0;   // split-K partials
+  void *d_split = nullptr; size_t split_bytes = 0;   // mixed form: operands split into bf16 runs
+  // mixed form: bf16 runs of wave-function arrays (rows, or rows at -G) kept
+  // for the products of one step section while runs_scope is set (the arrays
+  // do not change inside such a section)
+  struct WaveRuns {
+    const void *psi = nullptr;
+    bool minus = false, valid = false;
+  };
+  void *d_wruns = nullptr; size_t wruns_bytes = 0;
+  WaveRuns wruns[4];
+  bool runs_scope = false;
   double *d_small = nullptr;   // small matrices / scalars scratch
   double *d_small_mats = nullptr;  // MAT, OMAT, OOMAT, LAMBDA, GS work (NB*NB each)
   size_t small_bytes = 0;
@@ -254,6 +265,7 @@ void ws_flush_timers(pw_waveset *ws);
 // kernels in pw_kernels.hip
 int k_phase_tables(pw_waveset *ws, const double *r_dev_host, double2 *phase);
 int k_pfac(pw_waveset *ws);
+int k_runs_scope(pw_waveset *ws, bool on);
 int k_projections(pw_waveset *ws, const double2 *psi, const double2 *phase, double2 *proj);
 int k_addpro(pw_waveset *ws, double2 *psi, const double2 *phase, const double2 *proj);
 // reductions: partial sums into device memory dst (read back once per section)

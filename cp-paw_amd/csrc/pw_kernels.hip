@@ -679,43 +679,123 @@ struct RowPf {
   }
 };
 
-// stacked [conj(psi1_i(G)); psi1_i(-G)] rows of the band overlaps
-struct RowOvl {
-  const double2 *psi1;
-  const int *minusg;
-  int ngl, nbh, nbhp;
-  __device__ void operator()(int r, int g0, int, Planes8 &o) const {
-    const bool tm = r >= nbhp;
-    const int b = tm ? r - nbhp : r;
-    if (b >= nbh) { zero8(o); return; }
-    const double2 *src = psi1 + (size_t)b * ngl;
+// ---- operands split once into bf16 runs --------------------------------------
+// SP[row][run] holds the six planes (bf16x8 each, 96 B) of the complex values
+// (row, 8 run .. 8 run + 7).  nrun covers K rounded up to whole 32-chunks and
+// the tail is zero.  An operand that several tiles of a product read (the
+// waves of the row GEMMs, the coefficients of the column GEMMs) is split once
+// by k_split_runs; the product then stages a run with six 16-byte copies.
+// Layouts: row-major SP[row][run] (a thread's run is 8 consecutive K, the
+// threads of a wave take consecutive runs) or run-major SP[run][row] (the
+// threads of a wave take consecutive rows: column operands, whose K runs
+// gather across bands).
+template <class Src, bool RUN_MAJOR>
+__global__ void k_split_runs(Src src, int nrow, int nrun, u32x4 *sp) {
+  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (size_t)nrow * nrun) return;
+  const int r = RUN_MAJOR ? (int)(idx % nrow) : (int)(idx / nrun);
+  const int run = RUN_MAJOR ? (int)(idx / nrow) : (int)(idx % nrun);
+  Planes8 o;
 #pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      const int ig = g0 + e;
-      double2 v = czero();
-      if (ig < ngl) v = tm ? src[minusg[ig]] : cconj(src[ig]);
-      put(o, e, v);
+  for (int e = 0; e < 8; ++e) put(o, e, src(r, 8 * run + e));
+  u32x4 *d = sp + idx * NPL;
+#pragma unroll
+  for (int q = 0; q < NPL; ++q) d[q] = __builtin_bit_cast(u32x4, o.p[q]);
+}
+
+struct SplitRuns {
+  const u32x4 *sp;
+  int nrow, nrun;
+  int srow, srun;   // strides in runs: row-major (nrun, 1), run-major (1, nrow)
+  __device__ void load(int r, int k0, Planes8 &o) const {
+    const int run = k0 >> 3;
+    if (r >= nrow || run >= nrun) {
+      zero8(o);
+      return;
+    }
+    const u32x4 *src = sp + ((size_t)r * srow + (size_t)run * srun) * NPL;
+#pragma unroll
+    for (int q = 0; q < NPL; ++q) o.p[q] = __builtin_bit_cast(bf16x8, src[q]);
+  }
+};
+// rows [conj(psi1_i(G)); psi1_i(-G)] of the band overlaps, i < nbhp each, from
+// the runs of psi1 (conjugated by flipping the signs of its imaginary planes)
+// and of psi1(-G)
+struct OvlRuns {
+  SplitRuns rows, minus;
+  int nbhp;
+  __device__ void operator()(int r, int g0, int, Planes8 &o) const {
+    if (r < nbhp) {
+      rows.load(r, g0, o);
+#pragma unroll
+      for (int q = 3; q < NPL; ++q) o.p[q] = bf_neg(o.p[q]);
+    } else {
+      minus.load(r - nbhp, g0, o);
     }
   }
 };
+// band operand of k_rowgemm_bf: band b, G run at g0
+struct BandRuns {
+  SplitRuns s;
+  __device__ void operator()(int b, int g0, Planes8 &o) const { s.load(b, g0, o); }
+};
+// A operand of k_colgemm_bf and k_force_bf: row b, K run at k0
+struct ARuns {
+  SplitRuns s;
+  __device__ void operator()(int b, int k0, Planes8 &o) const { s.load(b, k0, o); }
+};
+// B operand of k_colgemm_bf and k_force_bf: K run at k0 of column G
+struct ColRuns {
+  SplitRuns s;
+  __device__ void operator()(int k0, int ig, Planes8 &o) const { s.load(ig, k0, o); }
+};
 
-// psi[b][G] as the second operand (8 consecutive G of band b)
-__device__ inline void psi_run(const double2 *psi, int ngl, int nbh, int b, int g0, Planes8 &o) {
-  if (b >= nbh) { zero8(o); return; }
-  const double2 *src = psi + (size_t)b * ngl;
-#pragma unroll
-  for (int e = 0; e < 8; ++e) put(o, e, g0 + e < ngl ? src[g0 + e] : czero());
-}
+// sources of k_split_runs, (row, k) -> complex value, zero outside
+struct SrcPsiRows {          // psi[b][G], rows b
+  const double2 *psi;
+  int ngl, nbh;
+  __device__ double2 operator()(int b, int g) const {
+    return (b < nbh && g < ngl) ? psi[(size_t)b * ngl + g] : czero();
+  }
+};
+struct SrcPsiCols {          // psi[b][G], rows G, K = b
+  const double2 *psi;
+  int ngl, nbh;
+  __device__ double2 operator()(int g, int b) const {
+    return (b < nbh && g < ngl) ? psi[(size_t)b * ngl + g] : czero();
+  }
+};
+struct SrcPsiMinusRows {     // psi[b][-G], rows b
+  const double2 *psi;
+  const int *minusg;
+  int ngl, nbh;
+  __device__ double2 operator()(int b, int g) const {
+    return (b < nbh && g < ngl) ? psi[(size_t)b * ngl + minusg[g]] : czero();
+  }
+};
+struct SrcXRows {            // x[k][b] (projector-major), rows b, K = k
+  const double2 *x;
+  int nbh, nk;
+  __device__ double2 operator()(int b, int k) const {
+    return (b < nbh && k < nk) ? x[(size_t)k * nbh + b] : czero();
+  }
+};
+struct SrcDedRows {          // conj(ded[ip][b]), rows ip, K = b
+  const double2 *ded;
+  int npro, nbh;
+  __device__ double2 operator()(int ip, int b) const {
+    return (ip < npro && b < nbh) ? cconj(ded[(size_t)ip * nbh + b]) : czero();
+  }
+};
 
 // part[split][r][b] = sum_{G in split} A(r,G) psi[b][G] (split-BF16x3).
 // Each 32-G chunk is summed by the MFMAs from zero in FP32 and then added to
 // FP64 accumulators: the FP32 roundings are those of one chunk's sum, never of
 // the running total (the products of localised projectors and extended waves
 // cancel strongly over G, and an FP32 running total loses ~1e-6 there).
-template <class Gen, int NCB, int NW>
-__global__ void __launch_bounds__(64 * NW) k_rowgemm_bf(Gen gen, int nrow, int ngl, int nbh,
-                                                        const double2 *__restrict__ psi, int gsplit, int tr,
-                                                        int tb, int ns, double2 *__restrict__ part) {
+template <class Gen, class GB, int NCB, int NW>
+__global__ void __launch_bounds__(64 * NW) k_rowgemm_bf(Gen gen, GB gb, int nrow, int ngl, int nbh, int gsplit,
+                                                        int tr, int tb, int ns, double2 *__restrict__ part) {
   constexpr int WR = 16 * NW, NB16 = 16 * NCB;
   __shared__ __bf16 sA[NPL * WR][KBP];
   __shared__ __bf16 sB[NPL * NB16][KBP];
@@ -741,7 +821,7 @@ __global__ void __launch_bounds__(64 * NW) k_rowgemm_bf(Gen gen, int nrow, int n
     for (int e = t; e < NB16 * 4; e += 64 * NW) {
       const int b = e >> 2, k8 = e & 3;
       Planes8 v;
-      psi_run(psi, ngl, nbh, b0 + b, gc + 8 * k8, v);
+      gb(b0 + b, gc + 8 * k8, v);
       store_planes(sB, NB16, b, k8, v);
     }
     __syncthreads();
@@ -776,33 +856,7 @@ __global__ void __launch_bounds__(64 * NW) k_rowgemm_bf(Gen gen, int nrow, int n
 }
 
 // ---- column GEMM: psi[b][G] += sum_k A(b,k) B(k,G) (split-BF16x3) -----------
-// A rows: 8 consecutive k of band b
-struct AddproAbf {
-  const double2 *x;
-  int nbh, nk;
-  __device__ void operator()(int b, int k0, Planes8 &o) const {
-#pragma unroll
-    for (int e = 0; e < 8; ++e) put(o, e, (k0 + e < nk && b < nbh) ? x[(size_t)(k0 + e) * nbh + b] : czero());
-  }
-};
-struct AddopsiAbf {
-  const double *lam;
-  int nb, nbh;
-  __device__ void operator()(int b, int k0, Planes8 &o) const {
-#pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      const int k = k0 + e;
-      double2 v = czero();
-      if (k < 2 * nbh && b < nbh) {
-        double2 l1, l2;
-        fold_lambda(lam, nb, k < nbh ? k : k - nbh, b, l1, l2);
-        v = k < nbh ? l1 : l2;
-      }
-      put(o, e, v);
-    }
-  }
-};
-// B columns: 8 consecutive k at one G
+// B operand of ADDPRO: 8 consecutive projectors k at one G, from the factors
 struct AddproBf {
   PFac pf;
   int nk;
@@ -827,19 +881,23 @@ struct AddproBf {
     for (int e = 0; e < 8; ++e) put(o, e, k0 + e < nk ? pf.at(k0 + e, ig) : czero());
   }
 };
-struct AddopsiBbf {
+struct SrcLamRows {          // LAMBDA1/LAMBDA2 stacked along K = 2 NBH, rows b
+  const double *lam;
+  int nb, nbh;
+  __device__ double2 operator()(int b, int k) const {
+    if (k >= 2 * nbh || b >= nbh) return czero();
+    double2 l1, l2;
+    fold_lambda(lam, nb, k < nbh ? k : k - nbh, b, l1, l2);
+    return k < nbh ? l1 : l2;
+  }
+};
+struct SrcChiCols {          // [chi_k(G); conj(chi_k(-G))] along K = 2 NBH, rows G
   const double2 *chi;
   const int *minusg;
   int ngl, nbh;
-  __device__ void operator()(int k0, int ig, Planes8 &o) const {
-#pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      const int k = k0 + e;
-      double2 v = czero();
-      if (ig < ngl && k < 2 * nbh)
-        v = k < nbh ? chi[(size_t)k * ngl + ig] : cconj(chi[(size_t)(k - nbh) * ngl + minusg[ig]]);
-      put(o, e, v);
-    }
+  __device__ double2 operator()(int g, int k) const {
+    if (g >= ngl || k >= 2 * nbh) return czero();
+    return k < nbh ? chi[(size_t)k * ngl + g] : cconj(chi[(size_t)(k - nbh) * ngl + minusg[g]]);
   }
 };
 
@@ -906,35 +964,12 @@ __global__ void __launch_bounds__(64 * NW) k_colgemm_bf(GA ga, GB gb, int nk, in
 
 // ---- forces in the mixed form ------------------------------------------------
 // D(ipro,G) = sum_b conj(ded[ipro][b]) psi[b][G] as a split-BF16x3 column
-// product (rows ipro, columns G, FP32 accumulation).  The epilogue forms
-// P(ipro,G) in FP64 from the projector factors and sums Re(i P conj(D)) G over the block's G range in FP64:
-// part[gtile][ipro][3] (WAVES_DEDPRO + WAVES_PROFORCE, src/paw_waves1.f90:
-// 5694-5745, 6112-6234).
-struct ForceAbf {
-  const double2 *ded;
-  int npro, nbh;
-  __device__ void operator()(int ip, int k0, Planes8 &o) const {
-#pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      const int b = k0 + e;
-      put(o, e, (ip < npro && b < nbh) ? cconj(ded[(size_t)ip * nbh + b]) : czero());
-    }
-  }
-};
-struct ForceBbf {
-  const double2 *psi;
-  int ngl, nbh;
-  __device__ void operator()(int k0, int ig, Planes8 &o) const {
-#pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      const int b = k0 + e;
-      put(o, e, (ig < ngl && b < nbh) ? psi[(size_t)b * ngl + ig] : czero());
-    }
-  }
-};
-
-template <int NRB, int NW>
-__global__ void __launch_bounds__(64 * NW) k_force_bf(ForceAbf ga, ForceBbf gb, PFac pf,
+// product (rows ipro, columns G, FP32 chunk sums).  The epilogue forms
+// P(ipro,G) in FP64 from the projector factors and sums Re(i P conj(D)) G
+// over the block's G range in FP64: part[gtile][ipro][3] (WAVES_DEDPRO +
+// WAVES_PROFORCE, src/paw_waves1.f90:5694-5745, 6112-6234).
+template <class GA, class GB, int NRB, int NW>
+__global__ void __launch_bounds__(64 * NW) k_force_bf(GA ga, GB gb, int npro, int nk, int ngl, PFac pf,
                                                       const double *__restrict__ gvec, int nrt, int ngt,
                                                       double *__restrict__ part) {
   constexpr int WG = 16 * NW, NB16 = 16 * NRB;
@@ -949,7 +984,6 @@ __global__ void __launch_bounds__(64 * NW) k_force_bf(ForceAbf ga, ForceBbf gb, 
   const XcdTile xt = xcd_tile(nrt, ngt);
   if (!xt.valid) return;
   const int g0 = xt.shared * WG, p0 = xt.inner * NB16;
-  const int npro = ga.npro, nk = ga.nbh, ngl = gb.ngl;
   if (t < NB16) {
     int row = 0, atom = 0, l = 0;
     if (p0 + t < npro) {
@@ -1236,8 +1270,68 @@ void band_tiles(int nbh, int *nblk, int *ntiles) {
 }  // namespace
 
 int ws_part(pw_waveset *ws, size_t bytes);
+int ws_split(pw_waveset *ws, size_t bytes);
 
 namespace {
+
+// bytes of the split runs of an nrow x K operand
+size_t runs_bytes(int nrow, int K) { return (size_t)nrow * ((K + KB - 1) / KB * (KB / 8)) * NPL * sizeof(u32x4); }
+
+// split an nrow x K operand into runs at dst (nrow x K runs, runs_bytes)
+template <bool RUN_MAJOR, class Src>
+SplitRuns split_runs_to(pw_waveset *ws, const Src &src, int nrow, int K, void *dst) {
+  const int nrun = (K + KB - 1) / KB * (KB / 8);
+  const size_t n = (size_t)nrow * nrun;
+  u32x4 *sp = static_cast<u32x4 *>(dst);
+  k_split_runs<Src, RUN_MAJOR><<<(unsigned)((n + 255) / 256), 256, 0, ws->stream>>>(src, nrow, nrun, sp);
+  return RUN_MAJOR ? SplitRuns{sp, nrow, nrun, 1, nrow} : SplitRuns{sp, nrow, nrun, nrun, 1};
+}
+// ... at byte offset off of the scratch split buffer (sized with ws_split)
+template <bool RUN_MAJOR, class Src>
+SplitRuns split_runs(pw_waveset *ws, const Src &src, int nrow, int K, size_t off) {
+  return split_runs_to<RUN_MAJOR>(ws, src, nrow, K, static_cast<char *>(ws->d_split) + off);
+}
+
+// runs of the rows of a wave-function array (minus: of psi(-G)).  Inside a
+// runs scope they are taken from, or split into, one of four kept slots;
+// otherwise they are split into the scratch buffer at byte offset off.
+int wave_runs(pw_waveset *ws, const double2 *psi, bool minus, size_t off, SplitRuns *out) {
+  pw_plan *p = ws->plan;
+  const int nbh = ws->nbh, ngl = p->ngl;
+  const size_t bytes = runs_bytes(nbh, ngl);
+  auto split_to = [&](void *dst) {
+    if (minus) *out = split_runs_to<false>(ws, SrcPsiMinusRows{psi, p->d_minusg, ngl, nbh}, nbh, ngl, dst);
+    else *out = split_runs_to<false>(ws, SrcPsiRows{psi, ngl, nbh}, nbh, ngl, dst);
+  };
+  if (ws->runs_scope) {
+    int free_slot = -1;
+    for (int i = 0; i < 4; ++i) {
+      const pw_waveset::WaveRuns &w = ws->wruns[i];
+      if (w.valid && w.psi == psi && w.minus == minus) {
+        const int nrun = (ngl + KB - 1) / KB * (KB / 8);
+        *out = SplitRuns{reinterpret_cast<const u32x4 *>(static_cast<char *>(ws->d_wruns) + i * bytes), nbh, nrun,
+                         nrun, 1};
+        return PW_OK;
+      }
+      if (!w.valid && free_slot < 0) free_slot = i;
+    }
+    if (free_slot >= 0) {
+      if (4 * bytes > ws->wruns_bytes) {   // only at the scope's first split (all slots free)
+        if (ws->d_wruns) PW_HIP(hipFree(ws->d_wruns));
+        PW_HIP(hipMalloc(&ws->d_wruns, 4 * bytes));
+        ws->wruns_bytes = 4 * bytes;
+      }
+      ScopeTimer tm(ws, "split");
+      split_to(static_cast<char *>(ws->d_wruns) + free_slot * bytes);
+      ws->wruns[free_slot] = pw_waveset::WaveRuns{psi, minus, true};
+      return PW_OK;
+    }
+  }
+  PW_TRY(ws_split(ws, off + bytes));
+  ScopeTimer tm(ws, "split");
+  split_to(static_cast<char *>(ws->d_split) + off);
+  return PW_OK;
+}
 
 // out[r][b] = scale sum_G A(r,G) psi[b][G]: split over G (about two waves per
 // SIMD), deterministic reduction of the splits.  out == nullptr puts the
@@ -1293,7 +1387,7 @@ int run_colgemm(pw_waveset *ws, const GA &ga, const GB &gb, int nk, double2 *psi
 
 // split-BF16x3 counterpart of run_rowgemm (G splits are whole 32-chunks)
 template <int NW, class Gen>
-int run_rowgemm_bf(pw_waveset *ws, const Gen &gen, int nrow, const double2 *psi, double scale,
+int run_rowgemm_bf(pw_waveset *ws, const Gen &gen, int nrow, const BandRuns &gb, double scale,
                    double2 *out, double2 **res, const char *name) {
   const int ngl = ws->plan->ngl, nbh = ws->nbh;
   int nblk, tb;
@@ -1311,7 +1405,8 @@ int run_rowgemm_bf(pw_waveset *ws, const Gen &gen, int nrow, const double2 *psi,
     ScopeTimer tm(ws, name);
     const unsigned grid = xcd_blocks(tr * tb, ns);
 #define PW_LAUNCH(NB) \
-  k_rowgemm_bf<Gen, NB, NW><<<grid, 64 * NW, 0, ws->stream>>>(gen, nrow, ngl, nbh, psi, gsplit, tr, tb, ns, part)
+  k_rowgemm_bf<Gen, BandRuns, NB, NW><<<grid, 64 * NW, 0, ws->stream>>>(gen, gb, nrow, ngl, nbh, gsplit, tr, tb, \
+                                                                         ns, part)
     PW_NB_SWITCH(nblk, PW_LAUNCH)
 #undef PW_LAUNCH
     PW_HIP(hipGetLastError());
@@ -1352,6 +1447,21 @@ int ws_part(pw_waveset *ws, size_t bytes) {
     if (ws->d_part) PW_HIP(hipFree(ws->d_part));
     PW_HIP(hipMalloc(&ws->d_part, bytes));
     ws->part_bytes = bytes;
+  }
+  return PW_OK;
+}
+
+int k_runs_scope(pw_waveset *ws, bool on) {
+  ws->runs_scope = on;
+  for (auto &w : ws->wruns) w.valid = false;
+  return PW_OK;
+}
+
+int ws_split(pw_waveset *ws, size_t bytes) {
+  if (bytes > ws->split_bytes) {
+    if (ws->d_split) PW_HIP(hipFree(ws->d_split));
+    PW_HIP(hipMalloc(&ws->d_split, bytes));
+    ws->split_bytes = bytes;
   }
   return PW_OK;
 }
@@ -1415,8 +1525,10 @@ PFac pfac_of(const pw_waveset *ws) {
 
 int k_projections(pw_waveset *ws, const double2 *psi, const double2 *phase, double2 *proj) {
   if (ws->prec == PW_PREC_MIXED) {
+    BandRuns gb;
+    PW_TRY(wave_runs(ws, psi, false, 0, &gb.s));
     RowPf gen{pfac_of(ws)};
-    return run_rowgemm_bf<4>(ws, gen, ws->npro, psi, ws->plan->vol, proj, nullptr, "projections");
+    return run_rowgemm_bf<4>(ws, gen, ws->npro, gb, ws->plan->vol, proj, nullptr, "projections");
   }
   ProjGeo q = make_pgeo(ws);
   q.phase = phase;
@@ -1426,7 +1538,12 @@ int k_projections(pw_waveset *ws, const double2 *psi, const double2 *phase, doub
 
 int k_addpro(pw_waveset *ws, double2 *psi, const double2 *phase, const double2 *x) {
   if (ws->prec == PW_PREC_MIXED) {
-    AddproAbf ga{x, ws->nbh, ws->npro};
+    PW_TRY(ws_split(ws, runs_bytes(ws->nbh, ws->npro)));
+    ARuns ga;
+    {
+      ScopeTimer tm(ws, "split");
+      ga = ARuns{split_runs<false>(ws, SrcXRows{x, ws->nbh, ws->npro}, ws->nbh, ws->npro, 0)};
+    }
     AddproBf gb{pfac_of(ws), ws->npro};
     return run_colgemm_bf<2>(ws, ga, gb, ws->npro, psi, "addpro");
   }
@@ -1447,13 +1564,20 @@ int k_force(pw_waveset *ws, const double2 *psi, const double2 *phase, const doub
     constexpr int NRB = 4;
     const int ngt = (q.ngl + 16 * NW - 1) / (16 * NW);
     PW_TRY(ws_part(ws, (size_t)ngt * n * sizeof(double)));
+    const size_t ba = runs_bytes(ws->npro, ws->nbh), bb = runs_bytes(q.ngl, ws->nbh);
+    PW_TRY(ws_split(ws, ba + bb));
+    ARuns ga;
+    ColRuns gb;
+    {
+      ScopeTimer tm(ws, "split");
+      ga = ARuns{split_runs<false>(ws, SrcDedRows{ded, ws->npro, ws->nbh}, ws->npro, ws->nbh, 0)};
+      gb = ColRuns{split_runs<true>(ws, SrcPsiCols{psi, q.ngl, ws->nbh}, q.ngl, ws->nbh, ba)};
+    }
     {
       ScopeTimer tm(ws, "force");
       const int nrt = (ws->npro + 16 * NRB - 1) / (16 * NRB);
-      ForceAbf ga{ded, ws->npro, ws->nbh};
-      ForceBbf gb{psi, q.ngl, ws->nbh};
-      k_force_bf<NRB, NW><<<xcd_blocks(nrt, ngt), 64 * NW, 0, ws->stream>>>(ga, gb, pfac_of(ws), ws->plan->d_gvec,
-                                                                            nrt, ngt, ws->d_part);
+      k_force_bf<ARuns, ColRuns, NRB, NW><<<xcd_blocks(nrt, ngt), 64 * NW, 0, ws->stream>>>(
+          ga, gb, ws->npro, ws->nbh, q.ngl, pfac_of(ws), ws->plan->d_gvec, nrt, ngt, ws->d_part);
       PW_HIP(hipGetLastError());
     }
     k_reduce_split<double><<<(n + kRedC - 1) / kRedC, kRedC * kRedS, 0, ws->stream>>>(ws->d_part, ngt, (int)n,
@@ -1484,8 +1608,16 @@ int k_overlap(pw_waveset *ws, const double2 *psi1, const double2 *psi2, double *
   const int nbh = ws->nbh, nbhp = (nbh + 15) / 16 * 16;
   double2 *red = nullptr;
   if (ws->prec == PW_PREC_MIXED) {
-    RowOvl gen{psi1, p->d_minusg, p->ngl, nbh, nbhp};
-    PW_TRY(run_rowgemm_bf<2>(ws, gen, 2 * nbhp, psi2, p->vol, nullptr, &red, "overlap"));
+    const size_t bytes = runs_bytes(nbh, p->ngl);
+    PW_TRY(ws_split(ws, 3 * bytes));   // outside a runs scope all three go to the scratch
+    OvlRuns gen;
+    gen.nbhp = nbhp;
+    BandRuns gb;
+    PW_TRY(wave_runs(ws, psi1, false, 0, &gen.rows));
+    PW_TRY(wave_runs(ws, psi1, true, bytes, &gen.minus));
+    if (psi2 == psi1) gb.s = gen.rows;
+    else PW_TRY(wave_runs(ws, psi2, false, 2 * bytes, &gb.s));
+    PW_TRY(run_rowgemm_bf<2>(ws, gen, 2 * nbhp, gb, p->vol, nullptr, &red, "overlap"));
   } else {
     GenOvl gen{psi1, p->d_minusg, p->ngl, nbh, nbhp};
     PW_TRY(run_rowgemm<2>(ws, gen, 2 * nbhp, psi2, p->vol, nullptr, &red, "overlap"));
@@ -1523,9 +1655,17 @@ int k_atom_matrix(pw_waveset *ws, const double2 *proj, double2 *out, const doubl
 int k_addopsi(pw_waveset *ws, double2 *psibar, const double2 *chi, const double *lam) {
   pw_plan *p = ws->plan;
   if (ws->prec == PW_PREC_MIXED) {
-    AddopsiAbf ga{lam, ws->nb, ws->nbh};
-    AddopsiBbf gb{chi, p->d_minusg, p->ngl, ws->nbh};
-    return run_colgemm_bf<2>(ws, ga, gb, 2 * ws->nbh, psibar, "addopsi");
+    const int nk = 2 * ws->nbh;
+    const size_t ba = runs_bytes(ws->nbh, nk), bb = runs_bytes(p->ngl, nk);
+    PW_TRY(ws_split(ws, ba + bb));
+    ARuns ga;
+    ColRuns gb;
+    {
+      ScopeTimer tm(ws, "split");
+      ga = ARuns{split_runs<false>(ws, SrcLamRows{lam, ws->nb, ws->nbh}, ws->nbh, nk, 0)};
+      gb = ColRuns{split_runs<true>(ws, SrcChiCols{chi, p->d_minusg, p->ngl, ws->nbh}, p->ngl, nk, ba)};
+    }
+    return run_colgemm_bf<2>(ws, ga, gb, nk, psibar, "addopsi");
   }
   AddopsiA ga{lam, ws->nb, ws->nbh};
   AddopsiB gb{chi, p->d_minusg, p->ngl, ws->nbh};

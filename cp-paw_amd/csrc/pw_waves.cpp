@@ -189,8 +189,10 @@ static int gramschmidt_one(pw_waveset *ws, double2 *psi, const std::vector<doubl
   const int nb = ws->nb;
   pw_plan *p = ws->plan;
   PW_TRY(phase_at(ws, r));
+  PW_TRY(k_runs_scope(ws, true));
   PW_TRY(k_projections(ws, psi, ws->d_phase, ws->d_oproj));
   PW_TRY(k_overlap(ws, psi, psi, d_mat(ws, 4), false, true));
+  PW_TRY(k_runs_scope(ws, false));
   PW_TRY(k_overlap_1c(ws, ws->d_oproj, ws->d_oproj, d_mat(ws, 4)));
   std::vector<double> s((size_t)nb * nb);
   PW_HIP(hipMemcpyAsync(s.data(), d_mat(ws, 4), s.size() * sizeof(double), hipMemcpyDeviceToHost, ws->stream));
@@ -375,7 +377,8 @@ extern "C" int pw_waves_destroy(pw_waveset *ws) {
                   ws->d_xproj, ws->d_pro, ws->d_ylm, ws->d_mpsi, ws->d_v, ws->d_rho, ws->d_occ2,
                   ws->d_fpm, ws->d_fab, ws->d_domat, ws->d_dhmat, ws->d_atom_isp, ws->d_ipro0,
                   ws->d_ibpro, ws->d_lmnx, ws->d_lmn_ln, ws->d_lmn_lm, ws->d_lmn_l, ws->d_phase,
-                  ws->d_part, ws->d_small, ws->d_ortho, ws->d_small_mats, ws->d_ipro_tab, ws->d_pfac};
+                  ws->d_part, ws->d_small, ws->d_ortho, ws->d_small_mats, ws->d_ipro_tab, ws->d_pfac,
+                  ws->d_split, ws->d_wruns};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (ws->d_res) (void)hipFree(ws->d_res);
@@ -752,8 +755,9 @@ extern "C" int pw_waves_orthogonalize(pw_waveset *ws) {
   PW_TRY(k_atom_matrix(ws, ws->d_proj, ws->d_xproj, ws->d_domat));
   PW_TRY(k_addpro(ws, ws->d_opsi, ws->d_phase, ws->d_xproj));
   PW_TRY(k_opsi_scale(ws, ws->d_opsi));
-  // projections at R(+)
+  // projections at R(+); psibar and chi stay fixed until the overlaps are done
   PW_TRY(phase_at(ws, rp));
+  PW_TRY(k_runs_scope(ws, true));
   PW_TRY(k_projections(ws, ws->d_psim, ws->d_phase, ws->d_proj));
   PW_TRY(k_projections(ws, ws->d_opsi, ws->d_phase, ws->d_oproj));
   // MAT, OMAT, OOMAT
@@ -762,6 +766,7 @@ extern "C" int pw_waves_orthogonalize(pw_waveset *ws) {
   PW_TRY(k_overlap(ws, ws->d_opsi, ws->d_psim, d_mat(ws, 1), false, false));
   PW_TRY(k_overlap_1c(ws, ws->d_oproj, ws->d_proj, d_mat(ws, 1)));
   PW_TRY(k_overlap(ws, ws->d_opsi, ws->d_opsi, d_mat(ws, 2), false, true));
+  PW_TRY(k_runs_scope(ws, false));
   PW_TRY(k_overlap_1c(ws, ws->d_oproj, ws->d_oproj, d_mat(ws, 2)));
   // initial LAMBDA: hermitised RLAM0 times 0.5 f_i/(f_i+f_j) (src/paw_waves2.f90:295-359)
   std::vector<double> lam((size_t)nb * nb);
