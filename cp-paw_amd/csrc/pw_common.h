@@ -277,6 +277,7 @@ int k_addoproj(pw_waveset *ws, double2 *proj, const double2 *oproj, const double
 int k_atom_matrix(pw_waveset *ws, const double2 *proj, double2 *out, const double2 *mats);
 int k_fold_occ(pw_waveset *ws, double2 *x);
 int red_blocks(const pw_waveset *ws);
+int expect_chunks(const pw_waveset *ws);
 int vrho_blocks(const pw_waveset *ws);
 int k_ekin(pw_waveset *ws, const double2 *psi, const double *fpm_dev, double *dst);
 int k_wavekinetic(pw_waveset *ws, const double2 *psi0, const double2 *psim, const double *fab_dev, double *dst);

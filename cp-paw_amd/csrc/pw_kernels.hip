@@ -1080,17 +1080,27 @@ __global__ void __launch_bounds__(64 * NW) k_force_bf(GA ga, GB gb, int npro, in
 }
 
 // ADDOPROJ: proj[k][j] += sum_i oproj[k][i] L1[i][j] + conj(oproj[k][i]) L2[i][j]
-__global__ void k_addoproj(int npro, int nbh, int nb, double2 *proj, const double2 *oproj,
-                           const double *lam) {
+// (L1, L2) pairs of the whole NBH x NBH fold, l12[2 (i nbh + j) + 0/1]
+__global__ void k_fold_lam(int nb, int nbh, const double *lam, double2 *l12) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= nbh * nbh) return;
+  double2 l1, l2;
+  fold_lambda(lam, nb, idx / nbh, idx % nbh, l1, l2);
+  l12[2 * idx] = l1;
+  l12[2 * idx + 1] = l2;
+}
+
+__global__ void k_addoproj(int npro, int nbh, double2 *proj, const double2 *oproj, const double2 *l12) {
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= npro * nbh) return;
   const int k = idx / nbh, j = idx % nbh;
+  const double2 *o = oproj + (size_t)k * nbh;
   double2 s = czero();
+#pragma unroll 4
   for (int i = 0; i < nbh; ++i) {
-    double2 l1, l2;
-    fold_lambda(lam, nb, i, j, l1, l2);
-    const double2 o = oproj[(size_t)k * nbh + i];
-    s = cadd(s, cadd(cmul(o, l1), cmul(cconj(o), l2)));
+    const double2 l1 = l12[2 * ((size_t)i * nbh + j)], l2 = l12[2 * ((size_t)i * nbh + j) + 1];
+    const double2 oi = o[i];
+    s = cadd(s, cadd(cmul(oi, l1), cmul(cconj(oi), l2)));
   }
   proj[idx] = cadd(proj[idx], s);
 }
@@ -1121,15 +1131,20 @@ __device__ inline double block_sum(double v, double *red) {
   return s;
 }
 
+// The (G, band) reductions run on a grid of G blocks (x) by groups of
+// kBandGrp bands (y), one partial sum per workgroup.
+constexpr int kBandGrp = 8;
+
 // ekin partial per block: sum_G g2 * sum_b [fp |c|^2 + fm Re(c(G) c(-G))]
 __global__ void __launch_bounds__(kB) k_ekin(int ngl, int nbh, const double *g2, const int *minusg,
                                              const double *fpm, const double2 *psi, double *part) {
   __shared__ double red[kB / 64];
   double s = 0.0;
+  const int b0 = blockIdx.y * kBandGrp, b1 = min(nbh, b0 + kBandGrp);
   for (int ig = blockIdx.x * kB + threadIdx.x; ig < ngl; ig += gridDim.x * kB) {
     double d = 0.0;
     const int mg = minusg[ig];
-    for (int b = 0; b < nbh; ++b) {
+    for (int b = b0; b < b1; ++b) {
       const double fp = fpm[2 * b], fm = fpm[2 * b + 1];
       const double2 c = psi[(size_t)b * ngl + ig];
       if (fp != 0.0) d += fp * (c.x * c.x + c.y * c.y);
@@ -1141,7 +1156,7 @@ __global__ void __launch_bounds__(kB) k_ekin(int ngl, int nbh, const double *g2,
     s += g2[ig] * d;
   }
   s = block_sum<kB>(s, red);
-  if (threadIdx.x == 0) part[blockIdx.x] = s;
+  if (threadIdx.x == 0) part[blockIdx.y * gridDim.x + blockIdx.x] = s;
 }
 
 // wave kinetic partial: sum_G m(G) sum_b [fa |d|^2 + fb (dr dr(-G) - di di(-G))]
@@ -1151,10 +1166,11 @@ __global__ void __launch_bounds__(kB) k_wavekin(int ngl, int nbh, const double *
                                                 double *part) {
   __shared__ double red[kB / 64];
   double s = 0.0;
+  const int b0 = blockIdx.y * kBandGrp, b1 = min(nbh, b0 + kBandGrp);
   for (int ig = blockIdx.x * kB + threadIdx.x; ig < ngl; ig += gridDim.x * kB) {
     double d = 0.0;
     const int mg = minusg[ig];
-    for (int b = 0; b < nbh; ++b) {
+    for (int b = b0; b < b1; ++b) {
       const double fa = fab[2 * b], fb = fab[2 * b + 1];
       if (fa == 0.0 && fb == 0.0) continue;
       const double2 x = csub(p0[(size_t)b * ngl + ig], pm[(size_t)b * ngl + ig]);
@@ -1168,18 +1184,20 @@ __global__ void __launch_bounds__(kB) k_wavekin(int ngl, int nbh, const double *
     s += mpsi[ig] * d;
   }
   s = block_sum<kB>(s, red);
-  if (threadIdx.x == 0) part[blockIdx.x] = s;
+  if (threadIdx.x == 0) part[blockIdx.y * gridDim.x + blockIdx.x] = s;
 }
 
-// <psi|H psi> per super band: out[b] = (sum conj(psi) hpsi, sum psi hpsi(-G))
-__global__ void __launch_bounds__(kB) k_expect(int ngl, const int *minusg, const double2 *psi,
-                                               const double2 *hpsi, double *out) {
+// <psi|H psi> per super band, partial over the G chunk blockIdx.x:
+// out[chunk][b] = (sum conj(psi) hpsi, sum psi hpsi(-G))
+__global__ void __launch_bounds__(kB) k_expect(int ngl, int nbh, int chunk, const int *minusg,
+                                               const double2 *psi, const double2 *hpsi, double *out) {
   __shared__ double red[kB / 64];
-  const int b = blockIdx.x;
+  const int b = blockIdx.y;
   const double2 *p = psi + (size_t)b * ngl;
   const double2 *h = hpsi + (size_t)b * ngl;
   double tr = 0.0, tmr = 0.0;
-  for (int ig = threadIdx.x; ig < ngl; ig += kB) {
+  const int g1 = min(ngl, (blockIdx.x + 1) * chunk);
+  for (int ig = blockIdx.x * chunk + threadIdx.x; ig < g1; ig += kB) {
     const double2 a = p[ig], c = h[ig], cm = h[minusg[ig]];
     tr += a.x * c.x + a.y * c.y;
     tmr += a.x * cm.x - a.y * cm.y;
@@ -1187,8 +1205,9 @@ __global__ void __launch_bounds__(kB) k_expect(int ngl, const int *minusg, const
   tr = block_sum<kB>(tr, red);
   tmr = block_sum<kB>(tmr, red);
   if (threadIdx.x == 0) {
-    out[2 * b] = tr;
-    out[2 * b + 1] = tmr;
+    double *o = out + 2 * ((size_t)blockIdx.x * nbh + b);
+    o[0] = tr;
+    o[1] = tmr;
   }
 }
 
@@ -1674,8 +1693,11 @@ int k_addopsi(pw_waveset *ws, double2 *psibar, const double2 *chi, const double 
 
 int k_addoproj(pw_waveset *ws, double2 *proj, const double2 *oproj, const double *lam) {
   const int n = ws->npro * ws->nbh;
+  const int nbh = ws->nbh;
+  PW_TRY(ws_small(ws, (size_t)2 * nbh * nbh * sizeof(double2)));
   ScopeTimer tm(ws, "addoproj");
-  k_addoproj<<<(n + 255) / 256, 256, 0, ws->stream>>>(ws->npro, ws->nbh, ws->nb, proj, oproj, lam);
+  k_fold_lam<<<(nbh * nbh + 255) / 256, 256, 0, ws->stream>>>(ws->nb, nbh, lam, (double2 *)ws->d_small);
+  k_addoproj<<<(n + 255) / 256, 256, 0, ws->stream>>>(ws->npro, nbh, proj, oproj, (const double2 *)ws->d_small);
   PW_HIP(hipGetLastError());
   return PW_OK;
 }
@@ -1687,17 +1709,21 @@ int k_fold_occ(pw_waveset *ws, double2 *x) {
   return PW_OK;
 }
 
-static int reduce_blocks(pw_plan *p) { return std::min(1024, (p->ngl + kB - 1) / kB); }
+static int reduce_gblocks(pw_plan *p) { return std::min(1024, (p->ngl + kB - 1) / kB); }
+static int band_groups(int nbh) { return (nbh + kBandGrp - 1) / kBandGrp; }
 
 // Reductions of the step leave their partial sums on the device (dst); the
 // host reads them back with one copy per step section (see pw_waves.cpp).
-int red_blocks(const pw_waveset *ws) { return reduce_blocks(ws->plan); }
+int red_blocks(const pw_waveset *ws) { return reduce_gblocks(ws->plan) * band_groups(ws->nbh); }
+// G chunks of the <psi|H|psi> partials: about four G per thread
+int expect_chunks(const pw_waveset *ws) { return std::min(64, (ws->plan->ngl + 4 * kB - 1) / (4 * kB)); }
 int vrho_blocks(const pw_waveset *ws) { return std::min(1024, (ws->plan->nrl + kB - 1) / kB); }
 
 int k_ekin(pw_waveset *ws, const double2 *psi, const double *fpm_dev, double *dst) {
   pw_plan *p = ws->plan;
   ScopeTimer tm(ws, "ekin");
-  k_ekin<<<red_blocks(ws), kB, 0, ws->stream>>>(p->ngl, ws->nbh, p->d_g2, p->d_minusg, fpm_dev, psi, dst);
+  const dim3 grid(reduce_gblocks(p), band_groups(ws->nbh));
+  k_ekin<<<grid, kB, 0, ws->stream>>>(p->ngl, ws->nbh, p->d_g2, p->d_minusg, fpm_dev, psi, dst);
   PW_HIP(hipGetLastError());
   return PW_OK;
 }
@@ -1706,8 +1732,8 @@ int k_wavekinetic(pw_waveset *ws, const double2 *psi0, const double2 *psim, cons
                   double *dst) {
   pw_plan *p = ws->plan;
   ScopeTimer tm(ws, "wavekinetic");
-  k_wavekin<<<red_blocks(ws), kB, 0, ws->stream>>>(p->ngl, ws->nbh, ws->d_mpsi, p->d_minusg, fab_dev, psi0,
-                                                   psim, dst);
+  const dim3 grid(reduce_gblocks(p), band_groups(ws->nbh));
+  k_wavekin<<<grid, kB, 0, ws->stream>>>(p->ngl, ws->nbh, ws->d_mpsi, p->d_minusg, fab_dev, psi0, psim, dst);
   PW_HIP(hipGetLastError());
   return PW_OK;
 }
@@ -1715,7 +1741,8 @@ int k_wavekinetic(pw_waveset *ws, const double2 *psi0, const double2 *psim, cons
 int k_expect(pw_waveset *ws, const double2 *psi, const double2 *hpsi, double *dst) {
   pw_plan *p = ws->plan;
   ScopeTimer tm(ws, "expect");
-  k_expect<<<ws->nbh, kB, 0, ws->stream>>>(p->ngl, p->d_minusg, psi, hpsi, dst);
+  const int nc = expect_chunks(ws), chunk = (p->ngl + nc - 1) / nc;
+  k_expect<<<dim3(nc, ws->nbh), kB, 0, ws->stream>>>(p->ngl, ws->nbh, chunk, p->d_minusg, psi, hpsi, dst);
   PW_HIP(hipGetLastError());
   return PW_OK;
 }

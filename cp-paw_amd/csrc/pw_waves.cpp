@@ -330,7 +330,7 @@ extern "C" int pw_waves_create(pw_plan *plan, int nb, int nat, int nsp, const in
     ws->r_vrho = ws->r_ekin + rb;
     ws->r_wk = ws->r_vrho + vb;
     ws->r_expect = ws->r_wk + rb;
-    ws->r_denmat = ws->r_expect + 2 * (size_t)ws->nbh;
+    ws->r_denmat = ws->r_expect + 2 * (size_t)ws->nbh * expect_chunks(ws);
     ws->r_force = ws->r_denmat + (size_t)nat * ws->lmnxx * ws->lmnxx;
     ws->res_n = ws->r_force + 3 * (size_t)ws->npro;
     rc = dalloc((void **)&ws->d_res, ws->res_n * sizeof(double));
@@ -725,8 +725,14 @@ extern "C" int pw_waves_etot(pw_waveset *ws) {
   }
   // band energies from the super-band expectation values
   const double *ex = hr + ws->r_expect;
+  const int nc = expect_chunks(ws);
   for (int b = 0; b < ws->nbh; ++b) {
-    const double re = 0.5 * ex[2 * b] * p->vol, rem = 0.5 * ex[2 * b + 1] * p->vol;
+    double t = 0.0, tm = 0.0;
+    for (int c = 0; c < nc; ++c) {
+      t += ex[2 * ((size_t)c * ws->nbh + b)];
+      tm += ex[2 * ((size_t)c * ws->nbh + b) + 1];
+    }
+    const double re = 0.5 * t * p->vol, rem = 0.5 * tm * p->vol;
     ws->eig[2 * b] = re + rem;
     if (2 * b + 1 < ws->nb) ws->eig[2 * b + 1] = re - rem;
   }
