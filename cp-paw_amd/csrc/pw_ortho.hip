@@ -32,7 +32,8 @@ constexpr int kMaxIter = 200;      // src/paw_waves2.f90:1870 (NITERX)
 constexpr int kLzThreads = 1024;
 constexpr int kLzMaxSteps = 30;
 
-// Loop state of one ORTHO_X call, written only by k_check.
+// Loop state of one ORTHO_X call, written by the convergence test (last
+// workgroup of k_gemm_pair<kPairGamma>) and the acceptance test (k_prep_check).
 struct OrthoState {
   int done;   // 1 once converged or failed
   int iter;   // iteration at which max|GAMN| < tolerance
@@ -93,6 +94,163 @@ __global__ void __launch_bounds__(64 * kGemmWaves) k_gemm(int n, const double *_
       if (D) v += beta * D[(size_t)i * n + j];
       C[(size_t)i * n + j] = v;
     }
+  }
+}
+
+// Tile pairs.  k_gemm_pair computes the tile (I,J) of
+// C = alpha op(A) op(B) + beta D and also the tile (J,I), both with the same
+// K slicing and summation order as k_gemm (so every element has the value
+// k_gemm gives it), and finishes in the epilogue an elementwise step that
+// needs C(i,j) and C(j,i), which saves a launch:
+//   kPairGamma:   GAMN(i,j) = 0.5 (C(i,j) + C(j,i)) + PSIPSI(min,max) - delta,
+//                 block maxima of |GAMN| and, in the last workgroup to finish,
+//                 the convergence test of the iteration;
+//   kPairLambda:  the LAMBDA update (src/paw_waves2.f90:1979-2005) with X = C:
+//                   L'(i,j) = L(i,j) - 2 (f_i+ds)/(f_i+f_j+2ds) X(i,j)
+//                   Lnew(i,j) = L'(j,i) (f_i+ds)/(f_j+ds) if f_i < f_j, else L'(i,j)
+constexpr int kPairGamma = 0, kPairLambda = 1;
+
+struct PairArgs {
+  int n;
+  const double *A, *B, *D;
+  int ta, tb;
+  double alpha, beta;
+  double *out;                   // GAMN or the new LAMBDA
+  const double *psipsi;          // kPairGamma
+  double *bmax;                  // kPairGamma: per-workgroup max |GAMN|
+  unsigned long long *ticket;    // kPairGamma: finished-workgroup counter; any start value works
+  double tol;
+  int it;
+  const double *occ, *lam;       // kPairLambda
+};
+
+__device__ inline double nanmax2(double a, double b) { return (a != a) ? a : ((b != b) ? b : fmax(a, b)); }
+
+template <int MODE>
+__global__ void __launch_bounds__(64 * kGemmWaves) k_gemm_pair(PairArgs a, OrthoState *st) {
+  if (stopped(st)) return;
+  __shared__ dbl4 part[2][kGemmWaves - 1][64];
+  __shared__ double sT[16][17];
+  __shared__ int last_s;
+  const int n = a.n;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, li = lane & 15, lk = lane >> 4;
+  const int I0 = blockIdx.y * T16, J0 = blockIdx.x * T16;
+  const int nk = (n + 3) / 4;
+  constexpr int U = 8;
+  dbl4 acc[2] = {dbl4{0.0, 0.0, 0.0, 0.0}, dbl4{0.0, 0.0, 0.0, 0.0}};
+  // rows/columns of tile 0 = (I,J) and tile 1 = (J,I); the loads of both
+  // tiles are issued together (one memory round trip for n <= 256)
+  const int ia0 = I0 + li, jb0 = J0 + li, ia1 = J0 + li, jb1 = I0 + li;
+  for (int ks0 = wv; ks0 < nk; ks0 += kGemmWaves * U) {
+    double av[2][U], bv[2][U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int k = 4 * (ks0 + kGemmWaves * u) + lk;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int ia = h == 0 ? ia0 : ia1, jb = h == 0 ? jb0 : jb1;
+        av[h][u] = 0.0;
+        bv[h][u] = 0.0;
+        if (k < n) {
+          if (ia < n) av[h][u] = a.ta ? a.A[(size_t)k * n + ia] : a.A[(size_t)ia * n + k];
+          if (jb < n) bv[h][u] = a.tb ? a.B[(size_t)jb * n + k] : a.B[(size_t)k * n + jb];
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+#pragma unroll
+      for (int h = 0; h < 2; ++h) acc[h] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[h][u], bv[h][u], acc[h], 0, 0, 0);
+  }
+  if (wv > 0) {
+    part[0][wv - 1][lane] = acc[0];
+    part[1][wv - 1][lane] = acc[1];
+  }
+  __syncthreads();
+  double cv[4];
+  if (wv == 0) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+#pragma unroll
+      for (int w = 0; w < kGemmWaves - 1; ++w) acc[h] += part[h][w][lane];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        // element (i, j) of tile h: rows of the tile's first index
+        const int i = (h == 0 ? I0 : J0) + lk + 4 * r, j = (h == 0 ? J0 : I0) + li;
+        double v = a.alpha * acc[h][r];
+        if (a.D && i < n && j < n) v += a.beta * a.D[(size_t)i * n + j];
+        if (h == 0) cv[r] = v;
+        else sT[lk + 4 * r][li] = v;      // C(J+jj, I+ii) at sT[jj][ii]
+      }
+    }
+  }
+  __syncthreads();
+  double bmx = 0.0;
+  if (wv == 0) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int ti = lk + 4 * r, i = I0 + ti, j = J0 + li;
+      if (i >= n || j >= n) continue;
+      const double cij = cv[r], cji = sT[li][ti];
+      if (MODE == kPairGamma) {
+        const int lo = min(i, j), hi = max(i, j);
+        double v = 0.5 * (cij + cji) + a.psipsi[(size_t)lo * n + hi];
+        if (i == j) v -= 1.0;
+        a.out[(size_t)i * n + j] = v;
+        bmx = nanmax2(bmx, (v != v) ? v : fabs(v));
+      } else {
+        const double oi = a.occ[i] + 1.0e-12, oj = a.occ[j] + 1.0e-12;
+        if (a.occ[i] < a.occ[j]) {
+          const double lt = a.lam[(size_t)j * n + i] - 2.0 * oj / (oj + oi) * cji;
+          a.out[(size_t)i * n + j] = lt * oi / oj;
+        } else {
+          a.out[(size_t)i * n + j] = a.lam[(size_t)i * n + j] - 2.0 * oi / (oi + oj) * cij;
+        }
+      }
+    }
+  }
+  if (MODE != kPairGamma) return;
+  // ---- block maximum; the last workgroup applies the convergence test ------
+  // (src/paw_waves2.f90:1960-1975)
+  if (wv == 0) {
+    double m = bmx;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) m = nanmax2(m, __shfl_xor(m, o));
+    if (lane == 0) {
+      const unsigned nb = gridDim.x * gridDim.y;
+      a.bmax[blockIdx.y * gridDim.x + blockIdx.x] = m;
+      __threadfence();
+      const unsigned long long tk = atomicAdd(a.ticket, 1ull);
+      last_s = (tk % nb) == nb - 1;
+    }
+  }
+  __syncthreads();
+  if (!last_s) return;
+  __threadfence();
+  if (wv != 0) return;
+  const int nb = gridDim.x * gridDim.y;
+  double m = 0.0;
+  int nan = 0;
+  for (int b = lane; b < nb; b += 64) {
+    const double v = a.bmax[b];
+    if (v != v) nan = 1;
+    else m = fmax(m, v);
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    m = fmax(m, __shfl_xor(m, o));
+    nan |= __shfl_xor(nan, o);
+  }
+  if (lane != 0) return;
+  if (nan) {
+    st->code = 1;
+    st->done = 1;
+  } else if (m < a.tol) {
+    st->iter = a.it;
+    st->done = 1;
+  } else if (m > 1.0e200) {
+    st->code = 2;
+    st->done = 1;
   }
 }
 
@@ -248,67 +406,6 @@ __global__ void __launch_bounds__(kLzThreads) k_lanczos(int n, int m, const doub
   }
 }
 
-// GAMN(i,j) = 0.5(T(i,j)+T(j,i)) + A(min,max) - delta; block maxima of |GAMN|
-__global__ void k_gamma(int n, const double *T, const double *A, double *G, double *bmax,
-                        const OrthoState *st) {
-  if (stopped(st)) return;
-  __shared__ double red[256];
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  double m = 0.0;
-  if (idx < n * n) {
-    const int i = idx / n, j = idx % n;
-    const int lo = min(i, j), hi = max(i, j);
-    double v = 0.5 * (T[(size_t)lo * n + hi] + T[(size_t)hi * n + lo]) + A[(size_t)lo * n + hi];
-    if (i == j) v -= 1.0;
-    G[idx] = v;
-    m = fabs(v);
-    if (v != v) m = v;
-  }
-  red[threadIdx.x] = m;
-  __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) {
-      const double a = red[threadIdx.x], b = red[threadIdx.x + s];
-      red[threadIdx.x] = (a != a || b != b) ? (a != a ? a : b) : fmax(a, b);
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) bmax[blockIdx.x] = red[0];
-}
-
-// convergence test of iteration it (src/paw_waves2.f90:1960-1975)
-__global__ void k_check(int nblk, const double *bmax, double tol, int it, OrthoState *st) {
-  if (st->done) return;
-  __shared__ double red[256];
-  __shared__ int nan_s;
-  if (threadIdx.x == 0) nan_s = 0;
-  __syncthreads();
-  double m = 0.0;
-  for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
-    const double v = bmax[b];
-    if (v != v) nan_s = 1;
-    else m = fmax(m, v);
-  }
-  red[threadIdx.x] = m;
-  __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
-    __syncthreads();
-  }
-  if (threadIdx.x != 0) return;
-  const double digam = red[0];
-  if (nan_s) {
-    st->code = 1;
-    st->done = 1;
-  } else if (digam < tol) {
-    st->iter = it;
-    st->done = 1;
-  } else if (digam > 1.0e200) {
-    st->code = 2;
-    st->done = 1;
-  }
-}
-
 // Acceptance test of the spectral bounds (before the outer loop):
 // R = (B+qI)M - I must vanish and the last Smith power A^(2^J) must be small
 // enough that the truncated sum is exact to rounding; otherwise code 3 makes
@@ -373,25 +470,6 @@ __global__ void k_state_init(OrthoState *st) {
   }
 }
 
-// One LAMBDA update (src/paw_waves2.f90:1979-2005):
-//   L'(i,j) = L(i,j) - 2 (f_i+ds)/(f_i+f_j+2ds) X(i,j)
-//   Lnew(i,j) = L'(j,i) (f_i+ds)/(f_j+ds) if f_i < f_j, else L'(i,j)
-__global__ void k_lam_step(int n, const double *occ, const double *X, const double *lam,
-                           double *lnew, const OrthoState *st) {
-  if (stopped(st)) return;
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= n * n) return;
-  const int i = idx / n, j = idx % n;
-  const double oi = occ[i] + 1.0e-12, oj = occ[j] + 1.0e-12;
-  if (occ[i] < occ[j]) {
-    const size_t t = (size_t)j * n + i;
-    const double lt = lam[t] - 2.0 * oj / (oj + oi) * X[t];
-    lnew[idx] = lt * oi / oj;
-  } else {
-    lnew[idx] = lam[idx] - 2.0 * oi / (oi + oj) * X[idx];
-  }
-}
-
 __global__ void k_copy_d(size_t n, const double *src, double *dst) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) dst[i] = src[i];
@@ -414,6 +492,7 @@ struct OrthoWork {
   double *bs, *m0, *m1, *gam, *x0, *x1, *t1, *t2, *lam2, *occ, *red, *bounds;
   double *apow[kMaxLevels];
   OrthoState *st;
+  unsigned long long *ticket;   // bounds[4]
 };
 
 int ortho_work(pw_waveset *ws, OrthoWork &w) {
@@ -432,6 +511,7 @@ int ortho_work(pw_waveset *ws, OrthoWork &w) {
   for (int j = 0; j < kMaxLevels; ++j) { w.apow[j] = p; p += n2; }
   w.occ = p; p += ws->nb;
   w.bounds = p; p += 8;
+  w.ticket = reinterpret_cast<unsigned long long *>(w.bounds + 4);
   w.st = reinterpret_cast<OrthoState *>(p); p += 8;
   w.red = p;
   return PW_OK;
@@ -444,6 +524,12 @@ int gemm(pw_waveset *ws, const double *A, int ta, const double *B, int tb, doubl
   dim3 grid((n + T16 - 1) / T16, (n + T16 - 1) / T16);
   k_gemm<<<grid, 64 * kGemmWaves, 0, ws->stream>>>(n, A, ta, B, tb, C, alpha, D, beta, st);
   return PW_OK;
+}
+
+template <int MODE>
+void gemm_pair(pw_waveset *ws, const PairArgs &a, OrthoState *st) {
+  const dim3 grid((a.n + T16 - 1) / T16, (a.n + T16 - 1) / T16);
+  k_gemm_pair<MODE><<<grid, 64 * kGemmWaves, 0, ws->stream>>>(a, st);
 }
 
 int lanczos_bounds(pw_waveset *ws, const double *d_b, double *d_out, double *lo, double *hi) {
@@ -503,26 +589,52 @@ void enqueue_iters(pw_waveset *ws, const OrthoWork &w, const OrthoParams &P, con
                    double *const lbuf[2], const double *d_chichi, const double *d_psipsi,
                    const double *d_chipsi, int it0, int it1) {
   const int n = ws->nb;
-  const int nblk = (int)(((size_t)n * n + 255) / 256);
   const double eps = 1.0e-8;
   for (int it = it0; it <= it1; ++it) {
     const double *lam = lbuf[(it - 1) & 1];
     double *lnew = lbuf[it & 1];
     // HAUX = CHICHI LAMBDA + 2 CHIPSI ; T = LAMBDA^T HAUX
     gemm(ws, d_chichi, 0, lam, 0, w.t1, 1.0, d_chipsi, 2.0, w.st);
-    gemm(ws, lam, 1, w.t1, 0, w.t2, 1.0, nullptr, 0.0, w.st);
-    k_gamma<<<nblk, 256, 0, ws->stream>>>(n, w.t2, d_psipsi, w.gam, w.red, w.st);
-    k_check<<<1, 256, 0, ws->stream>>>(nblk, w.red, eps, it, w.st);
+    {
+      // T = LAMBDA^T HAUX, GAMN and the convergence test in one launch
+      PairArgs g{};
+      g.n = n;
+      g.A = lam;
+      g.ta = 1;
+      g.B = w.t1;
+      g.alpha = 1.0;
+      g.out = w.gam;
+      g.psipsi = d_psipsi;
+      g.bmax = w.red;
+      g.ticket = w.ticket;
+      g.tol = eps;
+      g.it = it;
+      gemm_pair<kPairGamma>(ws, g, w.st);
+    }
     // X = S^-1(GAMN): X0 = 2q M G M ; X <- X + A_j X A_j
     gemm(ws, w.gam, 0, mc, 0, w.t1, 1.0, nullptr, 0.0, w.st);
     gemm(ws, mc, 0, w.t1, 0, w.x0, 2.0 * P.q, nullptr, 0.0, w.st);
     double *xc = w.x0, *xn = w.x1;
     for (int j = 0; j < P.nlev; ++j) {
       gemm(ws, xc, 0, w.apow[j], 0, w.t1, 1.0, nullptr, 0.0, w.st);
-      gemm(ws, w.apow[j], 0, w.t1, 0, xn, 1.0, xc, 1.0, w.st);
-      std::swap(xc, xn);
+      if (j + 1 < P.nlev) {
+        gemm(ws, w.apow[j], 0, w.t1, 0, xn, 1.0, xc, 1.0, w.st);
+        std::swap(xc, xn);
+      } else {
+        // last level: X = A_j T + X and the LAMBDA update in one launch
+        PairArgs l{};
+        l.n = n;
+        l.A = w.apow[j];
+        l.B = w.t1;
+        l.D = xc;
+        l.alpha = 1.0;
+        l.beta = 1.0;
+        l.out = lnew;
+        l.occ = w.occ;
+        l.lam = lam;
+        gemm_pair<kPairLambda>(ws, l, w.st);
+      }
     }
-    k_lam_step<<<nblk, 256, 0, ws->stream>>>(n, w.occ, xc, lam, lnew, w.st);
   }
 }
 

@@ -147,7 +147,11 @@ bool fft_factor(int n, Fft1D &f) {
   f = Fft1D();
   f.n = n;
   int m = n;
-  const int order[] = {16, 8, 9, 4, 2, 3, 5, 7, 11};
+  // Odd radices first: the first DIT stage (and the last DIF stage) of the
+  // in-place LDS transforms reads with element stride = its radix across the
+  // lanes of a wave, which maps to few LDS banks for 8 or 16 but to (almost)
+  // all banks for 9, 3, 5, 7 or 11.  Then the largest powers of two.
+  const int order[] = {9, 3, 5, 7, 11, 16, 8, 4, 2};
   for (int r : order) {
     while (m % r == 0) {
       f.radix[f.nstage++] = r;
