@@ -179,7 +179,7 @@ struct pw_waveset {
   // results
   double ekin = 0, epot = 0, e1c = 0, etot = 0, wavekin1 = 0, wavekin2 = 0;
   int ortho_iter = 0;
-  int ortho_levels = 0, ortho_ns = 0;
+  int ortho_levels = 0, ortho_ns = 0, ortho_series = 0;
   // spectral bounds of sym(CHIPSI) from the last Lanczos run, reused while
   // the device checks of ORTHO_X accept them
   double lz_lo = 0.0, lz_hi = 0.0;

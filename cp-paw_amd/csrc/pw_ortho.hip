@@ -9,10 +9,12 @@
 // applies S^-1 exactly (to rounding) with GEMMs only, by the Cayley transform
 // and the squared Smith iteration:
 //   M = (B + qI)^-1 (Newton-Schulz), A = I - 2qM = (B - qI)M,
-//   S^-1(G) = sum_k A^k (2q M G M) A^k, summed by X <- X + A_j X A_j with
-//   A_j = A^(2^j); the error after J levels is rho^(2^J), rho =
-//   (sqrt(k)-1)/(sqrt(k)+1) for the spectral bounds of B (Lanczos on the
-//   device, widened).  B is positive definite in the CP step (B ~ dt^2/m).
+//   S^-1(G) = sum_{k<2^J} A^k (2q M G M) A^k; the error after 2^J terms is
+//   rho^(2^(J+1)), rho = (sqrt(k)-1)/(sqrt(k)+1) for the spectral bounds of
+//   B (Lanczos on the device, widened).  B is positive definite in the CP
+//   step (B ~ dt^2/m).  The first 2^5 terms are summed directly as a series
+//   with P_k = A^k M formed once per call (two launches per iteration); any
+//   further terms by the Smith recursion X <- X + A_j X A_j, A_j = A^(2^j).
 // Every product is an NB^3 FP64-MFMA GEMM; nothing is diagonalised.  The
 // outer loop runs on the device: the convergence test sets a flag that makes
 // every later kernel of the batch return at once, so the host waits only
@@ -50,14 +52,11 @@ __device__ inline bool stopped(const OrthoState *st) { return st != nullptr && s
 // v_mfma_f64_16x16x4f64: A(i,k) in lane i+16k, B(k,j) in lane j+16k, C(i,j)
 // in lane j+16(i%4), element i/4.
 constexpr int kGemmWaves = 8;
-__global__ void __launch_bounds__(64 * kGemmWaves) k_gemm(int n, const double *__restrict__ A, int ta,
-                                                          const double *__restrict__ B, int tb, double *C,
-                                                          double alpha, const double *D, double beta,
-                                                          const OrthoState *st) {
-  if (stopped(st)) return;
-  __shared__ dbl4 part[kGemmWaves - 1][64];
+__device__ __forceinline__ void gemm_tile(int n, const double *__restrict__ A, int ta,
+                                          const double *__restrict__ B, int tb, double *C, double alpha,
+                                          const double *D, double beta, int i0, int j0,
+                                          dbl4 (*part)[64]) {
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int i0 = blockIdx.y * T16, j0 = blockIdx.x * T16;
   const int li = lane & 15, lk = lane >> 4;
   const int ia = i0 + li, jb = j0 + li;
   const bool aok = ia < n, bok = jb < n;
@@ -97,14 +96,137 @@ __global__ void __launch_bounds__(64 * kGemmWaves) k_gemm(int n, const double *_
   }
 }
 
+__global__ void __launch_bounds__(64 * kGemmWaves) k_gemm(int n, const double *__restrict__ A, int ta,
+                                                          const double *__restrict__ B, int tb, double *C,
+                                                          double alpha, const double *D, double beta,
+                                                          const OrthoState *st) {
+  if (stopped(st)) return;
+  __shared__ dbl4 part[kGemmWaves - 1][64];
+  gemm_tile(n, A, ta, B, tb, C, alpha, D, beta, blockIdx.y * T16, blockIdx.x * T16, part);
+}
+
+// Row panels.  A workgroup of kPanelWaves waves forms a 16 x n row panel of
+// an n x n product: the A row panel is staged in LDS once and the waves take
+// the 16 x 16 column tiles j = w, w + kPanelWaves, ... each over the whole K
+// (no split of K, no reduction).  The B operand of a tile is loaded in
+// chunks of kKC MFMA steps, all loads of a chunk issued before its MFMAs:
+// one memory round trip per tile for n <= 4 kKC.  B is row-major n x n,
+// TB = 0: B(k, j) = B[k n + j], TB = 1: B(k, j) = B[j n + k].
+constexpr int kPanelWaves = 9;   // one wave per column tile at NB = 144
+constexpr int kKC = 36;
+
+// B operand of one chunk: b[u] = B(4 (s0 + u) + lk, j0 + li).  Row and
+// column indices are clamped into the matrix, so every load is valid and
+// unpredicated; a clamped row k >= n meets a zero A column (the LDS pad) and
+// a clamped column j >= n gives an output column that is not stored.
+template <int TB>
+__device__ __forceinline__ void load_bchunk(double (&b)[kKC], int n, const double *__restrict__ B, int j0,
+                                            int s0) {
+  const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+  const int jc = min(j0 + li, n - 1);
+#pragma unroll
+  for (int u = 0; u < kKC; ++u) {
+    const int kc = min(4 * (s0 + u) + lk, n - 1);
+    b[u] = TB ? B[(size_t)jc * n + kc] : B[(size_t)kc * n + jc];
+  }
+}
+
+// acc += A(:, chunk) B(chunk, tile), A(i, k) = sA[i ldr + k] in LDS; columns
+// n..ldr-1 of sA are zero and k is clamped to them
+__device__ __forceinline__ void mfma_chunk(dbl4 &acc, const double (&b)[kKC], const double *sA, int ldr, int s0) {
+  const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+#pragma unroll
+  for (int u = 0; u < kKC; ++u) {
+    const double av = sA[li * ldr + min(4 * (s0 + u) + lk, ldr - 1)];
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, b[u], acc, 0, 0, 0);
+  }
+}
+
+// the chunks s0 >= s1 of one tile (the earlier ones come prefetched)
+template <int TB>
+__device__ __forceinline__ void panel_rest(dbl4 &acc, int n, const double *sA, int ldr,
+                                           const double *__restrict__ B, int j0, int s1) {
+  const int nk = (n + 3) / 4;
+#pragma unroll 1
+  for (int s0 = s1; s0 < nk; s0 += kKC) {
+    double b[kKC];
+    load_bchunk<TB>(b, n, B, j0, s0);
+    mfma_chunk(acc, b, sA, ldr, s0);
+  }
+}
+
+// A rows 0..15 of a row panel (rows >= nr zero) into sA[16][ldr], columns
+// n..ldr-1 zero
+__device__ __forceinline__ void stage_panel(int n, const double *__restrict__ A, int nr, double *sA, int ldr) {
+  for (int e = threadIdx.x; e < T16 * ldr; e += blockDim.x) {
+    const int i = e / ldr, k = e - i * ldr;
+    sA[e] = (i < nr && k < n) ? A[(size_t)i * n + k] : 0.0;
+  }
+}
+
+// LDS row stride of a staged panel: >= n + 1 (a zero pad column) and == 2
+// (mod 32) doubles, so the 32 lanes of a half-wave reading A(li, 4s + lk)
+// hit distinct bank pairs
+__host__ __device__ inline int panel_ldr(int n) { return n + 1 + ((1 - n) % 32 + 32) % 32; }
+
+constexpr size_t kLdsMax = 160 * 1024;
+inline size_t panel_lds(int n) { return (size_t)T16 * panel_ldr(n) * sizeof(double); }
+// the series needs two staged panels per workgroup
+inline bool series_fits(int n) { return 2 * panel_lds(n) <= kLdsMax; }
+
+// Batched products with one shared left operand, by row panels:
+//   C_z = A B_z,  B_z = B + z sb, C_z = C + z sc,  z < nz,
+// and, when Bx is set, one more member z = nz: Cx = A Bx.
+// Grid (row panel, member).
+struct BatchArgs {
+  int n, nz;
+  const double *A;
+  const double *B;
+  size_t sb;
+  double *C;
+  size_t sc;
+  const double *Bx;
+  double *Cx;
+};
+
+__global__ void __launch_bounds__(64 * kPanelWaves) k_panel_batch(BatchArgs a) {
+  extern __shared__ double sA[];   // 16 x ldr
+  const int n = a.n, nt = (n + T16 - 1) / T16, ldr = panel_ldr(n);
+  const int z = blockIdx.y, I0 = blockIdx.x * T16;
+  const double *B = z < a.nz ? a.B + (size_t)z * a.sb : a.Bx;
+  double *C = z < a.nz ? a.C + (size_t)z * a.sc : a.Cx;
+  const int wv = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+  double b0[kKC];
+  if (wv < nt) load_bchunk<0>(b0, n, B, wv * T16, 0);
+  stage_panel(n, a.A + (size_t)I0 * n, n - I0, sA, ldr);
+  __syncthreads();
+#pragma unroll 1
+  for (int jt = wv; jt < nt; jt += kPanelWaves) {
+    dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+    if (jt == wv) {
+      mfma_chunk(acc, b0, sA, ldr, 0);
+      panel_rest<0>(acc, n, sA, ldr, B, jt * T16, kKC);
+    } else {
+      panel_rest<0>(acc, n, sA, ldr, B, jt * T16, 0);
+    }
+    const int j = jt * T16 + li;
+    if (j >= n) continue;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = I0 + lk + 4 * r;
+      if (i < n) C[(size_t)i * n + j] = acc[r];
+    }
+  }
+}
+
 // Tile pairs.  k_gemm_pair computes the tile (I,J) of
 // C = alpha op(A) op(B) + beta D and also the tile (J,I), both with the same
 // K slicing and summation order as k_gemm (so every element has the value
 // k_gemm gives it), and finishes in the epilogue an elementwise step that
 // needs C(i,j) and C(j,i), which saves a launch:
-//   kPairGamma:   GAMN(i,j) = 0.5 (C(i,j) + C(j,i)) + PSIPSI(min,max) - delta,
-//                 block maxima of |GAMN| and, in the last workgroup to finish,
-//                 the convergence test of the iteration;
+//   kPairGamma:   GAMN(i,j) = 0.5 (C(i,j) + C(j,i)) + PSIPSI(min,max) - delta
+//                 and the block maxima of |GAMN|;
 //   kPairLambda:  the LAMBDA update (src/paw_waves2.f90:1979-2005) with X = C:
 //                   L'(i,j) = L(i,j) - 2 (f_i+ds)/(f_i+f_j+2ds) X(i,j)
 //                   Lnew(i,j) = L'(j,i) (f_i+ds)/(f_j+ds) if f_i < f_j, else L'(i,j)
@@ -118,9 +240,6 @@ struct PairArgs {
   double *out;                   // GAMN or the new LAMBDA
   const double *psipsi;          // kPairGamma
   double *bmax;                  // kPairGamma: per-workgroup max |GAMN|
-  unsigned long long *ticket;    // kPairGamma: finished-workgroup counter; any start value works
-  double tol;
-  int it;
   const double *occ, *lam;       // kPairLambda
 };
 
@@ -131,7 +250,6 @@ __global__ void __launch_bounds__(64 * kGemmWaves) k_gemm_pair(PairArgs a, Ortho
   if (stopped(st)) return;
   __shared__ dbl4 part[2][kGemmWaves - 1][64];
   __shared__ double sT[16][17];
-  __shared__ int last_s;
   const int n = a.n;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, li = lane & 15, lk = lane >> 4;
   const int I0 = blockIdx.y * T16, J0 = blockIdx.x * T16;
@@ -210,28 +328,122 @@ __global__ void __launch_bounds__(64 * kGemmWaves) k_gemm_pair(PairArgs a, Ortho
     }
   }
   if (MODE != kPairGamma) return;
-  // ---- block maximum; the last workgroup applies the convergence test ------
-  // (src/paw_waves2.f90:1960-1975)
+  // block maximum of |GAMN| (NaN propagates); the next kernel of the
+  // iteration (k_series_sum) applies the convergence test
   if (wv == 0) {
     double m = bmx;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) m = nanmax2(m, __shfl_xor(m, o));
-    if (lane == 0) {
-      const unsigned nb = gridDim.x * gridDim.y;
-      a.bmax[blockIdx.y * gridDim.x + blockIdx.x] = m;
-      __threadfence();
-      const unsigned long long tk = atomicAdd(a.ticket, 1ull);
-      last_s = (tk % nb) == nb - 1;
-    }
+    if (lane == 0) a.bmax[blockIdx.y * gridDim.x + blockIdx.x] = m;
+  }
+}
+
+// The Smith sum written out as a series (the Cayley series of S^-1):
+//   S^-1(G) = 2q sum_{k<K} A^k M G M A^k = 2q sum_k P_k G P_k^T,  P_k = A^k M,
+// with the P_k formed once per call.  k_series_panel: workgroup (row panel
+// I, term k) forms R = P_k(I,:) G into LDS and then the row panel
+// part_k(I,:) = R P_k^T, so each term is a chain of two row-panel products
+// inside one workgroup.  k_series_sum adds the K terms of each element in
+// the order k = 0, 1, ..., and finishes with
+//   kSeriesX:      out = scale * sum            (Smith levels follow), or
+//   kSeriesLambda: the LAMBDA update of kPairLambda with X = scale * sum.
+// Two launches per iteration replace the 2 + 2 log2(K) of the Smith
+// recursion; the sum is the same to rounding.
+constexpr int kSeriesX = 0, kSeriesLambda = 1;
+constexpr int kSeriesMaxLevels = 5;      // at most K = 2^5 terms as a series
+constexpr double kAcceptPower = 1.0e-4;  // acceptance: n max|A^(2^(J-1))| <= this
+constexpr double kSelectPower = 3.0e-5;  // level choice: estimated n rho^(2^(J-1)) <= this
+
+struct SeriesArgs {
+  int n;
+  const double *P;             // K stacked n x n row-major matrices P_k
+  const double *G;             // GAMN
+  double *part;                // K stacked terms P_k G P_k^T
+};
+
+__global__ void __launch_bounds__(64 * kPanelWaves) k_series_panel(SeriesArgs a, const OrthoState *st) {
+  if (stopped(st)) return;
+  extern __shared__ double sm[];   // sA, sR: 16 x ldr each
+  const int n = a.n, nt = (n + T16 - 1) / T16, ldr = panel_ldr(n);
+  double *sA = sm, *sR = sm + T16 * ldr;
+  const size_t n2 = (size_t)n * n;
+  const int I0 = blockIdx.x * T16;
+  const double *Pk = a.P + (size_t)blockIdx.y * n2;
+  double *out = a.part + (size_t)blockIdx.y * n2;
+  const int wv = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+  // the B operand of the first product (first tile, first chunk) goes out
+  // with the A panel; that of the second as soon as the first's MFMAs are
+  // issued
+  double b1[kKC];
+  if (wv < nt) load_bchunk<0>(b1, n, a.G, wv * T16, 0);
+  stage_panel(n, Pk + (size_t)I0 * n, n - I0, sA, ldr);
+  for (int e = threadIdx.x; e < T16 * (ldr - n); e += blockDim.x) {
+    const int i = e / (ldr - n);
+    sR[i * ldr + n + (e - i * (ldr - n))] = 0.0;
   }
   __syncthreads();
-  if (!last_s) return;
-  __threadfence();
-  if (wv != 0) return;
-  const int nb = gridDim.x * gridDim.y;
+  // R = P_k(I,:) G
+#pragma unroll 1
+  for (int jt = wv; jt < nt; jt += kPanelWaves) {
+    dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+    if (jt == wv) {
+      mfma_chunk(acc, b1, sA, ldr, 0);
+      panel_rest<0>(acc, n, sA, ldr, a.G, jt * T16, kKC);
+    } else {
+      panel_rest<0>(acc, n, sA, ldr, a.G, jt * T16, 0);
+    }
+    const int j = jt * T16 + li;
+    if (j >= n) continue;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) sR[(lk + 4 * r) * ldr + j] = acc[r];
+  }
+  double b2[kKC];
+  if (wv < nt) load_bchunk<1>(b2, n, Pk, wv * T16, 0);
+  __syncthreads();
+  // part_k(I,:) = R P_k^T
+#pragma unroll 1
+  for (int jt = wv; jt < nt; jt += kPanelWaves) {
+    dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+    if (jt == wv) {
+      mfma_chunk(acc, b2, sR, ldr, 0);
+      panel_rest<1>(acc, n, sR, ldr, Pk, jt * T16, kKC);
+    } else {
+      panel_rest<1>(acc, n, sR, ldr, Pk, jt * T16, 0);
+    }
+    const int j = jt * T16 + li;
+    if (j >= n) continue;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = I0 + lk + 4 * r;
+      if (i < n) out[(size_t)i * n + j] = acc[r];
+    }
+  }
+}
+
+struct SumArgs {
+  int n, nterm;
+  const double *part;
+  double scale;
+  double *out;
+  const double *occ, *lam;     // kSeriesLambda
+  const double *bmax;          // block maxima of |GAMN| (k_gemm_pair<kPairGamma>)
+  int nbmax;
+  double tol;
+  int it;
+};
+
+// The convergence test of iteration it (src/paw_waves2.f90:1960-1975) from
+// the block maxima of |GAMN|: every workgroup reaches the same decision; the
+// first one records it.  Then the sum of the terms and the epilogue.
+template <int MODE>
+__global__ void __launch_bounds__(256) k_series_sum(SumArgs a, OrthoState *st) {
+  if (stopped(st)) return;
+  __shared__ double wm[4];
+  __shared__ int wn[4];
   double m = 0.0;
   int nan = 0;
-  for (int b = lane; b < nb; b += 64) {
+  for (int b = threadIdx.x; b < a.nbmax; b += 256) {
     const double v = a.bmax[b];
     if (v != v) nan = 1;
     else m = fmax(m, v);
@@ -241,16 +453,54 @@ __global__ void __launch_bounds__(64 * kGemmWaves) k_gemm_pair(PairArgs a, Ortho
     m = fmax(m, __shfl_xor(m, o));
     nan |= __shfl_xor(nan, o);
   }
-  if (lane != 0) return;
-  if (nan) {
-    st->code = 1;
-    st->done = 1;
-  } else if (m < a.tol) {
-    st->iter = a.it;
-    st->done = 1;
-  } else if (m > 1.0e200) {
-    st->code = 2;
-    st->done = 1;
+  if ((threadIdx.x & 63) == 0) {
+    wm[threadIdx.x >> 6] = m;
+    wn[threadIdx.x >> 6] = nan;
+  }
+  __syncthreads();
+  m = fmax(fmax(wm[0], wm[1]), fmax(wm[2], wm[3]));
+  nan = wn[0] | wn[1] | wn[2] | wn[3];
+  const int code = nan ? 1 : (m < a.tol ? 0 : (m > 1.0e200 ? 2 : -1));
+  if (code >= 0) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      st->code = code;
+      if (code == 0) st->iter = a.it;
+      st->done = 1;
+    }
+    return;
+  }
+  const int n = a.n;
+  const size_t n2 = (size_t)n * n;
+  const int idx = blockIdx.x * 256 + threadIdx.x;
+  if (idx >= n * n) return;
+  const int i = idx / n, j = idx - i * n;
+  const bool swap = MODE == kSeriesLambda && a.occ[i] < a.occ[j];
+  // element (i,j), or (j,i) where the update takes the transposed element;
+  // the loads of a group of terms are issued together, the adds in the
+  // order k = 0, 1, ...
+  const double *src = a.part + (swap ? (size_t)j * n + i : (size_t)i * n + j);
+  double sum = 0.0;
+  constexpr int V = 16;
+  for (int k0 = 0; k0 < a.nterm; k0 += V) {
+    double v[V];
+#pragma unroll
+    for (int u = 0; u < V; ++u) v[u] = k0 + u < a.nterm ? src[(size_t)(k0 + u) * n2] : 0.0;
+#pragma unroll
+    for (int u = 0; u < V; ++u)
+      if (k0 + u < a.nterm) sum += v[u];
+  }
+  const double c = a.scale * sum;
+  if (MODE == kSeriesX) {
+    a.out[(size_t)i * n + j] = c;
+    return;
+  }
+  const double oi = a.occ[i] + 1.0e-12, oj = a.occ[j] + 1.0e-12;
+  if (swap) {
+    const double cji = c;
+    const double lt = a.lam[(size_t)j * n + i] - 2.0 * oj / (oj + oi) * cji;
+    a.out[(size_t)i * n + j] = lt * oi / oj;
+  } else {
+    a.out[(size_t)i * n + j] = a.lam[(size_t)i * n + j] - 2.0 * oi / (oi + oj) * c;
   }
 }
 
@@ -491,27 +741,39 @@ __global__ void k_cayley(int n, double q, const double *M, double *A) {
 struct OrthoWork {
   double *bs, *m0, *m1, *gam, *x0, *x1, *t1, *t2, *lam2, *occ, *red, *bounds;
   double *apow[kMaxLevels];
+  double *pstk, *spart;         // Cayley series: P_k and the terms (K n x n each)
   OrthoState *st;
-  unsigned long long *ticket;   // bounds[4]
 };
 
 int ortho_work(pw_waveset *ws, OrthoWork &w) {
   const size_t n2 = (size_t)ws->nb * ws->nb;
   const size_t nblk = (n2 + 255) / 256;
-  const size_t need = (9 + kMaxLevels) * n2 + ws->nb + 2 * nblk + 64;
+  const size_t kser = (size_t)1 << kSeriesMaxLevels;
+  const size_t need = (9 + kMaxLevels + 2 * kser) * n2 + ws->nb + 2 * nblk + 64;
   if (need > ws->ortho_cap) {
     if (ws->d_ortho) PW_HIP(hipFree(ws->d_ortho));
+    ws->d_ortho = nullptr;
+    ws->ortho_cap = 0;
     PW_HIP(hipMalloc(&ws->d_ortho, need * sizeof(double)));
     ws->ortho_cap = need;
+    // LDS of the row-panel kernels (set outside any stream capture)
+    const size_t lds = (size_t)T16 * panel_ldr(ws->nb) * sizeof(double);
+    if (lds > 65536 && lds <= kLdsMax)
+      PW_HIP(hipFuncSetAttribute((const void *)k_panel_batch, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)lds));
+    if (2 * lds > 65536 && 2 * lds <= kLdsMax)
+      PW_HIP(hipFuncSetAttribute((const void *)k_series_panel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)(2 * lds)));
   }
   double *p = ws->d_ortho;
   w.bs = p; p += n2; w.m0 = p; p += n2; w.m1 = p; p += n2; w.gam = p; p += n2;
   w.x0 = p; p += n2; w.x1 = p; p += n2; w.t1 = p; p += n2; w.t2 = p; p += n2;
   w.lam2 = p; p += n2;
   for (int j = 0; j < kMaxLevels; ++j) { w.apow[j] = p; p += n2; }
+  w.pstk = p; p += kser * n2;
+  w.spart = p; p += kser * n2;
   w.occ = p; p += ws->nb;
   w.bounds = p; p += 8;
-  w.ticket = reinterpret_cast<unsigned long long *>(w.bounds + 4);
   w.st = reinterpret_cast<OrthoState *>(p); p += 8;
   w.red = p;
   return PW_OK;
@@ -530,6 +792,24 @@ template <int MODE>
 void gemm_pair(pw_waveset *ws, const PairArgs &a, OrthoState *st) {
   const dim3 grid((a.n + T16 - 1) / T16, (a.n + T16 - 1) / T16);
   k_gemm_pair<MODE><<<grid, 64 * kGemmWaves, 0, ws->stream>>>(a, st);
+}
+
+int panel_batch(pw_waveset *ws, const BatchArgs &a) {
+  if (panel_lds(a.n) > kLdsMax) return pw_fail(PW_ERR_UNSUPPORTED, "WAVES_ORTHO_X: NB too large");
+  const dim3 grid((a.n + T16 - 1) / T16, a.nz + (a.Bx ? 1 : 0));
+  k_panel_batch<<<grid, 64 * kPanelWaves, panel_lds(a.n), ws->stream>>>(a);
+  return PW_OK;
+}
+
+void series_panel(pw_waveset *ws, const SeriesArgs &a, int nterm, const OrthoState *st) {
+  const dim3 grid((a.n + T16 - 1) / T16, nterm);
+  k_series_panel<<<grid, 64 * kPanelWaves, 2 * panel_lds(a.n), ws->stream>>>(a, st);
+}
+
+template <int MODE>
+void series_sum(pw_waveset *ws, const SumArgs &a, OrthoState *st) {
+  const int nblk = (int)(((size_t)a.n * a.n + 255) / 256);
+  k_series_sum<MODE><<<nblk, 256, 0, ws->stream>>>(a, st);
 }
 
 int lanczos_bounds(pw_waveset *ws, const double *d_b, double *d_out, double *lo, double *hi) {
@@ -559,44 +839,68 @@ namespace {
 
 struct OrthoParams {
   double lo, hi, q;
-  int nlev, nns;
+  int nlev;   // J: S^-1 sums 2^J terms
+  int nser;   // the first 2^nser of them as a series (nser <= J), the rest by Smith levels
+  int nns;    // Newton-Schulz steps
 };
 
-// M, A^(2^j) and the acceptance test (everything before the outer loop)
-void enqueue_prep(pw_waveset *ws, const OrthoWork &w, const OrthoParams &P, double **mout) {
+// M, the series matrices P_k = A^k M, the powers A^(2^j) and the acceptance
+// test (everything before the outer loop)
+int enqueue_prep(pw_waveset *ws, const OrthoWork &w, const OrthoParams &P) {
   const int n = ws->nb;
-  const int nblk = (int)(((size_t)n * n + 255) / 256);
+  const size_t n2 = (size_t)n * n;
+  const int nblk = (int)((n2 + 255) / 256);
+  const int ls = P.nser;
   k_state_init<<<1, 64, 0, ws->stream>>>(w.st);
-  // M = (B + qI)^-1 by Newton-Schulz, X0 = 2/(lo+hi+2q) I
+  // M = (B + qI)^-1 by Newton-Schulz, X0 = 2/(lo+hi+2q) I; the last step
+  // writes M to P_0
   k_scaled_identity<<<nblk, 256, 0, ws->stream>>>(n, 2.0 / (P.lo + P.hi + 2.0 * P.q), w.m0);
   double *mc = w.m0, *mn = w.m1;
   for (int k = 0; k < P.nns; ++k) {
+    double *dst = (k + 1 == P.nns) ? w.pstk : mn;
     gemm(ws, w.bs, 0, mc, 0, w.t1, 1.0, mc, P.q);       // T = (B+qI) X
-    gemm(ws, mc, 0, w.t1, 0, mn, -1.0, mc, 2.0);        // X' = 2X - X T
-    std::swap(mc, mn);
+    gemm(ws, mc, 0, w.t1, 0, dst, -1.0, mc, 2.0);       // X' = 2X - X T
+    mn = mc;
+    mc = dst;
   }
-  // A^(2^j), j = 0..nlev-1
+  // A = I - 2qM; level j doubles the series, P_{2^j + k} = A^(2^j) P_k for
+  // k < 2^j, and squares the power, A^(2^(j+1)) = A^(2^j) A^(2^j)
   k_cayley<<<nblk, 256, 0, ws->stream>>>(n, P.q, mc, w.apow[0]);
-  for (int j = 1; j < P.nlev; ++j) gemm(ws, w.apow[j - 1], 0, w.apow[j - 1], 0, w.apow[j]);
+  for (int j = 0; j < ls; ++j) {
+    BatchArgs b{};
+    b.n = n;
+    b.nz = 1 << j;
+    b.A = w.apow[j];
+    b.B = w.pstk;
+    b.sb = n2;
+    b.C = w.pstk + ((size_t)1 << j) * n2;
+    b.sc = n2;
+    if (j + 1 < P.nlev) {
+      b.Bx = w.apow[j];
+      b.Cx = w.apow[j + 1];
+    }
+    PW_TRY(panel_batch(ws, b));
+  }
+  for (int j = ls; j + 1 < P.nlev; ++j) gemm(ws, w.apow[j], 0, w.apow[j], 0, w.apow[j + 1]);
   gemm(ws, w.bs, 0, mc, 0, w.t1, 1.0, mc, P.q);         // (B+qI) M
   k_prep_part<<<nblk, 256, 0, ws->stream>>>(n, w.t1, w.apow[P.nlev - 1], w.red);
-  k_prep_check<<<1, 256, 0, ws->stream>>>(n, nblk, w.red, 1.0e-11, 1.0e-5, w.st);
-  *mout = mc;
+  k_prep_check<<<1, 256, 0, ws->stream>>>(n, nblk, w.red, 1.0e-11, kAcceptPower, w.st);
+  return PW_OK;
 }
 
 // outer iterations it0..it1 (1-based, src/paw_waves2.f90:1905-2010)
-void enqueue_iters(pw_waveset *ws, const OrthoWork &w, const OrthoParams &P, const double *mc,
-                   double *const lbuf[2], const double *d_chichi, const double *d_psipsi,
-                   const double *d_chipsi, int it0, int it1) {
-  const int n = ws->nb;
+int enqueue_iters(pw_waveset *ws, const OrthoWork &w, const OrthoParams &P, double *const lbuf[2],
+                  const double *d_chichi, const double *d_psipsi, const double *d_chipsi, int it0, int it1) {
+  const int n = ws->nb, nt = (n + T16 - 1) / T16;
   const double eps = 1.0e-8;
+  const int ls = P.nser;
   for (int it = it0; it <= it1; ++it) {
     const double *lam = lbuf[(it - 1) & 1];
     double *lnew = lbuf[it & 1];
-    // HAUX = CHICHI LAMBDA + 2 CHIPSI ; T = LAMBDA^T HAUX
+    // HAUX = CHICHI LAMBDA + 2 CHIPSI
     gemm(ws, d_chichi, 0, lam, 0, w.t1, 1.0, d_chipsi, 2.0, w.st);
     {
-      // T = LAMBDA^T HAUX, GAMN and the convergence test in one launch
+      // T = LAMBDA^T HAUX, GAMN and its block maxima in one launch
       PairArgs g{};
       g.n = n;
       g.A = lam;
@@ -606,16 +910,43 @@ void enqueue_iters(pw_waveset *ws, const OrthoWork &w, const OrthoParams &P, con
       g.out = w.gam;
       g.psipsi = d_psipsi;
       g.bmax = w.red;
-      g.ticket = w.ticket;
-      g.tol = eps;
-      g.it = it;
       gemm_pair<kPairGamma>(ws, g, w.st);
     }
-    // X = S^-1(GAMN): X0 = 2q M G M ; X <- X + A_j X A_j
-    gemm(ws, w.gam, 0, mc, 0, w.t1, 1.0, nullptr, 0.0, w.st);
-    gemm(ws, mc, 0, w.t1, 0, w.x0, 2.0 * P.q, nullptr, 0.0, w.st);
+    // X = S^-1(GAMN): the series over the first 2^ls terms, then Smith
+    // levels X <- X + A_j X A_j for j >= ls; the convergence test runs first
+    // in the series sum.  Without a series (ls = 0) the one term M G M is
+    // formed by two products.
+    if (ls > 0) {
+      SeriesArgs sa{};
+      sa.n = n;
+      sa.P = w.pstk;
+      sa.G = w.gam;
+      sa.part = w.spart;
+      series_panel(ws, sa, 1 << ls, w.st);
+    } else {
+      gemm(ws, w.gam, 0, w.pstk, 0, w.t1, 1.0, nullptr, 0.0, w.st);
+      gemm(ws, w.pstk, 0, w.t1, 0, w.spart, 1.0, nullptr, 0.0, w.st);
+    }
+    SumArgs su{};
+    su.n = n;
+    su.nterm = 1 << ls;
+    su.part = w.spart;
+    su.scale = 2.0 * P.q;
+    su.occ = w.occ;
+    su.lam = lam;
+    su.bmax = w.red;
+    su.nbmax = nt * nt;
+    su.tol = eps;
+    su.it = it;
+    if (ls == P.nlev) {
+      su.out = lnew;
+      series_sum<kSeriesLambda>(ws, su, w.st);
+      continue;
+    }
+    su.out = w.x0;
+    series_sum<kSeriesX>(ws, su, w.st);
     double *xc = w.x0, *xn = w.x1;
-    for (int j = 0; j < P.nlev; ++j) {
+    for (int j = ls; j < P.nlev; ++j) {
       gemm(ws, xc, 0, w.apow[j], 0, w.t1, 1.0, nullptr, 0.0, w.st);
       if (j + 1 < P.nlev) {
         gemm(ws, w.apow[j], 0, w.t1, 0, xn, 1.0, xc, 1.0, w.st);
@@ -636,6 +967,7 @@ void enqueue_iters(pw_waveset *ws, const OrthoWork &w, const OrthoParams &P, con
       }
     }
   }
+  return PW_OK;
 }
 
 }  // namespace
@@ -683,16 +1015,19 @@ int ortho_x_device(pw_waveset *ws, double *d_lam, const double *d_chichi, const 
     P.q = std::sqrt(P.lo * P.hi);
     const double kap = P.hi / P.lo;
     const double rho = (std::sqrt(kap) - 1.0) / (std::sqrt(kap) + 1.0);
-    // J Smith levels sum A^k X0 A^k for k < 2^J; the tail is bounded by
-    // rho^(2^(J+1)).  J is the least level count whose last power
-    // A^(2^(J-1)) (<= rho^(2^(J-1))) passes the acceptance test with a
-    // factor 10 to spare, n max|A^(2^(J-1))| <= 1e-6; the tail is then below
-    // (1e-6/n)^4, far under FP64 rounding.
+    // S^-1 sums A^k X0 A^k for k < K = 2^J.  The acceptance test certifies
+    // h = 2^(J-1) by n max|A^h| <= kAcceptPower = 1e-4; then rho^h <=
+    // ||A^h||_2 <= n max|A^h| and the tail, at most rho^(2K)/(1-rho^2) of X0,
+    // is below 1e-16 of it.  J is the least level count whose estimate from
+    // the (widened) Lanczos bounds, n rho^h, is 3x inside the test; a second
+    // attempt (bounds rejected once) adds a level.
     P.nlev = 1;
     if (rho > 1.0e-300) {
-      const double need = std::log(1.0e-6 / n) / std::log(rho);
+      const double need = std::log(kSelectPower / n) / std::log(rho);
       P.nlev = std::max(1, (int)std::ceil(std::log2(std::max(need, 1.0))) + 1);
     }
+    P.nlev += attempt;
+    P.nser = series_fits(n) ? std::min(P.nlev, kSeriesMaxLevels) : 0;
     if (P.nlev > kMaxLevels) return pw_fail(PW_ERR_NOCONV, "WAVES_ORTHO_X: CHIPSI too ill-conditioned");
     // Newton-Schulz for (B+qI)^-1: |I - (B+qI)X_k| = e0^(2^k); stop at 1e-17
     // (the acceptance test asks for 1e-11)
@@ -701,12 +1036,12 @@ int ortho_x_device(pw_waveset *ws, double *d_lam, const double *d_chichi, const 
     P.nns = 1;
     if (e0 > 1.0e-300) P.nns = std::max(1, (int)std::ceil(std::log2(std::max(1.0, std::log(1.0e-17) / std::log(e0)))));
     ws->ortho_levels = P.nlev;
+    ws->ortho_series = 1 << P.nser;
     ws->ortho_ns = P.nns;
     const int batch = std::max(2, std::min(ws->ortho_iter > 0 ? ws->ortho_iter : 8, kMaxIter));
     // the preparation and the first batch of iterations are one captured graph,
     // replayed while the bounds, the batch and the buffers stay the same
     const OrthoGraphKey key{P.lo, P.hi, P.nlev, P.nns, batch, d_lam, d_chichi, d_psipsi, d_chipsi, ws->d_ortho};
-    double *mc = (P.nns & 1) ? w.m1 : w.m0;
     if (!ws->ortho_exec || !(ws->ortho_key == key)) {
       ortho_graph_release(ws);
       // with other plans or wavesets alive (a request loop) the work is issued
@@ -715,13 +1050,14 @@ int ortho_x_device(pw_waveset *ws, double *d_lam, const double *d_chichi, const 
       std::unique_lock<std::mutex> lk(gate.mu);
       if (gate.plans <= 1 && gate.wavesets <= 1) {
         PW_HIP(hipStreamBeginCapture(ws->stream, hipStreamCaptureModeRelaxed));
-        enqueue_prep(ws, w, P, &mc);
-        enqueue_iters(ws, w, P, mc, lbuf, d_chichi, d_psipsi, d_chipsi, 1, batch);
+        int rq = enqueue_prep(ws, w, P);
+        if (rq == PW_OK) rq = enqueue_iters(ws, w, P, lbuf, d_chichi, d_psipsi, d_chipsi, 1, batch);
         hipGraph_t graph = nullptr;
         const hipError_t ec = hipStreamEndCapture(ws->stream, &graph);
         const hipError_t el = hipGetLastError();
-        if (ec != hipSuccess || el != hipSuccess || !graph) {
+        if (rq != PW_OK || ec != hipSuccess || el != hipSuccess || !graph) {
           if (graph) (void)hipGraphDestroy(graph);
+          if (rq != PW_OK) return rq;
           return pw_fail(PW_ERR_HIP, std::string("WAVES_ORTHO_X: graph capture failed: ") +
                                          hipGetErrorString(ec != hipSuccess ? ec : el));
         }
@@ -738,8 +1074,8 @@ int ortho_x_device(pw_waveset *ws, double *d_lam, const double *d_chichi, const 
     if (ws->ortho_exec) {
       PW_HIP(hipGraphLaunch((hipGraphExec_t)ws->ortho_exec, ws->stream));
     } else {
-      enqueue_prep(ws, w, P, &mc);
-      enqueue_iters(ws, w, P, mc, lbuf, d_chichi, d_psipsi, d_chipsi, 1, batch);
+      PW_TRY(enqueue_prep(ws, w, P));
+      PW_TRY(enqueue_iters(ws, w, P, lbuf, d_chichi, d_psipsi, d_chipsi, 1, batch));
       PW_HIP(hipGetLastError());
       ws->ortho_graph_misses++;
     }
@@ -750,7 +1086,7 @@ int ortho_x_device(pw_waveset *ws, double *d_lam, const double *d_chichi, const 
       PW_HIP(hipStreamSynchronize(ws->stream));
       if (hs.done || launched >= kMaxIter) break;
       const int nit = std::min(4, kMaxIter - launched);
-      enqueue_iters(ws, w, P, mc, lbuf, d_chichi, d_psipsi, d_chipsi, launched + 1, launched + nit);
+      PW_TRY(enqueue_iters(ws, w, P, lbuf, d_chichi, d_psipsi, d_chipsi, launched + 1, launched + nit));
       PW_HIP(hipGetLastError());
       launched += nit;
     }
