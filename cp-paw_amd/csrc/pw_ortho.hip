@@ -342,8 +342,8 @@ __global__ void __launch_bounds__(64 * kGemmWaves) k_gemm_pair(PairArgs a, Ortho
 //   S^-1(G) = 2q sum_{k<K} A^k M G M A^k = 2q sum_k P_k G P_k^T,  P_k = A^k M,
 // with the P_k formed once per call.  k_series_panel: workgroup (row panel
 // I, term k) forms R = P_k(I,:) G into LDS and then the row panel
-// part_k(I,:) = R P_k^T, so each term is a chain of two row-panel products
-// inside one workgroup.  k_series_sum adds the K terms of each element in
+// part_k(I,:) = R P_k (P_k is symmetric), so each term is a chain of two
+// row-panel products inside one workgroup.  k_series_sum adds the K terms of each element in
 // the order k = 0, 1, ..., and finishes with
 //   kSeriesX:      out = scale * sum            (Smith levels follow), or
 //   kSeriesLambda: the LAMBDA update of kPairLambda with X = scale * sum.
@@ -374,7 +374,7 @@ __global__ void __launch_bounds__(64 * kPanelWaves) k_series_panel(SeriesArgs a,
   const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
   // the B operand of the first product (first tile, first chunk) goes out
   // with the A panel; that of the second as soon as the first's MFMAs are
-  // issued
+  // issued (both at once exceed the register budget of 9 waves per CU)
   double b1[kKC];
   if (wv < nt) load_bchunk<0>(b1, n, a.G, wv * T16, 0);
   stage_panel(n, Pk + (size_t)I0 * n, n - I0, sA, ldr);
@@ -399,17 +399,18 @@ __global__ void __launch_bounds__(64 * kPanelWaves) k_series_panel(SeriesArgs a,
     for (int r = 0; r < 4; ++r) sR[(lk + 4 * r) * ldr + j] = acc[r];
   }
   double b2[kKC];
-  if (wv < nt) load_bchunk<1>(b2, n, Pk, wv * T16, 0);
+  if (wv < nt) load_bchunk<0>(b2, n, Pk, wv * T16, 0);
   __syncthreads();
-  // part_k(I,:) = R P_k^T
+  // part_k(I,:) = R P_k   (P_k = A^k M is symmetric: the row-major read
+  // is the coalesced one)
 #pragma unroll 1
   for (int jt = wv; jt < nt; jt += kPanelWaves) {
     dbl4 acc = {0.0, 0.0, 0.0, 0.0};
     if (jt == wv) {
       mfma_chunk(acc, b2, sR, ldr, 0);
-      panel_rest<1>(acc, n, sR, ldr, Pk, jt * T16, kKC);
+      panel_rest<0>(acc, n, sR, ldr, Pk, jt * T16, kKC);
     } else {
-      panel_rest<1>(acc, n, sR, ldr, Pk, jt * T16, 0);
+      panel_rest<0>(acc, n, sR, ldr, Pk, jt * T16, 0);
     }
     const int j = jt * T16 + li;
     if (j >= n) continue;
@@ -441,12 +442,26 @@ __global__ void __launch_bounds__(256) k_series_sum(SumArgs a, OrthoState *st) {
   if (stopped(st)) return;
   __shared__ double wm[4];
   __shared__ int wn[4];
+  const int n = a.n;
+  const size_t n2 = (size_t)n * n;
+  const int idx = blockIdx.x * 256 + threadIdx.x;
+  const bool in = idx < n * n;
+  const int i = in ? idx / n : 0, j = in ? idx - i * n : 0;
+  const bool swap = MODE == kSeriesLambda && in && a.occ[i] < a.occ[j];
+  // element (i,j), or (j,i) where the update takes the transposed element;
+  // the loads of the first group of terms go out with those of the block
+  // maxima, the adds run in the order k = 0, 1, ...
+  const double *src = a.part + (swap ? (size_t)j * n + i : (size_t)i * n + j);
+  constexpr int V = 16;
+  double v[V];
+#pragma unroll
+  for (int u = 0; u < V; ++u) v[u] = (in && u < a.nterm) ? src[(size_t)u * n2] : 0.0;
   double m = 0.0;
   int nan = 0;
   for (int b = threadIdx.x; b < a.nbmax; b += 256) {
-    const double v = a.bmax[b];
-    if (v != v) nan = 1;
-    else m = fmax(m, v);
+    const double x = a.bmax[b];
+    if (x != x) nan = 1;
+    else m = fmax(m, x);
   }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) {
@@ -460,6 +475,8 @@ __global__ void __launch_bounds__(256) k_series_sum(SumArgs a, OrthoState *st) {
   __syncthreads();
   m = fmax(fmax(wm[0], wm[1]), fmax(wm[2], wm[3]));
   nan = wn[0] | wn[1] | wn[2] | wn[3];
+  // the convergence test of iteration it (src/paw_waves2.f90:1960-1975):
+  // every workgroup reaches the same decision; the first one records it
   const int code = nan ? 1 : (m < a.tol ? 0 : (m > 1.0e200 ? 2 : -1));
   if (code >= 0) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -469,20 +486,12 @@ __global__ void __launch_bounds__(256) k_series_sum(SumArgs a, OrthoState *st) {
     }
     return;
   }
-  const int n = a.n;
-  const size_t n2 = (size_t)n * n;
-  const int idx = blockIdx.x * 256 + threadIdx.x;
-  if (idx >= n * n) return;
-  const int i = idx / n, j = idx - i * n;
-  const bool swap = MODE == kSeriesLambda && a.occ[i] < a.occ[j];
-  // element (i,j), or (j,i) where the update takes the transposed element;
-  // the loads of a group of terms are issued together, the adds in the
-  // order k = 0, 1, ...
-  const double *src = a.part + (swap ? (size_t)j * n + i : (size_t)i * n + j);
+  if (!in) return;
   double sum = 0.0;
-  constexpr int V = 16;
-  for (int k0 = 0; k0 < a.nterm; k0 += V) {
-    double v[V];
+#pragma unroll
+  for (int u = 0; u < V; ++u)
+    if (u < a.nterm) sum += v[u];
+  for (int k0 = V; k0 < a.nterm; k0 += V) {
 #pragma unroll
     for (int u = 0; u < V; ++u) v[u] = k0 + u < a.nterm ? src[(size_t)(k0 + u) * n2] : 0.0;
 #pragma unroll
@@ -496,8 +505,7 @@ __global__ void __launch_bounds__(256) k_series_sum(SumArgs a, OrthoState *st) {
   }
   const double oi = a.occ[i] + 1.0e-12, oj = a.occ[j] + 1.0e-12;
   if (swap) {
-    const double cji = c;
-    const double lt = a.lam[(size_t)j * n + i] - 2.0 * oj / (oj + oi) * cji;
+    const double lt = a.lam[(size_t)j * n + i] - 2.0 * oj / (oj + oi) * c;
     a.out[(size_t)i * n + j] = lt * oi / oj;
   } else {
     a.out[(size_t)i * n + j] = a.lam[(size_t)i * n + j] - 2.0 * oi / (oi + oj) * c;
