@@ -117,6 +117,9 @@ struct pw_plan {
   void *d_plane_buf = nullptr; size_t plane_bytes = 0;
   void *d_dens_part = nullptr; size_t dens_bytes = 0;
   void *d_vt = nullptr; size_t vt_bytes = 0;   // V(r) as [x][z][y] in the work precision
+  // POTENTIAL$VOFRHO on this (density) plane-wave object: setup tables and work
+  double *d_pot_tabs = nullptr; int pot_nsp = 0;
+  void *d_pot = nullptr; size_t pot_bytes = 0;
 };
 
 // stripe work buffer for nbatch bands (and the z-plane buffer of the full-grid

@@ -403,7 +403,7 @@ extern "C" int pw_plan_destroy(pw_plan *p) {
   void *ptrs[] = {p->d_minusg, p->d_igvec, p->d_stripe_off, p->d_stripe_g, p->d_stripe_x,
                   p->d_stripe_s, p->d_zlist, p->d_yz2stripe, p->d_z2iz, p->d_g2, p->d_gvec, p->d_stripe_buf,
                   p->d_plane_buf, p->d_dens_part, p->d_vt, p->d_stripe_pos, p->d_zpos, p->d_tw[0], p->d_tw[1], p->d_tw[2], p->d_twf[0],
-                  p->d_twf[1], p->d_twf[2]};
+                  p->d_twf[1], p->d_twf[2], p->d_pot_tabs, p->d_pot};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
   delete p;

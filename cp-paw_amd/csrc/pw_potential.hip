@@ -1,0 +1,365 @@
+// POTENTIAL$VOFRHO (src/paw_potential.f90:324-896) on the GPU, FP64: the
+// density -> local potential step that follows WAVES_DENSITY, for one spin,
+// the local-density functional of type 1 (local exchange + Perdew-Zunger,
+// src/paw_dft.f90:124-130), compensation charges with l = 0 only, no pair
+// potential of the compensation charges and no isolation.
+//
+// Pipeline on the density plane-wave object (sphere of the density cutoff):
+//   rho(G)   = RTOG rho(r) + sum_a PSCORG Y0 e_a          (SUPFFT, ADDCORE :1038-1066)
+//   Hartree  per G: compensation, VBAR, V0, background, EHARTREE partials
+//            (POTENTIAL_HARTREE :1069-1334)
+//   rho(r)   = GTOR rho(G);  exc(r), vxc(r) by LDA       (POTENTIAL_XC :1339-1545)
+//   vtot(G)  = RTOG vxc(r) + Hartree potential
+//   per atom: VQLM, Hartree/VBAR forces and the pseudo-core force
+//            (POTENTIAL_FPSCORE :897-975) in one G pass
+//   v(r)     = GTOR vtot(G)
+// e_a(G) = exp(-i G.R_a) is formed from the integer G components and the
+// per-axis phases b_k.R_a (PLANEWAVE$STRUCTUREFACTOR, src/paw_fft.f90:1597).
+// The sums are block partials in a fixed order, then one ordered reduction,
+// so results do not depend on scheduling.
+#include <algorithm>
+#include <cmath>
+
+#include "pw_common.h"
+
+namespace {
+
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kY0 = 0.28209479177387814347;   // 1/sqrt(4 pi)
+constexpr int kPotThreads = 256;
+
+// src/paw_dft.f90:62, 1691-1693, 2783-2792, 2814
+constexpr double kRhoTMin = 1.0e-6;
+constexpr double kGammaU = -142.3e-3, kBeta1U = 1052.9e-3, kBeta2U = 333.4e-3;
+constexpr double kAU = 31.10e-3, kBU = -48.0e-3, kCU = 2.0e-3, kDU = -11.6e-3;
+
+struct PotGeo {
+  int ngl, nrl, nat, nsp, ngamma;
+  const int *igvec;      // 3 x ngl
+  const double *gvec;    // 3 x ngl
+  const double *g2;      // ngl
+  const double *tabs;    // [4][nsp][ngl]: PSCORG, VBARG, G0, V0
+  const int *isp;        // nat, 0-based
+  const double *gr;      // nat x 3: b_k . R_a
+  const double *qlm;     // nat
+  double vol;
+};
+
+__device__ inline double2 eigr(const PotGeo &g, int a, int ig) {
+  const double ph = (double)g.igvec[3 * ig] * g.gr[3 * a] + (double)g.igvec[3 * ig + 1] * g.gr[3 * a + 1] +
+                    (double)g.igvec[3 * ig + 2] * g.gr[3 * a + 2];
+  double s, c;
+  sincos(ph, &s, &c);
+  return make_double2(c, -s);
+}
+
+__device__ inline double tab(const PotGeo &g, int which, int s, int ig) {
+  return g.tabs[((size_t)which * g.nsp + s) * g.ngl + ig];
+}
+
+// deterministic block sum of v over kPotThreads threads; result in thread 0
+__device__ inline double block_sum(double v, double *sh) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) sh[wv] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < kPotThreads / 64; ++w) r += sh[w];
+  __syncthreads();
+  return r;
+}
+
+__global__ void k_real_to_c(int n, const double *__restrict__ r, double2 *__restrict__ c) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) c[i] = make_double2(r[i], 0.0);
+}
+
+__global__ void k_c_to_real(int n, const double2 *__restrict__ c, double *__restrict__ r) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) r[i] = c[i].x;
+}
+
+// Per G: core, compensation, VBAR and V0 sums over atoms; the background and
+// the Hartree potential; EHARTREE partial sums (src/paw_potential.f90:
+// 1038-1066, 1145-1205).  rhog gains the core; vg = VG; vh = POT + VG.
+// scal[0] = RHOB (written by the thread of G = 0).
+__global__ void __launch_bounds__(kPotThreads) k_pot_g(PotGeo g, double2 *__restrict__ rhog,
+                                                       double2 *__restrict__ vg_out, double2 *__restrict__ vh,
+                                                       double *__restrict__ epart, double *__restrict__ scal) {
+  __shared__ double sh[kPotThreads / 64];
+  const int ig = blockIdx.x * kPotThreads + threadIdx.x;
+  double e = 0.0;
+  if (ig < g.ngl) {
+    double2 core = {0.0, 0.0}, comp = {0.0, 0.0}, pot = {0.0, 0.0};
+    double vb = 0.0;
+    for (int a = 0; a < g.nat; ++a) {
+      const int s = g.isp[a];
+      const double2 ea = eigr(g, a, ig);
+      const double cs = g.qlm[a] * kY0;
+      const double fc = tab(g, 0, s, ig) * kY0;
+      const double fh = cs * tab(g, 2, s, ig);
+      const double fp = cs * tab(g, 3, s, ig) + tab(g, 1, s, ig) * kY0;
+      core.x += fc * ea.x;
+      core.y += fc * ea.y;
+      comp.x += fh * ea.x;
+      comp.y += fh * ea.y;
+      pot.x += fp * ea.x;
+      pot.y += fp * ea.y;
+      if (ig == g.ngamma) vb += cs * tab(g, 3, s, ig);
+    }
+    double2 rg = rhog[ig];
+    rg.x += core.x;
+    rg.y += core.y;
+    rhog[ig] = rg;
+    double2 r2 = {rg.x + comp.x, rg.y + comp.y};
+    double2 v;
+    if (ig == g.ngamma) {
+      const double rhob = -r2.x;
+      v = make_double2(-vb, 0.0);
+      e = (rg.x * pot.x + rg.y * pot.y) + rhob * vb;
+      scal[0] = rhob;
+    } else {
+      const double f = 4.0 * kPi / g.g2[ig];
+      v = make_double2(r2.x * f, r2.y * f);
+      e = 0.5 * (r2.x * v.x + r2.y * v.y) + (rg.x * pot.x + rg.y * pot.y);
+    }
+    vg_out[ig] = v;
+    vh[ig] = make_double2(pot.x + v.x, pot.y + v.y);
+  }
+  const double s = block_sum(e, sh);
+  if (threadIdx.x == 0) epart[blockIdx.x] = s;
+}
+
+// LDA on the grid (POTENTIAL_XC, src/paw_potential.f90:1397-1401, and DFT,
+// src/paw_dft.f90:780-965, 2137-2195, 2824-2910): exc(r) partial sums, and
+// c(r) <- vxc(r) for the RTOG that follows
+__device__ inline void lda_point(double rho, double &exc, double &vxc) {
+  const double rhot = fmax(rho, 1.0e-8);
+  const double val = fmax(rhot, kRhoTMin);
+  const double kf = cbrt(3.0 * kPi * kPi);
+  const double exfac = -3.0 / (4.0 * kPi) * kf;
+  const double r13 = cbrt(val);
+  const double r43 = (r13 * r13) * (r13 * r13);
+  const double ex = exfac * r43;
+  const double vx = ex * (4.0 / 3.0) / val;
+  const double rs = cbrt(3.0 / (4.0 * kPi)) / r13;
+  double ec, vc;
+  if (rs < 1.0) {
+    const double l = log(rs);
+    ec = val * (kBU + kDU * rs + (kAU + kCU * rs) * l);
+    vc = (1.0 / 3.0) * (-kAU + 3.0 * kBU - (kCU - 2.0 * kDU) * rs + 3.0 * kAU * l + 2.0 * kCU * rs * l);
+  } else {
+    const double sq = sqrt(rs);
+    const double sv = 1.0 / (1.0 + sq * (kBeta1U + kBeta2U * sq));
+    const double sv1 = sv / (6.0 * val);
+    ec = kGammaU * val * sv;
+    vc = ec * sv1 * (6.0 + sq * (7.0 * kBeta1U + 8.0 * kBeta2U * sq));
+  }
+  exc = ex + ec;
+  vxc = vx + vc;
+  if (rhot < kRhoTMin) {
+    vxc = exc / kRhoTMin;
+    exc = exc * rhot / kRhoTMin;
+  }
+}
+
+__global__ void __launch_bounds__(kPotThreads) k_xc(int n, double2 *__restrict__ c, double *__restrict__ xpart) {
+  __shared__ double sh[kPotThreads / 64];
+  const int i = blockIdx.x * kPotThreads + threadIdx.x;
+  double e = 0.0;
+  if (i < n) {
+    double vx;
+    lda_point(c[i].x, e, vx);
+    c[i] = make_double2(vx, 0.0);
+  }
+  const double s = block_sum(e, sh);
+  if (threadIdx.x == 0) xpart[blockIdx.x] = s;
+}
+
+__global__ void k_add_c(int n, const double2 *__restrict__ a, double2 *__restrict__ b) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = make_double2(a[i].x + b[i].x, a[i].y + b[i].y);
+}
+
+// Per (atom, G block): VQLM and the Hartree/VBAR force terms
+// (src/paw_potential.f90:1263-1290) and the pseudo-core force
+// (POTENTIAL_FPSCORE, :897-975), as block partial sums
+// apart[(a * nblk + block) * 4 + {0: vqlm, 1..3: force}]
+__global__ void __launch_bounds__(kPotThreads) k_pot_atoms(PotGeo g, const double2 *__restrict__ rhog,
+                                                           const double2 *__restrict__ vg,
+                                                           const double2 *__restrict__ vtot,
+                                                           const double *__restrict__ scal,
+                                                           double *__restrict__ apart) {
+  __shared__ double sh[kPotThreads / 64];
+  const int ig = blockIdx.x * kPotThreads + threadIdx.x;
+  const int a = blockIdx.y;
+  double q = 0.0, f[3] = {0.0, 0.0, 0.0};
+  if (ig < g.ngl) {
+    const int s = g.isp[a];
+    const double2 ea = eigr(g, a, ig);
+    const double2 rg = rhog[ig], v = vg[ig], vt = vtot[ig];
+    const double2 r2b = {rg.x + (ig == g.ngamma ? scal[0] : 0.0), rg.y};
+    const double g0 = tab(g, 2, s, ig), v0 = tab(g, 3, s, ig);
+    // AVAL = (G0 conj(VG) + V0 conj(RHO2)) EIGR
+    const double2 b = {g0 * v.x + v0 * r2b.x, -(g0 * v.y + v0 * r2b.y)};
+    const double2 aval = {b.x * ea.x - b.y * ea.y, b.x * ea.y + b.y * ea.x};
+    q = kY0 * aval.x;
+    const double cs = g.qlm[a] * kY0;
+    // CVAL = VBARG Y0 EIGR conj(RHO)
+    const double vb = tab(g, 1, s, ig) * kY0;
+    const double cval_im = vb * (ea.y * rg.x - ea.x * rg.y);
+    const double sv = cval_im + cs * aval.y;
+    // FPSCORE: -Im(conj(VTOT) PSCORG EIGR) Y0
+    const double pc = tab(g, 0, s, ig);
+    const double sc = -pc * (vt.x * ea.y - vt.y * ea.x);
+    const double t = -sv + kY0 * sc;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) f[k] = g.gvec[3 * ig + k] * t;
+  }
+  const size_t base = ((size_t)a * gridDim.x + blockIdx.x) * 4;
+  double r = block_sum(q, sh);
+  if (threadIdx.x == 0) apart[base] = r;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    r = block_sum(f[k], sh);
+    if (threadIdx.x == 0) apart[base + 1 + k] = r;
+  }
+}
+
+// Ordered reduction: output o sums in[o * stride * n + k * stride] over
+// k < n (one workgroup per output, fixed assignment of terms to threads)
+__global__ void __launch_bounds__(kPotThreads) k_sum_out(int n, int stride, const double *__restrict__ in,
+                                                         double scale, double *__restrict__ out) {
+  __shared__ double sh[kPotThreads / 64];
+  const int o = blockIdx.x;
+  const int c = o % stride, row = o / stride;
+  double s = 0.0;
+  for (int k = threadIdx.x; k < n; k += kPotThreads) s += in[((size_t)row * n + k) * stride + c];
+  const double r = block_sum(s, sh);
+  if (threadIdx.x == 0) out[o] = scale * r;
+}
+
+int pot_launch(pw_plan *p, int nat, const int *isp, const double *pos, const double *qlm, const double *rho,
+               double *vofr, double *energies, double *vqlm, double *force) {
+  const int ngl = p->ngl, nrl = p->nrl, nsp = p->pot_nsp;
+  const hipStream_t st = p->stream;
+  const int nbg = (ngl + kPotThreads - 1) / kPotThreads;
+  const int nbr = (nrl + kPotThreads - 1) / kPotThreads;
+  // work: grid (complex), rhog, vg, vh, vtot (sphere), partials, per-atom data
+  const size_t need = (size_t)nrl * 16 + 4 * (size_t)ngl * 16 + ((size_t)nbg * (4 * nat + 1) + nbr + 8) * 8 +
+                      (size_t)nat * (4 * 8 + 4 * 8) + 4096;
+  if (need > p->pot_bytes) {
+    if (p->d_pot) PW_HIP(hipFree(p->d_pot));
+    p->d_pot = nullptr;
+    p->pot_bytes = 0;
+    PW_HIP(hipMalloc(&p->d_pot, need));
+    p->pot_bytes = need;
+  }
+  char *w = (char *)p->d_pot;
+  double2 *grid = (double2 *)w; w += (size_t)nrl * 16;
+  double2 *rhog = (double2 *)w; w += (size_t)ngl * 16;
+  double2 *vg = (double2 *)w; w += (size_t)ngl * 16;
+  double2 *vh = (double2 *)w; w += (size_t)ngl * 16;
+  double2 *vtot = (double2 *)w; w += (size_t)ngl * 16;
+  double *epart = (double *)w; w += (size_t)nbg * 8;
+  double *xpart = (double *)w; w += (size_t)nbr * 8;
+  double *apart = (double *)w; w += (size_t)nbg * 4 * nat * 8;
+  double *scal = (double *)w; w += 8 * 8;
+  double *res = (double *)w; w += (size_t)(4 * nat + 2) * 8;   // vqlm/force per atom, then EHARTREE, EXC
+  int *d_isp = (int *)w; w += (size_t)nat * 8;
+  double *d_gr = (double *)w; w += (size_t)nat * 3 * 8;
+  double *d_qlm = (double *)w;
+  // per-atom data: 0-based species, b_k . R_a, QLM
+  std::vector<int> hisp(nat);
+  std::vector<double> hgr(3 * (size_t)nat);
+  for (int a = 0; a < nat; ++a) {
+    if (isp[a] < 1 || isp[a] > nsp) return pw_fail(PW_ERR_ARG, "POTENTIAL$VOFRHO: species index out of range");
+    hisp[a] = isp[a] - 1;
+    for (int k = 0; k < 3; ++k)
+      hgr[3 * a + k] = p->gbas[0 + 3 * k] * pos[3 * a] + p->gbas[1 + 3 * k] * pos[3 * a + 1] +
+                       p->gbas[2 + 3 * k] * pos[3 * a + 2];
+  }
+  PW_HIP(hipMemcpyAsync(d_isp, hisp.data(), nat * sizeof(int), hipMemcpyHostToDevice, st));
+  PW_HIP(hipMemcpyAsync(d_gr, hgr.data(), 3 * nat * sizeof(double), hipMemcpyHostToDevice, st));
+  PW_HIP(hipMemcpyAsync(d_qlm, qlm, nat * sizeof(double), hipMemcpyHostToDevice, st));
+  PotGeo g{};
+  g.ngl = ngl;
+  g.nrl = nrl;
+  g.nat = nat;
+  g.nsp = nsp;
+  g.ngamma = p->ngamma;
+  g.igvec = p->d_igvec;
+  g.gvec = p->d_gvec;
+  g.g2 = p->d_g2;
+  g.tabs = p->d_pot_tabs;
+  g.isp = d_isp;
+  g.gr = d_gr;
+  g.qlm = d_qlm;
+  g.vol = p->vol;
+  // rho(G) of the valence density
+  k_real_to_c<<<nbr, kPotThreads, 0, st>>>(nrl, rho, grid);
+  PW_TRY(fft_rtog_full(p, 1, rhog, grid, PW_PREC_FP64, st));
+  // core, compensation, Hartree
+  k_pot_g<<<nbg, kPotThreads, 0, st>>>(g, rhog, vg, vh, epart, scal);
+  // density + core on the grid, LDA, back to G
+  PW_TRY(fft_gtor_full(p, 1, rhog, grid, PW_PREC_FP64, st));
+  k_xc<<<nbr, kPotThreads, 0, st>>>(nrl, grid, xpart);
+  PW_TRY(fft_rtog_full(p, 1, vtot, grid, PW_PREC_FP64, st));
+  k_add_c<<<nbg, kPotThreads, 0, st>>>(ngl, vh, vtot);
+  // per-atom sums
+  if (nat > 0) {
+    k_pot_atoms<<<dim3(nbg, nat), kPotThreads, 0, st>>>(g, rhog, vg, vtot, scal, apart);
+    k_sum_out<<<4 * nat, kPotThreads, 0, st>>>(nbg, 4, apart, p->vol, res);
+  }
+  k_sum_out<<<1, kPotThreads, 0, st>>>(nbg, 1, epart, p->vol, res + 4 * nat);
+  k_sum_out<<<1, kPotThreads, 0, st>>>(nbr, 1, xpart, p->vol / (double)nrl, res + 4 * nat + 1);
+  // v(r)
+  PW_TRY(fft_gtor_full(p, 1, vtot, grid, PW_PREC_FP64, st));
+  k_c_to_real<<<nbr, kPotThreads, 0, st>>>(nrl, grid, vofr);
+  PW_HIP(hipGetLastError());
+  std::vector<double> h(4 * (size_t)nat + 2);
+  double rhob = 0.0;
+  PW_HIP(hipMemcpyAsync(h.data(), res, h.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+  PW_HIP(hipMemcpyAsync(&rhob, scal, sizeof(double), hipMemcpyDeviceToHost, st));
+  PW_HIP(hipStreamSynchronize(st));
+  if (p->ngamma < 0) rhob = 0.0;
+  energies[0] = h[4 * nat];
+  energies[1] = h[4 * nat + 1];
+  energies[2] = rhob;
+  for (int a = 0; a < nat; ++a) {
+    if (vqlm) vqlm[a] = h[4 * a];
+    if (force)
+      for (int k = 0; k < 3; ++k) force[3 * a + k] = h[4 * a + 1 + k];
+  }
+  return PW_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" int pw_potential_tables(pw_plan *p, int nsp, const double *tabs) {
+  if (!p || nsp <= 0 || !tabs) return pw_fail(PW_ERR_ARG, "pw_potential_tables: bad argument");
+  PW_TRY(plan_device(p));
+  const size_t n = 4 * (size_t)nsp * p->ngl;
+  if (p->d_pot_tabs) PW_HIP(hipFree(p->d_pot_tabs));
+  p->d_pot_tabs = nullptr;
+  p->pot_nsp = 0;
+  PW_HIP(hipMalloc(&p->d_pot_tabs, n * sizeof(double)));
+  PW_HIP(hipMemcpy(p->d_pot_tabs, tabs, n * sizeof(double), hipMemcpyHostToDevice));
+  p->pot_nsp = nsp;
+  return PW_OK;
+}
+
+extern "C" int pw_vofrho(pw_plan *p, int nat, const int *isp, const double *pos, const double *qlm,
+                         const double *rho, double *vofr, double *energies, double *vqlm, double *force) {
+  if (!p || nat < 0 || (nat > 0 && (!isp || !pos || !qlm)) || !rho || !vofr || !energies)
+    return pw_fail(PW_ERR_ARG, "pw_vofrho: bad argument");
+  if (!p->d_pot_tabs || p->pot_nsp <= 0)
+    return pw_fail(PW_ERR_STATE, "POTENTIAL$VOFRHO: setup tables not set (pw_potential_tables)");
+  PW_TRY(plan_device(p));
+  return pot_launch(p, nat, isp, pos, qlm, rho, vofr, energies, vqlm, force);
+}

@@ -13,7 +13,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(os.path.dirname(HERE), "csrc")
 LIB = os.path.join(HERE, "libpwgpu.so")
 
-SOURCES = ["pw_plan.cpp", "pw_fft.hip", "pw_kernels.hip", "pw_ortho.hip", "pw_waves.cpp", "pw_serve.cpp"]
+SOURCES = ["pw_plan.cpp", "pw_fft.hip", "pw_kernels.hip", "pw_ortho.hip", "pw_waves.cpp", "pw_serve.cpp",
+           "pw_potential.hip"]
 
 FLAGS = ["-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC", "-shared",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -28,6 +28,13 @@ class Species:
     sigma: list                  # Gaussian width of each radial projector (bohr)
     do_ln: np.ndarray            # (LNX,LNX) one-centre overlap DO
     dh_ln: np.ndarray            # (LNX,LNX) one-centre Hamiltonian (spherical part)
+    # model setup data of the density -> potential step (Gaussian forms)
+    zval: float = 4.0            # valence charge
+    rcsm: float = 1.0            # width of the compensation Gaussian G0
+    rcore: float = 0.9           # width of the pseudo core density
+    qcore: float = 0.3           # charge of the pseudo core density
+    vbar: float = -0.4           # depth of V-bar (Hartree)
+    rvbar: float = 1.1           # width of V-bar
 
     @property
     def lnx(self):
@@ -153,6 +160,63 @@ class System:
         v = np.fft.ifftn(vg) * float(nr1 * nr2 * nr3)
         return np.ascontiguousarray(v.real)
 
+    # -- setup data of the density -> potential step (POTENTIAL$VOFRHO) --------
+    def potential_tables(self, g2):
+        """(4, NSP, NG) PSCORG, VBARG, G0, V0 on a sphere with |G|^2 = g2.
+
+        Gaussian model forms (the reference obtains them from its setup
+        construction, src/paw_setups.f90): with Y0 = 1/sqrt(4 pi),
+          PSCORG Y0 = qcore exp(-G^2 rcore^2/4)/Omega   (core charge qcore),
+          VBARG Y0  = vbar (pi rvbar^2)^1.5 exp(-G^2 rvbar^2/4)/Omega,
+          G0        = exp(-G^2 rcsm^2/4)/Omega           (unit compensation charge),
+          V0        = 4 pi/G^2 (exp(-G^2 r1^2/4) - exp(-G^2 rcsm^2/4))/Omega,
+                      r1 = 0.6 rcsm, with its limit pi (rcsm^2 - r1^2)/Omega at G = 0.
+        """
+        g2 = np.asarray(g2, dtype=np.float64)
+        y0 = 1.0 / math.sqrt(4.0 * PI)
+        om = self.vol
+        out = np.zeros((4, len(self.species), len(g2)))
+        nz = g2 > 1.0e-12
+        for k, sp in enumerate(self.species):
+            out[0, k] = sp.qcore * np.exp(-g2 * sp.rcore ** 2 / 4.0) / om / y0
+            out[1, k] = sp.vbar * (PI * sp.rvbar ** 2) ** 1.5 * np.exp(-g2 * sp.rvbar ** 2 / 4.0) / om / y0
+            out[2, k] = np.exp(-g2 * sp.rcsm ** 2 / 4.0) / om
+            r1 = 0.6 * sp.rcsm
+            v0 = np.full(len(g2), PI * (sp.rcsm ** 2 - r1 ** 2) / om)
+            v0[nz] = 4.0 * PI / g2[nz] * (np.exp(-g2[nz] * r1 ** 2 / 4.0) - np.exp(-g2[nz] * sp.rcsm ** 2 / 4.0)) / om
+            out[3, k] = v0
+        return out
+
+    def qlm(self):
+        """QLM(1,NAT): monopoles of the one-centre charges, -zval/Y0 plus a
+        seeded deviation of 0.05/Y0 (so that dE/dQLM is exercised)."""
+        rng = np.random.default_rng(self.seed + 23)
+        y0 = 1.0 / math.sqrt(4.0 * PI)
+        z = np.array([self.species[s].zval for s in self.isp], dtype=np.float64)
+        return (-z + rng.normal(0.0, 0.05, self.nat)) / y0
+
+    def valence_density(self, nr, width=1.2, background=2.0e-3):
+        """Model pseudo valence density on the (NR1,NR2,NR3) grid as
+        (NR3,NR2,NR1): periodic sum of zval-normalised Gaussians of the given
+        width on the sites plus a constant background (keeps it above the
+        functional's density floor)."""
+        nr1, nr2, nr3 = nr
+        gb = 2.0 * PI * np.linalg.inv(self.rbas).T
+        f1 = np.fft.fftfreq(nr1, 1.0 / nr1)
+        f2 = np.fft.fftfreq(nr2, 1.0 / nr2)
+        f3 = np.fft.fftfreq(nr3, 1.0 / nr3)
+        g3, g2_, g1 = np.meshgrid(f3, f2, f1, indexing="ij")
+        gsq = sum((g1 * gb[k, 0] + g2_ * gb[k, 1] + g3 * gb[k, 2]) ** 2 for k in range(3))
+        rg = np.zeros((nr3, nr2, nr1), dtype=np.complex128)
+        for p, s in zip(self.pos, self.isp):
+            ph1 = np.exp(-1j * f1 * (gb[:, 0] @ p))
+            ph2 = np.exp(-1j * f2 * (gb[:, 1] @ p))
+            ph3 = np.exp(-1j * f3 * (gb[:, 2] @ p))
+            rg += self.species[s].zval / self.vol * np.exp(-gsq * width * width / 4.0) * \
+                ph3[:, None, None] * ph2[None, :, None] * ph1[None, None, :]
+        rho = np.fft.ifftn(rg).real * float(nr1 * nr2 * nr3)
+        return np.ascontiguousarray(rho + background)
+
     def initial_psi(self, g2):
         """c(G) = (N(0,1)+iN(0,1))/(1+|G|^2)^2, seeded; (NBH, NG) complex128."""
         rng = np.random.default_rng(self.seed + 11)
@@ -255,14 +319,16 @@ def o_species():
                    [0, 0, 0.34, 0.06], [0, 0, 0.06, 0.18]], dtype=np.float64)
     dh = np.array([[-0.80, 0.12, 0, 0], [0.12, 0.30, 0, 0],
                    [0, 0, -0.45, 0.08], [0, 0, 0.08, 0.25]], dtype=np.float64)
-    return Species(lox=[0, 0, 1, 1], sigma=[0.55, 0.85, 0.6, 0.9], do_ln=do, dh_ln=dh)
+    return Species(lox=[0, 0, 1, 1], sigma=[0.55, 0.85, 0.6, 0.9], do_ln=do, dh_ln=dh,
+                   zval=6.0, rcsm=0.7, rcore=0.5, qcore=0.2, vbar=-0.8, rvbar=0.7)
 
 
 def h_species():
     """Model H site: 2 s Gaussian projectors (LNX=2, LMNX=2)."""
     do = np.array([[0.25, 0.05], [0.05, 0.12]], dtype=np.float64)
     dh = np.array([[-0.35, 0.06], [0.06, 0.18]], dtype=np.float64)
-    return Species(lox=[0, 0], sigma=[0.45, 0.75], do_ln=do, dh_ln=dh)
+    return Species(lox=[0, 0], sigma=[0.45, 0.75], do_ln=do, dh_ln=dh,
+                   zval=1.0, rcsm=0.5, rcore=0.4, qcore=0.0, vbar=-0.3, rvbar=0.5)
 
 
 def water(nmol=32, a=19.7, seed=9, epw_psi_ry=40.0, cdual=4.0, dt=4.0, emass=1000.0,

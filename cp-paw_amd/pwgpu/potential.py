@@ -1,0 +1,57 @@
+"""Host mirror of the density -> potential step POTENTIAL$VOFRHO
+(src/paw_potential.f90:324-896) over the C ABI (pw_potential_tables,
+pw_vofrho): one spin, the local-density functional of type 1, l = 0
+compensation charges, no pair potential, no isolation.
+
+Potential(dplane, tables) stands for the POTENTIAL module data of one density
+plane-wave object (PLANEWAVE$SELECT('DENSITY')); .vofrho(...) for
+POTENTIAL$VOFRHO.  Errors surface as PwError.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, dptr, iptr
+from .planewave import DeviceArray
+
+
+class Potential:
+    """dplane: PlaneWave of the density cutoff; tables: (4, NSP, NGL) real
+    PSCORG, VBARG, G0, V0 on its sphere."""
+
+    def __init__(self, dplane, tables):
+        self.lib = _lib.load()
+        self.plane = dplane
+        t = np.ascontiguousarray(np.asarray(tables, dtype=np.float64))
+        if t.ndim != 3 or t.shape[0] != 4 or t.shape[2] != dplane.ngl:
+            raise ValueError("tables must be (4, NSP, NGL) on the density sphere")
+        self.nsp = t.shape[1]
+        check(self.lib.pw_potential_tables(dplane.handle, self.nsp, dptr(t)))
+
+    def vofrho_device(self, isp, pos, qlm, rho_dev, vofr_dev):
+        """POTENTIAL$VOFRHO on device arrays (NRL doubles); isp 0-based.
+        Returns (energies{ehartree, exc, rhob}, vqlm(NAT), force(NAT,3))."""
+        isp1 = np.ascontiguousarray(np.asarray(isp, dtype=np.int32) + 1)
+        pos = np.ascontiguousarray(np.asarray(pos, dtype=np.float64))
+        qlm = np.ascontiguousarray(np.asarray(qlm, dtype=np.float64))
+        nat = len(isp1)
+        en = np.zeros(3)
+        vq = np.zeros(max(nat, 1))
+        fo = np.zeros((max(nat, 1), 3))
+        check(self.lib.pw_vofrho(self.plane.handle, nat, iptr(isp1), dptr(pos), dptr(qlm), rho_dev.ptr,
+                                 vofr_dev.ptr, dptr(en), dptr(vq), dptr(fo)))
+        return {"ehartree": en[0], "exc": en[1], "rhob": en[2]}, vq[:nat], fo[:nat]
+
+    def vofrho(self, isp, pos, qlm, rho_r):
+        """Convenience on host arrays: rho_r (NR3, NR2, NR1) real.  Returns the
+        same dict as the oracle: v, ehartree, exc, rhob, vqlm, force."""
+        nr1, nr2, nr3 = self.plane.nr
+        r = DeviceArray.from_host(np.ascontiguousarray(np.asarray(rho_r, dtype=np.float64)).reshape(-1))
+        v = DeviceArray((self.plane.nrl,), np.float64)
+        en, vq, fo = self.vofrho_device(isp, pos, qlm, r, v)
+        out = dict(en)
+        out["v"] = v.to_host().reshape(nr3, nr2, nr1)
+        out["vqlm"] = vq
+        out["force"] = fo
+        return out

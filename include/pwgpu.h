@@ -172,6 +172,29 @@ int pw_ortho_x(int nb, const double *occ, const double *chichi,
                const double *psipsi, const double *chipsi, double *lambda,
                int *iter);
 
+/* ---- density -> potential (POTENTIAL$VOFRHO) --------------------------------
+ * Replaces the core of POTENTIAL$VOFRHO (src/paw_potential.f90:324-896:
+ * SUPFFT of the density, POTENTIAL_ADDCORE :1038, POTENTIAL_HARTREE :1069,
+ * POTENTIAL_XC :1339, POTENTIAL_FPSCORE :897) for one spin, the local-density
+ * functional of type 1 (local exchange + Perdew-Zunger, src/paw_dft.f90:
+ * 124-130), compensation charges with l = 0 (LMRXX = 1), no pair potential
+ * of the compensation charges and no isolation.  The plan is the density
+ * plane-wave object (PLANEWAVE$SELECT('DENSITY')): its sphere is that of the
+ * density cutoff.  FP64.
+ * pw_potential_tables sets the setup tables the reference keeps as module
+ * data, on the sphere and resident on the device until replaced:
+ *   tabs[4][nsp][NGL] = PSCORG, VBARG, G0, V0  (each G fastest).
+ * pw_vofrho: rho_dev (NRL, x fastest) is the pseudo valence density;
+ * vofr_dev (NRL) receives the total local potential dE/drho (may alias
+ * rho_dev).  isp(NAT) 1-based species, pos R(3,NAT), qlm QLM(1,NAT).
+ * Host outputs: energies = {EHARTREE, EXC, RHOB}, vqlm(NAT) = dE/dQLM(1,:),
+ * force(3,NAT) = the Hartree, V-bar and pseudo-core forces (each may be
+ * NULL except energies). */
+int pw_potential_tables(pw_plan *dplan, int nsp, const double *tabs);
+int pw_vofrho(pw_plan *dplan, int nat, const int *isp, const double *pos,
+              const double *qlm, const double *rho_dev, double *vofr_dev,
+              double *energies, double *vqlm, double *force);
+
 /* ---- launch-shape autotuner and device footprint ---------------------------
  * No counterpart in the reference (FFTW plans with FFTW_MEASURE,
  * src/paw_library.f90:2688-2707, are its nearest analogue).

@@ -81,3 +81,20 @@ def test_bench_rejects_multi_gpu():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
                        capture_output=True, text=True)
     assert r.returncode == 2 and "one GPU" in r.stdout
+
+
+def test_vofrho_requires_tables():
+    # POTENTIAL$VOFRHO before its setup tables: a status, no GPU work
+    lib = _lib.load()
+    p = pwgpu.PlaneWave(np.diag([10.0] * 3), 5.0, 5.0)
+    isp = np.array([1], dtype=np.int32)
+    pos = np.zeros(3)
+    qlm = np.zeros(1)
+    en = np.zeros(3)
+    fake = ctypes.c_void_p(256)
+    rc = lib.pw_vofrho(p.handle, 1, isp.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                       pos.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                       qlm.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), fake, fake,
+                       en.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), None, None)
+    assert rc == 4
+    assert "tables" in _lib.last_error()

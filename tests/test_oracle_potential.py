@@ -1,0 +1,147 @@
+"""Pins the oracle of the density -> potential step (oracle/potential.py,
+POTENTIAL$VOFRHO, src/paw_potential.f90:324-896) without the reference:
+
+* local exchange: the closed form of the homogeneous electron gas
+  e_x = -(3/4)(3/pi)^(1/3) rho^(4/3);
+* Perdew-Zunger correlation: published correlation energies per electron of
+  the homogeneous gas (Ceperley-Alder as fitted by Perdew and Zunger, Phys.
+  Rev. B 23, 5048 (1981), Table XII), continuity of both branches at rs = 1,
+  and v_c = d(rho e_c)/drho by finite differences in both branches;
+* Hartree: the closed-form energy of a single cosine density;
+* the returned potential, forces and VQLM are the derivatives of the returned
+  energy EHARTREE + EXC with respect to the density, the positions and QLM
+  (central finite differences), with two species, the pseudo core, V-bar,
+  the compensation charges and the background all active.
+A dropped term, a wrong sign or factor in any of these fails one of the
+finite-difference checks.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import planewave as opw
+from oracle import potential as opot
+from pwgpu import model
+
+Y0 = 1.0 / math.sqrt(4.0 * math.pi)
+
+
+def tabs_dict(t):
+    return {"pscorg": t[0], "vbarg": t[1], "g0": t[2], "v0": t[3]}
+
+
+def test_exchange_closed_form():
+    rho = np.array([1e-4, 3e-3, 0.05, 0.7, 4.0])
+    ex, vx = opot.exchange_lda(rho)
+    cx = -0.75 * (3.0 / math.pi) ** (1.0 / 3.0)
+    assert np.allclose(ex, cx * rho ** (4.0 / 3.0), rtol=1e-14)
+    h = 1e-6 * rho
+    fd = (opot.exchange_lda(rho + h)[0] - opot.exchange_lda(rho - h)[0]) / (2 * h)
+    assert np.allclose(vx, fd, rtol=1e-8)
+
+
+@pytest.mark.parametrize("rs,ec", [(1.0, -0.0600), (2.0, -0.0448), (5.0, -0.0282), (10.0, -0.0186)])
+def test_pz_correlation_published_values(rs, ec):
+    rho = 3.0 / (4.0 * math.pi * rs ** 3)
+    e, _ = opot.pz_unpolarised(np.array([rho]))
+    assert abs(e[0] / rho - ec) < 4e-4, (rs, e[0] / rho)
+
+
+def test_pz_branches_continuous_and_derivative():
+    rs1 = 3.0 / (4.0 * math.pi)     # rho at rs = 1
+    lo = np.array([rs1 * (1 + 1e-9)])
+    hi = np.array([rs1 * (1 - 1e-9)])
+    elo = opot.pz_unpolarised(lo)[0][0] / lo[0]
+    ehi = opot.pz_unpolarised(hi)[0][0] / hi[0]
+    assert abs(elo - ehi) < 1e-4
+    for rs in (0.3, 0.8, 1.7, 6.0):
+        rho = np.array([3.0 / (4.0 * math.pi * rs ** 3)])
+        h = 1e-6 * rho
+        fd = (opot.pz_unpolarised(rho + h)[0] - opot.pz_unpolarised(rho - h)[0]) / (2 * h)
+        assert abs(opot.pz_unpolarised(rho)[1][0] - fd[0]) < 1e-8 * abs(fd[0]), rs
+
+
+def test_lda_floor():
+    # below RHOTMIN the energy density scales linearly and the potential is
+    # e(RHOTMIN)/RHOTMIN (src/paw_dft.f90:780-965)
+    exc, vxc = opot.lda(np.array([1e-7, 5e-7]))
+    e0, _ = opot.lda(np.array([opot.RHOTMIN]))
+    assert np.allclose(exc, e0[0] * np.array([1e-7, 5e-7]) / opot.RHOTMIN, rtol=1e-12)
+    assert np.allclose(vxc, e0[0] / opot.RHOTMIN, rtol=1e-12)
+
+
+def test_hartree_single_cosine():
+    # rho(r) = rho0 + 2A cos(G.r): E_H = Omega 4 pi A^2 / |G|^2, RHOB = -rho0
+    s = model.si8()
+    pl = opw.Plane(s.rbas, s.epw_rho, s.epw_rho)
+    n = np.array([1, 2, 0])
+    g = pl.gbas @ n
+    i3, i2, i1 = np.meshgrid(np.arange(pl.nr3), np.arange(pl.nr2), np.arange(pl.nr1), indexing="ij")
+    phase = 2 * np.pi * (n[0] * i1 / pl.nr1 + n[1] * i2 / pl.nr2 + n[2] * i3 / pl.nr3)
+    rho0, amp = 0.02, 0.003
+    rho = rho0 + 2 * amp * np.cos(phase)
+    t = np.zeros((4, 1, pl.ngl))
+    r = opot.vofrho(pl, rho, np.zeros((0, 3)), np.zeros(0, dtype=int), np.zeros(0), tabs_dict(t))
+    assert abs(r["ehartree"] - pl.vol * 4 * np.pi * amp ** 2 / (g @ g)) < 1e-12 * r["ehartree"]
+    assert abs(r["rhob"] + rho0) < 1e-15
+
+
+def system_case():
+    s = model.water(nmol=2, a=9.0, seed=4, epw_psi_ry=10.0, cdual=4.0)
+    pl = opw.Plane(s.rbas, s.epw_rho, s.epw_rho)
+    rho = s.valence_density((pl.nr1, pl.nr2, pl.nr3))
+    return s, pl, rho, tabs_dict(s.potential_tables(pl.g2))
+
+
+def energy(pl, rho, pos, isp, qlm, tabs):
+    r = opot.vofrho(pl, rho, pos, isp, qlm, tabs)
+    return r["ehartree"] + r["exc"], r
+
+
+def test_potential_is_density_derivative():
+    s, pl, rho, tabs = system_case()
+    e0, r = energy(pl, rho, s.pos, s.isp, s.qlm(), tabs)
+    rng = np.random.default_rng(3)
+    # a smooth direction inside the density sphere
+    c = (rng.normal(size=pl.ngl) + 1j * rng.normal(size=pl.ngl)) * np.exp(-pl.g2)
+    c = 0.5 * (c + np.conj(c[pl.minusg]))
+    d = pl.fft_gtor(c).real
+    d *= 1e-3 / np.max(np.abs(d))
+    eps = 1e-3
+    ep, _ = energy(pl, rho + eps * d, s.pos, s.isp, s.qlm(), tabs)
+    em, _ = energy(pl, rho - eps * d, s.pos, s.isp, s.qlm(), tabs)
+    fd = (ep - em) / (2 * eps)
+    an = float(np.sum(r["v"] * d)) * pl.rweight
+    assert abs(fd - an) < 1e-7 * abs(an), (fd, an)
+
+
+def test_forces_are_position_derivatives():
+    s, pl, rho, tabs = system_case()
+    qlm = s.qlm()
+    _, r = energy(pl, rho, s.pos, s.isp, qlm, tabs)
+    h = 1e-4
+    for a in (0, 1, 4):
+        for k in range(3):
+            p = s.pos.copy()
+            p[a, k] += h
+            ep, _ = energy(pl, rho, p, s.isp, qlm, tabs)
+            p[a, k] -= 2 * h
+            em, _ = energy(pl, rho, p, s.isp, qlm, tabs)
+            fd = -(ep - em) / (2 * h)
+            assert abs(fd - r["force"][a, k]) < 1e-6 * max(1e-3, np.max(np.abs(r["force"]))), (a, k)
+
+
+def test_vqlm_is_qlm_derivative():
+    s, pl, rho, tabs = system_case()
+    qlm = s.qlm()
+    _, r = energy(pl, rho, s.pos, s.isp, qlm, tabs)
+    h = 1e-4
+    for a in range(s.nat):
+        q = qlm.copy()
+        q[a] += h
+        ep, _ = energy(pl, rho, s.pos, s.isp, q, tabs)
+        q[a] -= 2 * h
+        em, _ = energy(pl, rho, s.pos, s.isp, q, tabs)
+        fd = (ep - em) / (2 * h)
+        assert abs(fd - r["vqlm"][a]) < 1e-7 * max(1.0, abs(fd)), a
