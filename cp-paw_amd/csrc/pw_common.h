@@ -120,7 +120,14 @@ struct pw_plan {
   // POTENTIAL$VOFRHO on this (density) plane-wave object: setup tables and work
   double *d_pot_tabs = nullptr; int pot_nsp = 0;
   void *d_pot = nullptr; size_t pot_bytes = 0;
+  double *h_pot = nullptr; size_t h_pot_bytes = 0; int pot_nat = 0;   // pinned: results and staged atom data
 };
+
+// POTENTIAL$VOFRHO on a density plan (pw_potential.hip): enqueue on stream s
+// (isp 1-based, pos R(3,NAT)); results are valid after the stream passes.
+int vofrho_enqueue(pw_plan *dp, int nat, const int *isp, const double *pos, const double *qlm,
+                   const double *rho_dev, double *vofr_dev, hipStream_t s);
+void vofrho_results(const pw_plan *dp, int nat, double *energies, double *vqlm, double *force);
 
 // stripe work buffer for nbatch bands (and the z-plane buffer of the full-grid
 // transforms when planes is set)
@@ -175,6 +182,12 @@ struct pw_waveset {
   std::vector<double> do_lmn;  // per species lmnxx*lmnxx (row-major [i*lmnxx+j])
   std::vector<double> dh;      // per atom lmnxx*lmnxx complex interleaved
   std::vector<double> r0, rp, rm;
+  // density -> potential step linked into ETOT (pw_waves_set_potential)
+  pw_plan *dplan = nullptr;
+  std::vector<int> isp1;             // NAT, 1-based
+  std::vector<double> qlm;           // QLM(1,NAT)
+  double ehartree = 0.0, exc = 0.0, rhob = 0.0;
+  std::vector<double> vqlm, fpot;    // dE/dQLM (NAT), potential forces (3 NAT)
   // lambda (real, Gamma point)
   std::vector<double> rlam0, rlamm;
   bool have_rlamm = false;

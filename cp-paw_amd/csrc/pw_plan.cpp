@@ -406,6 +406,7 @@ extern "C" int pw_plan_destroy(pw_plan *p) {
                   p->d_twf[1], p->d_twf[2], p->d_pot_tabs, p->d_pot};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
+  if (p->h_pot) (void)hipHostFree(p->h_pot);
   delete p;
   std::lock_guard<std::mutex> lk(capture_gate().mu);
   capture_gate().plans--;

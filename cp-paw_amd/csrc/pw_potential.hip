@@ -178,6 +178,14 @@ __global__ void __launch_bounds__(kPotThreads) k_xc(int n, double2 *__restrict__
   if (threadIdx.x == 0) xpart[blockIdx.x] = s;
 }
 
+__global__ void k_zero_d(int n, double *x) {
+  if ((int)threadIdx.x < n) x[threadIdx.x] = 0.0;
+}
+
+__global__ void k_copy1(const double *a, double *b) {
+  if (threadIdx.x == 0) b[0] = a[0];
+}
+
 __global__ void k_add_c(int n, const double2 *__restrict__ a, double2 *__restrict__ b) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) b[i] = make_double2(a[i].x + b[i].x, a[i].y + b[i].y);
@@ -241,10 +249,16 @@ __global__ void __launch_bounds__(kPotThreads) k_sum_out(int n, int stride, cons
   if (threadIdx.x == 0) out[o] = scale * r;
 }
 
-int pot_launch(pw_plan *p, int nat, const int *isp, const double *pos, const double *qlm, const double *rho,
-               double *vofr, double *energies, double *vqlm, double *force) {
+}  // namespace
+
+// Issues the whole step on stream st; the results land in pinned host memory
+// (p->h_pot) once the stream has passed this point (vofrho_results reads them).
+int vofrho_enqueue(pw_plan *p, int nat, const int *isp, const double *pos, const double *qlm, const double *rho,
+                   double *vofr, hipStream_t st) {
   const int ngl = p->ngl, nrl = p->nrl, nsp = p->pot_nsp;
-  const hipStream_t st = p->stream;
+  if (!p->d_pot_tabs || nsp <= 0)
+    return pw_fail(PW_ERR_STATE, "POTENTIAL$VOFRHO: setup tables not set (pw_potential_tables)");
+  PW_TRY(plan_device(p));
   const int nbg = (ngl + kPotThreads - 1) / kPotThreads;
   const int nbr = (nrl + kPotThreads - 1) / kPotThreads;
   // work: grid (complex), rhog, vg, vh, vtot (sphere), partials, per-atom data
@@ -257,6 +271,14 @@ int pot_launch(pw_plan *p, int nat, const int *isp, const double *pos, const dou
     PW_HIP(hipMalloc(&p->d_pot, need));
     p->pot_bytes = need;
   }
+  const size_t hneed = (4 * (size_t)nat + 3 + 6 * (size_t)nat) * sizeof(double);
+  if (hneed > p->h_pot_bytes) {
+    if (p->h_pot) PW_HIP(hipHostFree(p->h_pot));
+    p->h_pot = nullptr;
+    p->h_pot_bytes = 0;
+    PW_HIP(hipHostMalloc((void **)&p->h_pot, hneed, hipHostMallocDefault));
+    p->h_pot_bytes = hneed;
+  }
   char *w = (char *)p->d_pot;
   double2 *grid = (double2 *)w; w += (size_t)nrl * 16;
   double2 *rhog = (double2 *)w; w += (size_t)ngl * 16;
@@ -267,23 +289,27 @@ int pot_launch(pw_plan *p, int nat, const int *isp, const double *pos, const dou
   double *xpart = (double *)w; w += (size_t)nbr * 8;
   double *apart = (double *)w; w += (size_t)nbg * 4 * nat * 8;
   double *scal = (double *)w; w += 8 * 8;
-  double *res = (double *)w; w += (size_t)(4 * nat + 2) * 8;   // vqlm/force per atom, then EHARTREE, EXC
+  double *res = (double *)w; w += (size_t)(4 * nat + 3) * 8;   // vqlm/force per atom, EHARTREE, EXC, RHOB
   int *d_isp = (int *)w; w += (size_t)nat * 8;
   double *d_gr = (double *)w; w += (size_t)nat * 3 * 8;
   double *d_qlm = (double *)w;
-  // per-atom data: 0-based species, b_k . R_a, QLM
-  std::vector<int> hisp(nat);
-  std::vector<double> hgr(3 * (size_t)nat);
+  // per-atom data: 0-based species, b_k . R_a, QLM (staged in pinned memory)
+  int *hisp = (int *)(p->h_pot + 4 * (size_t)nat + 3);
+  double *hgr = p->h_pot + 5 * (size_t)nat + 3;
+  double *hq = p->h_pot + 8 * (size_t)nat + 3;
   for (int a = 0; a < nat; ++a) {
     if (isp[a] < 1 || isp[a] > nsp) return pw_fail(PW_ERR_ARG, "POTENTIAL$VOFRHO: species index out of range");
     hisp[a] = isp[a] - 1;
     for (int k = 0; k < 3; ++k)
       hgr[3 * a + k] = p->gbas[0 + 3 * k] * pos[3 * a] + p->gbas[1 + 3 * k] * pos[3 * a + 1] +
                        p->gbas[2 + 3 * k] * pos[3 * a + 2];
+    hq[a] = qlm[a];
   }
-  PW_HIP(hipMemcpyAsync(d_isp, hisp.data(), nat * sizeof(int), hipMemcpyHostToDevice, st));
-  PW_HIP(hipMemcpyAsync(d_gr, hgr.data(), 3 * nat * sizeof(double), hipMemcpyHostToDevice, st));
-  PW_HIP(hipMemcpyAsync(d_qlm, qlm, nat * sizeof(double), hipMemcpyHostToDevice, st));
+  if (nat > 0) {
+    PW_HIP(hipMemcpyAsync(d_isp, hisp, nat * sizeof(int), hipMemcpyHostToDevice, st));
+    PW_HIP(hipMemcpyAsync(d_gr, hgr, 3 * nat * sizeof(double), hipMemcpyHostToDevice, st));
+    PW_HIP(hipMemcpyAsync(d_qlm, hq, nat * sizeof(double), hipMemcpyHostToDevice, st));
+  }
   PotGeo g{};
   g.ngl = ngl;
   g.nrl = nrl;
@@ -302,6 +328,7 @@ int pot_launch(pw_plan *p, int nat, const int *isp, const double *pos, const dou
   k_real_to_c<<<nbr, kPotThreads, 0, st>>>(nrl, rho, grid);
   PW_TRY(fft_rtog_full(p, 1, rhog, grid, PW_PREC_FP64, st));
   // core, compensation, Hartree
+  k_zero_d<<<1, 64, 0, st>>>(8, scal);
   k_pot_g<<<nbg, kPotThreads, 0, st>>>(g, rhog, vg, vh, epart, scal);
   // density + core on the grid, LDA, back to G
   PW_TRY(fft_gtor_full(p, 1, rhog, grid, PW_PREC_FP64, st));
@@ -315,28 +342,28 @@ int pot_launch(pw_plan *p, int nat, const int *isp, const double *pos, const dou
   }
   k_sum_out<<<1, kPotThreads, 0, st>>>(nbg, 1, epart, p->vol, res + 4 * nat);
   k_sum_out<<<1, kPotThreads, 0, st>>>(nbr, 1, xpart, p->vol / (double)nrl, res + 4 * nat + 1);
+  k_copy1<<<1, 64, 0, st>>>(scal, res + 4 * nat + 2);
   // v(r)
   PW_TRY(fft_gtor_full(p, 1, vtot, grid, PW_PREC_FP64, st));
   k_c_to_real<<<nbr, kPotThreads, 0, st>>>(nrl, grid, vofr);
   PW_HIP(hipGetLastError());
-  std::vector<double> h(4 * (size_t)nat + 2);
-  double rhob = 0.0;
-  PW_HIP(hipMemcpyAsync(h.data(), res, h.size() * sizeof(double), hipMemcpyDeviceToHost, st));
-  PW_HIP(hipMemcpyAsync(&rhob, scal, sizeof(double), hipMemcpyDeviceToHost, st));
-  PW_HIP(hipStreamSynchronize(st));
-  if (p->ngamma < 0) rhob = 0.0;
+  PW_HIP(hipMemcpyAsync(p->h_pot, res, (4 * (size_t)nat + 3) * sizeof(double), hipMemcpyDeviceToHost, st));
+  p->pot_nat = nat;
+  return PW_OK;
+}
+
+void vofrho_results(const pw_plan *p, int nat, double *energies, double *vqlm, double *force) {
+  const double *h = p->h_pot;
   energies[0] = h[4 * nat];
   energies[1] = h[4 * nat + 1];
-  energies[2] = rhob;
+  energies[2] = p->ngamma >= 0 ? h[4 * nat + 2] : 0.0;
   for (int a = 0; a < nat; ++a) {
     if (vqlm) vqlm[a] = h[4 * a];
     if (force)
       for (int k = 0; k < 3; ++k) force[3 * a + k] = h[4 * a + 1 + k];
   }
-  return PW_OK;
 }
 
-}  // namespace
 
 // ---------------------------------------------------------------------------
 // C ABI
@@ -358,8 +385,8 @@ extern "C" int pw_vofrho(pw_plan *p, int nat, const int *isp, const double *pos,
                          const double *rho, double *vofr, double *energies, double *vqlm, double *force) {
   if (!p || nat < 0 || (nat > 0 && (!isp || !pos || !qlm)) || !rho || !vofr || !energies)
     return pw_fail(PW_ERR_ARG, "pw_vofrho: bad argument");
-  if (!p->d_pot_tabs || p->pot_nsp <= 0)
-    return pw_fail(PW_ERR_STATE, "POTENTIAL$VOFRHO: setup tables not set (pw_potential_tables)");
-  PW_TRY(plan_device(p));
-  return pot_launch(p, nat, isp, pos, qlm, rho, vofr, energies, vqlm, force);
+  PW_TRY(vofrho_enqueue(p, nat, isp, pos, qlm, rho, vofr, p->stream));
+  PW_HIP(hipStreamSynchronize(p->stream));
+  vofrho_results(p, nat, energies, vqlm, force);
+  return PW_OK;
 }

@@ -443,6 +443,9 @@ extern "C" int pw_waves_setr8a(pw_waveset *ws, const char *id, int len, const do
     PW_TRY(chk((size_t)3 * ws->nat));
     std::vector<double> r(val, val + len);
     if (s == "R0") ws->r0 = r; else if (s == "RP") ws->rp = r; else ws->rm = r;
+  } else if (s == "QLM") {
+    PW_TRY(chk((size_t)ws->nat));
+    ws->qlm.assign(val, val + len);
   } else if (s == "ORTHOBOUNDS") {
     // spectral bounds of sym(CHIPSI) for the next ORTHO_X call; ORTHO_X checks
     // them on the device and recomputes them when they are not accepted
@@ -554,6 +557,10 @@ extern "C" int pw_waves_getr8a(pw_waveset *ws, const char *id, int len, double *
     const std::vector<double> &m = s == "MAT" ? ws->mat : (s == "OMAT" ? ws->omat : ws->oomat);
     if (m.size() != (size_t)len) return pw_fail(PW_ERR_STATE, "WAVES: no orthogonalisation yet");
     to_colmajor(m, ws->nb, val);
+  } else if (s == "VQLM") {
+    PW_TRY(chk((size_t)ws->nat));
+    if (!ws->dplan) return pw_fail(PW_ERR_STATE, "WAVES$GETR8A: VQLM needs the density -> potential step");
+    std::memcpy(val, ws->vqlm.data(), (size_t)len * sizeof(double));
   } else if (s == "DENMAT") {
     PW_TRY(chk(ws->denmat.size()));
     // row-major per atom -> DENMAT(LMNXX,LMNXX,NAT) column-major
@@ -577,6 +584,9 @@ extern "C" int pw_waves_getr8(pw_waveset *ws, const char *id, double *val) {
   else if (s == "WAVEEKIN1") *val = ws->wavekin1;
   else if (s == "WAVEEKIN2") *val = ws->wavekin2;
   else if (s == "EFICT") *val = ws->efict;
+  else if (s == "EHARTREE") *val = ws->ehartree;
+  else if (s == "EXC") *val = ws->exc;
+  else if (s == "RHOB") *val = ws->rhob;
   else return pw_fail(PW_ERR_ARG, "WAVES$GETR8: ID not recognised: " + s);
   return PW_OK;
 }
@@ -675,8 +685,17 @@ extern "C" int pw_waves_etot(pw_waveset *ws) {
     ScopeTimer tm(ws, "density");
     PW_TRY(fft_density(p, ws->nbh, ws->d_psi0, ws->d_occ2, ws->d_rho, ws->prec, ws->stream, ws->tune_dens_blocks));
   }
-  // energy of the fixed local potential: RWEIGHT sum_r V rho
-  PW_TRY(k_dot_vrho(ws, ws->d_res + ws->r_vrho));
+  if (ws->dplan) {
+    // POTENTIAL$VOFRHO: the local potential of this density, used by VPSI
+    // below; its energy EHARTREE + EXC and forces are read with the results
+    if (ws->qlm.size() != (size_t)ws->nat) return pw_fail(PW_ERR_STATE, "WAVES$ETOT: QLM not set");
+    ScopeTimer tm(ws, "vofrho");
+    PW_TRY(vofrho_enqueue(ws->dplan, ws->nat, ws->isp1.data(), ws->r0.data(), ws->qlm.data(), ws->d_rho, ws->d_v,
+                          ws->stream));
+  } else {
+    // energy of the fixed local potential: RWEIGHT sum_r V rho
+    PW_TRY(k_dot_vrho(ws, ws->d_res + ws->r_vrho));
+  }
   // WAVES$FORCE
   PW_TRY(phase_at(ws, ws->r0));
   PW_TRY(dedproj(ws));
@@ -694,7 +713,16 @@ extern "C" int pw_waves_etot(pw_waveset *ws) {
   PW_TRY(fetch_results(ws));
   const double *hr = ws->h_res;
   ws->ekin = 0.5 * sum_of(hr + ws->r_ekin, red_blocks(ws)) * p->vol;
-  ws->epot = sum_of(hr + ws->r_vrho, vrho_blocks(ws)) * p->vol / (double)p->nrl;
+  if (ws->dplan) {
+    double en[3];
+    vofrho_results(ws->dplan, ws->nat, en, ws->vqlm.data(), ws->fpot.data());
+    ws->ehartree = en[0];
+    ws->exc = en[1];
+    ws->rhob = en[2];
+    ws->epot = en[0] + en[1];
+  } else {
+    ws->epot = sum_of(hr + ws->r_vrho, vrho_blocks(ws)) * p->vol / (double)p->nrl;
+  }
   ws->wavekin1 = sum_of(hr + ws->r_wk, red_blocks(ws)) / (ws->delt * ws->delt) * p->vol;
   // density matrix (hermitised, real) and the one-centre energy sum_ij DH_ij D_ji
   const int L = ws->lmnxx;
@@ -721,6 +749,7 @@ extern "C" int pw_waves_etot(pw_waveset *ws) {
     for (int d = 0; d < 3; ++d) {
       double f = 0.0;
       for (int l = 0; l < nl; ++l) f += fp[(size_t)(i0 + l) * 3 + d];
+      if (ws->dplan) f += ws->fpot[3 * a + d];
       ws->force[3 * a + d] = f;
     }
   }
@@ -738,6 +767,28 @@ extern "C" int pw_waves_etot(pw_waveset *ws) {
     if (2 * b + 1 < ws->nb) ws->eig[2 * b + 1] = re - rem;
   }
   ws->have_hpsi = true;
+  return PW_OK;
+}
+
+extern "C" int pw_waves_set_potential(pw_waveset *ws, pw_plan *dplan) {
+  if (!ws) return pw_fail(PW_ERR_ARG, "pw_waves_set_potential: null waveset");
+  if (dplan) {
+    const pw_plan *p = ws->plan;
+    if (dplan->nr1 != p->nr1 || dplan->nr2 != p->nr2 || dplan->nr3 != p->nr3)
+      return pw_fail(PW_ERR_ARG, "pw_waves_set_potential: the density object has another r-space grid");
+    for (int k = 0; k < 9; ++k)
+      if (std::fabs(dplan->rbas[k] - p->rbas[k]) > 1e-12 * (1.0 + std::fabs(p->rbas[k])))
+        return pw_fail(PW_ERR_ARG, "pw_waves_set_potential: the density object has another cell");
+    if (!dplan->d_pot_tabs || dplan->pot_nsp < ws->nsp)
+      return pw_fail(PW_ERR_STATE, "pw_waves_set_potential: setup tables of the density object not set");
+  }
+  PW_HIP(hipStreamSynchronize(ws->stream));
+  ws->dplan = dplan;
+  ws->isp1.resize(ws->nat);
+  for (int a = 0; a < ws->nat; ++a) ws->isp1[a] = ws->isp[a] + 1;
+  ws->vqlm.assign(ws->nat, 0.0);
+  ws->fpot.assign(3 * (size_t)ws->nat, 0.0);
+  if (dplan) ws->have_v = true;   // VOFR is produced by ETOT
   return PW_OK;
 }
 

@@ -82,6 +82,7 @@ SIGNATURES = {
     "pw_waves_density": [_vp, _vp, _vp],
     "pw_ortho_x": [ctypes.c_int, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_int_p],
     "pw_potential_tables": [_vp, ctypes.c_int, _c_dbl_p],
+    "pw_waves_set_potential": [_vp, _vp],
     "pw_vofrho": [_vp, ctypes.c_int, _c_int_p, _c_dbl_p, _c_dbl_p, _vp, _vp, _c_dbl_p, _c_dbl_p, _c_dbl_p],
     "pw_waves_autotune": [_vp],
     "pw_waves_footprint": [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _c_int_p, _c_int_p, ctypes.c_int,

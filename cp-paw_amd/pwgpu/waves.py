@@ -99,7 +99,7 @@ class Waves:
         shapes = {"RHO": (self.plane.nr[2], self.plane.nr[1], self.plane.nr[0]),
                   "FORCE": (self.nat, 3), "EIG": (self.nb,), "MAT": (self.nb, self.nb),
                   "OMAT": (self.nb, self.nb), "OOMAT": (self.nb, self.nb),
-                  "DENMAT": (self.nat, L, L)}
+                  "DENMAT": (self.nat, L, L), "VQLM": (self.nat,)}
         out = np.empty(shapes[key], dtype=np.float64)
         check(self.lib.pw_waves_getr8a(self.handle, key.encode(), out.size, dptr(out)))
         if key in ("MAT", "OMAT", "OOMAT"):
@@ -151,6 +151,13 @@ class Waves:
     def etot(self):
         check(self.lib.pw_waves_etot(self.handle))
         return {k.lower(): self.getr8(k) for k in ("EKIN", "EPOT", "E1C", "ETOT")}
+
+    def set_potential(self, potential, qlm):
+        """Link the density -> potential step (a pwgpu.Potential on the
+        density object of this cell) into etot; qlm = QLM(1,NAT)."""
+        self.setr8a("QLM", qlm)
+        check(self.lib.pw_waves_set_potential(self.handle, potential.plane.handle))
+        self._potential = potential            # keeps the density object alive
 
     def propagate(self):
         check(self.lib.pw_waves_propagate(self.handle))

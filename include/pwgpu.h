@@ -194,6 +194,15 @@ int pw_potential_tables(pw_plan *dplan, int nsp, const double *tabs);
 int pw_vofrho(pw_plan *dplan, int nat, const int *isp, const double *pos,
               const double *qlm, const double *rho_dev, double *vofr_dev,
               double *energies, double *vqlm, double *force);
+/* Links the density -> potential step into WAVES$ETOT, as the reference's
+ * energy evaluation calls POTENTIAL$VOFRHO between WAVES_DENSITY and
+ * WAVES_VPSI: each ETOT then forms V(r) from the density of PSI0 on the
+ * device (positions R0, QLM from WAVES$SETR8A('QLM')), uses it for HPSI,
+ * takes EPOT = EHARTREE + EXC and adds the potential forces to FORCE.
+ * dplan must have the waveset's cell and r-space grid and its setup tables;
+ * NULL detaches (VOFR is then the caller's again).  Calls on one density
+ * object must not overlap. */
+int pw_waves_set_potential(pw_waveset *ws, pw_plan *dplan);
 
 /* ---- launch-shape autotuner and device footprint ---------------------------
  * No counterpart in the reference (FFTW plans with FFTW_MEASURE,

@@ -10,6 +10,7 @@ layout psi[nbh, ngl], proj[npro, nbh].
 import numpy as np
 
 from .planewave import invertg
+from .potential import vofrho
 
 PI = 4.0 * np.arctan(1.0)
 CI = 1j
@@ -484,6 +485,9 @@ class WaveSet:
         self.psim = None
         self.proj = None
         self.rlam0 = np.zeros((nb, nb))
+        # optional density -> potential step: (density Plane, tables dict,
+        # species (NAT,), QLM (NAT,)); None keeps the fixed potential v
+        self.potential = None
         self.rlamm = None
         self.first = True
         self.wavekin1 = 0.0
@@ -504,11 +508,21 @@ class WaveSet:
         e["ekin"] = ekin(pl, self.occ, self.psi0)
         dm = denmat(st, self.occ, self.proj)
         self.rho = density(pl, self.occ, self.psi0)
-        e["epot"] = pl.rweight * float(np.sum(self.v * self.rho))
+        if self.potential is not None:
+            # POTENTIAL$VOFRHO between WAVES_DENSITY and WAVES_VPSI: the local
+            # potential of this density (src/paw_potential.f90:324-896)
+            dplane, tabs, isp, qlm = self.potential
+            self.pot = vofrho(dplane, self.rho, r0, isp, qlm, tabs)
+            self.v = self.pot["v"]
+            e["epot"] = self.pot["ehartree"] + self.pot["exc"]
+        else:
+            e["epot"] = pl.rweight * float(np.sum(self.v * self.rho))
         e["e1c"] = float(sum(np.real(np.sum(self.dh[i] * dm[i].T)) for i in range(st.nat)))
         e["etot"] = e["ekin"] + e["epot"] + e["e1c"]
         self.force = force(pl, st, self.pro, self.ylm, r0, self.occ, self.psi0,
                            self.proj, self.dh, self.rlam0)
+        if self.potential is not None:
+            self.force = self.force + self.pot["force"]
         hp = vpsi(pl, self.v, self.psi0)
         self.hpsi = addpro(pl, st, self.pro, self.ylm, r0, hp,
                            hproj(st, self.dh, self.proj))

@@ -79,3 +79,38 @@ def test_vofrho_in_place_and_repeatable():
     v = buf.to_host().reshape(rho.shape)
     assert en["ehartree"] == a["ehartree"] and en["exc"] == a["exc"]
     assert np.array_equal(v, a["v"]) and np.array_equal(vq, a["vqlm"]) and np.array_equal(fo, a["force"])
+
+
+@pytest.mark.parametrize("name,nsteps", [("tiny", 4), ("si8", 3), ("h2o4", 3)])
+def test_scf_cp_steps_match_oracle(name, nsteps):
+    # CP steps with the density -> potential step linked into ETOT
+    # (pw_waves_set_potential) against the oracle with the same coupling
+    import pwgpu
+    from harness import gpu_waveset, oracle_waveset
+    s = CASES[name]()
+    say(f"case scf cp {name} steps={nsteps}")
+    o = oracle_waveset(s)
+    dpl = opw.Plane(s.rbas, s.epw_rho, s.epw_rho)
+    tabs = s.potential_tables(dpl.g2)
+    o.potential = (dpl, {"pscorg": tabs[0], "vbarg": tabs[1], "g0": tabs[2], "v0": tabs[3]}, s.isp, s.qlm())
+    plane, w = gpu_waveset(s)
+    dplane = pwgpu.PlaneWave(s.rbas, s.epw_rho, s.epw_rho)
+    pot = pwgpu.Potential(dplane, tabs)
+    w.set_potential(pot, s.qlm())
+    for it in range(nsteps):
+        eo = o.etot(s.pos)
+        eg = w.etot()
+        assert abs(eg["etot"] - eo["etot"]) <= 1e-10 * abs(eo["etot"]), it
+        assert abs(w.getr8("EHARTREE") - o.pot["ehartree"]) <= 1e-10 * abs(o.pot["ehartree"]), it
+        assert abs(w.getr8("EXC") - o.pot["exc"]) <= 1e-10 * abs(o.pot["exc"]), it
+        assert relerr(w.getr8a("FORCE"), o.force) < 1e-9, it
+        o.propagate()
+        o.orthogonalize(s.pos, s.pos)
+        w.propagate()
+        w.orthogonalize()
+        assert w.geti4("ORTHOITER") == o.ortho_iter, it
+        assert relerr(w.getc8a("RLAM0").real, o.rlam0) < 1e-9, it
+        o.switch()
+        w.switch()
+    assert relerr(w.getc8a("PSI0"), o.psi0) < 1e-9
+    say(f"  {name}: etot={eg['etot']:.12f} E_H={w.getr8('EHARTREE'):.10f} Exc={w.getr8('EXC'):.10f}")

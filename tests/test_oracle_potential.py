@@ -145,3 +145,42 @@ def test_vqlm_is_qlm_derivative():
         em, _ = energy(pl, rho, s.pos, s.isp, q, tabs)
         fd = (ep - em) / (2 * h)
         assert abs(fd - r["vqlm"][a]) < 1e-7 * max(1.0, abs(fd)), a
+
+
+def scf_waveset(s):
+    from harness import oracle_waveset
+    o = oracle_waveset(s)
+    dpl = opw.Plane(s.rbas, s.epw_rho, s.epw_rho)
+    o.potential = (dpl, tabs_dict(s.potential_tables(dpl.g2)), s.isp, s.qlm())
+    return o
+
+
+def test_scf_hpsi_is_energy_gradient():
+    # with the density -> potential step linked in, ETOT is no longer linear
+    # in the density; HPSI must still be its gradient: for a direction dpsi in
+    # fully occupied super bands, dE/deps = 2 f Omega Re sum_G conj(dpsi) Hpsi
+    from oracle import waves as owv
+    s = model.tiny()
+    o = scf_waveset(s)
+    o.etot(s.pos)                       # Gram-Schmidt start
+    psi = o.psi0.copy()
+    rng = np.random.default_rng(21)
+    d = np.zeros_like(psi)
+    d[:2] = (rng.normal(size=(2, o.plane.ngl)) + 1j * rng.normal(size=(2, o.plane.ngl))) / (1 + o.plane.g2) ** 2
+    d[:2] = 0.5 * (d[:2] + np.conj(d[:2][:, o.plane.minusg]))   # Gamma: c(-G) = conj c(G)
+
+    def e_at(p):
+        o.psi0 = p
+        o.proj = owv.projections(o.plane, o.setup, o.pro, o.ylm, s.pos, p)
+        return o.etot(s.pos)["etot"]
+
+    e_at(psi)
+    an = 2.0 * o.occ[0] * o.plane.vol * float(np.sum(np.conj(d[:2]) * o.hpsi[:2]).real)
+
+    def cd(eps):
+        return (e_at(psi + eps * d) - e_at(psi - eps * d)) / (2 * eps)
+
+    # the energy is not quadratic in psi here: Richardson-extrapolated
+    # central differences (error O(eps^4))
+    fd = (4.0 * cd(5e-4) - cd(1e-3)) / 3.0
+    assert abs(fd - an) < 1e-8 * abs(an), (fd, an)
