@@ -195,6 +195,56 @@ def run_gpu(system_name, prec, steps, warmup, profile):
     return res
 
 
+def potential_measure(system_name, calls, steps, with_cpu):
+    """POTENTIAL$VOFRHO on the density object of the cell (model valence
+    density, model setup tables): ms per call (host results included), the
+    mixed-form CP step rate with the step linked into ETOT, and the FP64 oracle
+    on the host for one call."""
+    import numpy as np
+    import pwgpu
+    from pwgpu import model
+    s = model.SYSTEMS[system_name]()
+    dplane = pwgpu.PlaneWave(s.rbas, s.epw_rho, s.epw_rho)
+    g2 = dplane.getr8a("G2")
+    tabs = s.potential_tables(g2)
+    pot = pwgpu.Potential(dplane, tabs)
+    rho = s.valence_density(dplane.nr)
+    r = pwgpu.DeviceArray.from_host(np.ascontiguousarray(rho).reshape(-1))
+    v = pwgpu.DeviceArray((dplane.nrl,), np.float64)
+    qlm = s.qlm()
+    for _ in range(3):
+        pot.vofrho_device(s.isp, s.pos, qlm, r, v)
+    t0 = time.perf_counter()
+    for _ in range(calls):
+        pot.vofrho_device(s.isp, s.pos, qlm, r, v)
+    ms = 1e3 * (time.perf_counter() - t0) / calls
+    plane = pwgpu.PlaneWave(s.rbas, s.epw_psi, s.epw_rho)
+    w = pwgpu.Waves.from_system(plane, s, prec=pwgpu.MIXED)
+    w.set_potential(pot, qlm)
+    for _ in range(3):
+        w.step()
+    w.lib.pw_device_synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        w.step()
+    w.lib.pw_device_synchronize()
+    sps = steps / (time.perf_counter() - t0)
+    out = {"vofrho_ms": ms, "density_G": dplane.ngl, "grid": list(dplane.nr), "atoms": s.nat,
+           "scf_steps_per_s": sps, "scf_form": "mixed (VOFRHO in FP64)",
+           "scope": "one spin, LDA (PZ), l=0 compensation, model setup tables"}
+    w.close()
+    plane.close()
+    if with_cpu:
+        from oracle import planewave as opw
+        from oracle import potential as opot
+        pl = opw.Plane(s.rbas, s.epw_rho, s.epw_rho)
+        t0 = time.perf_counter()
+        opot.vofrho(pl, rho, s.pos, s.isp, qlm,
+                    {"pscorg": tabs[0], "vbarg": tabs[1], "g0": tabs[2], "v0": tabs[3]})
+        out["cpu_oracle_vofrho_s"] = time.perf_counter() - t0
+    return out
+
+
 def cpu_baseline(system_name, nsteps=2):
     """The FP64 oracle (numpy) on the host: one untimed step, then nsteps timed;
     the rate is that of the median step."""
@@ -263,6 +313,8 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=4.0, help="Poisson arrival rate, requests/s")
     ap.add_argument("--serve-slots", type=int, default=4, help="concurrent requests on the GPU")
     ap.add_argument("--profile", action="store_true", help="print per-kernel times to stderr")
+    ap.add_argument("--pot-calls", type=int, default=20,
+                    help="calls of the density -> potential measurement (0: skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (calibrates the HBM "
                          "byte counters of a profiler run) and exit")
@@ -321,6 +373,11 @@ def main():
         log(f"serving: {args.serve_requests} requests at {args.serve_rate}/s, {args.serve_slots} slots ...")
         srv = serving(args.serve_requests, args.serve_rate, args.serve_slots)
         log(f"serving: {srv['req_per_s']:.2f} req/s p50 {srv['p50_latency_s']:.2f}s p99 {srv['p99_latency_s']:.2f}s")
+    potm = None
+    if args.pot_calls > 0:
+        potm = potential_measure(args.system, args.pot_calls, max(5, min(20, args.steps)), not args.no_cpu)
+        log(f"vofrho: {potm['vofrho_ms']:.3f} ms/call; self-consistent mixed step "
+            f"{potm['scf_steps_per_s']:.1f} steps/s")
     cpu = None
     if not args.no_cpu:
         log("cpu baseline (oracle) ...")
@@ -349,6 +406,8 @@ def main():
         out["mixed_error_vs_fp64"] = acc
     if srv is not None:
         out["serving"] = srv
+    if potm is not None:
+        out["potential"] = potm
     if 0 in results and 1 in results:
         out["fp64_steps_per_s"] = results[0]["steps_per_s"]
         out["mixed_over_fp64"] = results[1]["steps_per_s"] / results[0]["steps_per_s"]
