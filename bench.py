@@ -159,6 +159,7 @@ def run_gpu(system_name, prec, steps, warmup, profile):
     s = model.SYSTEMS[system_name]()
     plane = pwgpu.PlaneWave(s.rbas, s.epw_psi, s.epw_rho)
     w = pwgpu.Waves.from_system(plane, s, prec=prec)
+    w.autotune()                      # launch shapes of this system shape (the product's autotuner)
     lib = w.lib
     t0 = time.time()
     for i in range(warmup):
@@ -176,7 +177,8 @@ def run_gpu(system_name, prec, steps, warmup, profile):
     e = {k: w.getr8(k) for k in ("ETOT", "EKIN", "EFICT")}
     res = {"steps_per_s": steps / dt, "ms_per_step": 1e3 * dt / steps, "energies": e,
            "ortho_iter": w.geti4("ORTHOITER"), "ortho_levels": w.geti4("ORTHOLEVELS"),
-           "ortho_ns": w.geti4("ORTHONS"),
+           "ortho_ns": w.geti4("ORTHONS"), "ortho_series": w.geti4("ORTHOSERIES"),
+           "launch_shapes": {"split_waves": w.geti4("SPLITWAVES"), "density_blocks": w.geti4("DENSBLOCKS")},
            "desc": f"Gamma, {s.nat} atoms, {s.nb} bands, {plane.ngl} G, {plane.nr[0]}x{plane.nr[1]}x{plane.nr[2]}, "
                    f"{s.npro} projectors"}
     if profile:
@@ -412,6 +414,8 @@ def main():
         out["fp64_steps_per_s"] = results[0]["steps_per_s"]
         out["mixed_over_fp64"] = results[1]["steps_per_s"] / results[0]["steps_per_s"]
     out["ortho_iter_last"] = r["ortho_iter"]
+    out["ortho_series_terms"] = r["ortho_series"]
+    out["launch_shapes"] = r["launch_shapes"]
     print(json.dumps(out))
 
 

@@ -209,22 +209,30 @@ __device__ inline double2 radd(double2 a, double2 b) { return make_double2(a.x +
 __device__ inline double rscale(double a, double s) { return a * s; }
 __device__ inline double2 rscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
 
+// out[i] = scale * sum_k part[k][i]: kRedS lanes per column take the splits
+// k = sl, sl + kRedS, ...; each lane issues its loads 16 at a time and adds
+// them in k order, then the lanes are added in order (deterministic)
 template <typename T>
 __global__ void __launch_bounds__(kRedC * kRedS) k_reduce_split(const T *__restrict__ part, int ns, int n,
                                                                 double scale, T *__restrict__ out) {
   __shared__ T red[kRedS][kRedC];
   const int c = threadIdx.x % kRedC, sl = threadIdx.x / kRedC;
   const int i = blockIdx.x * kRedC + c;
-  T acc[4] = {rz(T()), rz(T()), rz(T()), rz(T())};
+  T acc = rz(T());
   if (i < n) {
-    int k = sl;
-    for (; k + 3 * kRedS < ns; k += 4 * kRedS) {
+    constexpr int V = 16;
+    for (int k0 = sl; k0 < ns; k0 += V * kRedS) {
+      T v[V];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) acc[u] = radd(acc[u], part[(size_t)(k + u * kRedS) * n + i]);
+      for (int u = 0; u < V; ++u) {
+        const int k = k0 + u * kRedS;
+        v[u] = k < ns ? part[(size_t)k * n + i] : rz(T());
+      }
+#pragma unroll
+      for (int u = 0; u < V; ++u) acc = radd(acc, v[u]);
     }
-    for (int u = 0; k < ns; k += kRedS, ++u) acc[u] = radd(acc[u], part[(size_t)k * n + i]);
   }
-  red[sl][c] = radd(radd(acc[0], acc[1]), radd(acc[2], acc[3]));
+  red[sl][c] = acc;
   __syncthreads();
   if (sl != 0 || i >= n) return;
   T r = red[0][c];
@@ -684,23 +692,70 @@ struct RowPf {
 // (row, 8 run .. 8 run + 7).  nrun covers K rounded up to whole 32-chunks and
 // the tail is zero.  An operand that several tiles of a product read (the
 // waves of the row GEMMs, the coefficients of the column GEMMs) is split once
-// by k_split_runs; the product then stages a run with six 16-byte copies.
+// by k_split_rows / k_split_cols; the product then stages a run with six
+// 16-byte copies.
 // Layouts: row-major SP[row][run] (a thread's run is 8 consecutive K, the
 // threads of a wave take consecutive runs) or run-major SP[run][row] (the
 // threads of a wave take consecutive rows: column operands, whose K runs
 // gather across bands).
-template <class Src, bool RUN_MAJOR>
-__global__ void k_split_runs(Src src, int nrow, int nrun, u32x4 *sp) {
+// Row-major runs: one lane per complex value.  The 8 lanes of a run read 8
+// consecutive values (a wave reads 1 KB contiguous), write their bf16 pieces
+// into an LDS image of the run (plane q, element e at 16 q + 2 e bytes) and
+// store the 96-byte run back as 12 contiguous bytes per lane.
+template <class Src>
+__global__ void __launch_bounds__(256) k_split_rows(Src src, int nrow, int nrun, unsigned *sp) {
+  __shared__ __bf16 img[32][8 * NPL];
+  const int t = threadIdx.x, e = t & 7, lr = t >> 3;
+  const size_t id = (size_t)blockIdx.x * 32 + lr;
+  const bool ok = id < (size_t)nrow * nrun;
+  double2 v = make_double2(0.0, 0.0);
+  if (ok) {
+    const int r = (int)(id / nrun), run = (int)(id % nrun);
+    v = src(r, 8 * run + e);
+  }
+  const Split3 a = split3(v.x), b = split3(v.y);
+  img[lr][e] = a.h;
+  img[lr][8 + e] = a.m;
+  img[lr][16 + e] = a.l;
+  img[lr][24 + e] = b.h;
+  img[lr][32 + e] = b.m;
+  img[lr][40 + e] = b.l;
+  __syncthreads();
+  if (!ok) return;
+  const unsigned *w = reinterpret_cast<const unsigned *>(&img[lr][0]) + 3 * e;
+  unsigned *d = sp + id * (NPL * 4) + 3 * e;
+  d[0] = w[0];
+  d[1] = w[1];
+  d[2] = w[2];
+}
+
+// Run-major runs: one thread per run (8 K of one row; the lanes of a wave
+// take consecutive rows, so each of its 8 loads is coalesced).  The wave's
+// 64 runs are contiguous in the output: they go through LDS and out as six
+// stores of 16 contiguous bytes per lane.
+template <class Src>
+__global__ void __launch_bounds__(256) k_split_cols(Src src, int nrow, int nrun, u32x4 *sp) {
+  __shared__ u32x4 img[256 * NPL];
   const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= (size_t)nrow * nrun) return;
-  const int r = RUN_MAJOR ? (int)(idx % nrow) : (int)(idx / nrun);
-  const int run = RUN_MAJOR ? (int)(idx / nrow) : (int)(idx % nrun);
+  const size_t n = (size_t)nrow * nrun;
   Planes8 o;
+  if (idx < n) {
+    const int r = (int)(idx % nrow), run = (int)(idx / nrow);
 #pragma unroll
-  for (int e = 0; e < 8; ++e) put(o, e, src(r, 8 * run + e));
-  u32x4 *d = sp + idx * NPL;
+    for (int e = 0; e < 8; ++e) put(o, e, src(r, 8 * run + e));
+  } else {
+    zero8(o);
+  }
 #pragma unroll
-  for (int q = 0; q < NPL; ++q) d[q] = __builtin_bit_cast(u32x4, o.p[q]);
+  for (int q = 0; q < NPL; ++q) img[threadIdx.x * NPL + q] = __builtin_bit_cast(u32x4, o.p[q]);
+  __syncthreads();
+  const size_t base = (size_t)blockIdx.x * blockDim.x * NPL;
+  const size_t lim = n * NPL;
+#pragma unroll
+  for (int q = 0; q < NPL; ++q) {
+    const size_t k = (size_t)q * blockDim.x + threadIdx.x;
+    if (base + k < lim) sp[base + k] = img[k];
+  }
 }
 
 struct SplitRuns {
@@ -750,7 +805,7 @@ struct ColRuns {
   __device__ void operator()(int k0, int ig, Planes8 &o) const { s.load(ig, k0, o); }
 };
 
-// sources of k_split_runs, (row, k) -> complex value, zero outside
+// sources of k_split_rows / k_split_cols, (row, k) -> complex value, zero outside
 struct SrcPsiRows {          // psi[b][G], rows b
   const double2 *psi;
   int ngl, nbh;
@@ -1302,7 +1357,11 @@ SplitRuns split_runs_to(pw_waveset *ws, const Src &src, int nrow, int K, void *d
   const int nrun = (K + KB - 1) / KB * (KB / 8);
   const size_t n = (size_t)nrow * nrun;
   u32x4 *sp = static_cast<u32x4 *>(dst);
-  k_split_runs<Src, RUN_MAJOR><<<(unsigned)((n + 255) / 256), 256, 0, ws->stream>>>(src, nrow, nrun, sp);
+  if constexpr (RUN_MAJOR)
+    k_split_cols<Src><<<(unsigned)((n + 255) / 256), 256, 0, ws->stream>>>(src, nrow, nrun, sp);
+  else
+    k_split_rows<Src><<<(unsigned)((n + 31) / 32), 256, 0, ws->stream>>>(src, nrow, nrun,
+                                                                          reinterpret_cast<unsigned *>(sp));
   return RUN_MAJOR ? SplitRuns{sp, nrow, nrun, 1, nrow} : SplitRuns{sp, nrow, nrun, nrun, 1};
 }
 // ... at byte offset off of the scratch split buffer (sized with ws_split)

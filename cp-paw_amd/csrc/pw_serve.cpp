@@ -210,10 +210,19 @@ extern "C" int pw_waves_autotune(pw_waveset *ws) {
   int rc = pw_waves_projections(ws);   // phase tables (and split projectors) for R0
   int best_split = ws->tune_split_waves, best_dens = ws->tune_dens_blocks;
   if (rc == PW_OK) {
+    // G-split width of the row products: more splits fill the GPU, fewer
+    // shrink the partial sums their ordered reduction reads back; timed on
+    // the projections and one band overlap (the two row-product shapes),
+    // reductions included, into the Gram-Schmidt scratch matrix
     double tbest = 1e30;
-    for (int cand : {1024, 2048, 4096}) {
+    double *mat = ws->d_small_mats + (size_t)4 * ws->nb * ws->nb;
+    for (int cand : {256, 512, 1024, 2048, 4096}) {
       ws->tune_split_waves = cand;
-      const double t = timed([&] { return k_projections(ws, ws->d_psi0, ws->d_phase, ws->d_oproj); });
+      const double t = timed([&] {
+        int r = k_projections(ws, ws->d_psi0, ws->d_phase, ws->d_oproj);
+        if (r == PW_OK) r = k_overlap(ws, ws->d_psi0, ws->d_psi0, mat, false, true);
+        return r;
+      });
       if (t < tbest) { tbest = t; best_split = cand; }
     }
     tbest = 1e30;

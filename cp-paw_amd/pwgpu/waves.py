@@ -152,6 +152,11 @@ class Waves:
         check(self.lib.pw_waves_etot(self.handle))
         return {k.lower(): self.getr8(k) for k in ("EKIN", "EPOT", "E1C", "ETOT")}
 
+    def autotune(self):
+        """Launch shapes for this system shape, timed on the device
+        (pw_waves_autotune; PSI0 and R0 must be set); cached per shape."""
+        check(self.lib.pw_waves_autotune(self.handle))
+
     def set_potential(self, potential, qlm):
         """Link the density -> potential step (a pwgpu.Potential on the
         density object of this cell) into etot; qlm = QLM(1,NAT)."""
