@@ -189,8 +189,10 @@ struct pw_waveset {
   double ehartree = 0.0, exc = 0.0, rhob = 0.0;
   std::vector<double> vqlm, fpot;    // dE/dQLM (NAT), potential forces (3 NAT)
   // lambda (real, Gamma point)
-  std::vector<double> rlam0, rlamm;
+  std::vector<double> rlam0, rlamm;  // host copies, filled by the getters
   bool have_rlamm = false;
+  double *d_rlam0 = nullptr, *d_rlamm = nullptr, *d_occ = nullptr;   // RLAM0, RLAMM (NB x NB), OCC (NB)
+  bool have_mats = false;            // MAT/OMAT/OOMAT of the last ORTHOGONALIZE on the device
   bool first = true;
   // results
   double ekin = 0, epot = 0, e1c = 0, etot = 0, wavekin1 = 0, wavekin2 = 0;

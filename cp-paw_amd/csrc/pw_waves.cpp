@@ -62,6 +62,43 @@ static double *d_mat(pw_waveset *ws, int k) {   // 0 MAT, 1 OMAT, 2 OOMAT, 3 LAM
   return ws->d_small_mats + (size_t)k * ws->nb * ws->nb;
 }
 
+// LAMBDA of DEDPROJ: -RLAM0 (f_i + f_j)/2 (src/paw_waves1.f90:5592-5691)
+__global__ void k_lam_dedproj(int nb, const double *occ, const double *rlam0, double *lam) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= nb * nb) return;
+  const int i = idx / nb, j = idx - i * nb;
+  lam[idx] = -rlam0[idx] * 0.5 * (occ[i] + occ[j]);
+}
+
+// initial LAMBDA of ORTHO_X: hermitised RLAM0 times 0.5 f_i/(f_i+f_j)
+// (src/paw_waves2.f90:295-359)
+__global__ void k_lam_guess(int nb, const double *occ, const double *rlam0, double *lam) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= nb * nb) return;
+  const int i = idx / nb, j = idx - i * nb;
+  double v = rlam0[idx];
+  if (i != j) v = 0.5 * (rlam0[idx] + rlam0[(size_t)j * nb + i]);
+  const double oi = occ[i];
+  double oj = occ[j];
+  if (oi + oj < 1.0e-12) oj = 1.0e-12;
+  lam[idx] = v * 0.5 * oi / (oi + oj);
+}
+
+// WAVES$SWITCH with NLAMBDAX=2 (src/paw_waves2.f90:2236-2297):
+// first: RLAMM = RLAM0; then RLAM(+) = 2 RLAM0 - RLAMM, RLAMM = RLAM0
+__global__ void k_lam_switch(int n, int first, double *rlam0, double *rlamm) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n) return;
+  const double l0 = rlam0[idx];
+  if (!first) rlam0[idx] = 2.0 * l0 - rlamm[idx];
+  rlamm[idx] = l0;
+}
+
+__global__ void k_copy_mat(int n, const double *src, double *dst) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx < n) dst[idx] = src[idx];
+}
+
 static int upload_occ(pw_waveset *ws) {
   const int nbh = ws->nbh;
   std::vector<double> occ2(2 * nbh), fpm(2 * nbh), fab(2 * nbh);
@@ -82,6 +119,7 @@ static int upload_occ(pw_waveset *ws) {
   PW_HIP(hipMemcpy(ws->d_occ2, occ2.data(), occ2.size() * sizeof(double), hipMemcpyHostToDevice));
   PW_HIP(hipMemcpy(ws->d_fpm, fpm.data(), fpm.size() * sizeof(double), hipMemcpyHostToDevice));
   PW_HIP(hipMemcpy(ws->d_fab, fab.data(), fab.size() * sizeof(double), hipMemcpyHostToDevice));
+  PW_HIP(hipMemcpy(ws->d_occ, ws->occ.data(), ws->nb * sizeof(double), hipMemcpyHostToDevice));
   return PW_OK;
 }
 
@@ -157,13 +195,9 @@ static int dedproj(pw_waveset *ws) {
   const int nb = ws->nb;
   PW_TRY(k_atom_matrix(ws, ws->d_proj, ws->d_xproj, ws->d_dhmat));
   PW_TRY(k_fold_occ(ws, ws->d_xproj));
-  // LAMBDA = -RLAM0 (f_i+f_j)/2, staged in pinned memory that is rewritten
-  // only after the step's closing synchronisation
-  double *lam = ws->h_res + ws->res_n;
-  for (int i = 0; i < nb; ++i)
-    for (int j = 0; j < nb; ++j)
-      lam[(size_t)i * nb + j] = -ws->rlam0[(size_t)i * nb + j] * 0.5 * (ws->occ[i] + ws->occ[j]);
-  PW_HIP(hipMemcpyAsync(d_mat(ws, 3), lam, (size_t)nb * nb * sizeof(double), hipMemcpyHostToDevice, ws->stream));
+  // LAMBDA = -RLAM0 (f_i+f_j)/2 from the device-resident RLAM0
+  k_lam_dedproj<<<(nb * nb + 255) / 256, 256, 0, ws->stream>>>(nb, ws->d_occ, ws->d_rlam0, d_mat(ws, 3));
+  PW_HIP(hipGetLastError());
   PW_TRY(k_atom_matrix(ws, ws->d_proj, ws->d_oproj, ws->d_domat));
   PW_TRY(k_addoproj(ws, ws->d_xproj, ws->d_oproj, d_mat(ws, 3)));
   return PW_OK;
@@ -317,6 +351,9 @@ extern "C" int pw_waves_create(pw_plan *plan, int nb, int nat, int nsp, const in
   if (rc == PW_OK) rc = dalloc((void **)&ws->d_v, plan->nrl * sizeof(double));
   if (rc == PW_OK) rc = dalloc((void **)&ws->d_rho, plan->nrl * sizeof(double));
   if (rc == PW_OK) rc = dalloc((void **)&ws->d_occ2, 2 * ws->nbh * sizeof(double));
+  if (rc == PW_OK) rc = dalloc((void **)&ws->d_occ, ws->nb * sizeof(double));
+  if (rc == PW_OK) rc = dalloc((void **)&ws->d_rlam0, (size_t)nb * nb * sizeof(double));
+  if (rc == PW_OK) rc = dalloc((void **)&ws->d_rlamm, (size_t)nb * nb * sizeof(double));
   if (rc == PW_OK) rc = dalloc((void **)&ws->d_fpm, 2 * ws->nbh * sizeof(double));
   if (rc == PW_OK) rc = dalloc((void **)&ws->d_fab, 2 * ws->nbh * sizeof(double));
   if (rc == PW_OK) rc = dalloc((void **)&ws->d_domat, (size_t)nat * ws->lmnxx * ws->lmnxx * sizeof(double2));
@@ -375,6 +412,7 @@ extern "C" int pw_waves_destroy(pw_waveset *ws) {
   ortho_graph_release(ws);
   void *ptrs[] = {ws->d_psi0, ws->d_psim, ws->d_hpsi, ws->d_opsi, ws->d_tmp, ws->d_proj, ws->d_oproj,
                   ws->d_xproj, ws->d_pro, ws->d_ylm, ws->d_mpsi, ws->d_v, ws->d_rho, ws->d_occ2,
+                  ws->d_occ, ws->d_rlam0, ws->d_rlamm,
                   ws->d_fpm, ws->d_fab, ws->d_domat, ws->d_dhmat, ws->d_atom_isp, ws->d_ipro0,
                   ws->d_ibpro, ws->d_lmnx, ws->d_lmn_ln, ws->d_lmn_lm, ws->d_lmn_l, ws->d_phase,
                   ws->d_part, ws->d_small, ws->d_ortho, ws->d_small_mats, ws->d_ipro_tab, ws->d_pfac,
@@ -489,10 +527,11 @@ extern "C" int pw_waves_setc8a(pw_waveset *ws, const char *id, int len, const do
   } else if (s == "RLAM0" || s == "RLAMM") {
     const size_t n = (size_t)ws->nb * ws->nb;
     if ((size_t)len != n) return pw_fail(PW_ERR_ARG, "WAVES$SETC8A: size inconsistent for " + s);
-    std::vector<double> &dst = s == "RLAM0" ? ws->rlam0 : ws->rlamm;
-    dst.assign(n, 0.0);
+    std::vector<double> h(n, 0.0);
     for (int i = 0; i < ws->nb; ++i)
-      for (int j = 0; j < ws->nb; ++j) dst[(size_t)i * ws->nb + j] = val[2 * ((size_t)j * ws->nb + i)];
+      for (int j = 0; j < ws->nb; ++j) h[(size_t)i * ws->nb + j] = val[2 * ((size_t)j * ws->nb + i)];
+    PW_HIP(hipMemcpy(s == "RLAM0" ? ws->d_rlam0 : ws->d_rlamm, h.data(), n * sizeof(double),
+                     hipMemcpyHostToDevice));
     if (s == "RLAMM") ws->have_rlamm = true;
   } else {
     return pw_fail(PW_ERR_ARG, "WAVES$SETC8A: ID not recognised: " + s);
@@ -521,6 +560,9 @@ extern "C" int pw_waves_getc8a(pw_waveset *ws, const char *id, int len, double *
   if (s == "RLAM0") {
     const size_t n = (size_t)ws->nb * ws->nb;
     if ((size_t)len != n) return pw_fail(PW_ERR_ARG, "WAVES$GETC8A: size inconsistent");
+    ws->rlam0.resize(n);
+    PW_HIP(hipStreamSynchronize(ws->stream));
+    PW_HIP(hipMemcpy(ws->rlam0.data(), ws->d_rlam0, n * sizeof(double), hipMemcpyDeviceToHost));
     for (int i = 0; i < ws->nb; ++i)
       for (int j = 0; j < ws->nb; ++j) {
         val[2 * ((size_t)j * ws->nb + i)] = ws->rlam0[(size_t)i * ws->nb + j];
@@ -554,8 +596,12 @@ extern "C" int pw_waves_getr8a(pw_waveset *ws, const char *id, int len, double *
     std::memcpy(val, ws->eig.data(), (size_t)len * sizeof(double));
   } else if (s == "MAT" || s == "OMAT" || s == "OOMAT") {
     PW_TRY(chk((size_t)ws->nb * ws->nb));
-    const std::vector<double> &m = s == "MAT" ? ws->mat : (s == "OMAT" ? ws->omat : ws->oomat);
-    if (m.size() != (size_t)len) return pw_fail(PW_ERR_STATE, "WAVES: no orthogonalisation yet");
+    if (!ws->have_mats) return pw_fail(PW_ERR_STATE, "WAVES: no orthogonalisation yet");
+    std::vector<double> &m = s == "MAT" ? ws->mat : (s == "OMAT" ? ws->omat : ws->oomat);
+    m.resize((size_t)len);
+    PW_HIP(hipStreamSynchronize(ws->stream));
+    PW_HIP(hipMemcpy(m.data(), d_mat(ws, s == "MAT" ? 0 : (s == "OMAT" ? 1 : 2)), (size_t)len * sizeof(double),
+                     hipMemcpyDeviceToHost));
     to_colmajor(m, ws->nb, val);
   } else if (s == "VQLM") {
     PW_TRY(chk((size_t)ws->nat));
@@ -829,33 +875,16 @@ extern "C" int pw_waves_orthogonalize(pw_waveset *ws) {
   PW_TRY(k_runs_scope(ws, false));
   PW_TRY(k_overlap_1c(ws, ws->d_oproj, ws->d_oproj, d_mat(ws, 2)));
   // initial LAMBDA: hermitised RLAM0 times 0.5 f_i/(f_i+f_j) (src/paw_waves2.f90:295-359)
-  std::vector<double> lam((size_t)nb * nb);
-  for (int i = 0; i < nb; ++i)
-    for (int j = 0; j < nb; ++j) {
-      double v = ws->rlam0[(size_t)i * nb + j];
-      if (i != j) v = 0.5 * (ws->rlam0[(size_t)i * nb + j] + ws->rlam0[(size_t)j * nb + i]);
-      const double oi = ws->occ[i];
-      double oj = ws->occ[j];
-      if (oi + oj < 1.0e-12) oj = 1.0e-12;
-      lam[(size_t)i * nb + j] = v * 0.5 * oi / (oi + oj);
-    }
-  PW_HIP(hipMemcpyAsync(d_mat(ws, 3), lam.data(), lam.size() * sizeof(double), hipMemcpyHostToDevice,
-                        ws->stream));
+  k_lam_guess<<<(nb * nb + 255) / 256, 256, 0, ws->stream>>>(nb, ws->d_occ, ws->d_rlam0, d_mat(ws, 3));
+  PW_HIP(hipGetLastError());
   PW_TRY(ortho_x_device(ws, d_mat(ws, 3), d_mat(ws, 2), d_mat(ws, 0), d_mat(ws, 1), &ws->ortho_iter));
   // psi(+) = psibar + chi Lambda ; proj(+) = proj + oproj Lambda
   PW_TRY(k_addopsi(ws, ws->d_psim, ws->d_opsi, d_mat(ws, 3)));
   PW_TRY(k_addoproj(ws, ws->d_proj, ws->d_oproj, d_mat(ws, 3)));
-  PW_HIP(hipMemcpyAsync(ws->rlam0.data(), d_mat(ws, 3), (size_t)nb * nb * sizeof(double),
-                        hipMemcpyDeviceToHost, ws->stream));
-  ws->mat.resize((size_t)nb * nb);
-  ws->omat.resize((size_t)nb * nb);
-  ws->oomat.resize((size_t)nb * nb);
-  PW_HIP(hipMemcpyAsync(ws->mat.data(), d_mat(ws, 0), ws->mat.size() * sizeof(double), hipMemcpyDeviceToHost,
-                        ws->stream));
-  PW_HIP(hipMemcpyAsync(ws->omat.data(), d_mat(ws, 1), ws->omat.size() * sizeof(double),
-                        hipMemcpyDeviceToHost, ws->stream));
-  PW_HIP(hipMemcpyAsync(ws->oomat.data(), d_mat(ws, 2), ws->oomat.size() * sizeof(double),
-                        hipMemcpyDeviceToHost, ws->stream));
+  // RLAM0 stays on the device; MAT, OMAT, OOMAT go to the host only when asked for
+  k_copy_mat<<<(nb * nb + 255) / 256, 256, 0, ws->stream>>>(nb * nb, d_mat(ws, 3), ws->d_rlam0);
+  PW_HIP(hipGetLastError());
+  ws->have_mats = true;
   PW_TRY(k_wavekinetic(ws, ws->d_psi0, ws->d_psim, ws->d_fab, ws->d_res + ws->r_wk));
   PW_HIP(hipMemcpyAsync(ws->h_res + ws->r_wk, ws->d_res + ws->r_wk, red_blocks(ws) * sizeof(double),
                         hipMemcpyDeviceToHost, ws->stream));
@@ -868,16 +897,11 @@ extern "C" int pw_waves_orthogonalize(pw_waveset *ws) {
 extern "C" int pw_waves_switch(pw_waveset *ws) {
   if (!ws) return pw_fail(PW_ERR_ARG, "pw_waves_switch: null waveset");
   std::swap(ws->d_psi0, ws->d_psim);
-  // NLAMBDAX=2 extrapolation (src/paw_waves2.f90:2236-2297)
-  if (!ws->have_rlamm) {
-    ws->rlamm = ws->rlam0;
-    ws->have_rlamm = true;
-  } else {
-    std::vector<double> rlamp(ws->rlam0.size());
-    for (size_t k = 0; k < rlamp.size(); ++k) rlamp[k] = 2.0 * ws->rlam0[k] - ws->rlamm[k];
-    ws->rlamm = ws->rlam0;
-    ws->rlam0 = rlamp;
-  }
+  // NLAMBDAX=2 extrapolation (src/paw_waves2.f90:2236-2297), on the device
+  const int n2 = ws->nb * ws->nb;
+  k_lam_switch<<<(n2 + 255) / 256, 256, 0, ws->stream>>>(n2, ws->have_rlamm ? 0 : 1, ws->d_rlam0, ws->d_rlamm);
+  PW_HIP(hipGetLastError());
+  ws->have_rlamm = true;
   ws->wavekin1 = 0.0;
   ws->wavekin2 = 0.0;
   return PW_OK;
