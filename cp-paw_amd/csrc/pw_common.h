@@ -260,6 +260,7 @@ struct pw_waveset {
   size_t small_bytes = 0;
   // orthogonalisation work (device, FP64)
   double *d_ortho = nullptr;
+  void *h_ortho_state = nullptr;   // pinned: ORTHO_X loop state read back per batch
   size_t ortho_cap = 0;
   // captured graph of the ORTHO_X preparation and first iteration batch
   void *ortho_exec = nullptr;

@@ -995,7 +995,9 @@ int ortho_x_device(pw_waveset *ws, double *d_lam, const double *d_chichi, const 
   const double *old_base = ws->d_ortho;
   PW_TRY(ortho_work(ws, w));
   if (ws->d_ortho != old_base) ortho_graph_release(ws);
-  PW_HIP(hipMemcpyAsync(w.occ, ws->occ.data(), n * sizeof(double), hipMemcpyHostToDevice, ws->stream));
+  if (ws->d_occ) w.occ = ws->d_occ;   // the waveset's occupations are on the device already
+  else PW_HIP(hipMemcpyAsync(w.occ, ws->occ.data(), n * sizeof(double), hipMemcpyHostToDevice, ws->stream));
+  if (!ws->h_ortho_state) PW_HIP(hipHostMalloc(&ws->h_ortho_state, sizeof(OrthoState), hipHostMallocDefault));
   k_symm<<<nblk, 256, 0, ws->stream>>>(n, d_chipsi, w.bs);
   PW_HIP(hipGetLastError());
   ws->ortho_lanczos = 0;
@@ -1089,9 +1091,11 @@ int ortho_x_device(pw_waveset *ws, double *d_lam, const double *d_chichi, const 
     }
     int launched = batch;
     OrthoState hs{};
+    OrthoState *hp = static_cast<OrthoState *>(ws->h_ortho_state);
     for (;;) {
-      PW_HIP(hipMemcpyAsync(&hs, w.st, sizeof(OrthoState), hipMemcpyDeviceToHost, ws->stream));
+      PW_HIP(hipMemcpyAsync(hp, w.st, sizeof(OrthoState), hipMemcpyDeviceToHost, ws->stream));
       PW_HIP(hipStreamSynchronize(ws->stream));
+      hs = *hp;
       if (hs.done || launched >= kMaxIter) break;
       const int nit = std::min(4, kMaxIter - launched);
       PW_TRY(enqueue_iters(ws, w, P, lbuf, d_chichi, d_psipsi, d_chipsi, launched + 1, launched + nit));
@@ -1154,6 +1158,7 @@ extern "C" int pw_ortho_x(int nb, const double *occ, const double *chichi, const
   ortho_graph_release(&ws);
   (void)hipFree(d);
   if (ws.d_ortho) (void)hipFree(ws.d_ortho);
+  if (ws.h_ortho_state) (void)hipHostFree(ws.h_ortho_state);
   (void)hipStreamDestroy(ws.stream);
   return rc;
 }

@@ -421,6 +421,7 @@ extern "C" int pw_waves_destroy(pw_waveset *ws) {
     if (p) (void)hipFree(p);
   if (ws->d_res) (void)hipFree(ws->d_res);
   if (ws->h_res) (void)hipHostFree(ws->h_res);
+  if (ws->h_ortho_state) (void)hipHostFree(ws->h_ortho_state);
   if (ws->own_stream && ws->stream) (void)hipStreamDestroy(ws->stream);
   delete ws;
   std::lock_guard<std::mutex> lk(capture_gate().mu);
