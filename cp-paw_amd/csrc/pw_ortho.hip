@@ -34,8 +34,9 @@ constexpr int kMaxIter = 200;      // src/paw_waves2.f90:1870 (NITERX)
 constexpr int kLzThreads = 1024;
 constexpr int kLzMaxSteps = 30;
 
-// Loop state of one ORTHO_X call, written by the convergence test (last
-// workgroup of k_gemm_pair<kPairGamma>) and the acceptance test (k_prep_check).
+// Loop state of one ORTHO_X call, written by the convergence test in each
+// iteration's k_series_sum (the first one also applies the acceptance test of
+// the preparation) and reset by k_ns_first.
 struct OrthoState {
   int done;   // 1 once converged or failed
   int iter;   // iteration at which max|GAMN| < tolerance
@@ -244,6 +245,7 @@ struct PairArgs {
 };
 
 __device__ inline double nanmax2(double a, double b) { return (a != a) ? a : ((b != b) ? b : fmax(a, b)); }
+__device__ inline double nanmax(double a, double b) { return (a != a || b != b) ? NAN : fmax(a, b); }
 
 template <int MODE>
 __global__ void __launch_bounds__(64 * kGemmWaves) k_gemm_pair(PairArgs a, OrthoState *st) {
@@ -432,6 +434,10 @@ struct SumArgs {
   int nbmax;
   double tol;
   int it;
+  const double *pred;          // first iteration: acceptance maxima of the preparation (2 per block)
+  int npred;
+  double ptol_res, ptol_pow;   // n max|(B+qI)X - I| and n max|A^(2^(J-1))| bounds
+  int n_check;                 // n
 };
 
 // The convergence test of iteration it (src/paw_waves2.f90:1960-1975) from
@@ -440,7 +446,7 @@ struct SumArgs {
 template <int MODE>
 __global__ void __launch_bounds__(256) k_series_sum(SumArgs a, OrthoState *st) {
   if (stopped(st)) return;
-  __shared__ double wm[4];
+  __shared__ double wm[4], wp2[4];
   __shared__ int wn[4];
   const int n = a.n;
   const size_t n2 = (size_t)n * n;
@@ -475,9 +481,34 @@ __global__ void __launch_bounds__(256) k_series_sum(SumArgs a, OrthoState *st) {
   __syncthreads();
   m = fmax(fmax(wm[0], wm[1]), fmax(wm[2], wm[3]));
   nan = wn[0] | wn[1] | wn[2] | wn[3];
+  // first iteration: the acceptance test of the preparation (spectral bounds
+  // and the Newton-Schulz inverse); a rejection (code 3) makes the host
+  // recompute the bounds
+  bool reject = false;
+  if (a.pred) {
+    __syncthreads();
+    double p1 = 0.0, p2 = 0.0;
+    for (int b = threadIdx.x; b < a.npred; b += 256) {
+      p1 = nanmax(p1, a.pred[2 * b]);
+      p2 = nanmax(p2, a.pred[2 * b + 1]);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      p1 = nanmax(p1, __shfl_xor(p1, o));
+      p2 = nanmax(p2, __shfl_xor(p2, o));
+    }
+    if ((threadIdx.x & 63) == 0) {
+      wm[threadIdx.x >> 6] = p1;
+      wp2[threadIdx.x >> 6] = p2;
+    }
+    __syncthreads();
+    p1 = nanmax(nanmax(wm[0], wm[1]), nanmax(wm[2], wm[3]));
+    p2 = nanmax(nanmax(wp2[0], wp2[1]), nanmax(wp2[2], wp2[3]));
+    reject = !((double)a.n_check * p1 <= a.ptol_res && (double)a.n_check * p2 <= a.ptol_pow);
+  }
   // the convergence test of iteration it (src/paw_waves2.f90:1960-1975):
   // every workgroup reaches the same decision; the first one records it
-  const int code = nan ? 1 : (m < a.tol ? 0 : (m > 1.0e200 ? 2 : -1));
+  const int code = reject ? 3 : (nan ? 1 : (m < a.tol ? 0 : (m > 1.0e200 ? 2 : -1)));
   if (code >= 0) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       st->code = code;
@@ -664,12 +695,12 @@ __global__ void __launch_bounds__(kLzThreads) k_lanczos(int n, int m, const doub
   }
 }
 
-// Acceptance test of the spectral bounds (before the outer loop):
-// R = (B+qI)M - I must vanish and the last Smith power A^(2^J) must be small
-// enough that the truncated sum is exact to rounding; otherwise code 3 makes
-// the host recompute the bounds.  Block maxima first (NaN propagates), then
-// one block decides.
-__device__ inline double nanmax(double a, double b) { return (a != a || b != b) ? NAN : fmax(a, b); }
+// Acceptance test of the spectral bounds (before the outer loop): the
+// Newton-Schulz residual R = (B+qI)X - I of the last step (the returned M has
+// residual R^2) must be small and so must the last power A^(2^(J-1)), so that
+// the truncated series is exact to rounding; otherwise code 3 makes the host
+// recompute the bounds.  k_prep_part forms block maxima (NaN propagates); the
+// first iteration's k_series_sum decides.
 
 __global__ void k_prep_part(int n, const double *T, const double *AJ, double *bmax) {
   __shared__ double r1[256], r2[256];
@@ -695,48 +726,24 @@ __global__ void k_prep_part(int n, const double *T, const double *AJ, double *bm
   }
 }
 
-__global__ void k_prep_check(int n, int nblk, const double *bmax, double rtol, double atol,
-                             OrthoState *st) {
-  __shared__ double r1[256], r2[256];
-  double m1 = 0.0, m2 = 0.0;
-  for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
-    m1 = nanmax(m1, bmax[2 * b]);
-    m2 = nanmax(m2, bmax[2 * b + 1]);
-  }
-  r1[threadIdx.x] = m1;
-  r2[threadIdx.x] = m2;
-  __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) {
-      r1[threadIdx.x] = nanmax(r1[threadIdx.x], r1[threadIdx.x + s]);
-      r2[threadIdx.x] = nanmax(r2[threadIdx.x], r2[threadIdx.x + s]);
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0 && !(r1[0] <= rtol && (double)n * r2[0] <= atol)) {
-    st->code = 3;
-    st->done = 1;
-  }
-}
-
-__global__ void k_state_init(OrthoState *st) {
-  if (threadIdx.x == 0) {
-    st->done = 0;
-    st->iter = 0;
-    st->code = 0;
-    st->pad = 0;
-  }
-}
-
 __global__ void k_copy_d(size_t n, const double *src, double *dst) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) dst[i] = src[i];
 }
 
-__global__ void k_scaled_identity(int n, double a, double *X) {
+// First Newton-Schulz step from X0 = c I, in closed form:
+// X1 = 2 X0 - X0 (B + qI) X0 = 2c I - c^2 (B + qI); also resets the loop state
+__global__ void k_ns_first(int n, double c, double q, const double *Bs, double *X, OrthoState *st) {
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx == 0) {
+    st->done = 0;
+    st->iter = 0;
+    st->code = 0;
+    st->pad = 0;
+  }
   if (idx >= n * n) return;
-  X[idx] = (idx / n == idx % n) ? a : 0.0;
+  const bool d = idx / n == idx % n;
+  X[idx] = (d ? 2.0 * c : 0.0) - c * c * (Bs[idx] + (d ? q : 0.0));
 }
 
 // A = I - 2 q M
@@ -750,6 +757,7 @@ struct OrthoWork {
   double *bs, *m0, *m1, *gam, *x0, *x1, *t1, *t2, *lam2, *occ, *red, *bounds;
   double *apow[kMaxLevels];
   double *pstk, *spart;         // Cayley series: P_k and the terms (K n x n each)
+  double *pred;                 // acceptance maxima of the preparation (2 per block)
   OrthoState *st;
 };
 
@@ -757,7 +765,7 @@ int ortho_work(pw_waveset *ws, OrthoWork &w) {
   const size_t n2 = (size_t)ws->nb * ws->nb;
   const size_t nblk = (n2 + 255) / 256;
   const size_t kser = (size_t)1 << kSeriesMaxLevels;
-  const size_t need = (9 + kMaxLevels + 2 * kser) * n2 + ws->nb + 2 * nblk + 64;
+  const size_t need = (9 + kMaxLevels + 2 * kser) * n2 + ws->nb + 4 * nblk + 64;
   if (need > ws->ortho_cap) {
     if (ws->d_ortho) PW_HIP(hipFree(ws->d_ortho));
     ws->d_ortho = nullptr;
@@ -783,6 +791,7 @@ int ortho_work(pw_waveset *ws, OrthoWork &w) {
   w.occ = p; p += ws->nb;
   w.bounds = p; p += 8;
   w.st = reinterpret_cast<OrthoState *>(p); p += 8;
+  w.pred = p; p += 2 * nblk;
   w.red = p;
   return PW_OK;
 }
@@ -859,12 +868,13 @@ int enqueue_prep(pw_waveset *ws, const OrthoWork &w, const OrthoParams &P) {
   const size_t n2 = (size_t)n * n;
   const int nblk = (int)((n2 + 255) / 256);
   const int ls = P.nser;
-  k_state_init<<<1, 64, 0, ws->stream>>>(w.st);
-  // M = (B + qI)^-1 by Newton-Schulz, X0 = 2/(lo+hi+2q) I; the last step
-  // writes M to P_0
-  k_scaled_identity<<<nblk, 256, 0, ws->stream>>>(n, 2.0 / (P.lo + P.hi + 2.0 * P.q), w.m0);
+  // M = (B + qI)^-1 by Newton-Schulz from X0 = c I, c = 2/(lo+hi+2q): the
+  // first step in closed form, the others as GEMM pairs; the last step
+  // writes M to P_0 and leaves T = (B+qI) X in t1 for the acceptance test
+  const double c = 2.0 / (P.lo + P.hi + 2.0 * P.q);
+  k_ns_first<<<nblk, 256, 0, ws->stream>>>(n, c, P.q, w.bs, w.m0, w.st);
   double *mc = w.m0, *mn = w.m1;
-  for (int k = 0; k < P.nns; ++k) {
+  for (int k = 1; k < P.nns; ++k) {
     double *dst = (k + 1 == P.nns) ? w.pstk : mn;
     gemm(ws, w.bs, 0, mc, 0, w.t1, 1.0, mc, P.q);       // T = (B+qI) X
     gemm(ws, mc, 0, w.t1, 0, dst, -1.0, mc, 2.0);       // X' = 2X - X T
@@ -890,9 +900,8 @@ int enqueue_prep(pw_waveset *ws, const OrthoWork &w, const OrthoParams &P) {
     PW_TRY(panel_batch(ws, b));
   }
   for (int j = ls; j + 1 < P.nlev; ++j) gemm(ws, w.apow[j], 0, w.apow[j], 0, w.apow[j + 1]);
-  gemm(ws, w.bs, 0, mc, 0, w.t1, 1.0, mc, P.q);         // (B+qI) M
-  k_prep_part<<<nblk, 256, 0, ws->stream>>>(n, w.t1, w.apow[P.nlev - 1], w.red);
-  k_prep_check<<<1, 256, 0, ws->stream>>>(n, nblk, w.red, 1.0e-11, kAcceptPower, w.st);
+  // acceptance maxima; the test itself runs in the first iteration's k_series_sum
+  k_prep_part<<<nblk, 256, 0, ws->stream>>>(n, w.t1, w.apow[P.nlev - 1], w.pred);
   return PW_OK;
 }
 
@@ -946,6 +955,15 @@ int enqueue_iters(pw_waveset *ws, const OrthoWork &w, const OrthoParams &P, doub
     su.nbmax = nt * nt;
     su.tol = eps;
     su.it = it;
+    if (it == 1) {
+      // Newton-Schulz residuals square per step: n max|R_k| <= 3.16e-6 for
+      // the last T = (B+qI) X_k bounds the returned M's residual by 1e-11
+      su.pred = w.pred;
+      su.npred = (int)(((size_t)n * n + 255) / 256);
+      su.ptol_res = 3.16e-6;
+      su.ptol_pow = kAcceptPower;
+      su.n_check = n;
+    }
     if (ls == P.nlev) {
       su.out = lnew;
       series_sum<kSeriesLambda>(ws, su, w.st);
@@ -1043,8 +1061,8 @@ int ortho_x_device(pw_waveset *ws, double *d_lam, const double *d_chichi, const 
     // (the acceptance test asks for 1e-11)
     const double kp = (P.hi + P.q) / (P.lo + P.q);
     const double e0 = (kp - 1.0) / (kp + 1.0);
-    P.nns = 1;
-    if (e0 > 1.0e-300) P.nns = std::max(1, (int)std::ceil(std::log2(std::max(1.0, std::log(1.0e-17) / std::log(e0)))));
+    P.nns = 2;
+    if (e0 > 1.0e-300) P.nns = std::max(2, (int)std::ceil(std::log2(std::max(1.0, std::log(1.0e-17) / std::log(e0)))));
     ws->ortho_levels = P.nlev;
     ws->ortho_series = 1 << P.nser;
     ws->ortho_ns = P.nns;
