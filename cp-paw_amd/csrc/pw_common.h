@@ -190,7 +190,7 @@ struct pw_waveset {
   std::vector<double> vqlm, fpot;    // dE/dQLM (NAT), potential forces (3 NAT)
   // lambda (real, Gamma point)
   std::vector<double> rlam0, rlamm;  // host copies, filled by the getters
-  bool have_rlamm = false;
+  bool have_rlam0 = false, have_rlamm = false;   // history depth of the multipliers (NLAMBDA)
   double *d_rlam0 = nullptr, *d_rlamm = nullptr, *d_occ = nullptr;   // RLAM0, RLAMM (NB x NB), OCC (NB)
   bool have_mats = false;            // MAT/OMAT/OOMAT of the last ORTHOGONALIZE on the device
   bool first = true;

@@ -533,7 +533,7 @@ extern "C" int pw_waves_setc8a(pw_waveset *ws, const char *id, int len, const do
       for (int j = 0; j < ws->nb; ++j) h[(size_t)i * ws->nb + j] = val[2 * ((size_t)j * ws->nb + i)];
     PW_HIP(hipMemcpy(s == "RLAM0" ? ws->d_rlam0 : ws->d_rlamm, h.data(), n * sizeof(double),
                      hipMemcpyHostToDevice));
-    if (s == "RLAMM") ws->have_rlamm = true;
+    if (s == "RLAMM") ws->have_rlamm = true; else ws->have_rlam0 = true;
   } else {
     return pw_fail(PW_ERR_ARG, "WAVES$SETC8A: ID not recognised: " + s);
   }
@@ -558,15 +558,19 @@ extern "C" int pw_waves_getc8a(pw_waveset *ws, const char *id, int len, double *
   if (s == "OPSI") return get_dev_c(ws, ws->d_opsi, npsi, len, val);
   if (s == "PROJ") return get_dev_c(ws, ws->d_proj, nproj, len, val);
   if (s == "OPROJ") return get_dev_c(ws, ws->d_oproj, nproj, len, val);
-  if (s == "RLAM0") {
+  if (s == "RLAM0" || s == "RLAMM") {
     const size_t n = (size_t)ws->nb * ws->nb;
     if ((size_t)len != n) return pw_fail(PW_ERR_ARG, "WAVES$GETC8A: size inconsistent");
-    ws->rlam0.resize(n);
+    const bool zero = s == "RLAM0" ? !ws->have_rlam0 : !ws->have_rlamm;
+    if (s == "RLAMM" && zero) return pw_fail(PW_ERR_STATE, "WAVES$GETC8A: RLAMM not set (NLAMBDA < 2)");
+    std::vector<double> &h = s == "RLAM0" ? ws->rlam0 : ws->rlamm;
+    h.resize(n);
     PW_HIP(hipStreamSynchronize(ws->stream));
-    PW_HIP(hipMemcpy(ws->rlam0.data(), ws->d_rlam0, n * sizeof(double), hipMemcpyDeviceToHost));
+    PW_HIP(hipMemcpy(h.data(), s == "RLAM0" ? ws->d_rlam0 : ws->d_rlamm, n * sizeof(double),
+                     hipMemcpyDeviceToHost));
     for (int i = 0; i < ws->nb; ++i)
       for (int j = 0; j < ws->nb; ++j) {
-        val[2 * ((size_t)j * ws->nb + i)] = ws->rlam0[(size_t)i * ws->nb + j];
+        val[2 * ((size_t)j * ws->nb + i)] = h[(size_t)i * ws->nb + j];
         val[2 * ((size_t)j * ws->nb + i) + 1] = 0.0;
       }
     return PW_OK;
@@ -652,6 +656,9 @@ extern "C" int pw_waves_geti4(pw_waveset *ws, const char *id, int *val) {
   else if (s == "DENSBLOCKS") *val = ws->tune_dens_blocks;
   else if (s == "ORTHOLANCZOS") *val = ws->ortho_lanczos;
   else if (s == "ORTHOGRAPHMISSES") *val = ws->ortho_graph_misses;
+  // depth of the multiplier history as WAVES_WRITELAMBDA counts it
+  // (src/paw_waves2.f90:3948-3954): 0 none, 1 RLAM0, 2 RLAM0 and RLAMM
+  else if (s == "NLAMBDA") *val = ws->have_rlamm ? 2 : (ws->have_rlam0 ? 1 : 0);
   else return pw_fail(PW_ERR_ARG, "WAVES$GETI4: ID not recognised: " + s);
   return PW_OK;
 }
@@ -886,6 +893,7 @@ extern "C" int pw_waves_orthogonalize(pw_waveset *ws) {
   k_copy_mat<<<(nb * nb + 255) / 256, 256, 0, ws->stream>>>(nb * nb, d_mat(ws, 3), ws->d_rlam0);
   PW_HIP(hipGetLastError());
   ws->have_mats = true;
+  ws->have_rlam0 = true;
   PW_TRY(k_wavekinetic(ws, ws->d_psi0, ws->d_psim, ws->d_fab, ws->d_res + ws->r_wk));
   PW_HIP(hipMemcpyAsync(ws->h_res + ws->r_wk, ws->d_res + ws->r_wk, red_blocks(ws) * sizeof(double),
                         hipMemcpyDeviceToHost, ws->stream));
@@ -902,7 +910,9 @@ extern "C" int pw_waves_switch(pw_waveset *ws) {
   const int n2 = ws->nb * ws->nb;
   k_lam_switch<<<(n2 + 255) / 256, 256, 0, ws->stream>>>(n2, ws->have_rlamm ? 0 : 1, ws->d_rlam0, ws->d_rlamm);
   PW_HIP(hipGetLastError());
-  ws->have_rlamm = true;
+  // NLAMBDA=0: RLAM0 becomes the zero guess; NLAMBDA>=1: the old RLAM0 becomes RLAMM
+  ws->have_rlamm = ws->have_rlam0;
+  ws->have_rlam0 = true;
   ws->wavekin1 = 0.0;
   ws->wavekin2 = 0.0;
   return PW_OK;

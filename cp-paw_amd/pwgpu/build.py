@@ -29,7 +29,7 @@ def hipcc():
 
 
 FDIR = os.path.join(os.path.dirname(HERE), "fortran")
-FSOURCES = ["pwgpu_c.f90", "pawgpu.f90", "pawgpu_cpstep.f90"]
+FSOURCES = ["pwgpu_c.f90", "pawgpu_restart.f90", "pawgpu.f90", "pawgpu_cpstep.f90"]
 FDRIVER = os.path.join(FDIR, "pawgpu_cpstep")
 
 

@@ -62,6 +62,9 @@ class PlaneWave:
         check(self.lib.pw_plan_create(dptr(rb), dptr(kv), int(bool(tinv)), float(epw_psi),
                                       float(epw_rho), ctypes.byref(h)))
         self.handle = h
+        self.rbas = np.array(rbas, dtype=np.float64)
+        self.kvec = np.array(kvec, dtype=np.float64)
+        self.tinv = bool(tinv)
         self.ngl = self.geti4("NGL")
         self.nrl = self.geti4("NRL")
         self.nr = (self.geti4("NR1"), self.geti4("NR2"), self.geti4("NR3"))

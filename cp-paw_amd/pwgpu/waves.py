@@ -89,10 +89,10 @@ class Waves:
         shapes = {"PSI0": (self.nbh, self.plane.ngl), "PSIM": (self.nbh, self.plane.ngl),
                   "HPSI": (self.nbh, self.plane.ngl), "OPSI": (self.nbh, self.plane.ngl),
                   "PROJ": (self.npro, self.nbh), "OPROJ": (self.npro, self.nbh),
-                  "RLAM0": (self.nb, self.nb)}
+                  "RLAM0": (self.nb, self.nb), "RLAMM": (self.nb, self.nb)}
         out = np.empty(shapes[key], dtype=np.complex128)
         check(self.lib.pw_waves_getc8a(self.handle, key.encode(), out.size, dptr(out)))
-        return out.T.copy() if key == "RLAM0" else out
+        return out.T.copy() if key in ("RLAM0", "RLAMM") else out
 
     def getr8a(self, key):
         L = self.lmnxx
@@ -163,6 +163,18 @@ class Waves:
         self.setr8a("QLM", qlm)
         check(self.lib.pw_waves_set_potential(self.handle, potential.plane.handle))
         self._potential = potential            # keeps the density object alive
+
+    def write_restart(self, path, static=False):
+        """WAVES$WRITE into a restart file of the reference's format
+        (pwgpu.restart.save)."""
+        from . import restart
+        restart.save(self, path, static=static)
+
+    def read_restart(self, path):
+        """WAVES$READ from a restart file of the reference's format
+        (pwgpu.restart.load); False when it holds no WAVES section."""
+        from . import restart
+        return restart.load(self, path)
 
     def propagate(self):
         check(self.lib.pw_waves_propagate(self.handle))

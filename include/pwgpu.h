@@ -113,7 +113,10 @@ int pw_waves_setr8a(pw_waveset *ws, const char *id, int len, const double *val);
  * "PSI0" / "PSIM" (NGL*NBH), "RLAM0" / "RLAMM" (NB*NB). */
 int pw_waves_setc8a(pw_waveset *ws, const char *id, int len, const double *val);
 /* Complex results: "PSI0","PSIM","HPSI","OPSI","PROJ","OPROJ" (NBH*NPRO),
- * "RLAM0" (NB*NB). */
+ * "RLAM0" / "RLAMM" (NB*NB; RLAMM fails with PW_ERR_STATE while NLAMBDA < 2).
+ * PSI0, PSIM, RLAM0 and RLAMM are what WAVES_WRITEPSI / WAVES_WRITELAMBDA
+ * (src/paw_waves2.f90:3168, 3920) put into the restart file; the setters take
+ * them back as WAVES_READPSI / WAVES_READLAMBDA (:5147, 4041) do. */
 int pw_waves_getc8a(pw_waveset *ws, const char *id, int len, double *val);
 /* Real results: "RHO" (NRL), "FORCE" (3*NAT), "EIG" (NB, <psi|H|psi>),
  * "MAT","OMAT","OOMAT" (NB*NB, last orthogonalisation), "DENMAT"
@@ -127,7 +130,8 @@ int pw_waves_getr8(pw_waveset *ws, const char *id, double *val);
  * the spectral bounds in the last solve: 0 when the previous bounds were
  * accepted), "ORTHOGRAPHMISSES" (solves issued without a captured hipGraph:
  * captures are made only while one plan and one waveset exist in the
- * process). */
+ * process), "NLAMBDA" (multiplier history held: 0 none, 1 RLAM0, 2 RLAM0
+ * and RLAMM; counted as in WAVES_WRITELAMBDA, src/paw_waves2.f90:3948). */
 int pw_waves_geti4(pw_waveset *ws, const char *id, int *val);
 /* Device pointer of a resident array ("PSI0","PSIM","HPSI","PROJ","VOFR",
  * "RHO") for zero-copy access by the caller. */
