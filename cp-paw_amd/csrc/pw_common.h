@@ -70,7 +70,11 @@ struct Fft1D {
   int n = 0;
   int nstage = 0;
   int radix[12] = {0};
+  // ceil(2^32/d) for the in-place stages' quotients by n/radix[st] and by
+  // L/radix[st], L = radix[0]...radix[st] (see fdiv in pw_fft.hip)
+  unsigned long long mnbf[12] = {0}, mlp[12] = {0};
 };
+inline unsigned long long fdiv_magic(unsigned d) { return (0x100000000ull + d - 1) / d; }
 bool fft_factor(int n, Fft1D &f);
 
 // ---------------------------------------------------------------------------
@@ -94,8 +98,9 @@ struct pw_plan {
   std::vector<int> stripe_x;   // ngl x-index of each entry
   std::vector<int> stripe_s;   // ngl stripe of each entry
   std::vector<int> yz2stripe;  // nr2*nr3, -1 if empty
-  std::vector<int> stripe_pos; // nstripe: digit-reversed z*nr2 + y of each stripe
-  std::vector<int> zpos;       // nzp: digit-reversed z*nr2 of each stripe-holding z plane
+  std::vector<int> stripe_pos; // nstripe: digit-reversed z*yzpitch + y of each stripe
+  std::vector<int> zpos;       // nzp: digit-reversed z*yzpitch of each stripe-holding z plane
+  int yzpitch = 0;             // row pitch of the LDS-resident (y, z) plane: nr2 made odd
   int nzp = 0;                 // z planes holding stripes
   std::vector<int> z2iz;       // nr3, -1 if empty
   std::vector<int> zlist;      // nzp: z of each used plane

@@ -264,6 +264,8 @@ __device__ T2 *lds_fft(T2 *b0, T2 *b1, const Fft1D &f, int npen, int es, int ps,
 
 struct Geo {
   int nr1, nr2, nr3, ngl, nstripe, nzp;
+  int pitch;                      // row pitch of the LDS-resident (y, z) plane (odd)
+  unsigned long long mnzp, mnr2;  // fdiv magics of nzp and nr2
   const int *stripe_off, *stripe_g, *stripe_x, *stripe_s, *yz2stripe, *z2iz, *zlist;
   const int *stripe_pos, *zpos;   // digit-reversed plane offsets of stripes / stripe rows
   const double *g2;
@@ -418,17 +420,18 @@ __device__ inline void store_zpencils(const Geo &g, T2 *planes_band, int y, int 
 // (the reversal of a length follows its radix sequence, Fft1D::radix).
 constexpr int kYZ = 512;
 
-// quotient of w < 2^16 by 0 < d < 2^16 from m = ceil(2^32/d) (exact in that range, d = 1 included)
+// quotient of w < 2^16 by 0 < d < 2^16 from m = ceil(2^32/d) = fdiv_magic(d)
+// (exact in that range, d = 1 included); the magics come from the host
 __device__ inline unsigned fdiv(unsigned w, unsigned long long m) { return (unsigned)(((unsigned long long)w * m) >> 32); }
-__device__ inline unsigned long long fmagic(unsigned d) { return (0x100000000ull + d - 1) / d; }
 
 // one in-place butterfly stage; dif=false: twiddle then DFT (DIT), dif=true:
 // DFT then twiddle (DIF).  tw: LDS copy of exp(+2 pi i m/n), m < n.
+// mpen, mnbf, mlp: fdiv magics of npen, n/R and L/R.
 template <typename T2, int R, bool DIF>
 __device__ inline void ip_stage(T2 *buf, int n, int L, int npen, int es, int ps, const int *pmap,
-                                const T2 *tw, int sign, bool pen_fast) {
+                                const T2 *tw, int sign, bool pen_fast, unsigned long long mpen,
+                                unsigned long long mnbf, unsigned long long mlp) {
   const int Lp = L / R, nbf = n / R, total = nbf * npen, tws = n / L;
-  const unsigned long long mpen = fmagic(npen), mnbf = fmagic(nbf), mlp = fmagic(Lp);
   for (int w = threadIdx.x; w < total; w += kYZ) {
     int pen, b;
     if (pen_fast) {
@@ -474,25 +477,27 @@ __device__ inline void ip_stage(T2 *buf, int n, int L, int npen, int es, int ps,
   }
 
 template <typename T2>
-__device__ void fft_dit(T2 *buf, const Fft1D &f, int npen, int es, int ps, const int *pmap, const T2 *tw,
-                        int sign, bool pen_fast) {
+__device__ void fft_dit(T2 *buf, const Fft1D &f, int npen, unsigned long long mpen, int es, int ps,
+                        const int *pmap, const T2 *tw, int sign, bool pen_fast) {
   int L = 1;
   for (int st = 0; st < f.nstage; ++st) {
     const int r = f.radix[st];
     L *= r;
-#define PW_CALL(R) ip_stage<T2, R, false>(buf, f.n, L, npen, es, ps, pmap, tw, sign, pen_fast)
+#define PW_CALL(R) \
+  ip_stage<T2, R, false>(buf, f.n, L, npen, es, ps, pmap, tw, sign, pen_fast, mpen, f.mnbf[st], f.mlp[st])
     PW_RADIX_SWITCH(r, PW_CALL)
 #undef PW_CALL
   }
 }
 
 template <typename T2>
-__device__ void fft_dif(T2 *buf, const Fft1D &f, int npen, int es, int ps, const int *pmap, const T2 *tw,
-                        int sign, bool pen_fast) {
+__device__ void fft_dif(T2 *buf, const Fft1D &f, int npen, unsigned long long mpen, int es, int ps,
+                        const int *pmap, const T2 *tw, int sign, bool pen_fast) {
   int L = f.n;
   for (int st = f.nstage - 1; st >= 0; --st) {
     const int r = f.radix[st];
-#define PW_CALL(R) ip_stage<T2, R, true>(buf, f.n, L, npen, es, ps, pmap, tw, sign, pen_fast)
+#define PW_CALL(R) \
+  ip_stage<T2, R, true>(buf, f.n, L, npen, es, ps, pmap, tw, sign, pen_fast, mpen, f.mnbf[st], f.mlp[st])
     PW_RADIX_SWITCH(r, PW_CALL)
 #undef PW_CALL
     L /= r;
@@ -502,17 +507,20 @@ __device__ void fft_dif(T2 *buf, const Fft1D &f, int npen, int es, int ps, const
 // zero the plane, scatter the x column of one band's stripes into the
 // digit-reversed (z, y) positions, transform y (DIT, on the rows of the z
 // planes that hold stripes) and z (DIT): pl becomes psi(x, y, z) in natural
-// [z][y] order (forward, +i).  zpos: offset of each stripe-holding row.
+// [z][y] order, rows g.pitch apart (forward, +i).  zpos: offset of each
+// stripe-holding row.  Both transforms run pencil-fast: neighbouring lanes
+// take neighbouring pencils, whose rows (y) or columns (z) fall into distinct
+// LDS banks.
 template <typename T2>
 __device__ inline void yz_forward(const Geo &g, const Fft1D &fy, const Fft1D &fz, const T2 *twy,
                                   const T2 *twz, const T2 *col, T2 *pl, const int *zpos) {
-  const int n23 = g.nr2 * g.nr3;
-  for (int i = threadIdx.x; i < n23; i += kYZ) pl[i] = czero<T2>();
+  const int npl = g.pitch * g.nr3;
+  for (int i = threadIdx.x; i < npl; i += kYZ) pl[i] = czero<T2>();
   __syncthreads();
   for (int s = threadIdx.x; s < g.nstripe; s += kYZ) pl[g.stripe_pos[s]] = col[s];
   __syncthreads();
-  fft_dit<T2>(pl, fy, g.nzp, 1, 0, zpos, twy, +1, false);
-  fft_dit<T2>(pl, fz, g.nr2, g.nr2, 1, nullptr, twz, +1, true);
+  fft_dit<T2>(pl, fy, g.nzp, g.mnzp, 1, 0, zpos, twy, +1, true);
+  fft_dit<T2>(pl, fz, g.nr2, g.mnr2, g.pitch, 1, nullptr, twz, +1, true);
 }
 
 // ---- yz pass of WAVES_VPSI: psi(x,y,z) * V(r), back to the stripes ----------------
@@ -524,7 +532,7 @@ __global__ void __launch_bounds__(kYZ) k_yz_vpsi(Geo g, Fft1D fy, Fft1D fz, cons
   extern __shared__ unsigned char smem_raw[];
   const int n23 = g.nr2 * g.nr3;
   T2 *pl = reinterpret_cast<T2 *>(smem_raw);
-  T2 *ty = pl + n23, *tz = ty + g.nr2;
+  T2 *ty = pl + g.pitch * g.nr3, *tz = ty + g.nr2;
   int *zpos = reinterpret_cast<int *>(tz + g.nr3);
   const int x = blockIdx.x, band = blockIdx.y;
   for (int i = threadIdx.x; i < g.nzp; i += kYZ) zpos[i] = g.zpos[i];
@@ -534,13 +542,15 @@ __global__ void __launch_bounds__(kYZ) k_yz_vpsi(Geo g, Fft1D fy, Fft1D fz, cons
   yz_forward<T2>(g, fy, fz, ty, tz, col, pl, zpos);
   const T *vx = vt + (size_t)x * n23;
   for (int i = threadIdx.x; i < n23; i += kYZ) {
+    const int z = (int)fdiv(i, g.mnr2);
+    T2 &c = pl[z * g.pitch + (i - z * g.nr2)];
     const T vv = vx[i];
-    pl[i].x *= vv;
-    pl[i].y *= vv;
+    c.x *= vv;
+    c.y *= vv;
   }
   __syncthreads();
-  fft_dif<T2>(pl, fz, g.nr2, g.nr2, 1, nullptr, tz, -1, true);
-  fft_dif<T2>(pl, fy, g.nzp, 1, 0, zpos, ty, -1, false);
+  fft_dif<T2>(pl, fz, g.nr2, g.mnr2, g.pitch, 1, nullptr, tz, -1, true);
+  fft_dif<T2>(pl, fy, g.nzp, g.mnzp, 1, 0, zpos, ty, -1, true);
   for (int s = threadIdx.x; s < g.nstripe; s += kYZ) col[s] = pl[g.stripe_pos[s]];
 }
 
@@ -554,7 +564,7 @@ __global__ void __launch_bounds__(kYZ) k_yz_dens(Geo g, Fft1D fy, Fft1D fz, cons
   const int n23 = g.nr2 * g.nr3;
   double *acc = reinterpret_cast<double *>(smem_raw);
   T2 *pl = reinterpret_cast<T2 *>(acc + n23);
-  T2 *ty = pl + n23, *tz = ty + g.nr2;
+  T2 *ty = pl + g.pitch * g.nr3, *tz = ty + g.nr2;
   int *zpos = reinterpret_cast<int *>(tz + g.nr3);
   const int x = blockIdx.x, grp = blockIdx.y;
   const int bstart = grp * bpg, bend = min(nb, bstart + bpg);
@@ -568,7 +578,9 @@ __global__ void __launch_bounds__(kYZ) k_yz_dens(Geo g, Fft1D fy, Fft1D fz, cons
     __syncthreads();
     yz_forward<T2>(g, fy, fz, ty, tz, stripes + ((size_t)band * g.nr1 + x) * g.nstripe, pl, zpos);
     for (int i = threadIdx.x; i < n23; i += kYZ) {
-      const double re = pl[i].x, im = pl[i].y;
+      const int z = (int)fdiv(i, g.mnr2);
+      const T2 c = pl[z * g.pitch + (i - z * g.nr2)];
+      const double re = c.x, im = c.y;
       // Re(psi (s1 conj(psi) + s2 psi)) = s1 |psi|^2 + s2 (re^2 - im^2)
       acc[i] += s1 * (re * re + im * im) + s2 * (re * re - im * im);
     }
@@ -643,7 +655,8 @@ __global__ void __launch_bounds__(kBlock) k_zin(Geo g, Fft1D f, const T2 *tw, co
 Geo make_geo(const pw_plan *p) {
   Geo g;
   g.nr1 = p->nr1; g.nr2 = p->nr2; g.nr3 = p->nr3; g.ngl = p->ngl;
-  g.nstripe = p->nstripe; g.nzp = p->nzp;
+  g.nstripe = p->nstripe; g.nzp = p->nzp; g.pitch = p->yzpitch;
+  g.mnzp = fdiv_magic((unsigned)std::max(1, p->nzp)); g.mnr2 = fdiv_magic((unsigned)p->nr2);
   g.stripe_off = p->d_stripe_off; g.stripe_g = p->d_stripe_g; g.stripe_x = p->d_stripe_x;
   g.stripe_s = p->d_stripe_s; g.yz2stripe = p->d_yz2stripe; g.z2iz = p->d_z2iz;
   g.zlist = p->d_zlist; g.stripe_pos = p->d_stripe_pos; g.zpos = p->d_zpos; g.g2 = p->d_g2;
@@ -745,7 +758,7 @@ template <typename T2>
 int vpsi_t(pw_plan *p, int nb, const double2 *psi, const double *v, double2 *hpsi, int add_kin,
            hipStream_t s) {
   typedef typename Prec<T2>::T T;
-  const size_t lds = ((size_t)p->nr2 * p->nr3 + p->nr2 + p->nr3) * sizeof(T2) + (size_t)p->nzp * sizeof(int);
+  const size_t lds = ((size_t)p->yzpitch * p->nr3 + p->nr2 + p->nr3) * sizeof(T2) + (size_t)p->nzp * sizeof(int);
   PW_TRY(yz_fits(p, lds));
   Geo g = make_geo(p);
   PW_TRY(vt_of<T2>(p, v, s));
@@ -761,7 +774,7 @@ template <typename T2>
 int density_t(pw_plan *p, int nb, const double2 *psi, const double *occ2, double *rho, int target_blocks,
               hipStream_t s) {
   const size_t n23 = (size_t)p->nr2 * p->nr3;
-  const size_t lds = n23 * (sizeof(double) + sizeof(T2)) + (size_t)(p->nr2 + p->nr3) * sizeof(T2) +
+  const size_t lds = n23 * sizeof(double) + ((size_t)p->yzpitch * p->nr3 + p->nr2 + p->nr3) * sizeof(T2) +
                      (size_t)p->nzp * sizeof(int);
   PW_TRY(yz_fits(p, lds));
   Geo g = make_geo(p);

@@ -159,6 +159,12 @@ bool fft_factor(int n, Fft1D &f) {
       if (f.nstage >= 12) return false;
     }
   }
+  int L = 1;
+  for (int st = 0; st < f.nstage; ++st) {
+    f.mnbf[st] = fdiv_magic((unsigned)(n / f.radix[st]));
+    f.mlp[st] = fdiv_magic((unsigned)L);
+    L *= f.radix[st];
+  }
   return m == 1;
 }
 
@@ -294,15 +300,18 @@ static int build_plan(pw_plan *p, const double *rbas, double epw_psi, double epw
   }
   // positions of the (y, z) plane in digit-reversed order, the input order
   // of the in-place decimation-in-time transforms of the yz pass
+  // rows of the plane are yzpitch apart: an odd pitch puts the rows of the
+  // y transforms, which neighbouring lanes work on, into distinct LDS banks
   const std::vector<int> py = digit_reversal_positions(p->f[1]), pz = digit_reversal_positions(p->f[2]);
+  p->yzpitch = p->nr2 | 1;
   p->stripe_pos.assign(p->nstripe, 0);
   for (int z = 0; z < p->nr3; ++z)
     for (int y = 0; y < p->nr2; ++y) {
       const int s = p->yz2stripe[y + p->nr2 * z];
-      if (s >= 0) p->stripe_pos[s] = pz[z] * p->nr2 + py[y];
+      if (s >= 0) p->stripe_pos[s] = pz[z] * p->yzpitch + py[y];
     }
   p->zpos.assign(p->nzp, 0);
-  for (int i = 0; i < p->nzp; ++i) p->zpos[i] = pz[p->zlist[i]] * p->nr2;
+  for (int i = 0; i < p->nzp; ++i) p->zpos[i] = pz[p->zlist[i]] * p->yzpitch;
   return PW_OK;
 }
 
