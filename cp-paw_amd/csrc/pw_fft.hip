@@ -476,11 +476,12 @@ __device__ inline void ip_stage(T2 *buf, int n, int L, int npen, int es, int ps,
     default: break;              \
   }
 
+// stages 0..nst-1 of the DIT transform (nst = f.nstage: all of it)
 template <typename T2>
-__device__ void fft_dit(T2 *buf, const Fft1D &f, int npen, unsigned long long mpen, int es, int ps,
+__device__ void fft_dit(T2 *buf, const Fft1D &f, int nst, int npen, unsigned long long mpen, int es, int ps,
                         const int *pmap, const T2 *tw, int sign, bool pen_fast) {
   int L = 1;
-  for (int st = 0; st < f.nstage; ++st) {
+  for (int st = 0; st < nst; ++st) {
     const int r = f.radix[st];
     L *= r;
 #define PW_CALL(R) \
@@ -490,11 +491,13 @@ __device__ void fft_dit(T2 *buf, const Fft1D &f, int npen, unsigned long long mp
   }
 }
 
+// stages nst-1..0 of the DIF transform (nst = f.nstage: all of it)
 template <typename T2>
-__device__ void fft_dif(T2 *buf, const Fft1D &f, int npen, unsigned long long mpen, int es, int ps,
+__device__ void fft_dif(T2 *buf, const Fft1D &f, int nst, int npen, unsigned long long mpen, int es, int ps,
                         const int *pmap, const T2 *tw, int sign, bool pen_fast) {
-  int L = f.n;
-  for (int st = f.nstage - 1; st >= 0; --st) {
+  int L = 1;
+  for (int st = 0; st < nst; ++st) L *= f.radix[st];
+  for (int st = nst - 1; st >= 0; --st) {
     const int r = f.radix[st];
 #define PW_CALL(R) \
   ip_stage<T2, R, true>(buf, f.n, L, npen, es, ps, pmap, tw, sign, pen_fast, mpen, f.mnbf[st], f.mlp[st])
@@ -506,8 +509,9 @@ __device__ void fft_dif(T2 *buf, const Fft1D &f, int npen, unsigned long long mp
 
 // zero the plane, scatter the x column of one band's stripes into the
 // digit-reversed (z, y) positions, transform y (DIT, on the rows of the z
-// planes that hold stripes) and z (DIT): pl becomes psi(x, y, z) in natural
-// [z][y] order, rows g.pitch apart (forward, +i).  zpos: offset of each
+// planes that hold stripes) and z (DIT) up to, not including, its last
+// stage, which the caller fuses with its use of psi(x, y, z) (z_last_*).
+// Rows of the plane are g.pitch apart (forward, +i).  zpos: offset of each
 // stripe-holding row.  Both transforms run pencil-fast: neighbouring lanes
 // take neighbouring pencils, whose rows (y) or columns (z) fall into distinct
 // LDS banks.
@@ -519,8 +523,71 @@ __device__ inline void yz_forward(const Geo &g, const Fft1D &fy, const Fft1D &fz
   __syncthreads();
   for (int s = threadIdx.x; s < g.nstripe; s += kYZ) pl[g.stripe_pos[s]] = col[s];
   __syncthreads();
-  fft_dit<T2>(pl, fy, g.nzp, g.mnzp, 1, 0, zpos, twy, +1, true);
-  fft_dit<T2>(pl, fz, g.nr2, g.mnr2, g.pitch, 1, nullptr, twz, +1, true);
+  fft_dit<T2>(pl, fy, fy.nstage, g.nzp, g.mnzp, 1, 0, zpos, twy, +1, true);
+  fft_dit<T2>(pl, fz, fz.nstage - 1, g.nr2, g.mnr2, g.pitch, 1, nullptr, twz, +1, true);
+}
+
+// The last DIT stage of the z transforms, one butterfly per (column y, j):
+// it yields psi(z = j + q Lp) for q < R (natural order), Lp = NR3/R.
+template <typename T2, int R>
+__device__ inline void z_last_load(const T2 *base, int j, int Lp, int es, const T2 *tw, T2 *v) {
+#pragma unroll
+  for (int q = 0; q < R; ++q) v[q] = base[q * Lp * es];
+  if (Lp > 1) {
+#pragma unroll
+    for (int q = 1; q < R; ++q) v[q] = cmul(v[q], tw[j * q]);
+  }
+  dft<T2, R>(v, +1);
+}
+
+// VPSI: the last forward z stage, psi V(r), and the first inverse (DIF) z
+// stage, which works on the same R positions, in one pass over the plane
+template <typename T2, int R>
+__device__ inline void z_last_vpsi(const Geo &g, T2 *pl, int nr3, const T2 *tw, const typename Prec<T2>::T *vx) {
+  typedef typename Prec<T2>::T T;
+  const int Lp = nr3 / R, total = Lp * g.nr2;
+  for (int w = threadIdx.x; w < total; w += kYZ) {
+    const int j = (int)fdiv(w, g.mnr2), y = w - j * g.nr2;
+    T2 *base = pl + y + j * g.pitch;
+    T2 v[R];
+    z_last_load<T2, R>(base, j, Lp, g.pitch, tw, v);
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const T vv = vx[(j + q * Lp) * g.nr2 + y];
+      v[q].x *= vv;
+      v[q].y *= vv;
+    }
+    dft<T2, R>(v, -1);
+    if (Lp > 1) {
+#pragma unroll
+      for (int q = 1; q < R; ++q) {
+        T2 t = tw[j * q];
+        t.y = -t.y;
+        v[q] = cmul(v[q], t);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < R; ++q) base[q * Lp * g.pitch] = v[q];
+  }
+  __syncthreads();
+}
+
+// DENSITY: the last forward z stage and the band's term of the density sum
+// acc[z][y] += s1 |psi|^2 + s2 Re(psi^2) = s1 (re^2 + im^2) + s2 (re^2 - im^2)
+template <typename T2, int R>
+__device__ inline void z_last_dens(const Geo &g, const T2 *pl, int nr3, const T2 *tw, double s1, double s2,
+                                   double *acc) {
+  const int Lp = nr3 / R, total = Lp * g.nr2;
+  for (int w = threadIdx.x; w < total; w += kYZ) {
+    const int j = (int)fdiv(w, g.mnr2), y = w - j * g.nr2;
+    T2 v[R];
+    z_last_load<T2, R>(pl + y + j * g.pitch, j, Lp, g.pitch, tw, v);
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const double re = v[q].x, im = v[q].y;
+      acc[(j + q * Lp) * g.nr2 + y] += s1 * (re * re + im * im) + s2 * (re * re - im * im);
+    }
+  }
 }
 
 // ---- yz pass of WAVES_VPSI: psi(x,y,z) * V(r), back to the stripes ----------------
@@ -541,16 +608,11 @@ __global__ void __launch_bounds__(kYZ) k_yz_vpsi(Geo g, Fft1D fy, Fft1D fz, cons
   T2 *col = stripes + ((size_t)band * g.nr1 + x) * g.nstripe;
   yz_forward<T2>(g, fy, fz, ty, tz, col, pl, zpos);
   const T *vx = vt + (size_t)x * n23;
-  for (int i = threadIdx.x; i < n23; i += kYZ) {
-    const int z = (int)fdiv(i, g.mnr2);
-    T2 &c = pl[z * g.pitch + (i - z * g.nr2)];
-    const T vv = vx[i];
-    c.x *= vv;
-    c.y *= vv;
-  }
-  __syncthreads();
-  fft_dif<T2>(pl, fz, g.nr2, g.mnr2, g.pitch, 1, nullptr, tz, -1, true);
-  fft_dif<T2>(pl, fy, g.nzp, g.mnzp, 1, 0, zpos, ty, -1, true);
+#define PW_CALL(R) z_last_vpsi<T2, R>(g, pl, g.nr3, tz, vx)
+  PW_RADIX_SWITCH(fz.radix[fz.nstage - 1], PW_CALL)
+#undef PW_CALL
+  fft_dif<T2>(pl, fz, fz.nstage - 1, g.nr2, g.mnr2, g.pitch, 1, nullptr, tz, -1, true);
+  fft_dif<T2>(pl, fy, fy.nstage, g.nzp, g.mnzp, 1, 0, zpos, ty, -1, true);
   for (int s = threadIdx.x; s < g.nstripe; s += kYZ) col[s] = pl[g.stripe_pos[s]];
 }
 
@@ -577,13 +639,10 @@ __global__ void __launch_bounds__(kYZ) k_yz_dens(Geo g, Fft1D fy, Fft1D fz, cons
     if (s1 == 0.0 && s2 == 0.0) continue;   // uniform across the workgroup
     __syncthreads();
     yz_forward<T2>(g, fy, fz, ty, tz, stripes + ((size_t)band * g.nr1 + x) * g.nstripe, pl, zpos);
-    for (int i = threadIdx.x; i < n23; i += kYZ) {
-      const int z = (int)fdiv(i, g.mnr2);
-      const T2 c = pl[z * g.pitch + (i - z * g.nr2)];
-      const double re = c.x, im = c.y;
-      // Re(psi (s1 conj(psi) + s2 psi)) = s1 |psi|^2 + s2 (re^2 - im^2)
-      acc[i] += s1 * (re * re + im * im) + s2 * (re * re - im * im);
-    }
+    // each acc position belongs to one butterfly of a fixed thread in every band
+#define PW_CALL(R) z_last_dens<T2, R>(g, pl, g.nr3, tz, s1, s2, acc)
+    PW_RADIX_SWITCH(fz.radix[fz.nstage - 1], PW_CALL)
+#undef PW_CALL
   }
   __syncthreads();
   double *dst = part + ((size_t)grp * g.nr1 + x) * n23;
