@@ -75,7 +75,9 @@ struct Fft1D {
   unsigned long long mnbf[12] = {0}, mlp[12] = {0};
 };
 inline unsigned long long fdiv_magic(unsigned d) { return (0x100000000ull + d - 1) / d; }
-bool fft_factor(int n, Fft1D &f);
+// inplace: factors for the in-place LDS transforms of the yz pass, which
+// take no radix 16 (8 x 2 instead) so that their kernels stay small in registers
+bool fft_factor(int n, Fft1D &f, bool inplace = false);
 
 // ---------------------------------------------------------------------------
 // PLANEWAVE instance
@@ -106,6 +108,7 @@ struct pw_plan {
   std::vector<int> zlist;      // nzp: z of each used plane
   int nmax[3] = {0, 0, 0};     // max |IGVEC(k)|
   Fft1D f[3];
+  Fft1D fyz[2];                // y and z factors of the in-place yz pass (no radix 16)
   // device copies
   int *d_minusg = nullptr, *d_igvec = nullptr;
   int *d_stripe_off = nullptr, *d_stripe_g = nullptr, *d_stripe_x = nullptr;

@@ -462,22 +462,23 @@ __device__ inline void ip_stage(T2 *buf, int n, int L, int npen, int es, int ps,
   __syncthreads();
 }
 
-#define PW_RADIX_SWITCH(r, CALL) \
-  switch (r) {                   \
-    case 2: CALL(2); break;      \
-    case 3: CALL(3); break;      \
-    case 4: CALL(4); break;      \
-    case 5: CALL(5); break;      \
-    case 7: CALL(7); break;      \
-    case 8: CALL(8); break;      \
-    case 11: CALL(11); break;    \
-    case 9: CALL(9); break;      \
-    case 16: CALL(16); break;    \
-    default: break;              \
+// radices of the in-place transforms (fft_factor with inplace: no 16); 11
+// only in the BIG instantiations, whose butterfly needs twice the registers
+#define PW_RADIX_SWITCH(r, CALL)               \
+  switch (r) {                                 \
+    case 2: CALL(2); break;                    \
+    case 3: CALL(3); break;                    \
+    case 4: CALL(4); break;                    \
+    case 5: CALL(5); break;                    \
+    case 7: CALL(7); break;                    \
+    case 8: CALL(8); break;                    \
+    case 9: CALL(9); break;                    \
+    case 11: if constexpr (BIG) CALL(11); break; \
+    default: break;                            \
   }
 
 // stages 0..nst-1 of the DIT transform (nst = f.nstage: all of it)
-template <typename T2>
+template <typename T2, bool BIG>
 __device__ void fft_dit(T2 *buf, const Fft1D &f, int nst, int npen, unsigned long long mpen, int es, int ps,
                         const int *pmap, const T2 *tw, int sign, bool pen_fast) {
   int L = 1;
@@ -492,7 +493,7 @@ __device__ void fft_dit(T2 *buf, const Fft1D &f, int nst, int npen, unsigned lon
 }
 
 // stages nst-1..0 of the DIF transform (nst = f.nstage: all of it)
-template <typename T2>
+template <typename T2, bool BIG>
 __device__ void fft_dif(T2 *buf, const Fft1D &f, int nst, int npen, unsigned long long mpen, int es, int ps,
                         const int *pmap, const T2 *tw, int sign, bool pen_fast) {
   int L = 1;
@@ -515,7 +516,7 @@ __device__ void fft_dif(T2 *buf, const Fft1D &f, int nst, int npen, unsigned lon
 // stripe-holding row.  Both transforms run pencil-fast: neighbouring lanes
 // take neighbouring pencils, whose rows (y) or columns (z) fall into distinct
 // LDS banks.
-template <typename T2>
+template <typename T2, bool BIG>
 __device__ inline void yz_forward(const Geo &g, const Fft1D &fy, const Fft1D &fz, const T2 *twy,
                                   const T2 *twz, const T2 *col, T2 *pl, const int *zpos) {
   const int npl = g.pitch * g.nr3;
@@ -523,8 +524,8 @@ __device__ inline void yz_forward(const Geo &g, const Fft1D &fy, const Fft1D &fz
   __syncthreads();
   for (int s = threadIdx.x; s < g.nstripe; s += kYZ) pl[g.stripe_pos[s]] = col[s];
   __syncthreads();
-  fft_dit<T2>(pl, fy, fy.nstage, g.nzp, g.mnzp, 1, 0, zpos, twy, +1, true);
-  fft_dit<T2>(pl, fz, fz.nstage - 1, g.nr2, g.mnr2, g.pitch, 1, nullptr, twz, +1, true);
+  fft_dit<T2, BIG>(pl, fy, fy.nstage, g.nzp, g.mnzp, 1, 0, zpos, twy, +1, true);
+  fft_dit<T2, BIG>(pl, fz, fz.nstage - 1, g.nr2, g.mnr2, g.pitch, 1, nullptr, twz, +1, true);
 }
 
 // The last DIT stage of the z transforms, one butterfly per (column y, j):
@@ -592,7 +593,7 @@ __device__ inline void z_last_dens(const Geo &g, const T2 *pl, int nr3, const T2
 
 // ---- yz pass of WAVES_VPSI: psi(x,y,z) * V(r), back to the stripes ----------------
 // One workgroup per (x, band); vt = V as [x][z][y] in the work precision.
-template <typename T2>
+template <typename T2, bool BIG>
 __global__ void __launch_bounds__(kYZ) k_yz_vpsi(Geo g, Fft1D fy, Fft1D fz, const T2 *twy, const T2 *twz,
                                                  T2 *stripes, const typename Prec<T2>::T *vt) {
   typedef typename Prec<T2>::T T;
@@ -606,19 +607,19 @@ __global__ void __launch_bounds__(kYZ) k_yz_vpsi(Geo g, Fft1D fy, Fft1D fz, cons
   for (int i = threadIdx.x; i < g.nr2; i += kYZ) ty[i] = twy[i];
   for (int i = threadIdx.x; i < g.nr3; i += kYZ) tz[i] = twz[i];
   T2 *col = stripes + ((size_t)band * g.nr1 + x) * g.nstripe;
-  yz_forward<T2>(g, fy, fz, ty, tz, col, pl, zpos);
+  yz_forward<T2, BIG>(g, fy, fz, ty, tz, col, pl, zpos);
   const T *vx = vt + (size_t)x * n23;
 #define PW_CALL(R) z_last_vpsi<T2, R>(g, pl, g.nr3, tz, vx)
   PW_RADIX_SWITCH(fz.radix[fz.nstage - 1], PW_CALL)
 #undef PW_CALL
-  fft_dif<T2>(pl, fz, fz.nstage - 1, g.nr2, g.mnr2, g.pitch, 1, nullptr, tz, -1, true);
-  fft_dif<T2>(pl, fy, fy.nstage, g.nzp, g.mnzp, 1, 0, zpos, ty, -1, true);
+  fft_dif<T2, BIG>(pl, fz, fz.nstage - 1, g.nr2, g.mnr2, g.pitch, 1, nullptr, tz, -1, true);
+  fft_dif<T2, BIG>(pl, fy, fy.nstage, g.nzp, g.mnzp, 1, 0, zpos, ty, -1, true);
   for (int s = threadIdx.x; s < g.nstripe; s += kYZ) col[s] = pl[g.stripe_pos[s]];
 }
 
 // ---- yz pass of WAVES_DENSITY: sum over a group of bands ----------------------
 // part[group][x][z][y] = sum_b s1 |psi|^2 + s2 Re(psi^2) over the group's bands
-template <typename T2>
+template <typename T2, bool BIG>
 __global__ void __launch_bounds__(kYZ) k_yz_dens(Geo g, Fft1D fy, Fft1D fz, const T2 *twy, const T2 *twz,
                                                  const T2 *stripes, int nb, int bpg, const double *occ2,
                                                  double *part) {
@@ -638,7 +639,7 @@ __global__ void __launch_bounds__(kYZ) k_yz_dens(Geo g, Fft1D fy, Fft1D fz, cons
     const double s1 = occ2[2 * band], s2 = occ2[2 * band + 1];
     if (s1 == 0.0 && s2 == 0.0) continue;   // uniform across the workgroup
     __syncthreads();
-    yz_forward<T2>(g, fy, fz, ty, tz, stripes + ((size_t)band * g.nr1 + x) * g.nstripe, pl, zpos);
+    yz_forward<T2, BIG>(g, fy, fz, ty, tz, stripes + ((size_t)band * g.nr1 + x) * g.nstripe, pl, zpos);
     // each acc position belongs to one butterfly of a fixed thread in every band
 #define PW_CALL(R) z_last_dens<T2, R>(g, pl, g.nr3, tz, s1, s2, acc)
     PW_RADIX_SWITCH(fz.radix[fz.nstage - 1], PW_CALL)
@@ -790,6 +791,14 @@ int yz_lds(K kernel, size_t lds) {
   return PW_OK;
 }
 
+// a radix-11 stage in the in-place y or z transforms needs the BIG kernels
+bool yz_big(const pw_plan *p) {
+  for (int k = 0; k < 2; ++k)
+    for (int st = 0; st < p->fyz[k].nstage; ++st)
+      if (p->fyz[k].radix[st] == 11) return true;
+  return false;
+}
+
 int yz_fits(const pw_plan *p, size_t lds) {
   // one (x, band) plane must stay resident in LDS
   if (lds > 160 * 1024 - 1024)
@@ -822,9 +831,15 @@ int vpsi_t(pw_plan *p, int nb, const double2 *psi, const double *v, double2 *hps
   Geo g = make_geo(p);
   PW_TRY(vt_of<T2>(p, v, s));
   PW_TRY(run_forward_x<T2>(p, g, nb, psi, s));
-  PW_TRY(yz_lds(k_yz_vpsi<T2>, lds));
-  k_yz_vpsi<T2><<<dim3(p->nr1, nb), kYZ, lds, s>>>(g, p->f[1], p->f[2], twid<T2>(p, 1), twid<T2>(p, 2),
-                                                   (T2 *)p->d_stripe_buf, (const T *)p->d_vt);
+  if (yz_big(p)) {
+    PW_TRY(yz_lds(k_yz_vpsi<T2, true>, lds));
+    k_yz_vpsi<T2, true><<<dim3(p->nr1, nb), kYZ, lds, s>>>(g, p->fyz[0], p->fyz[1], twid<T2>(p, 1), twid<T2>(p, 2),
+                                                           (T2 *)p->d_stripe_buf, (const T *)p->d_vt);
+  } else {
+    PW_TRY(yz_lds(k_yz_vpsi<T2, false>, lds));
+    k_yz_vpsi<T2, false><<<dim3(p->nr1, nb), kYZ, lds, s>>>(g, p->fyz[0], p->fyz[1], twid<T2>(p, 1), twid<T2>(p, 2),
+                                                            (T2 *)p->d_stripe_buf, (const T *)p->d_vt);
+  }
   PW_HIP(hipGetLastError());
   return run_inverse_x<T2>(p, g, nb, hpsi, psi, 1.0 / (double)p->nrl, add_kin, s);
 }
@@ -848,10 +863,17 @@ int density_t(pw_plan *p, int nb, const double2 *psi, const double *occ2, double
     PW_HIP(hipMalloc(&p->d_dens_part, need));
     p->dens_bytes = need;
   }
-  PW_TRY(yz_lds(k_yz_dens<T2>, lds));
-  k_yz_dens<T2><<<dim3(p->nr1, ngrp), kYZ, lds, s>>>(g, p->f[1], p->f[2], twid<T2>(p, 1), twid<T2>(p, 2),
-                                                     (const T2 *)p->d_stripe_buf, nb, bpg, occ2,
-                                                     (double *)p->d_dens_part);
+  if (yz_big(p)) {
+    PW_TRY(yz_lds(k_yz_dens<T2, true>, lds));
+    k_yz_dens<T2, true><<<dim3(p->nr1, ngrp), kYZ, lds, s>>>(g, p->fyz[0], p->fyz[1], twid<T2>(p, 1),
+                                                             twid<T2>(p, 2), (const T2 *)p->d_stripe_buf, nb,
+                                                             bpg, occ2, (double *)p->d_dens_part);
+  } else {
+    PW_TRY(yz_lds(k_yz_dens<T2, false>, lds));
+    k_yz_dens<T2, false><<<dim3(p->nr1, ngrp), kYZ, lds, s>>>(g, p->fyz[0], p->fyz[1], twid<T2>(p, 1),
+                                                              twid<T2>(p, 2), (const T2 *)p->d_stripe_buf, nb,
+                                                              bpg, occ2, (double *)p->d_dens_part);
+  }
   // rho[zy][x] = sum_group part[group][x][zy]
   const int A = p->nr1, B = (int)n23;
   dim3 grid((B + 31) / 32, (A + 31) / 32);

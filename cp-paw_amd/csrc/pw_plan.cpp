@@ -143,16 +143,18 @@ std::vector<int> digit_reversal_positions(const Fft1D &f) {
   return pos;
 }
 
-bool fft_factor(int n, Fft1D &f) {
+bool fft_factor(int n, Fft1D &f, bool inplace) {
   f = Fft1D();
   f.n = n;
   int m = n;
-  // Odd radices first: the first DIT stage (and the last DIF stage) of the
-  // in-place LDS transforms reads with element stride = its radix across the
-  // lanes of a wave, which maps to few LDS banks for 8 or 16 but to (almost)
-  // all banks for 9, 3, 5, 7 or 11.  Then the largest powers of two.
-  const int order[] = {9, 3, 5, 7, 11, 16, 8, 4, 2};
-  for (int r : order) {
+  // Odd radices first, then the largest powers of two (16 only for the
+  // Stockham passes; its butterfly doubles the registers of the in-place ones)
+  const int order_s[] = {9, 3, 5, 7, 11, 16, 8, 4, 2};
+  const int order_i[] = {9, 3, 5, 7, 11, 8, 4, 2};
+  const int *order = inplace ? order_i : order_s;
+  const int norder = inplace ? 8 : 9;
+  for (int io = 0; io < norder; ++io) {
+    const int r = order[io];
     while (m % r == 0) {
       f.radix[f.nstage++] = r;
       m /= r;
@@ -187,6 +189,8 @@ static int build_plan(pw_plan *p, const double *rbas, double epw_psi, double epw
   for (int k = 0; k < 3; ++k)
     if (!fft_factor(k == 0 ? p->nr1 : (k == 1 ? p->nr2 : p->nr3), p->f[k]))
       return pw_fail(PW_ERR_UNSUPPORTED, "FFT length not factorisable");
+  if (!fft_factor(p->nr2, p->fyz[0], true) || !fft_factor(p->nr3, p->fyz[1], true))
+    return pw_fail(PW_ERR_UNSUPPORTED, "FFT length not factorisable");
   // GETIG
   boxsph(p->gbas, 0.0, 0.0, 0.0, std::sqrt(2.0 * epw_psi) + 1.0e-6, mn, mx);
   const double *G = p->gbas;
@@ -302,7 +306,7 @@ static int build_plan(pw_plan *p, const double *rbas, double epw_psi, double epw
   // of the in-place decimation-in-time transforms of the yz pass
   // rows of the plane are yzpitch apart: an odd pitch puts the rows of the
   // y transforms, which neighbouring lanes work on, into distinct LDS banks
-  const std::vector<int> py = digit_reversal_positions(p->f[1]), pz = digit_reversal_positions(p->f[2]);
+  const std::vector<int> py = digit_reversal_positions(p->fyz[0]), pz = digit_reversal_positions(p->fyz[1]);
   p->yzpitch = p->nr2 | 1;
   p->stripe_pos.assign(p->nstripe, 0);
   for (int z = 0; z < p->nr3; ++z)
