@@ -8,4 +8,4 @@ from ._lib import PW_PREC_FP64, PW_PREC_MIXED, PwError, load  # noqa: F401
 from .planewave import DeviceArray, PlaneWave  # noqa: F401
 from .waves import FP64, MIXED, Waves, ortho_x  # noqa: F401
 from .potential import Potential  # noqa: F401
-from . import model, restart  # noqa: F401
+from . import control, model, restart  # noqa: F401
