@@ -673,16 +673,23 @@ struct PFac {
 // conj(P(row, G)) from the projector factors
 struct RowPf {
   PFac pf;
-  __device__ void operator()(int row, int g0, int, Planes8 &o) const {
+  // a thread's row is the same in every K chunk: its table rows are looked up once
+  struct Row {
     const double *fr;
     const double2 *er;
     int l;
-    pf.row(row, fr, er, l);
+  };
+  __device__ Row row(int r) const {
+    Row h;
+    pf.row(r, h.fr, h.er, h.l);
+    return h;
+  }
+  __device__ void operator()(const Row &h, int g0, int, Planes8 &o) const {
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
-      const double s = fr[g0 + e];
-      const double2 ee = er[g0 + e];
-      put(o, e, cconj(rot_mil(make_double2(s * ee.x, s * ee.y), l)));
+      const double s = h.fr[g0 + e];
+      const double2 ee = h.er[g0 + e];
+      put(o, e, cconj(rot_mil(make_double2(s * ee.x, s * ee.y), h.l)));
     }
   }
 };
@@ -779,6 +786,8 @@ struct SplitRuns {
 struct OvlRuns {
   SplitRuns rows, minus;
   int nbhp;
+  typedef int Row;
+  __device__ Row row(int r) const { return r; }
   __device__ void operator()(int r, int g0, int, Planes8 &o) const {
     if (r < nbhp) {
       rows.load(r, g0, o);
@@ -865,13 +874,15 @@ __global__ void __launch_bounds__(64 * NW) k_rowgemm_bf(Gen gen, GB gb, int nrow
     accr[c] = dbl4{0.0, 0.0, 0.0, 0.0};
     acci[c] = dbl4{0.0, 0.0, 0.0, 0.0};
   }
+  const int ra = t >> 2, ka = t & 3;          // WR rows x 4 runs = 64 NW runs, one per thread
+  const bool rok = r0 + ra < nrow;
+  const typename Gen::Row rh = gen.row(rok ? r0 + ra : 0);
   for (int gc = g0; gc < g1; gc += KB) {
     {
-      const int r = t >> 2, k8 = t & 3;      // WR rows x 4 runs = 64 NW runs, one per thread
       Planes8 v;
-      if (r0 + r < nrow) gen(r0 + r, gc + 8 * k8, g1, v);
+      if (rok) gen(rh, gc + 8 * ka, g1, v);
       else zero8(v);
-      store_planes(sA, WR, r, k8, v);
+      store_planes(sA, WR, ra, ka, v);
     }
     for (int e = t; e < NB16 * 4; e += 64 * NW) {
       const int b = e >> 2, k8 = e & 3;
@@ -974,7 +985,9 @@ __global__ void __launch_bounds__(64 * NW) k_colgemm_bf(GA ga, GB gb, int nk, in
     {
       const int g = t % WG, k8 = t / WG;      // WG columns x 4 runs, one per thread
       Planes8 v;
-      gb(k0 + 8 * k8, g0 + g, v);
+      // with WG = 64 a wave's K run is uniform: its index lookups become scalar loads
+      const int kk = WG == 64 ? __builtin_amdgcn_readfirstlane(k0 + 8 * k8) : k0 + 8 * k8;
+      gb(kk, g0 + g, v);
       store_planes(sB, WG, g, k8, v);
     }
     for (int e = t; e < NB16 * 4; e += 64 * NW) {
