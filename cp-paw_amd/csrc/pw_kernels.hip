@@ -812,6 +812,12 @@ struct ARuns {
 struct ColRuns {
   SplitRuns s;
   __device__ void operator()(int k0, int ig, Planes8 &o) const { s.load(ig, k0, o); }
+  // k_colgemm_bf fetches the next K chunk's operand before the current chunk's MFMAs
+  struct Pre {
+    Planes8 v;
+  };
+  __device__ void fetch(int k0, int ig, Pre &p) const { s.load(ig, k0, p.v); }
+  __device__ void make(const Pre &p, int, int, Planes8 &o) const { o = p.v; }
 };
 
 // sources of k_split_rows / k_split_cols, (row, k) -> complex value, zero outside
@@ -946,6 +952,33 @@ struct AddproBf {
 #pragma unroll
     for (int e = 0; e < 8; ++e) put(o, e, k0 + e < nk ? pf.at(k0 + e, ig) : czero());
   }
+  // the table values of the next K chunk, fetched before the current chunk's
+  // MFMAs (the eight-projectors-of-one-atom case; others are formed in make)
+  struct Pre {
+    double2 ee;
+    double sv[8];
+    bool one;
+  };
+  __device__ void fetch(int k0, int ig, Pre &p) const {
+    p.one = ig < pf.nglp && k0 + 7 < nk && pf.iptab[k0].x == pf.iptab[k0 + 7].x;
+    if (!p.one) return;
+    const int2 al = pf.iptab[k0];
+    p.ee = pf.e[(size_t)al.x * pf.nglp + ig];
+    const int k = pf.atom_isp[al.x] * pf.lmnxx + al.y;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) p.sv[e] = pf.f[(size_t)(k + e) * pf.nglp + ig];
+  }
+  __device__ void make(const Pre &p, int k0, int ig, Planes8 &o) const {
+    if (!p.one) {
+      (*this)(k0, ig, o);
+      return;
+    }
+    const int2 al = pf.iptab[k0];
+    const int k = pf.atom_isp[al.x] * pf.lmnxx + al.y;
+#pragma unroll
+    for (int e = 0; e < 8; ++e)
+      put(o, e, rot_mil(make_double2(p.sv[e] * p.ee.x, p.sv[e] * p.ee.y), pf.lmn_l[k + e]));
+  }
 };
 struct SrcLamRows {          // LAMBDA1/LAMBDA2 stacked along K = 2 NBH, rows b
   const double *lam;
@@ -981,14 +1014,16 @@ __global__ void __launch_bounds__(64 * NW) k_colgemm_bf(GA ga, GB gb, int nk, in
     accr[c] = dbl4{0.0, 0.0, 0.0, 0.0};
     acci[c] = dbl4{0.0, 0.0, 0.0, 0.0};
   }
+  const int gs = t % WG, ks = t / WG;          // WG columns x 4 runs, one per thread
+  // with WG = 64 a wave's K run is uniform: its index lookups become scalar loads
+  auto krun = [&](int k0) { return WG == 64 ? __builtin_amdgcn_readfirstlane(k0 + 8 * ks) : k0 + 8 * ks; };
+  typename GB::Pre pre;
+  gb.fetch(krun(0), g0 + gs, pre);
   for (int k0 = 0; k0 < nk; k0 += KB) {
     {
-      const int g = t % WG, k8 = t / WG;      // WG columns x 4 runs, one per thread
       Planes8 v;
-      // with WG = 64 a wave's K run is uniform: its index lookups become scalar loads
-      const int kk = WG == 64 ? __builtin_amdgcn_readfirstlane(k0 + 8 * k8) : k0 + 8 * k8;
-      gb(kk, g0 + g, v);
-      store_planes(sB, WG, g, k8, v);
+      gb.make(pre, krun(k0), g0 + gs, v);
+      store_planes(sB, WG, gs, ks, v);
     }
     for (int e = t; e < NB16 * 4; e += 64 * NW) {
       const int b = e >> 2, k8 = e & 3;
@@ -997,6 +1032,7 @@ __global__ void __launch_bounds__(64 * NW) k_colgemm_bf(GA ga, GB gb, int nk, in
       store_planes(sA, NB16, b, k8, v);
     }
     __syncthreads();
+    if (k0 + KB < nk) gb.fetch(krun(k0 + KB), g0 + gs, pre);
     bf16x8 bb[NPL];
     load_planes(sB, WG, 16 * wv + li, lk, bb);
 #pragma unroll
