@@ -684,13 +684,22 @@ struct RowPf {
     pf.row(r, h.fr, h.er, h.l);
     return h;
   }
-  __device__ void operator()(const Row &h, int g0, int, Planes8 &o) const {
+  // k_rowgemm_bf fetches the next G chunk's table values before the current chunk's MFMAs
+  struct Pre {
+    double sv[8];
+    double2 ee[8];
+  };
+  __device__ void fetch(const Row &h, int g0, Pre &p) const {
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
-      const double s = h.fr[g0 + e];
-      const double2 ee = h.er[g0 + e];
-      put(o, e, cconj(rot_mil(make_double2(s * ee.x, s * ee.y), h.l)));
+      p.sv[e] = h.fr[g0 + e];
+      p.ee[e] = h.er[g0 + e];
     }
+  }
+  __device__ void make(const Row &h, const Pre &p, Planes8 &o) const {
+#pragma unroll
+    for (int e = 0; e < 8; ++e)
+      put(o, e, cconj(rot_mil(make_double2(p.sv[e] * p.ee[e].x, p.sv[e] * p.ee[e].y), h.l)));
   }
 };
 
@@ -788,13 +797,18 @@ struct OvlRuns {
   int nbhp;
   typedef int Row;
   __device__ Row row(int r) const { return r; }
-  __device__ void operator()(int r, int g0, int, Planes8 &o) const {
+  struct Pre {
+    Planes8 v;
+  };
+  __device__ void fetch(int r, int g0, Pre &p) const {
+    if (r < nbhp) rows.load(r, g0, p.v);
+    else minus.load(r - nbhp, g0, p.v);
+  }
+  __device__ void make(int r, const Pre &p, Planes8 &o) const {
+    o = p.v;
     if (r < nbhp) {
-      rows.load(r, g0, o);
 #pragma unroll
       for (int q = 3; q < NPL; ++q) o.p[q] = bf_neg(o.p[q]);
-    } else {
-      minus.load(r - nbhp, g0, o);
     }
   }
 };
@@ -883,10 +897,12 @@ __global__ void __launch_bounds__(64 * NW) k_rowgemm_bf(Gen gen, GB gb, int nrow
   const int ra = t >> 2, ka = t & 3;          // WR rows x 4 runs = 64 NW runs, one per thread
   const bool rok = r0 + ra < nrow;
   const typename Gen::Row rh = gen.row(rok ? r0 + ra : 0);
+  typename Gen::Pre pa;
+  if (rok && g0 < g1) gen.fetch(rh, g0 + 8 * ka, pa);
   for (int gc = g0; gc < g1; gc += KB) {
     {
       Planes8 v;
-      if (rok) gen(rh, gc + 8 * ka, g1, v);
+      if (rok) gen.make(rh, pa, v);
       else zero8(v);
       store_planes(sA, WR, ra, ka, v);
     }
@@ -897,6 +913,7 @@ __global__ void __launch_bounds__(64 * NW) k_rowgemm_bf(Gen gen, GB gb, int nrow
       store_planes(sB, NB16, b, k8, v);
     }
     __syncthreads();
+    if (rok && gc + KB < g1) gen.fetch(rh, gc + KB + 8 * ka, pa);
     bf16x8 a[NPL], an[3];
     load_planes(sA, WR, 16 * wv + li, lk, a);
 #pragma unroll
