@@ -1123,12 +1123,14 @@ __global__ void __launch_bounds__(64 * NW) k_force_bf(GA ga, GB gb, int npro, in
     accr[c] = flt4{0.f, 0.f, 0.f, 0.f};
     acci[c] = flt4{0.f, 0.f, 0.f, 0.f};
   }
+  const int gs = t % WG, ks = t / WG;
+  typename GB::Pre pre;
+  gb.fetch(8 * ks, g0 + gs, pre);
   for (int k0 = 0; k0 < nk; k0 += KB) {
     {
-      const int g = t % WG, k8 = t / WG;
       Planes8 v;
-      gb(k0 + 8 * k8, g0 + g, v);
-      store_planes(sB, WG, g, k8, v);
+      gb.make(pre, k0 + 8 * ks, g0 + gs, v);
+      store_planes(sB, WG, gs, ks, v);
     }
     for (int e = t; e < NB16 * 4; e += 64 * NW) {
       const int r = e >> 2, k8 = e & 3;
@@ -1137,6 +1139,7 @@ __global__ void __launch_bounds__(64 * NW) k_force_bf(GA ga, GB gb, int npro, in
       store_planes(sA, NB16, r, k8, v);
     }
     __syncthreads();
+    if (k0 + KB < nk) gb.fetch(k0 + KB + 8 * ks, g0 + gs, pre);
     bf16x8 bb[NPL];
     load_planes(sB, WG, 16 * wv + li, lk, bb);
 #pragma unroll
@@ -1211,19 +1214,32 @@ __global__ void k_fold_lam(int nb, int nbh, const double *lam, double2 *l12) {
   l12[2 * idx + 1] = l2;
 }
 
-__global__ void k_addoproj(int npro, int nbh, double2 *proj, const double2 *oproj, const double2 *l12) {
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= npro * nbh) return;
-  const int k = idx / nbh, j = idx % nbh;
-  const double2 *o = oproj + (size_t)k * nbh;
+// A workgroup takes 64 consecutive elements; its four waves sum quarters of i,
+// which are then added in the order of the quarters (deterministic).
+__global__ void __launch_bounds__(256) k_addoproj(int npro, int nbh, double2 *proj, const double2 *oproj,
+                                                  const double2 *l12) {
+  __shared__ double2 part[4][64];
+  const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
+  const int idx = blockIdx.x * 64 + lane;
+  const bool ok = idx < npro * nbh;
   double2 s = czero();
+  if (ok) {
+    const int k = idx / nbh, j = idx % nbh;
+    const double2 *o = oproj + (size_t)k * nbh;
+    const int i1 = (q + 1) * nbh / 4;
 #pragma unroll 4
-  for (int i = 0; i < nbh; ++i) {
-    const double2 l1 = l12[2 * ((size_t)i * nbh + j)], l2 = l12[2 * ((size_t)i * nbh + j) + 1];
-    const double2 oi = o[i];
-    s = cadd(s, cadd(cmul(oi, l1), cmul(cconj(oi), l2)));
+    for (int i = q * nbh / 4; i < i1; ++i) {
+      const double2 l1 = l12[2 * ((size_t)i * nbh + j)], l2 = l12[2 * ((size_t)i * nbh + j) + 1];
+      const double2 oi = o[i];
+      s = cadd(s, cadd(cmul(oi, l1), cmul(cconj(oi), l2)));
+    }
   }
-  proj[idx] = cadd(proj[idx], s);
+  part[q][lane] = s;
+  __syncthreads();
+  if (q == 0 && ok) {
+    double2 t = cadd(cadd(part[0][lane], part[1][lane]), cadd(part[2][lane], part[3][lane]));
+    proj[idx] = cadd(proj[idx], t);
+  }
 }
 
 // dE/dproj fold (WAVES_DEDPROJ, src/paw_waves1.f90:5651-5666): d = F1 x + F2 conj(x)
@@ -1822,7 +1838,7 @@ int k_addoproj(pw_waveset *ws, double2 *proj, const double2 *oproj, const double
   PW_TRY(ws_small(ws, (size_t)2 * nbh * nbh * sizeof(double2)));
   ScopeTimer tm(ws, "addoproj");
   k_fold_lam<<<(nbh * nbh + 255) / 256, 256, 0, ws->stream>>>(ws->nb, nbh, lam, (double2 *)ws->d_small);
-  k_addoproj<<<(n + 255) / 256, 256, 0, ws->stream>>>(ws->npro, nbh, proj, oproj, (const double2 *)ws->d_small);
+  k_addoproj<<<(n + 63) / 64, 256, 0, ws->stream>>>(ws->npro, nbh, proj, oproj, (const double2 *)ws->d_small);
   PW_HIP(hipGetLastError());
   return PW_OK;
 }
