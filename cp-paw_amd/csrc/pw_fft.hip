@@ -591,6 +591,45 @@ __device__ inline void z_last_dens(const Geo &g, const T2 *pl, int nr3, const T2
   }
 }
 
+// The same with the band sum in registers: thread t owns the positions of the
+// butterflies t and t + kYZ of that stage (R <= kRegR, at most 2 kYZ butterflies;
+// the launcher checks both),
+// the same in every band; flushed to part by dens_reg_store.
+constexpr int kRegR = 8;
+template <typename T2, int R>
+__device__ inline void z_last_dens_reg(const Geo &g, const T2 *pl, int nr3, const T2 *tw, double s1, double s2,
+                                       double (&accr)[2][kRegR]) {
+  const int Lp = nr3 / R, total = Lp * g.nr2;
+#pragma unroll
+  for (int it = 0; it < 2; ++it) {
+    const int w = threadIdx.x + it * kYZ;
+    if (w < total) {
+      const int j = (int)fdiv(w, g.mnr2), y = w - j * g.nr2;
+      T2 v[R];
+      z_last_load<T2, R>(pl + y + j * g.pitch, j, Lp, g.pitch, tw, v);
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        const double re = v[q].x, im = v[q].y;
+        accr[it][q] += s1 * (re * re + im * im) + s2 * (re * re - im * im);
+      }
+    }
+  }
+}
+
+template <int R>
+__device__ inline void dens_reg_store(const Geo &g, int nr3, const double (&accr)[2][kRegR], double *dst) {
+  const int Lp = nr3 / R, total = Lp * g.nr2;
+#pragma unroll
+  for (int it = 0; it < 2; ++it) {
+    const int w = threadIdx.x + it * kYZ;
+    if (w < total) {
+      const int j = (int)fdiv(w, g.mnr2), y = w - j * g.nr2;
+#pragma unroll
+      for (int q = 0; q < R; ++q) dst[(j + q * Lp) * g.nr2 + y] = accr[it][q];
+    }
+  }
+}
+
 // ---- yz pass of WAVES_VPSI: psi(x,y,z) * V(r), back to the stripes ----------------
 // One workgroup per (x, band); vt = V as [x][z][y] in the work precision.
 template <typename T2, bool BIG>
@@ -619,14 +658,24 @@ __global__ void __launch_bounds__(kYZ) k_yz_vpsi(Geo g, Fft1D fy, Fft1D fz, cons
 
 // ---- yz pass of WAVES_DENSITY: sum over a group of bands ----------------------
 // part[group][x][z][y] = sum_b s1 |psi|^2 + s2 Re(psi^2) over the group's bands
-template <typename T2, bool BIG>
-__global__ void __launch_bounds__(kYZ) k_yz_dens(Geo g, Fft1D fy, Fft1D fz, const T2 *twy, const T2 *twz,
+// REG: the band sum in registers (z_last_dens_reg), LDS holds only the plane;
+// the FP32 form is held to 128 registers so that two workgroups share a SIMD set
+template <typename T2, bool REG> struct DensWaves { static constexpr int v = 1; };
+template <> struct DensWaves<float2, true> { static constexpr int v = 4; };
+template <typename T2, bool BIG, bool REG>
+__global__ void __launch_bounds__(kYZ) __attribute__((amdgpu_waves_per_eu(DensWaves<T2, REG>::v))) k_yz_dens(Geo g, Fft1D fy, Fft1D fz, const T2 *twy, const T2 *twz,
                                                  const T2 *stripes, int nb, int bpg, const double *occ2,
                                                  double *part) {
+  static_assert(!(REG && BIG), "the register band sum takes radices up to kRegR");
   extern __shared__ unsigned char smem_raw[];
   const int n23 = g.nr2 * g.nr3;
   double *acc = reinterpret_cast<double *>(smem_raw);
-  T2 *pl = reinterpret_cast<T2 *>(acc + n23);
+  T2 *pl = reinterpret_cast<T2 *>(REG ? acc : acc + n23);
+  double accr[2][kRegR];
+#pragma unroll
+  for (int it = 0; it < 2; ++it)
+#pragma unroll
+    for (int q = 0; q < kRegR; ++q) accr[it][q] = 0.0;
   T2 *ty = pl + g.pitch * g.nr3, *tz = ty + g.nr2;
   int *zpos = reinterpret_cast<int *>(tz + g.nr3);
   const int x = blockIdx.x, grp = blockIdx.y;
@@ -634,20 +683,33 @@ __global__ void __launch_bounds__(kYZ) k_yz_dens(Geo g, Fft1D fy, Fft1D fz, cons
   for (int i = threadIdx.x; i < g.nzp; i += kYZ) zpos[i] = g.zpos[i];
   for (int i = threadIdx.x; i < g.nr2; i += kYZ) ty[i] = twy[i];
   for (int i = threadIdx.x; i < g.nr3; i += kYZ) tz[i] = twz[i];
-  for (int i = threadIdx.x; i < n23; i += kYZ) acc[i] = 0.0;
+  if (!REG)
+    for (int i = threadIdx.x; i < n23; i += kYZ) acc[i] = 0.0;
   for (int band = bstart; band < bend; ++band) {
     const double s1 = occ2[2 * band], s2 = occ2[2 * band + 1];
     if (s1 == 0.0 && s2 == 0.0) continue;   // uniform across the workgroup
     __syncthreads();
     yz_forward<T2, BIG>(g, fy, fz, ty, tz, stripes + ((size_t)band * g.nr1 + x) * g.nstripe, pl, zpos);
     // each acc position belongs to one butterfly of a fixed thread in every band
+    if constexpr (REG) {
+#define PW_CALL(R) if constexpr (R <= kRegR) z_last_dens_reg<T2, R>(g, pl, g.nr3, tz, s1, s2, accr)
+      PW_RADIX_SWITCH(fz.radix[fz.nstage - 1], PW_CALL)
+#undef PW_CALL
+    } else {
 #define PW_CALL(R) z_last_dens<T2, R>(g, pl, g.nr3, tz, s1, s2, acc)
+      PW_RADIX_SWITCH(fz.radix[fz.nstage - 1], PW_CALL)
+#undef PW_CALL
+    }
+  }
+  double *dst = part + ((size_t)grp * g.nr1 + x) * n23;
+  if constexpr (REG) {
+#define PW_CALL(R) if constexpr (R <= kRegR) dens_reg_store<R>(g, g.nr3, accr, dst)
     PW_RADIX_SWITCH(fz.radix[fz.nstage - 1], PW_CALL)
 #undef PW_CALL
+  } else {
+    __syncthreads();
+    for (int i = threadIdx.x; i < n23; i += kYZ) dst[i] = acc[i];
   }
-  __syncthreads();
-  double *dst = part + ((size_t)grp * g.nr1 + x) * n23;
-  for (int i = threadIdx.x; i < n23; i += kYZ) dst[i] = acc[i];
 }
 
 // out[b][a] = (sum_{k<nsum} in[k][a][b]) for in as nsum x [A][B]; 32x32 LDS tiles
@@ -848,8 +910,12 @@ template <typename T2>
 int density_t(pw_plan *p, int nb, const double2 *psi, const double *occ2, double *rho, int target_blocks,
               hipStream_t s) {
   const size_t n23 = (size_t)p->nr2 * p->nr3;
-  const size_t lds = n23 * sizeof(double) + ((size_t)p->yzpitch * p->nr3 + p->nr2 + p->nr3) * sizeof(T2) +
-                     (size_t)p->nzp * sizeof(int);
+  // the band sum in registers when the last z stage has at most 2 kYZ butterflies
+  const Fft1D &fzl = p->fyz[1];
+  const int rl = fzl.radix[fzl.nstage - 1];
+  const bool reg = !yz_big(p) && rl <= kRegR && (size_t)(p->nr3 / rl) * p->nr2 <= 2 * (size_t)kYZ;
+  const size_t lds = (reg ? 0 : n23 * sizeof(double)) +
+                     ((size_t)p->yzpitch * p->nr3 + p->nr2 + p->nr3) * sizeof(T2) + (size_t)p->nzp * sizeof(int);
   PW_TRY(yz_fits(p, lds));
   Geo g = make_geo(p);
   PW_TRY(run_forward_x<T2>(p, g, nb, psi, s));
@@ -863,17 +929,17 @@ int density_t(pw_plan *p, int nb, const double2 *psi, const double *occ2, double
     PW_HIP(hipMalloc(&p->d_dens_part, need));
     p->dens_bytes = need;
   }
-  if (yz_big(p)) {
-    PW_TRY(yz_lds(k_yz_dens<T2, true>, lds));
-    k_yz_dens<T2, true><<<dim3(p->nr1, ngrp), kYZ, lds, s>>>(g, p->fyz[0], p->fyz[1], twid<T2>(p, 1),
-                                                             twid<T2>(p, 2), (const T2 *)p->d_stripe_buf, nb,
-                                                             bpg, occ2, (double *)p->d_dens_part);
-  } else {
-    PW_TRY(yz_lds(k_yz_dens<T2, false>, lds));
-    k_yz_dens<T2, false><<<dim3(p->nr1, ngrp), kYZ, lds, s>>>(g, p->fyz[0], p->fyz[1], twid<T2>(p, 1),
-                                                              twid<T2>(p, 2), (const T2 *)p->d_stripe_buf, nb,
-                                                              bpg, occ2, (double *)p->d_dens_part);
-  }
+#define PW_DENS(BIG, REG)                                                                                  \
+  do {                                                                                                     \
+    PW_TRY(yz_lds(k_yz_dens<T2, BIG, REG>, lds));                                                          \
+    k_yz_dens<T2, BIG, REG><<<dim3(p->nr1, ngrp), kYZ, lds, s>>>(g, p->fyz[0], p->fyz[1], twid<T2>(p, 1),  \
+                                                                 twid<T2>(p, 2), (const T2 *)p->d_stripe_buf, \
+                                                                 nb, bpg, occ2, (double *)p->d_dens_part);  \
+  } while (0)
+  if (yz_big(p)) PW_DENS(true, false);
+  else if (reg) PW_DENS(false, true);
+  else PW_DENS(false, false);
+#undef PW_DENS
   // rho[zy][x] = sum_group part[group][x][zy]
   const int A = p->nr1, B = (int)n23;
   dim3 grid((B + 31) / 32, (A + 31) / 32);

@@ -226,7 +226,7 @@ extern "C" int pw_waves_autotune(pw_waveset *ws) {
       if (t < tbest) { tbest = t; best_split = cand; }
     }
     tbest = 1e30;
-    for (int cand : {128, 192, 256, 384, 512, 1024}) {
+    for (int cand : {192, 256, 384, 448, 512, 1024}) {
       const double t = timed([&] {
         return fft_density(p, ws->nbh, ws->d_psi0, ws->d_occ2, ws->d_rho, ws->prec, ws->stream, cand);
       });
