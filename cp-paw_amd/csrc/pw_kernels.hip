@@ -53,6 +53,23 @@ __device__ inline double2 eigr_of(const ProjGeo &q, int atom, int ig) {
 }
 
 // z * (-i)^l
+// sum of v over the 16 lanes of a DPP row (lanes 16k .. 16k+15), in every lane
+// of the row: four rotate-and-add steps on the cross-lane (DPP) path of the
+// VALU (row_ror 8, 4, 2, 1), no LDS traffic
+template <int CTRL>
+__device__ inline double dpp_ror(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+__device__ inline double row16_sum(double v) {
+  v += dpp_ror<0x128>(v);
+  v += dpp_ror<0x124>(v);
+  v += dpp_ror<0x122>(v);
+  v += dpp_ror<0x121>(v);
+  return v;
+}
+
 __device__ inline double2 rot_mil(double2 z, int l) {
   switch (l & 3) {
     case 0: return z;
@@ -389,13 +406,11 @@ __global__ void __launch_bounds__(64 * NW) k_force_m(ProjGeo q, const int2 *ipta
       f[r][2] += cc * gz;
     }
   }
-  // sum over the 16 lanes of a row (fixed butterfly)
+  // sum over the 16 lanes of a row
 #pragma unroll
   for (int r = 0; r < 4; ++r)
 #pragma unroll
-    for (int d = 0; d < 3; ++d)
-#pragma unroll
-      for (int off = 8; off > 0; off >>= 1) f[r][d] += __shfl_xor(f[r][d], off);
+    for (int d = 0; d < 3; ++d) f[r][d] = row16_sum(f[r][d]);
   if (li != 0) return;
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
@@ -1182,9 +1197,7 @@ __global__ void __launch_bounds__(64 * NW) k_force_bf(GA ga, GB gb, int npro, in
 #pragma unroll
     for (int r = 0; r < 4; ++r)
 #pragma unroll
-      for (int d = 0; d < 3; ++d)
-#pragma unroll
-        for (int off = 8; off > 0; off >>= 1) f[r][d] += __shfl_xor(f[r][d], off);
+      for (int d = 0; d < 3; ++d) f[r][d] = row16_sum(f[r][d]);
     if (li == 0) {
 #pragma unroll
       for (int r = 0; r < 4; ++r)
