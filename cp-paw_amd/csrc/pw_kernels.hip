@@ -52,7 +52,6 @@ __device__ inline double2 eigr_of(const ProjGeo &q, int atom, int ig) {
   return cmul(e, ph[2 * q.nphase + iv[2] + q.nmaxx]);
 }
 
-// z * (-i)^l
 // sum of v over the 16 lanes of a DPP row (lanes 16k .. 16k+15), in every lane
 // of the row: four rotate-and-add steps on the cross-lane (DPP) path of the
 // VALU (row_ror 8, 4, 2, 1), no LDS traffic
@@ -70,6 +69,7 @@ __device__ inline double row16_sum(double v) {
   return v;
 }
 
+// z * (-i)^l
 __device__ inline double2 rot_mil(double2 z, int l) {
   switch (l & 3) {
     case 0: return z;
@@ -578,13 +578,22 @@ struct Split3 {
   __bf16 h, m, l;
 };
 
+// f rounded to bf16 (nearest, ties to even) on the integer path; returns the
+// bf16 value as a float, its 16 bits in b
+__device__ inline float bf16_rne(float f, __bf16 &b) {
+  unsigned u = __float_as_uint(f);
+  u = (u + 0x7FFFu + ((u >> 16) & 1u)) & 0xFFFF0000u;
+  b = __builtin_bit_cast(__bf16, (unsigned short)(u >> 16));
+  return __uint_as_float(u);
+}
+
+// x -> FP32 -> h + m + l, each a bf16 and each remainder exact in FP32
 __device__ inline Split3 split3(double x) {
   Split3 s;
   const float f = (float)x;
-  s.h = (__bf16)f;
-  const float r = f - (float)s.h;
-  s.m = (__bf16)r;
-  s.l = (__bf16)(r - (float)s.m);
+  const float r = f - bf16_rne(f, s.h);
+  const float q = r - bf16_rne(r, s.m);
+  bf16_rne(q, s.l);
   return s;
 }
 
@@ -619,6 +628,14 @@ __device__ inline flt4 rmfma3(const bf16x8 *x, const bf16x8 *y, flt4 c) {
 __device__ inline void cmfma_bf(flt4 &cr, flt4 &ci, const bf16x8 *a, const bf16x8 *an, const bf16x8 *b) {
   cr = rmfma3(a, b, cr);
   cr = rmfma3(an, b + 3, cr);
+  ci = rmfma3(a, b + 3, ci);
+  ci = rmfma3(a + 3, b, ci);
+}
+// the same with the sign on B's imaginary planes (bn = -Im b): for products
+// whose B operand is shared by the A tiles of a wave, negated once per chunk
+__device__ inline void cmfma_bf_nb(flt4 &cr, flt4 &ci, const bf16x8 *a, const bf16x8 *b, const bf16x8 *bn) {
+  cr = rmfma3(a, b, cr);
+  cr = rmfma3(a + 3, bn, cr);
   ci = rmfma3(a, b + 3, ci);
   ci = rmfma3(a + 3, b, ci);
 }
@@ -1065,16 +1082,16 @@ __global__ void __launch_bounds__(64 * NW) k_colgemm_bf(GA ga, GB gb, int nk, in
     }
     __syncthreads();
     if (k0 + KB < nk) gb.fetch(krun(k0 + KB), g0 + gs, pre);
-    bf16x8 bb[NPL];
+    bf16x8 bb[NPL], bn[3];
     load_planes(sB, WG, 16 * wv + li, lk, bb);
 #pragma unroll
-    for (int c = 0; c < NRB; ++c) {
-      bf16x8 a[NPL], an[3];
-      load_planes(sA, NB16, 16 * c + li, lk, a);
+    for (int q = 0; q < 3; ++q) bn[q] = bf_neg(bb[3 + q]);
 #pragma unroll
-      for (int q = 0; q < 3; ++q) an[q] = bf_neg(a[3 + q]);
+    for (int c = 0; c < NRB; ++c) {
+      bf16x8 a[NPL];
+      load_planes(sA, NB16, 16 * c + li, lk, a);
       flt4 cr = flt4{0.f, 0.f, 0.f, 0.f}, ci = flt4{0.f, 0.f, 0.f, 0.f};
-      cmfma_bf(cr, ci, a, an, bb);
+      cmfma_bf_nb(cr, ci, a, bb, bn);
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         accr[c][r] += (double)cr[r];
@@ -1155,15 +1172,15 @@ __global__ void __launch_bounds__(64 * NW) k_force_bf(GA ga, GB gb, int npro, in
     }
     __syncthreads();
     if (k0 + KB < nk) gb.fetch(k0 + KB + 8 * ks, g0 + gs, pre);
-    bf16x8 bb[NPL];
+    bf16x8 bb[NPL], bn[3];
     load_planes(sB, WG, 16 * wv + li, lk, bb);
 #pragma unroll
-    for (int c = 0; c < NRB; ++c) {
-      bf16x8 a[NPL], an[3];
-      load_planes(sA, NB16, 16 * c + li, lk, a);
+    for (int q = 0; q < 3; ++q) bn[q] = bf_neg(bb[3 + q]);
 #pragma unroll
-      for (int q = 0; q < 3; ++q) an[q] = bf_neg(a[3 + q]);
-      cmfma_bf(accr[c], acci[c], a, an, bb);
+    for (int c = 0; c < NRB; ++c) {
+      bf16x8 a[NPL];
+      load_planes(sA, NB16, 16 * c + li, lk, a);
+      cmfma_bf_nb(accr[c], acci[c], a, bb, bn);
     }
     __syncthreads();
   }
