@@ -496,17 +496,20 @@ __global__ void k_overlap1c_sum(int nbh, int nb, int nz, const double4 *part, do
 }
 
 // per-atom matrix on projections: out[ipro0+i][b] = sum_j M_atom[i][j] proj[ipro0+j][b]
-__global__ void k_atom_mat(ProjGeo q, const double2 *proj, const double2 *mats, double2 *out) {
+// grid (NAT, LMNXX*NBH/64), 64 threads: one output element per thread
+__global__ void __launch_bounds__(64) k_atom_mat(ProjGeo q, const double2 *proj, const double2 *mats,
+                                                 double2 *out) {
   const int atom = blockIdx.x;
   const int lmnx = q.lmnx[q.atom_isp[atom]];
+  const int w = blockIdx.y * 64 + threadIdx.x;
+  if (w >= lmnx * q.nbh) return;
   const int base = q.ipro0[atom];
   const double2 *m = mats + (size_t)atom * q.lmnxx * q.lmnxx;
-  for (int w = threadIdx.x; w < lmnx * q.nbh; w += blockDim.x) {
-    const int i = w / q.nbh, b = w % q.nbh;
-    double2 s = czero();
-    for (int j = 0; j < lmnx; ++j) s = cadd(s, cmul(m[i * q.lmnxx + j], proj[(size_t)(base + j) * q.nbh + b]));
-    out[(size_t)(base + i) * q.nbh + b] = s;
-  }
+  const int i = w / q.nbh, b = w % q.nbh;
+  double2 s = czero();
+#pragma unroll 8
+  for (int j = 0; j < lmnx; ++j) s = cadd(s, cmul(m[i * q.lmnxx + j], proj[(size_t)(base + j) * q.nbh + b]));
+  out[(size_t)(base + i) * q.nbh + b] = s;
 }
 
 // lambda (real NB x NB, row-major) -> LAMBDA1/LAMBDA2 element (i,j) of src/paw_waves2.f90:582-593
@@ -758,7 +761,9 @@ __global__ void __launch_bounds__(256) k_split_rows(Src src, int nrow, int nrun,
   const bool ok = id < (size_t)nrow * nrun;
   double2 v = make_double2(0.0, 0.0);
   if (ok) {
-    const int r = (int)(id / nrun), run = (int)(id % nrun);
+    // nrow * nrun < 2^32 (an operand of 2^32 runs would take 384 GiB): 32-bit quotients
+    const unsigned u = (unsigned)id;
+    const int r = (int)(u / (unsigned)nrun), run = (int)(u % (unsigned)nrun);
     v = src(r, 8 * run + e);
   }
   const Split3 a = split3(v.x), b = split3(v.y);
@@ -788,7 +793,8 @@ __global__ void __launch_bounds__(256) k_split_cols(Src src, int nrow, int nrun,
   const size_t n = (size_t)nrow * nrun;
   Planes8 o;
   if (idx < n) {
-    const int r = (int)(idx % nrow), run = (int)(idx / nrow);
+    const unsigned u = (unsigned)idx;   // < nrow * nrun < 2^32
+    const int r = (int)(u % (unsigned)nrow), run = (int)(u / (unsigned)nrow);
 #pragma unroll
     for (int e = 0; e < 8; ++e) put(o, e, src(r, 8 * run + e));
   } else {
@@ -1405,21 +1411,23 @@ __global__ void __launch_bounds__(kB) k_dot(size_t n, const double *a, const dou
   if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
+// grid (NGL/256, NBH): G along x, the band along y
 __global__ void k_propagate_g(int ngl, int nbh, double a1, double a2, double a3n,
                               const double *mpsi, const double2 *psi0, double2 *psim,
                               const double2 *hpsi) {
-  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= (size_t)ngl * nbh) return;
-  const int ig = idx % ngl;
+  const int ig = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ig >= ngl) return;
+  const size_t idx = (size_t)blockIdx.y * ngl + ig;
   const double a3 = a3n / mpsi[ig];
   const double2 p0 = psi0[idx], pm = psim[idx], h = hpsi[idx];
   psim[idx] = make_double2(a1 * p0.x + a2 * pm.x + a3 * h.x, a1 * p0.y + a2 * pm.y + a3 * h.y);
 }
 
 __global__ void k_scale_mass(int ngl, int nbh, double num, const double *mpsi, double2 *x) {
-  const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= (size_t)ngl * nbh) return;
-  const double f = num / mpsi[idx % ngl];
+  const int ig = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ig >= ngl) return;
+  const size_t idx = (size_t)blockIdx.y * ngl + ig;
+  const double f = num / mpsi[ig];
   x[idx] = make_double2(f * x[idx].x, f * x[idx].y);
 }
 
@@ -1825,7 +1833,7 @@ int k_overlap_1c(pw_waveset *ws, const double2 *proj1, const double2 *proj2, dou
   // the overlap matrix read from d_part (k_unravel_pw) is ahead on the stream
   PW_TRY(ws_part(ws, (size_t)nz * nbh * nbh * sizeof(double4)));
   ScopeTimer tm(ws, "overlap1c");
-  k_atom_mat<<<ws->nat, 256, 0, ws->stream>>>(q, proj2, ws->d_domat, ws->d_xproj);
+  k_atom_mat<<<dim3(ws->nat, (ws->lmnxx * nbh + 63) / 64), 64, 0, ws->stream>>>(q, proj2, ws->d_domat, ws->d_xproj);
   dim3 grid((nbh + 15) / 16, (nbh + 15) / 16, nz);
   k_overlap1c<<<grid, 256, 0, ws->stream>>>(ws->npro, nbh, proj1, ws->d_xproj, (double4 *)ws->d_part);
   k_overlap1c_sum<<<(nbh * nbh + 255) / 256, 256, 0, ws->stream>>>(nbh, ws->nb, nz, (const double4 *)ws->d_part,
@@ -1837,7 +1845,7 @@ int k_overlap_1c(pw_waveset *ws, const double2 *proj1, const double2 *proj2, dou
 int k_atom_matrix(pw_waveset *ws, const double2 *proj, double2 *out, const double2 *mats) {
   ProjGeo q = make_pgeo(ws);
   ScopeTimer tm(ws, "atom_mat");
-  k_atom_mat<<<ws->nat, 256, 0, ws->stream>>>(q, proj, mats, out);
+  k_atom_mat<<<dim3(ws->nat, (ws->lmnxx * ws->nbh + 63) / 64), 64, 0, ws->stream>>>(q, proj, mats, out);
   PW_HIP(hipGetLastError());
   return PW_OK;
 }
@@ -1938,9 +1946,8 @@ int k_propagate(pw_waveset *ws) {
   pw_plan *p = ws->plan;
   const double a1 = 2.0 / (1.0 + ws->annee), a2 = 1.0 - a1;
   const double a3n = -ws->delt * ws->delt / (1.0 + ws->annee);
-  const size_t n = (size_t)p->ngl * ws->nbh;
   ScopeTimer tm(ws, "propagate");
-  k_propagate_g<<<(n + 255) / 256, 256, 0, ws->stream>>>(p->ngl, ws->nbh, a1, a2, a3n, ws->d_mpsi,
+  k_propagate_g<<<dim3((p->ngl + 255) / 256, ws->nbh), 256, 0, ws->stream>>>(p->ngl, ws->nbh, a1, a2, a3n, ws->d_mpsi,
                                                          ws->d_psi0, ws->d_psim, ws->d_hpsi);
   PW_HIP(hipGetLastError());
   return PW_OK;
@@ -1948,9 +1955,8 @@ int k_propagate(pw_waveset *ws) {
 
 int k_opsi_scale(pw_waveset *ws, double2 *opsi) {
   pw_plan *p = ws->plan;
-  const size_t n = (size_t)p->ngl * ws->nbh;
   ScopeTimer tm(ws, "opsi_scale");
-  k_scale_mass<<<(n + 255) / 256, 256, 0, ws->stream>>>(p->ngl, ws->nbh,
+  k_scale_mass<<<dim3((p->ngl + 255) / 256, ws->nbh), 256, 0, ws->stream>>>(p->ngl, ws->nbh,
                                                         ws->delt * ws->delt / (1.0 + ws->annee),
                                                         ws->d_mpsi, opsi);
   PW_HIP(hipGetLastError());
