@@ -388,3 +388,51 @@ def test_fortran_driver_restart(tmp_path):
         assert abs(steps[it, 1] - e["etot"]) <= 1e-10 * abs(e["etot"]), it
         assert int(steps[it, 6]) == o.ortho_iter, it
         o.switch()
+
+
+def test_map_g_between_real_sphere_sets():
+    # the G spheres of two cutoffs and of a strained cell (the reference's
+    # sphere and order, oracle/planewave.py): every present vector inside the
+    # file's sphere maps to the file's vector with the same integer coordinates
+    from oracle import planewave as opw
+    from pwgpu import model
+    s = model.si8()
+    lo = opw.Plane(s.rbas, 6.0, 24.0)
+    hi = opw.Plane(s.rbas, 8.0, 32.0)
+    m = rs.map_g(lo.gbas, lo.gvec, hi.gvec)
+    key = {tuple(v): i + 1 for i, v in enumerate(lo.igvec.tolist())}
+    want = np.array([key.get(tuple(v), 0) for v in hi.igvec.tolist()])
+    assert np.array_equal(m, want) and np.count_nonzero(m) == lo.ngl
+    # down to the smaller sphere: a permutation-free subset
+    m2 = rs.map_g(hi.gbas, hi.gvec, lo.gvec)
+    assert np.array_equal(hi.igvec[m2 - 1], lo.igvec)
+    # a cell strained by 0.5 %: same integer set at the same cutoff maps fully
+    st = opw.Plane(s.rbas * 1.005, 6.0, 24.0)
+    m3 = rs.map_g(lo.gbas, lo.gvec, st.gvec)
+    assert np.array_equal(lo.igvec[m3[m3 > 0] - 1], st.igvec[m3 > 0])
+
+
+@pytest.mark.gpu
+def test_gpu_restart_across_cutoffs(tmp_path):
+    # a file written by a set at a lower cutoff, read into a set at a higher
+    # one: the common plane waves carry over, the new ones are zero
+    import pwgpu
+    from harness import SYSTEMS
+    s = SYSTEMS["si8"]()
+    print("case restart si8 cutoff 6 -> 8 Ha", flush=True)
+    plo = pwgpu.PlaneWave(s.rbas, 6.0, 24.0)
+    wlo = pwgpu.Waves.from_system(plo, s)
+    path = str(tmp_path / "lo.rstrt")
+    wlo.write_restart(path, static=True)
+    psi_lo = wlo.getc8a("PSI0")
+    ig_lo = plo.geti4a("IGVEC")
+    wlo.close()
+    phi = pwgpu.PlaneWave(s.rbas, 8.0, 32.0)
+    whi = pwgpu.Waves.from_system(phi, s)
+    assert whi.read_restart(path)
+    got = whi.getc8a("PSI0")
+    key = {tuple(v): i for i, v in enumerate(ig_lo.tolist())}
+    idx = np.array([key.get(tuple(v), -1) for v in phi.geti4a("IGVEC").tolist()])
+    assert np.array_equal(got[:, idx >= 0], psi_lo[:, idx[idx >= 0]])
+    assert np.all(got[:, idx < 0] == 0)
+    assert np.array_equal(whi.getc8a("PSIM"), got)
