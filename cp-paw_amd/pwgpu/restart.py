@@ -355,7 +355,7 @@ def save(waves, path, static=False):
     if not static:
         psi.append(waves.getc8a("PSIM"))
     nlambda = 1 if static else waves.geti4("NLAMBDA")
-    lam = [waves.getc8a("RLAM0"), waves.getc8a("RLAMM")][:nlambda]
+    lam = [waves.getc8a(key) for key in ("RLAM0", "RLAMM")[:nlambda]]
     k = KPointWaves(waves.nb, plane.tinv, relative_k(plane), plane.geti4a("IGVEC"),
                     np.stack(psi)[:, None, :, None, :],
                     np.array(lam, dtype=np.complex128).reshape(nlambda, 1, waves.nb, waves.nb))
