@@ -37,6 +37,9 @@ def gpu_waveset(system, prec=0):
 SYSTEMS = {
     "tiny": model.tiny,
     "si8": model.si8,
+    # Si8 compressed to a = 8.7 bohr: a 22^3 grid, so the y and z transforms take
+    # a radix-11 stage (the in-place kernels' second instantiation)
+    "si8c": lambda: model.si8(a=8.7),
     # two species with different projector sets (O: 2s2p, H: 2s), 12 atoms
     "h2o4": lambda: model.water(nmol=4, a=9.85, seed=3),
     # 256 bands, 64^3 grid, 512 projectors (the oracle takes ~11 s per etot)

@@ -38,7 +38,7 @@ def ortho_residual(o, psi):
     return float(np.max(np.abs(s - np.eye(o.nb))))
 
 
-@pytest.mark.parametrize("name,nsteps", [("tiny", 4), ("si8", 5), ("h2o4", 4), ("synthetic", 2),
+@pytest.mark.parametrize("name,nsteps", [("tiny", 4), ("si8", 5), ("si8c", 3), ("h2o4", 4), ("synthetic", 2),
                                           ("h2o32", 2)])
 def test_mixed_form_error_gates(name, nsteps):
     s = SYSTEMS[name]()

@@ -23,7 +23,7 @@ def say(msg):
     sys.stdout.flush()
 
 
-@pytest.mark.parametrize("name", ["tiny", "si8", "h2o4"])
+@pytest.mark.parametrize("name", ["tiny", "si8", "si8c", "h2o4"])
 @pytest.mark.parametrize("prec", [0, 1])
 def test_fft_gtor_rtog(name, prec):
     import pwgpu
@@ -46,7 +46,7 @@ def test_fft_gtor_rtog(name, prec):
     assert relerr(got, ref) < tol
 
 
-@pytest.mark.parametrize("name", ["tiny", "si8", "h2o4"])
+@pytest.mark.parametrize("name", ["tiny", "si8", "si8c", "h2o4"])
 def test_vpsi_density_projections(name):
     import pwgpu
     s = SYSTEMS[name]()
@@ -70,7 +70,7 @@ def test_vpsi_density_projections(name):
     assert relerr(w.getc8a("PROJ"), ref) < TOL_KERNEL
 
 
-@pytest.mark.parametrize("name", ["tiny", "si8", "h2o4"])
+@pytest.mark.parametrize("name", ["tiny", "si8", "si8c", "h2o4"])
 def test_etot_first_call(name):
     s = SYSTEMS[name]()
     say(f"case etot {name}")
@@ -92,7 +92,7 @@ def test_etot_first_call(name):
         assert relerr(dm[iat, :n, :n], ref) < 1e-11, iat
 
 
-@pytest.mark.parametrize("name,nsteps", [("tiny", 6), ("si8", 4), ("h2o4", 4), ("synthetic", 2),
+@pytest.mark.parametrize("name,nsteps", [("tiny", 6), ("si8", 4), ("si8c", 3), ("h2o4", 4), ("synthetic", 2),
                                           ("h2o32", 2)])
 def test_cp_steps(name, nsteps):
     s = SYSTEMS[name]()
