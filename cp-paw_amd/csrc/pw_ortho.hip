@@ -353,8 +353,11 @@ __global__ void __launch_bounds__(64 * kGemmWaves) k_gemm_pair(PairArgs a, Ortho
 // recursion; the sum is the same to rounding.
 constexpr int kSeriesX = 0, kSeriesLambda = 1;
 constexpr int kSeriesMaxLevels = 5;      // at most K = 2^5 terms as a series
-constexpr double kAcceptPower = 1.0e-4;  // acceptance: n max|A^(2^(J-1))| <= this
-constexpr double kSelectPower = 3.0e-5;  // level choice: estimated n rho^(2^(J-1)) <= this
+// acceptance: ||A^h||_inf = max row sum |A^h| <= this, h = 2^(J-1); A^h is
+// symmetric, so rho^h = ||A^h||_2 <= ||A^h||_inf and the truncated series' tail,
+// at most rho^(4h)/(1-rho^2) of X0, is below 1.6e-11 of it
+constexpr double kAcceptPower = 2.0e-3;
+constexpr double kSelectPower = 4.0e-4;  // level choice: estimated rho^h <= this (5x inside)
 
 struct SeriesArgs {
   int n;
@@ -504,7 +507,7 @@ __global__ void __launch_bounds__(256) k_series_sum(SumArgs a, OrthoState *st) {
     __syncthreads();
     p1 = nanmax(nanmax(wm[0], wm[1]), nanmax(wm[2], wm[3]));
     p2 = nanmax(nanmax(wp2[0], wp2[1]), nanmax(wp2[2], wp2[3]));
-    reject = !((double)a.n_check * p1 <= a.ptol_res && (double)a.n_check * p2 <= a.ptol_pow);
+    reject = !((double)a.n_check * p1 <= a.ptol_res && p2 <= a.ptol_pow);
   }
   // the convergence test of iteration it (src/paw_waves2.f90:1960-1975):
   // every workgroup reaches the same decision; the first one records it
@@ -699,30 +702,33 @@ __global__ void __launch_bounds__(kLzThreads) k_lanczos(int n, int m, const doub
 // Newton-Schulz residual R = (B+qI)X - I of the last step (the returned M has
 // residual R^2) must be small and so must the last power A^(2^(J-1)), so that
 // the truncated series is exact to rounding; otherwise code 3 makes the host
-// recompute the bounds.  k_prep_part forms block maxima (NaN propagates); the
-// first iteration's k_series_sum decides.
-
-__global__ void k_prep_part(int n, const double *T, const double *AJ, double *bmax) {
-  __shared__ double r1[256], r2[256];
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  double m1 = 0.0, m2 = 0.0;
-  if (idx < n * n) {
-    m1 = fabs(T[idx] - ((idx / n == idx % n) ? 1.0 : 0.0));
-    m2 = fabs(AJ[idx]);
-  }
-  r1[threadIdx.x] = m1;
-  r2[threadIdx.x] = m2;
-  __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) {
-      r1[threadIdx.x] = nanmax(r1[threadIdx.x], r1[threadIdx.x + s]);
-      r2[threadIdx.x] = nanmax(r2[threadIdx.x], r2[threadIdx.x + s]);
+// recompute the bounds.  k_prep_part: one wave per row i forms max_j |R(i,j)|
+// and the row sum sum_j |A^h(i,j)|, a workgroup (4 rows) their maxima (NaN
+// propagates); the first iteration's k_series_sum decides.
+__global__ void __launch_bounds__(256) k_prep_part(int n, const double *T, const double *AJ, double *bmax) {
+  __shared__ double r1[4], r2[4];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int i = blockIdx.x * 4 + wv;
+  double m1 = 0.0, s2 = 0.0;
+  if (i < n) {
+    for (int j = lane; j < n; j += 64) {
+      m1 = nanmax(m1, fabs(T[(size_t)i * n + j] - (i == j ? 1.0 : 0.0)));
+      s2 += fabs(AJ[(size_t)i * n + j]);
     }
-    __syncthreads();
   }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    m1 = nanmax(m1, __shfl_xor(m1, o));
+    s2 += __shfl_xor(s2, o);
+  }
+  if (lane == 0) {
+    r1[wv] = m1;
+    r2[wv] = s2;
+  }
+  __syncthreads();
   if (threadIdx.x == 0) {
-    bmax[2 * blockIdx.x] = r1[0];
-    bmax[2 * blockIdx.x + 1] = r2[0];
+    bmax[2 * blockIdx.x] = nanmax(nanmax(r1[0], r1[1]), nanmax(r1[2], r1[3]));
+    bmax[2 * blockIdx.x + 1] = nanmax(nanmax(r2[0], r2[1]), nanmax(r2[2], r2[3]));
   }
 }
 
@@ -763,7 +769,9 @@ struct OrthoWork {
 
 int ortho_work(pw_waveset *ws, OrthoWork &w) {
   const size_t n2 = (size_t)ws->nb * ws->nb;
-  const size_t nblk = (n2 + 255) / 256;
+  // block-maxima slots: (n^2+255)/256 for the GAMN maxima, (n+3)/4 row groups
+  // for the acceptance test (k_prep_part)
+  const size_t nblk = std::max((n2 + 255) / 256, ((size_t)ws->nb + 3) / 4);
   const size_t kser = (size_t)1 << kSeriesMaxLevels;
   const size_t need = (9 + kMaxLevels + 2 * kser) * n2 + ws->nb + 4 * nblk + 64;
   if (need > ws->ortho_cap) {
@@ -901,7 +909,7 @@ int enqueue_prep(pw_waveset *ws, const OrthoWork &w, const OrthoParams &P) {
   }
   for (int j = ls; j + 1 < P.nlev; ++j) gemm(ws, w.apow[j], 0, w.apow[j], 0, w.apow[j + 1]);
   // acceptance maxima; the test itself runs in the first iteration's k_series_sum
-  k_prep_part<<<nblk, 256, 0, ws->stream>>>(n, w.t1, w.apow[P.nlev - 1], w.pred);
+  k_prep_part<<<(n + 3) / 4, 256, 0, ws->stream>>>(n, w.t1, w.apow[P.nlev - 1], w.pred);
   return PW_OK;
 }
 
@@ -959,7 +967,7 @@ int enqueue_iters(pw_waveset *ws, const OrthoWork &w, const OrthoParams &P, doub
       // Newton-Schulz residuals square per step: n max|R_k| <= 3.16e-6 for
       // the last T = (B+qI) X_k bounds the returned M's residual by 1e-11
       su.pred = w.pred;
-      su.npred = (int)(((size_t)n * n + 255) / 256);
+      su.npred = (n + 3) / 4;
       su.ptol_res = 3.16e-6;
       su.ptol_pow = kAcceptPower;
       su.n_check = n;
@@ -1044,14 +1052,14 @@ int ortho_x_device(pw_waveset *ws, double *d_lam, const double *d_chichi, const 
     const double kap = P.hi / P.lo;
     const double rho = (std::sqrt(kap) - 1.0) / (std::sqrt(kap) + 1.0);
     // S^-1 sums A^k X0 A^k for k < K = 2^J.  The acceptance test certifies
-    // h = 2^(J-1) by n max|A^h| <= kAcceptPower = 1e-4; then rho^h <=
-    // ||A^h||_2 <= n max|A^h| and the tail, at most rho^(2K)/(1-rho^2) of X0,
-    // is below 1e-16 of it.  J is the least level count whose estimate from
-    // the (widened) Lanczos bounds, n rho^h, is 3x inside the test; a second
+    // h = 2^(J-1) by ||A^h||_inf <= kAcceptPower = 2e-3; then rho^h =
+    // ||A^h||_2 <= ||A^h||_inf and the tail, at most rho^(2K)/(1-rho^2) of X0,
+    // is below 1.6e-11 of it.  J is the least level count whose estimate from
+    // the (widened) Lanczos bounds, rho^h, is 5x inside the test; a second
     // attempt (bounds rejected once) adds a level.
     P.nlev = 1;
     if (rho > 1.0e-300) {
-      const double need = std::log(kSelectPower / n) / std::log(rho);
+      const double need = std::log(kSelectPower) / std::log(rho);
       P.nlev = std::max(1, (int)std::ceil(std::log2(std::max(need, 1.0))) + 1);
     }
     P.nlev += attempt;
