@@ -112,11 +112,11 @@ def accuracy(system_name, nsteps):
             "gates": {"dE_per_atom": 1e-6, "dF": 1e-4}, "pass": bool(de < 1e-6 and df < 1e-4)}
 
 
-def serving(nreq, rate, slots, seed=7):
+def serving(nreq, rate, slots, seed=7, reference=True):
     """Poisson stream of independent relaxations (Si8/16/32/64, seeded
     displacements) through the request loop in the mixed form: req/s,
-    p50/p99 latency (submit to result) and, per request, |dE| per atom
-    against the FP64 form run to the same step."""
+    p50/p99 latency (submit to result) and, per request (reference=True),
+    |dE| per atom against the FP64 form run to the same step."""
     import numpy as np
     from pwgpu import model, serve
     rng = np.random.default_rng(seed)
@@ -135,12 +135,17 @@ def serving(nreq, rate, slots, seed=7):
     res = [srv.wait(i) for i in ids]
     wall = time.perf_counter() - t0
     st = srv.stats()
+    lat = np.array([r["t_end"] - r["t_submit"] for r in res])
+    if not reference:
+        srv.close()
+        return {"arrival_rate_per_s": rate, "requests": nreq, "req_per_s": nreq / wall,
+                "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
+                "converged": int(sum(r["converged"] for r in res)), "peak_active": st["peak_active"]}
     # FP64 form of every request, to the step the mixed form stopped at
     ref = [srv.submit(c, prec=0, arrays=a, friction=opts["friction"], etol=0.0, nconv=1, max_steps=r["steps"])
            for c, a, r in zip(cells, arrays, res)]
     ref = [srv.wait(i) for i in ref]
     srv.close()
-    lat = np.array([r["t_end"] - r["t_submit"] for r in res])
     de = [abs(r["etot"] - f["etot"]) / c.nat for r, f, c in zip(res, ref, cells)]
     return {"requests": nreq, "arrival_rate_per_s": rate, "slots": slots, "form": "mixed",
             "req_per_s": nreq / wall, "p50_latency_s": float(np.percentile(lat, 50)),
@@ -314,6 +319,8 @@ def main():
                     help="requests of the serving measurement (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=4.0, help="Poisson arrival rate, requests/s")
     ap.add_argument("--serve-slots", type=int, default=4, help="concurrent requests on the GPU")
+    ap.add_argument("--serve-sweep", default="1,4,16,50",
+                    help="arrival rates (req/s) of the sweep, 8 requests each; empty for none")
     ap.add_argument("--profile", action="store_true", help="print per-kernel times to stderr")
     ap.add_argument("--pot-calls", type=int, default=20,
                     help="calls of the density -> potential measurement (0: skip)")
@@ -375,6 +382,15 @@ def main():
         log(f"serving: {args.serve_requests} requests at {args.serve_rate}/s, {args.serve_slots} slots ...")
         srv = serving(args.serve_requests, args.serve_rate, args.serve_slots)
         log(f"serving: {srv['req_per_s']:.2f} req/s p50 {srv['p50_latency_s']:.2f}s p99 {srv['p99_latency_s']:.2f}s")
+        rates = [float(x) for x in args.serve_sweep.split(",") if x.strip()]
+        if rates:
+            # the arrival-rate sweep: 8 requests per rate (p99 is then the slowest request)
+            srv["sweep"] = []
+            for k, rate in enumerate(rates):
+                pt = serving(8, rate, args.serve_slots, seed=11 + k, reference=False)
+                srv["sweep"].append(pt)
+                log(f"serving sweep {rate:g}/s: {pt['req_per_s']:.2f} req/s p50 {pt['p50_latency_s']:.2f}s "
+                    f"p99 {pt['p99_latency_s']:.2f}s")
     potm = None
     if args.pot_calls > 0:
         potm = potential_measure(args.system, args.pot_calls, max(5, min(20, args.steps)), not args.no_cpu)
