@@ -188,8 +188,8 @@ def run_gpu(system_name, prec, steps, warmup, profile):
                    f"{s.npro} projectors"}
     if profile:
         names = ["projections", "addpro", "force", "overlap", "overlap1c", "addopsi", "addoproj",
-                 "vpsi", "density", "propagate", "opsi_scale", "ortho_x", "ekin", "wavekinetic",
-                 "expect", "denmat", "atom_mat", "phase", "reduce", "vrho", "split", "pgen"]
+                 "vpsi", "density", "propagate", "opsi_scale", "ortho_x", "etot_sums", "wavekinetic",
+                 "denmat", "atom_mat", "phase", "reduce", "vrho", "split", "pgen"]
         kt = {}
         for n in names:
             ms, cnt = w.kernel_time(n)

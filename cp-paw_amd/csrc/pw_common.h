@@ -306,9 +306,9 @@ int k_fold_occ(pw_waveset *ws, double2 *x);
 int red_blocks(const pw_waveset *ws);
 int expect_chunks(const pw_waveset *ws);
 int vrho_blocks(const pw_waveset *ws);
-int k_ekin(pw_waveset *ws, const double2 *psi, const double *fpm_dev, double *dst);
 int k_wavekinetic(pw_waveset *ws, const double2 *psi0, const double2 *psim, const double *fab_dev, double *dst);
-int k_expect(pw_waveset *ws, const double2 *psi, const double2 *hpsi, double *dst);
+// EKIN, WAVEKINETIC (first half) and <psi|H|psi> of WAVES$ETOT in one pass
+int k_etot_sums(pw_waveset *ws, double *ekin, double *wk, double *expect);
 int k_propagate(pw_waveset *ws);
 int k_opsi_scale(pw_waveset *ws, double2 *opsi);
 int k_copy(pw_waveset *ws, double2 *dst, const double2 *src, size_t n);

@@ -1308,30 +1308,6 @@ __device__ inline double block_sum(double v, double *red) {
 // kBandGrp bands (y), one partial sum per workgroup.
 constexpr int kBandGrp = 8;
 
-// ekin partial per block: sum_G g2 * sum_b [fp |c|^2 + fm Re(c(G) c(-G))]
-__global__ void __launch_bounds__(kB) k_ekin(int ngl, int nbh, const double *g2, const int *minusg,
-                                             const double *fpm, const double2 *psi, double *part) {
-  __shared__ double red[kB / 64];
-  double s = 0.0;
-  const int b0 = blockIdx.y * kBandGrp, b1 = min(nbh, b0 + kBandGrp);
-  for (int ig = blockIdx.x * kB + threadIdx.x; ig < ngl; ig += gridDim.x * kB) {
-    double d = 0.0;
-    const int mg = minusg[ig];
-    for (int b = b0; b < b1; ++b) {
-      const double fp = fpm[2 * b], fm = fpm[2 * b + 1];
-      const double2 c = psi[(size_t)b * ngl + ig];
-      if (fp != 0.0) d += fp * (c.x * c.x + c.y * c.y);
-      if (fm != 0.0) {
-        const double2 cm = psi[(size_t)b * ngl + mg];
-        d += fm * (c.x * cm.x - c.y * cm.y);
-      }
-    }
-    s += g2[ig] * d;
-  }
-  s = block_sum<kB>(s, red);
-  if (threadIdx.x == 0) part[blockIdx.y * gridDim.x + blockIdx.x] = s;
-}
-
 // wave kinetic partial: sum_G m(G) sum_b [fa |d|^2 + fb (dr dr(-G) - di di(-G))]
 __global__ void __launch_bounds__(kB) k_wavekin(int ngl, int nbh, const double *mpsi,
                                                 const int *minusg, const double *fab,
@@ -1360,47 +1336,117 @@ __global__ void __launch_bounds__(kB) k_wavekin(int ngl, int nbh, const double *
   if (threadIdx.x == 0) part[blockIdx.y * gridDim.x + blockIdx.x] = s;
 }
 
-// <psi|H psi> per super band, partial over the G chunk blockIdx.x:
-// out[chunk][b] = (sum conj(psi) hpsi, sum psi hpsi(-G))
-__global__ void __launch_bounds__(kB) k_expect(int ngl, int nbh, int chunk, const int *minusg,
-                                               const double2 *psi, const double2 *hpsi, double *out) {
-  __shared__ double red[kB / 64];
-  const int b = blockIdx.y;
-  const double2 *p = psi + (size_t)b * ngl;
-  const double2 *h = hpsi + (size_t)b * ngl;
-  double tr = 0.0, tmr = 0.0;
-  const int g1 = min(ngl, (blockIdx.x + 1) * chunk);
-  for (int ig = blockIdx.x * chunk + threadIdx.x; ig < g1; ig += kB) {
-    const double2 a = p[ig], c = h[ig], cm = h[minusg[ig]];
-    tr += a.x * c.x + a.y * c.y;
-    tmr += a.x * cm.x - a.y * cm.y;
+// The three reductions of WAVES$ETOT over psi0 in one pass, so that psi0 is
+// read once: grid (G blocks, band groups of kBandGrp) as k_wavekin, whose
+// per-block partial layout it keeps for the kinetic energy
+//   ekin: sum_G g2 * sum_b [fp |c|^2 + fm Re(c(G) c(-G))]
+// and the wave kinetic energy (the first half, as k_wavekin), plus the
+// <psi|H psi> pair of each band of the group
+//   expect[2*(blockIdx.x*nbh + b)] = (sum conj(psi) hpsi, sum psi hpsi(-G)).
+// Per G the band sums of ekin and the wave kinetic energy run in band order.
+__global__ void __launch_bounds__(kB) k_etot_sums(int ngl, int nbh, const double *g2, const double *mpsi,
+                                                  const int *minusg, const double *fpm, const double *fab,
+                                                  const double2 *p0, const double2 *pm, const double2 *hp,
+                                                  double *ekin, double *wk, double *expect) {
+  constexpr int kV = 2 + 2 * kBandGrp;
+  __shared__ double red[kV][kB / 64];
+  const int b0 = blockIdx.y * kBandGrp, nbg = min(kBandGrp, nbh - b0);
+  double se = 0.0, sw = 0.0, tr[kBandGrp], tmr[kBandGrp];
+#pragma unroll
+  for (int k = 0; k < kBandGrp; ++k) tr[k] = tmr[k] = 0.0;
+  for (int ig = blockIdx.x * kB + threadIdx.x; ig < ngl; ig += gridDim.x * kB) {
+    double d = 0.0, e = 0.0;
+    const int mg = minusg[ig];
+#pragma unroll
+    for (int k = 0; k < kBandGrp; ++k) {
+      if (k >= nbg) break;
+      const int b = b0 + k;
+      const size_t o = (size_t)b * ngl;
+      const double2 c = p0[o + ig], h = hp[o + ig], hm = hp[o + mg];
+      tr[k] += c.x * h.x + c.y * h.y;
+      tmr[k] += c.x * hm.x - c.y * hm.y;
+      const double fp = fpm[2 * b], fm = fpm[2 * b + 1];
+      if (fp != 0.0) d += fp * (c.x * c.x + c.y * c.y);
+      if (fm != 0.0) {
+        const double2 cm = p0[o + mg];
+        d += fm * (c.x * cm.x - c.y * cm.y);
+      }
+      const double fa = fab[2 * b], fb = fab[2 * b + 1];
+      if (fa == 0.0 && fb == 0.0) continue;
+      const double2 x = csub(c, pm[o + ig]);
+      e += fa * (x.x * x.x + x.y * x.y);
+      if (fb != 0.0) {
+        const double2 y = csub(p0[o + mg], pm[o + mg]);
+        e += fb * (x.x * y.x - x.y * y.y);
+      }
+    }
+    se += g2[ig] * d;
+    sw += mpsi[ig] * e;
   }
-  tr = block_sum<kB>(tr, red);
-  tmr = block_sum<kB>(tmr, red);
-  if (threadIdx.x == 0) {
-    double *o = out + 2 * ((size_t)blockIdx.x * nbh + b);
-    o[0] = tr;
-    o[1] = tmr;
+  double v[kV];
+  v[0] = se;
+  v[1] = sw;
+#pragma unroll
+  for (int k = 0; k < kBandGrp; ++k) {
+    v[2 + 2 * k] = tr[k];
+    v[3 + 2 * k] = tmr[k];
+  }
+  const int w = threadIdx.x / 64, l = threadIdx.x % 64;
+#pragma unroll
+  for (int k = 0; k < kV; ++k) {
+    double t = v[k];
+    for (int off = 32; off > 0; off >>= 1) t += __shfl_down(t, off, 64);
+    if (l == 0) red[k][w] = t;
+  }
+  __syncthreads();
+  if (threadIdx.x < kV) {
+    const int k = threadIdx.x;
+    double t = 0.0;
+    for (int q = 0; q < kB / 64; ++q) t += red[k][q];
+    if (k == 0) ekin[blockIdx.y * gridDim.x + blockIdx.x] = t;
+    else if (k == 1) wk[blockIdx.y * gridDim.x + blockIdx.x] = t;
+    else if ((k - 2) / 2 < nbg) expect[2 * ((size_t)blockIdx.x * nbh + b0 + (k - 2) / 2) + (k & 1)] = t;
   }
 }
 
-// one-centre density matrix (real part, before hermitisation) per atom
-__global__ void k_denmat(ProjGeo q, const double *occ2, const double2 *proj, double *dm) {
+// one-centre density matrix (real part, before hermitisation) per atom.
+// Grid (NAT, entry groups of blockDim.x): entry w = i*LMNXX + j of the atom's
+// LMNXX x LMNXX block.  The atom's projections PROJ(B, BASE+I) are staged in
+// LDS by chunks of BC bands (row pitch BC+1 double2, so the rows j of one
+// wave fall in different banks) with the occupations beside them; each entry
+// sums over the bands in order 0..NBH-1.  Dynamic LDS: (LMNXX*(BC+1)+BC)*16 B.
+__global__ void __launch_bounds__(kB) k_denmat(ProjGeo q, int bc, const double *occ2, const double2 *proj,
+                                               double *dm) {
+  extern __shared__ double2 sp[];
   const int atom = blockIdx.x;
   const int lmnx = q.lmnx[q.atom_isp[atom]];
   const int base = q.ipro0[atom];
-  for (int w = threadIdx.x; w < q.lmnxx * q.lmnxx; w += blockDim.x) {
-    const int i = w / q.lmnxx, j = w % q.lmnxx;
-    double s = 0.0;
-    for (int b = 0; b < q.nbh && i < lmnx && j < lmnx; ++b) {
-      const double s1 = occ2[2 * b], s2 = occ2[2 * b + 1];
-      const double2 pi = proj[(size_t)(base + i) * q.nbh + b];
-      const double2 pj = proj[(size_t)(base + j) * q.nbh + b];
-      // Re(pi (s1 conj(pj) + s2 pj))
-      s += s1 * (pi.x * pj.x + pi.y * pj.y) + s2 * (pi.x * pj.x - pi.y * pj.y);
+  const int L = q.lmnxx, pitch = bc + 1;
+  double2 *so = sp + (size_t)L * pitch;
+  const int w = blockIdx.y * blockDim.x + threadIdx.x;
+  const int i = w / L, j = w % L;
+  const bool on = w < L * L && i < lmnx && j < lmnx;
+  double s = 0.0;
+  for (int b0 = 0; b0 < q.nbh; b0 += bc) {
+    const int nbc = min(bc, q.nbh - b0);
+    __syncthreads();
+    for (int t = threadIdx.x; t < lmnx * nbc; t += blockDim.x) {
+      const int r = t / nbc, c = t - r * nbc;
+      sp[r * pitch + c] = proj[(size_t)(base + r) * q.nbh + b0 + c];
     }
-    dm[((size_t)atom * q.lmnxx + i) * q.lmnxx + j] = s;
+    for (int t = threadIdx.x; t < nbc; t += blockDim.x)
+      so[t] = make_double2(occ2[2 * (b0 + t)], occ2[2 * (b0 + t) + 1]);
+    __syncthreads();
+    if (on) {
+      const double2 *ri = sp + i * pitch, *rj = sp + j * pitch;
+      for (int c = 0; c < nbc; ++c) {
+        const double2 o = so[c], pi = ri[c], pj = rj[c];
+        // Re(pi (s1 conj(pj) + s2 pj))
+        s += o.x * (pi.x * pj.x + pi.y * pj.y) + o.y * (pi.x * pj.x - pi.y * pj.y);
+      }
+    }
   }
+  if (w < L * L) dm[(size_t)atom * L * L + w] = s;
 }
 
 __global__ void __launch_bounds__(kB) k_dot(size_t n, const double *a, const double *b, double *part) {
@@ -1894,18 +1940,9 @@ static int band_groups(int nbh) { return (nbh + kBandGrp - 1) / kBandGrp; }
 // Reductions of the step leave their partial sums on the device (dst); the
 // host reads them back with one copy per step section (see pw_waves.cpp).
 int red_blocks(const pw_waveset *ws) { return reduce_gblocks(ws->plan) * band_groups(ws->nbh); }
-// G chunks of the <psi|H|psi> partials: about four G per thread
-int expect_chunks(const pw_waveset *ws) { return std::min(64, (ws->plan->ngl + 4 * kB - 1) / (4 * kB)); }
+// G chunks of the <psi|H|psi> partials: the G blocks of k_etot_sums
+int expect_chunks(const pw_waveset *ws) { return reduce_gblocks(ws->plan); }
 int vrho_blocks(const pw_waveset *ws) { return std::min(1024, (ws->plan->nrl + kB - 1) / kB); }
-
-int k_ekin(pw_waveset *ws, const double2 *psi, const double *fpm_dev, double *dst) {
-  pw_plan *p = ws->plan;
-  ScopeTimer tm(ws, "ekin");
-  const dim3 grid(reduce_gblocks(p), band_groups(ws->nbh));
-  k_ekin<<<grid, kB, 0, ws->stream>>>(p->ngl, ws->nbh, p->d_g2, p->d_minusg, fpm_dev, psi, dst);
-  PW_HIP(hipGetLastError());
-  return PW_OK;
-}
 
 int k_wavekinetic(pw_waveset *ws, const double2 *psi0, const double2 *psim, const double *fab_dev,
                   double *dst) {
@@ -1917,11 +1954,12 @@ int k_wavekinetic(pw_waveset *ws, const double2 *psi0, const double2 *psim, cons
   return PW_OK;
 }
 
-int k_expect(pw_waveset *ws, const double2 *psi, const double2 *hpsi, double *dst) {
+int k_etot_sums(pw_waveset *ws, double *ekin, double *wk, double *expect) {
   pw_plan *p = ws->plan;
-  ScopeTimer tm(ws, "expect");
-  const int nc = expect_chunks(ws), chunk = (p->ngl + nc - 1) / nc;
-  k_expect<<<dim3(nc, ws->nbh), kB, 0, ws->stream>>>(p->ngl, ws->nbh, chunk, p->d_minusg, psi, hpsi, dst);
+  ScopeTimer tm(ws, "etot_sums");
+  const dim3 grid(reduce_gblocks(p), band_groups(ws->nbh));
+  k_etot_sums<<<grid, kB, 0, ws->stream>>>(p->ngl, ws->nbh, p->d_g2, ws->d_mpsi, p->d_minusg, ws->d_fpm, ws->d_fab,
+                                           ws->d_psi0, ws->d_psim, ws->d_hpsi, ekin, wk, expect);
   PW_HIP(hipGetLastError());
   return PW_OK;
 }
@@ -1929,7 +1967,13 @@ int k_expect(pw_waveset *ws, const double2 *psi, const double2 *hpsi, double *ds
 int k_denmat(pw_waveset *ws, const double2 *proj, double *dst) {
   ProjGeo q = make_pgeo(ws);
   ScopeTimer tm(ws, "denmat");
-  k_denmat<<<ws->nat, 256, 0, ws->stream>>>(q, ws->d_occ2, proj, dst);
+  // band chunk: the staged rows and occupations within 64 KB of LDS
+  int bc = 64;
+  while (bc > 1 && ((size_t)q.lmnxx * (bc + 1) + bc) * sizeof(double2) > 65536) bc /= 2;
+  const size_t lds = ((size_t)q.lmnxx * (bc + 1) + bc) * sizeof(double2);
+  if (lds > 65536) return pw_fail(PW_ERR_ARG, "WAVES$DENMAT: LMNXX too large");
+  const dim3 grid(ws->nat, (q.lmnxx * q.lmnxx + kB - 1) / kB);
+  k_denmat<<<grid, kB, lds, ws->stream>>>(q, bc, ws->d_occ2, proj, dst);
   PW_HIP(hipGetLastError());
   return PW_OK;
 }

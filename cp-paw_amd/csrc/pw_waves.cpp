@@ -733,8 +733,7 @@ extern "C" int pw_waves_etot(pw_waveset *ws) {
     PW_TRY(pw_waves_projections(ws));
     ws->first = false;
   }
-  // WAVES_EKIN, WAVES$DENMAT
-  PW_TRY(k_ekin(ws, ws->d_psi0, ws->d_fpm, ws->d_res + ws->r_ekin));
+  // WAVES$DENMAT
   PW_TRY(k_denmat(ws, ws->d_proj, ws->d_res + ws->r_denmat));
   // WAVES_DENSITY
   {
@@ -763,9 +762,8 @@ extern "C" int pw_waves_etot(pw_waveset *ws) {
   }
   PW_TRY(k_atom_matrix(ws, ws->d_proj, ws->d_xproj, ws->d_dhmat));
   PW_TRY(k_addpro(ws, ws->d_hpsi, ws->d_phase, ws->d_xproj));
-  // <psi|H|psi>, WAVES_WAVEKINETIC (first half)
-  PW_TRY(k_expect(ws, ws->d_psi0, ws->d_hpsi, ws->d_res + ws->r_expect));
-  PW_TRY(k_wavekinetic(ws, ws->d_psi0, ws->d_psim, ws->d_fab, ws->d_res + ws->r_wk));
+  // WAVES_EKIN, <psi|H|psi>, WAVES_WAVEKINETIC (first half): one pass over psi0
+  PW_TRY(k_etot_sums(ws, ws->d_res + ws->r_ekin, ws->d_res + ws->r_wk, ws->d_res + ws->r_expect));
   PW_TRY(fetch_results(ws));
   const double *hr = ws->h_res;
   ws->ekin = 0.5 * sum_of(hr + ws->r_ekin, red_blocks(ws)) * p->vol;
