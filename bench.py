@@ -32,12 +32,18 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def kernel_counts(system, plane, nb):
-    """Algorithmic work per launch of the step's kernels."""
+def kernel_counts(system, plane, nb, prec):
+    """Algorithmic work per launch of the step's kernels.
+
+    Complex GEMM M x N x K: 8 M N K flops (split-BF16x3 credited at the same
+    count).  Grid passes at the element sizes of the form: the FP64 form
+    transforms complex128 with float64 V and density, the mixed form complex64
+    with float32 V and density.  The coefficients are complex128 in both."""
     nbh = (nb + 1) // 2
     npro = system.npro
     ngl = plane.ngl
     nr = plane.nrl
+    c, r = (16.0, 8.0) if prec == 0 else (8.0, 4.0)
     return {
         "projections": ("flop", 8.0 * npro * nbh * ngl),
         "addpro": ("flop", 8.0 * npro * nbh * ngl),
@@ -45,9 +51,9 @@ def kernel_counts(system, plane, nb):
         "overlap": ("flop", 2 * 8.0 * nbh * nbh * ngl),
         "addopsi": ("flop", 2 * 8.0 * nbh * nbh * ngl),
         # V.psi for all super bands: N (2 sizeof(c) + sizeof(r)) per band
-        "vpsi": ("byte", nbh * nr * (2 * 16.0 + 8.0)),
+        "vpsi": ("byte", nbh * nr * (2 * c + r)),
         # density: N (sizeof(c) + 2 sizeof(r)) per band
-        "density": ("byte", nbh * nr * (16.0 + 2 * 8.0)),
+        "density": ("byte", nbh * nr * (c + 2 * r)),
         "propagate": ("byte", 4.0 * nbh * ngl * 16.0),
     }
 
@@ -70,46 +76,109 @@ def kernel_peak(name, prec):
     return HBM_PEAK_GBS, "hbm"
 
 
-def accuracy(system_name, nsteps):
-    """Error of the mixed form against the FP64 form on the Si64 trajectory.
+def _copy_state(src, dst):
+    """PSI0, PSIM and the multiplier history of one set into another."""
+    dst.setc8a("PSI0", src.getc8a("PSI0"))
+    dst.setc8a("PSIM", src.getc8a("PSIM"))
+    dst.set_matrix_c("RLAM0", src.getc8a("RLAM0"))
+    if src.geti4("NLAMBDA") >= 2:
+        dst.set_matrix_c("RLAMM", src.getc8a("RLAMM"))
 
-    per step: at every step of an FP64 trajectory the mixed waveset is given
-    the same state (PSI0, PSIM, RLAM0; projections recomputed) and evaluates
-    WAVES$ETOT; |dE| per atom and |dF| are the differences of that evaluation.
-    trajectory: both forms run nsteps steps from the same start and the
-    largest energy/force difference along the way is reported (it includes
-    the divergence of the two trajectories, not only one step's error)."""
+
+def _trajectory(s, plane, start, nsteps, per_step):
+    """Both forms from the state of `start` (an FP64 set) for nsteps CP steps.
+
+    trajectory: the two forms run free; the largest |dE|/atom and |dF| along
+    the way, their values at a few steps (how the difference grows), and the
+    orthonormality residual of each form's wave functions in the FP64 metric
+    (WAVES$GETR8('ORTHOERR')).  per step (per_step=True): at every step the
+    mixed form is also handed the FP64 state and evaluates WAVES$ETOT; that
+    difference is one step's error without the divergence of trajectories."""
     import numpy as np
     import pwgpu
-    from pwgpu import model
-    s = model.SYSTEMS[system_name]()
-    plane = pwgpu.PlaneWave(s.rbas, s.epw_psi, s.epw_rho)
-    w64, wmx, wtr = [pwgpu.Waves.from_system(plane, s, prec=p) for p in (0, 1, 1)]
+    w64, wtr = [pwgpu.Waves.from_system(plane, s, prec=p) for p in (0, 1)]
+    wmx = pwgpu.Waves.from_system(plane, s, prec=1) if per_step else None
+    for w in (w64, wtr) + ((wmx,) if wmx else ()):
+        w.etot()                     # first call: Gram-Schmidt of its own start, then replaced
+        _copy_state(start, w)
+        w.projections()
     de = df = tde = tdf = 0.0
-    wmx.etot()                       # first call: Gram-Schmidt of its own start
-    for _ in range(nsteps):
+    ortho_mx = ortho_64 = 0.0
+    grow = {}
+    marks = {1, 2, 5, 10, 20, 50, 100, 150, 200, nsteps}
+    for it in range(nsteps):
         e0 = w64.etot()["etot"]
         f0 = w64.getr8a("FORCE")
-        wmx.setc8a("PSI0", w64.getc8a("PSI0"))
-        wmx.setc8a("PSIM", w64.getc8a("PSIM"))
-        wmx.set_matrix_c("RLAM0", w64.getc8a("RLAM0"))
-        wmx.projections()
-        e1 = wmx.etot()["etot"]
-        de = max(de, abs(e1 - e0) / s.nat)
-        df = max(df, float(np.max(np.abs(wmx.getr8a("FORCE") - f0))))
+        if wmx is not None:
+            _copy_state(w64, wmx)
+            wmx.projections()
+            e1 = wmx.etot()["etot"]
+            de = max(de, abs(e1 - e0) / s.nat)
+            df = max(df, float(np.max(np.abs(wmx.getr8a("FORCE") - f0))))
         et = wtr.etot()["etot"]
-        tde = max(tde, abs(et - e0) / s.nat)
-        tdf = max(tdf, float(np.max(np.abs(wtr.getr8a("FORCE") - f0))))
+        ste = abs(et - e0) / s.nat
+        stf = float(np.max(np.abs(wtr.getr8a("FORCE") - f0)))
+        tde, tdf = max(tde, ste), max(tdf, stf)
         for w in (w64, wtr):
             w.propagate()
             w.orthogonalize()
             w.switch()
-    for w in (w64, wmx, wtr):
+        om, o6 = wtr.getr8("ORTHOERR"), w64.getr8("ORTHOERR")
+        ortho_mx, ortho_64 = max(ortho_mx, om), max(ortho_64, o6)
+        if it + 1 in marks:
+            grow[str(it + 1)] = {"dE_per_atom": ste, "dF": stf, "ortho_mixed": om}
+    for w in (w64, wtr) + ((wmx,) if wmx else ()):
         w.close()
+    out = {"trajectory_max_dE_per_atom_Ha": tde, "trajectory_max_dF_Ha_per_bohr": tdf,
+           "max_ortho_residual_mixed": ortho_mx, "max_ortho_residual_fp64": ortho_64, "growth": grow}
+    if per_step:
+        out.update({"per_step_max_dE_per_atom_Ha": de, "per_step_max_dF_Ha_per_bohr": df})
+    return out
+
+
+def accuracy(system_name, nsteps, relax_tol=1.0e-8, relax_max=3000, friction=0.1, random_steps=20):
+    """Error of the mixed form against the FP64 form on the Si64 cell.
+
+    The CP run starts, as a CP simulation does, from relaxed wave functions:
+    the FP64 form runs damped CP (FRICTION) from the seeded start until the
+    energy changes by less than relax_tol per step for 5 steps; both forms
+    then continue without friction for nsteps steps from that state (see
+    _trajectory).  The same comparison from the raw seeded start (far from
+    the ground state, where the energy is first order in any deviation of the
+    wave functions) is reported beside it over random_steps steps."""
+    import pwgpu
+    from pwgpu import model
+    s = model.SYSTEMS[system_name]()
+    plane = pwgpu.PlaneWave(s.rbas, s.epw_psi, s.epw_rho)
+    w = pwgpu.Waves.from_system(plane, s, prec=0)
+    w.setr8("FRICTION", friction)
+    e_old, calm, nrelax = None, 0, 0
+    while nrelax < relax_max and calm < 5:
+        e = w.etot()["etot"]
+        w.propagate()
+        w.orthogonalize()
+        w.switch()
+        nrelax += 1
+        calm = calm + 1 if (e_old is not None and abs(e - e_old) < relax_tol) else 0
+        e_old = e
+    w.setr8("FRICTION", s.annee)
+    t0 = time.perf_counter()
+    res = _trajectory(s, plane, w, nsteps, True)
+    w.close()
+    log(f"accuracy: relaxed start ({nrelax} damped steps), {nsteps} steps in {time.perf_counter() - t0:.1f}s")
+    w = pwgpu.Waves.from_system(plane, s, prec=0)
+    w.etot()
+    rnd = _trajectory(s, plane, w, random_steps, False)
+    w.close()
     plane.close()
-    return {"steps": nsteps, "per_step_max_dE_per_atom_Ha": de, "per_step_max_dF_Ha_per_bohr": df,
-            "trajectory_max_dE_per_atom_Ha": tde, "trajectory_max_dF_Ha_per_bohr": tdf,
-            "gates": {"dE_per_atom": 1e-6, "dF": 1e-4}, "pass": bool(de < 1e-6 and df < 1e-4)}
+    gates = {"dE_per_atom": 1e-6, "dF": 1e-4}
+    p_step = bool(res["per_step_max_dE_per_atom_Ha"] < 1e-6 and res["per_step_max_dF_Ha_per_bohr"] < 1e-4)
+    p_traj = bool(res["trajectory_max_dE_per_atom_Ha"] < 1e-6 and res["trajectory_max_dF_Ha_per_bohr"] < 1e-4)
+    res.update({"steps": nsteps, "start": f"relaxed: damped FP64 CP (friction {friction}) for {nrelax} steps "
+                                          f"to |dE| < {relax_tol:g} Ha per step",
+                "random_start": dict(rnd, steps=random_steps),
+                "gates": gates, "pass_per_step": p_step, "pass_trajectory": p_traj, "pass": p_step and p_traj})
+    return res
 
 
 def serving(nreq, rate, slots, seed=7, reference=True):
@@ -196,7 +265,19 @@ def run_gpu(system_name, prec, steps, warmup, profile):
             if cnt:
                 kt[n] = {"ms_total": ms, "count": cnt, "ms_avg": ms / cnt}
         res["kernels"] = kt
-        res["counts"] = kernel_counts(s, plane, s.nb)
+        res["counts"] = kernel_counts(s, plane, s.nb, prec)
+        # per-section times under the reference's clock names (TIMING$CLOCKON
+        # in src/paw_waves1.f90 / paw_waves2.f90), microseconds per step
+        sec = {}
+        for n in ("WAVES$ETOT", "W:FORCE", "W:HPSI", "WAVES$PROPAGATE", "WAVES$ORTHOGONALIZE", "WAVES_OPSI",
+                  "WAVES_OVERLAP", "WAVES_ADDOPSI"):
+            ms, cnt = w.kernel_time(n)
+            if cnt:
+                sec[n] = 1e3 * ms / steps
+        for ours, ref in (("density", "W:RHO"), ("vpsi", "W:HPSI.VPSI"), ("ortho_x", "WAVES_ORTHO_X")):
+            if ours in kt:
+                sec[ref] = 1e3 * kt[ours]["ms_total"] / steps
+        res["sections_us_per_step"] = sec
     w.close()
     plane.close()
     return res
@@ -252,18 +333,28 @@ def potential_measure(system_name, calls, steps, with_cpu):
     return out
 
 
-def cpu_baseline(system_name, nsteps=2):
-    """The FP64 oracle (numpy) on the host: one untimed step, then nsteps timed;
-    the rate is that of the median step."""
+def cpu_baseline(system_name, nsteps=20, nsteps_1t=6):
+    """The FP64 oracle (numpy) on the host, as the reference's clock takes it
+    (first step excluded, src/paw_driver.f90:156-160): on all the BLAS
+    threads of the process (the host's share of cores) the median of nsteps
+    steps after one untimed step, and the same on one thread over nsteps_1t."""
+    from threadpoolctl import threadpool_info, threadpool_limits
+    threads = max([i.get("num_threads", 1) for i in threadpool_info()] + [1])
+    out = _cpu_oracle_steps(system_name, nsteps)
+    out["cores"] = int(threads)
+    with threadpool_limits(limits=1):
+        one = _cpu_oracle_steps(system_name, nsteps_1t)
+    one["cores"] = 1
+    out["single_thread"] = one
+    out["sample"] += f" on {threads} threads; single_thread: the same on one thread"
+    return out
+
+
+def _cpu_oracle_steps(system_name, nsteps):
     import numpy as np
     from pwgpu import model
     from oracle import planewave as opw
     from oracle import waves as owv
-    try:
-        from threadpoolctl import threadpool_info
-        threads = max([i.get("num_threads", 1) for i in threadpool_info()] + [1])
-    except Exception:
-        threads = os.cpu_count() or 1
     s = model.SYSTEMS[system_name]()
     pl = opw.Plane(s.rbas, s.epw_psi, s.epw_rho)
     st = owv.Setup(s.isp, s.lox_list, s.do_list())
@@ -285,7 +376,7 @@ def cpu_baseline(system_name, nsteps=2):
         step()
         times.append(time.perf_counter() - t0)
     med = float(np.median(times))
-    return {"value": 1.0 / med, "unit": "CP steps/s", "cores": int(threads), "kind": "port",
+    return {"value": 1.0 / med, "unit": "CP steps/s", "kind": "port",
             "sample": f"median of {nsteps} CP steps of {s.name} (FP64 numpy oracle, one band per "
                       f"FFT as the reference) after one untimed step"}
 
@@ -312,8 +403,11 @@ def main():
     ap.add_argument("--system", default="si64")
     ap.add_argument("--prec", default="both", choices=["fp64", "mixed", "both"])
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=8)
-    ap.add_argument("--acc-steps", type=int, default=20,
+    ap.add_argument("--cpu-steps", type=int, default=20,
+                    help="timed oracle steps of the CPU baseline on all cores (median)")
+    ap.add_argument("--cpu-steps-1t", type=int, default=6,
+                    help="timed oracle steps of the CPU baseline on one thread (median)")
+    ap.add_argument("--acc-steps", type=int, default=200,
                     help="steps of the mixed-vs-FP64 error measurement (0: skip)")
     ap.add_argument("--serve-requests", type=int, default=12,
                     help="requests of the serving measurement (0: skip)")
@@ -365,6 +459,7 @@ def main():
     kind, work = counts[dom]
     avg_s = kt[dom]["ms_avg"] * 1e-3
     peak, bound = kernel_peak(dom, head)
+    traffic = profiled_traffic(dom, args.system, head)
     if kind == "flop":
         achieved = work / avg_s / 1e12
         unit = "TFLOP/s"
@@ -372,7 +467,8 @@ def main():
         achieved = work / avg_s / 1e9
         unit = "GB/s"
     roof = {"kernel": dom, "bound": bound, "achieved": achieved, "peak": peak, "unit": unit,
-            "frac": achieved / peak, "traffic": profiled_traffic(dom, args.system, head),
+            "frac": achieved / peak, "traffic": traffic,
+            "traffic_rate_GBs": None if traffic is None else traffic / avg_s / 1e9,
             "work_per_launch": work, "avg_us": avg_s * 1e6}
     if peak == BF16_PEAK_TFLOPS:
         roof["split_products"] = SPLIT_PRODUCTS
@@ -399,8 +495,9 @@ def main():
     cpu = None
     if not args.no_cpu:
         log("cpu baseline (oracle) ...")
-        cpu = cpu_baseline(args.system, args.cpu_steps)
-        log(f"cpu baseline: {cpu['value']:.4f} steps/s on {cpu['cores']} threads")
+        cpu = cpu_baseline(args.system, args.cpu_steps, args.cpu_steps_1t)
+        log(f"cpu baseline: {cpu['value']:.4f} steps/s on {cpu['cores']} threads, "
+            f"{cpu['single_thread']['value']:.4f} steps/s on 1 thread")
     r = results[head]
     out = {
         "metric": "CP steps/s Si64 mixed vs FP64 @|dE|<1e-6 Ha/atom; serving req/s + p50/p99 lat",
@@ -429,6 +526,7 @@ def main():
     if 0 in results and 1 in results:
         out["fp64_steps_per_s"] = results[0]["steps_per_s"]
         out["mixed_over_fp64"] = results[1]["steps_per_s"] / results[0]["steps_per_s"]
+    out["sections_us_per_step"] = prof["sections_us_per_step"]
     out["ortho_iter_last"] = r["ortho_iter"]
     out["ortho_series_terms"] = r["ortho_series"]
     out["launch_shapes"] = r["launch_shapes"]

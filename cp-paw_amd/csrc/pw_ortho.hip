@@ -6,15 +6,18 @@
 // until max|GAMN| < 1e-8, applying S^-1 through the eigenvectors of B (DSYEV).
 // The iterates depend on S^-1 only, not on an eigen-basis, so this
 // implementation keeps the reference's outer iteration and stopping rule and
-// applies S^-1 exactly (to rounding) with GEMMs only, by the Cayley transform
-// and the squared Smith iteration:
+// applies S^-1 exactly (to rounding) with GEMMs only, as a series from the
+// Cayley transform:
 //   M = (B + qI)^-1 (Newton-Schulz), A = I - 2qM = (B - qI)M,
-//   S^-1(G) = sum_{k<2^J} A^k (2q M G M) A^k; the error after 2^J terms is
-//   rho^(2^(J+1)), rho = (sqrt(k)-1)/(sqrt(k)+1) for the spectral bounds of
-//   B (Lanczos on the device, widened).  B is positive definite in the CP
-//   step (B ~ dt^2/m).  The first 2^5 terms are summed directly as a series
-//   with P_k = A^k M formed once per call (two launches per iteration); any
-//   further terms by the Smith recursion X <- X + A_j X A_j, A_j = A^(2^j).
+//   S^-1(G) = 2q sum_{k<K} P_k G P_k,  P_k = A^k M (symmetric), K = 2^J;
+//   the error after K terms is rho^(2K), rho = (sqrt(k)-1)/(sqrt(k)+1) for
+//   the spectral bounds of B (Lanczos on the device, widened).  B is
+//   positive definite in the CP step (B ~ dt^2/m).  The P_k are formed once
+//   per call by doubling (one row-panel batch launch per level); each outer
+//   iteration then applies the series in two launches (row-panel chains
+//   P_k G P_k, then the ordered sum over k with the convergence test and the
+//   LAMBDA update).  Only terms beyond 2^5 would continue by the Smith
+//   recursion X <- X + A_j X A_j, A_j = A^(2^j); no tested cell needs them.
 // Every product is an NB^3 FP64-MFMA GEMM; nothing is diagonalised.  The
 // outer loop runs on the device: the convergence test sets a flag that makes
 // every later kernel of the batch return at once, so the host waits only

@@ -149,21 +149,70 @@ static int upload_do(pw_waveset *ws) {
   return PW_OK;
 }
 
-static int ylm_table(pw_waveset *ws) {
-  // real spherical harmonics, order 1,x,z,y for l<=1 (src/paw_spherical.f90:66-77)
-  pw_plan *p = ws->plan;
-  if (ws->lmx > 4) return pw_fail(PW_ERR_UNSUPPORTED, "WAVES: projectors with l>1 are not supported");
-  std::vector<double> y((size_t)ws->lmx * p->ngl, 0.0);
-  const double c0 = 1.0 / std::sqrt(4.0 * M_PI), c1 = std::sqrt(3.0 / (4.0 * M_PI));
-  for (int g = 0; g < p->ngl; ++g) {
-    const double gx = p->gvec[3 * g], gy = p->gvec[3 * g + 1], gz = p->gvec[3 * g + 2];
-    const double r = std::sqrt(gx * gx + gy * gy + gz * gz);
-    y[g] = c0;
-    if (ws->lmx > 1 && std::fabs(r) >= 1.0e-12) {
-      y[(size_t)1 * p->ngl + g] = c1 * gx / r;
-      y[(size_t)2 * p->ngl + g] = c1 * gz / r;
-      y[(size_t)3 * p->ngl + g] = c1 * gy / r;
+// Real spherical harmonics at one direction, LM(L,M) = 1+L^2+L-M
+// (SPHERICAL$YLM, src/paw_spherical.f90:66-148): associated Legendre
+// functions by the recursion of SPHERICAL_PLGNDR (:366-453) scaled by
+// sqrt((L-M)!/(L+M)!), then Y(LM0) = c P_{L,0}, Y(LM0-M) = sqrt2 c
+// (-1)^M P_{L,M} cos(M phi), Y(LM0+M) = ... sin(M phi), c = sqrt((2L+1)/4pi).
+static void ylm_at(int lx, double gx, double gy, double gz, double *y) {
+  const int lmx = (lx + 1) * (lx + 1);
+  const double disxy = std::sqrt(gx * gx + gy * gy);
+  const double dis = std::sqrt(disxy * disxy + gz * gz);
+  for (int lm = 0; lm < lmx; ++lm) y[lm] = 0.0;
+  y[0] = 1.0 / std::sqrt(4.0 * M_PI);
+  if (lx == 0 || std::fabs(dis) < 1.0e-12) return;
+  const double x = gz / dis;
+  const double fact = std::sqrt(std::max(1.0 - x * x, 0.0));
+  double plm[64];
+  double pmm = 1.0;
+  for (int m = 0; m <= lx; ++m) {
+    double cm = 0.0, c0 = pmm;
+    plm[m * m] = c0;
+    for (int l = m + 1; l <= lx; ++l) {
+      const double cp = (x * (2 * l - 1) * c0 - (double)(l + m - 1) * cm) / (double)(l - m);
+      plm[l * (l + 1) - m] = cp;
+      cm = c0;
+      c0 = cp;
     }
+    pmm = -(double)(2 * m + 1) * fact * pmm;
+  }
+  for (int l = 1; l <= lx; ++l) {
+    const int i1 = l * (l + 1);
+    double f = 1.0, sg = 1.0;
+    for (int m = 1; m <= l; ++m) {
+      f *= std::sqrt((double)((l - m + 1) * (l + m)));
+      sg = -sg;
+      plm[i1 - m] /= f;
+      plm[i1 + m] = sg * plm[i1 - m];
+    }
+  }
+  const double sinphi = disxy != 0.0 ? gy / disxy : 0.0;
+  const double cosphi = disxy != 0.0 ? gx / disxy : 1.0;
+  for (int l = 0; l <= lx; ++l) {
+    const int lm0 = l * (l + 1);
+    const double fac = std::sqrt((2 * l + 1) / (4.0 * M_PI));
+    y[lm0] = plm[lm0] * fac;
+    double er = 1.0, ei = 0.0;
+    for (int m = 1; m <= l; ++m) {
+      const double nr = er * cosphi - ei * sinphi, ni = er * sinphi + ei * cosphi;
+      er = nr;
+      ei = ni;
+      const double s = fac * plm[lm0 + m] * std::sqrt(2.0);
+      y[lm0 - m] = s * er;
+      y[lm0 + m] = s * ei;
+    }
+  }
+}
+
+static int ylm_table(pw_waveset *ws) {
+  pw_plan *p = ws->plan;
+  const int lx = (int)std::lround(std::sqrt((double)ws->lmx)) - 1;
+  if (lx > 6) return pw_fail(PW_ERR_UNSUPPORTED, "WAVES: projectors with l>6 are not supported");
+  std::vector<double> y((size_t)ws->lmx * p->ngl, 0.0);
+  double yl[64];
+  for (int g = 0; g < p->ngl; ++g) {
+    ylm_at(lx, p->gvec[3 * g], p->gvec[3 * g + 1], p->gvec[3 * g + 2], yl);
+    for (int lm = 0; lm < ws->lmx; ++lm) y[(size_t)lm * p->ngl + g] = yl[lm];
   }
   PW_HIP(hipMemcpy(ws->d_ylm, y.data(), y.size() * sizeof(double), hipMemcpyHostToDevice));
   return PW_OK;
@@ -269,8 +318,12 @@ static int gramschmidt_one(pw_waveset *ws, double2 *psi, const std::vector<doubl
 extern "C" int pw_waves_create(pw_plan *plan, int nb, int nat, int nsp, const int *isp,
                                const int *lnx, int lnxx, const int *lox, int prec, pw_waveset **out) {
   if (!plan || !isp || !lnx || !lox || !out) return pw_fail(PW_ERR_ARG, "pw_waves_create: null argument");
+  // the reference's input layer admits only an even number of states
+  // (NB=2*INT((NB+1)/2), src/paw_ioroutines.f90:4813), so the last super
+  // band is never a lone real band
   if (nb <= 0 || nb % 2 != 0)
-    return pw_fail(PW_ERR_UNSUPPORTED, "pw_waves_create: NB must be positive and even for super wave functions");
+    return pw_fail(PW_ERR_UNSUPPORTED, "pw_waves_create: NB must be positive and even (the reference rounds NB up "
+                                       "to even, src/paw_ioroutines.f90:4813)");
   if (nat <= 0 || nsp <= 0 || lnxx <= 0) return pw_fail(PW_ERR_ARG, "pw_waves_create: bad sizes");
   if (nb > 1024) return pw_fail(PW_ERR_UNSUPPORTED, "pw_waves_create: NB > 1024 is not supported");
   if (prec != PW_PREC_FP64 && prec != PW_PREC_MIXED) return pw_fail(PW_ERR_ARG, "pw_waves_create: bad precision");
@@ -625,9 +678,44 @@ extern "C" int pw_waves_getr8a(pw_waveset *ws, const char *id, int len, double *
   return PW_OK;
 }
 
+// The orthonormality test of WAVES$ORTHOGONALIZE's TTEST branch
+// (src/paw_waves2.f90:425-464): <psi|S|psi> of PSI0 at R(0) from the
+// projections, WAVES_1COVERLAP and WAVES_OVERLAP, returned as
+// max_ij |<psi_i|S|psi_j> - delta_ij|.  Always evaluated with the FP64
+// kernels, also for a mixed-form set, so that it measures the mixed form's
+// wave functions in the FP64 metric.  Uses OPROJ and the GS matrix as
+// scratch (both are recomputed before their next use in the step).
+static int ortho_error(pw_waveset *ws, double *out) {
+  PW_TRY(need(ws->have_psi0, "PSI0"));
+  PW_TRY(need(!ws->r0.empty(), "R0"));
+  PW_TRY(need(!ws->do_ln.empty(), "DO"));
+  struct Restore {
+    pw_waveset *w;
+    int prec;
+    ~Restore() { w->prec = prec; }
+  } restore{ws, ws->prec};
+  ws->prec = PW_PREC_FP64;
+  const int nb = ws->nb;
+  PW_TRY(k_phase_tables(ws, ws->r0.data(), ws->d_phase));
+  PW_TRY(k_projections(ws, ws->d_psi0, ws->d_phase, ws->d_oproj));
+  PW_TRY(k_overlap(ws, ws->d_psi0, ws->d_psi0, d_mat(ws, 4), false, true));
+  PW_TRY(k_overlap_1c(ws, ws->d_oproj, ws->d_oproj, d_mat(ws, 4)));
+  std::vector<double> s((size_t)nb * nb);
+  PW_HIP(hipMemcpyAsync(s.data(), d_mat(ws, 4), s.size() * sizeof(double), hipMemcpyDeviceToHost, ws->stream));
+  PW_HIP(hipStreamSynchronize(ws->stream));
+  double m = 0.0;
+  for (int i = 0; i < nb; ++i)
+    for (int j = 0; j < nb; ++j) m = std::max(m, std::fabs(s[(size_t)i * nb + j] - (i == j ? 1.0 : 0.0)));
+  // the phase tables are those of R(0) again; the mixed form's projector
+  // factors were not touched
+  *out = m;
+  return PW_OK;
+}
+
 extern "C" int pw_waves_getr8(pw_waveset *ws, const char *id, double *val) {
   if (!ws || !id || !val) return pw_fail(PW_ERR_ARG, "pw_waves_getr8: null argument");
   std::string s(id);
+  if (s == "ORTHOERR") return ortho_error(ws, val);
   if (s == "EKIN") *val = ws->ekin;
   else if (s == "EPOT") *val = ws->epot;
   else if (s == "E1C") *val = ws->e1c;
@@ -728,6 +816,7 @@ extern "C" int pw_waves_etot(pw_waveset *ws) {
   PW_TRY(ready(ws));
   pw_plan *p = ws->plan;
   ws->pfac_valid = false;   // the projector factors are redone at least once per step
+  ScopeTimer whole(ws, "WAVES$ETOT");
   if (ws->first) {
     PW_TRY(pw_waves_gramschmidt(ws));
     PW_TRY(pw_waves_projections(ws));
@@ -753,15 +842,21 @@ extern "C" int pw_waves_etot(pw_waveset *ws) {
   }
   // WAVES$FORCE
   PW_TRY(phase_at(ws, ws->r0));
-  PW_TRY(dedproj(ws));
-  PW_TRY(k_force(ws, ws->d_psi0, ws->d_phase, ws->d_xproj, ws->d_res + ws->r_force));
+  {
+    ScopeTimer sec(ws, "W:FORCE");
+    PW_TRY(dedproj(ws));
+    PW_TRY(k_force(ws, ws->d_psi0, ws->d_phase, ws->d_xproj, ws->d_res + ws->r_force));
+  }
   // WAVES$HPSI: VPSI + HPROJ + ADDPRO
   {
-    ScopeTimer tm(ws, "vpsi");
-    PW_TRY(fft_vpsi(p, ws->nbh, ws->d_psi0, ws->d_v, ws->d_hpsi, 1, ws->prec, ws->stream));
+    ScopeTimer sec(ws, "W:HPSI");
+    {
+      ScopeTimer tm(ws, "vpsi");
+      PW_TRY(fft_vpsi(p, ws->nbh, ws->d_psi0, ws->d_v, ws->d_hpsi, 1, ws->prec, ws->stream));
+    }
+    PW_TRY(k_atom_matrix(ws, ws->d_proj, ws->d_xproj, ws->d_dhmat));
+    PW_TRY(k_addpro(ws, ws->d_hpsi, ws->d_phase, ws->d_xproj));
   }
-  PW_TRY(k_atom_matrix(ws, ws->d_proj, ws->d_xproj, ws->d_dhmat));
-  PW_TRY(k_addpro(ws, ws->d_hpsi, ws->d_phase, ws->d_xproj));
   // WAVES_EKIN, <psi|H|psi>, WAVES_WAVEKINETIC (first half): one pass over psi0
   PW_TRY(k_etot_sums(ws, ws->d_res + ws->r_ekin, ws->d_res + ws->r_wk, ws->d_res + ws->r_expect));
   PW_TRY(fetch_results(ws));
@@ -850,7 +945,10 @@ extern "C" int pw_waves_propagate(pw_waveset *ws) {
   if (!ws) return pw_fail(PW_ERR_ARG, "pw_waves_propagate: null waveset");
   if (!ws->have_hpsi)
     return pw_fail(PW_ERR_STATE, "WAVES$PROPAGATE: HPSI not available (call WAVES$ETOT first)");
-  PW_TRY(k_propagate(ws));
+  {
+    ScopeTimer sec(ws, "WAVES$PROPAGATE");
+    PW_TRY(k_propagate(ws));
+  }
   ws->have_hpsi = false;
   return PW_OK;
 }
@@ -861,32 +959,42 @@ extern "C" int pw_waves_orthogonalize(pw_waveset *ws) {
   pw_plan *p = ws->plan;
   const int nb = ws->nb;
   const std::vector<double> &rp = ws->rp.empty() ? ws->r0 : ws->rp;
+  ScopeTimer whole(ws, "WAVES$ORTHOGONALIZE");
   // OPSI = dt^2/((1+a) m(G)) (psi0 + |p>DO<p|psi0>) at R(0)
-  PW_TRY(k_copy(ws, ws->d_opsi, ws->d_psi0, (size_t)ws->nbh * p->ngl));
-  PW_TRY(phase_at(ws, ws->r0));
-  PW_TRY(k_atom_matrix(ws, ws->d_proj, ws->d_xproj, ws->d_domat));
-  PW_TRY(k_addpro(ws, ws->d_opsi, ws->d_phase, ws->d_xproj));
-  PW_TRY(k_opsi_scale(ws, ws->d_opsi));
+  {
+    ScopeTimer sec(ws, "WAVES_OPSI");
+    PW_TRY(k_copy(ws, ws->d_opsi, ws->d_psi0, (size_t)ws->nbh * p->ngl));
+    PW_TRY(phase_at(ws, ws->r0));
+    PW_TRY(k_atom_matrix(ws, ws->d_proj, ws->d_xproj, ws->d_domat));
+    PW_TRY(k_addpro(ws, ws->d_opsi, ws->d_phase, ws->d_xproj));
+    PW_TRY(k_opsi_scale(ws, ws->d_opsi));
+  }
   // projections at R(+); psibar and chi stay fixed until the overlaps are done
-  PW_TRY(phase_at(ws, rp));
-  PW_TRY(k_runs_scope(ws, true));
-  PW_TRY(k_projections(ws, ws->d_psim, ws->d_phase, ws->d_proj));
-  PW_TRY(k_projections(ws, ws->d_opsi, ws->d_phase, ws->d_oproj));
-  // MAT, OMAT, OOMAT
-  PW_TRY(k_overlap(ws, ws->d_psim, ws->d_psim, d_mat(ws, 0), false, true));
-  PW_TRY(k_overlap_1c(ws, ws->d_proj, ws->d_proj, d_mat(ws, 0)));
-  PW_TRY(k_overlap(ws, ws->d_opsi, ws->d_psim, d_mat(ws, 1), false, false));
-  PW_TRY(k_overlap_1c(ws, ws->d_oproj, ws->d_proj, d_mat(ws, 1)));
-  PW_TRY(k_overlap(ws, ws->d_opsi, ws->d_opsi, d_mat(ws, 2), false, true));
-  PW_TRY(k_runs_scope(ws, false));
-  PW_TRY(k_overlap_1c(ws, ws->d_oproj, ws->d_oproj, d_mat(ws, 2)));
+  {
+    ScopeTimer sec(ws, "WAVES_OVERLAP");
+    PW_TRY(phase_at(ws, rp));
+    PW_TRY(k_runs_scope(ws, true));
+    PW_TRY(k_projections(ws, ws->d_psim, ws->d_phase, ws->d_proj));
+    PW_TRY(k_projections(ws, ws->d_opsi, ws->d_phase, ws->d_oproj));
+    // MAT, OMAT, OOMAT
+    PW_TRY(k_overlap(ws, ws->d_psim, ws->d_psim, d_mat(ws, 0), false, true));
+    PW_TRY(k_overlap_1c(ws, ws->d_proj, ws->d_proj, d_mat(ws, 0)));
+    PW_TRY(k_overlap(ws, ws->d_opsi, ws->d_psim, d_mat(ws, 1), false, false));
+    PW_TRY(k_overlap_1c(ws, ws->d_oproj, ws->d_proj, d_mat(ws, 1)));
+    PW_TRY(k_overlap(ws, ws->d_opsi, ws->d_opsi, d_mat(ws, 2), false, true));
+    PW_TRY(k_runs_scope(ws, false));
+    PW_TRY(k_overlap_1c(ws, ws->d_oproj, ws->d_oproj, d_mat(ws, 2)));
+  }
   // initial LAMBDA: hermitised RLAM0 times 0.5 f_i/(f_i+f_j) (src/paw_waves2.f90:295-359)
   k_lam_guess<<<(nb * nb + 255) / 256, 256, 0, ws->stream>>>(nb, ws->d_occ, ws->d_rlam0, d_mat(ws, 3));
   PW_HIP(hipGetLastError());
   PW_TRY(ortho_x_device(ws, d_mat(ws, 3), d_mat(ws, 2), d_mat(ws, 0), d_mat(ws, 1), &ws->ortho_iter));
   // psi(+) = psibar + chi Lambda ; proj(+) = proj + oproj Lambda
-  PW_TRY(k_addopsi(ws, ws->d_psim, ws->d_opsi, d_mat(ws, 3)));
-  PW_TRY(k_addoproj(ws, ws->d_proj, ws->d_oproj, d_mat(ws, 3)));
+  {
+    ScopeTimer sec(ws, "WAVES_ADDOPSI");
+    PW_TRY(k_addopsi(ws, ws->d_psim, ws->d_opsi, d_mat(ws, 3)));
+    PW_TRY(k_addoproj(ws, ws->d_proj, ws->d_oproj, d_mat(ws, 3)));
+  }
   // RLAM0 stays on the device; MAT, OMAT, OOMAT go to the host only when asked for
   k_copy_mat<<<(nb * nb + 255) / 256, 256, 0, ws->stream>>>(nb * nb, d_mat(ws, 3), ws->d_rlam0);
   PW_HIP(hipGetLastError());

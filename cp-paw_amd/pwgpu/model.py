@@ -237,9 +237,45 @@ def si_species():
     return Species(lox=[0, 0, 1, 1], sigma=[0.9, 1.3, 1.0, 1.4], do_ln=do, dh_ln=dh)
 
 
+def si_species_d():
+    """Model Si site with the projector set of the reference's si2 deck
+    (NPRO= 2 2 1: 2 s + 2 p + 1 d, LNX=5, LMNX=13)."""
+    sp = si_species()
+    do = np.zeros((5, 5))
+    dh = np.zeros((5, 5))
+    do[:4, :4] = sp.do_ln
+    dh[:4, :4] = sp.dh_ln
+    do[4, 4] = 0.12
+    dh[4, 4] = 0.35
+    return Species(lox=[0, 0, 1, 1, 2], sigma=[0.9, 1.3, 1.0, 1.4, 1.1], do_ln=do, dh_ln=dh)
+
+
+def species_from_npro(npro, base=None):
+    """Model species with NPRO(l) radial projectors per angular momentum l
+    (the !STRUCTURE!SPECIES NPRO keyword of the reference's decks): Gaussian
+    radial functions of widths 0.9, 1.3, ... and block-diagonal DO/DH."""
+    base = si_species() if base is None else base
+    lox, sigma = [], []
+    for l, n in enumerate(npro):
+        for k in range(int(n)):
+            lox.append(l)
+            sigma.append(0.9 + 0.4 * k + 0.1 * l)
+    nl = len(lox)
+    do = np.zeros((nl, nl))
+    dh = np.zeros((nl, nl))
+    for i in range(nl):
+        for j in range(nl):
+            if lox[i] == lox[j]:
+                k = abs(i - j)
+                do[i, j] = 0.3 / (1 + lox[i]) if k == 0 else 0.05
+                dh[i, j] = (-0.45 + 0.25 * lox[i] + 0.3 * (i - lox.index(lox[i]))) if k == 0 else 0.08
+    return Species(lox=lox, sigma=sigma, do_ln=do, dh_ln=dh, zval=base.zval, rcsm=base.rcsm,
+                   rcore=base.rcore, qcore=base.qcore, vbar=base.vbar, rvbar=base.rvbar)
+
+
 def diamond_si(ncell=(1, 1, 1), a=10.26, nempty_per_cell=None, seed=1,
                disp_sigma=0.05, epw_psi_ry=30.0, epw_rho_ry=None, cdual=4.0,
-               dt=5.0, emass=1000.0, nempty=None):
+               dt=5.0, emass=1000.0, nempty=None, species=None):
     """Diamond Si supercell of the 8-atom simple-cubic cell, Gamma point.
 
     16 doubly occupied bands per 8 atoms; ``nempty`` empty bands.
@@ -268,8 +304,8 @@ def diamond_si(ncell=(1, 1, 1), a=10.26, nempty_per_cell=None, seed=1,
     epw_rho = (epw_rho_ry * RY) if epw_rho_ry is not None else cdual * epw_psi
     wells = [(p, -1.6, 1.5) for p in pos]
     return System(name=f"Si{nat}", rbas=rbas, pos=pos, isp=np.zeros(nat, dtype=np.int64),
-                  species=[si_species()], nb=nb, occ=occ, epw_psi=epw_psi,
-                  epw_rho=epw_rho, wells=wells, dt=dt, emass=emass, seed=seed)
+                  species=[si_species() if species is None else species], nb=nb, occ=occ,
+                  epw_psi=epw_psi, epw_rho=epw_rho, wells=wells, dt=dt, emass=emass, seed=seed)
 
 
 def si8(**kw):

@@ -5,7 +5,7 @@
  * and WAVES objects that form the CP wavefunction step.  Each declaration
  * names the reference routine (file:line, relative to the cp-paw checkout) it
  * stands in for.  A Fortran host binds them through ISO_C_BINDING (see
- * cp-paw_amd/fortran/pwgpu_mod.f90); a Python host binds them through ctypes
+ * cp-paw_amd/fortran/pwgpu_c.f90); a Python host binds them through ctypes
  * (cp-paw_amd/pwgpu/_lib.py).
  *
  * Conventions
@@ -122,7 +122,11 @@ int pw_waves_getc8a(pw_waveset *ws, const char *id, int len, double *val);
  * "MAT","OMAT","OOMAT" (NB*NB, last orthogonalisation), "DENMAT"
  * (LMNXX*LMNXX*NAT). */
 int pw_waves_getr8a(pw_waveset *ws, const char *id, int len, double *val);
-/* Scalars: "EKIN","EPOT","E1C","ETOT","WAVEEKIN1","WAVEEKIN2","EFICT". */
+/* Scalars: "EKIN","EPOT","E1C","ETOT","WAVEEKIN1","WAVEEKIN2","EFICT".
+ * "ORTHOERR": the orthonormality test of WAVES$ORTHOGONALIZE's TTEST branch
+ * (src/paw_waves2.f90:425-464) on PSI0 at R(0), max_ij |<psi_i|S|psi_j> -
+ * delta_ij|, always evaluated with the FP64 kernels (for a mixed-form set it
+ * measures the mixed wave functions in the FP64 metric); overwrites OPROJ. */
 int pw_waves_getr8(pw_waveset *ws, const char *id, double *val);
 /* "ORTHOITER" (iterations of the last WAVES_ORTHO_X), "NPRO", "NBH",
  * "LMNXX", "ORTHOLEVELS" (squared-Smith levels of the last solve), "ORTHONS"

@@ -66,24 +66,66 @@ class Setup:
 
 
 def ylm(gvec, lmx):
-    """Real spherical harmonics (order 1,x,z,y for l<=1) at each G.
+    """Real spherical harmonics at each G, index LM(L,M) = 1+L^2+L-M.
 
-    src/paw_spherical.f90:66-77 (SPHERICAL$YLM); Y=(1/sqrt(4pi),0,0,0) at G=0
-    (src/paw_spherical.f90:117-123).  Returns (LMX, NG).
+    src/paw_spherical.f90:66-148 (SPHERICAL$YLM) with the associated Legendre
+    recursion of SPHERICAL_PLGNDR (:366-453): P_{M,M} = (-1)^M (2M-1)!!
+    sin^M, (L-M) P_{L,M} = x(2L-1) P_{L-1,M} - (L+M-1) P_{L-2,M}, scaled by
+    sqrt((L-M)!/(L+M)!); Y(LM0) = sqrt((2L+1)/4pi) P_{L,0} and, for M>0,
+    Y(LM0-M) = sqrt(2) c (-1)^M P_{L,M} cos(M phi), Y(LM0+M) = same sin(M phi)
+    with LM0 = L(L+1)+1.  Y = (1/sqrt(4pi), 0, ...) at G = 0 (:117-123).
+    Returns (LMX, NG); LMX must be a full shell (L+1)^2.
     """
-    if lmx > 4:
-        raise NotImplementedError("oracle restates l<=1 only")
-    ng = len(gvec)
+    lx = int(round(np.sqrt(lmx))) - 1
+    if (lx + 1) ** 2 != lmx:
+        raise ValueError("LMX does not correspond to a full shell")
+    g = np.asarray(gvec, dtype=np.float64)
+    ng = len(g)
     out = np.zeros((lmx, ng))
     out[0] = 1.0 / np.sqrt(4.0 * PI)
-    if lmx > 1:
-        r = np.sqrt(gvec[:, 0] ** 2 + gvec[:, 1] ** 2 + gvec[:, 2] ** 2)
-        ok = np.abs(r) >= 1.0e-12
-        c = np.sqrt(3.0 / (4.0 * PI))
-        rs = np.where(ok, r, 1.0)
-        out[1] = np.where(ok, c * gvec[:, 0] / rs, 0.0)
-        out[2] = np.where(ok, c * gvec[:, 2] / rs, 0.0)
-        out[3] = np.where(ok, c * gvec[:, 1] / rs, 0.0)
+    if lmx == 1:
+        return out
+    disxy = np.sqrt(g[:, 0] ** 2 + g[:, 1] ** 2)
+    dis = np.sqrt(disxy ** 2 + g[:, 2] ** 2)
+    ok = np.abs(dis) >= 1.0e-12
+    x = np.where(ok, g[:, 2] / np.where(ok, dis, 1.0), 1.0)        # cos(theta)
+    fact = np.sqrt(np.maximum(1.0 - x * x, 0.0))
+    # P_{L,M} for M >= 0 at LM(L,M) - 1 = L(L+1) - M (0-based)
+    plm = np.zeros((lmx, ng))
+    pmm = np.ones(ng)
+    for m in range(lx + 1):
+        cm = np.zeros(ng)
+        c0 = pmm.copy()
+        plm[m * m] = c0
+        for l in range(m + 1, lx + 1):
+            cp = (x * (2 * l - 1) * c0 - (l + m - 1) * cm) / (l - m)
+            plm[l * (l + 1) - m] = cp
+            cm, c0 = c0, cp
+        pmm = -(2 * m + 1) * fact * pmm
+    for l in range(1, lx + 1):
+        i1 = l * (l + 1)
+        f = 1.0
+        sgn = 1.0
+        for m in range(1, l + 1):
+            f *= np.sqrt((l - m + 1) * (l + m))
+            sgn = -sgn
+            plm[i1 - m] = plm[i1 - m] / f
+            plm[i1 + m] = sgn * plm[i1 - m]
+    sinphi = np.where(disxy != 0.0, g[:, 1] / np.where(disxy != 0.0, disxy, 1.0), 0.0)
+    cosphi = np.where(disxy != 0.0, g[:, 0] / np.where(disxy != 0.0, disxy, 1.0), 1.0)
+    eiphi = cosphi + 1j * sinphi
+    for l in range(lx + 1):
+        lm0 = l * (l + 1)
+        fac = np.sqrt((2 * l + 1) / (4.0 * PI))
+        out[lm0] = plm[lm0] * fac
+        eim = np.ones(ng, dtype=np.complex128)
+        for m in range(1, l + 1):
+            eim = eim * eiphi
+            svar = fac * plm[lm0 + m] * np.sqrt(2.0)
+            out[lm0 - m] = svar * eim.real
+            out[lm0 + m] = svar * eim.imag
+    out[:, ~ok] = 0.0
+    out[0, ~ok] = 1.0 / np.sqrt(4.0 * PI)
     return out
 
 

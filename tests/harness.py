@@ -46,4 +46,9 @@ SYSTEMS = {
     "synthetic": model.synthetic,
     # 32 H2O, 40 Ry, 80^3 grid, 128 bands, 384 projectors (~13 s per etot)
     "h2o32": model.water,
+    # Si8 with the si2 deck's projector set 2 s + 2 p + 1 d (LMNX = 13)
+    "si8d": lambda: model.si8(species=model.si_species_d()),
+    # the benchmarked cell: 64 atoms, 144 bands, 23 847 G, 72^3, 512 projectors
+    # (the oracle takes ~2 s per step)
+    "si64": model.si64,
 }

@@ -95,13 +95,16 @@ def test_si2_plane_matches_oracle():
 
 @pytest.mark.gpu
 def test_si2_deck_cp_steps_match_oracle():
-    # the deck's cell, bands, cutoffs and dynamics with the model Si setup
+    # the deck's cell, bands, cutoffs and dynamics, with a model Si setup that
+    # has the deck's projector set (NPRO= 2 2 1: 2 s, 2 p and 1 d projector)
     import pwgpu
     from harness import relerr
     from oracle import planewave as opw
     from oracle import waves as owv
     d = si2()
-    s = d.to_system({"Si": model.si_species()})
+    assert d.species["Si"]["npro"] == [2, 2, 1]
+    s = d.to_system({"Si": model.species_from_npro(d.species["Si"]["npro"])})
+    assert s.species[0].lox == [0, 0, 1, 1, 2]
     print(f"case si2 deck: nb={s.nb} epw={s.epw_psi}/{s.epw_rho} dt={s.dt}", flush=True)
     pl = opw.Plane(s.rbas, s.epw_psi, s.epw_rho)
     o = owv.WaveSet(pl, owv.Setup(s.isp, s.lox_list, s.do_list()), s.pro_table(pl.g2), s.nb, s.occ,

@@ -30,26 +30,33 @@ def say(msg):
     print(msg, flush=True)
 
 
-def ortho_residual(o, psi):
-    """max |<psi_i|S|psi_j> - delta_ij| with the oracle's FP64 overlaps."""
+def ortho_residual(o, psi, r=None):
+    """max |<psi_i|S|psi_j> - delta_ij| with the oracle's FP64 overlaps at
+    positions r (default R(0))."""
     pl, st = o.plane, o.setup
-    proj = owv.projections(pl, st, o.pro, o.ylm, o.r0, psi)
+    proj = owv.projections(pl, st, o.pro, o.ylm, o.r0 if r is None else r, psi)
     s = owv.overlap(pl, o.nb, psi, psi) + owv.overlap_1c(st, o.nb, proj, proj)
     return float(np.max(np.abs(s - np.eye(o.nb))))
 
 
-@pytest.mark.parametrize("name,nsteps", [("tiny", 4), ("si8", 5), ("si8c", 3), ("h2o4", 4), ("synthetic", 2),
-                                          ("h2o32", 2)])
-def test_mixed_form_error_gates(name, nsteps):
+@pytest.mark.parametrize("name,nsteps,moving", [("tiny", 4, False), ("si8", 5, False), ("si8c", 3, False),
+                                                 ("h2o4", 4, False), ("synthetic", 2, False), ("h2o32", 2, False),
+                                                 ("si8d", 3, False), ("si64", 2, False),
+                                                 ("si8", 3, True), ("si64", 2, True)])
+def test_mixed_form_error_gates(name, nsteps, moving):
     s = SYSTEMS[name]()
-    say(f"case mixed gates {name} x{nsteps}")
+    say(f"case mixed gates {name} x{nsteps}{' moving ions' if moving else ''}")
     o = oracle_waveset(s)
     o.r0 = s.pos
+    vel = np.random.default_rng(12).normal(0.0, 0.03, s.pos.shape) if moving else np.zeros_like(s.pos)
     plane, w = gpu_waveset(s, prec=1)
     w.etot()                          # first call: Gram-Schmidt of its own start
     worst = [0.0, 0.0, 0.0]
     for it in range(nsteps):
-        eo = o.etot(s.pos)
+        r0, rp = s.pos + it * vel, s.pos + (it + 1) * vel
+        eo = o.etot(r0)
+        w.setr8a("R0", r0)
+        w.setr8a("RP", rp)
         w.setc8a("PSI0", o.psi0)
         w.setc8a("PSIM", o.psim)
         w.set_matrix_c("RLAM0", np.asarray(o.rlam0, dtype=np.complex128))
@@ -58,10 +65,10 @@ def test_mixed_form_error_gates(name, nsteps):
         de = abs(e["etot"] - eo["etot"]) / s.nat
         df = float(np.max(np.abs(w.getr8a("FORCE") - o.force)))
         o.propagate()
-        o.orthogonalize(s.pos, s.pos)
+        o.orthogonalize(r0, rp)
         w.propagate()
         w.orthogonalize()
-        res = ortho_residual(o, w.getc8a("PSIM"))
+        res = ortho_residual(o, w.getc8a("PSIM"), rp)
         worst = [max(worst[0], de), max(worst[1], df), max(worst[2], res)]
         assert de < DE_ATOM, (it, de)
         assert df < DF, (it, df)

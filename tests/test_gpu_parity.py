@@ -46,7 +46,7 @@ def test_fft_gtor_rtog(name, prec):
     assert relerr(got, ref) < tol
 
 
-@pytest.mark.parametrize("name", ["tiny", "si8", "si8c", "h2o4"])
+@pytest.mark.parametrize("name", ["tiny", "si8", "si8c", "h2o4", "si8d"])
 def test_vpsi_density_projections(name):
     import pwgpu
     s = SYSTEMS[name]()
@@ -70,7 +70,7 @@ def test_vpsi_density_projections(name):
     assert relerr(w.getc8a("PROJ"), ref) < TOL_KERNEL
 
 
-@pytest.mark.parametrize("name", ["tiny", "si8", "si8c", "h2o4"])
+@pytest.mark.parametrize("name", ["tiny", "si8", "si8c", "h2o4", "si8d"])
 def test_etot_first_call(name):
     s = SYSTEMS[name]()
     say(f"case etot {name}")
@@ -93,7 +93,7 @@ def test_etot_first_call(name):
 
 
 @pytest.mark.parametrize("name,nsteps", [("tiny", 6), ("si8", 4), ("si8c", 3), ("h2o4", 4), ("synthetic", 2),
-                                          ("h2o32", 2)])
+                                          ("h2o32", 2), ("si8d", 3), ("si64", 2)])
 def test_cp_steps(name, nsteps):
     s = SYSTEMS[name]()
     say(f"case cp steps {name} x{nsteps}")
@@ -167,3 +167,61 @@ def test_ortho_x_standalone():
                             captured["chipsi"], captured["lam"])
     assert it == o.ortho_iter
     assert relerr(lam, o.rlam0) < 1e-11
+
+
+def moving_positions(s, seed=12, speed=0.03):
+    """R(n) = R0 + n v: seeded ion velocities, so that R(-) != R(0) != R(+)."""
+    v = np.random.default_rng(seed).normal(0.0, speed, s.pos.shape)
+    return lambda n: s.pos + n * v
+
+
+@pytest.mark.parametrize("name,nsteps", [("tiny", 4), ("si8", 3), ("h2o4", 3), ("si8d", 2), ("si64", 2)])
+def test_cp_steps_moving_ions(name, nsteps):
+    """Ions move every step: ETOT at R(0), the first Gram-Schmidt of PSIM at
+    R(-), OPSI at R(0), the new projections and overlaps at R(+)
+    (src/paw_waves2.f90:129-252), then R(0) <- R(+) after SWITCH."""
+    s = SYSTEMS[name]()
+    say(f"case cp steps moving ions {name} x{nsteps}")
+    o = oracle_waveset(s)
+    plane, w = gpu_waveset(s)
+    R = moving_positions(s)
+    for it in range(nsteps):
+        r0, rp, rm = R(it), R(it + 1), R(it - 1)
+        w.setr8a("R0", r0)
+        w.setr8a("RP", rp)
+        w.setr8a("RM", rm)
+        eo = o.etot(r0, rm)
+        o.propagate()
+        o.orthogonalize(r0, rp)
+        eg = w.etot()
+        w.propagate()
+        w.orthogonalize()
+        assert abs(eg["etot"] - eo["etot"]) <= TOL_STEP * abs(eo["etot"]), it
+        assert relerr(w.getr8a("FORCE"), o.force) < 1e-9, it
+        assert w.geti4("ORTHOITER") == o.ortho_iter, it
+        assert relerr(w.getr8a("OMAT"), o.omat) < 1e-11, it
+        assert relerr(w.getc8a("PROJ"), o.proj) < 1e-10, it
+        assert relerr(w.getc8a("RLAM0").real, o.rlam0) < 1e-9, it
+        o.switch()
+        w.switch()
+    assert relerr(w.getc8a("PSI0"), o.psi0) < 1e-9
+
+
+def test_odd_band_count_rejected():
+    """NB odd is refused as the reference's input layer refuses it
+    (NB=2*INT((NB+1)/2), src/paw_ioroutines.f90:4813)."""
+    import ctypes
+    import pwgpu
+    from pwgpu import _lib
+    s = SYSTEMS["tiny"]()
+    say("case odd NB")
+    plane = pwgpu.PlaneWave(s.rbas, s.epw_psi, s.epw_rho)
+    lib = _lib.load()
+    isp = np.ones(s.nat, dtype=np.int32)
+    lnx = np.array([4], dtype=np.int32)
+    lox = np.array([0, 0, 1, 1], dtype=np.int32)
+    h = ctypes.c_void_p()
+    rc = lib.pw_waves_create(plane.handle, 7, s.nat, 1, _lib.iptr(isp), _lib.iptr(lnx), 4, _lib.iptr(lox), 0,
+                             ctypes.byref(h))
+    assert rc == 2
+    assert "even" in _lib.last_error()

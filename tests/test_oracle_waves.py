@@ -319,3 +319,86 @@ def test_cp_energy_conservation_small_timestep():
         o.switch()
     cons = np.array(cons[1:])
     assert np.max(cons) - np.min(cons) < 2e-3 * abs(cons[0])
+
+
+def _ylm_closed_forms(g):
+    """The real harmonics listed in the header of SPHERICAL$YLM
+    (src/paw_spherical.f90:70-89), l <= 3, as explicit polynomials."""
+    x, y, z = g[:, 0], g[:, 1], g[:, 2]
+    r = np.sqrt(x * x + y * y + z * z)
+    x, y, z = x / r, y / r, z / r
+    pi = np.pi
+    return np.array([
+        np.full_like(x, np.sqrt(1 / (4 * pi))),
+        np.sqrt(3 / (4 * pi)) * x, np.sqrt(3 / (4 * pi)) * z, np.sqrt(3 / (4 * pi)) * y,
+        np.sqrt(15 / (16 * pi)) * (x * x - y * y), np.sqrt(60 / (16 * pi)) * x * z,
+        np.sqrt(5 / (16 * pi)) * (3 * z * z - 1), np.sqrt(60 / (16 * pi)) * y * z,
+        np.sqrt(60 / (16 * pi)) * x * y,
+        np.sqrt(35 / (32 * pi)) * (x ** 3 - 3 * x * y * y), np.sqrt(210 / (32 * pi)) * z * (x * x - y * y),
+        np.sqrt(21 / (32 * pi)) * x * (5 * z * z - 1), np.sqrt(14 / (32 * pi)) * z * (5 * z * z - 3),
+        np.sqrt(21 / (32 * pi)) * y * (5 * z * z - 1), np.sqrt(840 / (32 * pi)) * x * y * z,
+        np.sqrt(35 / (32 * pi)) * y * (3 * x * x - y * y)])
+
+
+def test_ylm_closed_forms_and_orthonormality():
+    rng = np.random.default_rng(4)
+    g = rng.normal(size=(200, 3))
+    g[:3] = [[0.0, 0.0, 2.0], [0.0, 0.0, -1.0], [1.5, 0.0, 0.0]]    # poles and an axis
+    got = owv.ylm(g, 16)
+    assert np.max(np.abs(got - _ylm_closed_forms(g))) < 1e-13
+    # G = 0: only the l = 0 harmonic (src/paw_spherical.f90:117-123)
+    z = owv.ylm(np.zeros((1, 3)), 16)
+    assert abs(z[0, 0] - 1 / np.sqrt(4 * np.pi)) < 1e-16 and np.all(z[1:, 0] == 0.0)
+    # orthonormal on the sphere (Gauss-Legendre in cos(theta) x uniform phi,
+    # exact for these polynomial degrees), l up to 4
+    ct, wt = np.polynomial.legendre.leggauss(12)
+    phi = 2 * np.pi * np.arange(24) / 24
+    c, p = np.meshgrid(ct, phi, indexing="ij")
+    s = np.sqrt(1 - c * c)
+    pts = np.stack([s * np.cos(p), s * np.sin(p), c], axis=-1).reshape(-1, 3)
+    w = (wt[:, None] * np.full(24, 2 * np.pi / 24)[None, :]).ravel()
+    y = owv.ylm(pts, 25)
+    assert np.max(np.abs((y * w) @ y.T - np.eye(25))) < 1e-13
+
+
+def test_propagate_friction_damped_oscillator():
+    """WAVES$PROPAGATE with friction (src/paw_waves1.f90:6553-6583) on a
+    harmonic Hamiltonian H psi = k m(G) psi: the coefficients follow the
+    recurrence (1+a) x(n+1) = (2 - dt^2 k) x(n) - (1-a) x(n-1), solved in
+    closed form by the roots of its characteristic polynomial, and for small
+    dt the damped oscillator x'' = -k x - (2a/dt) x' (amplitude decaying as
+    exp(-a t/dt), frequency sqrt(k - (a/dt)^2))."""
+    s = model.tiny()
+    o = oracle_waveset(s)
+    pl = o.plane
+    rng = np.random.default_rng(2)
+    x0 = (rng.normal(size=(2, pl.ngl)) + 1j * rng.normal(size=(2, pl.ngl))).astype(complex)
+    for annee, dt, k, nstep, tol in ((0.02, 5.0, 2.0e-3, 60, 1e-12), (0.0, 5.0, 2.0e-3, 60, 1e-12),
+                                     (0.004, 0.5, 1.0e-2, 1000, 3e-3)):
+        o.annee, o.delt = annee, dt
+        o.psim, o.psi0 = x0[0].copy()[None], x0[1].copy()[None]
+        traj = [o.psim[0].copy(), o.psi0[0].copy()]
+        for n in range(nstep):
+            o.hpsi = k * o.mpsi[None, :] * o.psi0
+            o.propagate()
+            o.psi0, o.psim = o.psim, o.psi0
+            traj.append(o.psi0[0].copy())
+        roots = np.roots([1 + annee, -(2 - dt * dt * k), 1 - annee])
+        z1, z2 = roots
+        # x(n) = A z1^n + B z2^n through x(0) = psim, x(1) = psi0
+        b = (x0[1] - z1 * x0[0]) / (z2 - z1)
+        a = x0[0] - b
+        n = nstep + 1
+        want = a * z1 ** n + b * z2 ** n
+        if tol < 1e-6:
+            assert np.max(np.abs(traj[-1] - want)) < tol * np.max(np.abs(x0))
+        # continuum limit: decay rate gamma/2 = a/dt, frequency sqrt(k - (a/dt)^2)
+        gam = annee / dt
+        w = np.sqrt(k - gam * gam)
+        t = n * dt
+        v0 = (x0[1] - x0[0]) / dt
+        xc = np.exp(-gam * t) * (x0[0] * np.cos(w * t) + (v0 + gam * x0[0]) / w * np.sin(w * t))
+        if tol >= 1e-6:
+            assert np.max(np.abs(traj[-1] - xc)) < tol * np.max(np.abs(x0))
+        # the product of the roots is the per-step damping (1-a)/(1+a)
+        assert abs(z1 * z2 - (1 - annee) / (1 + annee)) < 1e-14
