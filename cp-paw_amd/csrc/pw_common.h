@@ -127,14 +127,19 @@ struct pw_plan {
   void *d_vt = nullptr; size_t vt_bytes = 0;   // V(r) as [x][z][y] in the work precision
   // POTENTIAL$VOFRHO on this (density) plane-wave object: setup tables and work
   double *d_pot_tabs = nullptr; int pot_nsp = 0;
+  std::vector<int> pot_lmrx;   // LMRX per species (POTENTIAL_HARTREE's LMRX(NSP))
   void *d_pot = nullptr; size_t pot_bytes = 0;
   double *h_pot = nullptr; size_t h_pot_bytes = 0; int pot_nat = 0;   // pinned: results and staged atom data
+  int pot_lmrxx = 1;           // LMRXX of the last call
+  double *d_pot_res = nullptr; // device results of the last call: per atom VQLM(LMRXX), force(3);
+                               // then EHARTREE, EXC, RHOB
 };
 
 // POTENTIAL$VOFRHO on a density plan (pw_potential.hip): enqueue on stream s
-// (isp 1-based, pos R(3,NAT)); results are valid after the stream passes.
-int vofrho_enqueue(pw_plan *dp, int nat, const int *isp, const double *pos, const double *qlm,
-                   const double *rho_dev, double *vofr_dev, hipStream_t s);
+// (isp 1-based, pos R(3,NAT), QLM(LMRXX,NAT) from the host or the device);
+// results are valid after the stream passes.
+int vofrho_enqueue(pw_plan *dp, int nat, int lmrxx, const int *isp, const double *pos, const double *qlm,
+                   const double *qlm_dev, const double *rho_dev, double *vofr_dev, hipStream_t s);
 void vofrho_results(const pw_plan *dp, int nat, double *energies, double *vqlm, double *force);
 
 // stripe work buffer for nbatch bands (and the z-plane buffer of the full-grid
@@ -193,7 +198,10 @@ struct pw_waveset {
   // density -> potential step linked into ETOT (pw_waves_set_potential)
   pw_plan *dplan = nullptr;
   std::vector<int> isp1;             // NAT, 1-based
-  std::vector<double> qlm;           // QLM(1,NAT)
+  std::vector<double> qlm;           // QLM(LMRXX,NAT)
+  int qlm_lmrxx = 1;
+  void *sph = nullptr;               // one-centre set-up and work (pw_augment.hip)
+  double potb = 0.0;                 // POTB of the last ETOT with the one-centre terms
   double ehartree = 0.0, exc = 0.0, rhob = 0.0;
   std::vector<double> vqlm, fpot;    // dE/dQLM (NAT), potential forces (3 NAT)
   // lambda (real, Gamma point)
@@ -288,6 +296,20 @@ struct ScopeTimer {
   ~ScopeTimer();
 };
 void ws_flush_timers(pw_waveset *ws);
+
+// real spherical harmonics at one direction, LM(L,M) = 1+L^2+L-M, L <= lx <= 6
+// (SPHERICAL$YLM; pw_waves.cpp)
+void ylm_dir(int lx, double gx, double gy, double gz, double *y);
+
+// one-centre terms (pw_augment.hip): set-up state lives in ws->sph
+void sphere_release(pw_waveset *ws);
+bool sphere_ready(const pw_waveset *ws);
+int sphere_lmrxx(const pw_waveset *ws);
+int sphere_lmrx(const pw_waveset *ws, int isp);
+int sphere_moments(pw_waveset *ws, const double **qlm_dev);
+int sphere_terms(pw_waveset *ws, const double *vqlm_dev, const double *rhob_dev);
+void sphere_results(const pw_waveset *ws, double *e1c, double *potb);
+const double *sphere_qlm_dev(const pw_waveset *ws);
 
 // kernels in pw_kernels.hip
 int k_phase_tables(pw_waveset *ws, const double *r_dev_host, double2 *phase);

@@ -1,16 +1,18 @@
 // POTENTIAL$VOFRHO (src/paw_potential.f90:324-896) on the GPU, FP64: the
 // density -> local potential step that follows WAVES_DENSITY, for one spin,
 // the local-density functional of type 1 (local exchange + Perdew-Zunger,
-// src/paw_dft.f90:124-130), compensation charges with l = 0 only, no pair
-// potential of the compensation charges and no isolation.
+// src/paw_dft.f90:124-130), compensation charges of the moments QLM(LMRXX,NAT)
+// with l <= 4 (LMRXX <= 25), no pair potential of the compensation charges
+// and no isolation.
 //
 // Pipeline on the density plane-wave object (sphere of the density cutoff):
 //   rho(G)   = RTOG rho(r) + sum_a PSCORG Y0 e_a          (SUPFFT, ADDCORE :1038-1066)
-//   Hartree  per G: compensation, VBAR, V0, background, EHARTREE partials
-//            (POTENTIAL_HARTREE :1069-1334)
+//   Hartree  per G: compensation sum_LM CFAC(LM) QLM(LM) |G|^l Y_LM(G) G0 e_a,
+//            VBAR, V0, background, EHARTREE partials (POTENTIAL_HARTREE
+//            :1069-1334, CFAC = (-i)^l/(2l+1)!!, YLMOFG of POTENTIAL_YLMOFG :1548)
 //   rho(r)   = GTOR rho(G);  exc(r), vxc(r) by LDA       (POTENTIAL_XC :1339-1545)
 //   vtot(G)  = RTOG vxc(r) + Hartree potential
-//   per atom: VQLM, Hartree/VBAR forces and the pseudo-core force
+//   per atom: VQLM(LM), Hartree/VBAR forces and the pseudo-core force
 //            (POTENTIAL_FPSCORE :897-975) in one G pass
 //   v(r)     = GTOR vtot(G)
 // e_a(G) = exp(-i G.R_a) is formed from the integer G components and the
@@ -34,16 +36,100 @@ constexpr double kGammaU = -142.3e-3, kBeta1U = 1052.9e-3, kBeta2U = 333.4e-3;
 constexpr double kAU = 31.10e-3, kBU = -48.0e-3, kCU = 2.0e-3, kDU = -11.6e-3;
 
 struct PotGeo {
-  int ngl, nrl, nat, nsp, ngamma;
+  int ngl, nrl, nat, nsp, ngamma, lmrxx;
+  const int *lmrx;       // nsp: LMRX of each species
   const int *igvec;      // 3 x ngl
   const double *gvec;    // 3 x ngl
   const double *g2;      // ngl
   const double *tabs;    // [4][nsp][ngl]: PSCORG, VBARG, G0, V0
   const int *isp;        // nat, 0-based
   const double *gr;      // nat x 3: b_k . R_a
-  const double *qlm;     // nat
+  const double *qlm;     // nat x lmrxx (QLM(LMRXX,NAT))
   double vol;
 };
+
+constexpr int kLmrxxMax = 25;   // l <= 4
+
+// |G|^l Y_LM(G) for LM < lmx (POTENTIAL_YLMOFG, src/paw_potential.f90:1548-1614):
+// the real harmonics of SPHERICAL$YLM (src/paw_spherical.f90:66-148, recursion
+// of SPHERICAL_PLGNDR :366-453) times |G|^l; (Y0, 0, ...) for |G| < 1e-7
+__device__ inline void solid_ylm(int lmx, double gx, double gy, double gz, double *y) {
+  const double disxy = sqrt(gx * gx + gy * gy);
+  const double dis = sqrt(disxy * disxy + gz * gz);
+  for (int lm = 0; lm < lmx; ++lm) y[lm] = 0.0;
+  y[0] = kY0;
+  if (lmx == 1 || dis < 1.0e-7) return;
+  const int lx = lmx <= 1 ? 0 : (lmx <= 4 ? 1 : (lmx <= 9 ? 2 : (lmx <= 16 ? 3 : 4)));
+  const double x = gz / dis;
+  const double fact = sqrt(fmax(1.0 - x * x, 0.0));
+  double plm[kLmrxxMax];
+  double pmm = 1.0;
+  for (int m = 0; m <= lx; ++m) {
+    double cm = 0.0, c0 = pmm;
+    plm[m * m] = c0;
+    for (int l = m + 1; l <= lx; ++l) {
+      const double cp = (x * (2 * l - 1) * c0 - (double)(l + m - 1) * cm) / (double)(l - m);
+      plm[l * (l + 1) - m] = cp;
+      cm = c0;
+      c0 = cp;
+    }
+    pmm = -(double)(2 * m + 1) * fact * pmm;
+  }
+  for (int l = 1; l <= lx; ++l) {
+    const int i1 = l * (l + 1);
+    double f = 1.0, sg = 1.0;
+    for (int m = 1; m <= l; ++m) {
+      f *= sqrt((double)((l - m + 1) * (l + m)));
+      sg = -sg;
+      plm[i1 - m] /= f;
+      plm[i1 + m] = sg * plm[i1 - m];
+    }
+  }
+  const double sinphi = disxy != 0.0 ? gy / disxy : 0.0;
+  const double cosphi = disxy != 0.0 ? gx / disxy : 1.0;
+  double gl = 1.0;
+  for (int l = 0; l <= lx; ++l) {
+    const int lm0 = l * (l + 1);
+    const double fac = sqrt((2 * l + 1) / (4.0 * kPi)) * gl;
+    y[lm0] = plm[lm0] * fac;
+    double er = 1.0, ei = 0.0;
+    for (int m = 1; m <= l; ++m) {
+      const double nr = er * cosphi - ei * sinphi, ni = er * sinphi + ei * cosphi;
+      er = nr;
+      ei = ni;
+      const double sv = fac * plm[lm0 + m] * sqrt(2.0);
+      y[lm0 - m] = sv * er;
+      y[lm0 + m] = sv * ei;
+    }
+    gl *= dis;
+  }
+}
+
+// CFAC(LM) = (-i)^l/(2l+1)!! as (re, im) (src/paw_potential.f90:1146-1152)
+__device__ inline double2 cfac_l(int l) {
+  double df = 1.0;
+  for (int k = 3; k <= 2 * l + 1; k += 2) df *= (double)k;
+  switch (l & 3) {
+    case 0: return make_double2(1.0 / df, 0.0);
+    case 1: return make_double2(0.0, -1.0 / df);
+    case 2: return make_double2(-1.0 / df, 0.0);
+    default: return make_double2(0.0, 1.0 / df);
+  }
+}
+
+__device__ inline int l_of(int lm) { return lm < 1 ? 0 : (lm < 4 ? 1 : (lm < 9 ? 2 : (lm < 16 ? 3 : 4))); }
+
+// sum_{LM < LMRX} CFAC(LM) QLM(LM) YLMOFG(LM) (complex)
+__device__ inline double2 comp_factor(const PotGeo &g, int a, int n, const double *y) {
+  double re = 0.0, im = 0.0;
+  const double *q = g.qlm + (size_t)a * g.lmrxx;
+  for (int lm = 0; lm < n; ++lm) {
+    const double2 c = cfac_l(l_of(lm));
+    re += c.x * q[lm] * y[lm];
+    im += c.y * q[lm] * y[lm];
+  }
+  return make_double2(re, im);
+}
 
 __device__ inline double2 eigr(const PotGeo &g, int a, int ig) {
   const double ph = (double)g.igvec[3 * ig] * g.gr[3 * a] + (double)g.igvec[3 * ig + 1] * g.gr[3 * a + 1] +
@@ -94,20 +180,22 @@ __global__ void __launch_bounds__(kPotThreads) k_pot_g(PotGeo g, double2 *__rest
   if (ig < g.ngl) {
     double2 core = {0.0, 0.0}, comp = {0.0, 0.0}, pot = {0.0, 0.0};
     double vb = 0.0;
+    double y[kLmrxxMax];
+    solid_ylm(g.lmrxx, g.gvec[3 * ig], g.gvec[3 * ig + 1], g.gvec[3 * ig + 2], y);
     for (int a = 0; a < g.nat; ++a) {
       const int s = g.isp[a];
       const double2 ea = eigr(g, a, ig);
-      const double cs = g.qlm[a] * kY0;
+      // CSVAR = sum_LM CFAC(LM) QLM(LM) YLMOFG(LM,G) EIGR(G)
+      const double2 cs = cmul(comp_factor(g, a, g.lmrx[s], y), ea);
       const double fc = tab(g, 0, s, ig) * kY0;
-      const double fh = cs * tab(g, 2, s, ig);
-      const double fp = cs * tab(g, 3, s, ig) + tab(g, 1, s, ig) * kY0;
+      const double g0 = tab(g, 2, s, ig), v0 = tab(g, 3, s, ig), vbar = tab(g, 1, s, ig) * kY0;
       core.x += fc * ea.x;
       core.y += fc * ea.y;
-      comp.x += fh * ea.x;
-      comp.y += fh * ea.y;
-      pot.x += fp * ea.x;
-      pot.y += fp * ea.y;
-      if (ig == g.ngamma) vb += cs * tab(g, 3, s, ig);
+      comp.x += cs.x * g0;
+      comp.y += cs.y * g0;
+      pot.x += cs.x * v0 + vbar * ea.x;
+      pot.y += cs.y * v0 + vbar * ea.y;
+      if (ig == g.ngamma) vb += g.qlm[(size_t)a * g.lmrxx] * kY0 * v0;
     }
     double2 rg = rhog[ig];
     rg.x += core.x;
@@ -191,21 +279,24 @@ __global__ void k_add_c(int n, const double2 *__restrict__ a, double2 *__restric
   if (i < n) b[i] = make_double2(a[i].x + b[i].x, a[i].y + b[i].y);
 }
 
-// Per (atom, G block): VQLM and the Hartree/VBAR force terms
-// (src/paw_potential.f90:1263-1290) and the pseudo-core force
+// Per (atom, G block): VQLM(LM) and the Hartree/VBAR force terms
+// (src/paw_potential.f90:1255-1290) and the pseudo-core force
 // (POTENTIAL_FPSCORE, :897-975), as block partial sums
-// apart[(a * nblk + block) * 4 + {0: vqlm, 1..3: force}]
+// apart[(a * nblk + block) * (lmrxx + 3) + {0..lmrxx-1: VQLM, lmrxx..+2: force}]
 __global__ void __launch_bounds__(kPotThreads) k_pot_atoms(PotGeo g, const double2 *__restrict__ rhog,
                                                            const double2 *__restrict__ vg,
                                                            const double2 *__restrict__ vtot,
                                                            const double *__restrict__ scal,
                                                            double *__restrict__ apart) {
-  __shared__ double sh[kPotThreads / 64];
+  __shared__ double sh[kLmrxxMax + 3][kPotThreads / 64];
   const int ig = blockIdx.x * kPotThreads + threadIdx.x;
   const int a = blockIdx.y;
-  double q = 0.0, f[3] = {0.0, 0.0, 0.0};
+  const int nval = g.lmrxx + 3;
+  double val[kLmrxxMax + 3];
+  for (int k = 0; k < nval; ++k) val[k] = 0.0;
   if (ig < g.ngl) {
     const int s = g.isp[a];
+    const int n = g.lmrx[s];
     const double2 ea = eigr(g, a, ig);
     const double2 rg = rhog[ig], v = vg[ig], vt = vtot[ig];
     const double2 r2b = {rg.x + (ig == g.ngamma ? scal[0] : 0.0), rg.y};
@@ -213,26 +304,39 @@ __global__ void __launch_bounds__(kPotThreads) k_pot_atoms(PotGeo g, const doubl
     // AVAL = (G0 conj(VG) + V0 conj(RHO2)) EIGR
     const double2 b = {g0 * v.x + v0 * r2b.x, -(g0 * v.y + v0 * r2b.y)};
     const double2 aval = {b.x * ea.x - b.y * ea.y, b.x * ea.y + b.y * ea.x};
-    q = kY0 * aval.x;
-    const double cs = g.qlm[a] * kY0;
+    double y[kLmrxxMax];
+    solid_ylm(g.lmrxx, g.gvec[3 * ig], g.gvec[3 * ig + 1], g.gvec[3 * ig + 2], y);
+    // VQLM(LM) += Re(CFAC(LM) YLMOFG(LM,G) AVAL)
+    for (int lm = 0; lm < n; ++lm) {
+      const double2 c = cfac_l(l_of(lm));
+      val[lm] = c.x * y[lm] * aval.x - c.y * y[lm] * aval.y;
+    }
+    // CSUM = sum_LM QLM(LM) CFAC(LM) YLMOFG(LM,G)
+    const double2 csum = comp_factor(g, a, n, y);
     // CVAL = VBARG Y0 EIGR conj(RHO)
     const double vb = tab(g, 1, s, ig) * kY0;
     const double cval_im = vb * (ea.y * rg.x - ea.x * rg.y);
-    const double sv = cval_im + cs * aval.y;
+    const double sv = cval_im + csum.x * aval.y + csum.y * aval.x;
     // FPSCORE: -Im(conj(VTOT) PSCORG EIGR) Y0
     const double pc = tab(g, 0, s, ig);
     const double sc = -pc * (vt.x * ea.y - vt.y * ea.x);
     const double t = -sv + kY0 * sc;
 #pragma unroll
-    for (int k = 0; k < 3; ++k) f[k] = g.gvec[3 * ig + k] * t;
+    for (int k = 0; k < 3; ++k) val[g.lmrxx + k] = g.gvec[3 * ig + k] * t;
   }
-  const size_t base = ((size_t)a * gridDim.x + blockIdx.x) * 4;
-  double r = block_sum(q, sh);
-  if (threadIdx.x == 0) apart[base] = r;
+  // all values of the block at once: wave sums, then the waves in order
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int k = 0; k < nval; ++k) {
+    double v = val[k];
 #pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    r = block_sum(f[k], sh);
-    if (threadIdx.x == 0) apart[base + 1 + k] = r;
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    if (lane == 0) sh[k][wv] = v;
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < nval) {
+    double r = 0.0;
+    for (int w = 0; w < kPotThreads / 64; ++w) r += sh[threadIdx.x][w];
+    apart[((size_t)a * gridDim.x + blockIdx.x) * nval + threadIdx.x] = r;
   }
 }
 
@@ -252,18 +356,28 @@ __global__ void __launch_bounds__(kPotThreads) k_sum_out(int n, int stride, cons
 }  // namespace
 
 // Issues the whole step on stream st; the results land in pinned host memory
-// (p->h_pot) once the stream has passed this point (vofrho_results reads them).
-int vofrho_enqueue(pw_plan *p, int nat, const int *isp, const double *pos, const double *qlm, const double *rho,
-                   double *vofr, hipStream_t st) {
+// (p->h_pot) once the stream has passed this point (vofrho_results reads them)
+// and stay on the device (vofrho_device_results).  QLM(LMRXX,NAT) comes from
+// the host (qlm) or from device memory (qlm_dev).
+int vofrho_enqueue(pw_plan *p, int nat, int lmrxx, const int *isp, const double *pos, const double *qlm,
+                   const double *qlm_dev, const double *rho, double *vofr, hipStream_t st) {
   const int ngl = p->ngl, nrl = p->nrl, nsp = p->pot_nsp;
   if (!p->d_pot_tabs || nsp <= 0)
     return pw_fail(PW_ERR_STATE, "POTENTIAL$VOFRHO: setup tables not set (pw_potential_tables)");
+  if (lmrxx < 1 || lmrxx > kLmrxxMax)
+    return pw_fail(PW_ERR_UNSUPPORTED, "POTENTIAL$VOFRHO: LMRXX must be 1..25 (l <= 4)");
+  const int lx = (int)std::lround(std::sqrt((double)lmrxx)) - 1;
+  if ((lx + 1) * (lx + 1) != lmrxx) return pw_fail(PW_ERR_ARG, "POTENTIAL$VOFRHO: LMRXX is not a full shell");
+  for (int s = 0; s < nsp; ++s)
+    if (p->pot_lmrx[s] > lmrxx) return pw_fail(PW_ERR_ARG, "POTENTIAL$VOFRHO: LMRX of a species exceeds LMRXX");
   PW_TRY(plan_device(p));
   const int nbg = (ngl + kPotThreads - 1) / kPotThreads;
   const int nbr = (nrl + kPotThreads - 1) / kPotThreads;
+  const int nval = lmrxx + 3;
   // work: grid (complex), rhog, vg, vh, vtot (sphere), partials, per-atom data
-  const size_t need = (size_t)nrl * 16 + 4 * (size_t)ngl * 16 + ((size_t)nbg * (4 * nat + 1) + nbr + 8) * 8 +
-                      (size_t)nat * (4 * 8 + 4 * 8) + 4096;
+  const size_t need = (size_t)nrl * 16 + 4 * (size_t)ngl * 16 +
+                      ((size_t)nbg * (nval * (size_t)nat + 1) + nbr + 8) * 8 +
+                      ((size_t)nat * nval + 3) * 8 + (size_t)nat * (8 + 3 * 8 + lmrxx * 8) + (size_t)nsp * 8 + 4096;
   if (need > p->pot_bytes) {
     if (p->d_pot) PW_HIP(hipFree(p->d_pot));
     p->d_pot = nullptr;
@@ -271,7 +385,10 @@ int vofrho_enqueue(pw_plan *p, int nat, const int *isp, const double *pos, const
     PW_HIP(hipMalloc(&p->d_pot, need));
     p->pot_bytes = need;
   }
-  const size_t hneed = (4 * (size_t)nat + 3 + 6 * (size_t)nat) * sizeof(double);
+  // pinned: results (nat * nval + 3), then staged isp, b.R, QLM, LMRX
+  const size_t hres = (size_t)nat * nval + 3;
+  const size_t hneed =
+      (hres + (size_t)nat + 3 * (size_t)nat + (size_t)nat * lmrxx + (size_t)nsp + 8) * sizeof(double);
   if (hneed > p->h_pot_bytes) {
     if (p->h_pot) PW_HIP(hipHostFree(p->h_pot));
     p->h_pot = nullptr;
@@ -287,28 +404,37 @@ int vofrho_enqueue(pw_plan *p, int nat, const int *isp, const double *pos, const
   double2 *vtot = (double2 *)w; w += (size_t)ngl * 16;
   double *epart = (double *)w; w += (size_t)nbg * 8;
   double *xpart = (double *)w; w += (size_t)nbr * 8;
-  double *apart = (double *)w; w += (size_t)nbg * 4 * nat * 8;
+  double *apart = (double *)w; w += (size_t)nbg * nval * nat * 8;
   double *scal = (double *)w; w += 8 * 8;
-  double *res = (double *)w; w += (size_t)(4 * nat + 3) * 8;   // vqlm/force per atom, EHARTREE, EXC, RHOB
+  double *res = (double *)w; w += hres * 8;   // per atom VQLM(LMRXX), force(3); EHARTREE, EXC, RHOB
   int *d_isp = (int *)w; w += (size_t)nat * 8;
+  int *d_lmrx = (int *)w; w += (size_t)nsp * 8;
   double *d_gr = (double *)w; w += (size_t)nat * 3 * 8;
   double *d_qlm = (double *)w;
   // per-atom data: 0-based species, b_k . R_a, QLM (staged in pinned memory)
-  int *hisp = (int *)(p->h_pot + 4 * (size_t)nat + 3);
-  double *hgr = p->h_pot + 5 * (size_t)nat + 3;
-  double *hq = p->h_pot + 8 * (size_t)nat + 3;
+  double *hbase = p->h_pot + hres;
+  int *hisp = (int *)hbase;
+  int *hlmrx = hisp + nat;
+  double *hgr = hbase + nat + (nsp + 1) / 2 + 1;
+  double *hq = hgr + 3 * (size_t)nat;
   for (int a = 0; a < nat; ++a) {
     if (isp[a] < 1 || isp[a] > nsp) return pw_fail(PW_ERR_ARG, "POTENTIAL$VOFRHO: species index out of range");
     hisp[a] = isp[a] - 1;
     for (int k = 0; k < 3; ++k)
       hgr[3 * a + k] = p->gbas[0 + 3 * k] * pos[3 * a] + p->gbas[1 + 3 * k] * pos[3 * a + 1] +
                        p->gbas[2 + 3 * k] * pos[3 * a + 2];
-    hq[a] = qlm[a];
+    if (qlm)
+      for (int lm = 0; lm < lmrxx; ++lm) hq[(size_t)a * lmrxx + lm] = qlm[(size_t)a * lmrxx + lm];
   }
+  for (int s = 0; s < nsp; ++s) hlmrx[s] = p->pot_lmrx[s];
+  PW_HIP(hipMemcpyAsync(d_lmrx, hlmrx, nsp * sizeof(int), hipMemcpyHostToDevice, st));
   if (nat > 0) {
     PW_HIP(hipMemcpyAsync(d_isp, hisp, nat * sizeof(int), hipMemcpyHostToDevice, st));
     PW_HIP(hipMemcpyAsync(d_gr, hgr, 3 * nat * sizeof(double), hipMemcpyHostToDevice, st));
-    PW_HIP(hipMemcpyAsync(d_qlm, hq, nat * sizeof(double), hipMemcpyHostToDevice, st));
+    if (qlm)
+      PW_HIP(hipMemcpyAsync(d_qlm, hq, (size_t)nat * lmrxx * sizeof(double), hipMemcpyHostToDevice, st));
+    else
+      PW_HIP(hipMemcpyAsync(d_qlm, qlm_dev, (size_t)nat * lmrxx * sizeof(double), hipMemcpyDeviceToDevice, st));
   }
   PotGeo g{};
   g.ngl = ngl;
@@ -316,6 +442,8 @@ int vofrho_enqueue(pw_plan *p, int nat, const int *isp, const double *pos, const
   g.nat = nat;
   g.nsp = nsp;
   g.ngamma = p->ngamma;
+  g.lmrxx = lmrxx;
+  g.lmrx = d_lmrx;
   g.igvec = p->d_igvec;
   g.gvec = p->d_gvec;
   g.g2 = p->d_g2;
@@ -338,29 +466,34 @@ int vofrho_enqueue(pw_plan *p, int nat, const int *isp, const double *pos, const
   // per-atom sums
   if (nat > 0) {
     k_pot_atoms<<<dim3(nbg, nat), kPotThreads, 0, st>>>(g, rhog, vg, vtot, scal, apart);
-    k_sum_out<<<4 * nat, kPotThreads, 0, st>>>(nbg, 4, apart, p->vol, res);
+    k_sum_out<<<nval * nat, kPotThreads, 0, st>>>(nbg, nval, apart, p->vol, res);
   }
-  k_sum_out<<<1, kPotThreads, 0, st>>>(nbg, 1, epart, p->vol, res + 4 * nat);
-  k_sum_out<<<1, kPotThreads, 0, st>>>(nbr, 1, xpart, p->vol / (double)nrl, res + 4 * nat + 1);
-  k_copy1<<<1, 64, 0, st>>>(scal, res + 4 * nat + 2);
+  k_sum_out<<<1, kPotThreads, 0, st>>>(nbg, 1, epart, p->vol, res + (size_t)nval * nat);
+  k_sum_out<<<1, kPotThreads, 0, st>>>(nbr, 1, xpart, p->vol / (double)nrl, res + (size_t)nval * nat + 1);
+  k_copy1<<<1, 64, 0, st>>>(scal, res + (size_t)nval * nat + 2);
   // v(r)
   PW_TRY(fft_gtor_full(p, 1, vtot, grid, PW_PREC_FP64, st));
   k_c_to_real<<<nbr, kPotThreads, 0, st>>>(nrl, grid, vofr);
   PW_HIP(hipGetLastError());
-  PW_HIP(hipMemcpyAsync(p->h_pot, res, (4 * (size_t)nat + 3) * sizeof(double), hipMemcpyDeviceToHost, st));
+  PW_HIP(hipMemcpyAsync(p->h_pot, res, hres * sizeof(double), hipMemcpyDeviceToHost, st));
   p->pot_nat = nat;
+  p->pot_lmrxx = lmrxx;
+  p->d_pot_res = res;
   return PW_OK;
 }
 
+// host results: energies {EHARTREE, EXC, RHOB}, VQLM(LMRXX,NAT), force(3,NAT)
 void vofrho_results(const pw_plan *p, int nat, double *energies, double *vqlm, double *force) {
   const double *h = p->h_pot;
-  energies[0] = h[4 * nat];
-  energies[1] = h[4 * nat + 1];
-  energies[2] = p->ngamma >= 0 ? h[4 * nat + 2] : 0.0;
+  const int lmrxx = p->pot_lmrxx, nval = lmrxx + 3;
+  energies[0] = h[(size_t)nval * nat];
+  energies[1] = h[(size_t)nval * nat + 1];
+  energies[2] = p->ngamma >= 0 ? h[(size_t)nval * nat + 2] : 0.0;
   for (int a = 0; a < nat; ++a) {
-    if (vqlm) vqlm[a] = h[4 * a];
+    if (vqlm)
+      for (int lm = 0; lm < lmrxx; ++lm) vqlm[(size_t)a * lmrxx + lm] = h[(size_t)a * nval + lm];
     if (force)
-      for (int k = 0; k < 3; ++k) force[3 * a + k] = h[4 * a + 1 + k];
+      for (int k = 0; k < 3; ++k) force[3 * a + k] = h[(size_t)a * nval + lmrxx + k];
   }
 }
 
@@ -368,8 +501,16 @@ void vofrho_results(const pw_plan *p, int nat, double *energies, double *vqlm, d
 // ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
-extern "C" int pw_potential_tables(pw_plan *p, int nsp, const double *tabs) {
+extern "C" int pw_potential_tables(pw_plan *p, int nsp, const int *lmrx, const double *tabs) {
   if (!p || nsp <= 0 || !tabs) return pw_fail(PW_ERR_ARG, "pw_potential_tables: bad argument");
+  std::vector<int> l(nsp, 1);
+  if (lmrx)
+    for (int s = 0; s < nsp; ++s) {
+      const int lx = (int)std::lround(std::sqrt((double)lmrx[s])) - 1;
+      if (lmrx[s] < 1 || lmrx[s] > kLmrxxMax || (lx + 1) * (lx + 1) != lmrx[s])
+        return pw_fail(PW_ERR_ARG, "pw_potential_tables: LMRX must be a full shell (L+1)^2 with L <= 4");
+      l[s] = lmrx[s];
+    }
   PW_TRY(plan_device(p));
   const size_t n = 4 * (size_t)nsp * p->ngl;
   if (p->d_pot_tabs) PW_HIP(hipFree(p->d_pot_tabs));
@@ -378,14 +519,15 @@ extern "C" int pw_potential_tables(pw_plan *p, int nsp, const double *tabs) {
   PW_HIP(hipMalloc(&p->d_pot_tabs, n * sizeof(double)));
   PW_HIP(hipMemcpy(p->d_pot_tabs, tabs, n * sizeof(double), hipMemcpyHostToDevice));
   p->pot_nsp = nsp;
+  p->pot_lmrx = l;
   return PW_OK;
 }
 
-extern "C" int pw_vofrho(pw_plan *p, int nat, const int *isp, const double *pos, const double *qlm,
+extern "C" int pw_vofrho(pw_plan *p, int nat, int lmrxx, const int *isp, const double *pos, const double *qlm,
                          const double *rho, double *vofr, double *energies, double *vqlm, double *force) {
   if (!p || nat < 0 || (nat > 0 && (!isp || !pos || !qlm)) || !rho || !vofr || !energies)
     return pw_fail(PW_ERR_ARG, "pw_vofrho: bad argument");
-  PW_TRY(vofrho_enqueue(p, nat, isp, pos, qlm, rho, vofr, p->stream));
+  PW_TRY(vofrho_enqueue(p, nat, lmrxx, isp, pos, qlm, nullptr, rho, vofr, p->stream));
   PW_HIP(hipStreamSynchronize(p->stream));
   vofrho_results(p, nat, energies, vqlm, force);
   return PW_OK;

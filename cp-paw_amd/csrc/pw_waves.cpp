@@ -154,7 +154,7 @@ static int upload_do(pw_waveset *ws) {
 // functions by the recursion of SPHERICAL_PLGNDR (:366-453) scaled by
 // sqrt((L-M)!/(L+M)!), then Y(LM0) = c P_{L,0}, Y(LM0-M) = sqrt2 c
 // (-1)^M P_{L,M} cos(M phi), Y(LM0+M) = ... sin(M phi), c = sqrt((2L+1)/4pi).
-static void ylm_at(int lx, double gx, double gy, double gz, double *y) {
+void ylm_dir(int lx, double gx, double gy, double gz, double *y) {
   const int lmx = (lx + 1) * (lx + 1);
   const double disxy = std::sqrt(gx * gx + gy * gy);
   const double dis = std::sqrt(disxy * disxy + gz * gz);
@@ -211,7 +211,7 @@ static int ylm_table(pw_waveset *ws) {
   std::vector<double> y((size_t)ws->lmx * p->ngl, 0.0);
   double yl[64];
   for (int g = 0; g < p->ngl; ++g) {
-    ylm_at(lx, p->gvec[3 * g], p->gvec[3 * g + 1], p->gvec[3 * g + 2], yl);
+    ylm_dir(lx, p->gvec[3 * g], p->gvec[3 * g + 1], p->gvec[3 * g + 2], yl);
     for (int lm = 0; lm < ws->lmx; ++lm) y[(size_t)lm * p->ngl + g] = yl[lm];
   }
   PW_HIP(hipMemcpy(ws->d_ylm, y.data(), y.size() * sizeof(double), hipMemcpyHostToDevice));
@@ -463,6 +463,7 @@ extern "C" int pw_waves_destroy(pw_waveset *ws) {
   if (!ws) return PW_OK;
   ws_flush_timers(ws);
   ortho_graph_release(ws);
+  sphere_release(ws);
   void *ptrs[] = {ws->d_psi0, ws->d_psim, ws->d_hpsi, ws->d_opsi, ws->d_tmp, ws->d_proj, ws->d_oproj,
                   ws->d_xproj, ws->d_pro, ws->d_ylm, ws->d_mpsi, ws->d_v, ws->d_rho, ws->d_occ2,
                   ws->d_occ, ws->d_rlam0, ws->d_rlamm,
@@ -536,8 +537,13 @@ extern "C" int pw_waves_setr8a(pw_waveset *ws, const char *id, int len, const do
     std::vector<double> r(val, val + len);
     if (s == "R0") ws->r0 = r; else if (s == "RP") ws->rp = r; else ws->rm = r;
   } else if (s == "QLM") {
-    PW_TRY(chk((size_t)ws->nat));
+    // QLM(LMRXX,NAT): LMRXX = LEN/NAT must be a full shell (L+1)^2, L <= 4
+    const int lmrxx = ws->nat > 0 ? len / ws->nat : 0;
+    const int lx = (int)std::lround(std::sqrt((double)std::max(lmrxx, 1))) - 1;
+    if (lmrxx < 1 || (size_t)lmrxx * ws->nat != (size_t)len || (lx + 1) * (lx + 1) != lmrxx || lx > 4)
+      return pw_fail(PW_ERR_ARG, "WAVES$SETR8A: QLM must be QLM(LMRXX,NAT) with LMRXX = (L+1)^2, L <= 4");
     ws->qlm.assign(val, val + len);
+    ws->qlm_lmrxx = lmrxx;
   } else if (s == "ORTHOBOUNDS") {
     // spectral bounds of sym(CHIPSI) for the next ORTHO_X call; ORTHO_X checks
     // them on the device and recomputes them when they are not accepted
@@ -606,6 +612,23 @@ extern "C" int pw_waves_getc8a(pw_waveset *ws, const char *id, int len, double *
   const size_t npsi = (size_t)ws->nbh * ws->plan->ngl;
   const size_t nproj = (size_t)ws->nbh * ws->npro;
   if (s == "PSI0") return get_dev_c(ws, ws->d_psi0, npsi, len, val);
+  if (s == "DH") {
+    // DH(LMNXX,LMNXX,NAT) of the last ETOT (or as set), column-major
+    const int L = ws->lmnxx;
+    const size_t n = (size_t)L * L * ws->nat;
+    if ((size_t)len != n) return pw_fail(PW_ERR_ARG, "WAVES$GETC8A: size inconsistent");
+    std::vector<double2> m(n);
+    PW_HIP(hipStreamSynchronize(ws->stream));
+    PW_HIP(hipMemcpy(m.data(), ws->d_dhmat, n * sizeof(double2), hipMemcpyDeviceToHost));
+    for (int a = 0; a < ws->nat; ++a)
+      for (int i = 0; i < L; ++i)
+        for (int j = 0; j < L; ++j) {
+          const size_t dst = ((size_t)a * L + j) * L + i;
+          val[2 * dst] = m[((size_t)a * L + i) * L + j].x;
+          val[2 * dst + 1] = m[((size_t)a * L + i) * L + j].y;
+        }
+    return PW_OK;
+  }
   if (s == "PSIM") return get_dev_c(ws, ws->d_psim, npsi, len, val);
   if (s == "HPSI") return get_dev_c(ws, ws->d_hpsi, npsi, len, val);
   if (s == "OPSI") return get_dev_c(ws, ws->d_opsi, npsi, len, val);
@@ -661,10 +684,19 @@ extern "C" int pw_waves_getr8a(pw_waveset *ws, const char *id, int len, double *
     PW_HIP(hipMemcpy(m.data(), d_mat(ws, s == "MAT" ? 0 : (s == "OMAT" ? 1 : 2)), (size_t)len * sizeof(double),
                      hipMemcpyDeviceToHost));
     to_colmajor(m, ws->nb, val);
-  } else if (s == "VQLM") {
-    PW_TRY(chk((size_t)ws->nat));
-    if (!ws->dplan) return pw_fail(PW_ERR_STATE, "WAVES$GETR8A: VQLM needs the density -> potential step");
-    std::memcpy(val, ws->vqlm.data(), (size_t)len * sizeof(double));
+  } else if (s == "VQLM" || s == "QLM") {
+    if (s == "QLM" && ws->dplan && sphere_ready(ws)) {
+      // the moments WAVES$MOMENTS made on the device in the last ETOT
+      ws->qlm_lmrxx = sphere_lmrxx(ws);
+      ws->qlm.resize((size_t)ws->nat * ws->qlm_lmrxx);
+      PW_HIP(hipStreamSynchronize(ws->stream));
+      PW_HIP(hipMemcpy(ws->qlm.data(), sphere_qlm_dev(ws), ws->qlm.size() * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    // VQLM(LMRXX,NAT) = dE/dQLM of the last ETOT, and the moments it used
+    if (!ws->dplan) return pw_fail(PW_ERR_STATE, "WAVES$GETR8A: " + s + " needs the density -> potential step");
+    const std::vector<double> &src = s == "VQLM" ? ws->vqlm : ws->qlm;
+    PW_TRY(chk(src.size()));
+    std::memcpy(val, src.data(), (size_t)len * sizeof(double));
   } else if (s == "DENMAT") {
     PW_TRY(chk(ws->denmat.size()));
     // row-major per atom -> DENMAT(LMNXX,LMNXX,NAT) column-major
@@ -726,6 +758,7 @@ extern "C" int pw_waves_getr8(pw_waveset *ws, const char *id, double *val) {
   else if (s == "EHARTREE") *val = ws->ehartree;
   else if (s == "EXC") *val = ws->exc;
   else if (s == "RHOB") *val = ws->rhob;
+  else if (s == "POTB") *val = ws->potb;
   else return pw_fail(PW_ERR_ARG, "WAVES$GETR8: ID not recognised: " + s);
   return PW_OK;
 }
@@ -786,7 +819,8 @@ static int ready(pw_waveset *ws) {
   PW_TRY(need(ws->have_psim, "PSIM"));
   PW_TRY(need(!ws->occ.empty(), "OCC"));
   PW_TRY(need(!ws->do_ln.empty(), "DO"));
-  PW_TRY(need(ws->have_dh, "DH"));
+  // DH is an input unless the one-centre terms produce it
+  PW_TRY(need(ws->have_dh || (ws->dplan && sphere_ready(ws)), "DH"));
   PW_TRY(need(ws->have_v, "VOFR"));
   PW_TRY(need(!ws->r0.empty(), "R0"));
   PW_TRY(need(ws->have_mass, "TIMESTEP/EMASS"));
@@ -832,10 +866,31 @@ extern "C" int pw_waves_etot(pw_waveset *ws) {
   if (ws->dplan) {
     // POTENTIAL$VOFRHO: the local potential of this density, used by VPSI
     // below; its energy EHARTREE + EXC and forces are read with the results
-    if (ws->qlm.size() != (size_t)ws->nat) return pw_fail(PW_ERR_STATE, "WAVES$ETOT: QLM not set");
-    ScopeTimer tm(ws, "vofrho");
-    PW_TRY(vofrho_enqueue(ws->dplan, ws->nat, ws->isp1.data(), ws->r0.data(), ws->qlm.data(), ws->d_rho, ws->d_v,
-                          ws->stream));
+    if (sphere_ready(ws)) {
+      // the one-centre terms linked in (src/paw_waves1.f90:1843-1915):
+      // WAVES$MOMENTS -> QLM, POTENTIAL$VOFRHO(rho, QLM) -> VQLM and RHOB,
+      // WAVES$SPHERE -> DH, POTB, E1C, then WAVES$ADDCONSTANTPOT
+      const int lmrxx = sphere_lmrxx(ws);
+      for (int s = 0; s < ws->nsp; ++s)
+        if (ws->dplan->pot_lmrx[s] != sphere_lmrx(ws, s))
+          return pw_fail(PW_ERR_STATE, "WAVES$ETOT: LMRX of the density object's tables differs from the "
+                                       "one-centre set-up");
+      const double *qlm_dev = nullptr;
+      PW_TRY(sphere_moments(ws, &qlm_dev));
+      {
+        ScopeTimer tm(ws, "vofrho");
+        PW_TRY(vofrho_enqueue(ws->dplan, ws->nat, lmrxx, ws->isp1.data(), ws->r0.data(), nullptr, qlm_dev,
+                              ws->d_rho, ws->d_v, ws->stream));
+      }
+      const double *res = ws->dplan->d_pot_res;
+      PW_TRY(sphere_terms(ws, res, res + (size_t)(lmrxx + 3) * ws->nat + 2));
+    } else {
+      if (ws->qlm.size() != (size_t)ws->nat * ws->qlm_lmrxx || ws->qlm.empty())
+        return pw_fail(PW_ERR_STATE, "WAVES$ETOT: QLM not set");
+      ScopeTimer tm(ws, "vofrho");
+      PW_TRY(vofrho_enqueue(ws->dplan, ws->nat, ws->qlm_lmrxx, ws->isp1.data(), ws->r0.data(), ws->qlm.data(),
+                            nullptr, ws->d_rho, ws->d_v, ws->stream));
+    }
   } else {
     // energy of the fixed local potential: RWEIGHT sum_r V rho
     PW_TRY(k_dot_vrho(ws, ws->d_res + ws->r_vrho));
@@ -864,6 +919,7 @@ extern "C" int pw_waves_etot(pw_waveset *ws) {
   ws->ekin = 0.5 * sum_of(hr + ws->r_ekin, red_blocks(ws)) * p->vol;
   if (ws->dplan) {
     double en[3];
+    ws->vqlm.resize((size_t)ws->nat * ws->dplan->pot_lmrxx);
     vofrho_results(ws->dplan, ws->nat, en, ws->vqlm.data(), ws->fpot.data());
     ws->ehartree = en[0];
     ws->exc = en[1];
@@ -882,14 +938,19 @@ extern "C" int pw_waves_etot(pw_waveset *ws) {
         const size_t ij = ((size_t)a * L + i) * L + j, ji = ((size_t)a * L + j) * L + i;
         ws->denmat[ij] = 0.5 * (dm[ij] + dm[ji]);
       }
-  double e1 = 0.0;
-  for (int a = 0; a < ws->nat; ++a)
-    for (int i = 0; i < L; ++i)
-      for (int j = 0; j < L; ++j) {
-        const size_t src = ((size_t)a * L + j) * L + i;   // DH(i,j,a) column-major
-        e1 += ws->dh[2 * src] * ws->denmat[((size_t)a * L + j) * L + i];
-      }
-  ws->e1c = e1;
+  if (ws->dplan && sphere_ready(ws)) {
+    // AUGMENTATION$SPHERE's one-centre energy and POTB, from the device
+    sphere_results(ws, &ws->e1c, &ws->potb);
+  } else {
+    double e1 = 0.0;
+    for (int a = 0; a < ws->nat; ++a)
+      for (int i = 0; i < L; ++i)
+        for (int j = 0; j < L; ++j) {
+          const size_t src = ((size_t)a * L + j) * L + i;   // DH(i,j,a) column-major
+          e1 += ws->dh[2 * src] * ws->denmat[((size_t)a * L + j) * L + i];
+        }
+    ws->e1c = e1;
+  }
   ws->etot = ws->ekin + ws->epot + ws->e1c;
   // forces: sum the projector contributions of each atom (fixed order)
   const double *fp = hr + ws->r_force;
@@ -935,7 +996,7 @@ extern "C" int pw_waves_set_potential(pw_waveset *ws, pw_plan *dplan) {
   ws->dplan = dplan;
   ws->isp1.resize(ws->nat);
   for (int a = 0; a < ws->nat; ++a) ws->isp1[a] = ws->isp[a] + 1;
-  ws->vqlm.assign(ws->nat, 0.0);
+  ws->vqlm.assign((size_t)ws->nat * std::max(ws->qlm_lmrxx, 1), 0.0);
   ws->fpot.assign(3 * (size_t)ws->nat, 0.0);
   if (dplan) ws->have_v = true;   // VOFR is produced by ETOT
   return PW_OK;

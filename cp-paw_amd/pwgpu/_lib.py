@@ -56,6 +56,9 @@ SIGNATURES = {
     "pw_plan_getr8a": [_vp, _cp, ctypes.c_int, _c_dbl_p],
     "pw_plan_geti4a": [_vp, _cp, ctypes.c_int, _c_int_p],
     "pw_fft": [_vp, _cp, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int, _vp, ctypes.c_int],
+    "pw_waves_sphere_setup": [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                              ctypes.c_double, ctypes.c_double, ctypes.c_int, _c_dbl_p, _c_dbl_p, _c_dbl_p,
+                              _c_dbl_p, _c_dbl_p, _c_dbl_p],
     "pw_plan_set_stream": [_vp, _vp],
     "pw_waves_create": [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _c_int_p, _c_int_p,
                         ctypes.c_int, _c_int_p, ctypes.c_int, ctypes.POINTER(_vp)],
@@ -81,9 +84,10 @@ SIGNATURES = {
     "pw_waves_vpsi": [_vp, _vp, _vp],
     "pw_waves_density": [_vp, _vp, _vp],
     "pw_ortho_x": [ctypes.c_int, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_int_p],
-    "pw_potential_tables": [_vp, ctypes.c_int, _c_dbl_p],
+    "pw_potential_tables": [_vp, ctypes.c_int, _c_int_p, _c_dbl_p],
     "pw_waves_set_potential": [_vp, _vp],
-    "pw_vofrho": [_vp, ctypes.c_int, _c_int_p, _c_dbl_p, _c_dbl_p, _vp, _vp, _c_dbl_p, _c_dbl_p, _c_dbl_p],
+    "pw_vofrho": [_vp, ctypes.c_int, ctypes.c_int, _c_int_p, _c_dbl_p, _c_dbl_p, _vp, _vp, _c_dbl_p, _c_dbl_p,
+                  _c_dbl_p],
     "pw_waves_autotune": [_vp],
     "pw_waves_footprint": [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _c_int_p, _c_int_p, ctypes.c_int,
                            _c_int_p, ctypes.c_int, ctypes.POINTER(ctypes.c_longlong)],

@@ -14,7 +14,7 @@ CSRC = os.path.join(os.path.dirname(HERE), "csrc")
 LIB = os.path.join(HERE, "libpwgpu.so")
 
 SOURCES = ["pw_plan.cpp", "pw_fft.hip", "pw_kernels.hip", "pw_ortho.hip", "pw_waves.cpp", "pw_serve.cpp",
-           "pw_potential.hip"]
+           "pw_potential.hip", "pw_augment.hip"]
 
 FLAGS = ["-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC", "-shared",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -35,6 +35,12 @@ class Species:
     qcore: float = 0.3           # charge of the pseudo core density
     vbar: float = -0.4           # depth of V-bar (Hartree)
     rvbar: float = 1.1           # width of V-bar
+    # model setup data of the one-centre terms (AUGMENTATION$SPHERE)
+    aez: float = 14.0            # atomic number
+    mass: float = 28.0855        # atomic mass (u), fixes the nuclear radius
+    rcore_ae: float = 0.45       # width of the all-electron core density
+    ae_bump: float = 0.6         # size of the all-electron partial waves' inner part
+    lrhox: int = None            # LRHOX of the structure file (default 2 max(LOX))
 
     @property
     def lnx(self):
@@ -43,6 +49,83 @@ class Species:
     @property
     def lmnx(self):
         return sum(2 * l + 1 for l in self.lox)
+
+    @property
+    def lmrx(self):
+        """LMRX = (LRHOX+1)^2 with LRHOX = min(2 max(LOX), LRHOX of the
+        structure file) (src/paw_setups.f90:1223-1230)."""
+        lr = 2 * max(self.lox)
+        if self.lrhox is not None:
+            lr = min(lr, self.lrhox)
+        return (lr + 1) ** 2
+
+    def sphere_setup(self, r1=1.0e-4, dex=0.05, nr=250):
+        """Setup data of AUGMENTATION$SPHERE on the shifted logarithmic grid
+        r_i = r1 (exp(dex (i-1)) - 1), as a dict of host arrays (Gaussian
+        model forms; the reference builds them in its setup construction):
+          PSPHI_ln = N r^l exp(-r^2/(2 s^2)) for the projector widths s,
+          AEPHI_ln = PSPHI_ln (1 + ae_bump exp(-r^2/(0.35 s)^2)),
+          AECORE Y0 = (aez - zval) Gaussian of width rcore_ae,
+          PSCORE Y0 = qcore Gaussian of width rcore   (as PSCORG),
+          VADD Y0   = vbar exp(-r^2/rvbar^2)           (as VBARG),
+          DEKIN     = 0.8 DH (the spherical one-centre Hamiltonian), DO,
+          RNUC      = mass^(1/3) 1.85635065215e-6 (src/paw_periodictable.f90:95-97,274)."""
+        y0 = 1.0 / math.sqrt(4.0 * PI)
+        xexp = math.exp(dex)
+        ri = np.empty(nr)
+        v = r1 / xexp
+        for i in range(nr):
+            v *= xexp
+            ri[i] = v
+        r = ri - r1
+        r[0] = 1.0e-9
+        phi_ps, phi_ae = [], []
+        for l, s in zip(self.lox, self.sigma):
+            norm = 1.0 / math.sqrt(math.gamma(l + 1.5) * s ** (2 * l + 3) / 2.0)
+            p = norm * r ** l * np.exp(-r * r / (2.0 * s * s))
+            phi_ps.append(p)
+            phi_ae.append(p * (1.0 + self.ae_bump * np.exp(-r * r / (0.35 * s) ** 2)))
+        zc = self.aez - self.zval
+
+        def gauss(q, w):
+            return q * np.exp(-r * r / (w * w)) / (PI * w * w) ** 1.5 / y0
+
+        return {"r1": r1, "dex": dex, "nr": nr, "aez": self.aez, "rnuc": self.mass ** (1.0 / 3.0) * 1.85635065215e-6,
+                "lox": list(self.lox), "aephi": np.array(phi_ae), "psphi": np.array(phi_ps),
+                "aecore": gauss(zc, self.rcore_ae), "pscore": gauss(self.qcore, self.rcore),
+                "rcsm": self.rcsm, "vadd": self.vbar * np.exp(-r * r / self.rvbar ** 2) / y0,
+                "dover": np.asarray(self.do_ln, dtype=np.float64),
+                "dekin": 0.8 * np.asarray(self.dh_ln, dtype=np.float64), "lmrx": self.lmrx}
+
+    def partial_wave_overlaps(self):
+        """DO(LN,LN') = <AEPHI|AEPHI'> - <PSPHI|PSPHI'> for equal l (zero
+        otherwise) of the model partial waves, by a fine trapezoid rule; with
+        it as DO the one-centre charge makes the cell neutral (RHOB ~ 0)."""
+        r = np.linspace(0.0, 30.0, 300001)
+        n = len(self.lox)
+        out = np.zeros((n, n))
+        ph = []
+        for l, s in zip(self.lox, self.sigma):
+            norm = 1.0 / math.sqrt(math.gamma(l + 1.5) * s ** (2 * l + 3) / 2.0)
+            p = norm * r ** l * np.exp(-r * r / (2.0 * s * s))
+            ph.append((p, p * (1.0 + self.ae_bump * np.exp(-r * r / (0.35 * s) ** 2))))
+        for i in range(n):
+            for j in range(n):
+                if self.lox[i] == self.lox[j]:
+                    f = (ph[i][1] * ph[j][1] - ph[i][0] * ph[j][0]) * r * r
+                    out[i, j] = float(np.sum(0.5 * (f[1:] + f[:-1])) * (r[1] - r[0]))
+        return out
+
+
+def paw_system(system):
+    """A copy of a system whose species' DO are their partial-wave overlap
+    differences (Species.partial_wave_overlaps), for runs with the one-centre
+    terms on the device."""
+    import copy
+    s = copy.deepcopy(system)
+    for sp in s.species:
+        sp.do_ln = sp.partial_wave_overlaps()
+    return s
 
 
 @dataclass
@@ -168,9 +251,11 @@ class System:
         construction, src/paw_setups.f90): with Y0 = 1/sqrt(4 pi),
           PSCORG Y0 = qcore exp(-G^2 rcore^2/4)/Omega   (core charge qcore),
           VBARG Y0  = vbar (pi rvbar^2)^1.5 exp(-G^2 rvbar^2/4)/Omega,
-          G0        = exp(-G^2 rcsm^2/4)/Omega           (unit compensation charge),
-          V0        = 4 pi/G^2 (exp(-G^2 r1^2/4) - exp(-G^2 rcsm^2/4))/Omega,
-                      r1 = 0.6 rcsm, with its limit pi (rcsm^2 - r1^2)/Omega at G = 0.
+          G0        = 4 pi exp(-G^2 rcsm^2/4)/Omega      (the Gaussian of unit moment
+                      C_0 exp(-r^2/rcsm^2) Y0 of GAUSSN, in the reference's normalisation:
+                      a moment QLM(1) carries the charge QLM(1)/Y0),
+          V0        = (4 pi)^2/G^2 (exp(-G^2 r1^2/4) - exp(-G^2 rcsm^2/4))/Omega,
+                      r1 = 0.6 rcsm, with its limit 4 pi^2 (rcsm^2 - r1^2)/Omega at G = 0.
         """
         g2 = np.asarray(g2, dtype=np.float64)
         y0 = 1.0 / math.sqrt(4.0 * PI)
@@ -180,20 +265,27 @@ class System:
         for k, sp in enumerate(self.species):
             out[0, k] = sp.qcore * np.exp(-g2 * sp.rcore ** 2 / 4.0) / om / y0
             out[1, k] = sp.vbar * (PI * sp.rvbar ** 2) ** 1.5 * np.exp(-g2 * sp.rvbar ** 2 / 4.0) / om / y0
-            out[2, k] = np.exp(-g2 * sp.rcsm ** 2 / 4.0) / om
+            out[2, k] = 4.0 * PI * np.exp(-g2 * sp.rcsm ** 2 / 4.0) / om
             r1 = 0.6 * sp.rcsm
-            v0 = np.full(len(g2), PI * (sp.rcsm ** 2 - r1 ** 2) / om)
-            v0[nz] = 4.0 * PI / g2[nz] * (np.exp(-g2[nz] * r1 ** 2 / 4.0) - np.exp(-g2[nz] * sp.rcsm ** 2 / 4.0)) / om
+            v0 = np.full(len(g2), 4.0 * PI * PI * (sp.rcsm ** 2 - r1 ** 2) / om)
+            v0[nz] = (4.0 * PI) ** 2 / g2[nz] * (np.exp(-g2[nz] * r1 ** 2 / 4.0)
+                                                 - np.exp(-g2[nz] * sp.rcsm ** 2 / 4.0)) / om
             out[3, k] = v0
         return out
 
-    def qlm(self):
-        """QLM(1,NAT): monopoles of the one-centre charges, -zval/Y0 plus a
-        seeded deviation of 0.05/Y0 (so that dE/dQLM is exercised)."""
+    def qlm(self, lmrxx=1):
+        """QLM(LMRXX,NAT) as (NAT,) for LMRXX = 1, else (NAT, LMRXX): the
+        monopole -zval Y0 (charge -zval) plus a seeded deviation of 0.05 Y0,
+        and seeded higher moments of 0.02 (so that every dE/dQLM is exercised)."""
         rng = np.random.default_rng(self.seed + 23)
         y0 = 1.0 / math.sqrt(4.0 * PI)
         z = np.array([self.species[s].zval for s in self.isp], dtype=np.float64)
-        return (-z + rng.normal(0.0, 0.05, self.nat)) / y0
+        q0 = (-z + rng.normal(0.0, 0.05, self.nat)) * y0
+        if lmrxx == 1:
+            return q0
+        out = rng.normal(0.0, 0.02, (self.nat, lmrxx))
+        out[:, 0] = q0
+        return out
 
     def valence_density(self, nr, width=1.2, background=2.0e-3):
         """Model pseudo valence density on the (NR1,NR2,NR3) grid as
@@ -356,7 +448,8 @@ def o_species():
     dh = np.array([[-0.80, 0.12, 0, 0], [0.12, 0.30, 0, 0],
                    [0, 0, -0.45, 0.08], [0, 0, 0.08, 0.25]], dtype=np.float64)
     return Species(lox=[0, 0, 1, 1], sigma=[0.55, 0.85, 0.6, 0.9], do_ln=do, dh_ln=dh,
-                   zval=6.0, rcsm=0.7, rcore=0.5, qcore=0.2, vbar=-0.8, rvbar=0.7)
+                   zval=6.0, rcsm=0.7, rcore=0.5, qcore=0.2, vbar=-0.8, rvbar=0.7,
+                   aez=8.0, mass=15.9994, rcore_ae=0.25)
 
 
 def h_species():
@@ -364,7 +457,8 @@ def h_species():
     do = np.array([[0.25, 0.05], [0.05, 0.12]], dtype=np.float64)
     dh = np.array([[-0.35, 0.06], [0.06, 0.18]], dtype=np.float64)
     return Species(lox=[0, 0], sigma=[0.45, 0.75], do_ln=do, dh_ln=dh,
-                   zval=1.0, rcsm=0.5, rcore=0.4, qcore=0.0, vbar=-0.3, rvbar=0.5)
+                   zval=1.0, rcsm=0.5, rcore=0.4, qcore=0.0, vbar=-0.3, rvbar=0.5,
+                   aez=1.0, mass=1.00794, rcore_ae=0.3)
 
 
 def water(nmol=32, a=19.7, seed=9, epw_psi_ry=40.0, cdual=4.0, dt=4.0, emass=1000.0,

@@ -61,6 +61,7 @@ class Waves:
         self.handle = h
         self.prec = prec
         self.npro = self.geti4("NPRO")
+        self.qlm_lmrxx = 1
 
     def close(self):
         if self.handle:
@@ -99,7 +100,8 @@ class Waves:
         shapes = {"RHO": (self.plane.nr[2], self.plane.nr[1], self.plane.nr[0]),
                   "FORCE": (self.nat, 3), "EIG": (self.nb,), "MAT": (self.nb, self.nb),
                   "OMAT": (self.nb, self.nb), "OOMAT": (self.nb, self.nb),
-                  "DENMAT": (self.nat, L, L), "VQLM": (self.nat,)}
+                  "DENMAT": (self.nat, L, L), "VQLM": (self.nat, self.qlm_lmrxx),
+                  "QLM": (self.nat, self.qlm_lmrxx)}
         out = np.empty(shapes[key], dtype=np.float64)
         check(self.lib.pw_waves_getr8a(self.handle, key.encode(), out.size, dptr(out)))
         if key in ("MAT", "OMAT", "OOMAT"):
@@ -157,12 +159,35 @@ class Waves:
         (pw_waves_autotune; PSI0 and R0 must be set); cached per shape."""
         check(self.lib.pw_waves_autotune(self.handle))
 
-    def set_potential(self, potential, qlm):
+    def set_potential(self, potential, qlm=None):
         """Link the density -> potential step (a pwgpu.Potential on the
-        density object of this cell) into etot; qlm = QLM(1,NAT)."""
-        self.setr8a("QLM", qlm)
+        density object of this cell) into etot; qlm = QLM(LMRXX,NAT) as (NAT,
+        LMRXX) or (NAT,), or None when the one-centre terms produce it."""
+        if qlm is not None:
+            q = np.asarray(qlm, dtype=np.float64).reshape(self.nat, -1)
+            self.qlm_lmrxx = q.shape[1]
+            self.setr8a("QLM", q)
         check(self.lib.pw_waves_set_potential(self.handle, potential.plane.handle))
         self._potential = potential            # keeps the density object alive
+
+    def sphere_setup(self, isp, d):
+        """Setup data of the one-centre terms for species isp (0-based), from
+        a dict with the keys of pwgpu.model.Species.sphere_setup():
+        r1, dex, nr, aez, rnuc, rcsm, lmrx, aephi/psphi (LNX, NR), aecore,
+        pscore, vadd (NR), dekin (LNX, LNX) (pw_waves_sphere_setup)."""
+        f = lambda a: np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+        check(self.lib.pw_waves_sphere_setup(
+            self.handle, int(isp) + 1, int(d["nr"]), float(d["r1"]), float(d["dex"]), float(d["aez"]),
+            float(d["rnuc"]), float(d["rcsm"]), int(d["lmrx"]), dptr(f(d["aephi"])), dptr(f(d["psphi"])),
+            dptr(f(d["aecore"])), dptr(f(d["pscore"])), dptr(f(d["vadd"])), dptr(f(np.asarray(d["dekin"]).T))))
+        self.qlm_lmrxx = max(self.qlm_lmrxx, int(d["lmrx"]))
+
+    def getc8a_dh(self):
+        """DH(LMNXX,LMNXX,NAT) as per-atom (LMNXX, LMNXX) complex matrices [atom][i][j]."""
+        L = self.lmnxx
+        out = np.empty((self.nat, L, L), dtype=np.complex128)
+        check(self.lib.pw_waves_getc8a(self.handle, b"DH", out.size, dptr(out)))
+        return np.transpose(out, (0, 2, 1)).copy()
 
     def write_restart(self, path, static=False):
         """WAVES$WRITE into a restart file of the reference's format

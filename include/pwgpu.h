@@ -185,32 +185,60 @@ int pw_ortho_x(int nb, const double *occ, const double *chichi,
  * SUPFFT of the density, POTENTIAL_ADDCORE :1038, POTENTIAL_HARTREE :1069,
  * POTENTIAL_XC :1339, POTENTIAL_FPSCORE :897) for one spin, the local-density
  * functional of type 1 (local exchange + Perdew-Zunger, src/paw_dft.f90:
- * 124-130), compensation charges with l = 0 (LMRXX = 1), no pair potential
- * of the compensation charges and no isolation.  The plan is the density
- * plane-wave object (PLANEWAVE$SELECT('DENSITY')): its sphere is that of the
- * density cutoff.  FP64.
+ * 124-130), compensation charges of the multipole moments QLM(LMRXX,NAT)
+ * with l <= 4 (LMRXX <= 25; POTENTIAL_HARTREE :1145-1173, POTENTIAL_YLMOFG
+ * :1548), no pair potential of the compensation charges and no isolation.
+ * The plan is the density plane-wave object (PLANEWAVE$SELECT('DENSITY')):
+ * its sphere is that of the density cutoff.  FP64.
  * pw_potential_tables sets the setup tables the reference keeps as module
  * data, on the sphere and resident on the device until replaced:
- *   tabs[4][nsp][NGL] = PSCORG, VBARG, G0, V0  (each G fastest).
+ *   tabs[4][nsp][NGL] = PSCORG, VBARG, G0, V0  (each G fastest),
+ *   lmrx(NSP) = LMRX of each species (SETUP$GETI4('LMRX')), NULL for all 1.
  * pw_vofrho: rho_dev (NRL, x fastest) is the pseudo valence density;
  * vofr_dev (NRL) receives the total local potential dE/drho (may alias
- * rho_dev).  isp(NAT) 1-based species, pos R(3,NAT), qlm QLM(1,NAT).
- * Host outputs: energies = {EHARTREE, EXC, RHOB}, vqlm(NAT) = dE/dQLM(1,:),
- * force(3,NAT) = the Hartree, V-bar and pseudo-core forces (each may be
- * NULL except energies). */
-int pw_potential_tables(pw_plan *dplan, int nsp, const double *tabs);
-int pw_vofrho(pw_plan *dplan, int nat, const int *isp, const double *pos,
+ * rho_dev).  isp(NAT) 1-based species, pos R(3,NAT), qlm QLM(LMRXX,NAT).
+ * Host outputs: energies = {EHARTREE, EXC, RHOB}, vqlm VQLM(LMRXX,NAT) =
+ * dE/dQLM (zero beyond a species' LMRX), force(3,NAT) = the Hartree, V-bar
+ * and pseudo-core forces (each may be NULL except energies). */
+int pw_potential_tables(pw_plan *dplan, int nsp, const int *lmrx, const double *tabs);
+int pw_vofrho(pw_plan *dplan, int nat, int lmrxx, const int *isp, const double *pos,
               const double *qlm, const double *rho_dev, double *vofr_dev,
               double *energies, double *vqlm, double *force);
 /* Links the density -> potential step into WAVES$ETOT, as the reference's
  * energy evaluation calls POTENTIAL$VOFRHO between WAVES_DENSITY and
  * WAVES_VPSI: each ETOT then forms V(r) from the density of PSI0 on the
- * device (positions R0, QLM from WAVES$SETR8A('QLM')), uses it for HPSI,
+ * device (positions R0, QLM(LMRXX,NAT) from WAVES$SETR8A('QLM'), or from
+ * the one-centre terms when pw_waves_sphere_setup has given every species
+ * its setup data), uses it for HPSI,
  * takes EPOT = EHARTREE + EXC and adds the potential forces to FORCE.
  * dplan must have the waveset's cell and r-space grid and its setup tables;
  * NULL detaches (VOFR is then the caller's again).  Calls on one density
  * object must not overlap. */
 int pw_waves_set_potential(pw_waveset *ws, pw_plan *dplan);
+
+/* ---- one-centre terms (AUGMENTATION$SPHERE) ---------------------------------
+ * The setup data of species isp (1-based) that AUGMENTATION$SPHERE and
+ * AUGMENTATION$MOMENTS read from the SETUP object (src/paw_augmentation.f90:
+ * 265-305, 171-193): the shifted logarithmic radial grid r_i = r1 (exp(dex
+ * (i-1)) - 1), i = 1..nr (SHLOGRADIAL, src/paw_radial.f90:1626-1655), the
+ * atomic number aez and nuclear radius rnuc (RADIAL$NUCPOT :1464), the width
+ * rcsm of the compensation Gaussians, LMRX, AEPHI(NR,LNX), PSPHI(NR,LNX),
+ * AECORE(NR), PSCORE(NR), VADD(NR) and DEKIN(LNX,LNX); the one-centre
+ * overlap is the waveset's DO.  Limits: NR <= 512, LNX <= 8, LMNX <= 32,
+ * LMRX <= 25.  Once every species has its setup data and a density object is
+ * linked (pw_waves_set_potential, tables with the same LMRX), each
+ * WAVES$ETOT runs on the device, as src/paw_waves1.f90:1843-1915 does:
+ * WAVES$MOMENTS (QLM from the density matrix), POTENTIAL$VOFRHO with these
+ * QLM, WAVES$SPHERE (DH, the one-centre energy E1C, POTB) and
+ * WAVES$ADDCONSTANTPOT (V += POTB, DH += POTB DO); DH is then no input.
+ * One spin (NDIMD=1), local functional of type 1, no soft core, no external
+ * one-centre potential.  Results: WAVES$GETR8 "E1C", "POTB"; GETR8A "QLM",
+ * "VQLM" (LMRXX*NAT); GETC8A "DH" (LMNXX*LMNXX*NAT). */
+int pw_waves_sphere_setup(pw_waveset *ws, int isp, int nr, double r1, double dex,
+                          double aez, double rnuc, double rcsm, int lmrx,
+                          const double *aephi, const double *psphi,
+                          const double *aecore, const double *pscore,
+                          const double *vadd, const double *dekin);
 
 /* ---- launch-shape autotuner and device footprint ---------------------------
  * No counterpart in the reference (FFTW plans with FFTW_MEASURE,

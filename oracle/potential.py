@@ -1,8 +1,9 @@
 """Oracle restatement of the density -> potential step POTENTIAL$VOFRHO
 (cp-paw src/paw_potential.f90:324-896) for one scope: no spin (NDIMD=1), the
 local-density functional of type 1 (local exchange + Perdew-Zunger
-correlation, src/paw_dft.f90:124-130), compensation charges with l = 0 only
-(LMRXX = 1), no confinement, no isolation (ISOLATE) and no pair potential of
+correlation, src/paw_dft.f90:124-130), compensation charges of any l
+(LMRXX = (LX+1)^2, LMRX per species), no confinement, no isolation (ISOLATE)
+and no pair potential of
 the compensation charges (POTENTIAL_PAIRP, which depends only on QLM and the
 positions, not on the density).
 
@@ -82,42 +83,84 @@ def lda(rho):
     return exc, dexc
 
 
-def vofrho(plane, rho_r, pos, isp, qlm, tabs):
+def ylmofg(gvec, lmx):
+    """POTENTIAL_YLMOFG (src/paw_potential.f90:1548-1614): solid harmonics
+    |G|^l Y_LM(G) as (LMX, NG); (Y0, 0, ...) for |G| < 1e-7."""
+    from .waves import ylm
+    g = np.asarray(gvec, dtype=np.float64)
+    out = ylm(g, lmx)
+    gl = np.sqrt(np.sum(g * g, axis=1))
+    for lm in range(lmx):
+        out[lm] *= gl ** int(np.sqrt(lm + 1e-5))
+    small = gl < 1.0e-7
+    out[:, small] = 0.0
+    out[0, small] = Y0
+    return out
+
+
+def cfac(lmx):
+    """CFAC(LM) = (-i)^L / (2L+1)!! of POTENTIAL_HARTREE
+    (src/paw_potential.f90:1146-1152, 1246-1253)."""
+    out = np.zeros(lmx, dtype=np.complex128)
+    c = 1j
+    lm = 0
+    l = 0
+    while lm < lmx:
+        c = -c * 1j / (2 * l + 1)
+        for _ in range(2 * l + 1):
+            if lm < lmx:
+                out[lm] = c
+            lm += 1
+        l += 1
+    return out
+
+
+def vofrho(plane, rho_r, pos, isp, qlm, tabs, lmrx=None):
     """POTENTIAL$VOFRHO -> POTENTIAL_VOFRHO (src/paw_potential.f90:324-484,
     487-896) in the scope of the module docstring.
 
     plane: the density plane-wave object (oracle.planewave.Plane built with the
       density cutoff for the sphere); rho_r (NR3,NR2,NR1) real pseudo density;
-    pos (NAT,3); isp (NAT) species 0..NSP-1; qlm (NAT) monopole moments QLM(1,:);
+    pos (NAT,3); isp (NAT) species 0..NSP-1; qlm: multipole moments of the
+      one-centre densities, QLM(LMRXX,NAT) as (NAT, LMRXX), or (NAT,) for
+      LMRXX = 1; lmrx: LMRX per species (default LMRXX for all);
     tabs: dict of (NSP,NGL) real tables on the sphere: 'pscorg' (PS core),
       'vbarg' (V-bar), 'g0' (compensation Gaussians), 'v0' (their potential).
     Returns dict: v (NR3,NR2,NR1) total local potential, ehartree, exc, rhob,
-      vqlm (NAT), force (NAT,3) (Hartree + pseudo-core terms).
+      vqlm (shape of qlm) = dE/dQLM, force (NAT,3) (Hartree + pseudo-core terms).
     """
     om = plane.vol
     ig0 = plane.ngamma
     g2, gv = plane.g2, plane.gvec
     nonzero = np.arange(plane.ngl) != ig0
     isp = np.asarray(isp)
+    flat = np.ndim(qlm) == 1
     qlm = np.asarray(qlm, dtype=np.float64)
+    qlm = qlm.reshape(len(isp), -1) if qlm.size else np.zeros((len(isp), 1))
+    lmrxx = qlm.shape[1]
+    nsp = int(np.max(isp)) + 1 if len(isp) else 1
+    lmrx = np.full(nsp, lmrxx) if lmrx is None else np.asarray(lmrx)
     psc, vbar, g0, v0 = (np.asarray(tabs[k], dtype=np.float64) for k in ("pscorg", "vbarg", "g0", "v0"))
     eigr = [structurefactor(p, plane.gbas, plane.igvec) for p in np.asarray(pos)]
+    ylmg = ylmofg(gv, lmrxx)
+    cf = cfac(lmrxx)
     # PLANEWAVE$SUPFFT('RTOG') of the density
     rhog = plane.fft_rtog(np.asarray(rho_r, dtype=np.complex128))
     # POTENTIAL_ADDCORE (src/paw_potential.f90:1038-1066)
     for a, e in enumerate(eigr):
         rhog = rhog + psc[isp[a]] * Y0 * e
-    # POTENTIAL_HARTREE (src/paw_potential.f90:1069-1334), LMRX = 1: CFAC(1) = QLM(1),
-    # YLMOFG(1,G) = Y0 (POTENTIAL_YLMOFG, src/paw_potential.f90:1548-1614)
+    # POTENTIAL_HARTREE (src/paw_potential.f90:1069-1334): compensation charges
+    # sum_LM CFAC(LM) QLM(LM) YLMOFG(LM,G) G0(G) exp(-iG.R)
     rho2 = rhog.copy()
     pot = np.zeros(plane.ngl, dtype=np.complex128)
     vb = 0.0
     for a, e in enumerate(eigr):
         s = isp[a]
-        cs = qlm[a] * Y0 * e
+        n = int(lmrx[s])
+        cs = (cf[:n] * qlm[a, :n]) @ ylmg[:n] * e
         rho2 = rho2 + cs * g0[s]
         pot = pot + cs * v0[s] + vbar[s] * Y0 * e
-        vb += qlm[a] * Y0 * v0[s, ig0]
+        vb += qlm[a, 0] * Y0 * v0[s, ig0]
     rhob = -rho2[ig0].real
     rho2[ig0] = 0.0
     vg = np.zeros(plane.ngl, dtype=np.complex128)
@@ -129,16 +172,20 @@ def vofrho(plane, rho_r, pos, isp, qlm, tabs):
     pot = pot + vg
     rho2b = rhog.copy()
     rho2b[ig0] += rhob
-    vqlm = np.zeros(len(eigr))
+    vqlm = np.zeros((len(eigr), lmrxx))
     force = np.zeros((len(eigr), 3))
     for a, e in enumerate(eigr):
         s = isp[a]
+        n = int(lmrx[s])
         aval = (g0[s] * np.conj(vg) + v0[s] * np.conj(rho2b)) * e
-        vqlm[a] = om * np.sum((Y0 * aval).real)
-        csum = qlm[a] * Y0
+        csv = cf[:n, None] * ylmg[:n]                       # CFAC(LM) YLMOFG(LM,G)
+        vqlm[a, :n] = om * np.sum((csv * aval[None, :]).real, axis=1)
+        csum = qlm[a, :n] @ csv
         cval = vbar[s] * Y0 * e * np.conj(rhog)
         sv = (cval + csum * aval).imag
         force[a] = -om * (gv * sv[:, None]).sum(axis=0)
+    if flat:
+        vqlm = vqlm[:, 0]
     # PLANEWAVE$SUPFFT('GTOR') of density + core, POTENTIAL_XC (src/paw_potential.f90:1339-1545)
     rho_tot = plane.fft_gtor(rhog).real
     exc_r, vxc_r = lda(rho_tot)

@@ -530,6 +530,10 @@ class WaveSet:
         # optional density -> potential step: (density Plane, tables dict,
         # species (NAT,), QLM (NAT,)); None keeps the fixed potential v
         self.potential = None
+        # optional one-centre terms: list per species of
+        # oracle.augmentation.SphereSetup (needs the potential step above);
+        # None keeps the fixed one-centre matrices dh
+        self.sphere = None
         self.rlamm = None
         self.first = True
         self.wavekin1 = 0.0
@@ -550,7 +554,29 @@ class WaveSet:
         e["ekin"] = ekin(pl, self.occ, self.psi0)
         dm = denmat(st, self.occ, self.proj)
         self.rho = density(pl, self.occ, self.psi0)
-        if self.potential is not None:
+        if self.potential is not None and self.sphere is not None:
+            # the one-centre terms linked in (src/paw_waves1.f90:1843-1915):
+            # WAVES$MOMENTS -> QLM, POTENTIAL$VOFRHO(rho, QLM) -> VQLM, RHOB,
+            # WAVES$SPHERE -> DH, DO, POTB, WAVES$ADDCONSTANTPOT
+            from . import augmentation as oaug
+            dplane, tabs = self.potential[0], self.potential[1]
+            lmrx = np.array([s.lmrx for s in self.sphere])
+            lmrxx = int(np.max(lmrx))
+            qlm = np.zeros((st.nat, lmrxx))
+            for a in range(st.nat):
+                sp = self.sphere[st.isp[a]]
+                qlm[a, :sp.lmrx] = oaug.moments(sp, dm[a])
+            self.qlm = qlm
+            self.pot = vofrho(dplane, self.rho, r0, st.isp, qlm, tabs, lmrx=lmrx)
+            sph = oaug.waves_sphere(self.sphere, st.isp, dm,
+                                    [self.pot["vqlm"][a, :lmrx[st.isp[a]]] for a in range(st.nat)],
+                                    self.pot["rhob"], pl.vol)
+            self.potb = sph["potb"]
+            self.v = self.pot["v"] + self.potb
+            self.dh = [np.asarray(d, dtype=np.complex128) for d in sph["dh"]]
+            e["epot"] = self.pot["ehartree"] + self.pot["exc"]
+            e["e1c"] = sph["e1c"]
+        elif self.potential is not None:
             # POTENTIAL$VOFRHO between WAVES_DENSITY and WAVES_VPSI: the local
             # potential of this density (src/paw_potential.f90:324-896)
             dplane, tabs, isp, qlm = self.potential
@@ -559,7 +585,8 @@ class WaveSet:
             e["epot"] = self.pot["ehartree"] + self.pot["exc"]
         else:
             e["epot"] = pl.rweight * float(np.sum(self.v * self.rho))
-        e["e1c"] = float(sum(np.real(np.sum(self.dh[i] * dm[i].T)) for i in range(st.nat)))
+        if self.sphere is None or self.potential is None:
+            e["e1c"] = float(sum(np.real(np.sum(self.dh[i] * dm[i].T)) for i in range(st.nat)))
         e["etot"] = e["ekin"] + e["epot"] + e["e1c"]
         self.force = force(pl, st, self.pro, self.ylm, r0, self.occ, self.psi0,
                            self.proj, self.dh, self.rlam0)

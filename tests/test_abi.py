@@ -92,7 +92,7 @@ def test_vofrho_requires_tables():
     qlm = np.zeros(1)
     en = np.zeros(3)
     fake = ctypes.c_void_p(256)
-    rc = lib.pw_vofrho(p.handle, 1, isp.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+    rc = lib.pw_vofrho(p.handle, 1, 1, isp.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
                        pos.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
                        qlm.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), fake, fake,
                        en.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), None, None)

@@ -65,6 +65,31 @@ def test_vofrho_matches_oracle(name):
         f"dv={relerr(g['v'], o['v']):.1e} dF={relerr(g['force'], o['force']):.1e}")
 
 
+@pytest.mark.parametrize("name", ["si8", "h2o4", "si64"])
+def test_vofrho_higher_moments_matches_oracle(name):
+    """Compensation charges of moments to l = 2 (LMRXX = 9), with the LMRX of
+    each species (O and Si 9, H 1)."""
+    import pwgpu
+    s = CASES[name]()
+    say(f"case vofrho LMRXX=9 {name}")
+    pl, rho, tabs = inputs(s)
+    lmrx = np.array([sp.lmrx for sp in s.species])
+    qlm = s.qlm(9)
+    for a, k in enumerate(s.isp):
+        qlm[a, lmrx[k]:] = 0.0
+    o = opot.vofrho(pl, rho, s.pos, s.isp, qlm, {"pscorg": tabs[0], "vbarg": tabs[1], "g0": tabs[2], "v0": tabs[3]},
+                    lmrx=lmrx)
+    dplane = pwgpu.PlaneWave(s.rbas, s.epw_rho, s.epw_rho)
+    pot = pwgpu.Potential(dplane, tabs, lmrx=lmrx)
+    g = pot.vofrho(s.isp, s.pos, qlm, rho)
+    for k in ("ehartree", "exc", "rhob"):
+        assert abs(g[k] - o[k]) <= 1e-10 * max(abs(o[k]), 1e-12), (k, g[k], o[k])
+    assert relerr(g["v"], o["v"]) < 1e-10
+    assert g["vqlm"].shape == (s.nat, 9)
+    assert relerr(g["vqlm"], o["vqlm"]) < 1e-10
+    assert relerr(g["force"], o["force"]) < 1e-9
+
+
 def test_vofrho_in_place_and_repeatable():
     # vofr may alias rho; two calls give bit-identical results (ordered sums)
     import pwgpu

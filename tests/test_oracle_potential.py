@@ -184,3 +184,75 @@ def test_scf_hpsi_is_energy_gradient():
     # central differences (error O(eps^4))
     fd = (4.0 * cd(5e-4) - cd(1e-3)) / 3.0
     assert abs(fd - an) < 1e-8 * abs(an), (fd, an)
+
+
+def test_compensation_gaussians_closed_form():
+    """The compensation density of a moment QLM(LM) in G space, CFAC(LM)
+    YLMOFG(LM,G) G0(G) exp(-iG.R) with G0 = 4 pi exp(-G^2 rcsm^2/4)/Omega
+    (POTENTIAL_HARTREE, src/paw_potential.f90:1146-1173), equals the plane-wave
+    coefficients of the real-space Gaussian QLM C_l exp(-r^2/rcsm^2) r^l Y_LM
+    normalised by GAUSSN (src/paw_generalpurpose.f90:1088-1121) to unit
+    multipole moment, for l = 0, 1, 2 and every m: pins (-i)^l/(2l+1)!!, the
+    solid harmonics and their order."""
+    from oracle import augmentation as oaug
+    from oracle import waves as owv
+    a = 9.0
+    pl = opw.Plane(np.diag([a, a, a]), 60.0, 60.0)
+    rcsm = 0.9
+    alpha = 1.0 / rcsm ** 2
+    R = np.array([3.1, 4.7, 5.3])
+    i3, i2, i1 = np.meshgrid(np.arange(pl.nr3), np.arange(pl.nr2), np.arange(pl.nr1), indexing="ij")
+    pts = np.stack([i1 / pl.nr1 * a, i2 / pl.nr2 * a, i3 / pl.nr3 * a], axis=-1).reshape(-1, 3)
+    g0 = 4.0 * np.pi * np.exp(-pl.g2 * rcsm ** 2 / 4.0) / pl.vol
+    ylmg = opot.ylmofg(pl.gvec, 9)
+    cf = opot.cfac(9)
+    eig = opw.structurefactor(R, pl.gbas, pl.igvec)
+    for lm in range(9):
+        l = int(np.sqrt(lm + 1e-5))
+        c = oaug.gaussn(l, alpha)
+        f = np.zeros(len(pts))
+        for sh in np.array(np.meshgrid([-1, 0, 1], [-1, 0, 1], [-1, 0, 1])).reshape(3, -1).T:
+            d = pts - R - sh * a
+            rr = np.sqrt(np.sum(d * d, axis=1))
+            y = owv.ylm(d, 9)[lm]
+            f += c * np.exp(-alpha * rr ** 2) * rr ** l * y
+        want = pl.fft_rtog(f.reshape(pl.nr3, pl.nr2, pl.nr1).astype(complex))
+        got = cf[lm] * ylmg[lm] * g0 * eig
+        assert np.max(np.abs(got - want)) < 1e-9 * np.max(np.abs(want)), lm
+
+
+def test_vqlm_and_forces_with_higher_moments():
+    """LMRXX = 9 (moments to l = 2) with per-species LMRX 9 and 4: VQLM is the
+    QLM derivative of EHARTREE + EXC for every LM, zero beyond a species'
+    LMRX, and the forces are the position derivatives."""
+    s, pl, rho, tabs = system_case()
+    qlm = s.qlm(9)
+    lmrx = np.array([9, 4])
+    qlm[s.isp == 1, 4:] = 0.0
+
+    def en(q, pos):
+        r = opot.vofrho(pl, rho, pos, s.isp, q, tabs, lmrx=lmrx)
+        return r["ehartree"] + r["exc"], r
+
+    _, r = en(qlm, s.pos)
+    assert r["vqlm"].shape == (s.nat, 9)
+    assert np.all(r["vqlm"][s.isp == 1, 4:] == 0.0)
+    h = 1e-4
+    for a in (0, 1):
+        for lm in range(int(lmrx[s.isp[a]])):
+            q = qlm.copy()
+            q[a, lm] += h
+            ep, _ = en(q, s.pos)
+            q[a, lm] -= 2 * h
+            em, _ = en(q, s.pos)
+            fd = (ep - em) / (2 * h)
+            assert abs(fd - r["vqlm"][a, lm]) < 1e-7 * max(1.0, abs(fd)), (a, lm)
+    for a in (0, 2):
+        for k in range(3):
+            p = s.pos.copy()
+            p[a, k] += h
+            ep, _ = en(qlm, p)
+            p[a, k] -= 2 * h
+            em, _ = en(qlm, p)
+            fd = -(ep - em) / (2 * h)
+            assert abs(fd - r["force"][a, k]) < 1e-6 * max(1e-3, np.max(np.abs(r["force"]))), (a, k)
