@@ -252,7 +252,8 @@ def run_gpu(system_name, prec, steps, warmup, profile):
     res = {"steps_per_s": steps / dt, "ms_per_step": 1e3 * dt / steps, "energies": e,
            "ortho_iter": w.geti4("ORTHOITER"), "ortho_levels": w.geti4("ORTHOLEVELS"),
            "ortho_ns": w.geti4("ORTHONS"), "ortho_series": w.geti4("ORTHOSERIES"),
-           "launch_shapes": {"split_waves": w.geti4("SPLITWAVES"), "density_blocks": w.geti4("DENSBLOCKS")},
+           "launch_shapes": {"split_waves": w.geti4("SPLITWAVES"), "density_blocks": w.geti4("DENSBLOCKS"),
+                             "col_waves": w.geti4("COLWAVES")},
            "desc": f"Gamma, {s.nat} atoms, {s.nb} bands, {plane.ngl} G, {plane.nr[0]}x{plane.nr[1]}x{plane.nr[2]}, "
                    f"{s.npro} projectors"}
     if profile:

@@ -775,7 +775,7 @@ static int sphere_upload(pw_waveset *ws) {
     if (*dst) PW_HIP(hipFree(*dst));
     *dst = nullptr;
     PW_HIP(hipMalloc(dst, std::max<size_t>(bytes, 8)));
-    if (bytes) PW_HIP(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
+    if (bytes && src) PW_HIP(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
     return PW_OK;
   };
   PW_TRY(up((void **)&s->d_spec, spec.data(), spec.size() * sizeof(SphSpec)));

@@ -222,6 +222,7 @@ struct pw_waveset {
   int ortho_graph_misses = 0;  // ORTHO_X solves issued without a graph (other objects alive)
   // launch shapes chosen by pw_waves_autotune
   int tune_split_waves = 2048;  // waves of a G-split GEMM launch
+  int tune_col_waves = 4;       // waves (16 G each) of a column-product workgroup (mixed form: 2 or 4)
   // device results of a step section and their pinned host copy; the host
   // block has NB*NB more doubles that stage the DEDPROJ constraint matrix
   double *d_res = nullptr, *h_res = nullptr;

@@ -1789,6 +1789,7 @@ int k_addpro(pw_waveset *ws, double2 *psi, const double2 *phase, const double2 *
       ga = ARuns{split_runs<false>(ws, SrcXRows{x, ws->nbh, ws->npro}, ws->nbh, ws->npro, 0)};
     }
     AddproBf gb{pfac_of(ws), ws->npro};
+    if (ws->tune_col_waves == 2) return run_colgemm_bf<2>(ws, ga, gb, ws->npro, psi, "addpro");
     return run_colgemm_bf<4>(ws, ga, gb, ws->npro, psi, "addpro");
   }
   ProjGeo q = make_pgeo(ws);
@@ -1909,6 +1910,7 @@ int k_addopsi(pw_waveset *ws, double2 *psibar, const double2 *chi, const double 
       ga = ARuns{split_runs<false>(ws, SrcLamRows{lam, ws->nb, ws->nbh}, ws->nbh, nk, 0)};
       gb = ColRuns{split_runs<true>(ws, SrcChiCols{chi, p->d_minusg, p->ngl, ws->nbh}, p->ngl, nk, ba)};
     }
+    if (ws->tune_col_waves == 2) return run_colgemm_bf<2>(ws, ga, gb, nk, psibar, "addopsi");
     return run_colgemm_bf<4>(ws, ga, gb, nk, psibar, "addopsi");
   }
   AddopsiA ga{lam, ws->nb, ws->nbh};

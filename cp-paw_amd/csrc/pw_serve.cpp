@@ -10,6 +10,7 @@
 // resident in HBM from admission to completion.  Results do not depend on
 // what else runs: every kernel sums in a fixed order and no state is shared.
 // Launch shapes come from pw_waves_autotune, once per system shape.
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
@@ -48,7 +49,7 @@ using Clock = std::chrono::steady_clock;
 
 // launch shapes per system shape, shared by all jobs of the process
 std::mutex g_tune_mu;
-std::map<std::tuple<int, int, int, int, int>, std::pair<int, int>> g_tuned;
+std::map<std::tuple<int, int, int, int, int>, std::array<int, 3>> g_tuned;
 
 }  // namespace
 
@@ -187,8 +188,9 @@ extern "C" int pw_waves_autotune(pw_waveset *ws) {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     auto it = g_tuned.find(key);
     if (it != g_tuned.end()) {
-      ws->tune_split_waves = it->second.first;
-      ws->tune_dens_blocks = it->second.second;
+      ws->tune_split_waves = it->second[0];
+      ws->tune_dens_blocks = it->second[1];
+      ws->tune_col_waves = it->second[2];
       return PW_OK;
     }
   }
@@ -208,7 +210,7 @@ extern "C" int pw_waves_autotune(pw_waveset *ws) {
     return best;
   };
   int rc = pw_waves_projections(ws);   // phase tables (and split projectors) for R0
-  int best_split = ws->tune_split_waves, best_dens = ws->tune_dens_blocks;
+  int best_split = ws->tune_split_waves, best_dens = ws->tune_dens_blocks, best_col = ws->tune_col_waves;
   if (rc == PW_OK) {
     // G-split width of the row products: more splits fill the GPU, fewer
     // shrink the partial sums their ordered reduction reads back; timed on
@@ -232,14 +234,27 @@ extern "C" int pw_waves_autotune(pw_waveset *ws) {
       });
       if (t < tbest) { tbest = t; best_dens = cand; }
     }
+    if (ws->prec == PW_PREC_MIXED) {
+      // workgroup width of the column products (ADDPRO, ADDOPSI): 4 waves
+      // share the coefficient tile over 64 G, 2 waves give twice the
+      // workgroups over 32 G; timed on ADDPRO into the OPSI buffer, which
+      // every ORTHOGONALIZE rebuilds from PSI0
+      tbest = 1e30;
+      for (int cand : {2, 4}) {
+        ws->tune_col_waves = cand;
+        const double t = timed([&] { return k_addpro(ws, ws->d_opsi, ws->d_phase, ws->d_proj); });
+        if (t < tbest) { tbest = t; best_col = cand; }
+      }
+    }
   }
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
   if (rc != PW_OK) return rc;
   ws->tune_split_waves = best_split;
   ws->tune_dens_blocks = best_dens;
+  ws->tune_col_waves = best_col;
   std::lock_guard<std::mutex> lk(g_tune_mu);
-  g_tuned[key] = {best_split, best_dens};
+  g_tuned[key] = {best_split, best_dens, best_col};
   return PW_OK;
 }
 

@@ -774,6 +774,7 @@ extern "C" int pw_waves_geti4(pw_waveset *ws, const char *id, int *val) {
   else if (s == "ORTHONS") *val = ws->ortho_ns;
   else if (s == "ORTHOSERIES") *val = ws->ortho_series;
   else if (s == "SPLITWAVES") *val = ws->tune_split_waves;
+  else if (s == "COLWAVES") *val = ws->tune_col_waves;
   else if (s == "DENSBLOCKS") *val = ws->tune_dens_blocks;
   else if (s == "ORTHOLANCZOS") *val = ws->ortho_lanczos;
   else if (s == "ORTHOGRAPHMISSES") *val = ws->ortho_graph_misses;

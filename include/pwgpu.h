@@ -135,7 +135,10 @@ int pw_waves_getr8(pw_waveset *ws, const char *id, double *val);
  * accepted), "ORTHOGRAPHMISSES" (solves issued without a captured hipGraph:
  * captures are made only while one plan and one waveset exist in the
  * process), "NLAMBDA" (multiplier history held: 0 none, 1 RLAM0, 2 RLAM0
- * and RLAMM; counted as in WAVES_WRITELAMBDA, src/paw_waves2.f90:3948). */
+ * and RLAMM; counted as in WAVES_WRITELAMBDA, src/paw_waves2.f90:3948),
+ * and the launch shapes of pw_waves_autotune: "SPLITWAVES" (waves of a
+ * G-split row-product launch), "DENSBLOCKS" (workgroups of the density pass),
+ * "COLWAVES" (waves of a column-product workgroup, mixed form). */
 int pw_waves_geti4(pw_waveset *ws, const char *id, int *val);
 /* Device pointer of a resident array ("PSI0","PSIM","HPSI","PROJ","VOFR",
  * "RHO") for zero-copy access by the caller. */
