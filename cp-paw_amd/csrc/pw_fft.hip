@@ -813,9 +813,8 @@ int run_inverse_x(pw_plan *p, const Geo &g, int nb, double2 *out, const double2 
 }
 
 template <typename T2>
-int run_forward_xy(pw_plan *p, const Geo &g, int nb, const double2 *psi, hipStream_t s) {
-  T2 *stripes = (T2 *)p->d_stripe_buf;
-  T2 *planes = (T2 *)p->d_plane_buf;
+int run_forward_xy(pw_plan *p, const Geo &g, int nb, const double2 *psi, T2 *stripes, T2 *planes,
+                   hipStream_t s) {
   const int ntx = (p->nr1 + kTX - 1) / kTX;
   constexpr int PX = XTile<T2>::n;
   dim3 gx((p->nstripe + PX - 1) / PX, nb);
@@ -829,10 +828,8 @@ int run_forward_xy(pw_plan *p, const Geo &g, int nb, const double2 *psi, hipStre
 }
 
 template <typename T2>
-int run_inverse_yx(pw_plan *p, const Geo &g, int nb, double2 *out, const double2 *psi,
-                   double scale, int add_kin, hipStream_t s) {
-  T2 *stripes = (T2 *)p->d_stripe_buf;
-  T2 *planes = (T2 *)p->d_plane_buf;
+int run_inverse_yx(pw_plan *p, const Geo &g, int nb, double2 *out, T2 *stripes, T2 *planes,
+                   hipStream_t s) {
   const int ntx = (p->nr1 + kTX - 1) / kTX;
   dim3 gy(p->nzp * ntx, nb);
   k_yinv<T2><<<gy, kBlock, 2 * kTX * p->nr2 * sizeof(T2), s>>>(g, p->f[1], twid<T2>(p, 1),
@@ -840,7 +837,7 @@ int run_inverse_yx(pw_plan *p, const Geo &g, int nb, double2 *out, const double2
   constexpr int PX = XTile<T2>::n;
   dim3 gx((p->nstripe + PX - 1) / PX, nb);
   k_xinv<T2><<<gx, kBlock, 2 * PX * p->nr1 * sizeof(T2), s>>>(g, p->f[0], twid<T2>(p, 0),
-                                                              stripes, out, psi, scale, add_kin);
+                                                              stripes, out, nullptr, 1.0 / (double)p->nrl, 0);
   PW_HIP(hipGetLastError());
   return PW_OK;
 }
@@ -948,27 +945,47 @@ int density_t(pw_plan *p, int nb, const double2 *psi, const double *occ2, double
   return PW_OK;
 }
 
+// full-grid transforms of nb bands through the work buffers stripes
+// (nb x NSTRIPE x NR1) and planes (nb x NZPLANE x NR2 x NR1); every element of
+// both is written before it is read, so neither needs clearing
 template <typename T2>
-int gtor_t(pw_plan *p, int nb, const double2 *fofg, T2 *fofr, hipStream_t s) {
+int gtor_t(pw_plan *p, int nb, const double2 *fofg, T2 *fofr, T2 *stripes, T2 *planes, hipStream_t s) {
   Geo g = make_geo(p);
-  PW_TRY(run_forward_xy<T2>(p, g, nb, fofg, s));
+  PW_TRY(run_forward_xy<T2>(p, g, nb, fofg, stripes, planes, s));
   const int ntx = (p->nr1 + kTX - 1) / kTX;
   dim3 gz(p->nr2 * ntx, nb);
-  k_zout<T2><<<gz, kBlock, 2 * kTX * p->nr3 * sizeof(T2), s>>>(g, p->f[2], twid<T2>(p, 2),
-                                                               (const T2 *)p->d_plane_buf, fofr);
+  k_zout<T2><<<gz, kBlock, 2 * kTX * p->nr3 * sizeof(T2), s>>>(g, p->f[2], twid<T2>(p, 2), planes, fofr);
   PW_HIP(hipGetLastError());
   return PW_OK;
 }
 
 template <typename T2>
-int rtog_t(pw_plan *p, int nb, double2 *fofg, const T2 *fofr, hipStream_t s) {
+int rtog_t(pw_plan *p, int nb, double2 *fofg, const T2 *fofr, T2 *stripes, T2 *planes, hipStream_t s) {
   Geo g = make_geo(p);
   const int ntx = (p->nr1 + kTX - 1) / kTX;
   dim3 gz(p->nr2 * ntx, nb);
-  k_zin<T2><<<gz, kBlock, 2 * kTX * p->nr3 * sizeof(T2), s>>>(g, p->f[2], twid<T2>(p, 2), fofr,
-                                                              (T2 *)p->d_plane_buf);
+  k_zin<T2><<<gz, kBlock, 2 * kTX * p->nr3 * sizeof(T2), s>>>(g, p->f[2], twid<T2>(p, 2), fofr, planes);
   PW_HIP(hipGetLastError());
-  return run_inverse_yx<T2>(p, g, nb, fofg, nullptr, 1.0 / (double)p->nrl, 0, s);
+  return run_inverse_yx<T2>(p, g, nb, fofg, stripes, planes, s);
+}
+
+// full-grid transforms in batches of at most c bands through the given buffers
+int full_fft(pw_plan *p, bool gtor, int nfft, int c, double2 *fofg, void *fofr, int prec, void *stripes,
+             void *planes, hipStream_t s) {
+  for (int b0 = 0; b0 < nfft; b0 += c) {
+    const int n = std::min(c, nfft - b0);
+    double2 *fg = fofg + (size_t)b0 * p->ngl;
+    if (prec == PW_PREC_FP64) {
+      double2 *fr = (double2 *)fofr + (size_t)b0 * p->nrl;
+      if (gtor) PW_TRY(gtor_t<double2>(p, n, fg, fr, (double2 *)stripes, (double2 *)planes, s));
+      else PW_TRY(rtog_t<double2>(p, n, fg, fr, (double2 *)stripes, (double2 *)planes, s));
+    } else {
+      float2 *fr = (float2 *)fofr + (size_t)b0 * p->nrl;
+      if (gtor) PW_TRY(gtor_t<float2>(p, n, fg, fr, (float2 *)stripes, (float2 *)planes, s));
+      else PW_TRY(rtog_t<float2>(p, n, fg, fr, (float2 *)stripes, (float2 *)planes, s));
+    }
+  }
+  return PW_OK;
 }
 
 }  // namespace
@@ -1008,25 +1025,78 @@ int fft_density(pw_plan *p, int nb, const double2 *psi, const double *occ2, doub
 int fft_gtor_full(pw_plan *p, int nfft, const double2 *fofg, void *fofr, int prec, hipStream_t s) {
   const int c = chunk_of(p, prec);
   PW_TRY(plan_work(p, std::min(nfft, c), prec, true));
-  for (int b0 = 0; b0 < nfft; b0 += c) {
-    const int n = std::min(c, nfft - b0);
-    if (prec == PW_PREC_FP64)
-      PW_TRY(gtor_t<double2>(p, n, fofg + (size_t)b0 * p->ngl, (double2 *)fofr + (size_t)b0 * p->nrl, s));
-    else
-      PW_TRY(gtor_t<float2>(p, n, fofg + (size_t)b0 * p->ngl, (float2 *)fofr + (size_t)b0 * p->nrl, s));
-  }
-  return PW_OK;
+  return full_fft(p, true, nfft, c, const_cast<double2 *>(fofg), fofr, prec, p->d_stripe_buf, p->d_plane_buf, s);
 }
 
 int fft_rtog_full(pw_plan *p, int nfft, double2 *fofg, const void *fofr, int prec, hipStream_t s) {
   const int c = chunk_of(p, prec);
   PW_TRY(plan_work(p, std::min(nfft, c), prec, true));
-  for (int b0 = 0; b0 < nfft; b0 += c) {
-    const int n = std::min(c, nfft - b0);
-    if (prec == PW_PREC_FP64)
-      PW_TRY(rtog_t<double2>(p, n, fofg + (size_t)b0 * p->ngl, (const double2 *)fofr + (size_t)b0 * p->nrl, s));
-    else
-      PW_TRY(rtog_t<float2>(p, n, fofg + (size_t)b0 * p->ngl, (const float2 *)fofr + (size_t)b0 * p->nrl, s));
+  return full_fft(p, false, nfft, c, fofg, const_cast<void *>(fofr), prec, p->d_stripe_buf, p->d_plane_buf, s);
+}
+
+// ---------------------------------------------------------------------------
+// PLANEWAVE$FFT without allocation (pw_fft_noalloc).
+//
+// The stripe and plane work buffers of the full-grid transforms live in the
+// file-scope device array `scratch` (kNoAllocScratch bytes per GPU, held
+// until the process ends) instead of the plan's lazily allocated buffers, so
+// the call makes no hipMalloc/hipFree.  In host code `scratch` names a host
+// stand-in; the device address comes from hipGetSymbolAddress, taken once per
+// device with that device current on the first call there, and only that
+// address is handed to kernels.  Every call on one GPU shares the array:
+// calls that overlap on one GPU (two streams, two threads) race.  The
+// kernels write every element of the work buffers before reading it, so the
+// contents left by an earlier call need no clearing.
+// ---------------------------------------------------------------------------
+constexpr size_t kNoAllocScratch = (size_t)64 << 20;
+constexpr int kNoAllocMaxDev = 64;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratch];
+
+static std::mutex g_scratch_mu;
+static void *g_scratch_dev[kNoAllocMaxDev];
+
+// device address of `scratch` on the current device
+static int scratch_ptr(void **out) {
+  int dev = 0;
+  PW_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kNoAllocMaxDev) return pw_fail(PW_ERR_UNSUPPORTED, "pw_fft_noalloc: device index too large");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return pw_fail(PW_ERR_HIP, std::string("pw_fft_noalloc: hipGetSymbolAddress(scratch): ") +
+                                     hipGetErrorString(e));
+    g_scratch_dev[dev] = ptr;
   }
+  *out = g_scratch_dev[dev];
   return PW_OK;
+}
+
+extern "C" int pw_fft_noalloc(pw_plan *p, const char *id, int nfft, int ngl, void *fofg, int nrl, void *fofr,
+                              int prec) {
+  if (!p || !id || !fofg || !fofr) return pw_fail(PW_ERR_ARG, "pw_fft_noalloc: null argument");
+  if (ngl != p->ngl) return pw_fail(PW_ERR_ARG, "PLANEWAVE$FFT: size of F(G) inconsistent");
+  if (nrl != p->nrl) return pw_fail(PW_ERR_ARG, "PLANEWAVE$FFT: size of F(R) inconsistent");
+  if (nfft <= 0) return pw_fail(PW_ERR_ARG, "PLANEWAVE$FFT: NFFT must be positive");
+  if (prec != PW_PREC_FP64 && prec != PW_PREC_MIXED) return pw_fail(PW_ERR_ARG, "pw_fft_noalloc: unknown precision");
+  const std::string s(id);
+  if (s != "GTOR" && s != "RTOG") return pw_fail(PW_ERR_ARG, "PLANEWAVE$FFT: ID must be either GTOR or RTOG");
+  // work bytes of one transform: stripes, then planes at a 256-byte boundary
+  const size_t es = prec == PW_PREC_FP64 ? sizeof(double2) : sizeof(float2);
+  const size_t sb = es * (size_t)p->nstripe * p->nr1, pb = es * (size_t)p->nzp * p->nr2 * p->nr1;
+  const size_t per = sb + pb + 256;
+  if (per > kNoAllocScratch)
+    return pw_fail(PW_ERR_ARG, "pw_fft_noalloc: one transform needs " + std::to_string(per) +
+                                   " bytes of work space, more than the " + std::to_string(kNoAllocScratch) +
+                                   " bytes of the static scratch; use pw_fft");
+  if (!p->uploaded)
+    return pw_fail(PW_ERR_STATE, "pw_fft_noalloc: the plan's tables are not on the device yet "
+                                 "(any pw_fft or waveset call on the plan uploads them)");
+  void *base = nullptr;
+  PW_TRY(scratch_ptr(&base));
+  const int c = (int)std::min<size_t>((size_t)nfft, kNoAllocScratch / per);
+  const size_t soff = ((size_t)c * sb + 255) / 256 * 256;
+  unsigned char *b = (unsigned char *)base;
+  return full_fft(p, s == "GTOR", nfft, c, (double2 *)fofg, fofr, prec, b, b + soff, p->stream);
 }

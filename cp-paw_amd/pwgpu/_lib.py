@@ -56,6 +56,7 @@ SIGNATURES = {
     "pw_plan_getr8a": [_vp, _cp, ctypes.c_int, _c_dbl_p],
     "pw_plan_geti4a": [_vp, _cp, ctypes.c_int, _c_int_p],
     "pw_fft": [_vp, _cp, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int, _vp, ctypes.c_int],
+    "pw_fft_noalloc": [_vp, _cp, ctypes.c_int, ctypes.c_int, _vp, ctypes.c_int, _vp, ctypes.c_int],
     "pw_waves_sphere_setup": [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                               ctypes.c_double, ctypes.c_double, ctypes.c_int, _c_dbl_p, _c_dbl_p, _c_dbl_p,
                               _c_dbl_p, _c_dbl_p, _c_dbl_p],

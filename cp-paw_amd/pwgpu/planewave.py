@@ -109,6 +109,12 @@ class PlaneWave:
         check(self.lib.pw_fft(self.handle, key.encode(), int(nfft), self.ngl, fofg_dev.ptr,
                               self.nrl, fofr_dev.ptr, int(prec)))
 
+    def fft_noalloc(self, key, nfft, fofg_dev, fofr_dev, prec=_lib.PW_PREC_FP64):
+        """PLANEWAVE$FFT on device buffers with the library's static work space
+        (pw_fft_noalloc: no device allocation; the plan must be on the device)."""
+        check(self.lib.pw_fft_noalloc(self.handle, key.encode(), int(nfft), self.ngl, fofg_dev.ptr,
+                                      self.nrl, fofr_dev.ptr, int(prec)))
+
     def fft_host(self, key, fofg=None, fofr=None, prec=_lib.PW_PREC_FP64):
         """Convenience: PLANEWAVE$FFT on host arrays (copies in and out).
 

@@ -88,6 +88,19 @@ int pw_plan_geti4a(const pw_plan *plan, const char *id, int len, int *val);
  * plan's stream. */
 int pw_fft(pw_plan *plan, const char *id, int nfft, int ngl, void *fofg_dev,
            int nrl, void *fofr_dev, int prec);
+/* PLANEWAVE$FFT as pw_fft, for callers that cannot allocate device memory
+ * (code under hipGraph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  The work buffers come from
+ * a static device array of 64 MiB per GPU instead of the plan's lazily
+ * allocated ones, so the call makes no hipMalloc/hipFree; one transform must
+ * fit in it (es*(NSTRIPE*NR1 + NZPLANE*NR2*NR1) + 256 bytes, es = 16 for
+ * PW_PREC_FP64, 8 for PW_PREC_MIXED), else PW_ERR_ARG; larger NFFT run in
+ * batches that fit.  The plan's tables must already be on the device (after
+ * any pw_fft or waveset call on the plan), else PW_ERR_STATE.  All calls on
+ * one GPU share the array: calls that overlap on one GPU race.  Results are
+ * bit-identical to pw_fft's. */
+int pw_fft_noalloc(pw_plan *plan, const char *id, int nfft, int ngl, void *fofg_dev,
+                   int nrl, void *fofr_dev, int prec);
 /* Stream used by the plan (hipStream_t), or 0 for the default stream. */
 int pw_plan_set_stream(pw_plan *plan, void *stream);
 
