@@ -152,14 +152,16 @@ def accuracy(system_name, nsteps, relax_tol=1.0e-8, relax_max=3000, friction=0.1
     plane = pwgpu.PlaneWave(s.rbas, s.epw_psi, s.epw_rho)
     w = pwgpu.Waves.from_system(plane, s, prec=0)
     w.setr8("FRICTION", friction)
-    e_old, calm, nrelax = None, 0, 0
+    e_old, calm, nrelax, de_last = None, 0, 0, None
     while nrelax < relax_max and calm < 5:
         e = w.etot()["etot"]
         w.propagate()
         w.orthogonalize()
         w.switch()
         nrelax += 1
-        calm = calm + 1 if (e_old is not None and abs(e - e_old) < relax_tol) else 0
+        if e_old is not None:
+            de_last = abs(e - e_old)
+        calm = calm + 1 if (de_last is not None and de_last < relax_tol) else 0
         e_old = e
     w.setr8("FRICTION", s.annee)
     t0 = time.perf_counter()
@@ -174,8 +176,11 @@ def accuracy(system_name, nsteps, relax_tol=1.0e-8, relax_max=3000, friction=0.1
     gates = {"dE_per_atom": 1e-6, "dF": 1e-4}
     p_step = bool(res["per_step_max_dE_per_atom_Ha"] < 1e-6 and res["per_step_max_dF_Ha_per_bohr"] < 1e-4)
     p_traj = bool(res["trajectory_max_dE_per_atom_Ha"] < 1e-6 and res["trajectory_max_dF_Ha_per_bohr"] < 1e-4)
-    res.update({"steps": nsteps, "start": f"relaxed: damped FP64 CP (friction {friction}) for {nrelax} steps "
-                                          f"to |dE| < {relax_tol:g} Ha per step",
+    reached = calm >= 5
+    res.update({"steps": nsteps, "start": f"relaxed: damped FP64 CP (friction {friction}) for {nrelax} steps, "
+                                          + (f"|dE| < {relax_tol:g} Ha per step reached" if reached else
+                                             f"stopped at the step limit with |dE| = {de_last:.2e} Ha per step"),
+                "relax_converged": bool(reached), "relax_last_dE_Ha": de_last,
                 "random_start": dict(rnd, steps=random_steps),
                 "gates": gates, "pass_per_step": p_step, "pass_trajectory": p_traj, "pass": p_step and p_traj})
     return res
