@@ -339,7 +339,7 @@ def potential_measure(system_name, calls, steps, with_cpu):
     return out
 
 
-def cpu_baseline(system_name, nsteps=20, nsteps_1t=6):
+def cpu_baseline(system_name, nsteps=20, nsteps_1t=20):
     """The FP64 oracle (numpy) on the host, as the reference's clock takes it
     (first step excluded, src/paw_driver.f90:156-160): on all the BLAS
     threads of the process (the host's share of cores) the median of nsteps
@@ -348,6 +348,7 @@ def cpu_baseline(system_name, nsteps=20, nsteps_1t=6):
     threads = max([i.get("num_threads", 1) for i in threadpool_info()] + [1])
     out = _cpu_oracle_steps(system_name, nsteps)
     out["cores"] = int(threads)
+    out["host_cpus"] = {"nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0))}
     with threadpool_limits(limits=1):
         one = _cpu_oracle_steps(system_name, nsteps_1t)
     one["cores"] = 1
@@ -411,7 +412,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=20,
                     help="timed oracle steps of the CPU baseline on all cores (median)")
-    ap.add_argument("--cpu-steps-1t", type=int, default=6,
+    ap.add_argument("--cpu-steps-1t", type=int, default=20,
                     help="timed oracle steps of the CPU baseline on one thread (median)")
     ap.add_argument("--acc-steps", type=int, default=200,
                     help="steps of the mixed-vs-FP64 error measurement (0: skip)")

@@ -661,7 +661,7 @@ __device__ inline void load_planes(const __bf16 (*pl)[KBP], int plane_rows, int 
 // NAT*16) bytes per G instead of the NPRO*16 of a stored projector matrix, so
 // the products read them from cache, and each product forms P in FP64 while
 // it stages its operands.
-__global__ void k_pfac_f(ProjGeo q, int nsp, int nglp, double *f) {
+__global__ void k_pfac_f(ProjGeo q, int nsp, int nglp, double *f, float *f32) {
   const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= (size_t)nsp * q.lmnxx * nglp) return;
   const int k = (int)(idx / nglp), ig = (int)(idx % nglp);
@@ -670,18 +670,50 @@ __global__ void k_pfac_f(ProjGeo q, int nsp, int nglp, double *f) {
   if (ig < q.ngl && lmn < q.lmnx[isp])
     v = q.ylm[(size_t)q.lmn_lm[k] * q.ngl + ig] * q.pro[(size_t)(q.ibpro[isp] + q.lmn_ln[k]) * q.ngl + ig];
   f[idx] = v;
+  f32[idx] = (float)v;
 }
 
-__global__ void k_pfac_e(ProjGeo q, int nglp, double2 *e) {
+__global__ void k_pfac_e(ProjGeo q, int nglp, double2 *e, float2 *e32) {
   const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= (size_t)q.nat * nglp) return;
   const int atom = (int)(idx / nglp), ig = (int)(idx % nglp);
-  e[idx] = ig < q.ngl ? eigr_of(q, atom, ig) : czero();
+  const double2 v = ig < q.ngl ? eigr_of(q, atom, ig) : czero();
+  e[idx] = v;
+  e32[idx] = make_float2((float)v.x, (float)v.y);
+}
+
+// (-i)^l = cr + i ci
+__device__ inline void mil(int l, float &cr, float &ci) {
+  const int m = l & 3;
+  cr = (m & 1) ? 0.f : (m ? -1.f : 1.f);
+  ci = (m & 1) ? (m == 1 ? -1.f : 1.f) : 0.f;
+}
+
+// split of an FP32 value: h + m + l, each a bf16 and each remainder exact
+__device__ inline Split3 split3f(float f) {
+  Split3 s;
+  const float r = f - bf16_rne(f, s.h);
+  const float q = r - bf16_rne(r, s.m);
+  // q holds at most 8 significant bits: its bf16 is exact
+  s.l = __builtin_bit_cast(__bf16, (unsigned short)(__float_as_uint(q) >> 16));
+  return s;
+}
+
+__device__ inline void put_f(Planes8 &o, int e, float re, float im) {
+  const Split3 a = split3f(re), b = split3f(im);
+  o.p[0][e] = a.h;
+  o.p[1][e] = a.m;
+  o.p[2][e] = a.l;
+  o.p[3][e] = b.h;
+  o.p[4][e] = b.m;
+  o.p[5][e] = b.l;
 }
 
 struct PFac {
   const double *f;
   const double2 *e;
+  const float *f32;    // F and E rounded to FP32: the products stage P from these
+  const float2 *e32;
   const int2 *iptab;
   const int *atom_isp, *lmn_l;
   int lmnxx, nglp;
@@ -708,21 +740,26 @@ struct PFac {
 // conj(P(row, G)) from the projector factors
 struct RowPf {
   PFac pf;
-  // a thread's row is the same in every K chunk: its table rows are looked up once
+  // a thread's row is the same in every K chunk: its FP32 table rows and
+  // (-i)^l are looked up once
   struct Row {
-    const double *fr;
-    const double2 *er;
-    int l;
+    const float *fr;
+    const float2 *er;
+    float cr, ci;
   };
   __device__ Row row(int r) const {
+    const int2 al = pf.iptab[r];
+    const int k = pf.atom_isp[al.x] * pf.lmnxx + al.y;
     Row h;
-    pf.row(r, h.fr, h.er, h.l);
+    h.fr = pf.f32 + (size_t)k * pf.nglp;
+    h.er = pf.e32 + (size_t)al.x * pf.nglp;
+    mil(pf.lmn_l[k], h.cr, h.ci);
     return h;
   }
   // k_rowgemm_bf fetches the next G chunk's table values before the current chunk's MFMAs
   struct Pre {
-    double sv[8];
-    double2 ee[8];
+    float sv[8];
+    float2 ee[8];
   };
   __device__ void fetch(const Row &h, int g0, Pre &p) const {
 #pragma unroll
@@ -731,10 +768,14 @@ struct RowPf {
       p.ee[e] = h.er[g0 + e];
     }
   }
+  // conj(P) = conj(F (-i)^l E)
   __device__ void make(const Row &h, const Pre &p, Planes8 &o) const {
 #pragma unroll
-    for (int e = 0; e < 8; ++e)
-      put(o, e, cconj(rot_mil(make_double2(p.sv[e] * p.ee[e].x, p.sv[e] * p.ee[e].y), h.l)));
+    for (int e = 0; e < 8; ++e) {
+      const float2 ee = p.ee[e];
+      const float tr = h.cr * ee.x - h.ci * ee.y, ti = h.cr * ee.y + h.ci * ee.x;
+      put_f(o, e, p.sv[e] * tr, -(p.sv[e] * ti));
+    }
   }
 };
 
@@ -937,6 +978,23 @@ __global__ void __launch_bounds__(64 * NW) k_rowgemm_bf(Gen gen, GB gb, int nrow
   const typename Gen::Row rh = gen.row(rok ? r0 + ra : 0);
   typename Gen::Pre pa;
   if (rok && g0 < g1) gen.fetch(rh, g0 + 8 * ka, pa);
+  // the band runs of a chunk (NB16 x 4 runs, NSH per thread) are loaded into
+  // registers one chunk ahead, so their latency overlaps the MFMAs, when at
+  // most two runs fall to a thread; with more (2-wave workgroups) the
+  // registers would halve the resident waves, and they are loaded in place
+  constexpr int NSH = (NB16 * 4 + 64 * NW - 1) / (64 * NW);
+  constexpr bool PRE = NSH <= 2;
+  Planes8 shb[PRE ? NSH : 1];
+  auto fetch_b = [&](int gc) {
+#pragma unroll
+    for (int i = 0; i < NSH; ++i) {
+      const int e = t + i * 64 * NW;
+      if (e < NB16 * 4) gb(b0 + (e >> 2), gc + 8 * (e & 3), shb[i]);
+    }
+  };
+  if constexpr (PRE) {
+    if (g0 < g1) fetch_b(g0);
+  }
   for (int gc = g0; gc < g1; gc += KB) {
     {
       Planes8 v;
@@ -944,14 +1002,24 @@ __global__ void __launch_bounds__(64 * NW) k_rowgemm_bf(Gen gen, GB gb, int nrow
       else zero8(v);
       store_planes(sA, WR, ra, ka, v);
     }
-    for (int e = t; e < NB16 * 4; e += 64 * NW) {
-      const int b = e >> 2, k8 = e & 3;
-      Planes8 v;
-      gb(b0 + b, gc + 8 * k8, v);
-      store_planes(sB, NB16, b, k8, v);
+    if constexpr (PRE) {
+#pragma unroll
+      for (int i = 0; i < NSH; ++i) {
+        const int e = t + i * 64 * NW;
+        if (e < NB16 * 4) store_planes(sB, NB16, e >> 2, e & 3, shb[i]);
+      }
+    } else {
+      for (int e = t; e < NB16 * 4; e += 64 * NW) {
+        Planes8 v;
+        gb(b0 + (e >> 2), gc + 8 * (e & 3), v);
+        store_planes(sB, NB16, e >> 2, e & 3, v);
+      }
     }
     __syncthreads();
     if (rok && gc + KB < g1) gen.fetch(rh, gc + KB + 8 * ka, pa);
+    if constexpr (PRE) {
+      if (gc + KB < g1) fetch_b(gc + KB);
+    }
     bf16x8 a[NPL], an[3];
     load_planes(sA, WR, 16 * wv + li, lk, a);
 #pragma unroll
@@ -987,52 +1055,56 @@ __global__ void __launch_bounds__(64 * NW) k_rowgemm_bf(Gen gen, GB gb, int nrow
 struct AddproBf {
   PFac pf;
   int nk;
-  __device__ void operator()(int k0, int ig, Planes8 &o) const {
-    if (ig >= pf.nglp) {
-      zero8(o);
-      return;
-    }
-    if (k0 + 7 < nk && pf.iptab[k0].x == pf.iptab[k0 + 7].x) {
-      // eight projectors of one atom: one structure factor
-      const int2 al = pf.iptab[k0];
-      const double2 ee = pf.e[(size_t)al.x * pf.nglp + ig];
-      const int k = pf.atom_isp[al.x] * pf.lmnxx + al.y;
-#pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        const double sv = pf.f[(size_t)(k + e) * pf.nglp + ig];
-        put(o, e, rot_mil(make_double2(sv * ee.x, sv * ee.y), pf.lmn_l[k + e]));
-      }
-      return;
-    }
-#pragma unroll
-    for (int e = 0; e < 8; ++e) put(o, e, k0 + e < nk ? pf.at(k0 + e, ig) : czero());
+  // P(k, G) from the FP32 tables, zero beyond NPRO
+  __device__ float2 at(int k, int ig) const {
+    if (k >= nk) return make_float2(0.f, 0.f);
+    const int2 al = pf.iptab[k];
+    const int kr = pf.atom_isp[al.x] * pf.lmnxx + al.y;
+    const float sv = pf.f32[(size_t)kr * pf.nglp + ig];
+    const float2 ee = pf.e32[(size_t)al.x * pf.nglp + ig];
+    float cr, ci;
+    mil(pf.lmn_l[kr], cr, ci);
+    return make_float2(sv * (cr * ee.x - ci * ee.y), sv * (cr * ee.y + ci * ee.x));
   }
   // the table values of the next K chunk, fetched before the current chunk's
-  // MFMAs (the eight-projectors-of-one-atom case; others are formed in make)
+  // MFMAs: eight projectors of one atom share one structure factor; other
+  // runs (atom boundaries, the tail) are formed element by element in make
   struct Pre {
-    double2 ee;
-    double sv[8];
+    float2 ee;
+    float sv[8];
     bool one;
   };
   __device__ void fetch(int k0, int ig, Pre &p) const {
     p.one = ig < pf.nglp && k0 + 7 < nk && pf.iptab[k0].x == pf.iptab[k0 + 7].x;
     if (!p.one) return;
     const int2 al = pf.iptab[k0];
-    p.ee = pf.e[(size_t)al.x * pf.nglp + ig];
+    p.ee = pf.e32[(size_t)al.x * pf.nglp + ig];
     const int k = pf.atom_isp[al.x] * pf.lmnxx + al.y;
 #pragma unroll
-    for (int e = 0; e < 8; ++e) p.sv[e] = pf.f[(size_t)(k + e) * pf.nglp + ig];
+    for (int e = 0; e < 8; ++e) p.sv[e] = pf.f32[(size_t)(k + e) * pf.nglp + ig];
   }
   __device__ void make(const Pre &p, int k0, int ig, Planes8 &o) const {
+    if (ig >= pf.nglp) {
+      zero8(o);
+      return;
+    }
     if (!p.one) {
-      (*this)(k0, ig, o);
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const float2 v = at(k0 + e, ig);
+        put_f(o, e, v.x, v.y);
+      }
       return;
     }
     const int2 al = pf.iptab[k0];
     const int k = pf.atom_isp[al.x] * pf.lmnxx + al.y;
 #pragma unroll
-    for (int e = 0; e < 8; ++e)
-      put(o, e, rot_mil(make_double2(p.sv[e] * p.ee.x, p.sv[e] * p.ee.y), pf.lmn_l[k + e]));
+    for (int e = 0; e < 8; ++e) {
+      float cr, ci;
+      mil(pf.lmn_l[k + e], cr, ci);
+      const float tr = cr * p.ee.x - ci * p.ee.y, ti = cr * p.ee.y + ci * p.ee.x;
+      put_f(o, e, p.sv[e] * tr, p.sv[e] * ti);
+    }
   }
 };
 struct SrcLamRows {          // LAMBDA1/LAMBDA2 stacked along K = 2 NBH, rows b
@@ -1074,20 +1146,34 @@ __global__ void __launch_bounds__(64 * NW) k_colgemm_bf(GA ga, GB gb, int nk, in
   auto krun = [&](int k0) { return WG == 64 ? __builtin_amdgcn_readfirstlane(k0 + 8 * ks) : k0 + 8 * ks; };
   typename GB::Pre pre;
   gb.fetch(krun(0), g0 + gs, pre);
+  // the coefficient runs of a chunk (NB16 x 4 runs, up to NSH per thread) are
+  // loaded into registers one chunk ahead, so their latency overlaps the MFMAs
+  constexpr int NSH = (NB16 * 4 + 64 * NW - 1) / (64 * NW);
+  Planes8 sha[NSH];
+  auto fetch_a = [&](int k0) {
+#pragma unroll
+    for (int i = 0; i < NSH; ++i) {
+      const int e = t + i * 64 * NW;
+      if (e < NB16 * 4) ga(b0 + (e >> 2), k0 + 8 * (e & 3), sha[i]);
+    }
+  };
+  fetch_a(0);
   for (int k0 = 0; k0 < nk; k0 += KB) {
     {
       Planes8 v;
       gb.make(pre, krun(k0), g0 + gs, v);
       store_planes(sB, WG, gs, ks, v);
     }
-    for (int e = t; e < NB16 * 4; e += 64 * NW) {
-      const int b = e >> 2, k8 = e & 3;
-      Planes8 v;
-      ga(b0 + b, k0 + 8 * k8, v);
-      store_planes(sA, NB16, b, k8, v);
+#pragma unroll
+    for (int i = 0; i < NSH; ++i) {
+      const int e = t + i * 64 * NW;
+      if (e < NB16 * 4) store_planes(sA, NB16, e >> 2, e & 3, sha[i]);
     }
     __syncthreads();
-    if (k0 + KB < nk) gb.fetch(krun(k0 + KB), g0 + gs, pre);
+    if (k0 + KB < nk) {
+      gb.fetch(krun(k0 + KB), g0 + gs, pre);
+      fetch_a(k0 + KB);
+    }
     bf16x8 bb[NPL], bn[3];
     load_planes(sB, WG, 16 * wv + li, lk, bb);
 #pragma unroll
@@ -1164,20 +1250,33 @@ __global__ void __launch_bounds__(64 * NW) k_force_bf(GA ga, GB gb, int npro, in
   const int gs = t % WG, ks = t / WG;
   typename GB::Pre pre;
   gb.fetch(8 * ks, g0 + gs, pre);
+  // the projector-row runs of a chunk, loaded into registers one chunk ahead
+  constexpr int NSH = (NB16 * 4 + 64 * NW - 1) / (64 * NW);
+  Planes8 sha[NSH];
+  auto fetch_a = [&](int k0) {
+#pragma unroll
+    for (int i = 0; i < NSH; ++i) {
+      const int e = t + i * 64 * NW;
+      if (e < NB16 * 4) ga(p0 + (e >> 2), k0 + 8 * (e & 3), sha[i]);
+    }
+  };
+  fetch_a(0);
   for (int k0 = 0; k0 < nk; k0 += KB) {
     {
       Planes8 v;
       gb.make(pre, k0 + 8 * ks, g0 + gs, v);
       store_planes(sB, WG, gs, ks, v);
     }
-    for (int e = t; e < NB16 * 4; e += 64 * NW) {
-      const int r = e >> 2, k8 = e & 3;
-      Planes8 v;
-      ga(p0 + r, k0 + 8 * k8, v);
-      store_planes(sA, NB16, r, k8, v);
+#pragma unroll
+    for (int i = 0; i < NSH; ++i) {
+      const int e = t + i * 64 * NW;
+      if (e < NB16 * 4) store_planes(sA, NB16, e >> 2, e & 3, sha[i]);
     }
     __syncthreads();
-    if (k0 + KB < nk) gb.fetch(k0 + KB + 8 * ks, g0 + gs, pre);
+    if (k0 + KB < nk) {
+      gb.fetch(k0 + KB + 8 * ks, g0 + gs, pre);
+      fetch_a(k0 + KB);
+    }
     bf16x8 bb[NPL], bn[3];
     load_planes(sB, WG, 16 * wv + li, lk, bb);
 #pragma unroll
@@ -1744,7 +1843,7 @@ int k_phase_tables(pw_waveset *ws, const double *r, double2 *phase) {
 int k_pfac(pw_waveset *ws) {
   const int nglp = nglp_of(ws);
   const size_t nf = (size_t)ws->nsp * ws->lmnxx * nglp, ne = (size_t)ws->nat * nglp;
-  const size_t bytes = nf * sizeof(double) + ne * sizeof(double2);
+  const size_t bytes = nf * (sizeof(double) + sizeof(float)) + ne * (sizeof(double2) + sizeof(float2));
   if (bytes > ws->pfac_bytes) {
     if (ws->d_pfac) PW_HIP(hipFree(ws->d_pfac));
     PW_HIP(hipMalloc(&ws->d_pfac, bytes));
@@ -1754,8 +1853,12 @@ int k_pfac(pw_waveset *ws) {
   q.phase = ws->d_phase;
   double *f = (double *)ws->d_pfac;
   ScopeTimer tm(ws, "pgen");
-  k_pfac_f<<<(nf + 255) / 256, 256, 0, ws->stream>>>(q, ws->nsp, nglp, f);
-  k_pfac_e<<<(ne + 255) / 256, 256, 0, ws->stream>>>(q, nglp, (double2 *)(f + nf));
+  // [F FP64][E FP64][E FP32][F FP32]: every part starts 8-byte aligned
+  double2 *e = (double2 *)(f + nf);
+  float2 *e32 = (float2 *)(e + ne);
+  float *f32 = (float *)(e32 + ne);
+  k_pfac_f<<<(nf + 255) / 256, 256, 0, ws->stream>>>(q, ws->nsp, nglp, f, f32);
+  k_pfac_e<<<(ne + 255) / 256, 256, 0, ws->stream>>>(q, nglp, e, e32);
   PW_HIP(hipGetLastError());
   return PW_OK;
 }
@@ -1763,8 +1866,11 @@ int k_pfac(pw_waveset *ws) {
 PFac pfac_of(const pw_waveset *ws) {
   const int nglp = nglp_of(ws);
   const double *f = (const double *)ws->d_pfac;
-  const size_t nf = (size_t)ws->nsp * ws->lmnxx * nglp;
-  return PFac{f, (const double2 *)(f + nf), ws->d_ipro_tab, ws->d_atom_isp, ws->d_lmn_l, ws->lmnxx, nglp};
+  const size_t nf = (size_t)ws->nsp * ws->lmnxx * nglp, ne = (size_t)ws->nat * nglp;
+  const double2 *e = (const double2 *)(f + nf);
+  const float2 *e32 = (const float2 *)(e + ne);
+  const float *f32 = (const float *)(e32 + ne);
+  return PFac{f, e, f32, e32, ws->d_ipro_tab, ws->d_atom_isp, ws->d_lmn_l, ws->lmnxx, nglp};
 }
 
 int k_projections(pw_waveset *ws, const double2 *psi, const double2 *phase, double2 *proj) {

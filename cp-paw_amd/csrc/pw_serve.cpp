@@ -292,7 +292,7 @@ extern "C" int pw_waves_footprint(pw_plan *plan, int nb, int nat, int nsp, const
   b += 16 * nrl * 8;                                // density partials
   b += (long long)(21 + 6) * nb * nb * 8;           // NB x NB matrices, ORTHO_X work
   b += ngl * 48 + (long long)nat * lmnxx * lmnxx * 32;
-  if (prec == PW_PREC_MIXED) b += (nsp * lmnxx * 8 + nat * 16) * nglp;  // projector factors
+  if (prec == PW_PREC_MIXED) b += (nsp * lmnxx * 12 + nat * 24) * nglp;  // projector factors, FP64 and FP32
   *bytes = b + b / 10 + (8ll << 20);
   return PW_OK;
 }
