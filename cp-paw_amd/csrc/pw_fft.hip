@@ -262,6 +262,34 @@ __device__ T2 *lds_fft(T2 *b0, T2 *b1, const Fft1D &f, int npen, int es, int ps,
   return in;
 }
 
+// The Stockham chain of a length N fixed at compile time (fft_factor's rule
+// for these passes: odd radices, then 16, 8, 4, 2); STAGE ST of NST with the
+// span NS = product of the earlier radices.  Returns the result buffer.
+__host__ __device__ constexpr Fft1D fft_stockham_ct(int n) {
+  Fft1D f{};
+  f.n = n;
+  int m = n;
+  const int order[9] = {9, 3, 5, 7, 11, 16, 8, 4, 2};
+  for (int io = 0; io < 9; ++io)
+    while (m % order[io] == 0 && f.nstage < 12) {
+      f.radix[f.nstage++] = order[io];
+      m /= order[io];
+    }
+  return f;
+}
+template <typename T2, int N, int ST, int NS>
+__device__ inline T2 *lds_fft_ct(T2 *in, T2 *out, int npen, int es, int ps, const T2 *tw, int sign, bool pen_fast) {
+  constexpr Fft1D f = fft_stockham_ct(N);
+  if constexpr (ST < f.nstage) {
+    constexpr int R = f.radix[ST];
+    stage<T2, R>(in, out, N, NS, npen, es, ps, tw, sign, pen_fast);
+    __syncthreads();
+    return lds_fft_ct<T2, N, ST + 1, NS * R>(out, in, npen, es, ps, tw, sign, pen_fast);
+  } else {
+    return in;
+  }
+}
+
 struct Geo {
   int nr1, nr2, nr3, ngl, nstripe, nzp;
   int pitch;                      // row pitch of the LDS-resident (y, z) plane (odd)
@@ -277,10 +305,11 @@ template <> struct XTile<double2> { static constexpr int n = 16; };
 template <> struct XTile<float2> { static constexpr int n = 32; };
 
 // ---- x pass, forward: sphere -> stripes [band][x][stripe] --------------------
-template <typename T2>
+template <typename T2, int NX = 0>
 __global__ void __launch_bounds__(kBlock) k_xfwd(Geo g, Fft1D f, const T2 *tw, const double2 *psi,
                                                  T2 *stripes) {
   constexpr int PX = XTile<T2>::n;
+  if constexpr (NX > 0) g.nr1 = NX;
   extern __shared__ unsigned char smem_raw[];
   T2 *b0 = reinterpret_cast<T2 *>(smem_raw);
   T2 *b1 = b0 + PX * g.nr1;
@@ -294,7 +323,9 @@ __global__ void __launch_bounds__(kBlock) k_xfwd(Geo g, Fft1D f, const T2 *tw, c
   for (int e = e0 + threadIdx.x; e < e1; e += blockDim.x)
     b0[(g.stripe_s[e] - s0) * g.nr1 + g.stripe_x[e]] = Prec<T2>::from(src[g.stripe_g[e]]);
   __syncthreads();
-  T2 *r = lds_fft<T2>(b0, b1, f, PX, 1, g.nr1, tw, +1, false);
+  T2 *r;
+  if constexpr (NX > 0) r = lds_fft_ct<T2, NX, 0, 1>(b0, b1, PX, 1, NX, tw, +1, false);
+  else r = lds_fft<T2>(b0, b1, f, PX, 1, g.nr1, tw, +1, false);
   T2 *dst = stripes + (size_t)band * g.nr1 * g.nstripe + s0;
   for (int i = threadIdx.x; i < ns * g.nr1; i += blockDim.x) {
     const int x = i / ns, sl = i % ns;
@@ -303,11 +334,12 @@ __global__ void __launch_bounds__(kBlock) k_xfwd(Geo g, Fft1D f, const T2 *tw, c
 }
 
 // ---- x pass, inverse: stripes -> sphere (gather, scale, optional G^2/2 psi) ---
-template <typename T2>
+template <typename T2, int NX = 0>
 __global__ void __launch_bounds__(kBlock) k_xinv(Geo g, Fft1D f, const T2 *tw, const T2 *stripes,
                                                  double2 *out, const double2 *psi, double scale,
                                                  int add_kin) {
   constexpr int PX = XTile<T2>::n;
+  if constexpr (NX > 0) g.nr1 = NX;
   extern __shared__ unsigned char smem_raw[];
   T2 *b0 = reinterpret_cast<T2 *>(smem_raw);
   T2 *b1 = b0 + PX * g.nr1;
@@ -320,7 +352,9 @@ __global__ void __launch_bounds__(kBlock) k_xinv(Geo g, Fft1D f, const T2 *tw, c
     b0[sl * g.nr1 + x] = sl < ns ? src[(size_t)x * g.nstripe + sl] : czero<T2>();
   }
   __syncthreads();
-  T2 *r = lds_fft<T2>(b0, b1, f, PX, 1, g.nr1, tw, -1, false);
+  T2 *r;
+  if constexpr (NX > 0) r = lds_fft_ct<T2, NX, 0, 1>(b0, b1, PX, 1, NX, tw, -1, false);
+  else r = lds_fft<T2>(b0, b1, f, PX, 1, g.nr1, tw, -1, false);
   const int e0 = g.stripe_off[s0], e1 = g.stripe_off[s0 + ns];
   double2 *dst = out + (size_t)band * g.ngl;
   const double2 *ps = psi + (size_t)band * g.ngl;
@@ -508,6 +542,75 @@ __device__ void fft_dif(T2 *buf, const Fft1D &f, int nst, int npen, unsigned lon
   }
 }
 
+// ---- plane lengths fixed at compile time ---------------------------------------
+// A yz kernel instantiated for plane lengths NY x NZ (0: taken from the plan)
+// runs its stages from a compile-time factorisation: the stage sequence is
+// unrolled, radices and fdiv magics are constants and the trip counts of the
+// z stages are known.  The factorisation follows fft_factor's in-place rule;
+// the launcher checks it against the plan before it picks such a kernel.
+__host__ __device__ constexpr unsigned long long fdiv_magic_ct(unsigned d) {
+  return (0x100000000ull + d - 1) / d;
+}
+__host__ __device__ constexpr Fft1D fft_inplace_ct(int n) {
+  Fft1D f{};
+  f.n = n;
+  int m = n;
+  const int order[8] = {9, 3, 5, 7, 11, 8, 4, 2};
+  for (int io = 0; io < 8; ++io)
+    while (m % order[io] == 0 && f.nstage < 12) {
+      f.radix[f.nstage++] = order[io];
+      m /= order[io];
+    }
+  int L = 1;
+  for (int st = 0; st < f.nstage; ++st) {
+    f.mnbf[st] = fdiv_magic_ct((unsigned)(n / f.radix[st]));
+    f.mlp[st] = fdiv_magic_ct((unsigned)L);
+    L *= f.radix[st];
+  }
+  return f;
+}
+__host__ __device__ constexpr int fft_lprod(const Fft1D &f, int st) {
+  int L = 1;
+  for (int k = 0; k <= st; ++k) L *= f.radix[k];
+  return L;
+}
+
+// stages ST..NST-1 of the DIT transform of length N
+template <typename T2, int N, int ST, int NST>
+__device__ inline void fft_dit_ct(T2 *buf, int npen, unsigned long long mpen, int es, int ps, const int *pmap,
+                                  const T2 *tw, int sign, bool pen_fast) {
+  if constexpr (ST < NST) {
+    constexpr Fft1D f = fft_inplace_ct(N);
+    ip_stage<T2, f.radix[ST], false>(buf, N, fft_lprod(f, ST), npen, es, ps, pmap, tw, sign, pen_fast, mpen,
+                                     f.mnbf[ST], f.mlp[ST]);
+    fft_dit_ct<T2, N, ST + 1, NST>(buf, npen, mpen, es, ps, pmap, tw, sign, pen_fast);
+  }
+}
+// stages ST..0 of the DIF transform of length N (called with ST = NST - 1)
+template <typename T2, int N, int ST>
+__device__ inline void fft_dif_ct(T2 *buf, int npen, unsigned long long mpen, int es, int ps, const int *pmap,
+                                  const T2 *tw, int sign, bool pen_fast) {
+  if constexpr (ST >= 0) {
+    constexpr Fft1D f = fft_inplace_ct(N);
+    ip_stage<T2, f.radix[ST], true>(buf, N, fft_lprod(f, ST), npen, es, ps, pmap, tw, sign, pen_fast, mpen,
+                                    f.mnbf[ST], f.mlp[ST]);
+    fft_dif_ct<T2, N, ST - 1>(buf, npen, mpen, es, ps, pmap, tw, sign, pen_fast);
+  }
+}
+template <int N> constexpr int nstage_ct() { return fft_inplace_ct(N).nstage; }
+template <int N> constexpr int last_radix_ct() { return fft_inplace_ct(N).radix[fft_inplace_ct(N).nstage - 1]; }
+
+// the plane's scalar lengths as constants
+template <int NY, int NZ>
+__device__ inline void yz_fix(Geo &g) {
+  if constexpr (NY > 0) {
+    g.nr2 = NY;
+    g.pitch = NY | 1;
+    g.mnr2 = fdiv_magic_ct(NY);
+  }
+  if constexpr (NZ > 0) g.nr3 = NZ;
+}
+
 // zero the plane, scatter the x column of one band's stripes into the
 // digit-reversed (z, y) positions, transform y (DIT, on the rows of the z
 // planes that hold stripes) and z (DIT) up to, not including, its last
@@ -516,7 +619,7 @@ __device__ void fft_dif(T2 *buf, const Fft1D &f, int nst, int npen, unsigned lon
 // stripe-holding row.  Both transforms run pencil-fast: neighbouring lanes
 // take neighbouring pencils, whose rows (y) or columns (z) fall into distinct
 // LDS banks.
-template <typename T2, bool BIG>
+template <typename T2, bool BIG, int NY = 0, int NZ = 0>
 __device__ inline void yz_forward(const Geo &g, const Fft1D &fy, const Fft1D &fz, const T2 *twy,
                                   const T2 *twz, const T2 *col, T2 *pl, const int *zpos) {
   const int npl = g.pitch * g.nr3;
@@ -524,8 +627,14 @@ __device__ inline void yz_forward(const Geo &g, const Fft1D &fy, const Fft1D &fz
   __syncthreads();
   for (int s = threadIdx.x; s < g.nstripe; s += kYZ) pl[g.stripe_pos[s]] = col[s];
   __syncthreads();
-  fft_dit<T2, BIG>(pl, fy, fy.nstage, g.nzp, g.mnzp, 1, 0, zpos, twy, +1, true);
-  fft_dit<T2, BIG>(pl, fz, fz.nstage - 1, g.nr2, g.mnr2, g.pitch, 1, nullptr, twz, +1, true);
+  if constexpr (NY > 0)
+    fft_dit_ct<T2, NY, 0, nstage_ct<NY>()>(pl, g.nzp, g.mnzp, 1, 0, zpos, twy, +1, true);
+  else
+    fft_dit<T2, BIG>(pl, fy, fy.nstage, g.nzp, g.mnzp, 1, 0, zpos, twy, +1, true);
+  if constexpr (NZ > 0)
+    fft_dit_ct<T2, NZ, 0, nstage_ct<NZ>() - 1>(pl, g.nr2, g.mnr2, g.pitch, 1, nullptr, twz, +1, true);
+  else
+    fft_dit<T2, BIG>(pl, fz, fz.nstage - 1, g.nr2, g.mnr2, g.pitch, 1, nullptr, twz, +1, true);
 }
 
 // The last DIT stage of the z transforms, one butterfly per (column y, j):
@@ -632,10 +741,11 @@ __device__ inline void dens_reg_store(const Geo &g, int nr3, const double (&accr
 
 // ---- yz pass of WAVES_VPSI: psi(x,y,z) * V(r), back to the stripes ----------------
 // One workgroup per (x, band); vt = V as [x][z][y] in the work precision.
-template <typename T2, bool BIG>
+template <typename T2, bool BIG, int NY = 0, int NZ = 0>
 __global__ void __launch_bounds__(kYZ) k_yz_vpsi(Geo g, Fft1D fy, Fft1D fz, const T2 *twy, const T2 *twz,
                                                  T2 *stripes, const typename Prec<T2>::T *vt) {
   typedef typename Prec<T2>::T T;
+  yz_fix<NY, NZ>(g);
   extern __shared__ unsigned char smem_raw[];
   const int n23 = g.nr2 * g.nr3;
   T2 *pl = reinterpret_cast<T2 *>(smem_raw);
@@ -646,13 +756,21 @@ __global__ void __launch_bounds__(kYZ) k_yz_vpsi(Geo g, Fft1D fy, Fft1D fz, cons
   for (int i = threadIdx.x; i < g.nr2; i += kYZ) ty[i] = twy[i];
   for (int i = threadIdx.x; i < g.nr3; i += kYZ) tz[i] = twz[i];
   T2 *col = stripes + ((size_t)band * g.nr1 + x) * g.nstripe;
-  yz_forward<T2, BIG>(g, fy, fz, ty, tz, col, pl, zpos);
+  yz_forward<T2, BIG, NY, NZ>(g, fy, fz, ty, tz, col, pl, zpos);
   const T *vx = vt + (size_t)x * n23;
+  if constexpr (NZ > 0) {
+    z_last_vpsi<T2, last_radix_ct<NZ>()>(g, pl, NZ, tz, vx);
+    fft_dif_ct<T2, NZ, nstage_ct<NZ>() - 2>(pl, g.nr2, g.mnr2, g.pitch, 1, nullptr, tz, -1, true);
+  } else {
 #define PW_CALL(R) z_last_vpsi<T2, R>(g, pl, g.nr3, tz, vx)
-  PW_RADIX_SWITCH(fz.radix[fz.nstage - 1], PW_CALL)
+    PW_RADIX_SWITCH(fz.radix[fz.nstage - 1], PW_CALL)
 #undef PW_CALL
-  fft_dif<T2, BIG>(pl, fz, fz.nstage - 1, g.nr2, g.mnr2, g.pitch, 1, nullptr, tz, -1, true);
-  fft_dif<T2, BIG>(pl, fy, fy.nstage, g.nzp, g.mnzp, 1, 0, zpos, ty, -1, true);
+    fft_dif<T2, BIG>(pl, fz, fz.nstage - 1, g.nr2, g.mnr2, g.pitch, 1, nullptr, tz, -1, true);
+  }
+  if constexpr (NY > 0)
+    fft_dif_ct<T2, NY, nstage_ct<NY>() - 1>(pl, g.nzp, g.mnzp, 1, 0, zpos, ty, -1, true);
+  else
+    fft_dif<T2, BIG>(pl, fy, fy.nstage, g.nzp, g.mnzp, 1, 0, zpos, ty, -1, true);
   for (int s = threadIdx.x; s < g.nstripe; s += kYZ) col[s] = pl[g.stripe_pos[s]];
 }
 
@@ -662,11 +780,12 @@ __global__ void __launch_bounds__(kYZ) k_yz_vpsi(Geo g, Fft1D fy, Fft1D fz, cons
 // the FP32 form is held to 128 registers so that two workgroups share a SIMD set
 template <typename T2, bool REG> struct DensWaves { static constexpr int v = 1; };
 template <> struct DensWaves<float2, true> { static constexpr int v = 4; };
-template <typename T2, bool BIG, bool REG>
+template <typename T2, bool BIG, bool REG, int NY = 0, int NZ = 0>
 __global__ void __launch_bounds__(kYZ) __attribute__((amdgpu_waves_per_eu(DensWaves<T2, REG>::v))) k_yz_dens(Geo g, Fft1D fy, Fft1D fz, const T2 *twy, const T2 *twz,
                                                  const T2 *stripes, int nb, int bpg, const double *occ2,
                                                  double *part) {
   static_assert(!(REG && BIG), "the register band sum takes radices up to kRegR");
+  yz_fix<NY, NZ>(g);
   extern __shared__ unsigned char smem_raw[];
   const int n23 = g.nr2 * g.nr3;
   double *acc = reinterpret_cast<double *>(smem_raw);
@@ -689,9 +808,16 @@ __global__ void __launch_bounds__(kYZ) __attribute__((amdgpu_waves_per_eu(DensWa
     const double s1 = occ2[2 * band], s2 = occ2[2 * band + 1];
     if (s1 == 0.0 && s2 == 0.0) continue;   // uniform across the workgroup
     __syncthreads();
-    yz_forward<T2, BIG>(g, fy, fz, ty, tz, stripes + ((size_t)band * g.nr1 + x) * g.nstripe, pl, zpos);
+    yz_forward<T2, BIG, NY, NZ>(g, fy, fz, ty, tz, stripes + ((size_t)band * g.nr1 + x) * g.nstripe, pl, zpos);
     // each acc position belongs to one butterfly of a fixed thread in every band
-    if constexpr (REG) {
+    if constexpr (NZ > 0) {
+      constexpr int RL = last_radix_ct<NZ>();
+      if constexpr (REG) {
+        if constexpr (RL <= kRegR) z_last_dens_reg<T2, RL>(g, pl, NZ, tz, s1, s2, accr);
+      } else {
+        z_last_dens<T2, RL>(g, pl, NZ, tz, s1, s2, acc);
+      }
+    } else if constexpr (REG) {
 #define PW_CALL(R) if constexpr (R <= kRegR) z_last_dens_reg<T2, R>(g, pl, g.nr3, tz, s1, s2, accr)
       PW_RADIX_SWITCH(fz.radix[fz.nstage - 1], PW_CALL)
 #undef PW_CALL
@@ -702,7 +828,10 @@ __global__ void __launch_bounds__(kYZ) __attribute__((amdgpu_waves_per_eu(DensWa
     }
   }
   double *dst = part + ((size_t)grp * g.nr1 + x) * n23;
-  if constexpr (REG) {
+  if constexpr (REG && NZ > 0) {
+    constexpr int RL = last_radix_ct<NZ>();
+    if constexpr (RL <= kRegR) dens_reg_store<RL>(g, NZ, accr, dst);
+  } else if constexpr (REG) {
 #define PW_CALL(R) if constexpr (R <= kRegR) dens_reg_store<R>(g, g.nr3, accr, dst)
     PW_RADIX_SWITCH(fz.radix[fz.nstage - 1], PW_CALL)
 #undef PW_CALL
@@ -789,13 +918,25 @@ template <typename T2> const T2 *twid(const pw_plan *p, int k);
 template <> const double2 *twid<double2>(const pw_plan *p, int k) { return p->d_tw[k]; }
 template <> const float2 *twid<float2>(const pw_plan *p, int k) { return p->d_twf[k]; }
 
+// the plan's x factorisation equals the compile-time one of length n
+bool x_is(const Fft1D &f, int n) {
+  const Fft1D c = fft_stockham_ct(n);
+  if (f.n != n || f.nstage != c.nstage) return false;
+  for (int st = 0; st < c.nstage; ++st)
+    if (f.radix[st] != c.radix[st]) return false;
+  return true;
+}
+
 template <typename T2>
 int run_forward_x(pw_plan *p, const Geo &g, int nb, const double2 *psi, hipStream_t s) {
   T2 *stripes = (T2 *)p->d_stripe_buf;
   constexpr int PX = XTile<T2>::n;
   dim3 gx((p->nstripe + PX - 1) / PX, nb);
-  k_xfwd<T2><<<gx, kBlock, 2 * PX * p->nr1 * sizeof(T2), s>>>(g, p->f[0], twid<T2>(p, 0), psi,
-                                                              stripes);
+  const size_t lds = 2 * PX * p->nr1 * sizeof(T2);
+  if (x_is(p->f[0], 72))
+    k_xfwd<T2, 72><<<gx, kBlock, lds, s>>>(g, p->f[0], twid<T2>(p, 0), psi, stripes);
+  else
+    k_xfwd<T2><<<gx, kBlock, lds, s>>>(g, p->f[0], twid<T2>(p, 0), psi, stripes);
   PW_HIP(hipGetLastError());
   return PW_OK;
 }
@@ -806,8 +947,11 @@ int run_inverse_x(pw_plan *p, const Geo &g, int nb, double2 *out, const double2 
   T2 *stripes = (T2 *)p->d_stripe_buf;
   constexpr int PX = XTile<T2>::n;
   dim3 gx((p->nstripe + PX - 1) / PX, nb);
-  k_xinv<T2><<<gx, kBlock, 2 * PX * p->nr1 * sizeof(T2), s>>>(g, p->f[0], twid<T2>(p, 0),
-                                                              stripes, out, psi, scale, add_kin);
+  const size_t lds = 2 * PX * p->nr1 * sizeof(T2);
+  if (x_is(p->f[0], 72))
+    k_xinv<T2, 72><<<gx, kBlock, lds, s>>>(g, p->f[0], twid<T2>(p, 0), stripes, out, psi, scale, add_kin);
+  else
+    k_xinv<T2><<<gx, kBlock, lds, s>>>(g, p->f[0], twid<T2>(p, 0), stripes, out, psi, scale, add_kin);
   PW_HIP(hipGetLastError());
   return PW_OK;
 }
@@ -881,6 +1025,23 @@ int vt_of(pw_plan *p, const double *v, hipStream_t s) {
   return PW_OK;
 }
 
+// the plan's in-place factorisation of length n equals the compile-time one
+bool yz_is(const Fft1D &f, int n) {
+  const Fft1D c = fft_inplace_ct(n);
+  if (f.n != n || f.nstage != c.nstage) return false;
+  for (int st = 0; st < c.nstage; ++st)
+    if (f.radix[st] != c.radix[st] || f.mnbf[st] != c.mnbf[st] || f.mlp[st] != c.mlp[st]) return false;
+  return true;
+}
+// plane lengths with compile-time yz kernels (square planes; 72: the Si64 cell)
+constexpr int kYZFixed[] = {72};
+int yz_fixed(const pw_plan *p) {
+  for (int n : kYZFixed)
+    if (p->nr2 == n && p->nr3 == n && p->yzpitch == (n | 1) && yz_is(p->fyz[0], n) && yz_is(p->fyz[1], n))
+      return n;
+  return 0;
+}
+
 template <typename T2>
 int vpsi_t(pw_plan *p, int nb, const double2 *psi, const double *v, double2 *hpsi, int add_kin,
            hipStream_t s) {
@@ -894,6 +1055,11 @@ int vpsi_t(pw_plan *p, int nb, const double2 *psi, const double *v, double2 *hps
     PW_TRY(yz_lds(k_yz_vpsi<T2, true>, lds));
     k_yz_vpsi<T2, true><<<dim3(p->nr1, nb), kYZ, lds, s>>>(g, p->fyz[0], p->fyz[1], twid<T2>(p, 1), twid<T2>(p, 2),
                                                            (T2 *)p->d_stripe_buf, (const T *)p->d_vt);
+  } else if (yz_fixed(p) == 72) {
+    PW_TRY(yz_lds(k_yz_vpsi<T2, false, 72, 72>, lds));
+    k_yz_vpsi<T2, false, 72, 72><<<dim3(p->nr1, nb), kYZ, lds, s>>>(g, p->fyz[0], p->fyz[1], twid<T2>(p, 1),
+                                                                    twid<T2>(p, 2), (T2 *)p->d_stripe_buf,
+                                                                    (const T *)p->d_vt);
   } else {
     PW_TRY(yz_lds(k_yz_vpsi<T2, false>, lds));
     k_yz_vpsi<T2, false><<<dim3(p->nr1, nb), kYZ, lds, s>>>(g, p->fyz[0], p->fyz[1], twid<T2>(p, 1), twid<T2>(p, 2),
@@ -926,15 +1092,17 @@ int density_t(pw_plan *p, int nb, const double2 *psi, const double *occ2, double
     PW_HIP(hipMalloc(&p->d_dens_part, need));
     p->dens_bytes = need;
   }
-#define PW_DENS(BIG, REG)                                                                                  \
+#define PW_DENS(BIG, REG, ...)                                                                             \
   do {                                                                                                     \
-    PW_TRY(yz_lds(k_yz_dens<T2, BIG, REG>, lds));                                                          \
-    k_yz_dens<T2, BIG, REG><<<dim3(p->nr1, ngrp), kYZ, lds, s>>>(g, p->fyz[0], p->fyz[1], twid<T2>(p, 1),  \
+    PW_TRY(yz_lds(k_yz_dens<T2, BIG, REG, ##__VA_ARGS__>, lds));                                           \
+    k_yz_dens<T2, BIG, REG, ##__VA_ARGS__><<<dim3(p->nr1, ngrp), kYZ, lds, s>>>(g, p->fyz[0], p->fyz[1], twid<T2>(p, 1),  \
                                                                  twid<T2>(p, 2), (const T2 *)p->d_stripe_buf, \
                                                                  nb, bpg, occ2, (double *)p->d_dens_part);  \
   } while (0)
   if (yz_big(p)) PW_DENS(true, false);
+  else if (reg && yz_fixed(p) == 72) PW_DENS(false, true, 72, 72);
   else if (reg) PW_DENS(false, true);
+  else if (yz_fixed(p) == 72) PW_DENS(false, false, 72, 72);
   else PW_DENS(false, false);
 #undef PW_DENS
   // rho[zy][x] = sum_group part[group][x][zy]
