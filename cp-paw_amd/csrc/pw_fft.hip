@@ -15,6 +15,8 @@
 //            weighted |psi|^2 sum (density); no grid round trip through HBM
 //   y / z passes over HBM z planes serve the full-grid PLANEWAVE$FFT calls
 // Work buffers: stripes [band][x][stripe], planes [band][iz][y][x].
+#include <type_traits>
+
 #include "pw_common.h"
 
 namespace {
@@ -927,16 +929,27 @@ bool x_is(const Fft1D &f, int n) {
   return true;
 }
 
+// calls f(IC<NX>) with the x length when a compile-time instantiation exists
+template <int N> using ICX = std::integral_constant<int, N>;
+template <class F>
+int with_x(const pw_plan *p, F &&f) {
+  if (x_is(p->f[0], 72)) return f(ICX<72>{});
+  if (x_is(p->f[0], 36)) return f(ICX<36>{});
+  if (x_is(p->f[0], 64)) return f(ICX<64>{});
+  if (x_is(p->f[0], 80)) return f(ICX<80>{});
+  return f(ICX<0>{});
+}
+
 template <typename T2>
 int run_forward_x(pw_plan *p, const Geo &g, int nb, const double2 *psi, hipStream_t s) {
   T2 *stripes = (T2 *)p->d_stripe_buf;
   constexpr int PX = XTile<T2>::n;
   dim3 gx((p->nstripe + PX - 1) / PX, nb);
   const size_t lds = 2 * PX * p->nr1 * sizeof(T2);
-  if (x_is(p->f[0], 72))
-    k_xfwd<T2, 72><<<gx, kBlock, lds, s>>>(g, p->f[0], twid<T2>(p, 0), psi, stripes);
-  else
-    k_xfwd<T2><<<gx, kBlock, lds, s>>>(g, p->f[0], twid<T2>(p, 0), psi, stripes);
+  PW_TRY(with_x(p, [&](auto nx) -> int {
+    k_xfwd<T2, decltype(nx)::value><<<gx, kBlock, lds, s>>>(g, p->f[0], twid<T2>(p, 0), psi, stripes);
+    return PW_OK;
+  }));
   PW_HIP(hipGetLastError());
   return PW_OK;
 }
@@ -948,10 +961,11 @@ int run_inverse_x(pw_plan *p, const Geo &g, int nb, double2 *out, const double2 
   constexpr int PX = XTile<T2>::n;
   dim3 gx((p->nstripe + PX - 1) / PX, nb);
   const size_t lds = 2 * PX * p->nr1 * sizeof(T2);
-  if (x_is(p->f[0], 72))
-    k_xinv<T2, 72><<<gx, kBlock, lds, s>>>(g, p->f[0], twid<T2>(p, 0), stripes, out, psi, scale, add_kin);
-  else
-    k_xinv<T2><<<gx, kBlock, lds, s>>>(g, p->f[0], twid<T2>(p, 0), stripes, out, psi, scale, add_kin);
+  PW_TRY(with_x(p, [&](auto nx) -> int {
+    k_xinv<T2, decltype(nx)::value><<<gx, kBlock, lds, s>>>(g, p->f[0], twid<T2>(p, 0), stripes, out, psi, scale,
+                                                            add_kin);
+    return PW_OK;
+  }));
   PW_HIP(hipGetLastError());
   return PW_OK;
 }
@@ -1033,13 +1047,23 @@ bool yz_is(const Fft1D &f, int n) {
     if (f.radix[st] != c.radix[st] || f.mnbf[st] != c.mnbf[st] || f.mlp[st] != c.mlp[st]) return false;
   return true;
 }
-// plane lengths with compile-time yz kernels (square planes; 72: the Si64 cell)
-constexpr int kYZFixed[] = {72};
-int yz_fixed(const pw_plan *p) {
-  for (int n : kYZFixed)
-    if (p->nr2 == n && p->nr3 == n && p->yzpitch == (n | 1) && yz_is(p->fyz[0], n) && yz_is(p->fyz[1], n))
-      return n;
-  return 0;
+// the plan's yz plane is NY x NZ with the compile-time factorisations
+bool yz_match(const pw_plan *p, int ny, int nz) {
+  return p->nr2 == ny && p->nr3 == nz && p->yzpitch == (ny | 1) && yz_is(p->fyz[0], ny) && yz_is(p->fyz[1], nz);
+}
+template <int N> using IC = std::integral_constant<int, N>;
+// calls f(IC<NY>, IC<NZ>) with the plane's lengths when a compile-time
+// instantiation exists (the cells of the benchmarks and of the request mix:
+// 72^3 Si64, 36^3 Si8, 36 x 72 planes of Si16/Si32, 64^3, 80^3), else with 0, 0
+template <class F>
+int with_yz(const pw_plan *p, F &&f) {
+  if (yz_match(p, 72, 72)) return f(IC<72>{}, IC<72>{});
+  if (yz_match(p, 36, 36)) return f(IC<36>{}, IC<36>{});
+  if (yz_match(p, 36, 72)) return f(IC<36>{}, IC<72>{});
+  if (yz_match(p, 72, 36)) return f(IC<72>{}, IC<36>{});
+  if (yz_match(p, 64, 64)) return f(IC<64>{}, IC<64>{});
+  if (yz_match(p, 80, 80)) return f(IC<80>{}, IC<80>{});
+  return f(IC<0>{}, IC<0>{});
 }
 
 template <typename T2>
@@ -1055,15 +1079,14 @@ int vpsi_t(pw_plan *p, int nb, const double2 *psi, const double *v, double2 *hps
     PW_TRY(yz_lds(k_yz_vpsi<T2, true>, lds));
     k_yz_vpsi<T2, true><<<dim3(p->nr1, nb), kYZ, lds, s>>>(g, p->fyz[0], p->fyz[1], twid<T2>(p, 1), twid<T2>(p, 2),
                                                            (T2 *)p->d_stripe_buf, (const T *)p->d_vt);
-  } else if (yz_fixed(p) == 72) {
-    PW_TRY(yz_lds(k_yz_vpsi<T2, false, 72, 72>, lds));
-    k_yz_vpsi<T2, false, 72, 72><<<dim3(p->nr1, nb), kYZ, lds, s>>>(g, p->fyz[0], p->fyz[1], twid<T2>(p, 1),
-                                                                    twid<T2>(p, 2), (T2 *)p->d_stripe_buf,
-                                                                    (const T *)p->d_vt);
   } else {
-    PW_TRY(yz_lds(k_yz_vpsi<T2, false>, lds));
-    k_yz_vpsi<T2, false><<<dim3(p->nr1, nb), kYZ, lds, s>>>(g, p->fyz[0], p->fyz[1], twid<T2>(p, 1), twid<T2>(p, 2),
-                                                            (T2 *)p->d_stripe_buf, (const T *)p->d_vt);
+    PW_TRY(with_yz(p, [&](auto ny, auto nz) -> int {
+      constexpr int NY = decltype(ny)::value, NZ = decltype(nz)::value;
+      PW_TRY(yz_lds(k_yz_vpsi<T2, false, NY, NZ>, lds));
+      k_yz_vpsi<T2, false, NY, NZ><<<dim3(p->nr1, nb), kYZ, lds, s>>>(
+          g, p->fyz[0], p->fyz[1], twid<T2>(p, 1), twid<T2>(p, 2), (T2 *)p->d_stripe_buf, (const T *)p->d_vt);
+      return PW_OK;
+    }));
   }
   PW_HIP(hipGetLastError());
   return run_inverse_x<T2>(p, g, nb, hpsi, psi, 1.0 / (double)p->nrl, add_kin, s);
@@ -1099,11 +1122,16 @@ int density_t(pw_plan *p, int nb, const double2 *psi, const double *occ2, double
                                                                  twid<T2>(p, 2), (const T2 *)p->d_stripe_buf, \
                                                                  nb, bpg, occ2, (double *)p->d_dens_part);  \
   } while (0)
-  if (yz_big(p)) PW_DENS(true, false);
-  else if (reg && yz_fixed(p) == 72) PW_DENS(false, true, 72, 72);
-  else if (reg) PW_DENS(false, true);
-  else if (yz_fixed(p) == 72) PW_DENS(false, false, 72, 72);
-  else PW_DENS(false, false);
+  if (yz_big(p)) {
+    PW_DENS(true, false);
+  } else {
+    PW_TRY(with_yz(p, [&](auto ny, auto nz) -> int {
+      constexpr int NY = decltype(ny)::value, NZ = decltype(nz)::value;
+      if (reg) PW_DENS(false, true, NY, NZ);
+      else PW_DENS(false, false, NY, NZ);
+      return PW_OK;
+    }));
+  }
 #undef PW_DENS
   // rho[zy][x] = sum_group part[group][x][zy]
   const int A = p->nr1, B = (int)n23;

@@ -48,6 +48,10 @@ SYSTEMS = {
     "h2o32": model.water,
     # Si8 with the si2 deck's projector set 2 s + 2 p + 1 d (LMNX = 13)
     "si8d": lambda: model.si8(species=model.si_species_d()),
+    # the Si8 and Si16 cells of the request mix (36^3 and 36 x 36 x 72 grids:
+    # the x, y and z lengths of the compile-time FFT kernels besides 72)
+    "si8s": lambda: model.si_supercell(8, seed=5),
+    "si16": lambda: model.si_supercell(16, seed=6),
     # the benchmarked cell: 64 atoms, 144 bands, 23 847 G, 72^3, 512 projectors
     # (the oracle takes ~2 s per step)
     "si64": model.si64,

@@ -41,7 +41,8 @@ def ortho_residual(o, psi, r=None):
 
 @pytest.mark.parametrize("name,nsteps,moving", [("tiny", 4, False), ("si8", 5, False), ("si8c", 3, False),
                                                  ("h2o4", 4, False), ("synthetic", 2, False), ("h2o32", 2, False),
-                                                 ("si8d", 3, False), ("si64", 2, False),
+                                                 ("si8d", 3, False), ("si8s", 2, False), ("si16", 2, False),
+                                                 ("si64", 2, False),
                                                  ("si8", 3, True), ("si64", 2, True)])
 def test_mixed_form_error_gates(name, nsteps, moving):
     s = SYSTEMS[name]()

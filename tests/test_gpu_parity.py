@@ -93,7 +93,7 @@ def test_etot_first_call(name):
 
 
 @pytest.mark.parametrize("name,nsteps", [("tiny", 6), ("si8", 4), ("si8c", 3), ("h2o4", 4), ("synthetic", 2),
-                                          ("h2o32", 2), ("si8d", 3), ("si64", 2)])
+                                          ("h2o32", 2), ("si8d", 3), ("si8s", 2), ("si16", 2), ("si64", 2)])
 def test_cp_steps(name, nsteps):
     s = SYSTEMS[name]()
     say(f"case cp steps {name} x{nsteps}")
