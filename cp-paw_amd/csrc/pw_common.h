@@ -131,6 +131,7 @@ struct pw_plan {
   void *d_pot = nullptr; size_t pot_bytes = 0;
   double *h_pot = nullptr; size_t h_pot_bytes = 0; int pot_nat = 0;   // pinned: results and staged atom data
   int pot_lmrxx = 1;           // LMRXX of the last call
+  int pot_xc = 1;              // functional type (DFT$SETI4('TYPE')): 1 LDA (PZ), 10 PBE
   double *d_pot_res = nullptr; // device results of the last call: per atom VQLM(LMRXX), force(3);
                                // then EHARTREE, EXC, RHOB
 };

@@ -1,7 +1,8 @@
 // POTENTIAL$VOFRHO (src/paw_potential.f90:324-896) on the GPU, FP64: the
 // density -> local potential step that follows WAVES_DENSITY, for one spin,
 // the local-density functional of type 1 (local exchange + Perdew-Zunger,
-// src/paw_dft.f90:124-130), compensation charges of the moments QLM(LMRXX,NAT)
+// src/paw_dft.f90:124-130) or the gradient-corrected type 10 (PBE,
+// :248-262), compensation charges of the moments QLM(LMRXX,NAT)
 // with l <= 4 (LMRXX <= 25), no pair potential of the compensation charges
 // and no isolation.
 //
@@ -11,6 +12,8 @@
 //            VBAR, V0, background, EHARTREE partials (POTENTIAL_HARTREE
 //            :1069-1334, CFAC = (-i)^l/(2l+1)!!, YLMOFG of POTENTIAL_YLMOFG :1548)
 //   rho(r)   = GTOR rho(G);  exc(r), vxc(r) by LDA       (POTENTIAL_XC :1339-1545)
+//            type 10: grad rho = GTOR(i G rho(G)), PBE, and the divergence
+//            -i G . RTOG(2 de/d|grad rho|^2 grad rho) added to vxc(G)
 //   vtot(G)  = RTOG vxc(r) + Hartree potential
 //   per atom: VQLM(LM), Hartree/VBAR forces and the pseudo-core force
 //            (POTENTIAL_FPSCORE :897-975) in one G pass
@@ -266,6 +269,128 @@ __global__ void __launch_bounds__(kPotThreads) k_xc(int n, double2 *__restrict__
   if (threadIdx.x == 0) xpart[blockIdx.x] = s;
 }
 
+// Functional type 10 (src/paw_dft.f90:248-262) without spin: PBE exchange
+// (EXCHANGE_X1 with EXCHANGE_FX2's PBE enhancement, src/paw_dft.f90:2137-2236,
+// 2577-2603) and PBE correlation over the Perdew-Wang 1992 local part
+// (PBE$EVAL1 :6416-6519, PBE_H0OFTEG1 :5919, PBE_FOFTV1 :6174,
+// PERDEWWANG91L_RHODEP1 :4519), called as DFT does (:780-965: densities below
+// RHOTMIN evaluated at RHOTMIN with the same gradient and scaled back).
+// e: energy density; vt = de/drho; gv = de/d|grad rho|^2.
+constexpr double kPwA = 3.10907e-2, kPwAlpha1 = 2.1370e-1;
+constexpr double kPwB1 = 7.5957, kPwB2 = 3.5876, kPwB3 = 1.6382, kPwB4 = 4.9294e-1;
+constexpr double kPbeKappa = 0.804, kPbeCC0 = 4.235e-3;
+
+__device__ inline void pbe_point(double rho, double g2, double &e, double &vt, double &gv) {
+  const double rhot = fmax(rho, 1.0e-8);
+  const double val = fmax(rhot, kRhoTMin);
+  const double kf3 = 3.0 * kPi * kPi, kffac = cbrt(kf3);
+  const double beta = 16.0 / kPi * kffac * kPbeCC0;
+  const double mu = beta * kPi * kPi / 3.0;
+  const double gam = (1.0 - log(2.0)) / (kPi * kPi);
+  // exchange: EXFAC rho^(4/3) (1 + kappa - kappa/(1 + mu s^2/kappa))
+  const double r13 = cbrt(val), r43 = r13 * r13 * r13 * r13;
+  const double exl = -3.0 / (4.0 * kPi) * kffac * r43;
+  const double s2g = 0.25 / (kffac * kffac) / (r43 * r43);
+  const double s2 = s2g * g2;
+  const double den = 1.0 + mu / kPbeKappa * s2;
+  const double fx = 1.0 + kPbeKappa - kPbeKappa / den;
+  const double dfx = mu / (den * den);
+  const double ex = exl * fx;
+  const double ex_r = exl * (4.0 / 3.0) / val * fx - exl * dfx * (8.0 / 3.0) * s2 / val;
+  const double ex_g = exl * dfx * s2g;
+  // PW92 eps_c(rs) and d eps_c/d rho
+  const double rs = cbrt(3.0 / (4.0 * kPi)) / r13, sq = sqrt(rs);
+  const double q0 = -2.0 * kPwA * (1.0 + kPwAlpha1 * rs);
+  const double q1 = 2.0 * kPwA * (kPwB1 * sq + kPwB2 * rs + kPwB3 * rs * sq + kPwB4 * rs * rs);
+  const double dq1 = kPwA * (kPwB1 / sq + 2.0 * kPwB2 + 3.0 * kPwB3 * sq + 4.0 * kPwB4 * rs);
+  const double lg = log1p(1.0 / q1);
+  const double eps = q0 * lg;
+  const double deps = (-2.0 * kPwA * kPwAlpha1 * lg - q0 * dq1 / (q1 * q1 + q1)) * (-rs / (3.0 * val));
+  // H(t^2, eps) = gamma ln(1 + F), F = (beta/gamma) t^2 (1 + A t^2)/(1 + A t^2 + A^2 t^4)
+  const double tn = kPi / (16.0 * kffac) / (r43 * r13 * r13 * r13);   // t^2 / |grad rho|^2 = TFAC rho^(-7/3)
+  const double t2 = tn * g2;
+  const double c = beta / gam;
+  const double x = -eps / gam;
+  double f, f_t, h_eps;
+  if (x > 100.0) {
+    f = c * t2;
+    f_t = c;
+    h_eps = 0.0;
+  } else {
+    const double ee = exp(x);
+    const double ai = (ee - 1.0) / c;          // 1/A
+    const double d2 = ai * ai + ai * t2 + t2 * t2;
+    f = c * ai * t2 * (ai + t2) / d2;
+    f_t = c * ai * ai * ai * (2.0 * t2 + ai) / (d2 * d2);
+    const double f_a = c * t2 * t2 * t2 * (2.0 * ai + t2) / (d2 * d2);
+    h_eps = gam / (1.0 + f) * f_a * (-ee / (c * gam));
+  }
+  const double h = gam * log1p(f);
+  const double h_t = gam / (1.0 + f) * f_t;
+  const double h_r = h_t * (-7.0 / 3.0) * t2 / val + h_eps * deps;
+  const double ec = val * (eps + h);
+  const double ec_r = eps + h + val * (deps + h_r);
+  const double ec_g = val * h_t * tn;
+  e = ex + ec;
+  vt = ex_r + ec_r;
+  gv = ex_g + ec_g;
+  if (rhot < kRhoTMin) {
+    vt = e / kRhoTMin;
+    gv = gv * rhot / kRhoTMin;
+    e = e * rhot / kRhoTMin;
+  }
+}
+
+// F_k(G) = i G_k rho(G), k = 0..2 (POTENTIAL_GRADIENT 'RHO', src/paw_potential.f90:1003-1012)
+__global__ void k_grad_g(int ngl, const double *__restrict__ gvec, const double2 *__restrict__ rhog,
+                         double2 *__restrict__ f) {
+  const int ig = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ig >= ngl) return;
+  const double2 r = rhog[ig];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const double gk = gvec[3 * ig + k];
+    f[(size_t)k * ngl + ig] = make_double2(-gk * r.y, gk * r.x);
+  }
+}
+
+// PBE on the grid: c = rho + core (real part), d[k] = d rho/dx_k; c <- vt,
+// d[k] <- 2 de/d|grad rho|^2 d rho/dx_k (POTENTIAL_XC :1405-1470); exc partials
+__global__ void __launch_bounds__(kPotThreads) k_xc_gga(int n, double2 *__restrict__ c, double2 *__restrict__ d,
+                                                        double *__restrict__ xpart) {
+  __shared__ double sh[kPotThreads / 64];
+  const int i = blockIdx.x * kPotThreads + threadIdx.x;
+  double e = 0.0;
+  if (i < n) {
+    const double gx = d[i].x, gy = d[(size_t)n + i].x, gz = d[2 * (size_t)n + i].x;
+    double vt, gv;
+    pbe_point(c[i].x, gx * gx + gy * gy + gz * gz, e, vt, gv);
+    c[i] = make_double2(vt, 0.0);
+    const double s = 2.0 * gv;
+    d[i] = make_double2(s * gx, 0.0);
+    d[(size_t)n + i] = make_double2(s * gy, 0.0);
+    d[2 * (size_t)n + i] = make_double2(s * gz, 0.0);
+  }
+  const double s = block_sum(e, sh);
+  if (threadIdx.x == 0) xpart[blockIdx.x] = s;
+}
+
+// v(G) += sum_k -i G_k F_k(G) (POTENTIAL_GRADIENT 'POT', :1013-1027)
+__global__ void k_div_g(int ngl, const double *__restrict__ gvec, const double2 *__restrict__ f,
+                        double2 *__restrict__ v) {
+  const int ig = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ig >= ngl) return;
+  double2 o = v[ig];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const double gk = gvec[3 * ig + k];
+    const double2 fk = f[(size_t)k * ngl + ig];
+    o.x += gk * fk.y;
+    o.y -= gk * fk.x;
+  }
+  v[ig] = o;
+}
+
 __global__ void k_zero_d(int n, double *x) {
   if ((int)threadIdx.x < n) x[threadIdx.x] = 0.0;
 }
@@ -375,7 +500,8 @@ int vofrho_enqueue(pw_plan *p, int nat, int lmrxx, const int *isp, const double 
   const int nbr = (nrl + kPotThreads - 1) / kPotThreads;
   const int nval = lmrxx + 3;
   // work: grid (complex), rhog, vg, vh, vtot (sphere), partials, per-atom data
-  const size_t need = (size_t)nrl * 16 + 4 * (size_t)ngl * 16 +
+  const bool gga = p->pot_xc == 10;
+  const size_t need = (size_t)nrl * 16 + 4 * (size_t)ngl * 16 + (gga ? 3 * ((size_t)nrl + ngl) * 16 : 0) +
                       ((size_t)nbg * (nval * (size_t)nat + 1) + nbr + 8) * 8 +
                       ((size_t)nat * nval + 3) * 8 + (size_t)nat * (8 + 3 * 8 + lmrxx * 8) + (size_t)nsp * 8 + 4096;
   if (need > p->pot_bytes) {
@@ -410,7 +536,11 @@ int vofrho_enqueue(pw_plan *p, int nat, int lmrxx, const int *isp, const double 
   int *d_isp = (int *)w; w += (size_t)nat * 8;
   int *d_lmrx = (int *)w; w += (size_t)nsp * 8;
   double *d_gr = (double *)w; w += (size_t)nat * 3 * 8;
-  double *d_qlm = (double *)w;
+  double *d_qlm = (double *)w; w += (size_t)nat * lmrxx * 8;
+  w = (char *)p->d_pot + ((size_t)(w - (char *)p->d_pot) + 255) / 256 * 256;   // inside the 4096-byte slack
+  // the gradient of rho + core on the grid and its Fourier coefficients (type 10)
+  double2 *grid3 = gga ? (double2 *)w : nullptr;
+  double2 *fg3 = gga ? grid3 + 3 * (size_t)nrl : nullptr;
   // per-atom data: 0-based species, b_k . R_a, QLM (staged in pinned memory)
   double *hbase = p->h_pot + hres;
   int *hisp = (int *)hbase;
@@ -458,10 +588,20 @@ int vofrho_enqueue(pw_plan *p, int nat, int lmrxx, const int *isp, const double 
   // core, compensation, Hartree
   k_zero_d<<<1, 64, 0, st>>>(8, scal);
   k_pot_g<<<nbg, kPotThreads, 0, st>>>(g, rhog, vg, vh, epart, scal);
-  // density + core on the grid, LDA, back to G
+  // density + core on the grid, XC, back to G
   PW_TRY(fft_gtor_full(p, 1, rhog, grid, PW_PREC_FP64, st));
-  k_xc<<<nbr, kPotThreads, 0, st>>>(nrl, grid, xpart);
-  PW_TRY(fft_rtog_full(p, 1, vtot, grid, PW_PREC_FP64, st));
+  if (gga) {
+    // POTENTIAL_GRADIENT('RHO'), DFT type 10, POTENTIAL_GRADIENT('POT')
+    k_grad_g<<<nbg, kPotThreads, 0, st>>>(ngl, p->d_gvec, rhog, fg3);
+    PW_TRY(fft_gtor_full(p, 3, fg3, grid3, PW_PREC_FP64, st));
+    k_xc_gga<<<nbr, kPotThreads, 0, st>>>(nrl, grid, grid3, xpart);
+    PW_TRY(fft_rtog_full(p, 1, vtot, grid, PW_PREC_FP64, st));
+    PW_TRY(fft_rtog_full(p, 3, fg3, grid3, PW_PREC_FP64, st));
+    k_div_g<<<nbg, kPotThreads, 0, st>>>(ngl, p->d_gvec, fg3, vtot);
+  } else {
+    k_xc<<<nbr, kPotThreads, 0, st>>>(nrl, grid, xpart);
+    PW_TRY(fft_rtog_full(p, 1, vtot, grid, PW_PREC_FP64, st));
+  }
   k_add_c<<<nbg, kPotThreads, 0, st>>>(ngl, vh, vtot);
   // per-atom sums
   if (nat > 0) {
@@ -520,6 +660,16 @@ extern "C" int pw_potential_tables(pw_plan *p, int nsp, const int *lmrx, const d
   PW_HIP(hipMemcpy(p->d_pot_tabs, tabs, n * sizeof(double), hipMemcpyHostToDevice));
   p->pot_nsp = nsp;
   p->pot_lmrx = l;
+  return PW_OK;
+}
+
+extern "C" int pw_potential_functional(pw_plan *p, int type) {
+  if (!p) return pw_fail(PW_ERR_ARG, "pw_potential_functional: null plan");
+  if (type != 1 && type != 10)
+    return pw_fail(PW_ERR_UNSUPPORTED, "DFT$SETI4('TYPE'): only type 1 (LDA, Perdew-Zunger) and 10 (PBE) "
+                                       "are supported");
+  PW_HIP(hipStreamSynchronize(p->stream));
+  p->pot_xc = type;
   return PW_OK;
 }
 

@@ -868,6 +868,9 @@ extern "C" int pw_waves_etot(pw_waveset *ws) {
     // POTENTIAL$VOFRHO: the local potential of this density, used by VPSI
     // below; its energy EHARTREE + EXC and forces are read with the results
     if (sphere_ready(ws)) {
+      if (ws->dplan->pot_xc != 1)
+        return pw_fail(PW_ERR_UNSUPPORTED, "WAVES$ETOT: the one-centre terms take the local functional (type 1) "
+                                           "only; gradient corrections inside the spheres are not implemented");
       // the one-centre terms linked in (src/paw_waves1.f90:1843-1915):
       // WAVES$MOMENTS -> QLM, POTENTIAL$VOFRHO(rho, QLM) -> VQLM and RHOB,
       // WAVES$SPHERE -> DH, POTB, E1C, then WAVES$ADDCONSTANTPOT

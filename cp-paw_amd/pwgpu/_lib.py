@@ -86,6 +86,7 @@ SIGNATURES = {
     "pw_waves_density": [_vp, _vp, _vp],
     "pw_ortho_x": [ctypes.c_int, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_int_p],
     "pw_potential_tables": [_vp, ctypes.c_int, _c_int_p, _c_dbl_p],
+    "pw_potential_functional": [_vp, ctypes.c_int],
     "pw_waves_set_potential": [_vp, _vp],
     "pw_vofrho": [_vp, ctypes.c_int, ctypes.c_int, _c_int_p, _c_dbl_p, _c_dbl_p, _vp, _vp, _c_dbl_p, _c_dbl_p,
                   _c_dbl_p],

@@ -17,9 +17,10 @@ from .planewave import DeviceArray
 
 class Potential:
     """dplane: PlaneWave of the density cutoff; tables: (4, NSP, NGL) real
-    PSCORG, VBARG, G0, V0 on its sphere; lmrx: LMRX per species (default 1)."""
+    PSCORG, VBARG, G0, V0 on its sphere; lmrx: LMRX per species (default 1);
+    functional: DFT type, 1 (local exchange + Perdew-Zunger) or 10 (PBE)."""
 
-    def __init__(self, dplane, tables, lmrx=None):
+    def __init__(self, dplane, tables, lmrx=None, functional=1):
         self.lib = _lib.load()
         self.plane = dplane
         t = np.ascontiguousarray(np.asarray(tables, dtype=np.float64))
@@ -29,6 +30,8 @@ class Potential:
         self.lmrx = np.ones(self.nsp, dtype=np.int32) if lmrx is None else \
             np.ascontiguousarray(np.asarray(lmrx, dtype=np.int32))
         check(self.lib.pw_potential_tables(dplane.handle, self.nsp, iptr(self.lmrx), dptr(t)))
+        self.functional = int(functional)
+        check(self.lib.pw_potential_functional(dplane.handle, self.functional))
 
     def vofrho_device(self, isp, pos, qlm, rho_dev, vofr_dev):
         """POTENTIAL$VOFRHO on device arrays (NRL doubles); isp 0-based; qlm

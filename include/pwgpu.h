@@ -203,7 +203,8 @@ int pw_ortho_x(int nb, const double *occ, const double *chichi,
  * functional of type 1 (local exchange + Perdew-Zunger, src/paw_dft.f90:
  * 124-130), compensation charges of the multipole moments QLM(LMRXX,NAT)
  * with l <= 4 (LMRXX <= 25; POTENTIAL_HARTREE :1145-1173, POTENTIAL_YLMOFG
- * :1548), no pair potential of the compensation charges and no isolation.
+ * :1548), no pair potential of the compensation charges and no isolation;
+ * functional type 10 (PBE) through pw_potential_functional.
  * The plan is the density plane-wave object (PLANEWAVE$SELECT('DENSITY')):
  * its sphere is that of the density cutoff.  FP64.
  * pw_potential_tables sets the setup tables the reference keeps as module
@@ -217,6 +218,14 @@ int pw_ortho_x(int nb, const double *occ, const double *chichi,
  * dE/dQLM (zero beyond a species' LMRX), force(3,NAT) = the Hartree, V-bar
  * and pseudo-core forces (each may be NULL except energies). */
 int pw_potential_tables(pw_plan *dplan, int nsp, const int *lmrx, const double *tabs);
+/* DFT$SETI4('TYPE',type) (src/paw_dft.f90:511) for the density object:
+ * 1 = local exchange + Perdew-Zunger (the default), 10 = PBE (exchange type 3
+ * and PBE correlation over Perdew-Wang 1992, src/paw_dft.f90:248-262), whose
+ * gradient of rho + core is formed from its Fourier coefficients and whose
+ * divergence term joins the potential in G space (POTENTIAL_GRADIENT,
+ * src/paw_potential.f90:977-1034).  Other types: PW_ERR_UNSUPPORTED.  The
+ * one-centre terms (pw_waves_sphere_setup) take type 1 only. */
+int pw_potential_functional(pw_plan *dplan, int type);
 int pw_vofrho(pw_plan *dplan, int nat, int lmrxx, const int *isp, const double *pos,
               const double *qlm, const double *rho_dev, double *vofr_dev,
               double *energies, double *vqlm, double *force);

@@ -83,6 +83,110 @@ def lda(rho):
     return exc, dexc
 
 
+# ---------------------------------------------------------------------------
+# Functional type 10 (src/paw_dft.f90:248-262): PBE exchange (EXCHANGE type 3,
+# src/paw_dft.f90:1647-1651, 1706-1720) and PBE correlation with the
+# Perdew-Wang 1992 local part (PBE$EVAL1, src/paw_dft.f90:6416-6519;
+# PERDEWWANG91L, :4398-4830), without spin; no table interpolation
+# (TSAFE, src/paw_dft.f90:56).
+# ---------------------------------------------------------------------------
+# PW92 parameters of the unpolarised gas (src/paw_dft.f90:4428-4433)
+PW_A, PW_ALPHA1, PW_BETA = 3.10907e-2, 2.1370e-1, (7.5957, 3.5876, 1.6382, 4.9294e-1)
+# PBE: kappa, beta = nu * CC0, mu = beta pi^2 / 3, gamma = (1 - ln 2)/pi^2
+# (src/paw_dft.f90:5846-5896, 1706-1720); t^2 = TFAC |grad rho|^2 / (phi^2 rho^(7/3))
+# with the corrected TFAC (T_BUGFIX1, src/paw_dft.f90:5870-5879); the reduced
+# exchange gradient s^2 = S2FAC |grad rho|^2 / rho^(8/3) (src/paw_dft.f90:1693)
+PBE_CC0 = 4.235e-3
+PBE_NU = 16.0 / PI * (3.0 * PI ** 2) ** (1.0 / 3.0)
+PBE_BETA = PBE_NU * PBE_CC0
+PBE_MU = PBE_BETA * PI ** 2 / 3.0
+PBE_KAPPA = 0.804
+PBE_GAMMA = (1.0 - np.log(2.0)) / PI ** 2
+PBE_TFAC = PI / (16.0 * (3.0 * PI ** 2) ** (1.0 / 3.0))
+PBE_S2FAC = 0.25 / KFFAC ** 2
+
+
+def pw92_unpolarised(rho):
+    """PERDEWWANG91L_RHODEP1 (src/paw_dft.f90:4519-4574): eps_c(rho) of the
+    unpolarised gas, G(rs) = -2A (1 + a1 rs) ln(1 + 1/(2A (b1 rs^1/2 + b2 rs +
+    b3 rs^3/2 + b4 rs^2))), and d eps_c / d rho.  rho > 0 (array)."""
+    rs = RSFAC / rho ** (1.0 / 3.0)
+    sq = np.sqrt(rs)
+    b1, b2, b3, b4 = PW_BETA
+    q0 = -2.0 * PW_A * (1.0 + PW_ALPHA1 * rs)
+    q1 = 2.0 * PW_A * (b1 * sq + b2 * rs + b3 * rs * sq + b4 * rs * rs)
+    dq1 = PW_A * (b1 / sq + 2.0 * b2 + 3.0 * b3 * sq + 4.0 * b4 * rs)
+    lg = np.log1p(1.0 / q1)
+    eps = q0 * lg
+    deps_drs = -2.0 * PW_A * PW_ALPHA1 * lg - q0 * dq1 / (q1 * q1 + q1)
+    return eps, deps_drs * (-rs / (3.0 * rho))
+
+
+def pbe_exchange(rho, g2):
+    """EXCHANGE_X1 with the PBE enhancement (src/paw_dft.f90:2137-2236,
+    EXCHANGE_FX2 :2577-2603): EX = EXFAC rho^(4/3) Fx(s^2), Fx = 1 + kappa -
+    kappa/(1 + mu s^2/kappa); returns EX, dEX/drho, dEX/d|grad rho|^2."""
+    rho43 = rho ** (4.0 / 3.0)
+    exl = EXFAC * rho43
+    s2g = PBE_S2FAC / rho43 ** 2            # d s^2 / d |grad rho|^2
+    s2 = s2g * g2
+    den = 1.0 + PBE_MU / PBE_KAPPA * s2
+    fx = 1.0 + PBE_KAPPA - PBE_KAPPA / den
+    dfx = PBE_MU / (den * den)              # d Fx / d s^2
+    ex = exl * fx
+    ex_r = exl * (4.0 / 3.0) / rho * fx + exl * dfx * (-8.0 / 3.0) * s2 / rho
+    ex_g = exl * dfx * s2g
+    return ex, ex_r, ex_g
+
+
+def pbe_correlation(rho, g2):
+    """PBE$EVAL1 without spin (phi = 1): EC = rho (eps_c + H),
+    H = gamma ln(1 + (beta/gamma) t^2 (1 + A t^2)/(1 + A t^2 + A^2 t^4)),
+    A = (beta/gamma)/(exp(-eps_c/gamma) - 1) (PBE_H0OFTEG1 :5919-5977,
+    PBE_FOFTV1 :6174-6234); returns EC, dEC/drho, dEC/d|grad rho|^2."""
+    eps, deps = pw92_unpolarised(rho)
+    tn = PBE_TFAC / rho ** (7.0 / 3.0)      # d t^2 / d |grad rho|^2
+    t2 = tn * g2
+    c = PBE_BETA / PBE_GAMMA
+    x = -eps / PBE_GAMMA
+    big = x > 100.0                          # 1/A -> infinity: F = (beta/gamma) t^2
+    ex = np.exp(np.where(big, 0.0, x))
+    ai = (ex - 1.0) / c                      # 1/A
+    dai_deps = -ex / (c * PBE_GAMMA)
+    den = ai * ai + ai * t2 + t2 * t2
+    f = np.where(big, c * t2, c * ai * t2 * (ai + t2) / den)
+    f_t = np.where(big, c, c * ai ** 3 * (2.0 * t2 + ai) / den ** 2)
+    f_a = np.where(big, 0.0, c * t2 ** 3 * (2.0 * ai + t2) / den ** 2)
+    h = PBE_GAMMA * np.log1p(f)
+    h_f = PBE_GAMMA / (1.0 + f)
+    h_eps = h_f * f_a * dai_deps
+    h_t = h_f * f_t
+    h_rho = h_t * (-7.0 / 3.0) * t2 / rho + h_eps * deps
+    ec = rho * (eps + h)
+    ec_r = eps + h + rho * (deps + h_rho)
+    ec_g = rho * h_t * tn
+    return ec, ec_r, ec_g
+
+
+def pbe(rho, g2):
+    """Energy density, dE/drho and dE/d|grad rho|^2 of functional type 10 at
+    grid points, as POTENTIAL_XC (src/paw_potential.f90:1397-1401: rho clamped
+    to >= 1e-8) calls DFT (src/paw_dft.f90:780-965: densities below RHOTMIN
+    are evaluated at RHOTMIN with the same gradient and the results scaled
+    back linearly)."""
+    rhot = np.maximum(np.asarray(rho, dtype=np.float64), 1.0e-8)
+    g2 = np.asarray(g2, dtype=np.float64)
+    val = np.maximum(rhot, RHOTMIN)
+    ex, ex_r, ex_g = pbe_exchange(val, g2)
+    ec, ec_r, ec_g = pbe_correlation(val, g2)
+    exc, vt, gv = ex + ec, ex_r + ec_r, ex_g + ec_g
+    small = rhot < RHOTMIN
+    vt = np.where(small, exc / RHOTMIN, vt)
+    gv = np.where(small, gv * rhot / RHOTMIN, gv)
+    exc = np.where(small, exc * rhot / RHOTMIN, exc)
+    return exc, vt, gv
+
+
 def ylmofg(gvec, lmx):
     """POTENTIAL_YLMOFG (src/paw_potential.f90:1548-1614): solid harmonics
     |G|^l Y_LM(G) as (LMX, NG); (Y0, 0, ...) for |G| < 1e-7."""
@@ -115,7 +219,7 @@ def cfac(lmx):
     return out
 
 
-def vofrho(plane, rho_r, pos, isp, qlm, tabs, lmrx=None):
+def vofrho(plane, rho_r, pos, isp, qlm, tabs, lmrx=None, functional=1):
     """POTENTIAL$VOFRHO -> POTENTIAL_VOFRHO (src/paw_potential.f90:324-484,
     487-896) in the scope of the module docstring.
 
@@ -126,9 +230,15 @@ def vofrho(plane, rho_r, pos, isp, qlm, tabs, lmrx=None):
       LMRXX = 1; lmrx: LMRX per species (default LMRXX for all);
     tabs: dict of (NSP,NGL) real tables on the sphere: 'pscorg' (PS core),
       'vbarg' (V-bar), 'g0' (compensation Gaussians), 'v0' (their potential).
+    functional: 1 (local exchange + Perdew-Zunger) or 10 (PBE, with the
+      gradient of rho + core from its Fourier coefficients and the divergence
+      of 2 dE/d|grad rho|^2 grad rho added in G space, POTENTIAL_GRADIENT
+      src/paw_potential.f90:977-1034 and :749-787).
     Returns dict: v (NR3,NR2,NR1) total local potential, ehartree, exc, rhob,
       vqlm (shape of qlm) = dE/dQLM, force (NAT,3) (Hartree + pseudo-core terms).
     """
+    if functional not in (1, 10):
+        raise ValueError("functional type must be 1 or 10")
     om = plane.vol
     ig0 = plane.ngamma
     g2, gv = plane.g2, plane.gvec
@@ -188,10 +298,21 @@ def vofrho(plane, rho_r, pos, isp, qlm, tabs, lmrx=None):
         vqlm = vqlm[:, 0]
     # PLANEWAVE$SUPFFT('GTOR') of density + core, POTENTIAL_XC (src/paw_potential.f90:1339-1545)
     rho_tot = plane.fft_gtor(rhog).real
-    exc_r, vxc_r = lda(rho_tot)
+    if functional == 1:
+        exc_r, vxc_r = lda(rho_tot)
+        # no gradient correction: RHOG = RTOG(vxc) + Hartree potential
+        vtot = plane.fft_rtog(vxc_r.astype(np.complex128)) + pot
+    else:
+        # grad rho(r) = GTOR(i G rho(G)); after XC the vector 2 dE/d|grad rho|^2
+        # grad rho goes back to G space and -i G . F(G) joins the potential
+        grad = [plane.fft_gtor(1j * gv[:, k] * rhog).real for k in range(3)]
+        g2r = grad[0] ** 2 + grad[1] ** 2 + grad[2] ** 2
+        exc_r, vxc_r, gvr = pbe(rho_tot, g2r)
+        vtot = plane.fft_rtog(vxc_r.astype(np.complex128)) + pot
+        for k in range(3):
+            fk = plane.fft_rtog((2.0 * gvr * grad[k]).astype(np.complex128))
+            vtot = vtot - 1j * gv[:, k] * fk
     exc = float(np.sum(exc_r)) * om / float(plane.nrl)
-    # no gradient correction: RHOG = RTOG(vxc) + Hartree potential
-    vtot = plane.fft_rtog(vxc_r.astype(np.complex128)) + pot
     # POTENTIAL_FPSCORE (src/paw_potential.f90:897-975)
     for a, e in enumerate(eigr):
         sv = -(np.conj(vtot) * psc[isp[a]] * e).imag

@@ -579,8 +579,9 @@ class WaveSet:
         elif self.potential is not None:
             # POTENTIAL$VOFRHO between WAVES_DENSITY and WAVES_VPSI: the local
             # potential of this density (src/paw_potential.f90:324-896)
-            dplane, tabs, isp, qlm = self.potential
-            self.pot = vofrho(dplane, self.rho, r0, isp, qlm, tabs)
+            dplane, tabs, isp, qlm = self.potential[:4]
+            functional = self.potential[4] if len(self.potential) > 4 else 1
+            self.pot = vofrho(dplane, self.rho, r0, isp, qlm, tabs, functional=functional)
             self.v = self.pot["v"]
             e["epot"] = self.pot["ehartree"] + self.pot["exc"]
         else:

@@ -1,7 +1,7 @@
 """GPU parity of POTENTIAL$VOFRHO (pw_vofrho, src/paw_potential.f90:324-896)
 against the FP64 oracle (oracle/potential.py) on identical seeded inputs:
 the model valence density of the cell and the model setup tables on the
-density sphere.
+density sphere, with the local functional (type 1) and PBE (type 10).
 
 Tolerances (FP64 on both sides; the transforms and sums differ only in
 rounding order): EHARTREE, EXC, RHOB and dE/dQLM within 1e-10 relative, the
@@ -40,21 +40,22 @@ def inputs(s):
     return pl, rho, tabs
 
 
-def run_oracle(pl, s, rho, tabs):
+def run_oracle(pl, s, rho, tabs, functional=1):
     return opot.vofrho(pl, rho, s.pos, s.isp, s.qlm(),
-                       {"pscorg": tabs[0], "vbarg": tabs[1], "g0": tabs[2], "v0": tabs[3]})
+                       {"pscorg": tabs[0], "vbarg": tabs[1], "g0": tabs[2], "v0": tabs[3]}, functional=functional)
 
 
+@pytest.mark.parametrize("functional", [1, 10])
 @pytest.mark.parametrize("name", list(CASES))
-def test_vofrho_matches_oracle(name):
+def test_vofrho_matches_oracle(name, functional):
     import pwgpu
     s = CASES[name]()
-    say(f"case vofrho {name}")
+    say(f"case vofrho {name} functional={functional}")
     pl, rho, tabs = inputs(s)
-    o = run_oracle(pl, s, rho, tabs)
+    o = run_oracle(pl, s, rho, tabs, functional)
     dplane = pwgpu.PlaneWave(s.rbas, s.epw_rho, s.epw_rho)
     assert dplane.ngl == pl.ngl and dplane.nr == (pl.nr1, pl.nr2, pl.nr3)
-    pot = pwgpu.Potential(dplane, tabs)
+    pot = pwgpu.Potential(dplane, tabs, functional=functional)
     g = pot.vofrho(s.isp, s.pos, s.qlm(), rho)
     for k in ("ehartree", "exc", "rhob"):
         assert abs(g[k] - o[k]) <= 1e-10 * max(abs(o[k]), 1e-12), (k, g[k], o[k])
@@ -106,21 +107,23 @@ def test_vofrho_in_place_and_repeatable():
     assert np.array_equal(v, a["v"]) and np.array_equal(vq, a["vqlm"]) and np.array_equal(fo, a["force"])
 
 
-@pytest.mark.parametrize("name,nsteps", [("tiny", 4), ("si8", 3), ("h2o4", 3)])
-def test_scf_cp_steps_match_oracle(name, nsteps):
+@pytest.mark.parametrize("name,nsteps,functional", [("tiny", 4, 1), ("si8", 3, 1), ("h2o4", 3, 1),
+                                                     ("si8", 3, 10), ("h2o4", 2, 10)])
+def test_scf_cp_steps_match_oracle(name, nsteps, functional):
     # CP steps with the density -> potential step linked into ETOT
     # (pw_waves_set_potential) against the oracle with the same coupling
     import pwgpu
     from harness import gpu_waveset, oracle_waveset
     s = CASES[name]()
-    say(f"case scf cp {name} steps={nsteps}")
+    say(f"case scf cp {name} steps={nsteps} functional={functional}")
     o = oracle_waveset(s)
     dpl = opw.Plane(s.rbas, s.epw_rho, s.epw_rho)
     tabs = s.potential_tables(dpl.g2)
-    o.potential = (dpl, {"pscorg": tabs[0], "vbarg": tabs[1], "g0": tabs[2], "v0": tabs[3]}, s.isp, s.qlm())
+    o.potential = (dpl, {"pscorg": tabs[0], "vbarg": tabs[1], "g0": tabs[2], "v0": tabs[3]}, s.isp, s.qlm(),
+                   functional)
     plane, w = gpu_waveset(s)
     dplane = pwgpu.PlaneWave(s.rbas, s.epw_rho, s.epw_rho)
-    pot = pwgpu.Potential(dplane, tabs)
+    pot = pwgpu.Potential(dplane, tabs, functional=functional)
     w.set_potential(pot, s.qlm())
     for it in range(nsteps):
         eo = o.etot(s.pos)
@@ -139,3 +142,23 @@ def test_scf_cp_steps_match_oracle(name, nsteps):
         w.switch()
     assert relerr(w.getc8a("PSI0"), o.psi0) < 1e-9
     say(f"  {name}: etot={eg['etot']:.12f} E_H={w.getr8('EHARTREE'):.10f} Exc={w.getr8('EXC'):.10f}")
+
+
+def test_pbe_with_one_centre_terms_is_refused():
+    """The one-centre terms take the local functional only: an ETOT that would
+    couple them to PBE fails with PW_ERR_UNSUPPORTED instead of mixing the two."""
+    import pwgpu
+    from pwgpu import _lib
+    from harness import gpu_waveset
+    s = model.paw_system(model.tiny())
+    say("case pbe with one-centre terms refused")
+    plane, w = gpu_waveset(s)
+    dplane = pwgpu.PlaneWave(s.rbas, s.epw_rho, s.epw_rho)
+    tabs = s.potential_tables(opw.Plane(s.rbas, s.epw_rho, s.epw_rho).g2)
+    pot = pwgpu.Potential(dplane, tabs, lmrx=[sp.lmrx for sp in s.species], functional=10)
+    for k, sp in enumerate(s.species):
+        w.sphere_setup(k, sp.sphere_setup())
+    w.set_potential(pot)
+    with pytest.raises(_lib.PwError) as e:
+        w.etot()
+    assert "type 1" in str(e.value)

@@ -256,3 +256,125 @@ def test_vqlm_and_forces_with_higher_moments():
             em, _ = en(qlm, p)
             fd = -(ep - em) / (2 * h)
             assert abs(fd - r["force"][a, k]) < 1e-6 * max(1e-3, np.max(np.abs(r["force"]))), (a, k)
+
+
+# ---------------------------------------------------------------------------
+# functional type 10 (PBE): limits, scaling and derivatives
+# ---------------------------------------------------------------------------
+def test_pbe_exchange_scaling_and_limits():
+    rng = np.random.default_rng(11)
+    rho = 10.0 ** rng.uniform(-3, 1, size=40)
+    g2 = rho ** (8.0 / 3.0) * 10.0 ** rng.uniform(-3, 2, size=40)
+    ex, _, _ = opot.pbe_exchange(rho, g2)
+    # uniform scaling rho -> lam^3 rho(lam r): |grad rho|^2 -> lam^8, e_x -> lam^4
+    for lam in (0.5, 1.7, 3.0):
+        exl, _, _ = opot.pbe_exchange(lam ** 3 * rho, lam ** 8 * g2)
+        assert np.allclose(exl, lam ** 4 * ex, rtol=1e-13)
+    # no gradient: the local exchange; s^2 = |grad rho|^2 / (4 kF^2 rho^2), kF = (3 pi^2 rho)^(1/3)
+    ex0, vx0, _ = opot.pbe_exchange(rho, 0.0 * g2)
+    exl, vxl = opot.exchange_lda(rho)
+    assert np.allclose(ex0, exl, rtol=1e-14) and np.allclose(vx0, vxl, rtol=1e-13)
+    kf = (3.0 * math.pi ** 2 * rho) ** (1.0 / 3.0)
+    s2 = g2 / (4.0 * kf ** 2 * rho ** 2)
+    fx = ex / exl
+    # PBE enhancement: 1 + mu s^2 for small s, 1 + kappa (Lieb-Oxford) for large s
+    assert np.allclose(fx, 1.0 + opot.PBE_KAPPA - opot.PBE_KAPPA / (1.0 + opot.PBE_MU * s2 / opot.PBE_KAPPA),
+                       rtol=1e-13)
+    small = opot.pbe_exchange(rho, 1e-8 * rho ** (8.0 / 3.0))[0] / exl - 1.0
+    assert np.allclose(small, opot.PBE_MU * 1e-8 / (4.0 * (3.0 * math.pi ** 2) ** (2.0 / 3.0)), rtol=1e-6)
+    big = opot.pbe_exchange(rho, 1e12 * rho ** (8.0 / 3.0))[0] / exl
+    assert np.allclose(big, 1.0 + opot.PBE_KAPPA, rtol=1e-9)
+    # mu is fixed by the correlation's gradient coefficient: mu = beta pi^2 / 3
+    assert abs(opot.PBE_MU - opot.PBE_BETA * math.pi ** 2 / 3.0) < 1e-15
+
+
+def test_pw92_limits_and_published_relations():
+    # high density: eps_c -> A ln rs + const with A = (1 - ln 2)/pi^2
+    # (Gell-Mann-Brueckner), and beta1 = exp(-c1/(2A))/(2A) with c1 = 0.046644
+    assert abs(opot.PW_A - (1.0 - math.log(2.0)) / math.pi ** 2) < 1e-7
+    assert abs(math.exp(-0.046644 / (2 * opot.PW_A)) / (2 * opot.PW_A) - opot.PW_BETA[0]) < 2e-3
+    rs = np.array([1e-4, 2e-4])
+    e, _ = opot.pw92_unpolarised(3.0 / (4.0 * math.pi * rs ** 3))
+    assert abs((e[1] - e[0]) / math.log(2.0) - opot.PW_A) < 2e-3 * opot.PW_A
+    # the same Ceperley-Alder energies as the Perdew-Zunger fit, to 1 mHa
+    for rs1 in (1.0, 2.0, 5.0, 10.0):
+        rho = np.array([3.0 / (4.0 * math.pi * rs1 ** 3)])
+        epw = opot.pw92_unpolarised(rho)[0][0]
+        epz = opot.pz_unpolarised(rho)[0][0] / rho[0]
+        assert abs(epw - epz) < 1e-3, (rs1, epw, epz)
+
+
+def test_pbe_correlation_gradient_limits():
+    rho = np.array([1e-3, 0.02, 0.3, 2.0])
+    eps, _ = opot.pw92_unpolarised(rho)
+    tn = opot.PBE_TFAC / rho ** (7.0 / 3.0)
+    # t = 0: the local correlation
+    ec0, _, _ = opot.pbe_correlation(rho, 0.0 * rho)
+    assert np.allclose(ec0, rho * eps, rtol=1e-14)
+    # small t: H = beta t^2 (second-order gradient expansion)
+    t2 = 1e-7
+    ec, _, _ = opot.pbe_correlation(rho, t2 / tn)
+    assert np.allclose((ec / rho - eps) / t2, opot.PBE_BETA, rtol=1e-4)
+    # large t: H -> -eps_c (the correlation energy vanishes)
+    ec, _, _ = opot.pbe_correlation(rho, 1e8 / tn)
+    assert np.all(np.abs(ec / rho) < 1e-6 * np.abs(eps))
+    # t^2 = |grad rho|^2 / (4 phi^2 ks^2 rho^2), ks^2 = 4 kF / pi
+    kf = (3.0 * math.pi ** 2 * rho) ** (1.0 / 3.0)
+    assert np.allclose(tn, 1.0 / (4.0 * (4.0 * kf / math.pi) * rho ** 2), rtol=1e-13)
+
+
+@pytest.mark.parametrize("which", ["x", "c", "xc"])
+def test_pbe_derivatives(which):
+    rng = np.random.default_rng(5)
+    rho = 10.0 ** rng.uniform(-5.5, 1, size=30)          # includes points below RHOTMIN
+    g2 = rho ** (8.0 / 3.0) * 10.0 ** rng.uniform(-4, 3, size=30)
+    f = {"x": opot.pbe_exchange, "c": opot.pbe_correlation, "xc": opot.pbe}[which]
+    e, v, gv = f(rho, g2)
+    h = 1e-6 * rho
+    fd_r = (f(rho + h, g2)[0] - f(rho - h, g2)[0]) / (2 * h)
+    fd_g = (f(rho, g2 * (1 + 1e-6))[0] - f(rho, g2 * (1 - 1e-6))[0]) / (2e-6 * g2)
+    # the floor at RHOTMIN makes E piecewise in rho: skip points within a step of it
+    ok = np.abs(rho - opot.RHOTMIN) > 3e-6 * rho if which == "xc" else np.ones(rho.shape, bool)
+    # central differences of E carry ~1e-10 |E| of rounding per unit step
+    assert np.all(np.abs(v - fd_r)[ok] * rho[ok] <= 1e-9 * np.abs(e[ok]) + 2e-6 * np.abs(v * rho)[ok]), which
+    assert np.all(np.abs(gv - fd_g) * g2 <= 1e-9 * np.abs(e) + 2e-6 * np.abs(gv * g2)), which
+
+
+def energy_f(pl, rho, pos, isp, qlm, tabs, functional):
+    r = opot.vofrho(pl, rho, pos, isp, qlm, tabs, functional=functional)
+    return r["ehartree"] + r["exc"], r
+
+
+def test_pbe_potential_is_density_derivative():
+    # the gradient term -div(2 dE/d|grad rho|^2 grad rho) of the returned V:
+    # FD of EXC + EHARTREE along a smooth density change
+    s, pl, rho, tabs = system_case()
+    e0, r = energy_f(pl, rho, s.pos, s.isp, s.qlm(), tabs, 10)
+    e_lda, _ = energy_f(pl, rho, s.pos, s.isp, s.qlm(), tabs, 1)
+    assert abs(e0 - e_lda) > 1e-4       # the gradient correction matters here
+    rng = np.random.default_rng(8)
+    c = (rng.normal(size=pl.ngl) + 1j * rng.normal(size=pl.ngl)) * np.exp(-pl.g2)
+    c = 0.5 * (c + np.conj(c[pl.minusg]))
+    d = pl.fft_gtor(c).real
+    d *= 1e-3 / np.max(np.abs(d))
+    eps = 1e-3
+    ep, _ = energy_f(pl, rho + eps * d, s.pos, s.isp, s.qlm(), tabs, 10)
+    em, _ = energy_f(pl, rho - eps * d, s.pos, s.isp, s.qlm(), tabs, 10)
+    fd = (ep - em) / (2 * eps)
+    an = float(np.sum(r["v"] * d)) * pl.rweight
+    assert abs(fd - an) < 1e-7 * abs(an), (fd, an)
+
+
+def test_pbe_forces_are_position_derivatives():
+    s, pl, rho, tabs = system_case()
+    _, r = energy_f(pl, rho, s.pos, s.isp, s.qlm(), tabs, 10)
+    h = 1e-4
+    for a, k in ((0, 0), (1, 2), (3, 1)):
+        pp = np.array(s.pos, dtype=float)
+        pm = pp.copy()
+        pp[a, k] += h
+        pm[a, k] -= h
+        ep, _ = energy_f(pl, rho, pp, s.isp, s.qlm(), tabs, 10)
+        em, _ = energy_f(pl, rho, pm, s.isp, s.qlm(), tabs, 10)
+        fd = -(ep - em) / (2 * h)
+        assert abs(fd - r["force"][a, k]) < 1e-6 * max(1.0, abs(fd)), (a, k, fd, r["force"][a, k])
