@@ -220,6 +220,7 @@ struct pw_waveset {
   double lz_lo = 0.0, lz_hi = 0.0;
   bool lz_valid = false;
   int ortho_lanczos = 0;       // Lanczos runs in the last ORTHO_X call
+  int ortho_eigen = 0;         // ORTHO_X solves through the eigenvectors of sym(CHIPSI) (B not positive definite)
   int ortho_graph_misses = 0;  // ORTHO_X solves issued without a graph (other objects alive)
   // launch shapes chosen by pw_waves_autotune
   int tune_split_waves = 2048;  // waves of a G-split GEMM launch

@@ -12,7 +12,10 @@
 //   S^-1(G) = 2q sum_{k<K} P_k G P_k,  P_k = A^k M (symmetric), K = 2^J;
 //   the error after K terms is rho^(2K), rho = (sqrt(k)-1)/(sqrt(k)+1) for
 //   the spectral bounds of B (Lanczos on the device, widened).  B is
-//   positive definite in the CP step (B ~ dt^2/m).  The P_k are formed once
+//   positive definite in the CP step (B ~ dt^2/m); for a B that is not
+//   (or too ill-conditioned for the series) S^-1 is applied as the reference
+//   does, through the eigenvectors of B, which one workgroup computes by
+//   cyclic Jacobi rotations (ortho_x_eig).  The P_k are formed once
 //   per call by doubling (one row-panel batch launch per level); each outer
 //   iteration then applies the series in two launches (row-panel chains
 //   P_k G P_k, then the ordered sum over k with the convergence test and the
@@ -762,6 +765,128 @@ __global__ void k_cayley(int n, double q, const double *M, double *A) {
   A[idx] = ((idx / n == idx % n) ? 1.0 : 0.0) - 2.0 * q * M[idx];
 }
 
+// ---------------------------------------------------------------------------
+// S^-1 through the eigenvectors of B = sym(CHIPSI), as the reference applies
+// it (src/paw_waves2.f90:1889, 1956-1991: LIB$DIAGR8, then
+// U[(U^T G U)_ij/(e_i + e_j)]U^T), for a B that is not positive definite,
+// where the Cayley series does not converge.
+// One workgroup diagonalises B by cyclic Jacobi rotations in the parallel
+// round-robin order: each round rotates n/2 disjoint index pairs, A <- J^T A J
+// (columns, then rows) and U <- U J; sweeps of n-1 rounds until the
+// off-diagonal part is below 1e-15 of the diagonal in the Frobenius norm.
+// A (n x n, row-major, overwritten), U[i n + k] = component i of vector k.
+constexpr int kJacThreads = 1024;
+constexpr int kJacMaxSweeps = 60;
+__global__ void __launch_bounds__(kJacThreads) k_jacobi_eig(int n, double *A, double *U, double *e,
+                                                            OrthoState *st) {
+  __shared__ int sp[kJacThreads], sq[kJacThreads];
+  __shared__ double sc[kJacThreads], ss[kJacThreads];
+  __shared__ double red[2][kJacThreads / 64];
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const int m = n + (n & 1);         // players of the round robin; index n is a bye for odd n
+  const int np = m / 2;
+  for (int idx = t; idx < n * n; idx += kJacThreads) U[idx] = (idx / n == idx % n) ? 1.0 : 0.0;
+  if (t == 0) {
+    st->done = 0;
+    st->iter = 0;
+    st->code = 0;
+    st->pad = 0;
+  }
+  __syncthreads();
+  for (int sweep = 0; sweep < kJacMaxSweeps; ++sweep) {
+    double off = 0.0, dia = 0.0;
+    for (int idx = t; idx < n * n; idx += kJacThreads) {
+      const double a = A[idx];
+      if (idx / n == idx % n) dia += a * a;
+      else off += a * a;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      off += __shfl_xor(off, o);
+      dia += __shfl_xor(dia, o);
+    }
+    if (lane == 0) {
+      red[0][wv] = off;
+      red[1][wv] = dia;
+    }
+    __syncthreads();
+    off = 0.0;
+    dia = 0.0;
+    for (int w = 0; w < kJacThreads / 64; ++w) {
+      off += red[0][w];
+      dia += red[1][w];
+    }
+    __syncthreads();
+    if (!(off > 1.0e-30 * dia)) break;   // the same decision in every thread
+    for (int r = 0; r < m - 1; ++r) {
+      // pairs of round r: (m-1, r) and ((r+k) mod (m-1), (r-k) mod (m-1)), k = 1..np-1
+      for (int k = t; k < np; k += kJacThreads) {
+        int p = k == 0 ? m - 1 : (r + k) % (m - 1);
+        int q = k == 0 ? r : (r - k + (m - 1)) % (m - 1);
+        if (p > q) {
+          const int x = p;
+          p = q;
+          q = x;
+        }
+        double c = 1.0, s = 0.0;
+        if (q < n) {
+          const double apq = A[(size_t)p * n + q];
+          if (apq != 0.0) {
+            const double th = (A[(size_t)q * n + q] - A[(size_t)p * n + p]) / (2.0 * apq);
+            const double tt = (th >= 0.0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+            c = 1.0 / sqrt(tt * tt + 1.0);
+            s = tt * c;
+          }
+        } else {
+          q = p;   // the bye: no rotation
+        }
+        sp[k] = p;
+        sq[k] = q;
+        sc[k] = c;
+        ss[k] = s;
+      }
+      __syncthreads();
+      // A <- A J and U <- U J: columns p, q of every row
+      for (int idx = t; idx < np * n; idx += kJacThreads) {
+        const int k = idx / n, i = idx - k * n;
+        const int p = sp[k], q = sq[k];
+        if (p == q) continue;
+        const double c = sc[k], s = ss[k];
+        double *ar = A + (size_t)i * n;
+        const double ap = ar[p], aq = ar[q];
+        ar[p] = c * ap - s * aq;
+        ar[q] = s * ap + c * aq;
+        double *ur = U + (size_t)i * n;
+        const double up = ur[p], uq = ur[q];
+        ur[p] = c * up - s * uq;
+        ur[q] = s * up + c * uq;
+      }
+      __syncthreads();
+      // A <- J^T A: rows p, q
+      for (int idx = t; idx < np * n; idx += kJacThreads) {
+        const int k = idx / n, j = idx - k * n;
+        const int p = sp[k], q = sq[k];
+        if (p == q) continue;
+        const double c = sc[k], s = ss[k];
+        const double ap = A[(size_t)p * n + j], aq = A[(size_t)q * n + j];
+        A[(size_t)p * n + j] = c * ap - s * aq;
+        A[(size_t)q * n + j] = s * ap + c * aq;
+      }
+      __syncthreads();
+    }
+  }
+  for (int i = t; i < n; i += kJacThreads) e[i] = A[(size_t)i * n + i];
+}
+
+// X_ij <- X_ij / (e_i + e_j)
+__global__ void k_eig_scale(int n, const double *e, double *X, const OrthoState *st) {
+  if (stopped(st)) return;
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= n * n) return;
+  const int i = idx / n, j = idx - i * n;
+  X[idx] /= e[i] + e[j];
+}
+
 struct OrthoWork {
   double *bs, *m0, *m1, *gam, *x0, *x1, *t1, *t2, *lam2, *occ, *red, *bounds;
   double *apow[kMaxLevels];
@@ -1007,6 +1132,94 @@ int enqueue_iters(pw_waveset *ws, const OrthoWork &w, const OrthoParams &P, doub
   return PW_OK;
 }
 
+// outer iterations it0..it1 with S^-1 from the eigenvectors U (w.m1) and the
+// eigenvalues e (w.x1) of B: the reference's own form of the update
+int enqueue_iters_eig(pw_waveset *ws, const OrthoWork &w, double *const lbuf[2], const double *d_chichi,
+                      const double *d_psipsi, const double *d_chipsi, int it0, int it1) {
+  const int n = ws->nb, nt = (n + T16 - 1) / T16;
+  const int nblk = (n * n + 255) / 256;
+  for (int it = it0; it <= it1; ++it) {
+    const double *lam = lbuf[(it - 1) & 1];
+    double *lnew = lbuf[it & 1];
+    gemm(ws, d_chichi, 0, lam, 0, w.t1, 1.0, d_chipsi, 2.0, w.st);
+    {
+      PairArgs g{};
+      g.n = n;
+      g.A = lam;
+      g.ta = 1;
+      g.B = w.t1;
+      g.alpha = 1.0;
+      g.out = w.gam;
+      g.psipsi = d_psipsi;
+      g.bmax = w.red;
+      gemm_pair<kPairGamma>(ws, g, w.st);
+    }
+    gemm(ws, w.m1, 1, w.gam, 0, w.t2, 1.0, nullptr, 0.0, w.st);    // U^T G
+    gemm(ws, w.t2, 0, w.m1, 0, w.x0, 1.0, nullptr, 0.0, w.st);     // (U^T G) U
+    k_eig_scale<<<nblk, 256, 0, ws->stream>>>(n, w.x1, w.x0, w.st);
+    gemm(ws, w.m1, 0, w.x0, 0, w.t2, 1.0, nullptr, 0.0, w.st);     // U (...)
+    gemm(ws, w.t2, 0, w.m1, 1, w.spart, 1.0, nullptr, 0.0, w.st);  // (...) U^T
+    SumArgs su{};
+    su.n = n;
+    su.nterm = 1;
+    su.part = w.spart;
+    su.scale = 1.0;
+    su.occ = w.occ;
+    su.lam = lam;
+    su.bmax = w.red;
+    su.nbmax = nt * nt;
+    su.tol = 1.0e-8;
+    su.it = it;
+    su.out = lnew;
+    series_sum<kSeriesLambda>(ws, su, w.st);
+  }
+  PW_HIP(hipGetLastError());
+  return PW_OK;
+}
+
+// WAVES_ORTHO_X with S^-1 from the eigenvectors of B (the reference's form),
+// for a B that is not positive definite (inside the Lanczos bounds or
+// rejected by the series' acceptance test)
+int ortho_x_eig(pw_waveset *ws, const OrthoWork &w, double *const lbuf[2], double *d_lam, const double *d_chichi,
+                const double *d_psipsi, const double *d_chipsi, int *iter) {
+  const int n = ws->nb;
+  const size_t n2 = (size_t)n * n;
+  const int nblk = (int)((n2 + 255) / 256);
+  if (n > 2 * kJacThreads) return pw_fail(PW_ERR_UNSUPPORTED, "WAVES_ORTHO_X: NB too large");
+  ws->ortho_eigen++;
+  k_copy_d<<<nblk, 256, 0, ws->stream>>>(n2, w.bs, w.m0);
+  k_jacobi_eig<<<1, kJacThreads, 0, ws->stream>>>(n, w.m0, w.m1, w.x1, w.st);
+  PW_HIP(hipGetLastError());
+  int launched = 0;
+  OrthoState hs{};
+  OrthoState *hp = static_cast<OrthoState *>(ws->h_ortho_state);
+  while (launched < kMaxIter) {
+    const int nit = std::min(launched == 0 ? 8 : 4, kMaxIter - launched);
+    PW_TRY(enqueue_iters_eig(ws, w, lbuf, d_chichi, d_psipsi, d_chipsi, launched + 1, launched + nit));
+    launched += nit;
+    PW_HIP(hipMemcpyAsync(hp, w.st, sizeof(OrthoState), hipMemcpyDeviceToHost, ws->stream));
+    PW_HIP(hipStreamSynchronize(ws->stream));
+    hs = *hp;
+    if (hs.done) break;
+  }
+  if (hs.code == 1) return pw_fail(PW_ERR_NOCONV, "WAVES_ORTHO_X: NaN in overlap");
+  if (hs.code == 2)
+    return pw_fail(PW_ERR_NOCONV, "WAVES_ORTHO_X: iteration loop for orthogonalization is about to diverge");
+  if (!hs.done)
+    return pw_fail(PW_ERR_NOCONV,
+                   "WAVES_ORTHO_X: loop for orthogonalization is not converged (specify SAFEORTHO=F, "
+                   "reduce the cutoff or the time step)");
+  if ((hs.iter - 1) & 1) {
+    k_copy_d<<<nblk, 256, 0, ws->stream>>>(n2, w.lam2, d_lam);
+    PW_HIP(hipGetLastError());
+  }
+  ws->ortho_levels = 0;
+  ws->ortho_series = 0;
+  ws->ortho_ns = 0;
+  *iter = hs.iter;
+  return PW_OK;
+}
+
 }  // namespace
 
 void ortho_graph_release(pw_waveset *ws) {
@@ -1046,8 +1259,9 @@ int ortho_x_device(pw_waveset *ws, double *d_lam, const double *d_chichi, const 
     P.lo = ws->lz_lo;
     P.hi = ws->lz_hi;
     if (!(P.lo > 0.0) || !(P.hi >= P.lo)) {
+      // sym(CHIPSI) is not positive definite: the eigenvector form of S^-1
       ws->lz_valid = false;
-      return pw_fail(PW_ERR_NOCONV, "WAVES_ORTHO_X: sym(CHIPSI) is not positive definite");
+      return ortho_x_eig(ws, w, lbuf, d_lam, d_chichi, d_psipsi, d_chipsi, iter);
     }
     P.lo *= 0.97;
     P.hi *= 1.03;
@@ -1067,7 +1281,10 @@ int ortho_x_device(pw_waveset *ws, double *d_lam, const double *d_chichi, const 
     }
     P.nlev += attempt;
     P.nser = series_fits(n) ? std::min(P.nlev, kSeriesMaxLevels) : 0;
-    if (P.nlev > kMaxLevels) return pw_fail(PW_ERR_NOCONV, "WAVES_ORTHO_X: CHIPSI too ill-conditioned");
+    if (P.nlev > kMaxLevels) {
+      ws->lz_valid = false;
+      return ortho_x_eig(ws, w, lbuf, d_lam, d_chichi, d_psipsi, d_chipsi, iter);
+    }
     // Newton-Schulz for (B+qI)^-1: |I - (B+qI)X_k| = e0^(2^k); stop at 1e-17
     // (the acceptance test asks for 1e-11)
     const double kp = (P.hi + P.q) / (P.lo + P.q);
@@ -1133,8 +1350,10 @@ int ortho_x_device(pw_waveset *ws, double *d_lam, const double *d_chichi, const 
     }
     if (hs.code == 3) {
       if (attempt == 0) continue;
+      // bounds rejected twice: B is indefinite or too ill-conditioned for
+      // the series; the eigenvector form takes any symmetric B
       ws->lz_valid = false;
-      return pw_fail(PW_ERR_NOCONV, "WAVES_ORTHO_X: spectral bounds of sym(CHIPSI) not accepted");
+      return ortho_x_eig(ws, w, lbuf, d_lam, d_chichi, d_psipsi, d_chipsi, iter);
     }
     if (hs.code == 1) return pw_fail(PW_ERR_NOCONV, "WAVES_ORTHO_X: NaN in overlap");
     if (hs.code == 2)

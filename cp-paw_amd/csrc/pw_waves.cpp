@@ -777,6 +777,7 @@ extern "C" int pw_waves_geti4(pw_waveset *ws, const char *id, int *val) {
   else if (s == "COLWAVES") *val = ws->tune_col_waves;
   else if (s == "DENSBLOCKS") *val = ws->tune_dens_blocks;
   else if (s == "ORTHOLANCZOS") *val = ws->ortho_lanczos;
+  else if (s == "ORTHOEIGEN") *val = ws->ortho_eigen;
   else if (s == "ORTHOGRAPHMISSES") *val = ws->ortho_graph_misses;
   // depth of the multiplier history as WAVES_WRITELAMBDA counts it
   // (src/paw_waves2.f90:3948-3954): 0 none, 1 RLAM0, 2 RLAM0 and RLAMM

@@ -145,7 +145,9 @@ int pw_waves_getr8(pw_waveset *ws, const char *id, double *val);
  * "LMNXX", "ORTHOLEVELS" (squared-Smith levels of the last solve), "ORTHONS"
  * (Newton-Schulz steps of the last solve), "ORTHOLANCZOS" (Lanczos runs for
  * the spectral bounds in the last solve: 0 when the previous bounds were
- * accepted), "ORTHOGRAPHMISSES" (solves issued without a captured hipGraph:
+ * accepted), "ORTHOEIGEN" (solves so far through the eigenvectors of
+ * sym(CHIPSI), taken when it is not positive definite), "ORTHOGRAPHMISSES"
+ * (solves issued without a captured hipGraph:
  * captures are made only while one plan and one waveset exist in the
  * process), "NLAMBDA" (multiplier history held: 0 none, 1 RLAM0, 2 RLAM0
  * and RLAMM; counted as in WAVES_WRITELAMBDA, src/paw_waves2.f90:3948),
@@ -191,7 +193,8 @@ int pw_waves_density(pw_waveset *ws, const void *psi_dev, double *rho_dev);
 
 /* WAVES_ORTHO_X (src/paw_waves2.f90:1844) on host matrices (NB,NB):
  * lambda in/out; returns the iteration count in *iter.  Runs on the GPU in
- * FP64. */
+ * FP64; any symmetric sym(CHIPSI) with e_i + e_j != 0 (a positive definite
+ * one by a Cayley series of GEMMs, others through its eigenvectors). */
 int pw_ortho_x(int nb, const double *occ, const double *chichi,
                const double *psipsi, const double *chipsi, double *lambda,
                int *iter);

@@ -169,6 +169,24 @@ def test_ortho_x_standalone():
     assert relerr(lam, o.rlam0) < 1e-11
 
 
+@pytest.mark.parametrize("nb", [12, 37, 130])
+def test_ortho_x_indefinite_chipsi(nb):
+    """sym(CHIPSI) with negative eigenvalues: S^-1 through the eigenvectors of
+    B (src/paw_waves2.f90:1889, 1956-1991) reaches the oracle's Lambda in the
+    same number of iterations; nb = 37 has an odd round-robin (one idle index
+    per rotation round)."""
+    import pwgpu
+    from test_oracle_waves import indefinite_ortho_case
+    from oracle import waves as owv
+    say(f"case ortho_x indefinite nb={nb}")
+    occ, chichi, psipsi, chipsi = indefinite_ortho_case(nb=nb)
+    lam0 = np.zeros_like(chichi)
+    ref, it_ref = owv.ortho_x(occ, chichi, psipsi, chipsi, lam0.copy())
+    lam, it = pwgpu.ortho_x(occ, chichi, psipsi, chipsi, lam0)
+    assert it == it_ref
+    assert relerr(lam, ref) < 1e-10
+
+
 def moving_positions(s, seed=12, speed=0.03):
     """R(n) = R0 + n v: seeded ion velocities, so that R(-) != R(0) != R(+)."""
     v = np.random.default_rng(seed).normal(0.0, speed, s.pos.shape)

@@ -402,3 +402,38 @@ def test_propagate_friction_damped_oscillator():
             assert np.max(np.abs(traj[-1] - xc)) < tol * np.max(np.abs(x0))
         # the product of the roots is the per-step damping (1-a)/(1+a)
         assert abs(z1 * z2 - (1 - annee) / (1 + annee)) < 1e-14
+
+
+def indefinite_ortho_case(nb=12, nempty=0, seed=21):
+    """ORTHO_X inputs whose sym(CHIPSI) has negative eigenvalues (none of the
+    pair sums e_i + e_j near zero): the Cayley series cannot apply S^-1, the
+    reference's eigenvector form can."""
+    rng = np.random.default_rng(seed)
+    q, _ = np.linalg.qr(rng.normal(size=(nb, nb)))
+    e = np.concatenate([rng.uniform(-0.3, -0.2, 4), rng.uniform(0.8, 1.2, nb - 4)])
+    b = (q * e) @ q.T
+    anti = 0.01 * rng.normal(size=(nb, nb))
+    chipsi = b + (anti - anti.T)
+    m = rng.normal(size=(nb, nb))
+    chichi = 0.02 * (m @ m.T) / nb
+    d = 0.01 * rng.normal(size=(nb, nb))
+    psipsi = np.eye(nb) + 0.5 * (d + d.T)
+    occ = np.concatenate([np.full(nb - nempty, 2.0), np.zeros(nempty)])
+    return occ, chichi, psipsi, chipsi
+
+
+def test_ortho_x_indefinite_chipsi():
+    """With an indefinite sym(CHIPSI) the iteration still reaches the
+    constraint: psi(+) = psibar + chi Lambda is orthonormal, i.e.
+    PSIPSI + Lambda^T CHIPSI + CHIPSI^T Lambda + Lambda^T CHICHI Lambda = 1
+    (symmetrised, src/paw_waves2.f90:1905-1930), and Lambda keeps the
+    occupation relation Lambda_ij f_j = Lambda_ji f_i."""
+    occ, chichi, psipsi, chipsi = indefinite_ortho_case()
+    assert np.min(np.linalg.eigvalsh(0.5 * (chipsi + chipsi.T))) < -0.1
+    lam, it = owv.ortho_x(occ, chichi, psipsi, chipsi, np.zeros_like(chichi))
+    assert 2 <= it < 50
+    g = psipsi + lam.T @ chipsi + chipsi.T @ lam + lam.T @ chichi @ lam
+    g = 0.5 * (g + g.T) - np.eye(len(occ))
+    assert np.max(np.abs(g)) < 1e-8
+    oi = occ + owv.DSMALL
+    assert np.max(np.abs(lam * oi[None, :] - lam.T * oi[:, None])) < 1e-10 * np.max(np.abs(lam))
