@@ -17,10 +17,10 @@ void pw_set_error(const std::string &msg);
 int pw_fail(int code, const std::string &msg);
 
 // Stream capture (the hipGraph of ORTHO_X) runs only while a single plan and a
-// single waveset exist in the process: a device-synchronising call made by
-// another thread during a capture invalidates it and leaves the stream
-// unusable.  Plans and wavesets are counted under the same lock that a capture
-// holds, before their first allocation and after their last free.
+// single waveset exist in the process: HIP calls that other threads make
+// during a capture (another request's frees among them) invalidate it.  Plans
+// and wavesets are counted under the same lock that a capture holds, before
+// their first allocation and after their last free.
 struct CaptureGate {
   std::mutex mu;
   int plans = 0, wavesets = 0;
@@ -229,7 +229,11 @@ struct pw_waveset {
   // block has NB*NB more doubles that stage the DEDPROJ constraint matrix
   double *d_res = nullptr, *h_res = nullptr;
   size_t res_n = 0;
-  size_t r_ekin = 0, r_vrho = 0, r_wk = 0, r_expect = 0, r_denmat = 0, r_force = 0;
+  size_t r_ekin = 0, r_vrho = 0, r_wk = 0, r_expect = 0, r_denmat = 0, r_force = 0, r_resid = 0;
+  // residual of the last ORTHOGONALIZE (max over super bands of the norm of
+  // the constrained force), evaluated when residual_on
+  bool residual_on = false;
+  double residual = 0.0;
   int tune_dens_blocks = 512;   // workgroups of the density yz pass
   std::vector<double> force;   // 3*nat
   std::vector<double> eig;     // nb
@@ -299,6 +303,8 @@ struct ScopeTimer {
   ~ScopeTimer();
 };
 void ws_flush_timers(pw_waveset *ws);
+// a waveset as created, keeping its buffers, stream, graph and launch shapes
+int ws_reset_run(pw_waveset *ws);
 
 // real spherical harmonics at one direction, LM(L,M) = 1+L^2+L-M, L <= lx <= 6
 // (SPHERICAL$YLM; pw_waves.cpp)
@@ -322,6 +328,8 @@ int k_projections(pw_waveset *ws, const double2 *psi, const double2 *phase, doub
 int k_addpro(pw_waveset *ws, double2 *psi, const double2 *phase, const double2 *proj);
 // reductions: partial sums into device memory dst (read back once per section)
 int k_force(pw_waveset *ws, const double2 *psi, const double2 *phase, const double2 *dedproj, double *dst);
+// max over super bands of |HPSI - (1+a) m/dt^2 (psi(+) - psibar)| into dst (device)
+int k_residual(pw_waveset *ws, const double2 *psibar, double *dst);
 int k_overlap(pw_waveset *ws, const double2 *psi1, const double2 *psi2, double *mat_host_or_dev, bool to_host, bool tid);
 int k_overlap_1c(pw_waveset *ws, const double2 *proj1, const double2 *proj2, double *mat_dev);
 int k_addopsi(pw_waveset *ws, double2 *psibar, const double2 *chi, const double *lam_dev);

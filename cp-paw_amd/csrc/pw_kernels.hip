@@ -1557,6 +1557,48 @@ __global__ void __launch_bounds__(kB) k_dot(size_t n, const double *a, const dou
 }
 
 // grid (NGL/256, NBH): G along x, the band along y
+// Residual of a step: the constrained force on the wave functions,
+//   F = HPSI - (1+a) m(G)/dt^2 (psi(+) - psibar) = HPSI - O psi(0) Lambda,
+// from the Verlet update psibar (src/paw_waves2.f90:129-252) and the
+// constraint term that ORTHOGONALIZE adds to it; F vanishes at the fixed
+// point of the damped dynamics, psi(+) = psi(0) = psi(-).  part[b][blk] =
+// sum over the block's G of |F(b,G)|^2 for super band b (the squares of its
+// two real bands' residuals added).
+__global__ void __launch_bounds__(kB) k_resid_part(int ngl, double cs, const double *mpsi, const double2 *hpsi,
+                                                   const double2 *psip, const double2 *psibar, double *part) {
+  __shared__ double red[kB / 64];
+  const int b = blockIdx.y;
+  double s = 0.0;
+  for (int ig = blockIdx.x * kB + threadIdx.x; ig < ngl; ig += gridDim.x * kB) {
+    const size_t i = (size_t)b * ngl + ig;
+    const double c = cs * mpsi[ig];
+    const double2 h = hpsi[i], pp = psip[i], pb = psibar[i];
+    const double fx = h.x - c * (pp.x - pb.x), fy = h.y - c * (pp.y - pb.y);
+    s += fx * fx + fy * fy;
+  }
+  s = block_sum<kB>(s, red);
+  if (threadIdx.x == 0) part[(size_t)b * gridDim.x + blockIdx.x] = s;
+}
+
+// out = max over super bands b of sqrt(scale sum_blk part[b][blk]) (one workgroup)
+__global__ void __launch_bounds__(kB) k_resid_max(int nbh, int nblk, double scale, const double *part, double *out) {
+  __shared__ double red[kB / 64];
+  double m = 0.0;
+  for (int b = threadIdx.x; b < nbh; b += kB) {
+    double s = 0.0;
+    for (int k = 0; k < nblk; ++k) s += part[(size_t)b * nblk + k];
+    m = fmax(m, sqrt(scale * s));
+  }
+  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_down(m, off, 64));
+  if (threadIdx.x % 64 == 0) red[threadIdx.x / 64] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double r = 0.0;
+    for (int k = 0; k < kB / 64; ++k) r = fmax(r, red[k]);
+    *out = r;
+  }
+}
+
 __global__ void k_propagate_g(int ngl, int nbh, double a1, double a2, double a3n,
                               const double *mpsi, const double2 *psi0, double2 *psim,
                               const double2 *hpsi) {
@@ -2101,6 +2143,19 @@ int k_propagate(pw_waveset *ws) {
   ScopeTimer tm(ws, "propagate");
   k_propagate_g<<<dim3((p->ngl + 255) / 256, ws->nbh), 256, 0, ws->stream>>>(p->ngl, ws->nbh, a1, a2, a3n, ws->d_mpsi,
                                                          ws->d_psi0, ws->d_psim, ws->d_hpsi);
+  PW_HIP(hipGetLastError());
+  return PW_OK;
+}
+
+int k_residual(pw_waveset *ws, const double2 *psibar, double *dst) {
+  pw_plan *p = ws->plan;
+  const int nblk = reduce_gblocks(p);
+  PW_TRY(ws_part(ws, (size_t)ws->nbh * nblk * sizeof(double)));
+  ScopeTimer tm(ws, "residual");
+  const double cs = (1.0 + ws->annee) / (ws->delt * ws->delt);
+  k_resid_part<<<dim3(nblk, ws->nbh), kB, 0, ws->stream>>>(p->ngl, cs, ws->d_mpsi, ws->d_hpsi, ws->d_psim, psibar,
+                                                           ws->d_part);
+  k_resid_max<<<1, kB, 0, ws->stream>>>(ws->nbh, nblk, p->vol, ws->d_part, dst);
   PW_HIP(hipGetLastError());
   return PW_OK;
 }

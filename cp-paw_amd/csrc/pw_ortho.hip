@@ -370,10 +370,45 @@ struct SeriesArgs {
   const double *P;             // K stacked n x n row-major matrices P_k
   const double *G;             // GAMN
   double *part;                // K stacked terms P_k G P_k^T
+  const double *bmax;          // block maxima of |GAMN| (k_gemm_pair<kPairGamma>), or null
+  int nbmax;
+  double tol;
 };
 
 __global__ void __launch_bounds__(64 * kPanelWaves) k_series_panel(SeriesArgs a, const OrthoState *st) {
   if (stopped(st)) return;
+  if (a.bmax) {
+    // the convergence test of this iteration, as k_series_sum applies it
+    // next: when GAMN is converged (or NaN, or diverging) the terms go unused
+    // and the loop ends there, so the products are skipped
+    __shared__ double wm[kPanelWaves];
+    __shared__ int wn[kPanelWaves];
+    double m = 0.0;
+    int nan = 0;
+    for (int b = threadIdx.x; b < a.nbmax; b += blockDim.x) {
+      const double x = a.bmax[b];
+      if (x != x) nan = 1;
+      else m = fmax(m, x);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      m = fmax(m, __shfl_xor(m, o));
+      nan |= __shfl_xor(nan, o);
+    }
+    if ((threadIdx.x & 63) == 0) {
+      wm[threadIdx.x >> 6] = m;
+      wn[threadIdx.x >> 6] = nan;
+    }
+    __syncthreads();
+    m = 0.0;
+    nan = 0;
+#pragma unroll
+    for (int w = 0; w < kPanelWaves; ++w) {
+      m = fmax(m, wm[w]);
+      nan |= wn[w];
+    }
+    if (nan || m < a.tol || m > 1.0e200) return;
+  }
   extern __shared__ double sm[];   // sA, sR: 16 x ldr each
   const int n = a.n, nt = (n + T16 - 1) / T16, ldr = panel_ldr(n);
   double *sA = sm, *sR = sm + T16 * ldr;
@@ -1075,6 +1110,9 @@ int enqueue_iters(pw_waveset *ws, const OrthoWork &w, const OrthoParams &P, doub
       sa.P = w.pstk;
       sa.G = w.gam;
       sa.part = w.spart;
+      sa.bmax = w.red;
+      sa.nbmax = nt * nt;
+      sa.tol = eps;
       series_panel(ws, sa, 1 << ls, w.st);
     } else {
       gemm(ws, w.gam, 0, w.pstk, 0, w.t1, 1.0, nullptr, 0.0, w.st);

@@ -10,6 +10,12 @@
 // resident in HBM from admission to completion.  Results do not depend on
 // what else runs: every kernel sums in a fixed order and no state is shared.
 // Launch shapes come from pw_waves_autotune, once per system shape.
+// Finished plans and wavesets stay allocated in a pool, per system shape, and
+// serve the next request of that shape (reset to their created state): once
+// each shape has been seen, a request makes no plan set-up and no device
+// allocation or free.  Pooled sets count against the memory budget and the
+// oldest are freed when a request does not fit otherwise.
+#include <algorithm>
 #include <array>
 #include <chrono>
 #include <cmath>
@@ -18,6 +24,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <string>
 #include <thread>
 #include <tuple>
 #include <vector>
@@ -28,12 +35,15 @@ namespace {
 
 struct Job {
   long long id = 0;
+  std::string key;                 // system shape (pool key)
+  pw_plan *plan = nullptr;         // a pooled plan and waveset of this shape, if one was free
+  pw_waveset *ws = nullptr;
   double rbas[9];
   double epw_psi = 0, epw_rho = 0;
   int nb = 0, nat = 0, nsp = 0, lnxx = 0, prec = 0, max_steps = 0, nconv = 0;
   std::vector<int> isp, lnx, lox;
   std::vector<double> pro, occ, dov, dh, vofr, r, psi;
-  double dt = 0, emass = 0, emasscg2 = 0, friction = 0, etol = 0;
+  double dt = 0, emass = 0, emasscg2 = 0, friction = 0, etol = 0, rtol = 0;
   long long footprint = 0;
   bool tune = true;
   // results
@@ -41,7 +51,7 @@ struct Job {
   int status = PW_OK;
   std::string err;
   int steps = 0, converged = 0;
-  double etot = 0, ekin = 0, epot = 0, e1c = 0;
+  double etot = 0, ekin = 0, epot = 0, e1c = 0, residual = 0;
   double t_submit = 0, t_start = 0, t_end = 0;
 };
 
@@ -56,6 +66,17 @@ std::map<std::tuple<int, int, int, int, int>, std::array<int, 3>> g_tuned;
 struct pw_server {
   long long budget = 0;
   int max_active = 1;
+  struct Idle {
+    std::string key;
+    pw_plan *plan;
+    pw_waveset *ws;
+    long long footprint;
+  };
+  std::deque<Idle> idle;           // finished sets, oldest first
+  long long idle_bytes = 0;
+  long long reused = 0, created = 0;
+  // per shape: footprint, NGL, NRL (computed once from a host-only plan)
+  std::map<std::string, std::array<long long, 3>> shapes;
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
   std::deque<std::shared_ptr<Job>> pending;
@@ -70,14 +91,21 @@ struct pw_server {
 
 namespace {
 
+// the job's CP loop on its plan and waveset (pooled ones reset, or new);
+// on success they are left in j->plan / j->ws for the pool, on failure freed
 int run_job(Job *j) {
-  pw_plan *plan = nullptr;
-  pw_waveset *ws = nullptr;
-  const double k0[3] = {0.0, 0.0, 0.0};
-  int rc = pw_plan_create(j->rbas, k0, 1, j->epw_psi, j->epw_rho, &plan);
-  if (rc == PW_OK)
-    rc = pw_waves_create(plan, j->nb, j->nat, j->nsp, j->isp.data(), j->lnx.data(), j->lnxx, j->lox.data(),
-                         j->prec, &ws);
+  pw_plan *plan = j->plan;
+  pw_waveset *ws = j->ws;
+  int rc = PW_OK;
+  if (ws) {
+    rc = ws_reset_run(ws);
+  } else {
+    const double k0[3] = {0.0, 0.0, 0.0};
+    rc = pw_plan_create(j->rbas, k0, 1, j->epw_psi, j->epw_rho, &plan);
+    if (rc == PW_OK)
+      rc = pw_waves_create(plan, j->nb, j->nat, j->nsp, j->isp.data(), j->lnx.data(), j->lnxx, j->lox.data(),
+                           j->prec, &ws);
+  }
   auto r8a = [&](const char *id, const std::vector<double> &v) {
     if (rc == PW_OK) rc = pw_waves_setr8a(ws, id, (int)v.size(), v.data());
   };
@@ -102,7 +130,9 @@ int run_job(Job *j) {
   c8a("PSI0", j->psi);
   c8a("PSIM", j->psi);
   if (rc == PW_OK && j->tune) rc = pw_waves_autotune(ws);
-  // damped CP steps until |dE| < etol on nconv consecutive steps
+  if (rc == PW_OK && j->rtol > 0.0) rc = pw_waves_setr8(ws, "RESIDUALCHECK", 1.0);
+  // damped CP steps until |dE| < etol (or, with rtol, the residual of the
+  // previous step's ORTHOGONALIZE < rtol) on nconv consecutive steps
   double eprev = 0.0;
   int calm = 0;
   for (int it = 0; rc == PW_OK && it < j->max_steps; ++it) {
@@ -112,8 +142,10 @@ int run_job(Job *j) {
     j->ekin = ws->ekin;
     j->epot = ws->epot;
     j->e1c = ws->e1c;
+    j->residual = ws->residual;
     j->steps = it + 1;
-    calm = (it > 0 && std::fabs(ws->etot - eprev) < j->etol) ? calm + 1 : 0;
+    const bool calm_now = j->rtol > 0.0 ? ws->residual < j->rtol : std::fabs(ws->etot - eprev) < j->etol;
+    calm = (it > 0 && calm_now) ? calm + 1 : 0;
     eprev = ws->etot;
     if (calm >= j->nconv) {
       j->converged = 1;
@@ -127,10 +159,22 @@ int run_job(Job *j) {
     char buf[512];
     pw_last_error(buf, sizeof(buf));
     j->err = buf;
+    if (ws) pw_waves_destroy(ws);
+    if (plan) pw_plan_destroy(plan);
+    ws = nullptr;
+    plan = nullptr;
   }
-  if (ws) pw_waves_destroy(ws);
-  if (plan) pw_plan_destroy(plan);
+  j->plan = plan;
+  j->ws = ws;
   return rc;
+}
+
+void destroy_idle(std::vector<pw_server::Idle> &v) {
+  for (auto &e : v) {
+    pw_waves_destroy(e.ws);
+    pw_plan_destroy(e.plan);
+  }
+  v.clear();
 }
 
 // first pending job that fits the free budget (FIFO among those that fit)
@@ -144,6 +188,7 @@ std::deque<std::shared_ptr<Job>>::iterator admissible(pw_server *s) {
 void worker(pw_server *s) {
   for (;;) {
     std::shared_ptr<Job> job;
+    std::vector<pw_server::Idle> evict;
     {
       std::unique_lock<std::mutex> lk(s->mu);
       s->cv_work.wait(lk, [&] { return (s->stop && s->pending.empty()) || admissible(s) != s->pending.end(); });
@@ -151,12 +196,31 @@ void worker(pw_server *s) {
       if (it == s->pending.end()) return;   // stopping and drained
       job = *it;
       s->pending.erase(it);
+      // a pooled set of the same shape, or room for a new one (oldest pooled
+      // sets freed first)
+      auto same = std::find_if(s->idle.begin(), s->idle.end(),
+                               [&](const pw_server::Idle &e) { return e.key == job->key; });
+      if (same != s->idle.end()) {
+        job->plan = same->plan;
+        job->ws = same->ws;
+        s->idle_bytes -= same->footprint;
+        s->idle.erase(same);
+        s->reused++;
+      } else {
+        while (!s->idle.empty() && s->used + s->idle_bytes + job->footprint > s->budget) {
+          evict.push_back(s->idle.front());
+          s->idle_bytes -= s->idle.front().footprint;
+          s->idle.pop_front();
+        }
+        s->created++;
+      }
       s->used += job->footprint;
       s->active++;
-      s->peak_used = std::max(s->peak_used, s->used);
+      s->peak_used = std::max(s->peak_used, s->used + s->idle_bytes);
       s->peak_active = std::max(s->peak_active, s->active);
       job->t_start = s->now();
     }
+    destroy_idle(evict);
     const int rc = run_job(job.get());
     {
       std::lock_guard<std::mutex> lk(s->mu);
@@ -165,6 +229,12 @@ void worker(pw_server *s) {
       job->done = true;
       s->used -= job->footprint;
       s->active--;
+      if (job->ws) {
+        s->idle.push_back(pw_server::Idle{job->key, job->plan, job->ws, job->footprint});
+        s->idle_bytes += job->footprint;
+        job->plan = nullptr;
+        job->ws = nullptr;
+      }
     }
     s->cv_done.notify_all();
     s->cv_work.notify_all();
@@ -319,6 +389,8 @@ extern "C" int pw_server_destroy(pw_server *s) {
   }
   s->cv_work.notify_all();
   for (auto &t : s->workers) t.join();
+  std::vector<pw_server::Idle> all(s->idle.begin(), s->idle.end());
+  destroy_idle(all);
   delete s;
   return PW_OK;
 }
@@ -328,16 +400,43 @@ extern "C" int pw_server_submit(pw_server *s, const pw_request *q, long long *id
   if (q->nb <= 0 || q->nat <= 0 || q->nsp <= 0 || q->lnxx <= 0 || !q->isp || !q->lnx || !q->lox || !q->pro ||
       !q->occ || !q->dov || !q->dh || !q->vofr || !q->r || !q->psi || q->nbarepro <= 0 || q->max_steps <= 0)
     return pw_fail(PW_ERR_ARG, "pw_server_submit: incomplete request");
-  // sizes from a host-only plan of the same cell (no device memory yet)
-  pw_plan *plan = nullptr;
-  const double k0[3] = {0.0, 0.0, 0.0};
-  PW_TRY(pw_plan_create(q->rbas, k0, 1, q->epw_psi, q->epw_rho, &plan));
+  // the system shape: cell, cutoffs, bands, atoms, species, projectors, form
+  std::string key;
+  auto put = [&](const void *p, size_t n) { key.append(static_cast<const char *>(p), n); };
+  put(q->rbas, 9 * sizeof(double));
+  put(&q->epw_psi, sizeof(double));
+  put(&q->epw_rho, sizeof(double));
+  const int dims[5] = {q->nb, q->nat, q->nsp, q->lnxx, q->prec};
+  put(dims, sizeof(dims));
+  put(q->isp, (size_t)q->nat * sizeof(int));
+  put(q->lnx, (size_t)q->nsp * sizeof(int));
+  put(q->lox, (size_t)q->lnxx * q->nsp * sizeof(int));
+  // footprint and sizes, once per shape from a host-only plan (no device memory)
+  std::array<long long, 3> shape{};
+  bool known = false;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->shapes.find(key);
+    if (it != s->shapes.end()) {
+      shape = it->second;
+      known = true;
+    }
+  }
+  if (!known) {
+    pw_plan *plan = nullptr;
+    const double k0[3] = {0.0, 0.0, 0.0};
+    PW_TRY(pw_plan_create(q->rbas, k0, 1, q->epw_psi, q->epw_rho, &plan));
+    long long f = 0;
+    const int rc = pw_waves_footprint(plan, q->nb, q->nat, q->nsp, q->isp, q->lnx, q->lnxx, q->lox, q->prec, &f);
+    shape = {f, plan->ngl, plan->nrl};
+    pw_plan_destroy(plan);
+    if (rc != PW_OK) return rc;
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->shapes[key] = shape;
+  }
   auto job = std::make_shared<Job>();
-  long long fp = 0;
-  int rc = pw_waves_footprint(plan, q->nb, q->nat, q->nsp, q->isp, q->lnx, q->lnxx, q->lox, q->prec, &fp);
-  const int ngl = plan->ngl, nrl = plan->nrl;
-  pw_plan_destroy(plan);
-  if (rc != PW_OK) return rc;
+  const long long fp = shape[0];
+  const int ngl = (int)shape[1], nrl = (int)shape[2];
   if (fp > s->budget) return pw_fail(PW_ERR_UNSUPPORTED, "pw_server_submit: request larger than the memory budget");
   int lmnxx = 1;
   for (int sp = 0; sp < q->nsp; ++sp) {
@@ -347,6 +446,7 @@ extern "C" int pw_server_submit(pw_server *s, const pw_request *q, long long *id
   }
   const int nbh = (q->nb + 1) / 2;
   Job &j = *job;
+  j.key = std::move(key);
   std::copy(q->rbas, q->rbas + 9, j.rbas);
   j.epw_psi = q->epw_psi;
   j.epw_rho = q->epw_rho;
@@ -372,6 +472,7 @@ extern "C" int pw_server_submit(pw_server *s, const pw_request *q, long long *id
   j.emasscg2 = q->emasscg2;
   j.friction = q->friction;
   j.etol = q->etol;
+  j.rtol = q->rtol;
   j.tune = q->autotune != 0;
   j.footprint = fp;
   {
@@ -409,6 +510,7 @@ extern "C" int pw_server_wait(pw_server *s, long long id, pw_result *res) {
   res->t_start = j->t_start;
   res->t_end = j->t_end;
   res->footprint = j->footprint;
+  res->residual = j->residual;
   if (j->status != PW_OK) return pw_fail(j->status, "request " + std::to_string(id) + ": " + j->err);
   return PW_OK;
 }
@@ -419,5 +521,14 @@ extern "C" int pw_server_stats(pw_server *s, long long *peak_bytes, int *peak_ac
   *peak_bytes = s->peak_used;
   *peak_active = s->peak_active;
   *pending = (int)s->pending.size();
+  return PW_OK;
+}
+
+extern "C" int pw_server_pool_stats(pw_server *s, long long *reused, long long *created, long long *idle_bytes) {
+  if (!s || !reused || !created || !idle_bytes) return pw_fail(PW_ERR_ARG, "pw_server_pool_stats: null argument");
+  std::lock_guard<std::mutex> lk(s->mu);
+  *reused = s->reused;
+  *created = s->created;
+  *idle_bytes = s->idle_bytes;
   return PW_OK;
 }

@@ -422,7 +422,8 @@ extern "C" int pw_waves_create(pw_plan *plan, int nb, int nat, int nsp, const in
     ws->r_expect = ws->r_wk + rb;
     ws->r_denmat = ws->r_expect + 2 * (size_t)ws->nbh * expect_chunks(ws);
     ws->r_force = ws->r_denmat + (size_t)nat * ws->lmnxx * ws->lmnxx;
-    ws->res_n = ws->r_force + 3 * (size_t)ws->npro;
+    ws->r_resid = ws->r_force + 3 * (size_t)ws->npro;
+    ws->res_n = ws->r_resid + 1;
     rc = dalloc((void **)&ws->d_res, ws->res_n * sizeof(double));
     if (rc == PW_OK && hipHostMalloc((void **)&ws->h_res, (ws->res_n + (size_t)nb * nb) * sizeof(double)) != hipSuccess)
       rc = pw_fail(PW_ERR_HIP, "pw_waves_create: pinned result buffer");
@@ -456,6 +457,67 @@ extern "C" int pw_waves_create(pw_plan *plan, int nb, int nat, int nsp, const in
   if (hipStreamCreateWithFlags(&ws->stream, hipStreamNonBlocking) != hipSuccess) return fail(pw_fail(PW_ERR_HIP, "hipStreamCreate failed"));
   ws->own_stream = true;
   *out = ws;
+  return PW_OK;
+}
+
+// A waveset as pw_waves_create left it, keeping its device buffers, stream,
+// captured graph and tuned launch shapes: the request loop hands a finished
+// set to the next request of the same shape.  The state of a run (settings,
+// positions, multipliers, results, flags, cached factors and runs, the linked
+// potential and one-centre set-up) comes from a default-constructed set, and
+// the device arrays a new set starts from zero are cleared again: the
+// multipliers RLAM0/RLAMM (the first constraint solve starts from RLAM0 and
+// the first DEDPROJ reads it), the NB x NB matrices and the projections.
+int ws_reset_run(pw_waveset *ws) {
+  ws_flush_timers(ws);
+  sphere_release(ws);
+  const size_t n2 = (size_t)ws->nb * ws->nb, jsz = (size_t)ws->npro * ws->nbh * sizeof(double2);
+  PW_HIP(hipMemsetAsync(ws->d_rlam0, 0, n2 * sizeof(double), ws->stream));
+  PW_HIP(hipMemsetAsync(ws->d_rlamm, 0, n2 * sizeof(double), ws->stream));
+  PW_HIP(hipMemsetAsync(ws->d_small_mats, 0, 6 * n2 * sizeof(double), ws->stream));
+  PW_HIP(hipMemsetAsync(ws->d_proj, 0, jsz, ws->stream));
+  PW_HIP(hipMemsetAsync(ws->d_oproj, 0, jsz, ws->stream));
+  PW_HIP(hipMemsetAsync(ws->d_xproj, 0, jsz, ws->stream));
+  PW_HIP(hipStreamSynchronize(ws->stream));
+  pw_waveset f;
+  // structure (pw_waves_create)
+  f.plan = ws->plan;
+  f.prec = ws->prec;
+  f.nb = ws->nb; f.nbh = ws->nbh; f.nat = ws->nat; f.nsp = ws->nsp; f.lnxx = ws->lnxx;
+  f.lmnxx = ws->lmnxx; f.lmx = ws->lmx; f.npro = ws->npro; f.nbarepro = ws->nbarepro;
+  f.isp = std::move(ws->isp); f.lnx = std::move(ws->lnx); f.lmnx = std::move(ws->lmnx); f.lox = std::move(ws->lox);
+  f.ipro0 = std::move(ws->ipro0); f.ibpro = std::move(ws->ibpro);
+  f.lmn_ln = std::move(ws->lmn_ln); f.lmn_lm = std::move(ws->lmn_lm); f.lmn_l = std::move(ws->lmn_l);
+  f.nphase = ws->nphase;
+  f.rlam0.assign((size_t)f.nb * f.nb, 0.0);
+  f.force.assign(3 * (size_t)f.nat, 0.0);
+  f.eig.assign(f.nb, 0.0);
+  f.denmat.assign((size_t)f.lmnxx * f.lmnxx * f.nat, 0.0);
+  f.res_n = ws->res_n;
+  f.r_ekin = ws->r_ekin; f.r_vrho = ws->r_vrho; f.r_wk = ws->r_wk; f.r_expect = ws->r_expect;
+  f.r_denmat = ws->r_denmat; f.r_force = ws->r_force; f.r_resid = ws->r_resid;
+  // device and pinned buffers (fixed and grown on demand)
+  f.d_psi0 = ws->d_psi0; f.d_psim = ws->d_psim; f.d_hpsi = ws->d_hpsi; f.d_opsi = ws->d_opsi; f.d_tmp = ws->d_tmp;
+  f.d_proj = ws->d_proj; f.d_oproj = ws->d_oproj; f.d_xproj = ws->d_xproj;
+  f.d_pro = ws->d_pro; f.d_ylm = ws->d_ylm; f.d_mpsi = ws->d_mpsi; f.d_v = ws->d_v; f.d_rho = ws->d_rho;
+  f.d_occ2 = ws->d_occ2; f.d_occ = ws->d_occ; f.d_rlam0 = ws->d_rlam0; f.d_rlamm = ws->d_rlamm;
+  f.d_fpm = ws->d_fpm; f.d_fab = ws->d_fab; f.d_domat = ws->d_domat; f.d_dhmat = ws->d_dhmat;
+  f.d_atom_isp = ws->d_atom_isp; f.d_ipro0 = ws->d_ipro0; f.d_ibpro = ws->d_ibpro; f.d_lmnx = ws->d_lmnx;
+  f.d_lmn_ln = ws->d_lmn_ln; f.d_lmn_lm = ws->d_lmn_lm; f.d_lmn_l = ws->d_lmn_l; f.d_ipro_tab = ws->d_ipro_tab;
+  f.d_phase = ws->d_phase; f.d_small_mats = ws->d_small_mats;
+  f.d_res = ws->d_res; f.h_res = ws->h_res;
+  f.d_pfac = ws->d_pfac; f.pfac_bytes = ws->pfac_bytes;
+  f.d_part = ws->d_part; f.part_bytes = ws->part_bytes;
+  f.d_split = ws->d_split; f.split_bytes = ws->split_bytes;
+  f.d_wruns = ws->d_wruns; f.wruns_bytes = ws->wruns_bytes;
+  f.d_small = ws->d_small; f.small_bytes = ws->small_bytes;
+  f.d_ortho = ws->d_ortho; f.h_ortho_state = ws->h_ortho_state; f.ortho_cap = ws->ortho_cap;
+  f.ortho_exec = ws->ortho_exec; f.ortho_key = ws->ortho_key;
+  f.stream = ws->stream; f.own_stream = ws->own_stream;
+  // launch shapes
+  f.tune_split_waves = ws->tune_split_waves; f.tune_col_waves = ws->tune_col_waves;
+  f.tune_dens_blocks = ws->tune_dens_blocks;
+  *ws = std::move(f);
   return PW_OK;
 }
 
@@ -501,6 +563,7 @@ extern "C" int pw_waves_setr8(pw_waveset *ws, const char *id, double val) {
   else if (s == "EMASS") ws->emass = val;
   else if (s == "EMASSCG2") ws->emasscg2 = val;
   else if (s == "FRICTION") ws->annee = val;
+  else if (s == "RESIDUALCHECK") ws->residual_on = val != 0.0;
   else return pw_fail(PW_ERR_ARG, "WAVES$SETR8: ID not recognised: " + s);
   return upload_mass(ws);
 }
@@ -755,6 +818,7 @@ extern "C" int pw_waves_getr8(pw_waveset *ws, const char *id, double *val) {
   else if (s == "WAVEEKIN1") *val = ws->wavekin1;
   else if (s == "WAVEEKIN2") *val = ws->wavekin2;
   else if (s == "EFICT") *val = ws->efict;
+  else if (s == "RESIDUAL") *val = ws->residual;
   else if (s == "EHARTREE") *val = ws->ehartree;
   else if (s == "EXC") *val = ws->exc;
   else if (s == "RHOB") *val = ws->rhob;
@@ -1058,8 +1122,14 @@ extern "C" int pw_waves_orthogonalize(pw_waveset *ws) {
   // psi(+) = psibar + chi Lambda ; proj(+) = proj + oproj Lambda
   {
     ScopeTimer sec(ws, "WAVES_ADDOPSI");
+    if (ws->residual_on) PW_TRY(k_copy(ws, ws->d_tmp, ws->d_psim, (size_t)ws->nbh * p->ngl));   // psibar
     PW_TRY(k_addopsi(ws, ws->d_psim, ws->d_opsi, d_mat(ws, 3)));
     PW_TRY(k_addoproj(ws, ws->d_proj, ws->d_oproj, d_mat(ws, 3)));
+  }
+  if (ws->residual_on) {
+    PW_TRY(k_residual(ws, ws->d_tmp, ws->d_res + ws->r_resid));
+    PW_HIP(hipMemcpyAsync(ws->h_res + ws->r_resid, ws->d_res + ws->r_resid, sizeof(double), hipMemcpyDeviceToHost,
+                          ws->stream));
   }
   // RLAM0 stays on the device; MAT, OMAT, OOMAT go to the host only when asked for
   k_copy_mat<<<(nb * nb + 255) / 256, 256, 0, ws->stream>>>(nb * nb, d_mat(ws, 3), ws->d_rlam0);
@@ -1071,6 +1141,7 @@ extern "C" int pw_waves_orthogonalize(pw_waveset *ws) {
                         hipMemcpyDeviceToHost, ws->stream));
   PW_HIP(hipStreamSynchronize(ws->stream));
   ws->wavekin2 = sum_of(ws->h_res + ws->r_wk, red_blocks(ws)) / (ws->delt * ws->delt) * p->vol;
+  if (ws->residual_on) ws->residual = ws->h_res[ws->r_resid];
   ws->efict = 0.5 * (ws->wavekin1 + ws->wavekin2);
   return PW_OK;
 }

@@ -29,7 +29,8 @@ class Request(ctypes.Structure):
                 ("vofr", _c_dbl_p), ("r", _c_dbl_p), ("psi", _c_dbl_p),
                 ("dt", ctypes.c_double), ("emass", ctypes.c_double), ("emasscg2", ctypes.c_double),
                 ("friction", ctypes.c_double), ("etol", ctypes.c_double),
-                ("nconv", ctypes.c_int), ("max_steps", ctypes.c_int), ("autotune", ctypes.c_int)]
+                ("nconv", ctypes.c_int), ("max_steps", ctypes.c_int), ("autotune", ctypes.c_int),
+                ("rtol", ctypes.c_double)]
 
 
 class Result(ctypes.Structure):
@@ -37,7 +38,7 @@ class Result(ctypes.Structure):
     _fields_ = [("status", ctypes.c_int), ("steps", ctypes.c_int), ("converged", ctypes.c_int),
                 ("etot", ctypes.c_double), ("ekin", ctypes.c_double), ("epot", ctypes.c_double),
                 ("e1c", ctypes.c_double), ("t_submit", ctypes.c_double), ("t_start", ctypes.c_double),
-                ("t_end", ctypes.c_double), ("footprint", ctypes.c_longlong)]
+                ("t_end", ctypes.c_double), ("footprint", ctypes.c_longlong), ("residual", ctypes.c_double)]
 
 # name -> argument types (all return int status)
 SIGNATURES = {
@@ -98,6 +99,8 @@ SIGNATURES = {
     "pw_server_submit": [_vp, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_longlong)],
     "pw_server_wait": [_vp, ctypes.c_longlong, ctypes.POINTER(Result)],
     "pw_server_stats": [_vp, ctypes.POINTER(ctypes.c_longlong), _c_int_p, _c_int_p],
+    "pw_server_pool_stats": [_vp, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong),
+                             ctypes.POINTER(ctypes.c_longlong)],
 }
 
 _lib = None

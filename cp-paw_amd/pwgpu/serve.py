@@ -53,7 +53,10 @@ class Server:
         self.handle = h
 
     def submit(self, system, prec=1, friction=0.1, etol=1.0e-5, nconv=3, max_steps=400, autotune=True,
-               arrays=None):
+               arrays=None, rtol=0.0):
+        """rtol > 0: converged when the residual (the largest super-band norm
+        of HPSI - O psi Lambda, Ha) stays below rtol for nconv steps; else
+        when |dE| < etol for nconv steps."""
         a = request_arrays(system) if arrays is None else arrays
         q = _lib.Request()
         q.rbas[:] = list(np.asarray(system.rbas, dtype=np.float64).T.ravel())
@@ -65,6 +68,7 @@ class Server:
         q.vofr, q.r, q.psi = dptr(a["vofr"]), dptr(a["r"]), dptr(a["psi"])
         q.dt, q.emass, q.emasscg2, q.friction = system.dt, system.emass, system.emasscg2, friction
         q.etol, q.nconv, q.max_steps, q.autotune = etol, nconv, max_steps, int(bool(autotune))
+        q.rtol = float(rtol)
         rid = ctypes.c_longlong()
         check(self.lib.pw_server_submit(self.handle, ctypes.byref(q), ctypes.byref(rid)))
         return rid.value
@@ -77,7 +81,10 @@ class Server:
     def stats(self):
         b, a, p = ctypes.c_longlong(), ctypes.c_int(), ctypes.c_int()
         check(self.lib.pw_server_stats(self.handle, ctypes.byref(b), ctypes.byref(a), ctypes.byref(p)))
-        return {"peak_bytes": b.value, "peak_active": a.value, "pending": p.value}
+        r, c, i = ctypes.c_longlong(), ctypes.c_longlong(), ctypes.c_longlong()
+        check(self.lib.pw_server_pool_stats(self.handle, ctypes.byref(r), ctypes.byref(c), ctypes.byref(i)))
+        return {"peak_bytes": b.value, "peak_active": a.value, "pending": p.value,
+                "pool_reused": r.value, "pool_created": c.value, "pool_idle_bytes": i.value}
 
     def close(self):
         if self.handle:

@@ -113,7 +113,9 @@ int pw_waves_create(pw_plan *plan, int nb, int nat, int nsp, const int *isp,
                     pw_waveset **ws);
 int pw_waves_destroy(pw_waveset *ws);
 /* WAVES$SETR8 (src/paw_waves1.f90:322): "TIMESTEP" (DELT), "EMASS",
- * "EMASSCG2", "FRICTION" (ANNEE). */
+ * "EMASSCG2", "FRICTION" (ANNEE); and "RESIDUALCHECK" (no counterpart in the
+ * reference): nonzero makes each ORTHOGONALIZE evaluate the residual that
+ * pw_waves_getr8("RESIDUAL") returns. */
 int pw_waves_setr8(pw_waveset *ws, const char *id, double val);
 /* Real arrays: "PRO" (NGL*NBAREPRO, GSET%PRO, already / cell volume),
  * "OCC" (NB), "DO" (LNXX*LNXX*NSP), "VOFR" (NRL, local potential),
@@ -136,6 +138,11 @@ int pw_waves_getc8a(pw_waveset *ws, const char *id, int len, double *val);
  * (LMNXX*LMNXX*NAT). */
 int pw_waves_getr8a(pw_waveset *ws, const char *id, int len, double *val);
 /* Scalars: "EKIN","EPOT","E1C","ETOT","WAVEEKIN1","WAVEEKIN2","EFICT".
+ * "RESIDUAL" (with RESIDUALCHECK set): of the last ORTHOGONALIZE, the largest
+ * norm over super bands of the constrained force HPSI - O PSI0 LAMBDA =
+ * HPSI - (1+ANNEE) m(G)/DELT^2 (PSI(+) - PSIBAR), which vanishes at the fixed
+ * point of the damped dynamics (a super band's norm is that of its two real
+ * bands' residuals together).
  * "ORTHOERR": the orthonormality test of WAVES$ORTHOGONALIZE's TTEST branch
  * (src/paw_waves2.f90:425-464) on PSI0 at R(0), max_ij |<psi_i|S|psi_j> -
  * delta_ij|, always evaluated with the FP64 kernels (for a mixed-form set it
@@ -147,9 +154,8 @@ int pw_waves_getr8(pw_waveset *ws, const char *id, double *val);
  * the spectral bounds in the last solve: 0 when the previous bounds were
  * accepted), "ORTHOEIGEN" (solves so far through the eigenvectors of
  * sym(CHIPSI), taken when it is not positive definite), "ORTHOGRAPHMISSES"
- * (solves issued without a captured hipGraph:
- * captures are made only while one plan and one waveset exist in the
- * process), "NLAMBDA" (multiplier history held: 0 none, 1 RLAM0, 2 RLAM0
+ * (solves issued without a captured hipGraph: captures are made only while
+ * one plan and one waveset exist in the process), "NLAMBDA" (multiplier history held: 0 none, 1 RLAM0, 2 RLAM0
  * and RLAMM; counted as in WAVES_WRITELAMBDA, src/paw_waves2.f90:3948),
  * and the launch shapes of pw_waves_autotune: "SPLITWAVES" (waves of a
  * G-split row-product launch), "DENSBLOCKS" (workgroups of the density pass),
@@ -311,12 +317,15 @@ typedef struct pw_request {
   double etol;
   int nconv, max_steps;
   int autotune;            /* 1: run pw_waves_autotune on admission */
+  double rtol;             /* > 0: converged when RESIDUAL < rtol (Ha) on
+                              nconv consecutive steps, instead of |dE| < etol */
 } pw_request;
 typedef struct pw_result {
   int status, steps, converged;
   double etot, ekin, epot, e1c;
   double t_submit, t_start, t_end;   /* seconds since pw_server_create */
   long long footprint;               /* bytes reserved while running */
+  double residual;                   /* RESIDUAL of the last step (rtol > 0) */
 } pw_result;
 int pw_server_create(long long mem_budget, int max_active, pw_server **srv);
 /* Waits for every submitted request, then stops the workers. */
@@ -324,8 +333,16 @@ int pw_server_destroy(pw_server *srv);
 int pw_server_submit(pw_server *srv, const pw_request *req, long long *id);
 /* Blocks until request id has finished; its record is then released. */
 int pw_server_wait(pw_server *srv, long long id, pw_result *res);
+/* peak_bytes: device memory reserved at the peak (running requests and
+ * pooled sets). */
 int pw_server_stats(pw_server *srv, long long *peak_bytes, int *peak_active,
                     int *pending);
+/* The pool of finished plans and wavesets (kept per system shape, reset and
+ * handed to the next request of that shape): requests served from the pool
+ * (reused), requests that built a new set (created), and the device memory
+ * the idle pooled sets reserve now. */
+int pw_server_pool_stats(pw_server *srv, long long *reused, long long *created,
+                         long long *idle_bytes);
 
 #ifdef __cplusplus
 }

@@ -187,6 +187,36 @@ def test_ortho_x_indefinite_chipsi(nb):
     assert relerr(lam, ref) < 1e-10
 
 
+@pytest.mark.parametrize("name,prec", [("si8", 0), ("h2o4", 0), ("si8", 1)])
+def test_residual_matches_oracle(name, prec):
+    """RESIDUAL after ORTHOGONALIZE: the GPU forms the constrained force from
+    the Verlet step, HPSI - (1+a) m/dt^2 (psi(+) - psibar); here it is formed
+    directly, HPSI - O psi(0) Lambda, with O psi(0) from ADDPRO(DO <p|psi0>)
+    and Lambda folded onto the super bands (src/paw_waves2.f90:547-615).  Both
+    routes reach the same vector; its largest super-band norm is compared."""
+    from oracle import waves as owv
+    s = SYSTEMS[name]()
+    say(f"case residual {name} prec={prec}")
+    o = oracle_waveset(s)
+    plane, w = gpu_waveset(s, prec=prec)
+    w.setr8("RESIDUALCHECK", 1.0)
+    for step in range(2):
+        o.etot(s.pos)
+        hpsi = o.hpsi.copy()
+        spsi = owv.addpro(o.plane, o.setup, o.pro, o.ylm, s.pos, o.psi0, owv.oproj(o.setup, o.proj))
+        o.propagate()
+        o.orthogonalize(s.pos, s.pos)
+        f = hpsi - owv.addopsi(o.plane, np.zeros_like(spsi), spsi, o.rlam0)
+        ref = np.sqrt(o.plane.vol * np.max(np.sum(np.abs(f) ** 2, axis=1)))
+        w.etot()
+        w.propagate()
+        w.orthogonalize()
+        got = w.getr8("RESIDUAL")
+        assert abs(got - ref) <= (1e-9 if prec == 0 else 5e-5) * ref, (step, got, ref)
+        o.switch()
+        w.switch()
+
+
 def moving_positions(s, seed=12, speed=0.03):
     """R(n) = R0 + n v: seeded ion velocities, so that R(-) != R(0) != R(+)."""
     v = np.random.default_rng(seed).normal(0.0, speed, s.pos.shape)
