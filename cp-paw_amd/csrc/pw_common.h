@@ -266,6 +266,8 @@ struct pw_waveset {
   bool pfac_valid = false;
   double2 *d_phase = nullptr;  // nat*3*(2*nmaxx+1)
   int nphase = 0;              // 2*nmaxx+1
+  std::vector<double> phase_pos;   // positions of the tables in d_phase
+  bool phase_valid = false;
   double *d_part = nullptr; size_t part_bytes = 0;   // split-K partials
   void *d_split = nullptr; size_t split_bytes = 0;   // mixed form: operands split into bf16 runs
   // mixed form: bf16 runs of wave-function arrays (rows, or rows at -G) kept
@@ -326,6 +328,8 @@ int k_pfac(pw_waveset *ws);
 int k_runs_scope(pw_waveset *ws, bool on);
 int k_projections(pw_waveset *ws, const double2 *psi, const double2 *phase, double2 *proj);
 int k_addpro(pw_waveset *ws, double2 *psi, const double2 *phase, const double2 *proj);
+// opsi = dt^2/((1+a) m(G)) (psi0 + |p> proj) in one pass
+int k_addpro_opsi(pw_waveset *ws, double2 *opsi, const double2 *psi0, const double2 *phase, const double2 *proj);
 // reductions: partial sums into device memory dst (read back once per section)
 int k_force(pw_waveset *ws, const double2 *psi, const double2 *phase, const double2 *dedproj, double *dst);
 // max over super bands of |HPSI - (1+a) m/dt^2 (psi(+) - psibar)| into dst (device)
@@ -343,7 +347,6 @@ int k_wavekinetic(pw_waveset *ws, const double2 *psi0, const double2 *psim, cons
 // EKIN, WAVEKINETIC (first half) and <psi|H|psi> of WAVES$ETOT in one pass
 int k_etot_sums(pw_waveset *ws, double *ekin, double *wk, double *expect);
 int k_propagate(pw_waveset *ws);
-int k_opsi_scale(pw_waveset *ws, double2 *opsi);
 int k_copy(pw_waveset *ws, double2 *dst, const double2 *src, size_t n);
 int k_denmat(pw_waveset *ws, const double2 *proj, double *dst);
 int k_dot_vrho(pw_waveset *ws, double *dst);

@@ -280,9 +280,27 @@ struct AddproB {
   }
 };
 
+// Destination of the column products: psi[b][G] = (src or psi)[b][G] + C,
+// then times num / m(G) when mpsi is set (OPSI: the copy of PSI0, ADDPRO and
+// the mass scaling of WAVES_OPSI in one pass, the same operations in order)
+struct ColOut {
+  double2 *psi;
+  const double2 *src;
+  const double *mpsi;
+  double num;
+  __device__ void store(size_t i, int ig, double cr, double ci) const {
+    const double2 s = src ? src[i] : psi[i];
+    double2 v = make_double2(s.x + cr, s.y + ci);
+    if (mpsi) {
+      const double f = num / mpsi[ig];
+      v = make_double2(f * v.x, f * v.y);
+    }
+    psi[i] = v;
+  }
+};
+
 template <class GA, class GB, int NRB, int NW>
-__global__ void __launch_bounds__(64 * NW) k_colgemm(GA ga, GB gb, int nk, int ngl, int nbh,
-                                                     double2 *__restrict__ psi) {
+__global__ void __launch_bounds__(64 * NW) k_colgemm(GA ga, GB gb, int nk, int ngl, int nbh, ColOut out) {
   constexpr int WG = 16 * NW, SP = padw(WG), NB16 = 16 * NRB, SX = padw(NB16);
   __shared__ double sBr[KC][SP], sBi[KC][SP], sAr[KC][SX], sAi[KC][SX];
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6, li = lane & 15, lk = lane >> 4;
@@ -325,10 +343,7 @@ __global__ void __launch_bounds__(64 * NW) k_colgemm(GA ga, GB gb, int nk, int n
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int b = b0 + 16 * c + lk + 4 * r;
-      if (b < nbh) {
-        double2 &o = psi[(size_t)b * ngl + ig];
-        o = make_double2(o.x + accr[c][r], o.y + acci[c][r]);
-      }
+      if (b < nbh) out.store((size_t)b * ngl + ig, ig, accr[c][r], acci[c][r]);
     }
   }
 }
@@ -1128,8 +1143,7 @@ struct SrcChiCols {          // [chi_k(G); conj(chi_k(-G))] along K = 2 NBH, row
 };
 
 template <class GA, class GB, int NRB, int NW>
-__global__ void __launch_bounds__(64 * NW) k_colgemm_bf(GA ga, GB gb, int nk, int ngl, int nbh,
-                                                        double2 *__restrict__ psi) {
+__global__ void __launch_bounds__(64 * NW) k_colgemm_bf(GA ga, GB gb, int nk, int ngl, int nbh, ColOut out) {
   constexpr int WG = 16 * NW, NB16 = 16 * NRB;
   __shared__ __bf16 sB[NPL * WG][KBP];
   __shared__ __bf16 sA[NPL * NB16][KBP];
@@ -1199,10 +1213,7 @@ __global__ void __launch_bounds__(64 * NW) k_colgemm_bf(GA ga, GB gb, int nk, in
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int b = b0 + 16 * c + 4 * lk + r;
-      if (b < nbh) {
-        double2 &o = psi[(size_t)b * ngl + ig];
-        o = make_double2(o.x + accr[c][r], o.y + acci[c][r]);
-      }
+      if (b < nbh) out.store((size_t)b * ngl + ig, ig, accr[c][r], acci[c][r]);
     }
   }
 }
@@ -1610,14 +1621,6 @@ __global__ void k_propagate_g(int ngl, int nbh, double a1, double a2, double a3n
   psim[idx] = make_double2(a1 * p0.x + a2 * pm.x + a3 * h.x, a1 * p0.y + a2 * pm.y + a3 * h.y);
 }
 
-__global__ void k_scale_mass(int ngl, int nbh, double num, const double *mpsi, double2 *x) {
-  const int ig = blockIdx.x * blockDim.x + threadIdx.x;
-  if (ig >= ngl) return;
-  const size_t idx = (size_t)blockIdx.y * ngl + ig;
-  const double f = num / mpsi[ig];
-  x[idx] = make_double2(f * x[idx].x, f * x[idx].y);
-}
-
 ProjGeo make_pgeo(pw_waveset *ws) {
   ProjGeo q;
   pw_plan *p = ws->plan;
@@ -1757,13 +1760,13 @@ int run_rowgemm(pw_waveset *ws, const Gen &gen, int nrow, const double2 *psi, do
 
 // psi[b][G] += sum_k A(b,k) B(k,G)
 template <int NW, class GA, class GB>
-int run_colgemm(pw_waveset *ws, const GA &ga, const GB &gb, int nk, double2 *psi, const char *name) {
+int run_colgemm(pw_waveset *ws, const GA &ga, const GB &gb, int nk, const ColOut &out, const char *name) {
   const int ngl = ws->plan->ngl, nbh = ws->nbh;
   int nblk, tb;
   band_tiles(nbh, &nblk, &tb);
   dim3 grid((ngl + 16 * NW - 1) / (16 * NW), tb);
   ScopeTimer tm(ws, name);
-#define PW_LAUNCH(NB) k_colgemm<GA, GB, NB, NW><<<grid, 64 * NW, 0, ws->stream>>>(ga, gb, nk, ngl, nbh, psi)
+#define PW_LAUNCH(NB) k_colgemm<GA, GB, NB, NW><<<grid, 64 * NW, 0, ws->stream>>>(ga, gb, nk, ngl, nbh, out)
   PW_NB_SWITCH(nblk, PW_LAUNCH)
 #undef PW_LAUNCH
   PW_HIP(hipGetLastError());
@@ -1807,13 +1810,13 @@ int run_rowgemm_bf(pw_waveset *ws, const Gen &gen, int nrow, const BandRuns &gb,
 }
 
 template <int NW, class GA, class GB>
-int run_colgemm_bf(pw_waveset *ws, const GA &ga, const GB &gb, int nk, double2 *psi, const char *name) {
+int run_colgemm_bf(pw_waveset *ws, const GA &ga, const GB &gb, int nk, const ColOut &out, const char *name) {
   const int ngl = ws->plan->ngl, nbh = ws->nbh;
   int nblk, tb;
   band_tiles(nbh, &nblk, &tb);
   dim3 grid((ngl + 16 * NW - 1) / (16 * NW), tb);
   ScopeTimer tm(ws, name);
-#define PW_LAUNCH(NB) k_colgemm_bf<GA, GB, NB, NW><<<grid, 64 * NW, 0, ws->stream>>>(ga, gb, nk, ngl, nbh, psi)
+#define PW_LAUNCH(NB) k_colgemm_bf<GA, GB, NB, NW><<<grid, 64 * NW, 0, ws->stream>>>(ga, gb, nk, ngl, nbh, out)
   PW_NB_SWITCH(nblk, PW_LAUNCH)
 #undef PW_LAUNCH
   PW_HIP(hipGetLastError());
@@ -1928,7 +1931,8 @@ int k_projections(pw_waveset *ws, const double2 *psi, const double2 *phase, doub
   return run_rowgemm<4>(ws, gen, ws->npro, psi, ws->plan->vol, proj, nullptr, "projections");
 }
 
-int k_addpro(pw_waveset *ws, double2 *psi, const double2 *phase, const double2 *x) {
+// psi = (src or psi) + sum_i |p_i> x_i, then times num / m(G) when scaled
+static int addpro_to(pw_waveset *ws, const ColOut &out, const double2 *phase, const double2 *x) {
   if (ws->prec == PW_PREC_MIXED) {
     PW_TRY(ws_split(ws, runs_bytes(ws->nbh, ws->npro)));
     ARuns ga;
@@ -1937,14 +1941,24 @@ int k_addpro(pw_waveset *ws, double2 *psi, const double2 *phase, const double2 *
       ga = ARuns{split_runs<false>(ws, SrcXRows{x, ws->nbh, ws->npro}, ws->nbh, ws->npro, 0)};
     }
     AddproBf gb{pfac_of(ws), ws->npro};
-    if (ws->tune_col_waves == 2) return run_colgemm_bf<2>(ws, ga, gb, ws->npro, psi, "addpro");
-    return run_colgemm_bf<4>(ws, ga, gb, ws->npro, psi, "addpro");
+    if (ws->tune_col_waves == 2) return run_colgemm_bf<2>(ws, ga, gb, ws->npro, out, "addpro");
+    return run_colgemm_bf<4>(ws, ga, gb, ws->npro, out, "addpro");
   }
   ProjGeo q = make_pgeo(ws);
   q.phase = phase;
   AddproA ga{x, ws->nbh};
   AddproB gb{q, ws->d_ipro_tab};
-  return run_colgemm<2>(ws, ga, gb, ws->npro, psi, "addpro");
+  return run_colgemm<2>(ws, ga, gb, ws->npro, out, "addpro");
+}
+
+int k_addpro(pw_waveset *ws, double2 *psi, const double2 *phase, const double2 *x) {
+  return addpro_to(ws, ColOut{psi, nullptr, nullptr, 0.0}, phase, x);
+}
+
+// WAVES_OPSI: opsi = dt^2/((1+a) m(G)) (psi0 + sum_i |p_i> x_i), x = DO <p|psi0>,
+// in the ADDPRO pass (no copy of PSI0, no separate scaling pass)
+int k_addpro_opsi(pw_waveset *ws, double2 *opsi, const double2 *psi0, const double2 *phase, const double2 *x) {
+  return addpro_to(ws, ColOut{opsi, psi0, ws->d_mpsi, ws->delt * ws->delt / (1.0 + ws->annee)}, phase, x);
 }
 
 int k_force(pw_waveset *ws, const double2 *psi, const double2 *phase, const double2 *ded, double *dst) {
@@ -2058,12 +2072,12 @@ int k_addopsi(pw_waveset *ws, double2 *psibar, const double2 *chi, const double 
       ga = ARuns{split_runs<false>(ws, SrcLamRows{lam, ws->nb, ws->nbh}, ws->nbh, nk, 0)};
       gb = ColRuns{split_runs<true>(ws, SrcChiCols{chi, p->d_minusg, p->ngl, ws->nbh}, p->ngl, nk, ba)};
     }
-    if (ws->tune_col_waves == 2) return run_colgemm_bf<2>(ws, ga, gb, nk, psibar, "addopsi");
-    return run_colgemm_bf<4>(ws, ga, gb, nk, psibar, "addopsi");
+    if (ws->tune_col_waves == 2) return run_colgemm_bf<2>(ws, ga, gb, nk, ColOut{psibar}, "addopsi");
+    return run_colgemm_bf<4>(ws, ga, gb, nk, ColOut{psibar}, "addopsi");
   }
   AddopsiA ga{lam, ws->nb, ws->nbh};
   AddopsiB gb{chi, p->d_minusg, p->ngl, ws->nbh};
-  return run_colgemm<2>(ws, ga, gb, 2 * ws->nbh, psibar, "addopsi");
+  return run_colgemm<2>(ws, ga, gb, 2 * ws->nbh, ColOut{psibar}, "addopsi");
 }
 
 int k_addoproj(pw_waveset *ws, double2 *proj, const double2 *oproj, const double *lam) {
@@ -2156,16 +2170,6 @@ int k_residual(pw_waveset *ws, const double2 *psibar, double *dst) {
   k_resid_part<<<dim3(nblk, ws->nbh), kB, 0, ws->stream>>>(p->ngl, cs, ws->d_mpsi, ws->d_hpsi, ws->d_psim, psibar,
                                                            ws->d_part);
   k_resid_max<<<1, kB, 0, ws->stream>>>(ws->nbh, nblk, p->vol, ws->d_part, dst);
-  PW_HIP(hipGetLastError());
-  return PW_OK;
-}
-
-int k_opsi_scale(pw_waveset *ws, double2 *opsi) {
-  pw_plan *p = ws->plan;
-  ScopeTimer tm(ws, "opsi_scale");
-  k_scale_mass<<<dim3((p->ngl + 255) / 256, ws->nbh), 256, 0, ws->stream>>>(p->ngl, ws->nbh,
-                                                        ws->delt * ws->delt / (1.0 + ws->annee),
-                                                        ws->d_mpsi, opsi);
   PW_HIP(hipGetLastError());
   return PW_OK;
 }

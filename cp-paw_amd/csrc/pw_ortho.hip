@@ -179,7 +179,7 @@ __host__ __device__ inline int panel_ldr(int n) { return n + 1 + ((1 - n) % 32 +
 constexpr size_t kLdsMax = 160 * 1024;
 inline size_t panel_lds(int n) { return (size_t)T16 * panel_ldr(n) * sizeof(double); }
 // the series needs two staged panels per workgroup
-inline bool series_fits(int n) { return 2 * panel_lds(n) <= kLdsMax; }
+inline bool series_fits(int n) { return 2 * panel_lds(n) + 1024 <= kLdsMax; }   // + the static LDS of the panel
 
 // Batched products with one shared left operand, by row panels:
 //   C_z = A B_z,  B_z = B + z sb, C_z = C + z sc,  z < nz,
@@ -943,15 +943,24 @@ int ortho_work(pw_waveset *ws, OrthoWork &w) {
     ws->ortho_cap = 0;
     PW_HIP(hipMalloc(&ws->d_ortho, need * sizeof(double)));
     ws->ortho_cap = need;
-    // LDS of the row-panel kernels (set outside any stream capture)
-    const size_t lds = (size_t)T16 * panel_ldr(ws->nb) * sizeof(double);
-    if (lds > 65536 && lds <= kLdsMax)
-      PW_HIP(hipFuncSetAttribute((const void *)k_panel_batch, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 (int)lds));
-    if (2 * lds > 65536 && 2 * lds <= kLdsMax)
-      PW_HIP(hipFuncSetAttribute((const void *)k_series_panel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 (int)(2 * lds)));
   }
+  // dynamic LDS limit of the panel kernels: the 160 KiB of a CU less the
+  // kernel's static LDS, once per process (the attribute is per kernel, not
+  // per waveset; set outside any stream capture)
+  static std::once_flag lds_once;
+  static hipError_t lds_err = hipSuccess;
+  std::call_once(lds_once, [] {
+    const void *fns[] = {(const void *)k_panel_batch, (const void *)k_series_panel};
+    for (const void *f : fns) {
+      hipFuncAttributes fa{};
+      hipError_t e = hipFuncGetAttributes(&fa, f);
+      if (e == hipSuccess)
+        e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)(kLdsMax - std::min(kLdsMax, (size_t)fa.sharedSizeBytes)));
+      if (e != hipSuccess) lds_err = e;
+    }
+  });
+  PW_HIP(lds_err);
   double *p = ws->d_ortho;
   w.bs = p; p += n2; w.m0 = p; p += n2; w.m1 = p; p += n2; w.gam = p; p += n2;
   w.x0 = p; p += n2; w.x1 = p; p += n2; w.t1 = p; p += n2; w.t2 = p; p += n2;

@@ -229,7 +229,13 @@ static int upload_mass(pw_waveset *ws) {
 }
 
 static int phase_at(pw_waveset *ws, const std::vector<double> &r) {
-  PW_TRY(k_phase_tables(ws, r.data(), ws->d_phase));
+  // the tables of these positions, unless they are already there (ETOT
+  // recomputes them at least once per step)
+  if (!ws->phase_valid || ws->phase_pos != r) {
+    PW_TRY(k_phase_tables(ws, r.data(), ws->d_phase));
+    ws->phase_pos = r;
+    ws->phase_valid = true;
+  }
   // the mixed form's products read the projector factors of these positions
   if (ws->prec == PW_PREC_MIXED && (!ws->pfac_valid || ws->pfac_pos != r)) {
     PW_TRY(k_pfac(ws));
@@ -791,7 +797,7 @@ static int ortho_error(pw_waveset *ws, double *out) {
   } restore{ws, ws->prec};
   ws->prec = PW_PREC_FP64;
   const int nb = ws->nb;
-  PW_TRY(k_phase_tables(ws, ws->r0.data(), ws->d_phase));
+  PW_TRY(phase_at(ws, ws->r0));
   PW_TRY(k_projections(ws, ws->d_psi0, ws->d_phase, ws->d_oproj));
   PW_TRY(k_overlap(ws, ws->d_psi0, ws->d_psi0, d_mat(ws, 4), false, true));
   PW_TRY(k_overlap_1c(ws, ws->d_oproj, ws->d_oproj, d_mat(ws, 4)));
@@ -915,7 +921,9 @@ extern "C" int pw_waves_etot(pw_waveset *ws) {
   if (!ws) return pw_fail(PW_ERR_ARG, "pw_waves_etot: null waveset");
   PW_TRY(ready(ws));
   pw_plan *p = ws->plan;
-  ws->pfac_valid = false;   // the projector factors are redone at least once per step
+  // the phase tables and projector factors are redone at least once per step
+  ws->phase_valid = false;
+  ws->pfac_valid = false;
   ScopeTimer whole(ws, "WAVES$ETOT");
   if (ws->first) {
     PW_TRY(pw_waves_gramschmidt(ws));
@@ -1093,11 +1101,9 @@ extern "C" int pw_waves_orthogonalize(pw_waveset *ws) {
   // OPSI = dt^2/((1+a) m(G)) (psi0 + |p>DO<p|psi0>) at R(0)
   {
     ScopeTimer sec(ws, "WAVES_OPSI");
-    PW_TRY(k_copy(ws, ws->d_opsi, ws->d_psi0, (size_t)ws->nbh * p->ngl));
     PW_TRY(phase_at(ws, ws->r0));
     PW_TRY(k_atom_matrix(ws, ws->d_proj, ws->d_xproj, ws->d_domat));
-    PW_TRY(k_addpro(ws, ws->d_opsi, ws->d_phase, ws->d_xproj));
-    PW_TRY(k_opsi_scale(ws, ws->d_opsi));
+    PW_TRY(k_addpro_opsi(ws, ws->d_opsi, ws->d_psi0, ws->d_phase, ws->d_xproj));
   }
   // projections at R(+); psibar and chi stay fixed until the overlaps are done
   {
