@@ -446,7 +446,7 @@ def main():
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                          "(the multi-GPU decomposition is a design in DESIGN.md only)"}))
+                          "(the decomposition over several GPUs is a design only)"}))
         sys.exit(2)
 
     import pwgpu

@@ -1,7 +1,7 @@
 """GPU parity of the HIP path against the FP64 oracle on identical seeded inputs.
 
 Tolerances: the plain FP64 GPU form must match within 1e-10 relative on
-energies, forces and the orthonormality residual (the BASELINE.json gate); the
+energies, forces and the orthonormality residual (the parity gate); the
 kernels themselves agree to ~1e-13.  FP32 transforms (mixed form) are checked
 at 2e-5 relative.
 """
