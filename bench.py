@@ -186,6 +186,111 @@ def accuracy(system_name, nsteps, relax_tol=1.0e-8, relax_max=3000, friction=0.1
     return res
 
 
+def fft_pair_us(system_name, prec, reps=10):
+    """PLANEWAVE$FFT of all super bands on device arrays: one GTOR and one RTOG
+    of NBH transforms (the full-grid FFT pair), microseconds per pair."""
+    import numpy as np
+    import pwgpu
+    from pwgpu import model
+    s = model.SYSTEMS[system_name]()
+    plane = pwgpu.PlaneWave(s.rbas, s.epw_psi, s.epw_rho)
+    nbh = (s.nb + 1) // 2
+    rng = np.random.default_rng(3)
+    g = pwgpu.DeviceArray.from_host(rng.normal(size=(nbh, plane.ngl)) + 1j * rng.normal(size=(nbh, plane.ngl)))
+    r = pwgpu.DeviceArray((nbh, plane.nrl), np.complex128 if prec == 0 else np.complex64)
+    lib = plane.lib
+    for _ in range(2):
+        plane.fft("GTOR", nbh, g, r, prec)
+        plane.fft("RTOG", nbh, g, r, prec)
+    lib.pw_device_synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        plane.fft("GTOR", nbh, g, r, prec)
+        plane.fft("RTOG", nbh, g, r, prec)
+    lib.pw_device_synchronize()
+    us = 1e6 * (time.perf_counter() - t0) / reps
+    g.free()
+    r.free()
+    plane.close()
+    return us
+
+
+def synthetic_components(steps=10, warmup=2):
+    """The kernel-level synthetic step (256 bands, 64^3, 512 projectors):
+    per-component times of both forms, microseconds per call."""
+    out = {}
+    for prec, form in ((0, "fp64"), (1, "mixed")):
+        r = run_gpu("synthetic", prec, steps, warmup, True)
+        kt = r["kernels"]
+        avg = lambda n: 1e3 * kt[n]["ms_avg"] if n in kt else None
+        per_step = lambda n: 1e3 * kt[n]["ms_total"] / steps if n in kt else 0.0
+        out[form] = {"steps_per_s": r["steps_per_s"], "ms_per_step": r["ms_per_step"],
+                     "fft_pair_us": fft_pair_us("synthetic", prec),
+                     "vpsi_us": avg("vpsi"), "density_us": avg("density"),
+                     "projections_us": avg("projections"), "overlap_us": avg("overlap"),
+                     "verlet_plus_ortho_us_per_step": per_step("propagate") + per_step("ortho_x")
+                     + per_step("addopsi") + per_step("addoproj"),
+                     "desc": r["desc"]}
+    return out
+
+
+def energy_drift(system_name, nsteps=500, relax_max=400, friction=0.1):
+    """Energy conservation of the undamped CP run in both forms: the FP64 form
+    relaxes the seeded start with friction (relax_max steps), then both forms
+    run nsteps steps without friction from that state.  The conserved quantity
+    is ETOT + EFICT (fixed ions); its drift is the slope of a straight-line fit,
+    in Ha per ps and per atom.  Beside it the per-step error of the mixed form
+    handed the FP64 state each step (as in accuracy())."""
+    import numpy as np
+    import pwgpu
+    from pwgpu import model
+    s = model.SYSTEMS[system_name]()
+    plane = pwgpu.PlaneWave(s.rbas, s.epw_psi, s.epw_rho)
+    w = pwgpu.Waves.from_system(plane, s, prec=0)
+    w.setr8("FRICTION", friction)
+    for _ in range(relax_max):
+        w.step()
+    w.setr8("FRICTION", 0.0)
+    forms = {}
+    wmx = pwgpu.Waves.from_system(plane, s, prec=1)
+    wmx.etot()
+    de = df = 0.0
+    for prec, form in ((0, "fp64"), (1, "mixed")):
+        x = pwgpu.Waves.from_system(plane, s, prec=prec)
+        x.etot()
+        _copy_state(w, x)
+        x.projections()
+        x.setr8("FRICTION", 0.0)
+        econs = []
+        t0 = time.perf_counter()
+        for it in range(nsteps):
+            e = x.etot()["etot"]
+            if prec == 0:
+                f0 = x.getr8a("FORCE")
+                _copy_state(x, wmx)
+                wmx.projections()
+                e1 = wmx.etot()["etot"]
+                de = max(de, abs(e1 - e) / s.nat)
+                df = max(df, float(np.max(np.abs(wmx.getr8a("FORCE") - f0))))
+            x.propagate()
+            x.orthogonalize()
+            econs.append(e + x.getr8("EFICT"))
+            x.switch()
+        secs = time.perf_counter() - t0
+        t_ps = np.arange(nsteps) * s.dt * 2.4188843265857e-5     # atomic time unit in ps
+        slope = float(np.polyfit(t_ps, np.array(econs), 1)[0])
+        forms[form] = {"drift_Ha_per_ps": slope, "drift_Ha_per_ps_per_atom": slope / s.nat,
+                       "max_dev_Ha": float(np.max(np.abs(np.array(econs) - econs[0]))),
+                       "wall_s": secs}
+        x.close()
+    wmx.close()
+    w.close()
+    plane.close()
+    return {"steps": nsteps, "dt_au": s.dt, "relax_steps": relax_max, "forms": forms,
+            "per_step_max_dE_per_atom_Ha": de, "per_step_max_dF_Ha_per_bohr": df,
+            "conserved": "ETOT + EFICT (ions fixed), undamped, from a damped FP64 relaxation"}
+
+
 def serving(nreq, rate, slots, seed=7, reference=True):
     """Poisson stream of independent relaxations (Si8/16/32/64, seeded
     displacements) through the request loop in the mixed form: req/s,
@@ -428,6 +533,9 @@ def main():
     ap.add_argument("--serve-p99-bound", type=float, default=2.0,
                     help="p99 latency (s) within which a swept arrival rate counts as sustained")
     ap.add_argument("--profile", action="store_true", help="print per-kernel times to stderr")
+    ap.add_argument("--other-configs", type=int, default=1,
+                    help="1: also measure the synthetic step's components and the 32 H2O run (throughput, "
+                         "500-step energy drift, per-step error)")
     ap.add_argument("--pot-calls", type=int, default=20,
                     help="calls of the density -> potential measurement (0: skip)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -516,6 +624,20 @@ def main():
         potm = potential_measure(args.system, args.pot_calls, max(5, min(20, args.steps)), not args.no_cpu)
         log(f"vofrho: {potm['vofrho_ms']:.3f} ms/call; self-consistent mixed step "
             f"{potm['scf_steps_per_s']:.1f} steps/s")
+    other = None
+    if args.other_configs:
+        log("synthetic step (256 bands, 64^3, 512 projectors): components ...")
+        syn = synthetic_components()
+        log("  " + ", ".join(f"{f}: {v['ms_per_step']:.2f} ms/step, FFT pair {v['fft_pair_us']:.0f} us"
+                             for f, v in syn.items()))
+        log("32 H2O: throughput and 500-step energy drift ...")
+        h2o = {f: run_gpu("h2o32", p, 20, 3, False) for p, f in ((0, "fp64"), (1, "mixed"))}
+        drift = energy_drift("h2o32")
+        log(f"  fp64 {h2o['fp64']['steps_per_s']:.1f}, mixed {h2o['mixed']['steps_per_s']:.1f} steps/s; drift "
+            + ", ".join(f"{f} {v['drift_Ha_per_ps']:.2e} Ha/ps" for f, v in drift["forms"].items()))
+        other = {"synthetic": syn,
+                 "h2o32": {"steps_per_s": {f: v["steps_per_s"] for f, v in h2o.items()},
+                           "desc": h2o["mixed"]["desc"], "energy_drift": drift}}
     cpu = None
     if not args.no_cpu:
         log("cpu baseline (oracle) ...")
@@ -547,6 +669,8 @@ def main():
         out["serving"] = srv
     if potm is not None:
         out["potential"] = potm
+    if other is not None:
+        out["other_configs"] = other
     if 0 in results and 1 in results:
         out["fp64_steps_per_s"] = results[0]["steps_per_s"]
         out["mixed_over_fp64"] = results[1]["steps_per_s"] / results[0]["steps_per_s"]
