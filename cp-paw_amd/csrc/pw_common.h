@@ -335,7 +335,9 @@ int k_force(pw_waveset *ws, const double2 *psi, const double2 *phase, const doub
 // max over super bands of |HPSI - (1+a) m/dt^2 (psi(+) - psibar)| into dst (device)
 int k_residual(pw_waveset *ws, const double2 *psibar, double *dst);
 int k_overlap(pw_waveset *ws, const double2 *psi1, const double2 *psi2, double *mat_host_or_dev, bool to_host, bool tid);
-int k_overlap_1c(pw_waveset *ws, const double2 *proj1, const double2 *proj2, double *mat_dev);
+// mat_dev = WAVES_OVERLAP(psi1, psi2) + WAVES_1COVERLAP(proj1, proj2) in one sequence
+int k_overlap_full(pw_waveset *ws, const double2 *psi1, const double2 *psi2, const double2 *proj1,
+                   const double2 *proj2, double *mat_dev);
 int k_addopsi(pw_waveset *ws, double2 *psibar, const double2 *chi, const double *lam_dev);
 int k_addoproj(pw_waveset *ws, double2 *proj, const double2 *oproj, const double *lam_dev);
 int k_atom_matrix(pw_waveset *ws, const double2 *proj, double2 *out, const double2 *mats);

@@ -459,7 +459,7 @@ __global__ void k_unravel_pw(int nbh, int nb, const double2 *t, const double2 *t
 // one-centre overlap (WAVES_1COVERLAP, src/paw_waves2.f90:856-1022), added to mat
 // c1 = sum_k conj(p1_ki) p2a_kj, c2 = sum_k p1_ki p2a_kj.  The projector index
 // k is split over blockIdx.z in runs of k1c rows; part[z][i][j] = (c1, c2)
-// of the run, summed in a fixed order by k_overlap1c_sum.
+// of the run, summed in z order by k_overlap_finish.
 // ---------------------------------------------------------------------------
 constexpr int k1c = 64;
 __global__ void __launch_bounds__(256) k_overlap1c(int npro, int nbh, const double2 *p1, const double2 *p2a,
@@ -489,25 +489,73 @@ __global__ void __launch_bounds__(256) k_overlap1c(int npro, int nbh, const doub
   part[((size_t)blockIdx.z * nbh + i) * nbh + j] = make_double4(c1.x, c1.y, c2.x, c2.y);
 }
 
-__global__ void k_overlap1c_sum(int nbh, int nb, int nz, const double4 *part, double *mat) {
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= nbh * nbh) return;
-  const int i = idx / nbh, j = idx % nbh;
-  double4 c = make_double4(0.0, 0.0, 0.0, 0.0);
-  for (int z = 0; z < nz; ++z) {
-    const double4 v = part[(size_t)z * nbh * nbh + idx];
-    c.x += v.x;
-    c.y += v.y;
-    c.z += v.z;
-    c.w += v.w;
+// WAVES_OVERLAP's matrix in one pass from the G-split partials of the
+// stacked product (rows 0..nbhp-1: T, rows nbhp..: TM): the sums of
+// k_reduce_split, the unravel of k_unravel_pw and, with part1c, the
+// one-centre sum over the runs of k_overlap1c in z order.
+__global__ void __launch_bounds__(kRedC * kRedS) k_overlap_finish(const double2 *__restrict__ part, int ns, size_t n,
+                                                                  double scale, int nbh, int nbhp, int nb,
+                                                                  const double4 *__restrict__ part1c, int nz,
+                                                                  double *mat) {
+  __shared__ double2 red[2][kRedS][kRedC];
+  const int c = threadIdx.x % kRedC, sl = threadIdx.x / kRedC;
+  const int idx = blockIdx.x * kRedC + c;
+  const bool in = idx < nbh * nbh;
+  const int i = in ? idx / nbh : 0, j = in ? idx - i * nbh : 0;
+  const size_t it = (size_t)i * nbh + j, itm = (size_t)(nbhp + i) * nbh + j;
+  double2 at = rz(double2()), am = rz(double2());
+  if (in) {
+    constexpr int V = 16;
+    for (int k0 = sl; k0 < ns; k0 += V * kRedS) {
+      double2 vt[V], vm[V];
+#pragma unroll
+      for (int u = 0; u < V; ++u) {
+        const int k = k0 + u * kRedS;
+        vt[u] = k < ns ? part[(size_t)k * n + it] : rz(double2());
+        vm[u] = k < ns ? part[(size_t)k * n + itm] : rz(double2());
+      }
+#pragma unroll
+      for (int u = 0; u < V; ++u) {
+        at = radd(at, vt[u]);
+        am = radd(am, vm[u]);
+      }
+    }
   }
-  const double2 s1 = make_double2(0.5 * (c.x + c.z), 0.5 * (c.y + c.w));
-  const double2 s2 = make_double2(0.5 * (c.x - c.z), 0.5 * (c.y - c.w));
+  red[0][sl][c] = at;
+  red[1][sl][c] = am;
+  __syncthreads();
+  if (sl != 0 || !in) return;
+  double2 t = red[0][0][c], tm = red[1][0][c];
+#pragma unroll
+  for (int q = 1; q < kRedS; ++q) {
+    t = radd(t, red[0][q][c]);
+    tm = radd(tm, red[1][q][c]);
+  }
+  t = rscale(t, scale);
+  tm = rscale(tm, scale);
+  const double re = 0.5 * t.x, im = 0.5 * t.y, rem = 0.5 * tm.x, imm = 0.5 * tm.y;
+  double m00 = re + rem, m01 = im + imm, m10 = -im + imm, m11 = re - rem;
+  if (part1c) {
+    double4 cc = make_double4(0.0, 0.0, 0.0, 0.0);
+    for (int z = 0; z < nz; ++z) {
+      const double4 v = part1c[(size_t)z * nbh * nbh + idx];
+      cc.x += v.x;
+      cc.y += v.y;
+      cc.z += v.z;
+      cc.w += v.w;
+    }
+    const double2 s1 = make_double2(0.5 * (cc.x + cc.z), 0.5 * (cc.y + cc.w));
+    const double2 s2 = make_double2(0.5 * (cc.x - cc.z), 0.5 * (cc.y - cc.w));
+    m00 += s1.x;
+    m01 += s1.y;
+    m10 += -s2.y;
+    m11 += s2.x;
+  }
   const int ia = 2 * i, ja = 2 * j;
-  mat[(size_t)ia * nb + ja] += s1.x;
-  mat[(size_t)ia * nb + ja + 1] += s1.y;
-  mat[(size_t)(ia + 1) * nb + ja] += -s2.y;
-  mat[(size_t)(ia + 1) * nb + ja + 1] += s2.x;
+  mat[(size_t)ia * nb + ja] = m00;
+  mat[(size_t)ia * nb + ja + 1] = m01;
+  mat[(size_t)(ia + 1) * nb + ja] = m10;
+  mat[(size_t)(ia + 1) * nb + ja + 1] = m11;
 }
 
 // per-atom matrix on projections: out[ipro0+i][b] = sum_j M_atom[i][j] proj[ipro0+j][b]
@@ -1724,9 +1772,17 @@ int wave_runs(pw_waveset *ws, const double2 *psi, bool minus, size_t off, SplitR
 // out[r][b] = scale sum_G A(r,G) psi[b][G]: split over G (about two waves per
 // SIMD), deterministic reduction of the splits.  out == nullptr puts the
 // result behind the partials and returns it in *res.
+// G-split partials left for the caller's own reduction (defer != nullptr)
+struct RowPart {
+  const double2 *part = nullptr;
+  int ns = 0;
+  size_t n = 0;
+  double scale = 1.0;
+};
+
 template <int NW, class Gen>
 int run_rowgemm(pw_waveset *ws, const Gen &gen, int nrow, const double2 *psi, double scale,
-                double2 *out, double2 **res, const char *name) {
+                double2 *out, double2 **res, const char *name, RowPart *defer = nullptr) {
   const int ngl = ws->plan->ngl, nbh = ws->nbh;
   int nblk, tb;
   band_tiles(nbh, &nblk, &tb);
@@ -1747,6 +1803,10 @@ int run_rowgemm(pw_waveset *ws, const Gen &gen, int nrow, const double2 *psi, do
     PW_NB_SWITCH(nblk, PW_LAUNCH)
 #undef PW_LAUNCH
     PW_HIP(hipGetLastError());
+  }
+  if (defer) {
+    *defer = RowPart{part, ns, n, scale};
+    return PW_OK;
   }
   {
     ScopeTimer tm(ws, "reduce");
@@ -1776,7 +1836,7 @@ int run_colgemm(pw_waveset *ws, const GA &ga, const GB &gb, int nk, const ColOut
 // split-BF16x3 counterpart of run_rowgemm (G splits are whole 32-chunks)
 template <int NW, class Gen>
 int run_rowgemm_bf(pw_waveset *ws, const Gen &gen, int nrow, const BandRuns &gb, double scale,
-                   double2 *out, double2 **res, const char *name) {
+                   double2 *out, double2 **res, const char *name, RowPart *defer = nullptr) {
   const int ngl = ws->plan->ngl, nbh = ws->nbh;
   int nblk, tb;
   band_tiles(nbh, &nblk, &tb);
@@ -1798,6 +1858,10 @@ int run_rowgemm_bf(pw_waveset *ws, const Gen &gen, int nrow, const BandRuns &gb,
     PW_NB_SWITCH(nblk, PW_LAUNCH)
 #undef PW_LAUNCH
     PW_HIP(hipGetLastError());
+  }
+  if (defer) {
+    *defer = RowPart{part, ns, n, scale};
+    return PW_OK;
   }
   {
     ScopeTimer tm(ws, "reduce");
@@ -2035,19 +2099,47 @@ int k_overlap(pw_waveset *ws, const double2 *psi1, const double2 *psi2, double *
   return PW_OK;
 }
 
-// mat_dev += WAVES_1COVERLAP(proj1, proj2); uses d_xproj as DO*proj2 scratch
-int k_overlap_1c(pw_waveset *ws, const double2 *proj1, const double2 *proj2, double *mat) {
+// mat_dev = WAVES_OVERLAP(psi1, psi2) + WAVES_1COVERLAP(proj1, proj2): the
+// one-centre partials first (DO*proj2 in d_xproj, the partials in the small
+// scratch), then the G-split product with its reduction deferred, then one
+// k_overlap_finish pass (four launches)
+int k_overlap_full(pw_waveset *ws, const double2 *psi1, const double2 *psi2, const double2 *proj1,
+                   const double2 *proj2, double *mat) {
+  pw_plan *p = ws->plan;
   ProjGeo q = make_pgeo(ws);
-  const int nbh = ws->nbh, nz = (ws->npro + k1c - 1) / k1c;
-  // the overlap matrix read from d_part (k_unravel_pw) is ahead on the stream
-  PW_TRY(ws_part(ws, (size_t)nz * nbh * nbh * sizeof(double4)));
-  ScopeTimer tm(ws, "overlap1c");
-  k_atom_mat<<<dim3(ws->nat, (ws->lmnxx * nbh + 63) / 64), 64, 0, ws->stream>>>(q, proj2, ws->d_domat, ws->d_xproj);
-  dim3 grid((nbh + 15) / 16, (nbh + 15) / 16, nz);
-  k_overlap1c<<<grid, 256, 0, ws->stream>>>(ws->npro, nbh, proj1, ws->d_xproj, (double4 *)ws->d_part);
-  k_overlap1c_sum<<<(nbh * nbh + 255) / 256, 256, 0, ws->stream>>>(nbh, ws->nb, nz, (const double4 *)ws->d_part,
-                                                                    mat);
-  PW_HIP(hipGetLastError());
+  const int nbh = ws->nbh, nbhp = (nbh + 15) / 16 * 16, nz = (ws->npro + k1c - 1) / k1c;
+  PW_TRY(ws_small(ws, (size_t)nz * nbh * nbh * sizeof(double4)));
+  double4 *p1c = (double4 *)ws->d_small;
+  {
+    ScopeTimer tm(ws, "overlap1c");
+    k_atom_mat<<<dim3(ws->nat, (ws->lmnxx * nbh + 63) / 64), 64, 0, ws->stream>>>(q, proj2, ws->d_domat,
+                                                                                   ws->d_xproj);
+    dim3 grid((nbh + 15) / 16, (nbh + 15) / 16, nz);
+    k_overlap1c<<<grid, 256, 0, ws->stream>>>(ws->npro, nbh, proj1, ws->d_xproj, p1c);
+    PW_HIP(hipGetLastError());
+  }
+  RowPart rp;
+  if (ws->prec == PW_PREC_MIXED) {
+    const size_t bytes = runs_bytes(nbh, p->ngl);
+    PW_TRY(ws_split(ws, 3 * bytes));   // outside a runs scope all three go to the scratch
+    OvlRuns gen;
+    gen.nbhp = nbhp;
+    BandRuns gb;
+    PW_TRY(wave_runs(ws, psi1, false, 0, &gen.rows));
+    PW_TRY(wave_runs(ws, psi1, true, bytes, &gen.minus));
+    if (psi2 == psi1) gb.s = gen.rows;
+    else PW_TRY(wave_runs(ws, psi2, false, 2 * bytes, &gb.s));
+    PW_TRY(run_rowgemm_bf<2>(ws, gen, 2 * nbhp, gb, p->vol, nullptr, nullptr, "overlap", &rp));
+  } else {
+    GenOvl gen{psi1, p->d_minusg, p->ngl, nbh, nbhp};
+    PW_TRY(run_rowgemm<2>(ws, gen, 2 * nbhp, psi2, p->vol, nullptr, nullptr, "overlap", &rp));
+  }
+  {
+    ScopeTimer tm(ws, "reduce");
+    k_overlap_finish<<<(nbh * nbh + kRedC - 1) / kRedC, kRedC * kRedS, 0, ws->stream>>>(
+        rp.part, rp.ns, rp.n, rp.scale, nbh, nbhp, ws->nb, p1c, nz, mat);
+    PW_HIP(hipGetLastError());
+  }
   return PW_OK;
 }
 

@@ -21,7 +21,8 @@
 //   P_k G P_k, then the ordered sum over k with the convergence test and the
 //   LAMBDA update).  Only terms beyond 2^5 would continue by the Smith
 //   recursion X <- X + A_j X A_j, A_j = A^(2^j); no tested cell needs them.
-// Every product is an NB^3 FP64-MFMA GEMM; nothing is diagonalised.  The
+// On the series path every product is an NB^3 FP64-MFMA GEMM and nothing is
+// diagonalised.  The
 // outer loop runs on the device: the convergence test sets a flag that makes
 // every later kernel of the batch return at once, so the host waits only
 // once per batch of iterations (batch = the previous call's count).

@@ -280,9 +280,8 @@ static int gramschmidt_one(pw_waveset *ws, double2 *psi, const std::vector<doubl
   PW_TRY(phase_at(ws, r));
   PW_TRY(k_runs_scope(ws, true));
   PW_TRY(k_projections(ws, psi, ws->d_phase, ws->d_oproj));
-  PW_TRY(k_overlap(ws, psi, psi, d_mat(ws, 4), false, true));
+  PW_TRY(k_overlap_full(ws, psi, psi, ws->d_oproj, ws->d_oproj, d_mat(ws, 4)));
   PW_TRY(k_runs_scope(ws, false));
-  PW_TRY(k_overlap_1c(ws, ws->d_oproj, ws->d_oproj, d_mat(ws, 4)));
   std::vector<double> s((size_t)nb * nb);
   PW_HIP(hipMemcpyAsync(s.data(), d_mat(ws, 4), s.size() * sizeof(double), hipMemcpyDeviceToHost, ws->stream));
   PW_HIP(hipStreamSynchronize(ws->stream));
@@ -799,8 +798,7 @@ static int ortho_error(pw_waveset *ws, double *out) {
   const int nb = ws->nb;
   PW_TRY(phase_at(ws, ws->r0));
   PW_TRY(k_projections(ws, ws->d_psi0, ws->d_phase, ws->d_oproj));
-  PW_TRY(k_overlap(ws, ws->d_psi0, ws->d_psi0, d_mat(ws, 4), false, true));
-  PW_TRY(k_overlap_1c(ws, ws->d_oproj, ws->d_oproj, d_mat(ws, 4)));
+  PW_TRY(k_overlap_full(ws, ws->d_psi0, ws->d_psi0, ws->d_oproj, ws->d_oproj, d_mat(ws, 4)));
   std::vector<double> s((size_t)nb * nb);
   PW_HIP(hipMemcpyAsync(s.data(), d_mat(ws, 4), s.size() * sizeof(double), hipMemcpyDeviceToHost, ws->stream));
   PW_HIP(hipStreamSynchronize(ws->stream));
@@ -1113,13 +1111,10 @@ extern "C" int pw_waves_orthogonalize(pw_waveset *ws) {
     PW_TRY(k_projections(ws, ws->d_psim, ws->d_phase, ws->d_proj));
     PW_TRY(k_projections(ws, ws->d_opsi, ws->d_phase, ws->d_oproj));
     // MAT, OMAT, OOMAT
-    PW_TRY(k_overlap(ws, ws->d_psim, ws->d_psim, d_mat(ws, 0), false, true));
-    PW_TRY(k_overlap_1c(ws, ws->d_proj, ws->d_proj, d_mat(ws, 0)));
-    PW_TRY(k_overlap(ws, ws->d_opsi, ws->d_psim, d_mat(ws, 1), false, false));
-    PW_TRY(k_overlap_1c(ws, ws->d_oproj, ws->d_proj, d_mat(ws, 1)));
-    PW_TRY(k_overlap(ws, ws->d_opsi, ws->d_opsi, d_mat(ws, 2), false, true));
+    PW_TRY(k_overlap_full(ws, ws->d_psim, ws->d_psim, ws->d_proj, ws->d_proj, d_mat(ws, 0)));
+    PW_TRY(k_overlap_full(ws, ws->d_opsi, ws->d_psim, ws->d_oproj, ws->d_proj, d_mat(ws, 1)));
+    PW_TRY(k_overlap_full(ws, ws->d_opsi, ws->d_opsi, ws->d_oproj, ws->d_oproj, d_mat(ws, 2)));
     PW_TRY(k_runs_scope(ws, false));
-    PW_TRY(k_overlap_1c(ws, ws->d_oproj, ws->d_oproj, d_mat(ws, 2)));
   }
   // initial LAMBDA: hermitised RLAM0 times 0.5 f_i/(f_i+f_j) (src/paw_waves2.f90:295-359)
   k_lam_guess<<<(nb * nb + 255) / 256, 256, 0, ws->stream>>>(nb, ws->d_occ, ws->d_rlam0, d_mat(ws, 3));
