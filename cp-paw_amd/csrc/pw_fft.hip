@@ -306,17 +306,31 @@ template <typename T2> struct XTile;
 template <> struct XTile<double2> { static constexpr int n = 16; };
 template <> struct XTile<float2> { static constexpr int n = 32; };
 
+// Workgroup -> (stripe group, band) of the x passes, a 1D grid of
+// 8 ceil(nb/8) ngroups workgroups: consecutive workgroups go to the eight XCDs
+// in turn, and each XCD takes every stripe group of its bands (band = xcd +
+// 8 k), so the coefficients of a band, gathered and scattered in G order by
+// all its stripe groups, pass through one XCD's L2 instead of eight.
+__device__ inline bool x_block(int ngroups, int nb, int &group, int &band) {
+  const int id = blockIdx.x, x = id % 8, j = id / 8;
+  band = x + 8 * (j / ngroups);
+  group = j % ngroups;
+  return band < nb;
+}
+inline dim3 x_grid(int ngroups, int nb) { return dim3((unsigned)(8 * ((nb + 7) / 8) * ngroups)); }
+
 // ---- x pass, forward: sphere -> stripes [band][x][stripe] --------------------
 template <typename T2, int NX = 0>
 __global__ void __launch_bounds__(kBlock) k_xfwd(Geo g, Fft1D f, const T2 *tw, const double2 *psi,
-                                                 T2 *stripes) {
+                                                 T2 *stripes, int nb) {
   constexpr int PX = XTile<T2>::n;
   if constexpr (NX > 0) g.nr1 = NX;
+  int grp, band;
+  if (!x_block((g.nstripe + PX - 1) / PX, nb, grp, band)) return;
   extern __shared__ unsigned char smem_raw[];
   T2 *b0 = reinterpret_cast<T2 *>(smem_raw);
   T2 *b1 = b0 + PX * g.nr1;
-  const int s0 = blockIdx.x * PX;
-  const int band = blockIdx.y;
+  const int s0 = grp * PX;
   const int ns = min(PX, g.nstripe - s0);
   for (int i = threadIdx.x; i < PX * g.nr1; i += blockDim.x) b0[i] = czero<T2>();
   __syncthreads();
@@ -339,14 +353,15 @@ __global__ void __launch_bounds__(kBlock) k_xfwd(Geo g, Fft1D f, const T2 *tw, c
 template <typename T2, int NX = 0>
 __global__ void __launch_bounds__(kBlock) k_xinv(Geo g, Fft1D f, const T2 *tw, const T2 *stripes,
                                                  double2 *out, const double2 *psi, double scale,
-                                                 int add_kin) {
+                                                 int add_kin, int nb) {
   constexpr int PX = XTile<T2>::n;
   if constexpr (NX > 0) g.nr1 = NX;
+  int grp, band;
+  if (!x_block((g.nstripe + PX - 1) / PX, nb, grp, band)) return;
   extern __shared__ unsigned char smem_raw[];
   T2 *b0 = reinterpret_cast<T2 *>(smem_raw);
   T2 *b1 = b0 + PX * g.nr1;
-  const int s0 = blockIdx.x * PX;
-  const int band = blockIdx.y;
+  const int s0 = grp * PX;
   const int ns = min(PX, g.nstripe - s0);
   const T2 *src = stripes + (size_t)band * g.nr1 * g.nstripe + s0;
   for (int i = threadIdx.x; i < PX * g.nr1; i += blockDim.x) {
@@ -944,10 +959,10 @@ template <typename T2>
 int run_forward_x(pw_plan *p, const Geo &g, int nb, const double2 *psi, hipStream_t s) {
   T2 *stripes = (T2 *)p->d_stripe_buf;
   constexpr int PX = XTile<T2>::n;
-  dim3 gx((p->nstripe + PX - 1) / PX, nb);
+  const dim3 gx = x_grid((p->nstripe + PX - 1) / PX, nb);
   const size_t lds = 2 * PX * p->nr1 * sizeof(T2);
   PW_TRY(with_x(p, [&](auto nx) -> int {
-    k_xfwd<T2, decltype(nx)::value><<<gx, kBlock, lds, s>>>(g, p->f[0], twid<T2>(p, 0), psi, stripes);
+    k_xfwd<T2, decltype(nx)::value><<<gx, kBlock, lds, s>>>(g, p->f[0], twid<T2>(p, 0), psi, stripes, nb);
     return PW_OK;
   }));
   PW_HIP(hipGetLastError());
@@ -959,11 +974,11 @@ int run_inverse_x(pw_plan *p, const Geo &g, int nb, double2 *out, const double2 
                   int add_kin, hipStream_t s) {
   T2 *stripes = (T2 *)p->d_stripe_buf;
   constexpr int PX = XTile<T2>::n;
-  dim3 gx((p->nstripe + PX - 1) / PX, nb);
+  const dim3 gx = x_grid((p->nstripe + PX - 1) / PX, nb);
   const size_t lds = 2 * PX * p->nr1 * sizeof(T2);
   PW_TRY(with_x(p, [&](auto nx) -> int {
     k_xinv<T2, decltype(nx)::value><<<gx, kBlock, lds, s>>>(g, p->f[0], twid<T2>(p, 0), stripes, out, psi, scale,
-                                                            add_kin);
+                                                            add_kin, nb);
     return PW_OK;
   }));
   PW_HIP(hipGetLastError());
@@ -975,9 +990,9 @@ int run_forward_xy(pw_plan *p, const Geo &g, int nb, const double2 *psi, T2 *str
                    hipStream_t s) {
   const int ntx = (p->nr1 + kTX - 1) / kTX;
   constexpr int PX = XTile<T2>::n;
-  dim3 gx((p->nstripe + PX - 1) / PX, nb);
+  const dim3 gx = x_grid((p->nstripe + PX - 1) / PX, nb);
   k_xfwd<T2><<<gx, kBlock, 2 * PX * p->nr1 * sizeof(T2), s>>>(g, p->f[0], twid<T2>(p, 0), psi,
-                                                              stripes);
+                                                              stripes, nb);
   dim3 gy(p->nzp * ntx, nb);
   k_yfwd<T2><<<gy, kBlock, 2 * kTX * p->nr2 * sizeof(T2), s>>>(g, p->f[1], twid<T2>(p, 1),
                                                                stripes, planes);
@@ -993,9 +1008,9 @@ int run_inverse_yx(pw_plan *p, const Geo &g, int nb, double2 *out, T2 *stripes, 
   k_yinv<T2><<<gy, kBlock, 2 * kTX * p->nr2 * sizeof(T2), s>>>(g, p->f[1], twid<T2>(p, 1),
                                                                planes, stripes);
   constexpr int PX = XTile<T2>::n;
-  dim3 gx((p->nstripe + PX - 1) / PX, nb);
+  const dim3 gx = x_grid((p->nstripe + PX - 1) / PX, nb);
   k_xinv<T2><<<gx, kBlock, 2 * PX * p->nr1 * sizeof(T2), s>>>(g, p->f[0], twid<T2>(p, 0),
-                                                              stripes, out, nullptr, 1.0 / (double)p->nrl, 0);
+                                                              stripes, out, nullptr, 1.0 / (double)p->nrl, 0, nb);
   PW_HIP(hipGetLastError());
   return PW_OK;
 }
