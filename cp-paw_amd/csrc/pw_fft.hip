@@ -151,6 +151,22 @@ __device__ inline void dft(T2 *v, int sign) {
   if constexpr (R == 2) {
     T2 a = v[0], b = v[1];
     v[0] = cadd(a, b); v[1] = csub(a, b);
+  } else if constexpr (R == 3) {
+    // m = v0 - t/2, out(1,2) = m +- sign sin(2 pi/3) i d: 12 operations
+    const T c = (T)-0.5, s = (T)0.86602540378443864676;
+    const T2 t = cadd(v[1], v[2]), d = csub(v[1], v[2]);
+    T2 m;
+    m.x = v[0].x + c * t.x;
+    m.y = v[0].y + c * t.y;
+    v[0] = cadd(v[0], t);
+    const T sx = sign > 0 ? s : -s;
+    T2 o1, o2;
+    o1.x = m.x - sx * d.y;
+    o1.y = m.y + sx * d.x;
+    o2.x = m.x + sx * d.y;
+    o2.y = m.y - sx * d.x;
+    v[1] = o1;
+    v[2] = o2;
   } else if constexpr (R == 4) {
     T2 t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
     T2 t2 = cadd(v[1], v[3]), t3 = mul_si(csub(v[1], v[3]), sign);
@@ -182,21 +198,32 @@ __device__ inline void dft(T2 *v, int sign) {
       sp[r - 1] = cadd(v[r], v[R - r]);
       sm[r - 1] = mul_si(csub(v[r], v[R - r]), sign);
     }
+    // out[q], out[R-q] = a_q +- b_q with a_q = v0 + sum_r cos(2 pi r q/R) sp_r
+    // and b_q = sum_r sin(2 pi r q/R) sm_r (cos even, sin odd in q)
     T2 out[R];
     out[0] = v[0];
 #pragma unroll
     for (int r = 0; r < H; ++r) out[0] = cadd(out[0], sp[r]);
 #pragma unroll
-    for (int q = 1; q < R; ++q) {
-      T2 acc = v[0];
+    for (int q = 1; q <= H; ++q) {
+      T2 a = v[0], b;
 #pragma unroll
       for (int r = 1; r <= H; ++r) {
         const int m = (r * q) % R;
-        const T c = (T)Roots<R>::c[m], s = (T)Roots<R>::s[m];
-        acc.x += sp[r - 1].x * c + sm[r - 1].x * s;
-        acc.y += sp[r - 1].y * c + sm[r - 1].y * s;
+        const T c = (T)Roots<R>::c[m];
+        a.x += sp[r - 1].x * c;
+        a.y += sp[r - 1].y * c;
       }
-      out[q] = acc;
+      b.x = sm[0].x * (T)Roots<R>::s[q % R];
+      b.y = sm[0].y * (T)Roots<R>::s[q % R];
+#pragma unroll
+      for (int r = 2; r <= H; ++r) {
+        const T s = (T)Roots<R>::s[(r * q) % R];
+        b.x += sm[r - 1].x * s;
+        b.y += sm[r - 1].y * s;
+      }
+      out[q] = cadd(a, b);
+      out[R - q] = csub(a, b);
     }
 #pragma unroll
     for (int q = 0; q < R; ++q) v[q] = out[q];
@@ -562,9 +589,10 @@ __device__ void fft_dif(T2 *buf, const Fft1D &f, int nst, int npen, unsigned lon
 // ---- plane lengths fixed at compile time ---------------------------------------
 // A yz kernel instantiated for plane lengths NY x NZ (0: taken from the plan)
 // runs its stages from a compile-time factorisation: the stage sequence is
-// unrolled, radices and fdiv magics are constants and the trip counts of the
-// z stages are known.  The factorisation follows fft_factor's in-place rule;
-// the launcher checks it against the plan before it picks such a kernel.
+// unrolled, radices, spans and strides are constants, each thread keeps one
+// pencil per transform and the twiddles sit in per-stage tables (ct_stage).
+// The factorisation follows fft_factor's in-place rule; the launcher checks it
+// against the plan before it picks such a kernel.
 __host__ __device__ constexpr unsigned long long fdiv_magic_ct(unsigned d) {
   return (0x100000000ull + d - 1) / d;
 }
@@ -592,30 +620,7 @@ __host__ __device__ constexpr int fft_lprod(const Fft1D &f, int st) {
   return L;
 }
 
-// stages ST..NST-1 of the DIT transform of length N
-template <typename T2, int N, int ST, int NST>
-__device__ inline void fft_dit_ct(T2 *buf, int npen, unsigned long long mpen, int es, int ps, const int *pmap,
-                                  const T2 *tw, int sign, bool pen_fast) {
-  if constexpr (ST < NST) {
-    constexpr Fft1D f = fft_inplace_ct(N);
-    ip_stage<T2, f.radix[ST], false>(buf, N, fft_lprod(f, ST), npen, es, ps, pmap, tw, sign, pen_fast, mpen,
-                                     f.mnbf[ST], f.mlp[ST]);
-    fft_dit_ct<T2, N, ST + 1, NST>(buf, npen, mpen, es, ps, pmap, tw, sign, pen_fast);
-  }
-}
-// stages ST..0 of the DIF transform of length N (called with ST = NST - 1)
-template <typename T2, int N, int ST>
-__device__ inline void fft_dif_ct(T2 *buf, int npen, unsigned long long mpen, int es, int ps, const int *pmap,
-                                  const T2 *tw, int sign, bool pen_fast) {
-  if constexpr (ST >= 0) {
-    constexpr Fft1D f = fft_inplace_ct(N);
-    ip_stage<T2, f.radix[ST], true>(buf, N, fft_lprod(f, ST), npen, es, ps, pmap, tw, sign, pen_fast, mpen,
-                                    f.mnbf[ST], f.mlp[ST]);
-    fft_dif_ct<T2, N, ST - 1>(buf, npen, mpen, es, ps, pmap, tw, sign, pen_fast);
-  }
-}
 template <int N> constexpr int nstage_ct() { return fft_inplace_ct(N).nstage; }
-template <int N> constexpr int last_radix_ct() { return fft_inplace_ct(N).radix[fft_inplace_ct(N).nstage - 1]; }
 
 // the plane's scalar lengths as constants
 template <int NY, int NZ>
@@ -628,6 +633,116 @@ __device__ inline void yz_fix(Geo &g) {
   if constexpr (NZ > 0) g.nr3 = NZ;
 }
 
+// ---- stages of the compile-time plane lengths ------------------------------------
+// A thread keeps one pencil through every stage of a transform: PenSet holds
+// the pencil's LDS offset and the butterflies b0, b0 + bstep, ... it takes in
+// each stage, so a butterfly's position needs no division.  The twiddles of
+// stage ST sit in LDS in butterfly order,
+//   tab[tw_off_ct(N, ST) + j (R - 1) + q - 1] = exp(+2 pi i j q N / L),
+// j < L/R, 1 <= q < R; the stages together hold N - 1 of them.
+struct PenSet {
+  int off, b0, bstep;   // b0 >= every stage's butterfly count: the thread has none
+};
+__host__ __device__ constexpr int tw_off_ct(int n, int st) {
+  const Fft1D f = fft_inplace_ct(n);
+  int off = 0, L = 1;
+  for (int s = 0; s < st; ++s) {
+    L *= f.radix[s];
+    off += (L / f.radix[s]) * (f.radix[s] - 1);
+  }
+  return off;
+}
+
+// pencils p = tid mod npen with offsets p * ps (pmap == nullptr) or pmap[p];
+// every thread below (kYZ / npen) npen takes part (npen <= kYZ)
+__device__ inline PenSet pen_set(int npen, int ps, const int *pmap) {
+  PenSet s;
+  const int t = threadIdx.x, step = npen > 0 ? kYZ / npen : 0;
+  if (npen > 0 && t < step * npen) {
+    const int q = t / npen, p = t - q * npen;
+    s.off = pmap ? pmap[p] : p * ps;
+    s.b0 = q;
+    s.bstep = step;
+  } else {
+    s.off = 0;
+    s.b0 = 1 << 30;
+    s.bstep = 1;
+  }
+  return s;
+}
+
+// the stage twiddle table of length N from root[m] = exp(+2 pi i m / N)
+template <typename T2, int N>
+__device__ inline void tw_table_ct(const T2 *root, T2 *tab) {
+  constexpr Fft1D f = fft_inplace_ct(N);
+  int off = 0, L = 1;
+  for (int st = 0; st < f.nstage; ++st) {
+    const int R = f.radix[st];
+    L *= R;
+    const int cnt = (L / R) * (R - 1);
+    for (int e = threadIdx.x; e < cnt; e += kYZ) {
+      const int j = e / (R - 1), q = e - j * (R - 1) + 1;
+      tab[off + e] = root[j * q * (N / L)];
+    }
+    off += cnt;
+  }
+}
+
+// stage ST of length N over the pencils of ps, elements ES apart; DIF: DFT
+// then twiddle, else twiddle then DFT; SIGN: exp(SIGN 2 pi i ...)
+template <typename T2, int N, int ST, bool DIF, int SIGN, int ES>
+__device__ inline void ct_stage(T2 *buf, const PenSet &ps, const T2 *tab) {
+  constexpr Fft1D f = fft_inplace_ct(N);
+  constexpr int R = f.radix[ST], L = fft_lprod(f, ST), Lp = L / R, NBF = N / R, TO = tw_off_ct(N, ST);
+  for (int b = ps.b0; b < NBF; b += ps.bstep) {
+    int gg, j;
+    if constexpr (Lp == 1) {
+      gg = b;
+      j = 0;
+    } else if constexpr (Lp == NBF) {
+      gg = 0;
+      j = b;
+    } else {
+      gg = b / Lp;
+      j = b - gg * Lp;
+    }
+    T2 *base = buf + ps.off + __mul24(__mul24(gg, L) + j, ES);
+    T2 v[R];
+#pragma unroll
+    for (int q = 0; q < R; ++q) v[q] = base[q * Lp * ES];
+    if constexpr (DIF) dft<T2, R>(v, SIGN);
+    if constexpr (Lp > 1) {
+      const T2 *t = tab + TO + __mul24(j, R - 1);
+#pragma unroll
+      for (int q = 1; q < R; ++q) {
+        T2 w = t[q - 1];
+        if (SIGN < 0) w.y = -w.y;
+        v[q] = cmul(v[q], w);
+      }
+    }
+    if constexpr (!DIF) dft<T2, R>(v, SIGN);
+#pragma unroll
+    for (int q = 0; q < R; ++q) base[q * Lp * ES] = v[q];
+  }
+  __syncthreads();
+}
+
+// stages ST..NST-1 of the DIT transform, stages ST..0 of the DIF transform
+template <typename T2, int N, int ST, int NST, int SIGN, int ES>
+__device__ inline void ct_dit(T2 *buf, const PenSet &ps, const T2 *tab) {
+  if constexpr (ST < NST) {
+    ct_stage<T2, N, ST, false, SIGN, ES>(buf, ps, tab);
+    ct_dit<T2, N, ST + 1, NST, SIGN, ES>(buf, ps, tab);
+  }
+}
+template <typename T2, int N, int ST, int SIGN, int ES>
+__device__ inline void ct_dif(T2 *buf, const PenSet &ps, const T2 *tab) {
+  if constexpr (ST >= 0) {
+    ct_stage<T2, N, ST, true, SIGN, ES>(buf, ps, tab);
+    ct_dif<T2, N, ST - 1, SIGN, ES>(buf, ps, tab);
+  }
+}
+
 // zero the plane, scatter the x column of one band's stripes into the
 // digit-reversed (z, y) positions, transform y (DIT, on the rows of the z
 // planes that hold stripes) and z (DIT) up to, not including, its last
@@ -636,7 +751,7 @@ __device__ inline void yz_fix(Geo &g) {
 // stripe-holding row.  Both transforms run pencil-fast: neighbouring lanes
 // take neighbouring pencils, whose rows (y) or columns (z) fall into distinct
 // LDS banks.
-template <typename T2, bool BIG, int NY = 0, int NZ = 0>
+template <typename T2, bool BIG>
 __device__ inline void yz_forward(const Geo &g, const Fft1D &fy, const Fft1D &fz, const T2 *twy,
                                   const T2 *twz, const T2 *col, T2 *pl, const int *zpos) {
   const int npl = g.pitch * g.nr3;
@@ -644,14 +759,8 @@ __device__ inline void yz_forward(const Geo &g, const Fft1D &fy, const Fft1D &fz
   __syncthreads();
   for (int s = threadIdx.x; s < g.nstripe; s += kYZ) pl[g.stripe_pos[s]] = col[s];
   __syncthreads();
-  if constexpr (NY > 0)
-    fft_dit_ct<T2, NY, 0, nstage_ct<NY>()>(pl, g.nzp, g.mnzp, 1, 0, zpos, twy, +1, true);
-  else
-    fft_dit<T2, BIG>(pl, fy, fy.nstage, g.nzp, g.mnzp, 1, 0, zpos, twy, +1, true);
-  if constexpr (NZ > 0)
-    fft_dit_ct<T2, NZ, 0, nstage_ct<NZ>() - 1>(pl, g.nr2, g.mnr2, g.pitch, 1, nullptr, twz, +1, true);
-  else
-    fft_dit<T2, BIG>(pl, fz, fz.nstage - 1, g.nr2, g.mnr2, g.pitch, 1, nullptr, twz, +1, true);
+  fft_dit<T2, BIG>(pl, fy, fy.nstage, g.nzp, g.mnzp, 1, 0, zpos, twy, +1, true);
+  fft_dit<T2, BIG>(pl, fz, fz.nstage - 1, g.nr2, g.mnr2, g.pitch, 1, nullptr, twz, +1, true);
 }
 
 // The last DIT stage of the z transforms, one butterfly per (column y, j):
@@ -756,6 +865,130 @@ __device__ inline void dens_reg_store(const Geo &g, int nr3, const double (&accr
   }
 }
 
+// ---- the same for compile-time plane lengths NY x NZ (row pitch P = NY | 1) ----------
+// py: the stripe rows (pencils of the y transforms), pz: the columns y (pencils
+// of the z transforms, off = y); tby, tbz: the stage twiddle tables.
+template <typename T2, int NY, int NZ>
+__device__ inline void yz_forward_ct(const Geo &g, const T2 *col, T2 *pl, const PenSet &py, const PenSet &pz,
+                                     const T2 *tby, const T2 *tbz) {
+  constexpr int P = NY | 1;
+  for (int i = threadIdx.x; i < P * NZ; i += kYZ) pl[i] = czero<T2>();
+  __syncthreads();
+  for (int s = threadIdx.x; s < g.nstripe; s += kYZ) pl[g.stripe_pos[s]] = col[s];
+  __syncthreads();
+  ct_dit<T2, NY, 0, nstage_ct<NY>(), +1, 1>(pl, py, tby);
+  ct_dit<T2, NZ, 0, nstage_ct<NZ>() - 1, +1, P>(pl, pz, tbz);
+}
+
+// the last forward z stage of butterfly j of column pz.off: psi(z = j + q Lp)
+template <typename T2, int NY, int NZ>
+__device__ inline void z_last_load_ct(const T2 *pl, int y, int j, const T2 *tbz, T2 *v) {
+  constexpr Fft1D f = fft_inplace_ct(NZ);
+  constexpr int ST = f.nstage - 1, R = f.radix[ST], Lp = NZ / R, P = NY | 1, TO = tw_off_ct(NZ, ST);
+  const T2 *base = pl + y + __mul24(j, P);
+#pragma unroll
+  for (int q = 0; q < R; ++q) v[q] = base[q * Lp * P];
+  if constexpr (Lp > 1) {
+    const T2 *t = tbz + TO + __mul24(j, R - 1);
+#pragma unroll
+    for (int q = 1; q < R; ++q) v[q] = cmul(v[q], t[q - 1]);
+  }
+  dft<T2, R>(v, +1);
+}
+
+// VPSI: last forward z stage, psi V(r), first inverse (DIF) z stage
+template <typename T2, int NY, int NZ>
+__device__ inline void z_last_vpsi_ct(T2 *pl, const PenSet &pz, const T2 *tbz, const typename Prec<T2>::T *vx) {
+  typedef typename Prec<T2>::T T;
+  constexpr Fft1D f = fft_inplace_ct(NZ);
+  constexpr int ST = f.nstage - 1, R = f.radix[ST], Lp = NZ / R, P = NY | 1, TO = tw_off_ct(NZ, ST);
+  const int y = pz.off;
+  for (int j = pz.b0; j < Lp; j += pz.bstep) {
+    T2 v[R];
+    z_last_load_ct<T2, NY, NZ>(pl, y, j, tbz, v);
+    const T *vj = vx + (unsigned)(__mul24(j, NY) + y);
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const T vv = vj[q * Lp * NY];
+      v[q].x *= vv;
+      v[q].y *= vv;
+    }
+    dft<T2, R>(v, -1);
+    if constexpr (Lp > 1) {
+      const T2 *t = tbz + TO + __mul24(j, R - 1);
+#pragma unroll
+      for (int q = 1; q < R; ++q) {
+        T2 w = t[q - 1];
+        w.y = -w.y;
+        v[q] = cmul(v[q], w);
+      }
+    }
+    T2 *base = pl + y + __mul24(j, P);
+#pragma unroll
+    for (int q = 0; q < R; ++q) base[q * Lp * P] = v[q];
+  }
+  __syncthreads();
+}
+
+// DENSITY: the band's term of the sum at the butterfly positions of the last
+// z stage, into LDS (acc[z][y]) or into registers (accr[it], butterfly
+// b0 + it bstep; needs L/R <= 2 bstep, which the launcher checks)
+template <typename T2, int NY, int NZ>
+__device__ inline void z_last_dens_ct(const T2 *pl, const PenSet &pz, const T2 *tbz, double s1, double s2,
+                                      double *acc) {
+  constexpr Fft1D f = fft_inplace_ct(NZ);
+  constexpr int R = f.radix[f.nstage - 1], Lp = NZ / R;
+  const int y = pz.off;
+  for (int j = pz.b0; j < Lp; j += pz.bstep) {
+    T2 v[R];
+    z_last_load_ct<T2, NY, NZ>(pl, y, j, tbz, v);
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const double re = v[q].x, im = v[q].y;
+      acc[(j + q * Lp) * NY + y] += s1 * (re * re + im * im) + s2 * (re * re - im * im);
+    }
+  }
+}
+template <typename T2, int NY, int NZ>
+__device__ inline void z_last_dens_reg_ct(const T2 *pl, const PenSet &pz, const T2 *tbz, double s1, double s2,
+                                          double (&accr)[2][kRegR]) {
+  constexpr Fft1D f = fft_inplace_ct(NZ);
+  constexpr int R = f.radix[f.nstage - 1], Lp = NZ / R;
+  static_assert(R <= kRegR, "register band sum: radix above kRegR");
+#pragma unroll
+  for (int it = 0; it < 2; ++it) {
+    const int j = pz.b0 + it * pz.bstep;
+    if (j < Lp) {
+      T2 v[R];
+      z_last_load_ct<T2, NY, NZ>(pl, pz.off, j, tbz, v);
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        const double re = v[q].x, im = v[q].y;
+        accr[it][q] += s1 * (re * re + im * im) + s2 * (re * re - im * im);
+      }
+    }
+  }
+}
+template <int NY, int NZ>
+__device__ inline void dens_reg_store_ct(const PenSet &pz, const double (&accr)[2][kRegR], double *dst) {
+  constexpr Fft1D f = fft_inplace_ct(NZ);
+  constexpr int R = f.radix[f.nstage - 1], Lp = NZ / R;
+#pragma unroll
+  for (int it = 0; it < 2; ++it) {
+    const int j = pz.b0 + it * pz.bstep;
+    if (j < Lp) {
+#pragma unroll
+      for (int q = 0; q < R; ++q) dst[(j + q * Lp) * NY + pz.off] = accr[it][q];
+    }
+  }
+}
+// the register band sum fits: every thread has at most two last-stage butterflies
+template <int NY, int NZ>
+constexpr bool dens_reg_ct() {
+  constexpr Fft1D f = fft_inplace_ct(NZ);
+  return f.radix[f.nstage - 1] <= kRegR && NZ / f.radix[f.nstage - 1] <= 2 * (kYZ / NY);
+}
+
 // ---- yz pass of WAVES_VPSI: psi(x,y,z) * V(r), back to the stripes ----------------
 // One workgroup per (x, band); vt = V as [x][z][y] in the work precision.
 template <typename T2, bool BIG, int NY = 0, int NZ = 0>
@@ -767,27 +1000,29 @@ __global__ void __launch_bounds__(kYZ) k_yz_vpsi(Geo g, Fft1D fy, Fft1D fz, cons
   const int n23 = g.nr2 * g.nr3;
   T2 *pl = reinterpret_cast<T2 *>(smem_raw);
   T2 *ty = pl + g.pitch * g.nr3, *tz = ty + g.nr2;
-  int *zpos = reinterpret_cast<int *>(tz + g.nr3);
   const int x = blockIdx.x, band = blockIdx.y;
-  for (int i = threadIdx.x; i < g.nzp; i += kYZ) zpos[i] = g.zpos[i];
-  for (int i = threadIdx.x; i < g.nr2; i += kYZ) ty[i] = twy[i];
-  for (int i = threadIdx.x; i < g.nr3; i += kYZ) tz[i] = twz[i];
   T2 *col = stripes + ((size_t)band * g.nr1 + x) * g.nstripe;
-  yz_forward<T2, BIG, NY, NZ>(g, fy, fz, ty, tz, col, pl, zpos);
   const T *vx = vt + (size_t)x * n23;
-  if constexpr (NZ > 0) {
-    z_last_vpsi<T2, last_radix_ct<NZ>()>(g, pl, NZ, tz, vx);
-    fft_dif_ct<T2, NZ, nstage_ct<NZ>() - 2>(pl, g.nr2, g.mnr2, g.pitch, 1, nullptr, tz, -1, true);
+  if constexpr (NY > 0 && NZ > 0) {
+    tw_table_ct<T2, NY>(twy, ty);
+    tw_table_ct<T2, NZ>(twz, tz);
+    const PenSet py = pen_set(g.nzp, 0, g.zpos), pz = pen_set(NY, 1, nullptr);
+    yz_forward_ct<T2, NY, NZ>(g, col, pl, py, pz, ty, tz);
+    z_last_vpsi_ct<T2, NY, NZ>(pl, pz, tz, vx);
+    ct_dif<T2, NZ, nstage_ct<NZ>() - 2, -1, NY | 1>(pl, pz, tz);
+    ct_dif<T2, NY, nstage_ct<NY>() - 1, -1, 1>(pl, py, ty);
   } else {
+    int *zpos = reinterpret_cast<int *>(tz + g.nr3);
+    for (int i = threadIdx.x; i < g.nzp; i += kYZ) zpos[i] = g.zpos[i];
+    for (int i = threadIdx.x; i < g.nr2; i += kYZ) ty[i] = twy[i];
+    for (int i = threadIdx.x; i < g.nr3; i += kYZ) tz[i] = twz[i];
+    yz_forward<T2, BIG>(g, fy, fz, ty, tz, col, pl, zpos);
 #define PW_CALL(R) z_last_vpsi<T2, R>(g, pl, g.nr3, tz, vx)
     PW_RADIX_SWITCH(fz.radix[fz.nstage - 1], PW_CALL)
 #undef PW_CALL
     fft_dif<T2, BIG>(pl, fz, fz.nstage - 1, g.nr2, g.mnr2, g.pitch, 1, nullptr, tz, -1, true);
-  }
-  if constexpr (NY > 0)
-    fft_dif_ct<T2, NY, nstage_ct<NY>() - 1>(pl, g.nzp, g.mnzp, 1, 0, zpos, ty, -1, true);
-  else
     fft_dif<T2, BIG>(pl, fy, fy.nstage, g.nzp, g.mnzp, 1, 0, zpos, ty, -1, true);
+  }
   for (int s = threadIdx.x; s < g.nstripe; s += kYZ) col[s] = pl[g.stripe_pos[s]];
 }
 
@@ -816,43 +1051,52 @@ __global__ void __launch_bounds__(kYZ) __attribute__((amdgpu_waves_per_eu(DensWa
   int *zpos = reinterpret_cast<int *>(tz + g.nr3);
   const int x = blockIdx.x, grp = blockIdx.y;
   const int bstart = grp * bpg, bend = min(nb, bstart + bpg);
-  for (int i = threadIdx.x; i < g.nzp; i += kYZ) zpos[i] = g.zpos[i];
-  for (int i = threadIdx.x; i < g.nr2; i += kYZ) ty[i] = twy[i];
-  for (int i = threadIdx.x; i < g.nr3; i += kYZ) tz[i] = twz[i];
+  double *dst = part + ((size_t)grp * g.nr1 + x) * n23;
   if (!REG)
     for (int i = threadIdx.x; i < n23; i += kYZ) acc[i] = 0.0;
-  for (int band = bstart; band < bend; ++band) {
-    const double s1 = occ2[2 * band], s2 = occ2[2 * band + 1];
-    if (s1 == 0.0 && s2 == 0.0) continue;   // uniform across the workgroup
-    __syncthreads();
-    yz_forward<T2, BIG, NY, NZ>(g, fy, fz, ty, tz, stripes + ((size_t)band * g.nr1 + x) * g.nstripe, pl, zpos);
-    // each acc position belongs to one butterfly of a fixed thread in every band
-    if constexpr (NZ > 0) {
-      constexpr int RL = last_radix_ct<NZ>();
-      if constexpr (REG) {
-        if constexpr (RL <= kRegR) z_last_dens_reg<T2, RL>(g, pl, NZ, tz, s1, s2, accr);
-      } else {
-        z_last_dens<T2, RL>(g, pl, NZ, tz, s1, s2, acc);
+  if constexpr (NY > 0 && NZ > 0) {
+    if constexpr (!REG || dens_reg_ct<NY, NZ>()) {
+      tw_table_ct<T2, NY>(twy, ty);
+      tw_table_ct<T2, NZ>(twz, tz);
+      const PenSet py = pen_set(g.nzp, 0, g.zpos), pz = pen_set(NY, 1, nullptr);
+      for (int band = bstart; band < bend; ++band) {
+        const double s1 = occ2[2 * band], s2 = occ2[2 * band + 1];
+        if (s1 == 0.0 && s2 == 0.0) continue;   // uniform across the workgroup
+        __syncthreads();
+        yz_forward_ct<T2, NY, NZ>(g, stripes + ((size_t)band * g.nr1 + x) * g.nstripe, pl, py, pz, ty, tz);
+        // each acc position belongs to one butterfly of a fixed thread in every band
+        if constexpr (REG) z_last_dens_reg_ct<T2, NY, NZ>(pl, pz, tz, s1, s2, accr);
+        else z_last_dens_ct<T2, NY, NZ>(pl, pz, tz, s1, s2, acc);
       }
-    } else if constexpr (REG) {
+      if constexpr (REG) dens_reg_store_ct<NY, NZ>(pz, accr, dst);
+    }
+  } else {
+    for (int i = threadIdx.x; i < g.nzp; i += kYZ) zpos[i] = g.zpos[i];
+    for (int i = threadIdx.x; i < g.nr2; i += kYZ) ty[i] = twy[i];
+    for (int i = threadIdx.x; i < g.nr3; i += kYZ) tz[i] = twz[i];
+    for (int band = bstart; band < bend; ++band) {
+      const double s1 = occ2[2 * band], s2 = occ2[2 * band + 1];
+      if (s1 == 0.0 && s2 == 0.0) continue;   // uniform across the workgroup
+      __syncthreads();
+      yz_forward<T2, BIG>(g, fy, fz, ty, tz, stripes + ((size_t)band * g.nr1 + x) * g.nstripe, pl, zpos);
+      // each acc position belongs to one butterfly of a fixed thread in every band
+      if constexpr (REG) {
 #define PW_CALL(R) if constexpr (R <= kRegR) z_last_dens_reg<T2, R>(g, pl, g.nr3, tz, s1, s2, accr)
-      PW_RADIX_SWITCH(fz.radix[fz.nstage - 1], PW_CALL)
+        PW_RADIX_SWITCH(fz.radix[fz.nstage - 1], PW_CALL)
 #undef PW_CALL
-    } else {
+      } else {
 #define PW_CALL(R) z_last_dens<T2, R>(g, pl, g.nr3, tz, s1, s2, acc)
+        PW_RADIX_SWITCH(fz.radix[fz.nstage - 1], PW_CALL)
+#undef PW_CALL
+      }
+    }
+    if constexpr (REG) {
+#define PW_CALL(R) if constexpr (R <= kRegR) dens_reg_store<R>(g, g.nr3, accr, dst)
       PW_RADIX_SWITCH(fz.radix[fz.nstage - 1], PW_CALL)
 #undef PW_CALL
     }
   }
-  double *dst = part + ((size_t)grp * g.nr1 + x) * n23;
-  if constexpr (REG && NZ > 0) {
-    constexpr int RL = last_radix_ct<NZ>();
-    if constexpr (RL <= kRegR) dens_reg_store<RL>(g, NZ, accr, dst);
-  } else if constexpr (REG) {
-#define PW_CALL(R) if constexpr (R <= kRegR) dens_reg_store<R>(g, g.nr3, accr, dst)
-    PW_RADIX_SWITCH(fz.radix[fz.nstage - 1], PW_CALL)
-#undef PW_CALL
-  } else {
+  if constexpr (!REG) {
     __syncthreads();
     for (int i = threadIdx.x; i < n23; i += kYZ) dst[i] = acc[i];
   }
@@ -1111,10 +1355,11 @@ template <typename T2>
 int density_t(pw_plan *p, int nb, const double2 *psi, const double *occ2, double *rho, int target_blocks,
               hipStream_t s) {
   const size_t n23 = (size_t)p->nr2 * p->nr3;
-  // the band sum in registers when the last z stage has at most 2 kYZ butterflies
+  // the band sum in registers when no thread has more than two butterflies of
+  // the last z stage (pencil y = tid mod NR2, butterflies tid / NR2 + k (kYZ / NR2))
   const Fft1D &fzl = p->fyz[1];
   const int rl = fzl.radix[fzl.nstage - 1];
-  const bool reg = !yz_big(p) && rl <= kRegR && (size_t)(p->nr3 / rl) * p->nr2 <= 2 * (size_t)kYZ;
+  const bool reg = !yz_big(p) && rl <= kRegR && p->nr3 / rl <= 2 * (kYZ / p->nr2);
   const size_t lds = (reg ? 0 : n23 * sizeof(double)) +
                      ((size_t)p->yzpitch * p->nr3 + p->nr2 + p->nr3) * sizeof(T2) + (size_t)p->nzp * sizeof(int);
   PW_TRY(yz_fits(p, lds));
