@@ -102,6 +102,7 @@ struct pw_plan {
   std::vector<int> yz2stripe;  // nr2*nr3, -1 if empty
   std::vector<int> stripe_pos; // nstripe: digit-reversed z*yzpitch + y of each stripe
   std::vector<int> zpos;       // nzp: digit-reversed z*yzpitch of each stripe-holding z plane
+  std::vector<int> ypen;       // the zpos rows in windows of 16 lane slots (-1: empty slot)
   int yzpitch = 0;             // row pitch of the LDS-resident (y, z) plane: nr2 made odd
   int nzp = 0;                 // z planes holding stripes
   std::vector<int> z2iz;       // nr3, -1 if empty
@@ -114,7 +115,7 @@ struct pw_plan {
   int *d_stripe_off = nullptr, *d_stripe_g = nullptr, *d_stripe_x = nullptr;
   int *d_stripe_s = nullptr;
   int *d_yz2stripe = nullptr, *d_z2iz = nullptr, *d_zlist = nullptr;
-  int *d_stripe_pos = nullptr, *d_zpos = nullptr;
+  int *d_stripe_pos = nullptr, *d_zpos = nullptr, *d_ypen = nullptr;
   double *d_g2 = nullptr, *d_gvec = nullptr;
   double2 *d_tw[3] = {nullptr, nullptr, nullptr};
   float2 *d_twf[3] = {nullptr, nullptr, nullptr};

@@ -322,9 +322,11 @@ __device__ inline T2 *lds_fft_ct(T2 *in, T2 *out, int npen, int es, int ps, cons
 struct Geo {
   int nr1, nr2, nr3, ngl, nstripe, nzp;
   int pitch;                      // row pitch of the LDS-resident (y, z) plane (odd)
+  int nypen;                      // lane slots of the y transforms (ypen)
   unsigned long long mnzp, mnr2;  // fdiv magics of nzp and nr2
   const int *stripe_off, *stripe_g, *stripe_x, *stripe_s, *yz2stripe, *z2iz, *zlist;
   const int *stripe_pos, *zpos;   // digit-reversed plane offsets of stripes / stripe rows
+  const int *ypen;                // stripe-row offset of each lane slot of the y transforms, -1: none
   const double *g2;
 };
 
@@ -653,14 +655,16 @@ __host__ __device__ constexpr int tw_off_ct(int n, int st) {
   return off;
 }
 
-// pencils p = tid mod npen with offsets p * ps (pmap == nullptr) or pmap[p];
-// every thread below (kYZ / npen) npen takes part (npen <= kYZ)
+// pencils p = tid mod npen with offsets p * ps (pmap == nullptr) or pmap[p]
+// (a negative entry: no pencil); every thread below (kYZ / npen) npen takes
+// part (npen <= kYZ)
 __device__ inline PenSet pen_set(int npen, int ps, const int *pmap) {
   PenSet s;
   const int t = threadIdx.x, step = npen > 0 ? kYZ / npen : 0;
-  if (npen > 0 && t < step * npen) {
-    const int q = t / npen, p = t - q * npen;
-    s.off = pmap ? pmap[p] : p * ps;
+  const int q = npen > 0 ? t / npen : 0, p = t - q * npen;
+  const int off = npen > 0 && t < step * npen ? (pmap ? pmap[p] : p * ps) : -1;
+  if (off >= 0) {
+    s.off = off;
     s.b0 = q;
     s.bstep = step;
   } else {
@@ -1006,7 +1010,7 @@ __global__ void __launch_bounds__(kYZ) k_yz_vpsi(Geo g, Fft1D fy, Fft1D fz, cons
   if constexpr (NY > 0 && NZ > 0) {
     tw_table_ct<T2, NY>(twy, ty);
     tw_table_ct<T2, NZ>(twz, tz);
-    const PenSet py = pen_set(g.nzp, 0, g.zpos), pz = pen_set(NY, 1, nullptr);
+    const PenSet py = pen_set(g.nypen, 0, g.ypen), pz = pen_set(NY, 1, nullptr);
     yz_forward_ct<T2, NY, NZ>(g, col, pl, py, pz, ty, tz);
     z_last_vpsi_ct<T2, NY, NZ>(pl, pz, tz, vx);
     ct_dif<T2, NZ, nstage_ct<NZ>() - 2, -1, NY | 1>(pl, pz, tz);
@@ -1058,7 +1062,7 @@ __global__ void __launch_bounds__(kYZ) __attribute__((amdgpu_waves_per_eu(DensWa
     if constexpr (!REG || dens_reg_ct<NY, NZ>()) {
       tw_table_ct<T2, NY>(twy, ty);
       tw_table_ct<T2, NZ>(twz, tz);
-      const PenSet py = pen_set(g.nzp, 0, g.zpos), pz = pen_set(NY, 1, nullptr);
+      const PenSet py = pen_set(g.nypen, 0, g.ypen), pz = pen_set(NY, 1, nullptr);
       for (int band = bstart; band < bend; ++band) {
         const double s1 = occ2[2 * band], s2 = occ2[2 * band + 1];
         if (s1 == 0.0 && s2 == 0.0) continue;   // uniform across the workgroup
@@ -1172,6 +1176,7 @@ Geo make_geo(const pw_plan *p) {
   g.stripe_off = p->d_stripe_off; g.stripe_g = p->d_stripe_g; g.stripe_x = p->d_stripe_x;
   g.stripe_s = p->d_stripe_s; g.yz2stripe = p->d_yz2stripe; g.z2iz = p->d_z2iz;
   g.zlist = p->d_zlist; g.stripe_pos = p->d_stripe_pos; g.zpos = p->d_zpos; g.g2 = p->d_g2;
+  g.ypen = p->d_ypen; g.nypen = (int)p->ypen.size();
   return g;
 }
 

@@ -316,6 +316,25 @@ static int build_plan(pw_plan *p, const double *rbas, double epw_psi, double epw
     }
   p->zpos.assign(p->nzp, 0);
   for (int i = 0; i < p->nzp; ++i) p->zpos[i] = pz[p->zlist[i]] * p->yzpitch;
+  // Lane slots of the y transforms (one stripe row per lane).  A 64-bit LDS
+  // access is served 16 lanes at a time over 32 four-byte banks, so 16 lanes
+  // that read the same element of 16 rows are conflict-free when the row
+  // offsets differ mod 16.  The k-th row whose offset is r mod 16 takes slot
+  // 16 k + r; when some residue holds more rows than ceil(nzp / 16) windows
+  // the rows keep their order, one slot each.
+  {
+    std::vector<std::vector<int>> byres(16);
+    for (int i = 0; i < p->nzp; ++i) byres[p->zpos[i] % 16].push_back(p->zpos[i]);
+    size_t nwin = 0;
+    for (const auto &v : byres) nwin = std::max(nwin, v.size());
+    if (nwin == (size_t)(p->nzp + 15) / 16) {
+      p->ypen.assign(16 * nwin, -1);
+      for (int r = 0; r < 16; ++r)
+        for (size_t k = 0; k < byres[r].size(); ++k) p->ypen[16 * k + r] = byres[r][k];
+    } else {
+      p->ypen = p->zpos;
+    }
+  }
   return PW_OK;
 }
 
@@ -340,6 +359,7 @@ static int upload_plan(pw_plan *p) {
   PW_TRY(upload(&p->d_yz2stripe, p->yz2stripe));
   PW_TRY(upload(&p->d_stripe_pos, p->stripe_pos));
   PW_TRY(upload(&p->d_zpos, p->zpos));
+  PW_TRY(upload(&p->d_ypen, p->ypen));
   PW_TRY(upload(&p->d_z2iz, p->z2iz));
   PW_TRY(upload(&p->d_g2, p->g2));
   PW_TRY(upload(&p->d_gvec, p->gvec));
@@ -415,7 +435,7 @@ extern "C" int pw_plan_destroy(pw_plan *p) {
   if (!p) return PW_OK;
   void *ptrs[] = {p->d_minusg, p->d_igvec, p->d_stripe_off, p->d_stripe_g, p->d_stripe_x,
                   p->d_stripe_s, p->d_zlist, p->d_yz2stripe, p->d_z2iz, p->d_g2, p->d_gvec, p->d_stripe_buf,
-                  p->d_plane_buf, p->d_dens_part, p->d_vt, p->d_stripe_pos, p->d_zpos, p->d_tw[0], p->d_tw[1], p->d_tw[2], p->d_twf[0],
+                  p->d_plane_buf, p->d_dens_part, p->d_vt, p->d_stripe_pos, p->d_zpos, p->d_ypen, p->d_tw[0], p->d_tw[1], p->d_tw[2], p->d_twf[0],
                   p->d_twf[1], p->d_twf[2], p->d_pot_tabs, p->d_pot};
   for (void *q : ptrs)
     if (q) (void)hipFree(q);
