@@ -869,6 +869,7 @@ extern "C" int pw_waves_sphere_setup(pw_waveset *ws, int isp, int nr, double r1,
                                      const double *dekin) {
   if (!ws || !aephi || !psphi || !aecore || !pscore || !vadd || !dekin)
     return pw_fail(PW_ERR_ARG, "pw_waves_sphere_setup: null argument");
+  PW_TRY(ws_finish(ws));
   if (isp < 1 || isp > ws->nsp) return pw_fail(PW_ERR_ARG, "pw_waves_sphere_setup: species index out of range");
   if (nr < 10 || nr > kMaxNr) return pw_fail(PW_ERR_UNSUPPORTED, "pw_waves_sphere_setup: NR must be 10..512");
   if (!(r1 > 0.0) || !(dex > 0.0) || !(rcsm > 0.0) || !(rnuc > 0.0))

@@ -231,6 +231,13 @@ struct pw_waveset {
   double *d_res = nullptr, *h_res = nullptr;
   size_t res_n = 0;
   size_t r_ekin = 0, r_vrho = 0, r_wk = 0, r_expect = 0, r_denmat = 0, r_force = 0, r_resid = 0;
+  // ETOT's and ORTHOGONALIZE's sums are copied into h_res without a wait:
+  // res_ev follows the copy, and ws_finish (called by every entry point that
+  // reads or changes the host state) waits for it and completes them
+  hipEvent_t res_ev = nullptr;
+  int res_pending = 0;         // 1: ETOT's sums, 2: ORTHOGONALIZE's
+  double pend_wk1 = 0.0;       // WAVEKIN1 when ORTHOGONALIZE's sums were enqueued
+  bool pend_switched = false;  // SWITCH ran after them: WAVEKIN2 stays zero
   // residual of the last ORTHOGONALIZE (max over super bands of the norm of
   // the constrained force), evaluated when residual_on
   bool residual_on = false;
@@ -307,6 +314,7 @@ struct ScopeTimer {
 };
 void ws_flush_timers(pw_waveset *ws);
 // a waveset as created, keeping its buffers, stream, graph and launch shapes
+int ws_finish(pw_waveset *ws);
 int ws_reset_run(pw_waveset *ws);
 
 // real spherical harmonics at one direction, LM(L,M) = 1+L^2+L-M, L <= lx <= 6

@@ -137,6 +137,7 @@ int run_job(Job *j) {
   int calm = 0;
   for (int it = 0; rc == PW_OK && it < j->max_steps; ++it) {
     rc = pw_waves_etot(ws);
+    if (rc == PW_OK) rc = ws_finish(ws);   // ETOT's energies and the last residual on the host
     if (rc != PW_OK) break;
     j->etot = ws->etot;
     j->ekin = ws->ekin;
@@ -250,6 +251,7 @@ void worker(pw_server *s) {
 // ---------------------------------------------------------------------------
 extern "C" int pw_waves_autotune(pw_waveset *ws) {
   if (!ws) return pw_fail(PW_ERR_ARG, "pw_waves_autotune: null waveset");
+  PW_TRY(ws_finish(ws));
   if (!ws->have_psi0 || ws->r0.empty())
     return pw_fail(PW_ERR_STATE, "pw_waves_autotune: PSI0 and R0 must be set");
   pw_plan *p = ws->plan;

@@ -259,9 +259,12 @@ static int dedproj(pw_waveset *ws) {
 }
 
 // read the device results of a step section back (one copy, one wait)
-static int fetch_results(pw_waveset *ws) {
-  PW_HIP(hipMemcpyAsync(ws->h_res, ws->d_res, ws->res_n * sizeof(double), hipMemcpyDeviceToHost, ws->stream));
-  PW_HIP(hipStreamSynchronize(ws->stream));
+// the copy of a section's sums into h_res is in flight: mark it (what: 1
+// ETOT, 2 ORTHOGONALIZE) behind an event for ws_finish
+static int defer_results(pw_waveset *ws, int what) {
+  if (!ws->res_ev) PW_HIP(hipEventCreateWithFlags(&ws->res_ev, hipEventDisableTiming));
+  PW_HIP(hipEventRecord(ws->res_ev, ws->stream));
+  ws->res_pending |= what;
   return PW_OK;
 }
 
@@ -474,6 +477,7 @@ extern "C" int pw_waves_create(pw_plan *plan, int nb, int nat, int nsp, const in
 // multipliers RLAM0/RLAMM (the first constraint solve starts from RLAM0 and
 // the first DEDPROJ reads it), the NB x NB matrices and the projections.
 int ws_reset_run(pw_waveset *ws) {
+  PW_TRY(ws_finish(ws));
   ws_flush_timers(ws);
   sphere_release(ws);
   const size_t n2 = (size_t)ws->nb * ws->nb, jsz = (size_t)ws->npro * ws->nbh * sizeof(double2);
@@ -518,7 +522,7 @@ int ws_reset_run(pw_waveset *ws) {
   f.d_small = ws->d_small; f.small_bytes = ws->small_bytes;
   f.d_ortho = ws->d_ortho; f.h_ortho_state = ws->h_ortho_state; f.ortho_cap = ws->ortho_cap;
   f.ortho_exec = ws->ortho_exec; f.ortho_key = ws->ortho_key;
-  f.stream = ws->stream; f.own_stream = ws->own_stream;
+  f.stream = ws->stream; f.own_stream = ws->own_stream; f.res_ev = ws->res_ev;
   // launch shapes
   f.tune_split_waves = ws->tune_split_waves; f.tune_col_waves = ws->tune_col_waves;
   f.tune_dens_blocks = ws->tune_dens_blocks;
@@ -528,6 +532,7 @@ int ws_reset_run(pw_waveset *ws) {
 
 extern "C" int pw_waves_destroy(pw_waveset *ws) {
   if (!ws) return PW_OK;
+  PW_TRY(ws_finish(ws));
   ws_flush_timers(ws);
   ortho_graph_release(ws);
   sphere_release(ws);
@@ -543,6 +548,7 @@ extern "C" int pw_waves_destroy(pw_waveset *ws) {
   if (ws->d_res) (void)hipFree(ws->d_res);
   if (ws->h_res) (void)hipHostFree(ws->h_res);
   if (ws->h_ortho_state) (void)hipHostFree(ws->h_ortho_state);
+  if (ws->res_ev) (void)hipEventDestroy(ws->res_ev);
   if (ws->own_stream && ws->stream) (void)hipStreamDestroy(ws->stream);
   delete ws;
   std::lock_guard<std::mutex> lk(capture_gate().mu);
@@ -552,6 +558,7 @@ extern "C" int pw_waves_destroy(pw_waveset *ws) {
 
 extern "C" int pw_waves_set_stream(pw_waveset *ws, void *stream) {
   if (!ws) return pw_fail(PW_ERR_ARG, "pw_waves_set_stream: null waveset");
+  PW_TRY(ws_finish(ws));
   if (ws->own_stream && ws->stream) (void)hipStreamDestroy(ws->stream);
   ws->stream = (hipStream_t)stream;
   ws->own_stream = false;
@@ -560,6 +567,7 @@ extern "C" int pw_waves_set_stream(pw_waveset *ws, void *stream) {
 
 extern "C" int pw_waves_setr8(pw_waveset *ws, const char *id, double val) {
   if (!ws || !id) return pw_fail(PW_ERR_ARG, "pw_waves_setr8: null argument");
+  PW_TRY(ws_finish(ws));
   // inputs are copied synchronously; work still queued on the waveset's
   // (non-blocking) stream must not see them change
   PW_HIP(hipStreamSynchronize(ws->stream));
@@ -575,6 +583,7 @@ extern "C" int pw_waves_setr8(pw_waveset *ws, const char *id, double val) {
 
 extern "C" int pw_waves_setr8a(pw_waveset *ws, const char *id, int len, const double *val) {
   if (!ws || !id || !val) return pw_fail(PW_ERR_ARG, "pw_waves_setr8a: null argument");
+  PW_TRY(ws_finish(ws));
   // inputs are copied synchronously; work still queued on the waveset's
   // (non-blocking) stream must not see them change
   PW_HIP(hipStreamSynchronize(ws->stream));
@@ -627,6 +636,7 @@ extern "C" int pw_waves_setr8a(pw_waveset *ws, const char *id, int len, const do
 
 extern "C" int pw_waves_setc8a(pw_waveset *ws, const char *id, int len, const double *val) {
   if (!ws || !id || !val) return pw_fail(PW_ERR_ARG, "pw_waves_setc8a: null argument");
+  PW_TRY(ws_finish(ws));
   // inputs are copied synchronously; work still queued on the waveset's
   // (non-blocking) stream must not see them change
   PW_HIP(hipStreamSynchronize(ws->stream));
@@ -676,6 +686,7 @@ static int get_dev_c(pw_waveset *ws, const double2 *src, size_t n, int len, doub
 
 extern "C" int pw_waves_getc8a(pw_waveset *ws, const char *id, int len, double *val) {
   if (!ws || !id || !val) return pw_fail(PW_ERR_ARG, "pw_waves_getc8a: null argument");
+  PW_TRY(ws_finish(ws));
   std::string s(id);
   const size_t npsi = (size_t)ws->nbh * ws->plan->ngl;
   const size_t nproj = (size_t)ws->nbh * ws->npro;
@@ -729,6 +740,7 @@ static void to_colmajor(const std::vector<double> &m, int n, double *out) {
 
 extern "C" int pw_waves_getr8a(pw_waveset *ws, const char *id, int len, double *val) {
   if (!ws || !id || !val) return pw_fail(PW_ERR_ARG, "pw_waves_getr8a: null argument");
+  PW_TRY(ws_finish(ws));
   std::string s(id);
   auto chk = [&](size_t n) {
     return (size_t)len == n ? PW_OK : pw_fail(PW_ERR_ARG, "WAVES$GETR8A: size inconsistent for " + s);
@@ -813,6 +825,7 @@ static int ortho_error(pw_waveset *ws, double *out) {
 
 extern "C" int pw_waves_getr8(pw_waveset *ws, const char *id, double *val) {
   if (!ws || !id || !val) return pw_fail(PW_ERR_ARG, "pw_waves_getr8: null argument");
+  PW_TRY(ws_finish(ws));
   std::string s(id);
   if (s == "ORTHOERR") return ortho_error(ws, val);
   if (s == "EKIN") *val = ws->ekin;
@@ -833,6 +846,7 @@ extern "C" int pw_waves_getr8(pw_waveset *ws, const char *id, double *val) {
 
 extern "C" int pw_waves_geti4(pw_waveset *ws, const char *id, int *val) {
   if (!ws || !id || !val) return pw_fail(PW_ERR_ARG, "pw_waves_geti4: null argument");
+  PW_TRY(ws_finish(ws));
   std::string s(id);
   if (s == "ORTHOITER") *val = ws->ortho_iter;
   else if (s == "NPRO") *val = ws->npro;
@@ -856,6 +870,7 @@ extern "C" int pw_waves_geti4(pw_waveset *ws, const char *id, int *val) {
 
 extern "C" int pw_waves_devptr(pw_waveset *ws, const char *id, void **ptr) {
   if (!ws || !id || !ptr) return pw_fail(PW_ERR_ARG, "pw_waves_devptr: null argument");
+  PW_TRY(ws_finish(ws));
   std::string s(id);
   if (s == "PSI0") *ptr = ws->d_psi0;
   else if (s == "PSIM") *ptr = ws->d_psim;
@@ -869,6 +884,7 @@ extern "C" int pw_waves_devptr(pw_waveset *ws, const char *id, void **ptr) {
 
 extern "C" int pw_waves_profile(pw_waveset *ws, int enable) {
   if (!ws) return pw_fail(PW_ERR_ARG, "pw_waves_profile: null waveset");
+  PW_TRY(ws_finish(ws));
   ws_flush_timers(ws);
   ws->profile = enable != 0;
   ws->ktime.clear();
@@ -877,6 +893,7 @@ extern "C" int pw_waves_profile(pw_waveset *ws, int enable) {
 
 extern "C" int pw_waves_kernel_time(pw_waveset *ws, const char *name, double *ms, int *count) {
   if (!ws || !name || !ms || !count) return pw_fail(PW_ERR_ARG, "pw_waves_kernel_time: null argument");
+  PW_TRY(ws_finish(ws));
   ws_flush_timers(ws);
   auto it = ws->ktime.find(name);
   *ms = it == ws->ktime.end() ? 0.0 : it->second.first;
@@ -899,6 +916,7 @@ static int ready(pw_waveset *ws) {
 
 extern "C" int pw_waves_gramschmidt(pw_waveset *ws) {
   if (!ws) return pw_fail(PW_ERR_ARG, "pw_waves_gramschmidt: null waveset");
+  PW_TRY(ws_finish(ws));
   PW_TRY(ready(ws));
   PW_TRY(gramschmidt_one(ws, ws->d_psi0, ws->r0));
   PW_TRY(gramschmidt_one(ws, ws->d_psim, ws->rm.empty() ? ws->r0 : ws->rm));
@@ -907,11 +925,98 @@ extern "C" int pw_waves_gramschmidt(pw_waveset *ws) {
 
 extern "C" int pw_waves_projections(pw_waveset *ws) {
   if (!ws) return pw_fail(PW_ERR_ARG, "pw_waves_projections: null waveset");
+  PW_TRY(ws_finish(ws));
   PW_TRY(need(ws->have_psi0, "PSI0"));
   PW_TRY(need(!ws->r0.empty(), "R0"));
   PW_TRY(phase_at(ws, ws->r0));
   PW_TRY(k_projections(ws, ws->d_psi0, ws->d_phase, ws->d_proj));
   PW_HIP(hipStreamSynchronize(ws->stream));
+  return PW_OK;
+}
+
+// ETOT's results from its sums in h_res (energies, DENMAT, forces, band
+// energies)
+static int etot_host(pw_waveset *ws) {
+  const pw_plan *p = ws->plan;
+  const double *hr = ws->h_res;
+  ws->ekin = 0.5 * sum_of(hr + ws->r_ekin, red_blocks(ws)) * p->vol;
+  if (ws->dplan) {
+    double en[3];
+    ws->vqlm.resize((size_t)ws->nat * ws->dplan->pot_lmrxx);
+    vofrho_results(ws->dplan, ws->nat, en, ws->vqlm.data(), ws->fpot.data());
+    ws->ehartree = en[0];
+    ws->exc = en[1];
+    ws->rhob = en[2];
+    ws->epot = en[0] + en[1];
+  } else {
+    ws->epot = sum_of(hr + ws->r_vrho, vrho_blocks(ws)) * p->vol / (double)p->nrl;
+  }
+  ws->wavekin1 = sum_of(hr + ws->r_wk, red_blocks(ws)) / (ws->delt * ws->delt) * p->vol;
+  // density matrix (hermitised, real) and the one-centre energy sum_ij DH_ij D_ji
+  const int L = ws->lmnxx;
+  const double *dm = hr + ws->r_denmat;
+  for (int a = 0; a < ws->nat; ++a)
+    for (int i = 0; i < L; ++i)
+      for (int j = 0; j < L; ++j) {
+        const size_t ij = ((size_t)a * L + i) * L + j, ji = ((size_t)a * L + j) * L + i;
+        ws->denmat[ij] = 0.5 * (dm[ij] + dm[ji]);
+      }
+  if (ws->dplan && sphere_ready(ws)) {
+    // AUGMENTATION$SPHERE's one-centre energy and POTB, from the device
+    sphere_results(ws, &ws->e1c, &ws->potb);
+  } else {
+    double e1 = 0.0;
+    for (int a = 0; a < ws->nat; ++a)
+      for (int i = 0; i < L; ++i)
+        for (int j = 0; j < L; ++j) {
+          const size_t src = ((size_t)a * L + j) * L + i;   // DH(i,j,a) column-major
+          e1 += ws->dh[2 * src] * ws->denmat[((size_t)a * L + j) * L + i];
+        }
+    ws->e1c = e1;
+  }
+  ws->etot = ws->ekin + ws->epot + ws->e1c;
+  // forces: sum the projector contributions of each atom (fixed order)
+  const double *fp = hr + ws->r_force;
+  for (int a = 0; a < ws->nat; ++a) {
+    const int i0 = ws->ipro0[a], nl = ws->lmnx[ws->isp[a]];
+    for (int d = 0; d < 3; ++d) {
+      double f = 0.0;
+      for (int l = 0; l < nl; ++l) f += fp[(size_t)(i0 + l) * 3 + d];
+      if (ws->dplan) f += ws->fpot[3 * a + d];
+      ws->force[3 * a + d] = f;
+    }
+  }
+  // band energies from the super-band expectation values
+  const double *ex = hr + ws->r_expect;
+  const int nc = expect_chunks(ws);
+  for (int b = 0; b < ws->nbh; ++b) {
+    double t = 0.0, tm = 0.0;
+    for (int c = 0; c < nc; ++c) {
+      t += ex[2 * ((size_t)c * ws->nbh + b)];
+      tm += ex[2 * ((size_t)c * ws->nbh + b) + 1];
+    }
+    const double re = 0.5 * t * p->vol, rem = 0.5 * tm * p->vol;
+    ws->eig[2 * b] = re + rem;
+    if (2 * b + 1 < ws->nb) ws->eig[2 * b + 1] = re - rem;
+  }
+  return PW_OK;
+}
+
+// ORTHOGONALIZE's results from its sums in h_res
+static void ortho_host(pw_waveset *ws) {
+  const double wk2 = sum_of(ws->h_res + ws->r_wk, red_blocks(ws)) / (ws->delt * ws->delt) * ws->plan->vol;
+  if (ws->residual_on) ws->residual = ws->h_res[ws->r_resid];
+  ws->efict = 0.5 * (ws->pend_wk1 + wk2);
+  if (!ws->pend_switched) ws->wavekin2 = wk2;
+}
+
+int ws_finish(pw_waveset *ws) {
+  if (!ws || !ws->res_pending) return PW_OK;
+  const int what = ws->res_pending;
+  ws->res_pending = 0;
+  PW_HIP(hipEventSynchronize(ws->res_ev));
+  if (what & 1) PW_TRY(etot_host(ws));
+  if (what & 2) ortho_host(ws);
   return PW_OK;
 }
 
@@ -989,67 +1094,15 @@ extern "C" int pw_waves_etot(pw_waveset *ws) {
   }
   // WAVES_EKIN, <psi|H|psi>, WAVES_WAVEKINETIC (first half): one pass over psi0
   PW_TRY(k_etot_sums(ws, ws->d_res + ws->r_ekin, ws->d_res + ws->r_wk, ws->d_res + ws->r_expect));
-  PW_TRY(fetch_results(ws));
-  const double *hr = ws->h_res;
-  ws->ekin = 0.5 * sum_of(hr + ws->r_ekin, red_blocks(ws)) * p->vol;
+  // the sums are copied without a wait (ws_finish completes them) unless the
+  // density -> potential step needs its results now
+  PW_TRY(ws_finish(ws));
+  PW_HIP(hipMemcpyAsync(ws->h_res, ws->d_res, ws->res_n * sizeof(double), hipMemcpyDeviceToHost, ws->stream));
   if (ws->dplan) {
-    double en[3];
-    ws->vqlm.resize((size_t)ws->nat * ws->dplan->pot_lmrxx);
-    vofrho_results(ws->dplan, ws->nat, en, ws->vqlm.data(), ws->fpot.data());
-    ws->ehartree = en[0];
-    ws->exc = en[1];
-    ws->rhob = en[2];
-    ws->epot = en[0] + en[1];
+    PW_HIP(hipStreamSynchronize(ws->stream));
+    PW_TRY(etot_host(ws));
   } else {
-    ws->epot = sum_of(hr + ws->r_vrho, vrho_blocks(ws)) * p->vol / (double)p->nrl;
-  }
-  ws->wavekin1 = sum_of(hr + ws->r_wk, red_blocks(ws)) / (ws->delt * ws->delt) * p->vol;
-  // density matrix (hermitised, real) and the one-centre energy sum_ij DH_ij D_ji
-  const int L = ws->lmnxx;
-  const double *dm = hr + ws->r_denmat;
-  for (int a = 0; a < ws->nat; ++a)
-    for (int i = 0; i < L; ++i)
-      for (int j = 0; j < L; ++j) {
-        const size_t ij = ((size_t)a * L + i) * L + j, ji = ((size_t)a * L + j) * L + i;
-        ws->denmat[ij] = 0.5 * (dm[ij] + dm[ji]);
-      }
-  if (ws->dplan && sphere_ready(ws)) {
-    // AUGMENTATION$SPHERE's one-centre energy and POTB, from the device
-    sphere_results(ws, &ws->e1c, &ws->potb);
-  } else {
-    double e1 = 0.0;
-    for (int a = 0; a < ws->nat; ++a)
-      for (int i = 0; i < L; ++i)
-        for (int j = 0; j < L; ++j) {
-          const size_t src = ((size_t)a * L + j) * L + i;   // DH(i,j,a) column-major
-          e1 += ws->dh[2 * src] * ws->denmat[((size_t)a * L + j) * L + i];
-        }
-    ws->e1c = e1;
-  }
-  ws->etot = ws->ekin + ws->epot + ws->e1c;
-  // forces: sum the projector contributions of each atom (fixed order)
-  const double *fp = hr + ws->r_force;
-  for (int a = 0; a < ws->nat; ++a) {
-    const int i0 = ws->ipro0[a], nl = ws->lmnx[ws->isp[a]];
-    for (int d = 0; d < 3; ++d) {
-      double f = 0.0;
-      for (int l = 0; l < nl; ++l) f += fp[(size_t)(i0 + l) * 3 + d];
-      if (ws->dplan) f += ws->fpot[3 * a + d];
-      ws->force[3 * a + d] = f;
-    }
-  }
-  // band energies from the super-band expectation values
-  const double *ex = hr + ws->r_expect;
-  const int nc = expect_chunks(ws);
-  for (int b = 0; b < ws->nbh; ++b) {
-    double t = 0.0, tm = 0.0;
-    for (int c = 0; c < nc; ++c) {
-      t += ex[2 * ((size_t)c * ws->nbh + b)];
-      tm += ex[2 * ((size_t)c * ws->nbh + b) + 1];
-    }
-    const double re = 0.5 * t * p->vol, rem = 0.5 * tm * p->vol;
-    ws->eig[2 * b] = re + rem;
-    if (2 * b + 1 < ws->nb) ws->eig[2 * b + 1] = re - rem;
+    PW_TRY(defer_results(ws, 1));
   }
   ws->have_hpsi = true;
   return PW_OK;
@@ -1057,6 +1110,7 @@ extern "C" int pw_waves_etot(pw_waveset *ws) {
 
 extern "C" int pw_waves_set_potential(pw_waveset *ws, pw_plan *dplan) {
   if (!ws) return pw_fail(PW_ERR_ARG, "pw_waves_set_potential: null waveset");
+  PW_TRY(ws_finish(ws));
   if (dplan) {
     const pw_plan *p = ws->plan;
     if (dplan->nr1 != p->nr1 || dplan->nr2 != p->nr2 || dplan->nr3 != p->nr3)
@@ -1127,6 +1181,7 @@ extern "C" int pw_waves_orthogonalize(pw_waveset *ws) {
     PW_TRY(k_addopsi(ws, ws->d_psim, ws->d_opsi, d_mat(ws, 3)));
     PW_TRY(k_addoproj(ws, ws->d_proj, ws->d_oproj, d_mat(ws, 3)));
   }
+  PW_TRY(ws_finish(ws));   // ETOT's sums: their copy ended before ORTHO_X's readback
   if (ws->residual_on) {
     PW_TRY(k_residual(ws, ws->d_tmp, ws->d_res + ws->r_resid));
     PW_HIP(hipMemcpyAsync(ws->h_res + ws->r_resid, ws->d_res + ws->r_resid, sizeof(double), hipMemcpyDeviceToHost,
@@ -1140,11 +1195,9 @@ extern "C" int pw_waves_orthogonalize(pw_waveset *ws) {
   PW_TRY(k_wavekinetic(ws, ws->d_psi0, ws->d_psim, ws->d_fab, ws->d_res + ws->r_wk));
   PW_HIP(hipMemcpyAsync(ws->h_res + ws->r_wk, ws->d_res + ws->r_wk, red_blocks(ws) * sizeof(double),
                         hipMemcpyDeviceToHost, ws->stream));
-  PW_HIP(hipStreamSynchronize(ws->stream));
-  ws->wavekin2 = sum_of(ws->h_res + ws->r_wk, red_blocks(ws)) / (ws->delt * ws->delt) * p->vol;
-  if (ws->residual_on) ws->residual = ws->h_res[ws->r_resid];
-  ws->efict = 0.5 * (ws->wavekin1 + ws->wavekin2);
-  return PW_OK;
+  ws->pend_wk1 = ws->wavekin1;
+  ws->pend_switched = false;
+  return defer_results(ws, 2);
 }
 
 extern "C" int pw_waves_switch(pw_waveset *ws) {
@@ -1157,6 +1210,7 @@ extern "C" int pw_waves_switch(pw_waveset *ws) {
   // NLAMBDA=0: RLAM0 becomes the zero guess; NLAMBDA>=1: the old RLAM0 becomes RLAMM
   ws->have_rlamm = ws->have_rlam0;
   ws->have_rlam0 = true;
+  if (ws->res_pending & 2) ws->pend_switched = true;
   ws->wavekin1 = 0.0;
   ws->wavekin2 = 0.0;
   return PW_OK;
@@ -1172,6 +1226,7 @@ extern "C" int pw_waves_step(pw_waveset *ws) {
 
 extern "C" int pw_waves_vpsi(pw_waveset *ws, const void *psi, void *hpsi) {
   if (!ws || !psi || !hpsi) return pw_fail(PW_ERR_ARG, "pw_waves_vpsi: null argument");
+  PW_TRY(ws_finish(ws));
   PW_TRY(need(ws->have_v, "VOFR"));
   ScopeTimer tm(ws, "vpsi");
   return fft_vpsi(ws->plan, ws->nbh, (const double2 *)psi, ws->d_v, (double2 *)hpsi, 1, ws->prec, ws->stream);
@@ -1179,6 +1234,7 @@ extern "C" int pw_waves_vpsi(pw_waveset *ws, const void *psi, void *hpsi) {
 
 extern "C" int pw_waves_density(pw_waveset *ws, const void *psi, double *rho) {
   if (!ws || !psi || !rho) return pw_fail(PW_ERR_ARG, "pw_waves_density: null argument");
+  PW_TRY(ws_finish(ws));
   PW_TRY(need(!ws->occ.empty(), "OCC"));
   ScopeTimer tm(ws, "density");
   return fft_density(ws->plan, ws->nbh, (const double2 *)psi, ws->d_occ2, rho, ws->prec, ws->stream);

@@ -181,7 +181,12 @@ int pw_waves_orthogonalize(pw_waveset *ws);
 /* WAVES$SWITCH (src/paw_waves2.f90:2180), NLAMBDAX=2. */
 int pw_waves_switch(pw_waveset *ws);
 /* One full CP step: etot, propagate, orthogonalize, switch (TIMESTEP order,
- * src/paw_driver.f90:333-561 restricted to the wave functions). */
+ * src/paw_driver.f90:333-561 restricted to the wave functions).
+ * pw_waves_etot (without a density object) and pw_waves_orthogonalize return
+ * once their work is queued: the energies, forces and EFICT they produce are
+ * completed by the next call that reads or changes the waveset (any getter or
+ * setter), which waits for them; an error of their kernels is reported
+ * there. */
 int pw_waves_step(pw_waveset *ws);
 /* Set the stream (hipStream_t) all waveset work is queued on. */
 int pw_waves_set_stream(pw_waveset *ws, void *stream);

@@ -118,6 +118,29 @@ def test_cp_steps(name, nsteps):
     assert relerr(w.getc8a("PSI0"), o.psi0) < 1e-9
 
 
+def test_step_results_after_switch():
+    """pw_waves_step returns before ETOT's and ORTHOGONALIZE's sums reach the
+    host.  The getters after it give the values that the separate calls give
+    when read before SWITCH (ETOT, FORCE, EFICT), and SWITCH's zero WAVEEKIN1
+    and WAVEEKIN2."""
+    s = SYSTEMS["si8"]()
+    say("case step results after switch si8")
+    _, a = gpu_waveset(s)
+    _, b = gpu_waveset(s)
+    for it in range(3):
+        a.step()
+        eb = b.etot()
+        b.propagate()
+        b.orthogonalize()
+        efict_b, wk2_b = b.getr8("EFICT"), b.getr8("WAVEEKIN2")
+        b.switch()
+        assert a.getr8("ETOT") == eb["etot"], it
+        assert np.array_equal(a.getr8a("FORCE"), b.getr8a("FORCE")), it
+        assert a.getr8("EFICT") == efict_b, it
+        assert wk2_b != 0.0, it
+        assert a.getr8("WAVEEKIN1") == 0.0 and a.getr8("WAVEEKIN2") == 0.0, it
+
+
 @pytest.mark.parametrize("bounds", [(10.0, 20.0), (0.01, 0.02)])
 def test_ortho_bounds_rejected(bounds):
     # spectral bounds far above (Newton-Schulz too slow) or far below (it
