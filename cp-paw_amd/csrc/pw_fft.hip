@@ -347,6 +347,10 @@ __device__ inline bool x_block(int ngroups, int nb, int &group, int &band) {
   return band < nb;
 }
 inline dim3 x_grid(int ngroups, int nb) { return dim3((unsigned)(8 * ((nb + 7) / 8) * ngroups)); }
+// LDS pitch of the x passes' stripe tiles: odd, so that the tile's transposes
+// (neighbouring lanes on neighbouring stripes, one pitch apart) fall into
+// distinct banks
+__host__ __device__ inline int x_pitch(int nr1) { return nr1 | 1; }
 
 // ---- x pass, forward: sphere -> stripes [band][x][stripe] --------------------
 template <typename T2, int NX = 0>
@@ -357,24 +361,25 @@ __global__ void __launch_bounds__(kBlock) k_xfwd(Geo g, Fft1D f, const T2 *tw, c
   int grp, band;
   if (!x_block((g.nstripe + PX - 1) / PX, nb, grp, band)) return;
   extern __shared__ unsigned char smem_raw[];
+  const int xp = x_pitch(g.nr1);
   T2 *b0 = reinterpret_cast<T2 *>(smem_raw);
-  T2 *b1 = b0 + PX * g.nr1;
+  T2 *b1 = b0 + PX * xp;
   const int s0 = grp * PX;
   const int ns = min(PX, g.nstripe - s0);
-  for (int i = threadIdx.x; i < PX * g.nr1; i += blockDim.x) b0[i] = czero<T2>();
+  for (int i = threadIdx.x; i < PX * xp; i += blockDim.x) b0[i] = czero<T2>();
   __syncthreads();
   const double2 *src = psi + (size_t)band * g.ngl;
   const int e0 = g.stripe_off[s0], e1 = g.stripe_off[s0 + ns];
   for (int e = e0 + threadIdx.x; e < e1; e += blockDim.x)
-    b0[(g.stripe_s[e] - s0) * g.nr1 + g.stripe_x[e]] = Prec<T2>::from(src[g.stripe_g[e]]);
+    b0[(g.stripe_s[e] - s0) * xp + g.stripe_x[e]] = Prec<T2>::from(src[g.stripe_g[e]]);
   __syncthreads();
   T2 *r;
-  if constexpr (NX > 0) r = lds_fft_ct<T2, NX, 0, 1>(b0, b1, PX, 1, NX, tw, +1, false);
-  else r = lds_fft<T2>(b0, b1, f, PX, 1, g.nr1, tw, +1, false);
+  if constexpr (NX > 0) r = lds_fft_ct<T2, NX, 0, 1>(b0, b1, PX, 1, xp, tw, +1, false);
+  else r = lds_fft<T2>(b0, b1, f, PX, 1, xp, tw, +1, false);
   T2 *dst = stripes + (size_t)band * g.nr1 * g.nstripe + s0;
   for (int i = threadIdx.x; i < ns * g.nr1; i += blockDim.x) {
     const int x = i / ns, sl = i % ns;
-    dst[(size_t)x * g.nstripe + sl] = r[sl * g.nr1 + x];
+    dst[(size_t)x * g.nstripe + sl] = r[sl * xp + x];
   }
 }
 
@@ -388,25 +393,26 @@ __global__ void __launch_bounds__(kBlock) k_xinv(Geo g, Fft1D f, const T2 *tw, c
   int grp, band;
   if (!x_block((g.nstripe + PX - 1) / PX, nb, grp, band)) return;
   extern __shared__ unsigned char smem_raw[];
+  const int xp = x_pitch(g.nr1);
   T2 *b0 = reinterpret_cast<T2 *>(smem_raw);
-  T2 *b1 = b0 + PX * g.nr1;
+  T2 *b1 = b0 + PX * xp;
   const int s0 = grp * PX;
   const int ns = min(PX, g.nstripe - s0);
   const T2 *src = stripes + (size_t)band * g.nr1 * g.nstripe + s0;
   for (int i = threadIdx.x; i < PX * g.nr1; i += blockDim.x) {
     const int x = i / PX, sl = i % PX;
-    b0[sl * g.nr1 + x] = sl < ns ? src[(size_t)x * g.nstripe + sl] : czero<T2>();
+    b0[sl * xp + x] = sl < ns ? src[(size_t)x * g.nstripe + sl] : czero<T2>();
   }
   __syncthreads();
   T2 *r;
-  if constexpr (NX > 0) r = lds_fft_ct<T2, NX, 0, 1>(b0, b1, PX, 1, NX, tw, -1, false);
-  else r = lds_fft<T2>(b0, b1, f, PX, 1, g.nr1, tw, -1, false);
+  if constexpr (NX > 0) r = lds_fft_ct<T2, NX, 0, 1>(b0, b1, PX, 1, xp, tw, -1, false);
+  else r = lds_fft<T2>(b0, b1, f, PX, 1, xp, tw, -1, false);
   const int e0 = g.stripe_off[s0], e1 = g.stripe_off[s0 + ns];
   double2 *dst = out + (size_t)band * g.ngl;
   const double2 *ps = psi + (size_t)band * g.ngl;
   for (int e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
     const int ig = g.stripe_g[e];
-    T2 v = r[(g.stripe_s[e] - s0) * g.nr1 + g.stripe_x[e]];
+    T2 v = r[(g.stripe_s[e] - s0) * xp + g.stripe_x[e]];
     double2 o = make_double2((double)v.x * scale, (double)v.y * scale);
     if (add_kin) {
       const double h = 0.5 * g.g2[ig];
@@ -1209,7 +1215,7 @@ int run_forward_x(pw_plan *p, const Geo &g, int nb, const double2 *psi, hipStrea
   T2 *stripes = (T2 *)p->d_stripe_buf;
   constexpr int PX = XTile<T2>::n;
   const dim3 gx = x_grid((p->nstripe + PX - 1) / PX, nb);
-  const size_t lds = 2 * PX * p->nr1 * sizeof(T2);
+  const size_t lds = 2 * PX * x_pitch(p->nr1) * sizeof(T2);
   PW_TRY(with_x(p, [&](auto nx) -> int {
     k_xfwd<T2, decltype(nx)::value><<<gx, kBlock, lds, s>>>(g, p->f[0], twid<T2>(p, 0), psi, stripes, nb);
     return PW_OK;
@@ -1224,7 +1230,7 @@ int run_inverse_x(pw_plan *p, const Geo &g, int nb, double2 *out, const double2 
   T2 *stripes = (T2 *)p->d_stripe_buf;
   constexpr int PX = XTile<T2>::n;
   const dim3 gx = x_grid((p->nstripe + PX - 1) / PX, nb);
-  const size_t lds = 2 * PX * p->nr1 * sizeof(T2);
+  const size_t lds = 2 * PX * x_pitch(p->nr1) * sizeof(T2);
   PW_TRY(with_x(p, [&](auto nx) -> int {
     k_xinv<T2, decltype(nx)::value><<<gx, kBlock, lds, s>>>(g, p->f[0], twid<T2>(p, 0), stripes, out, psi, scale,
                                                             add_kin, nb);
@@ -1240,7 +1246,7 @@ int run_forward_xy(pw_plan *p, const Geo &g, int nb, const double2 *psi, T2 *str
   const int ntx = (p->nr1 + kTX - 1) / kTX;
   constexpr int PX = XTile<T2>::n;
   const dim3 gx = x_grid((p->nstripe + PX - 1) / PX, nb);
-  k_xfwd<T2><<<gx, kBlock, 2 * PX * p->nr1 * sizeof(T2), s>>>(g, p->f[0], twid<T2>(p, 0), psi,
+  k_xfwd<T2><<<gx, kBlock, 2 * PX * x_pitch(p->nr1) * sizeof(T2), s>>>(g, p->f[0], twid<T2>(p, 0), psi,
                                                               stripes, nb);
   dim3 gy(p->nzp * ntx, nb);
   k_yfwd<T2><<<gy, kBlock, 2 * kTX * p->nr2 * sizeof(T2), s>>>(g, p->f[1], twid<T2>(p, 1),
@@ -1258,7 +1264,7 @@ int run_inverse_yx(pw_plan *p, const Geo &g, int nb, double2 *out, T2 *stripes, 
                                                                planes, stripes);
   constexpr int PX = XTile<T2>::n;
   const dim3 gx = x_grid((p->nstripe + PX - 1) / PX, nb);
-  k_xinv<T2><<<gx, kBlock, 2 * PX * p->nr1 * sizeof(T2), s>>>(g, p->f[0], twid<T2>(p, 0),
+  k_xinv<T2><<<gx, kBlock, 2 * PX * x_pitch(p->nr1) * sizeof(T2), s>>>(g, p->f[0], twid<T2>(p, 0),
                                                               stripes, out, nullptr, 1.0 / (double)p->nrl, 0, nb);
   PW_HIP(hipGetLastError());
   return PW_OK;
